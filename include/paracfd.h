@@ -1,0 +1,255 @@
+/*
+ * paracfd.h - C ABI of the MI355X solver step for ParaCFD's external-aerodynamics core.
+ *
+ * One solver step = conservative MUSCL cell-momentum transport, MGPCG pressure projection
+ * to a relative residual tolerance, and the surface-load integral, on static Cartesian AMR
+ * bricks (8^3 cells) with cut-cell embedded boundaries.
+ *
+ * The reference exposes this path as the C++ class ExternalAeroCore
+ * (src/core/fluid/external_aero_core.h:124-182); it has no C ABI.  Each entry point below
+ * names the reference member it replaces.  Conventions:
+ *   - every function returns a pcfd_status; PCFD_OK == 0.  The reference throws
+ *     std::invalid_argument / std::runtime_error (src/core/fluid/amr_fields.cu:53,
+ *     src/core/fluid/external_aero_core.cpp:242,306,308); here the same conditions return
+ *     PCFD_EINVAL / PCFD_EDEVICE / PCFD_ENONFINITE / PCFD_ESTATE with the message in
+ *     pcfd_last_error().
+ *   - a projection that does not reach the tolerance is a status, not an error
+ *     (external_aero_core.cpp:327): pcfd_step returns PCFD_OK with stats->converged == 0
+ *     and physical time does not advance.
+ *   - a solver handle owns all of its device memory and one HIP stream.  Calls on one
+ *     handle are not thread-safe; distinct handles are independent.
+ *   - host arrays are plain pointers with explicit element counts; nothing here needs a
+ *     torch or HIP type.
+ */
+#ifndef PARACFD_H
+#define PARACFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCFD_ABI_VERSION 1
+
+typedef enum pcfd_status {
+    PCFD_OK = 0,
+    PCFD_EINVAL = 1,          /* bad argument (reference: std::invalid_argument) */
+    PCFD_EDEVICE = 2,         /* HIP runtime failure (reference: check() -> runtime_error) */
+    PCFD_ENOTCONVERGED = 3,   /* reserved: projection failed and the caller asked for an error */
+    PCFD_ENONFINITE = 4,      /* non-finite state before a step (external_aero_core.cpp:308) */
+    PCFD_ESTATE = 5,          /* wrong call order, e.g. step before initialize (:306) */
+    PCFD_ENOMEM = 6           /* device or host allocation failed / over budget (:91-95) */
+} pcfd_status;
+
+typedef enum pcfd_geometry_kind {
+    PCFD_GEOM_NONE = 0,        /* empty box */
+    PCFD_GEOM_SPHERE = 1,      /* sphere: center, radius */
+    PCFD_GEOM_NACA4_WING = 2,  /* straight extruded NACA 4-digit wing, span along y */
+    PCFD_GEOM_PARAGLIDER = 3   /* arched, swept, elliptic-planform reflexed wing */
+} pcfd_geometry_kind;
+
+typedef struct pcfd_geometry_desc {
+    int kind;                 /* pcfd_geometry_kind */
+    double center[3];         /* sphere centre; wing: leading edge of the root section (pitch pivot) */
+    double radius;            /* sphere radius */
+    int naca_digits;          /* 4-digit code, e.g. 2412 (wing) */
+    double chord;             /* wing root chord [m] */
+    double span;              /* wing span along y [m], centred on center[1] */
+    double alpha_deg;         /* pitch about +y through center; positive = nose up */
+    double sweep_deg;         /* paraglider: leading-edge sweep */
+    double anhedral_deg;      /* paraglider: total arc anhedral angle */
+    double reflex;            /* paraglider: reflex camber magnitude (fraction of chord) */
+} pcfd_geometry_desc;
+
+typedef struct pcfd_grid_desc {
+    double lo[3], hi[3];      /* requested domain box; y/z are padded symmetrically and x downstream
+                                 to whole 8^3 bricks of cubic cells (amr_grid.cpp:66-96) */
+    int base_cells_x;         /* level-0 cells along x; h0 = (hi[0]-lo[0]) / base_cells_x */
+    int levels;               /* AMR levels, 1..4; level l has h = h0 / 2^l */
+    double refine_band[4];    /* level-l bricks within refine_band[l] of the surface are refined */
+    double min_volume_fraction; /* cut cells below this fluid fraction become solid */
+    int subsamples;           /* sub-samples per axis for cut-cell volume/aperture */
+    int outlet_xmax;          /* 1: Dirichlet p = 0 at x-max (pressure_outlet_xmax); 0: all Neumann */
+} pcfd_grid_desc;
+
+typedef struct pcfd_topology pcfd_topology;
+typedef struct pcfd_solver pcfd_solver;
+
+typedef struct pcfd_topology_info {
+    int64_t bricks;           /* stored (leaf, non-empty) bricks */
+    int64_t dofs;             /* bricks * 512 slots, brick-major */
+    int64_t active_cells;     /* N: the cells in cell-steps/s */
+    int64_t special_cells;    /* cells touching a compact edge, cut cells, EB patches */
+    int64_t compact_edges;    /* coarse/fine and partially open faces */
+    int64_t patches;          /* embedded-boundary surface patches */
+    int64_t mg_levels;        /* multigrid levels including the composite level */
+    int64_t mg_cells[12];     /* cells per multigrid level */
+    int levels;
+    double h0;
+    double lo[3];             /* padded domain origin */
+    int base_bricks[3];       /* level-0 brick grid */
+    double body_volume;       /* solid volume represented by the cut cells */
+    double reference_area;    /* wing planform area or sphere frontal area */
+    double finest_h;          /* cell size of the finest level */
+    double max_diffusion_coefficient; /* max over cells of sum(open area / distance) / volume */
+    int outlet;               /* 1 when x-max is a Dirichlet outlet */
+} pcfd_topology_info;
+
+/* Topology arrays exported for host-side checks (oracle, tests). */
+typedef enum pcfd_array_id {
+    PCFD_ARR_BRICK_LEVEL = 0,      /* int32[bricks] */
+    PCFD_ARR_BRICK_COORD = 1,      /* int32[bricks*3] brick coordinates in its level's brick grid */
+    PCFD_ARR_BRICK_FLAGS = 2,      /* uint8[bricks] domain-boundary bits */
+    PCFD_ARR_BRICK_NBR = 3,        /* int32[bricks*6] same-level neighbour brick or -1 */
+    PCFD_ARR_MASK = 4,             /* uint8[dofs] cell mask bits */
+    PCFD_ARR_SPECIAL_DOF = 5,      /* int32[special] */
+    PCFD_ARR_SPECIAL_VOLUME = 6,   /* double[special] fluid volume */
+    PCFD_ARR_SPECIAL_INC_OFF = 7,  /* int32[special+1] */
+    PCFD_ARR_SPECIAL_INC = 8,      /* int32[inc]: edge e if the cell is the lower side, ~e if upper */
+    PCFD_ARR_SPECIAL_PATCH = 9,    /* int32[special] patch index or -1 */
+    PCFD_ARR_BRICK_SPECIAL_OFF = 10, /* int32[bricks+1] */
+    PCFD_ARR_EDGE_A = 11,          /* int32[edges] lower cell along the axis */
+    PCFD_ARR_EDGE_B = 12,          /* int32[edges] upper cell */
+    PCFD_ARR_EDGE_AXIS = 13,       /* int8[edges] */
+    PCFD_ARR_EDGE_AREA = 14,       /* double[edges] open area */
+    PCFD_ARR_EDGE_COEF = 15,       /* double[edges] open area / centre distance */
+    PCFD_ARR_EDGE_W = 16,          /* double[edges] interpolation weight of the upper cell */
+    PCFD_ARR_PATCH_DOF = 17,       /* int32[patches] */
+    PCFD_ARR_PATCH_S = 18,         /* double[patches*3] area vector, fluid -> solid */
+    PCFD_ARR_PATCH_X = 19,         /* double[patches*3] patch centroid */
+    PCFD_ARR_PATCH_LSQ_OFF = 20,   /* int32[patches+1] */
+    PCFD_ARR_PATCH_LSQ_DOF = 21,   /* int32[lsq] */
+    PCFD_ARR_PATCH_LSQ_BETA = 22,  /* double[lsq] p_wall = p + sum beta (p_nbr - p) */
+    PCFD_ARR_CELL_CENTROID = 23,   /* double[dofs*3] fluid centroid (cell centre for full cells) */
+    PCFD_ARR_LEVEL_H = 24,         /* double[levels] */
+    PCFD_ARR_MG_ROW_OFF = 100,     /* + 8*m: int32[n_m+1] CSR of multigrid level m (m >= 1) */
+    PCFD_ARR_MG_COL = 101,         /* + 8*m: int32[nnz] */
+    PCFD_ARR_MG_COEF = 102,        /* + 8*m: double[nnz] off-diagonal couplings (positive) */
+    PCFD_ARR_MG_DIAG = 103,        /* + 8*m: double[n_m] */
+    PCFD_ARR_MG_PARENT = 104,      /* + 8*m: int32[n_m] (m=0: dofs) index in level m+1 or -1 */
+    PCFD_ARR_MG_COARSE_INV = 105   /* + 8*m: double[n_m*n_m] dense inverse, coarsest level only */
+} pcfd_array_id;
+
+typedef enum pcfd_precision { PCFD_FP64 = 0, PCFD_FP32 = 1, PCFD_BF16 = 2 } pcfd_precision;
+typedef enum pcfd_preconditioner {
+    PCFD_PC_MG = 0,       /* geometric V-cycle (plain MGPCG) */
+    PCFD_PC_JACOBI = 1,   /* diagonal */
+    PCFD_PC_LEARNED = 2   /* learned conv-net correction inside flexible PCG, MG fallback */
+} pcfd_preconditioner;
+
+typedef struct pcfd_options {
+    double rho;               /* density (freestream.rho) */
+    double nu;                /* molecular kinematic viscosity */
+    double speed;             /* freestream speed along +x */
+    double cfl;               /* solver.cfl */
+    double smagorinsky_cs;    /* 0 disables the Smagorinsky model */
+    double rtol;              /* projection_tolerance, relative to ||rhs|| */
+    int max_iterations;       /* projection_max_iterations */
+    int field_precision;      /* PCFD_FP64 or PCFD_FP32 velocity / face fields (Real) */
+    int vcycle_precision;     /* PCFD_FP64, PCFD_FP32 or PCFD_BF16 (storage) V-cycle */
+    int preconditioner;       /* pcfd_preconditioner */
+    int pre_smooth, post_smooth; /* V-cycle Jacobi sweeps */
+    int check_interval;       /* Krylov iterations between host convergence reads (>=1);
+                                 results do not depend on it */
+} pcfd_options;
+
+typedef struct pcfd_step_stats {
+    double dt;
+    double physical_time;
+    int iterations;           /* Krylov iterations of this projection */
+    int converged;
+    double relative_residual; /* ||r|| / ||b|| at exit */
+    double max_abs_velocity;
+    double cfl_rate;          /* max outflow rate used for dt */
+    double max_abs_divergence;/* after projection, integrated divergence / volume (FP64) */
+    int diffusion_substeps;
+    int fallback_used;        /* learned preconditioner fell back to plain MG */
+    double advect_ms, project_ms, step_ms; /* device-event timings */
+} pcfd_step_stats;
+
+typedef struct pcfd_loads {
+    double force[3];          /* pressure force on the body [N] */
+    double moment[3];         /* about the moment origin */
+    double drag, side, lift;  /* force x, y, z */
+    double cd, cs, cl;        /* coefficients, valid when q*S_ref > 0 */
+    int coefficients_valid;
+} pcfd_loads;
+
+/* Last error message of the calling thread (reference: exception what()). */
+const char* pcfd_last_error(void);
+int pcfd_abi_version(void);
+
+/* Static preprocessing: AMR bricks, cut cells, compact edges, surface patches and the
+ * multigrid hierarchy.  Replaces the ExternalAeroCore constructor's preprocessing
+ * (external_aero_core.cpp:45-236: AmrHierarchy::build_static, build_amr_embedded_boundary_atlas,
+ * build_composite_amr_pressure_system). Host only; needs no GPU. */
+pcfd_status pcfd_topology_build(const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid,
+                                pcfd_topology** out);
+void pcfd_topology_free(pcfd_topology* topology);
+pcfd_status pcfd_topology_get_info(const pcfd_topology* topology, pcfd_topology_info* info);
+/* Element count of an exported array (see pcfd_array_id), or -1. */
+int64_t pcfd_topology_array_size(const pcfd_topology* topology, int array_id);
+/* Copy an exported array into dst (bytes must equal count * element size). */
+pcfd_status pcfd_topology_export(const pcfd_topology* topology, int array_id, void* dst, size_t bytes);
+
+/* Device solver on the current HIP device.  Replaces ExternalAeroCore's device objects
+ * (DeviceAmrFields, DeviceCompositeAmrProjection, DeviceCompositeCellMomentumTransport;
+ * external_aero_core.cpp:77-200). The topology may be freed after creation. */
+pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* options, pcfd_solver** out);
+void pcfd_destroy(pcfd_solver* solver);
+void pcfd_default_options(pcfd_options* options);
+
+/* ExternalAeroCore::initialize (external_aero_core.cpp:245-302): freestream seed, face
+ * reconstruction, first projection with dt = cfl*h/U, first pressure impulse. */
+pcfd_status pcfd_initialize(pcfd_solver* solver, pcfd_step_stats* stats);
+
+/* ExternalAeroCore::step (external_aero_core.cpp:304-328), cell-momentum branch:
+ * CFL dt -> advect -> diffuse -> reconstruct face fluxes -> project -> pressure impulse. */
+pcfd_status pcfd_step(pcfd_solver* solver, pcfd_step_stats* stats);
+
+/* The three stages of one step, callable separately.
+ * pcfd_advect: DeviceCompositeCellMomentumTransport::step + diffuse_smagorinsky +
+ *   reconstruct_fluxes (amr_advection.cu:1322-1326,1339-1345,1423-1456) and the external
+ *   boundary faces (amr_fields.cu:86-118).  dt <= 0 selects the CFL timestep
+ *   (external_aero_core.cpp:309); the timestep used is written to *dt_used. */
+pcfd_status pcfd_advect(pcfd_solver* solver, double dt, double* dt_used, pcfd_step_stats* stats);
+/* DeviceCompositeAmrProjection::project(rho, dt, tol, maxit, warm) (amr_pressure.h:478,
+ * amr_pressure.cu:2085-2095): RHS, PCG solve, face-flux correction, divergence check. */
+pcfd_status pcfd_project(pcfd_solver* solver, double dt, int warm_start, pcfd_step_stats* stats);
+/* apply_projected_pressure_gradient (amr_advection.cu:1412-1421). */
+pcfd_status pcfd_apply_pressure_impulse(pcfd_solver* solver, double dt);
+/* ExternalAeroCore::pressure_loads(pref) -> compute_pressure_loads (external_aero_core.cpp:330-337,
+ * aero_loads.cpp:35-47), evaluated on the device in FP64. S_ref and the moment origin come
+ * from pcfd_set_reference. */
+pcfd_status pcfd_integrate_loads(pcfd_solver* solver, double pref, pcfd_loads* loads);
+pcfd_status pcfd_set_reference(pcfd_solver* solver, double area, const double origin[3]);
+
+/* State I/O (upload_state/download_state amr_advection.cu:1301-1309, upload_pressure
+ * amr_pressure.cu:2098-2108, download_pressure).  Arrays are FP64 host arrays of length
+ * dofs (velocity: three arrays).  Face velocities: +x/+y/+z face of every dof, then one
+ * value per compact edge. */
+pcfd_status pcfd_get_velocity(pcfd_solver* solver, double* x, double* y, double* z);
+pcfd_status pcfd_set_velocity(pcfd_solver* solver, const double* x, const double* y, const double* z);
+pcfd_status pcfd_get_pressure(pcfd_solver* solver, double* p);
+pcfd_status pcfd_set_pressure(pcfd_solver* solver, const double* p);
+pcfd_status pcfd_get_faces(pcfd_solver* solver, double* fx, double* fy, double* fz, double* edge);
+pcfd_status pcfd_set_faces(pcfd_solver* solver, const double* fx, const double* fy, const double* fz,
+                           const double* edge);
+/* Mark the state as initialized after pcfd_set_* (checkpoint resume). */
+pcfd_status pcfd_mark_initialized(pcfd_solver* solver, double physical_time);
+int64_t pcfd_device_bytes(const pcfd_solver* solver);
+
+/* Multigrid primitives on level `level` (0 = composite level), host arrays in, host array out.
+ * op: 0 apply A x, 1 residual b - A x, 2 damped-Jacobi sweeps on x (omega, sweeps),
+ *     3 one full V-cycle z = M^-1 b.  The uniform legacy primitives project.cu:286-328
+ *     (poisson_apply_cpu, poisson_residual_cpu, jacobi_smooth_cpu) are the special case
+ *     of a single-level box without outlet, scaled by h^3. */
+pcfd_status pcfd_mg_primitive(pcfd_solver* solver, int op, int level, const double* x, const double* b,
+                              double* out, double omega, int sweeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARACFD_H */

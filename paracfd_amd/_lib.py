@@ -1,0 +1,228 @@
+"""ctypes binding of the C ABI declared in include/paracfd.h (libparacfd.so, built in-tree).
+
+The library is the product: topology preprocessing on the host plus the HIP solver for gfx950.
+Loading fails loudly when it has not been built; there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparacfd.so")
+
+PCFD_OK, PCFD_EINVAL, PCFD_EDEVICE, PCFD_ENOTCONVERGED, PCFD_ENONFINITE, PCFD_ESTATE, PCFD_ENOMEM = range(7)
+GEOM_NONE, GEOM_SPHERE, GEOM_NACA4_WING, GEOM_PARAGLIDER = range(4)
+FP64, FP32, BF16 = range(3)
+PC_MG, PC_JACOBI, PC_LEARNED = range(3)
+
+ARR = dict(
+    brick_level=(0, np.int32), brick_coord=(1, np.int32), brick_flags=(2, np.uint8), brick_nbr=(3, np.int32),
+    mask=(4, np.uint8), special_dof=(5, np.int32), special_volume=(6, np.float64), special_inc_off=(7, np.int32),
+    special_inc=(8, np.int32), special_patch=(9, np.int32), brick_special_off=(10, np.int32), edge_a=(11, np.int32),
+    edge_b=(12, np.int32), edge_axis=(13, np.int8), edge_area=(14, np.float64), edge_coef=(15, np.float64),
+    edge_w=(16, np.float64), patch_dof=(17, np.int32), patch_S=(18, np.float64), patch_x=(19, np.float64),
+    patch_lsq_off=(20, np.int32), patch_lsq_dof=(21, np.int32), patch_lsq_beta=(22, np.float64),
+    centroid=(23, np.float64), level_h=(24, np.float64),
+)
+MG_ARR = dict(row_off=(0, np.int32), col=(1, np.int32), coef=(2, np.float64), diag=(3, np.float64),
+              parent=(4, np.int32), coarse_inv=(5, np.float64))
+
+
+class GeometryDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("center", C.c_double * 3), ("radius", C.c_double), ("naca_digits", C.c_int),
+                ("chord", C.c_double), ("span", C.c_double), ("alpha_deg", C.c_double), ("sweep_deg", C.c_double),
+                ("anhedral_deg", C.c_double), ("reflex", C.c_double)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("base_cells_x", C.c_int), ("levels", C.c_int),
+                ("refine_band", C.c_double * 4), ("min_volume_fraction", C.c_double), ("subsamples", C.c_int),
+                ("outlet_xmax", C.c_int)]
+
+
+class TopologyInfo(C.Structure):
+    _fields_ = [("bricks", C.c_int64), ("dofs", C.c_int64), ("active_cells", C.c_int64),
+                ("special_cells", C.c_int64), ("compact_edges", C.c_int64), ("patches", C.c_int64),
+                ("mg_levels", C.c_int64), ("mg_cells", C.c_int64 * 12), ("levels", C.c_int), ("h0", C.c_double),
+                ("lo", C.c_double * 3), ("base_bricks", C.c_int * 3), ("body_volume", C.c_double),
+                ("reference_area", C.c_double), ("finest_h", C.c_double),
+                ("max_diffusion_coefficient", C.c_double), ("outlet", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("rho", C.c_double), ("nu", C.c_double), ("speed", C.c_double), ("cfl", C.c_double),
+                ("smagorinsky_cs", C.c_double), ("rtol", C.c_double), ("max_iterations", C.c_int),
+                ("field_precision", C.c_int), ("vcycle_precision", C.c_int), ("preconditioner", C.c_int),
+                ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("dt", C.c_double), ("physical_time", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
+                ("relative_residual", C.c_double), ("max_abs_velocity", C.c_double), ("cfl_rate", C.c_double),
+                ("max_abs_divergence", C.c_double), ("diffusion_substeps", C.c_int), ("fallback_used", C.c_int),
+                ("advect_ms", C.c_double), ("project_ms", C.c_double), ("step_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Loads(C.Structure):
+    _fields_ = [("force", C.c_double * 3), ("moment", C.c_double * 3), ("drag", C.c_double), ("side", C.c_double),
+                ("lift", C.c_double), ("cd", C.c_double), ("cs", C.c_double), ("cl", C.c_double),
+                ("coefficients_valid", C.c_int)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["force"] = list(self.force)
+        d["moment"] = list(self.moment)
+        return d
+
+
+# every symbol the public header declares
+EXPORTED = [
+    "pcfd_last_error", "pcfd_abi_version", "pcfd_topology_build", "pcfd_topology_free", "pcfd_topology_get_info",
+    "pcfd_topology_array_size", "pcfd_topology_export", "pcfd_create", "pcfd_destroy", "pcfd_default_options",
+    "pcfd_initialize", "pcfd_step", "pcfd_advect", "pcfd_project", "pcfd_apply_pressure_impulse",
+    "pcfd_integrate_loads", "pcfd_set_reference", "pcfd_get_velocity", "pcfd_set_velocity", "pcfd_get_pressure",
+    "pcfd_set_pressure", "pcfd_get_faces", "pcfd_set_faces", "pcfd_mark_initialized", "pcfd_device_bytes",
+    "pcfd_mg_primitive",
+]
+
+_lib = None
+
+
+def load():
+    """Load libparacfd.so; raise if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"paracfd_amd: {LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "pcfd_last_error": (C.c_char_p, []),
+        "pcfd_abi_version": (C.c_int, []),
+        "pcfd_topology_build": (C.c_int, [C.POINTER(GeometryDesc), C.POINTER(GridDesc), C.POINTER(P)]),
+        "pcfd_topology_free": (None, [P]),
+        "pcfd_topology_get_info": (C.c_int, [P, C.POINTER(TopologyInfo)]),
+        "pcfd_topology_array_size": (C.c_int64, [P, C.c_int]),
+        "pcfd_topology_export": (C.c_int, [P, C.c_int, P, C.c_size_t]),
+        "pcfd_create": (C.c_int, [P, C.POINTER(Options), C.POINTER(P)]),
+        "pcfd_destroy": (None, [P]),
+        "pcfd_default_options": (None, [C.POINTER(Options)]),
+        "pcfd_initialize": (C.c_int, [P, C.POINTER(StepStats)]),
+        "pcfd_step": (C.c_int, [P, C.POINTER(StepStats)]),
+        "pcfd_advect": (C.c_int, [P, C.c_double, dp, C.POINTER(StepStats)]),
+        "pcfd_project": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(StepStats)]),
+        "pcfd_apply_pressure_impulse": (C.c_int, [P, C.c_double]),
+        "pcfd_integrate_loads": (C.c_int, [P, C.c_double, C.POINTER(Loads)]),
+        "pcfd_set_reference": (C.c_int, [P, C.c_double, dp]),
+        "pcfd_get_velocity": (C.c_int, [P, dp, dp, dp]),
+        "pcfd_set_velocity": (C.c_int, [P, dp, dp, dp]),
+        "pcfd_get_pressure": (C.c_int, [P, dp]),
+        "pcfd_set_pressure": (C.c_int, [P, dp]),
+        "pcfd_get_faces": (C.c_int, [P, dp, dp, dp, dp]),
+        "pcfd_set_faces": (C.c_int, [P, dp, dp, dp, dp]),
+        "pcfd_mark_initialized": (C.c_int, [P, C.c_double]),
+        "pcfd_device_bytes": (C.c_int64, [P]),
+        "pcfd_mg_primitive": (C.c_int, [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PcfdError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"pcfd status {status}: {message}")
+        self.status = status
+
+
+def check(status):
+    if status != PCFD_OK:
+        raise PcfdError(status, load().pcfd_last_error().decode())
+    return status
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Topology:
+    """Host topology (bricks, cut cells, compact edges, patches, multigrid hierarchy)."""
+
+    def __init__(self, geometry: GeometryDesc, grid: GridDesc):
+        lib = load()
+        self._lib = lib
+        h = C.c_void_p()
+        check(lib.pcfd_topology_build(C.byref(geometry), C.byref(grid), C.byref(h)))
+        self.handle = h
+        self.info = TopologyInfo()
+        check(lib.pcfd_topology_get_info(h, C.byref(self.info)))
+        self._cache = {}
+
+    def close(self):
+        if self.handle:
+            self._lib.pcfd_topology_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _export(self, aid, dtype):
+        n = self._lib.pcfd_topology_array_size(self.handle, aid)
+        if n < 0:
+            raise KeyError(aid)
+        out = np.empty(n, dtype=dtype)
+        check(self._lib.pcfd_topology_export(self.handle, aid, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def array(self, name):
+        if name not in self._cache:
+            aid, dt = ARR[name]
+            self._cache[name] = self._export(aid, dt)
+        return self._cache[name]
+
+    def mg_array(self, level, name):
+        key = (level, name)
+        if key not in self._cache:
+            kind, dt = MG_ARR[name]
+            self._cache[key] = self._export(100 + 8 * level + kind, dt)
+        return self._cache[key]
+
+    @property
+    def ndof(self):
+        return int(self.info.dofs)
+
+    @property
+    def n_active(self):
+        return int(self.info.active_cells)
+
+    @property
+    def mg_levels(self):
+        return int(self.info.mg_levels)
+
+    @property
+    def outlet(self):
+        return bool(self.info.outlet)
+
+    @property
+    def max_diffusion_coefficient(self):
+        return float(self.info.max_diffusion_coefficient)
+
+    def mg_n(self, level):
+        return self.ndof if level == 0 else int(self.mg_array(level, "diag").size)

@@ -1,0 +1,85 @@
+"""Case descriptions: the sphere box and the NACA wings at the sizes the benchmarks use.
+
+Cell-count interpretation (cubic cells, 8^3 bricks): the wing cases keep the stated domain
+[-3,6]x[-3,3]x[-2,3.03], chord 1, span 2.03 and refinement bands; the base resolution along x
+is chosen so the active-cell counts land near the stated ~4 M (2-level) and ~20 M (3-level).
+"""
+from __future__ import annotations
+
+from . import _lib as L
+
+
+def sphere_box(cells_x=128, levels=1, subsamples=4, min_volume_fraction=0.1, outlet=True, band=0.25):
+    """Sphere r=0.25 at (1,1,1) in [0,4]x[0,2]x[0,2]; h = 4/cells_x (128 -> h=1/32, 128x64x64)."""
+    g = L.GeometryDesc()
+    g.kind = L.GEOM_SPHERE
+    g.center[:] = (1.0, 1.0, 1.0)
+    g.radius = 0.25
+    d = L.GridDesc()
+    d.lo[:] = (0.0, 0.0, 0.0)
+    d.hi[:] = (4.0, 2.0, 2.0)
+    d.base_cells_x = cells_x
+    d.levels = levels
+    d.refine_band[:] = (band, band * 0.5, band * 0.25, 0.0)
+    d.min_volume_fraction = min_volume_fraction
+    d.subsamples = subsamples
+    d.outlet_xmax = 1 if outlet else 0
+    return g, d
+
+
+def naca_wing(digits=2412, alpha_deg=4.0, cells_x=128, levels=2, bands=(0.3, 0.0, 0.0), subsamples=4,
+              min_volume_fraction=0.1, chord=1.0, span=2.03):
+    """Straight NACA 4-digit wing, leading edge at the origin, span along y, pitched about +y."""
+    g = L.GeometryDesc()
+    g.kind = L.GEOM_NACA4_WING
+    g.center[:] = (0.0, 0.0, 0.0)
+    g.naca_digits = digits
+    g.chord = chord
+    g.span = span
+    g.alpha_deg = alpha_deg
+    d = L.GridDesc()
+    d.lo[:] = (-3.0, -3.0, -2.0)
+    d.hi[:] = (6.0, 3.0, 3.03)
+    d.base_cells_x = cells_x
+    d.levels = levels
+    d.refine_band[:] = tuple(bands) + (0.0,) * (4 - len(bands))
+    d.min_volume_fraction = min_volume_fraction
+    d.subsamples = subsamples
+    d.outlet_xmax = 1
+    return g, d
+
+
+def uniform_box(n=16, h=0.05, outlet=False):
+    """Empty n^3 box (legacy MAC golden-vector grid when n=16, h=0.05, no outlet)."""
+    g = L.GeometryDesc()
+    g.kind = L.GEOM_NONE
+    d = L.GridDesc()
+    d.lo[:] = (0.0, 0.0, 0.0)
+    d.hi[:] = (n * h, n * h, n * h)
+    d.base_cells_x = n
+    d.levels = 1
+    d.min_volume_fraction = 0.1
+    d.subsamples = 2
+    d.outlet_xmax = 1 if outlet else 0
+    return g, d
+
+
+# Configurations of the benchmark table (cells_x chosen as described in the module docstring).
+CASES = {
+    "sphere64": dict(kind="sphere", args=dict(cells_x=128), rho=1.0, nu=1e-3, speed=1.0, cfl=0.5, steps=100),
+    "naca2412_2lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=192, levels=2, bands=(0.3,)),
+                          rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=500),
+    "naca2412_3lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=256, levels=3, bands=(1.0, 0.25)),
+                          rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=200),
+}
+
+
+def build_case(name, **overrides):
+    spec = dict(CASES[name])
+    args = dict(spec["args"])
+    args.update(overrides)
+    if spec["kind"] == "sphere":
+        g, d = sphere_box(**args)
+    else:
+        g, d = naca_wing(**args)
+    return g, d, spec

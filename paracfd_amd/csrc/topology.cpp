@@ -1,0 +1,1147 @@
+// Host preprocessing: geometry, nested AMR bricks, cut cells, compact edges, surface
+// patches and the geometric multigrid hierarchy.  See topology.hpp.
+#include "topology.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <thread>
+
+namespace pcfd {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ------------------------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------------------------
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int threads = static_cast<int>(std::max(1u, std::min(hw ? hw : 1u, 16u)));
+    if (n < 4096 || threads == 1) { body(0, n); return; }
+    std::vector<std::thread> pool;
+    std::atomic<int64_t> next{0};
+    const int64_t chunk = std::max<int64_t>(1024, n / (threads * 8));
+    for (int t = 0; t < threads; ++t) {
+        pool.emplace_back([&] {
+            for (;;) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                body(b, std::min(n, b + chunk));
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+inline uint64_t morton(int x, int y, int z) {
+    return spread3(static_cast<uint64_t>(x)) | (spread3(static_cast<uint64_t>(y)) << 1) |
+           (spread3(static_cast<uint64_t>(z)) << 2);
+}
+
+// ------------------------------------------------------------------------------------------
+// analytic solids
+// ------------------------------------------------------------------------------------------
+class Geometry {
+public:
+    virtual ~Geometry() = default;
+    virtual bool solid(double x, double y, double z) const = 0;
+    virtual void surface(double spacing, std::vector<double>& pts) const = 0;
+    double aabb_lo[3] = {0, 0, 0}, aabb_hi[3] = {-1, -1, -1};
+    double ref_area = 0;
+    bool empty() const { return aabb_hi[0] < aabb_lo[0]; }
+};
+
+class NoGeometry final : public Geometry {
+public:
+    bool solid(double, double, double) const override { return false; }
+    void surface(double, std::vector<double>&) const override {}
+};
+
+class Sphere final : public Geometry {
+public:
+    Sphere(const double c[3], double r) : r_(r) {
+        for (int a = 0; a < 3; ++a) { c_[a] = c[a]; aabb_lo[a] = c[a] - r; aabb_hi[a] = c[a] + r; }
+        ref_area = kPi * r * r;
+    }
+    bool solid(double x, double y, double z) const override {
+        const double dx = x - c_[0], dy = y - c_[1], dz = z - c_[2];
+        return dx * dx + dy * dy + dz * dz < r_ * r_;
+    }
+    void surface(double spacing, std::vector<double>& pts) const override {
+        const int n = std::max(64, static_cast<int>(4 * kPi * r_ * r_ / (spacing * spacing)) + 1);
+        const double golden = kPi * (3.0 - std::sqrt(5.0));
+        for (int i = 0; i < n; ++i) {
+            const double z = 1.0 - 2.0 * (i + 0.5) / n, rr = std::sqrt(std::max(0.0, 1 - z * z));
+            const double t = golden * i;
+            pts.push_back(c_[0] + r_ * rr * std::cos(t));
+            pts.push_back(c_[1] + r_ * rr * std::sin(t));
+            pts.push_back(c_[2] + r_ * z);
+        }
+    }
+
+private:
+    double c_[3], r_;
+};
+
+// 2-D closed polygon with a bucketed vertical-ray crossing test.
+class Polygon2 {
+public:
+    void set(std::vector<double> x, std::vector<double> z) {
+        x_ = std::move(x); z_ = std::move(z);
+        xmin_ = *std::min_element(x_.begin(), x_.end());
+        xmax_ = *std::max_element(x_.begin(), x_.end());
+        nb_ = 512;
+        buckets_.assign(nb_, {});
+        const int n = static_cast<int>(x_.size());
+        for (int s = 0; s < n; ++s) {
+            const int t = (s + 1) % n;
+            const double lo = std::min(x_[s], x_[t]), hi = std::max(x_[s], x_[t]);
+            const int b0 = bucket(lo), b1 = bucket(hi);
+            for (int b = b0; b <= b1; ++b) buckets_[b].push_back(s);
+        }
+    }
+    bool inside(double x, double z) const {
+        if (x < xmin_ || x > xmax_) return false;
+        const auto& list = buckets_[bucket(x)];
+        const int n = static_cast<int>(x_.size());
+        int crossings = 0;
+        for (int s : list) {
+            const int t = (s + 1) % n;
+            const double xs = x_[s], xt = x_[t];
+            if ((xs <= x && x < xt) || (xt <= x && x < xs)) {
+                const double zc = z_[s] + (z_[t] - z_[s]) * (x - xs) / (xt - xs);
+                if (zc > z) ++crossings;
+            }
+        }
+        return crossings & 1;
+    }
+    const std::vector<double>& xs() const { return x_; }
+    const std::vector<double>& zs() const { return z_; }
+    double xmin() const { return xmin_; }
+    double xmax() const { return xmax_; }
+
+private:
+    int bucket(double x) const {
+        int b = static_cast<int>((x - xmin_) / (xmax_ - xmin_) * nb_);
+        return std::clamp(b, 0, nb_ - 1);
+    }
+    std::vector<double> x_, z_;
+    double xmin_ = 0, xmax_ = 0;
+    int nb_ = 1;
+    std::vector<std::vector<int>> buckets_;
+};
+
+// NACA 4-digit section in chord units: camber line and closed-trailing-edge thickness with
+// the -0.1036 quartic coefficient (naca_step.cpp:85-92), thickness applied normal to the
+// camber line.  Returns the closed polygon TE -> upper -> LE -> lower.
+void naca4_polygon(int digits, int n, std::vector<double>& px, std::vector<double>& pz) {
+    const double m = (digits / 1000) / 100.0, p = ((digits / 100) % 10) / 10.0, t = (digits % 100) / 100.0;
+    std::vector<double> xu(n + 1), zu(n + 1), xl(n + 1), zl(n + 1);
+    for (int i = 0; i <= n; ++i) {
+        const double beta = kPi * i / n, x = 0.5 * (1 - std::cos(beta));
+        const double yt = 5 * t * (0.2969 * std::sqrt(x) - 0.1260 * x - 0.3516 * x * x + 0.2843 * x * x * x -
+                                   0.1036 * x * x * x * x);
+        double yc = 0, slope = 0;
+        if (m > 0) {
+            if (x < p) { yc = m / (p * p) * (2 * p * x - x * x); slope = 2 * m / (p * p) * (p - x); }
+            else { yc = m / ((1 - p) * (1 - p)) * ((1 - 2 * p) + 2 * p * x - x * x); slope = 2 * m / ((1 - p) * (1 - p)) * (p - x); }
+        }
+        const double th = std::atan(slope);
+        xu[i] = x - yt * std::sin(th); zu[i] = yc + yt * std::cos(th);
+        xl[i] = x + yt * std::sin(th); zl[i] = yc - yt * std::cos(th);
+    }
+    px.clear(); pz.clear();
+    for (int i = n; i >= 0; --i) { px.push_back(xu[i]); pz.push_back(zu[i]); }
+    for (int i = 1; i < n; ++i) { px.push_back(xl[i]); pz.push_back(zl[i]); }
+}
+
+class NacaWing final : public Geometry {
+public:
+    NacaWing(const pcfd_geometry_desc& g) {
+        std::vector<double> px, pz;
+        naca4_polygon(g.naca_digits, 400, px, pz);
+        poly_.set(px, pz);
+        for (int a = 0; a < 3; ++a) le_[a] = g.center[a];
+        chord_ = g.chord; half_span_ = 0.5 * g.span;
+        const double al = g.alpha_deg * kPi / 180.0;
+        ca_ = std::cos(al); sa_ = std::sin(al);
+        aabb_lo[0] = aabb_lo[2] = 1e300; aabb_hi[0] = aabb_hi[2] = -1e300;
+        for (size_t i = 0; i < px.size(); ++i) {
+            double w[3]; to_world(px[i], pz[i], 0, w);
+            aabb_lo[0] = std::min(aabb_lo[0], w[0]); aabb_hi[0] = std::max(aabb_hi[0], w[0]);
+            aabb_lo[2] = std::min(aabb_lo[2], w[2]); aabb_hi[2] = std::max(aabb_hi[2], w[2]);
+        }
+        aabb_lo[1] = le_[1] - half_span_; aabb_hi[1] = le_[1] + half_span_;
+        ref_area = g.chord * g.span;
+    }
+    void to_world(double xl, double zl, double y, double w[3]) const {
+        w[0] = le_[0] + chord_ * (xl * ca_ + zl * sa_);
+        w[1] = le_[1] + y;
+        w[2] = le_[2] + chord_ * (-xl * sa_ + zl * ca_);
+    }
+    bool solid(double x, double y, double z) const override {
+        if (std::abs(y - le_[1]) > half_span_) return false;
+        const double dx = x - le_[0], dz = z - le_[2];
+        const double xl = (dx * ca_ - dz * sa_) / chord_, zl = (dx * sa_ + dz * ca_) / chord_;
+        return poly_.inside(xl, zl);
+    }
+    void surface(double spacing, std::vector<double>& pts) const override {
+        // resample the section outline by arc length
+        const auto& px = poly_.xs(); const auto& pz = poly_.zs();
+        std::vector<double> sx, sz;
+        const int n = static_cast<int>(px.size());
+        const double ds = spacing / chord_;
+        for (int s = 0; s < n; ++s) {
+            const int t = (s + 1) % n;
+            const double len = std::hypot(px[t] - px[s], pz[t] - pz[s]);
+            const int k = std::max(1, static_cast<int>(std::ceil(len / ds)));
+            for (int q = 0; q < k; ++q) {
+                const double f = static_cast<double>(q) / k;
+                sx.push_back(px[s] + f * (px[t] - px[s]));
+                sz.push_back(pz[s] + f * (pz[t] - pz[s]));
+            }
+        }
+        const int ns = std::max(2, static_cast<int>(std::ceil(2 * half_span_ / spacing)) + 1);
+        for (int j = 0; j < ns; ++j) {
+            const double y = -half_span_ + 2 * half_span_ * j / (ns - 1);
+            for (size_t q = 0; q < sx.size(); ++q) {
+                double w[3]; to_world(sx[q], sz[q], y, w);
+                pts.insert(pts.end(), w, w + 3);
+            }
+        }
+        // tip caps
+        for (double xl = poly_.xmin(); xl <= poly_.xmax(); xl += ds)
+            for (double zl = -0.3; zl <= 0.3; zl += ds)
+                if (poly_.inside(xl, zl))
+                    for (double y : {-half_span_, half_span_}) {
+                        double w[3]; to_world(xl, zl, y, w);
+                        pts.insert(pts.end(), w, w + 3);
+                    }
+    }
+
+private:
+    Polygon2 poly_;
+    double le_[3], chord_, half_span_, ca_, sa_;
+};
+
+// Uniform hash grid of surface points for "is any point within r of this box" queries.
+class PointGrid {
+public:
+    void build(const std::vector<double>& pts, double cell) {
+        g_ = cell;
+        const int64_t n = static_cast<int64_t>(pts.size() / 3);
+        if (n == 0) { dim_[0] = dim_[1] = dim_[2] = 0; return; }
+        for (int a = 0; a < 3; ++a) { lo_[a] = 1e300; double hi = -1e300;
+            for (int64_t i = 0; i < n; ++i) { lo_[a] = std::min(lo_[a], pts[3 * i + a]); hi = std::max(hi, pts[3 * i + a]); }
+            lo_[a] -= cell; dim_[a] = static_cast<int>((hi - lo_[a]) / cell) + 2; }
+        const int64_t cells = static_cast<int64_t>(dim_[0]) * dim_[1] * dim_[2];
+        off_.assign(cells + 1, 0);
+        std::vector<int64_t> key(n);
+        for (int64_t i = 0; i < n; ++i) {
+            int c[3];
+            for (int a = 0; a < 3; ++a) c[a] = std::clamp(static_cast<int>((pts[3 * i + a] - lo_[a]) / cell), 0, dim_[a] - 1);
+            key[i] = (static_cast<int64_t>(c[2]) * dim_[1] + c[1]) * dim_[0] + c[0];
+            ++off_[key[i] + 1];
+        }
+        for (int64_t c = 0; c < cells; ++c) off_[c + 1] += off_[c];
+        pts_.resize(3 * n);
+        std::vector<int64_t> fill(off_.begin(), off_.end() - 1);
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t d = fill[key[i]]++;
+            for (int a = 0; a < 3; ++a) pts_[3 * d + a] = pts[3 * i + a];
+        }
+    }
+    bool near_box(const double blo[3], const double bhi[3], double r) const {
+        if (dim_[0] == 0) return false;
+        int c0[3], c1[3];
+        for (int a = 0; a < 3; ++a) {
+            c0[a] = static_cast<int>(std::floor((blo[a] - r - lo_[a]) / g_));
+            c1[a] = static_cast<int>(std::floor((bhi[a] + r - lo_[a]) / g_));
+            if (c1[a] < 0 || c0[a] >= dim_[a]) return false;
+            c0[a] = std::max(c0[a], 0); c1[a] = std::min(c1[a], dim_[a] - 1);
+        }
+        const double r2 = r * r;
+        for (int z = c0[2]; z <= c1[2]; ++z)
+            for (int y = c0[1]; y <= c1[1]; ++y)
+                for (int x = c0[0]; x <= c1[0]; ++x) {
+                    const int64_t c = (static_cast<int64_t>(z) * dim_[1] + y) * dim_[0] + x;
+                    for (int64_t q = off_[c]; q < off_[c + 1]; ++q) {
+                        double d2 = 0;
+                        for (int a = 0; a < 3; ++a) {
+                            const double p = pts_[3 * q + a];
+                            const double e = p < blo[a] ? blo[a] - p : (p > bhi[a] ? p - bhi[a] : 0.0);
+                            d2 += e * e;
+                        }
+                        if (d2 <= r2) return true;
+                    }
+                }
+        return false;
+    }
+
+private:
+    double lo_[3] = {0, 0, 0}, g_ = 1;
+    int dim_[3] = {0, 0, 0};
+    std::vector<int64_t> off_;
+    std::vector<double> pts_;
+};
+
+// ------------------------------------------------------------------------------------------
+// builder
+// ------------------------------------------------------------------------------------------
+constexpr int ST_ABSENT = -2, ST_REFINED = -1, ST_EMPTY = -3;
+
+struct LevelGrid {
+    int nb[3] = {0, 0, 0};
+    double h = 0;
+    std::vector<int> status;
+    int64_t idx(int x, int y, int z) const { return (static_cast<int64_t>(z) * nb[1] + y) * nb[0] + x; }
+    bool inside(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < nb[0] && y < nb[1] && z < nb[2]; }
+};
+
+struct CutData {
+    double f = 1;            // fluid volume fraction
+    double c[3] = {0, 0, 0}; // fluid centroid
+    double ap[6] = {1, 1, 1, 1, 1, 1}; // face apertures -x,+x,-y,+y,-z,+z
+    double wall[3] = {0, 0, 0};        // mean of fluid samples next to solid samples
+    int wall_n = 0;
+};
+
+// Cell state during the build: 0 solid, 1 full fluid (far from the surface), 2 sampled.
+struct Leaf {
+    int level;
+    int coord[3];
+    std::vector<uint8_t> state;   // 512
+    std::vector<int> cut;         // 512, index into cut list or -1
+};
+
+struct Neighbor { int leaf; int local; double ap; int level; };
+
+class Builder {
+public:
+    Builder(const pcfd_geometry_desc& g, const pcfd_grid_desc& d) : gd_(g), dd_(d) {}
+    Topology run();
+
+private:
+    const pcfd_geometry_desc gd_;
+    const pcfd_grid_desc dd_;
+    std::unique_ptr<Geometry> geo_;
+    Topology T_;
+    int L_ = 1;
+    std::vector<LevelGrid> grid_;
+    std::vector<Leaf> leaves_;
+    std::vector<CutData> cuts_;
+    std::vector<std::vector<uint8_t>> active_;  // per leaf, 512
+    int64_t total_cells_[3];
+
+    void setup_geometry();
+    void setup_domain();
+    void refine();
+    void classify();
+    // neighbours of (leaf, local) in direction dir (0..5): same level (1 entry), coarser (1), finer (4)
+    int neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boundary) const;
+    double face_ap(int leaf, int local, int dir) const {
+        const int ci = leaves_[leaf].cut[local];
+        return ci >= 0 ? cuts_[ci].ap[dir] : 1.0;
+    }
+    bool is_active(int leaf, int local) const { return active_[leaf][local] != 0; }
+    void components();
+    void finalize();
+    void build_multigrid();
+};
+
+void Builder::setup_geometry() {
+    switch (gd_.kind) {
+        case PCFD_GEOM_NONE: geo_ = std::make_unique<NoGeometry>(); break;
+        case PCFD_GEOM_SPHERE:
+            if (!(gd_.radius > 0)) throw BuildError("sphere radius must be positive");
+            geo_ = std::make_unique<Sphere>(gd_.center, gd_.radius); break;
+        case PCFD_GEOM_NACA4_WING:
+            if (!(gd_.chord > 0) || !(gd_.span > 0) || gd_.naca_digits <= 0 || gd_.naca_digits > 9999)
+                throw BuildError("invalid NACA wing description");
+            geo_ = std::make_unique<NacaWing>(gd_); break;
+        default: throw BuildError("unsupported geometry kind");
+    }
+}
+
+void Builder::setup_domain() {
+    L_ = dd_.levels;
+    if (L_ < 1 || L_ > 4) throw BuildError("levels must be in 1..4");
+    if (dd_.base_cells_x < kBS) throw BuildError("base_cells_x must be at least one brick");
+    for (int a = 0; a < 3; ++a)
+        if (!(dd_.hi[a] > dd_.lo[a])) throw BuildError("empty domain");
+    const double h0 = (dd_.hi[0] - dd_.lo[0]) / dd_.base_cells_x;
+    T_.h0 = h0;
+    T_.levels = L_;
+    T_.outlet = dd_.outlet_xmax != 0;
+    // x: pad downstream to whole bricks; y, z: pad symmetrically (amr_grid.cpp:66-96)
+    int cells[3];
+    cells[0] = (dd_.base_cells_x + kBS - 1) / kBS * kBS;
+    T_.lo[0] = dd_.lo[0];
+    for (int a = 1; a < 3; ++a) {
+        const double len = dd_.hi[a] - dd_.lo[a];
+        cells[a] = static_cast<int>(std::ceil(len / h0 - 1e-9));
+        cells[a] = (cells[a] + kBS - 1) / kBS * kBS;
+        const double centre = 0.5 * (dd_.lo[a] + dd_.hi[a]);
+        T_.lo[a] = centre - 0.5 * cells[a] * h0;
+    }
+    for (int a = 0; a < 3; ++a) T_.base_bricks[a] = cells[a] / kBS;
+    grid_.resize(L_);
+    T_.level_h.resize(L_);
+    for (int l = 0; l < L_; ++l) {
+        for (int a = 0; a < 3; ++a) grid_[l].nb[a] = T_.base_bricks[a] << l;
+        grid_[l].h = h0 / (1 << l);
+        T_.level_h[l] = grid_[l].h;
+        grid_[l].status.assign(static_cast<size_t>(grid_[l].nb[0]) * grid_[l].nb[1] * grid_[l].nb[2], ST_ABSENT);
+    }
+    T_.finest_h = grid_[L_ - 1].h;
+}
+
+void Builder::refine() {
+    // desired refinement by distance band, then proper nesting from fine to coarse
+    std::vector<std::vector<uint8_t>> R(L_);
+    for (int l = 0; l + 1 < L_; ++l) {
+        LevelGrid& G = grid_[l];
+        R[l].assign(G.status.size(), 0);
+        if (geo_->empty()) continue;
+        const double band = dd_.refine_band[l];
+        const double spacing = 0.5 * grid_[l + 1].h;
+        std::vector<double> pts;
+        geo_->surface(spacing, pts);
+        PointGrid pg;
+        pg.build(pts, std::max(band, 8 * G.h) * 0.5);
+        const double bh = kBS * G.h;
+        parallel_for(static_cast<int64_t>(G.status.size()), [&](int64_t b0, int64_t b1) {
+            for (int64_t b = b0; b < b1; ++b) {
+                const int x = static_cast<int>(b % G.nb[0]), y = static_cast<int>((b / G.nb[0]) % G.nb[1]),
+                          z = static_cast<int>(b / (static_cast<int64_t>(G.nb[0]) * G.nb[1]));
+                double lo[3] = {T_.lo[0] + x * bh, T_.lo[1] + y * bh, T_.lo[2] + z * bh};
+                double hi[3] = {lo[0] + bh, lo[1] + bh, lo[2] + bh};
+                R[l][b] = pg.near_box(lo, hi, band + spacing) ? 1 : 0;
+            }
+        });
+    }
+    // nesting: a refined level-l brick needs all 26 level-l neighbours to exist
+    for (int l = L_ - 2; l >= 1; --l) {
+        LevelGrid& G = grid_[l];
+        LevelGrid& P = grid_[l - 1];
+        for (int z = 0; z < G.nb[2]; ++z)
+            for (int y = 0; y < G.nb[1]; ++y)
+                for (int x = 0; x < G.nb[0]; ++x) {
+                    if (!R[l][G.idx(x, y, z)]) continue;
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                const int nx = x + dx, ny = y + dy, nz = z + dz;
+                                if (!G.inside(nx, ny, nz)) continue;
+                                R[l - 1][P.idx(nx >> 1, ny >> 1, nz >> 1)] = 1;
+                            }
+                }
+    }
+    // existence top-down
+    for (int l = 0; l < L_; ++l) {
+        LevelGrid& G = grid_[l];
+        for (int z = 0; z < G.nb[2]; ++z)
+            for (int y = 0; y < G.nb[1]; ++y)
+                for (int x = 0; x < G.nb[0]; ++x) {
+                    bool exists = l == 0 || grid_[l - 1].status[grid_[l - 1].idx(x >> 1, y >> 1, z >> 1)] == ST_REFINED;
+                    if (!exists) continue;
+                    const int64_t b = G.idx(x, y, z);
+                    if (l + 1 < L_ && R[l][b]) { G.status[b] = ST_REFINED; continue; }
+                    G.status[b] = static_cast<int>(leaves_.size());
+                    Leaf leaf;
+                    leaf.level = l; leaf.coord[0] = x; leaf.coord[1] = y; leaf.coord[2] = z;
+                    leaves_.push_back(std::move(leaf));
+                }
+    }
+}
+
+void Builder::classify() {
+    const int ns = std::max(1, dd_.subsamples);
+    std::vector<double> pts;
+    PointGrid pg;
+    double spacing = 0.5 * T_.finest_h;
+    if (!geo_->empty()) {
+        geo_->surface(spacing, pts);
+        pg.build(pts, 2 * T_.finest_h);
+    }
+    std::vector<std::vector<CutData>> local_cuts(leaves_.size());
+    parallel_for(static_cast<int64_t>(leaves_.size()), [&](int64_t b0, int64_t b1) {
+        for (int64_t lb = b0; lb < b1; ++lb) {
+            Leaf& leaf = leaves_[lb];
+            leaf.state.assign(kBC, 1);
+            leaf.cut.assign(kBC, -1);
+            const double h = grid_[leaf.level].h, bh = kBS * h;
+            double blo[3], bhi[3];
+            for (int a = 0; a < 3; ++a) { blo[a] = T_.lo[a] + leaf.coord[a] * bh; bhi[a] = blo[a] + bh; }
+            const bool near = pg.near_box(blo, bhi, spacing);
+            if (!near) {
+                const bool s = geo_->solid(0.5 * (blo[0] + bhi[0]), 0.5 * (blo[1] + bhi[1]), 0.5 * (blo[2] + bhi[2]));
+                if (s) std::fill(leaf.state.begin(), leaf.state.end(), 0);
+                continue;
+            }
+            for (int q = 0; q < kBC; ++q) {
+                const int i = q & 7, j = (q >> 3) & 7, k = q >> 6;
+                const int I[3] = {leaf.coord[0] * kBS + i, leaf.coord[1] * kBS + j, leaf.coord[2] * kBS + k};
+                double clo[3], chi[3];
+                for (int a = 0; a < 3; ++a) { clo[a] = T_.lo[a] + I[a] * h; chi[a] = clo[a] + h; }
+                if (!pg.near_box(clo, chi, spacing)) {
+                    if (geo_->solid(0.5 * (clo[0] + chi[0]), 0.5 * (clo[1] + chi[1]), 0.5 * (clo[2] + chi[2]))) leaf.state[q] = 0;
+                    continue;
+                }
+                CutData cd;
+                // volume samples
+                std::vector<uint8_t> sol(ns * ns * ns);
+                int fluid = 0;
+                double cx = 0, cy = 0, cz = 0;
+                for (int c = 0; c < ns; ++c)
+                    for (int b = 0; b < ns; ++b)
+                        for (int a = 0; a < ns; ++a) {
+                            const double x = T_.lo[0] + (I[0] + (a + 0.5) / ns) * h;
+                            const double y = T_.lo[1] + (I[1] + (b + 0.5) / ns) * h;
+                            const double z = T_.lo[2] + (I[2] + (c + 0.5) / ns) * h;
+                            const bool s = geo_->solid(x, y, z);
+                            sol[(c * ns + b) * ns + a] = s;
+                            if (!s) { ++fluid; cx += x; cy += y; cz += z; }
+                        }
+                cd.f = static_cast<double>(fluid) / (ns * ns * ns);
+                if (fluid > 0) { cd.c[0] = cx / fluid; cd.c[1] = cy / fluid; cd.c[2] = cz / fluid; }
+                // fluid samples adjacent to a solid sample approximate the wall
+                for (int c = 0; c < ns; ++c)
+                    for (int b = 0; b < ns; ++b)
+                        for (int a = 0; a < ns; ++a) {
+                            if (sol[(c * ns + b) * ns + a]) continue;
+                            bool touches = false;
+                            const int nb6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+                            for (auto& o : nb6) {
+                                const int aa = a + o[0], bb = b + o[1], cc = c + o[2];
+                                if (aa < 0 || bb < 0 || cc < 0 || aa >= ns || bb >= ns || cc >= ns) continue;
+                                if (sol[(cc * ns + bb) * ns + aa]) touches = true;
+                            }
+                            if (touches) {
+                                cd.wall[0] += T_.lo[0] + (I[0] + (a + 0.5) / ns) * h;
+                                cd.wall[1] += T_.lo[1] + (I[1] + (b + 0.5) / ns) * h;
+                                cd.wall[2] += T_.lo[2] + (I[2] + (c + 0.5) / ns) * h;
+                                ++cd.wall_n;
+                            }
+                        }
+                if (cd.wall_n) for (int a = 0; a < 3; ++a) cd.wall[a] /= cd.wall_n;
+                // face apertures from the face's own integer position (identical on both sides)
+                for (int dir = 0; dir < 6; ++dir) {
+                    const int axis = dir >> 1, side = dir & 1;
+                    const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+                    int open = 0;
+                    for (int b = 0; b < ns; ++b)
+                        for (int a = 0; a < ns; ++a) {
+                            double p[3];
+                            p[axis] = T_.lo[axis] + (I[axis] + side) * h;
+                            p[t1] = T_.lo[t1] + (I[t1] + (a + 0.5) / ns) * h;
+                            p[t2] = T_.lo[t2] + (I[t2] + (b + 0.5) / ns) * h;
+                            if (!geo_->solid(p[0], p[1], p[2])) ++open;
+                        }
+                    cd.ap[dir] = static_cast<double>(open) / (ns * ns);
+                }
+                leaf.state[q] = 2;
+                leaf.cut[q] = static_cast<int>(local_cuts[lb].size());
+                local_cuts[lb].push_back(cd);
+            }
+        }
+    });
+    // gather cut data into one list
+    for (size_t lb = 0; lb < leaves_.size(); ++lb) {
+        const int base = static_cast<int>(cuts_.size());
+        for (auto& c : leaves_[lb].cut) if (c >= 0) c += base;
+        cuts_.insert(cuts_.end(), local_cuts[lb].begin(), local_cuts[lb].end());
+    }
+    // small cut cells become solid (min_volume_fraction)
+    const double fmin = dd_.min_volume_fraction;
+    active_.assign(leaves_.size(), std::vector<uint8_t>(kBC, 0));
+    for (size_t lb = 0; lb < leaves_.size(); ++lb)
+        for (int q = 0; q < kBC; ++q) {
+            const Leaf& leaf = leaves_[lb];
+            bool act = leaf.state[q] == 1 || (leaf.state[q] == 2 && cuts_[leaf.cut[q]].f >= fmin && cuts_[leaf.cut[q]].f > 0);
+            active_[lb][q] = act;
+        }
+}
+
+int Builder::neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boundary) const {
+    boundary = false;
+    const Leaf& lf = leaves_[leaf];
+    const int l = lf.level;
+    const int axis = dir >> 1, sign = (dir & 1) ? 1 : -1;
+    int I[3] = {lf.coord[0] * kBS + (local & 7), lf.coord[1] * kBS + ((local >> 3) & 7), lf.coord[2] * kBS + (local >> 6)};
+    int J[3] = {I[0], I[1], I[2]};
+    J[axis] += sign;
+    const LevelGrid& G = grid_[l];
+    if (J[axis] < 0 || J[axis] >= G.nb[axis] * kBS) { boundary = true; return 0; }
+    const int st = G.status[G.idx(J[0] >> 3, J[1] >> 3, J[2] >> 3)];
+    auto loc = [](const int c[3]) { return (c[0] & 7) + 8 * (c[1] & 7) + 64 * (c[2] & 7); };
+    if (st >= 0) {
+        out[0] = {st, loc(J), 0.0, l};
+        // same level: both sides sampled the same face points
+        const int ci = lf.cut[local], cj = leaves_[st].cut[loc(J)];
+        out[0].ap = ci >= 0 ? cuts_[ci].ap[dir] : (cj >= 0 ? cuts_[cj].ap[dir ^ 1] : 1.0);
+        return 1;
+    }
+    if (st == ST_EMPTY) return 0;
+    if (st == ST_REFINED) {
+        // four finer cells across the face
+        const LevelGrid& F = grid_[l + 1];
+        int n = 0;
+        const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) {
+                int K[3];
+                K[axis] = sign > 0 ? 2 * J[axis] : 2 * J[axis] + 1;
+                K[t1] = 2 * J[t1] + a;
+                K[t2] = 2 * J[t2] + b;
+                const int fs = F.status[F.idx(K[0] >> 3, K[1] >> 3, K[2] >> 3)];
+                if (fs < 0) continue;  // empty brick
+                const int fq = loc(K);
+                const int cf = leaves_[fs].cut[fq];
+                out[n++] = {fs, fq, cf >= 0 ? cuts_[cf].ap[dir ^ 1] : 1.0, l + 1};
+            }
+        return n;
+    }
+    // absent at level l: the coarser leaf at level l-1 covers it
+    if (l == 0) throw BuildError("internal: absent level-0 brick");
+    const LevelGrid& C = grid_[l - 1];
+    int K[3] = {J[0] >> 1, J[1] >> 1, J[2] >> 1};
+    const int cs = C.status[C.idx(K[0] >> 3, K[1] >> 3, K[2] >> 3)];
+    if (cs == ST_EMPTY) return 0;
+    if (cs < 0) throw BuildError("internal: AMR nesting violated");
+    const int ci = lf.cut[local];
+    out[0] = {cs, loc(K), ci >= 0 ? cuts_[ci].ap[dir] : 1.0, l - 1};
+    return 1;
+}
+
+void Builder::components() {
+    // keep only fluid connected to the outlet (or the largest component without an outlet)
+    const int nl = static_cast<int>(leaves_.size());
+    std::vector<std::vector<int>> comp(nl, std::vector<int>(kBC, -1));
+    std::vector<int64_t> sizes;
+    std::vector<uint8_t> touches_outlet;
+    std::vector<std::pair<int, int>> stack;
+    for (int lb = 0; lb < nl; ++lb)
+        for (int q = 0; q < kBC; ++q) {
+            if (!active_[lb][q] || comp[lb][q] >= 0) continue;
+            const int id = static_cast<int>(sizes.size());
+            sizes.push_back(0);
+            touches_outlet.push_back(0);
+            stack.clear();
+            stack.push_back({lb, q});
+            comp[lb][q] = id;
+            while (!stack.empty()) {
+                auto [cl, cq] = stack.back();
+                stack.pop_back();
+                ++sizes[id];
+                for (int dir = 0; dir < 6; ++dir) {
+                    Neighbor nb[4]; bool bnd;
+                    const int n = neighbors(cl, cq, dir, nb, bnd);
+                    if (bnd && dir == 1) touches_outlet[id] = 1;
+                    for (int k = 0; k < n; ++k) {
+                        if (nb[k].ap <= 0 || !active_[nb[k].leaf][nb[k].local] || comp[nb[k].leaf][nb[k].local] >= 0) continue;
+                        comp[nb[k].leaf][nb[k].local] = id;
+                        stack.push_back({nb[k].leaf, nb[k].local});
+                    }
+                }
+            }
+        }
+    std::vector<uint8_t> keep(sizes.size(), 0);
+    if (T_.outlet) {
+        for (size_t c = 0; c < sizes.size(); ++c) keep[c] = touches_outlet[c];
+    } else if (!sizes.empty()) {
+        keep[std::max_element(sizes.begin(), sizes.end()) - sizes.begin()] = 1;
+    }
+    for (int lb = 0; lb < nl; ++lb)
+        for (int q = 0; q < kBC; ++q)
+            if (active_[lb][q] && !keep[comp[lb][q]]) active_[lb][q] = 0;
+}
+
+void Builder::finalize() {
+    // drop empty leaves; order kept leaves by level then Morton order
+    const int nl = static_cast<int>(leaves_.size());
+    std::vector<int> order;
+    for (int lb = 0; lb < nl; ++lb) {
+        bool any = false;
+        for (int q = 0; q < kBC && !any; ++q) any = active_[lb][q];
+        const Leaf& lf = leaves_[lb];
+        if (!any) grid_[lf.level].status[grid_[lf.level].idx(lf.coord[0], lf.coord[1], lf.coord[2])] = ST_EMPTY;
+        else order.push_back(lb);
+    }
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        const Leaf& A = leaves_[a]; const Leaf& B = leaves_[b];
+        if (A.level != B.level) return A.level < B.level;
+        return morton(A.coord[0], A.coord[1], A.coord[2]) < morton(B.coord[0], B.coord[1], B.coord[2]);
+    });
+    std::vector<int> newid(nl, -1);
+    for (size_t i = 0; i < order.size(); ++i) newid[order[i]] = static_cast<int>(i);
+    const int nb = static_cast<int>(order.size());
+    T_.nbricks = nb;
+    T_.ndof = static_cast<int64_t>(nb) * kBC;
+    T_.brick_level.resize(nb);
+    T_.brick_coord.resize(3 * nb);
+    T_.brick_flags.assign(nb, 0);
+    T_.brick_nbr.assign(6 * nb, -1);
+    T_.level_brick_begin.assign(L_ + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        const Leaf& lf = leaves_[order[b]];
+        T_.brick_level[b] = lf.level;
+        for (int a = 0; a < 3; ++a) T_.brick_coord[3 * b + a] = lf.coord[a];
+        const LevelGrid& G = grid_[lf.level];
+        uint8_t fl = 0;
+        if (lf.coord[0] == 0) fl |= BF_XMIN;
+        if (lf.coord[0] == G.nb[0] - 1) fl |= BF_XMAX;
+        if (lf.coord[1] == 0) fl |= BF_YMIN;
+        if (lf.coord[1] == G.nb[1] - 1) fl |= BF_YMAX;
+        if (lf.coord[2] == 0) fl |= BF_ZMIN;
+        if (lf.coord[2] == G.nb[2] - 1) fl |= BF_ZMAX;
+        T_.brick_flags[b] = fl;
+        for (int dir = 0; dir < 6; ++dir) {
+            int c[3] = {lf.coord[0], lf.coord[1], lf.coord[2]};
+            c[dir >> 1] += (dir & 1) ? 1 : -1;
+            if (!G.inside(c[0], c[1], c[2])) continue;
+            const int st = G.status[G.idx(c[0], c[1], c[2])];
+            if (st >= 0) T_.brick_nbr[6 * b + dir] = newid[st];
+        }
+    }
+    for (int l = 0; l <= L_; ++l)
+        T_.level_brick_begin[l] = static_cast<int>(std::lower_bound(T_.brick_level.begin(), T_.brick_level.end(), l) - T_.brick_level.begin());
+    // leaf status now holds the final brick index
+    for (int l = 0; l < L_; ++l)
+        for (auto& s : grid_[l].status) if (s >= 0) s = newid[s];
+    {
+        std::vector<Leaf> nl2(nb);
+        std::vector<std::vector<uint8_t>> na(nb);
+        for (int b = 0; b < nb; ++b) { nl2[b] = std::move(leaves_[order[b]]); na[b] = std::move(active_[order[b]]); }
+        leaves_ = std::move(nl2);
+        active_ = std::move(na);
+    }
+
+    // masks, compact edges, volumes, centroids, closure area vectors
+    T_.mask.assign(T_.ndof, 0);
+    T_.centroid.assign(3 * T_.ndof, 0.0);
+    std::vector<double> volume(T_.ndof, 0.0);
+    std::vector<double> S(3 * T_.ndof, 0.0);
+    std::vector<uint8_t> has_compact(T_.ndof, 0);
+    std::vector<double> diffusion_sum(T_.ndof, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        const Leaf& lf = leaves_[b];
+        const double h = grid_[lf.level].h;
+        for (int q = 0; q < kBC; ++q) {
+            const int64_t a = static_cast<int64_t>(b) * kBC + q;
+            const int I[3] = {lf.coord[0] * kBS + (q & 7), lf.coord[1] * kBS + ((q >> 3) & 7), lf.coord[2] * kBS + (q >> 6)};
+            if (!active_[b][q]) {
+                for (int ax = 0; ax < 3; ++ax) T_.centroid[3 * a + ax] = T_.lo[ax] + (I[ax] + 0.5) * h;
+                continue;
+            }
+            T_.mask[a] |= CM_ACTIVE;
+            const int ci = lf.cut[q];
+            volume[a] = (ci >= 0 ? cuts_[ci].f : 1.0) * h * h * h;
+            for (int ax = 0; ax < 3; ++ax)
+                T_.centroid[3 * a + ax] = ci >= 0 ? cuts_[ci].c[ax] : T_.lo[ax] + (I[ax] + 0.5) * h;
+            for (int dir = 0; dir < 6; ++dir) {
+                const int axis = dir >> 1, up = dir & 1;
+                const double nsign = up ? 1.0 : -1.0;
+                Neighbor nbs[4]; bool bnd;
+                const int n = neighbors(b, q, dir, nbs, bnd);
+                double open = 0;
+                if (bnd) {
+                    if (ci >= 0 && cuts_[ci].ap[dir] < 1.0)
+                        throw BuildError("a cut cell touches the domain boundary; enlarge the domain");
+                    open = h * h;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const Neighbor& nb2 = nbs[k];
+                    if (!active_[nb2.leaf][nb2.local] || nb2.ap <= 0) continue;
+                    const double hf = std::min(h, grid_[nb2.level].h);
+                    const double area = nb2.ap * hf * hf;
+                    open += area;
+                    const int64_t other = static_cast<int64_t>(nb2.leaf) * kBC + nb2.local;
+                    const double hn = grid_[nb2.level].h;
+                    const double coef = area / (0.5 * (h + hn));
+                    diffusion_sum[a] += coef;
+                    if (nb2.level == lf.level && nb2.ap >= 1.0) {
+                        T_.mask[a] |= up ? (CM_PX << axis) : (CM_MX << axis);
+                        continue;
+                    }
+                    // compact edge, created once: same level from the lower cell, c/f from the fine cell
+                    const bool create = nb2.level == lf.level ? up : nb2.level < lf.level;
+                    has_compact[a] = 1;
+                    if (!create) continue;
+                    const int64_t lower = up ? a : other, upper = up ? other : a;
+                    const double hl = up ? h : hn, hu = up ? hn : h;
+                    T_.edge_a.push_back(static_cast<int>(lower));
+                    T_.edge_b.push_back(static_cast<int>(upper));
+                    T_.edge_axis.push_back(static_cast<int8_t>(axis));
+                    T_.edge_area.push_back(area);
+                    T_.edge_coef.push_back(coef);
+                    T_.edge_w.push_back(hl / (hl + hu));
+                }
+                S[3 * a + axis] -= nsign * open;
+            }
+        }
+    }
+    // boundary-face diffusion does not exist; the outlet adds nothing either (zero flux)
+    // special cells and patches
+    int64_t n_active = 0;
+    double max_rate = 0;
+    T_.brick_special_off.assign(nb + 1, 0);
+    std::vector<int> special_of(T_.ndof, -1);
+    for (int b = 0; b < nb; ++b) {
+        T_.brick_special_off[b] = static_cast<int>(T_.special_dof.size());
+        const double h = grid_[leaves_[b].level].h;
+        for (int q = 0; q < kBC; ++q) {
+            const int64_t a = static_cast<int64_t>(b) * kBC + q;
+            if (!(T_.mask[a] & CM_ACTIVE)) continue;
+            ++n_active;
+            max_rate = std::max(max_rate, diffusion_sum[a] / volume[a]);
+            const double smag = std::sqrt(S[3 * a] * S[3 * a] + S[3 * a + 1] * S[3 * a + 1] + S[3 * a + 2] * S[3 * a + 2]);
+            const bool wall = smag > 1e-12 * h * h;
+            const bool partial = volume[a] != h * h * h;
+            if (!(wall || partial || has_compact[a])) continue;
+            T_.mask[a] |= CM_SPECIAL;
+            special_of[a] = static_cast<int>(T_.special_dof.size());
+            T_.special_dof.push_back(static_cast<int>(a));
+            T_.special_volume.push_back(volume[a]);
+            if (wall) {
+                T_.special_patch.push_back(static_cast<int>(T_.patch_dof.size()));
+                T_.patch_dof.push_back(static_cast<int>(a));
+                for (int ax = 0; ax < 3; ++ax) T_.patch_S.push_back(S[3 * a + ax]);
+                const int ci = leaves_[b].cut[q];
+                if (ci >= 0 && cuts_[ci].wall_n > 0) {
+                    for (int ax = 0; ax < 3; ++ax) T_.patch_x.push_back(cuts_[ci].wall[ax]);
+                } else {
+                    for (int ax = 0; ax < 3; ++ax) T_.patch_x.push_back(T_.centroid[3 * a + ax] + 0.5 * h * S[3 * a + ax] / smag);
+                }
+            } else {
+                T_.special_patch.push_back(-1);
+            }
+        }
+    }
+    T_.brick_special_off[nb] = static_cast<int>(T_.special_dof.size());
+    T_.n_active = n_active;
+    T_.max_diffusion_coefficient = max_rate;
+    // incidence CSR of special cells, edges in ascending order
+    const int nsp = static_cast<int>(T_.special_dof.size());
+    T_.special_inc_off.assign(nsp + 1, 0);
+    const int ne = static_cast<int>(T_.edge_a.size());
+    for (int e = 0; e < ne; ++e) {
+        ++T_.special_inc_off[special_of[T_.edge_a[e]] + 1];
+        ++T_.special_inc_off[special_of[T_.edge_b[e]] + 1];
+    }
+    for (int s = 0; s < nsp; ++s) T_.special_inc_off[s + 1] += T_.special_inc_off[s];
+    T_.special_inc.assign(T_.special_inc_off[nsp], 0);
+    {
+        std::vector<int> fill(T_.special_inc_off.begin(), T_.special_inc_off.end() - 1);
+        for (int e = 0; e < ne; ++e) {
+            T_.special_inc[fill[special_of[T_.edge_a[e]]]++] = e;
+            T_.special_inc[fill[special_of[T_.edge_b[e]]]++] = ~e;
+        }
+    }
+    // least-squares wall-pressure reconstruction: p_wall = p_a + sum beta_q (p_q - p_a)
+    const int np = static_cast<int>(T_.patch_dof.size());
+    T_.patch_lsq_off.assign(np + 1, 0);
+    for (int pch = 0; pch < np; ++pch) {
+        T_.patch_lsq_off[pch] = static_cast<int>(T_.patch_lsq_dof.size());
+        const int64_t a = T_.patch_dof[pch];
+        const int b = static_cast<int>(a / kBC), q = static_cast<int>(a % kBC);
+        std::vector<int64_t> nbr;
+        for (int dir = 0; dir < 6; ++dir) {
+            Neighbor nbs[4]; bool bnd;
+            const int n = neighbors(b, q, dir, nbs, bnd);
+            for (int k = 0; k < n; ++k)
+                if (active_[nbs[k].leaf][nbs[k].local] && nbs[k].ap > 0)
+                    nbr.push_back(static_cast<int64_t>(nbs[k].leaf) * kBC + nbs[k].local);
+        }
+        double M[9] = {0};
+        for (int64_t o : nbr)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    M[3 * r + c] += (T_.centroid[3 * o + r] - T_.centroid[3 * a + r]) * (T_.centroid[3 * o + c] - T_.centroid[3 * a + c]);
+        const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
+        const double tr = (M[0] + M[4] + M[8]) / 3.0;
+        if (nbr.size() < 3 || !(det > 1e-6 * tr * tr * tr)) continue;
+        double inv[9];
+        inv[0] = (M[4] * M[8] - M[5] * M[7]) / det; inv[1] = (M[2] * M[7] - M[1] * M[8]) / det; inv[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+        inv[3] = (M[5] * M[6] - M[3] * M[8]) / det; inv[4] = (M[0] * M[8] - M[2] * M[6]) / det; inv[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+        inv[6] = (M[3] * M[7] - M[4] * M[6]) / det; inv[7] = (M[1] * M[6] - M[0] * M[7]) / det; inv[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+        double r[3];
+        for (int ax = 0; ax < 3; ++ax) r[ax] = T_.patch_x[3 * pch + ax] - T_.centroid[3 * a + ax];
+        double g[3];
+        for (int i = 0; i < 3; ++i) g[i] = inv[3 * i] * r[0] + inv[3 * i + 1] * r[1] + inv[3 * i + 2] * r[2];
+        for (int64_t o : nbr) {
+            double beta = 0;
+            for (int ax = 0; ax < 3; ++ax) beta += g[ax] * (T_.centroid[3 * o + ax] - T_.centroid[3 * a + ax]);
+            T_.patch_lsq_dof.push_back(static_cast<int>(o));
+            T_.patch_lsq_beta.push_back(beta);
+        }
+    }
+    T_.patch_lsq_off[np] = static_cast<int>(T_.patch_lsq_dof.size());
+    // solid volume of the body: every leaf cell's non-fluid part that lies inside the geometry
+    double body = 0;
+    for (int l = 0; l < L_; ++l) {
+        const LevelGrid& G = grid_[l];
+        const double h3 = G.h * G.h * G.h;
+        for (size_t s = 0; s < G.status.size(); ++s) {
+            const int st = G.status[s];
+            if (st == ST_EMPTY) body += kBC * h3;
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        const double h3 = std::pow(grid_[leaves_[b].level].h, 3);
+        for (int q = 0; q < kBC; ++q) {
+            const int64_t a = static_cast<int64_t>(b) * kBC + q;
+            body += h3 - volume[a];
+        }
+    }
+    T_.body_volume = body;
+    T_.reference_area = geo_->ref_area;
+}
+
+// Geometric multigrid by coarsening toward a uniform grid: at level m every cell smaller
+// than h_finest*2^m is merged with its siblings; coarse couplings are re-discretised as
+// open area / centre distance.  Transfers are piecewise constant (R = P^T).
+void Builder::build_multigrid() {
+    T_.mg.clear();
+    MGLevel L0;
+    L0.n = static_cast<int>(T_.ndof);
+    L0.exponent.assign(T_.ndof, 0);
+    // per-cell integer coordinates at its own exponent
+    std::vector<int> coord(3 * T_.ndof, 0);
+    for (int b = 0; b < T_.nbricks; ++b) {
+        const int l = T_.brick_level[b];
+        for (int q = 0; q < kBC; ++q) {
+            const int64_t a = static_cast<int64_t>(b) * kBC + q;
+            L0.exponent[a] = L_ - 1 - l;
+            coord[3 * a] = T_.brick_coord[3 * b] * kBS + (q & 7);
+            coord[3 * a + 1] = T_.brick_coord[3 * b + 1] * kBS + ((q >> 3) & 7);
+            coord[3 * a + 2] = T_.brick_coord[3 * b + 2] * kBS + (q >> 6);
+        }
+    }
+    // level-0 faces as (a, b, axis, area) records, outlet areas per cell
+    struct Face { int a, b; int axis; double area; };
+    std::vector<Face> faces;
+    std::vector<double> outlet(T_.ndof, 0.0);
+    for (int b = 0; b < T_.nbricks; ++b) {
+        const double h = T_.level_h[T_.brick_level[b]];
+        for (int q = 0; q < kBC; ++q) {
+            const int64_t a = static_cast<int64_t>(b) * kBC + q;
+            const uint8_t m = T_.mask[a];
+            if (!(m & CM_ACTIVE)) continue;
+            for (int axis = 0; axis < 3; ++axis) {
+                if (!(m & (CM_PX << axis))) continue;
+                int c[3] = {q & 7, (q >> 3) & 7, q >> 6};
+                int nbk = b;
+                if (++c[axis] == kBS) { c[axis] = 0; nbk = T_.brick_nbr[6 * b + 2 * axis + 1]; }
+                faces.push_back({static_cast<int>(a), nbk * kBC + c[0] + 8 * c[1] + 64 * c[2], axis, h * h});
+            }
+            if (T_.outlet && (T_.brick_flags[b] & BF_XMAX) && (q & 7) == 7) outlet[a] = h * h;
+        }
+    }
+    for (size_t e = 0; e < T_.edge_a.size(); ++e)
+        faces.push_back({T_.edge_a[e], T_.edge_b[e], T_.edge_axis[e], T_.edge_area[e]});
+    std::vector<uint8_t> act(T_.ndof);
+    for (int64_t a = 0; a < T_.ndof; ++a) act[a] = (T_.mask[a] & CM_ACTIVE) ? 1 : 0;
+
+    T_.mg.push_back(std::move(L0));
+    const double hf = T_.finest_h;
+    const int coarse_max = 1024;
+    int m = 0;
+    std::vector<int> cur_exp = T_.mg[0].exponent;
+    std::vector<int> cur_coord = coord;
+    std::vector<uint8_t> cur_act = act;
+    std::vector<double> cur_outlet = outlet;
+    int64_t cur_n = T_.ndof;
+    while (true) {
+        // count active cells at the current level
+        int64_t nact = 0;
+        for (int64_t a = 0; a < cur_n; ++a) nact += cur_act[a];
+        if (m >= 1 && (nact <= coarse_max || m >= 10)) break;
+        const int mm = m + 1;
+        // parent keys
+        std::vector<std::pair<uint64_t, int64_t>> keys;
+        keys.reserve(nact);
+        for (int64_t a = 0; a < cur_n; ++a) {
+            if (!cur_act[a]) continue;
+            const int e = cur_exp[a], ep = std::max(e, mm), sh = ep - e;
+            const uint64_t key = (static_cast<uint64_t>(ep) << 60) |
+                                 morton(cur_coord[3 * a] >> sh, cur_coord[3 * a + 1] >> sh, cur_coord[3 * a + 2] >> sh);
+            keys.push_back({key, a});
+        }
+        std::sort(keys.begin(), keys.end());
+        MGLevel C;
+        std::vector<int>& parent = T_.mg[m].parent;
+        parent.assign(cur_n, -1);
+        std::vector<int> nexp, ncoord;
+        int nc = -1;
+        uint64_t last = ~0ULL;
+        for (auto& [key, a] : keys) {
+            if (key != last) {
+                ++nc; last = key;
+                const int e = cur_exp[a], ep = std::max(e, mm), sh = ep - e;
+                nexp.push_back(ep);
+                ncoord.push_back(cur_coord[3 * a] >> sh);
+                ncoord.push_back(cur_coord[3 * a + 1] >> sh);
+                ncoord.push_back(cur_coord[3 * a + 2] >> sh);
+            }
+            parent[a] = nc;
+        }
+        C.n = nc + 1;
+        C.exponent = nexp;
+        // children CSR
+        C.child_off.assign(C.n + 1, 0);
+        for (int64_t a = 0; a < cur_n; ++a) if (parent[a] >= 0) ++C.child_off[parent[a] + 1];
+        for (int c = 0; c < C.n; ++c) C.child_off[c + 1] += C.child_off[c];
+        C.child.assign(C.child_off[C.n], 0);
+        {
+            std::vector<int> fill(C.child_off.begin(), C.child_off.end() - 1);
+            for (int64_t a = 0; a < cur_n; ++a) if (parent[a] >= 0) C.child[fill[parent[a]]++] = static_cast<int>(a);
+        }
+        // coarse faces: each face has a unique owner slot (finer side, or the lower side for equal sizes)
+        std::vector<int> slot_nbr(6 * static_cast<size_t>(C.n), -1);
+        std::vector<double> slot_area(6 * static_cast<size_t>(C.n), 0.0);
+        for (const Face& f : faces) {
+            const int pa = parent[f.a], pb = parent[f.b];
+            if (pa < 0 || pb < 0 || pa == pb) continue;
+            int owner, dir, other;
+            if (nexp[pa] <= nexp[pb]) { owner = pa; dir = 2 * f.axis + 1; other = pb; }
+            else { owner = pb; dir = 2 * f.axis; other = pa; }
+            if (nexp[pa] == nexp[pb]) { owner = pa; dir = 2 * f.axis + 1; other = pb; }
+            const size_t s = 6 * static_cast<size_t>(owner) + dir;
+            if (slot_nbr[s] >= 0 && slot_nbr[s] != other) throw BuildError("internal: inconsistent coarse face");
+            slot_nbr[s] = other;
+            slot_area[s] += f.area;
+        }
+        std::vector<double> coutlet(C.n, 0.0);
+        for (int64_t a = 0; a < cur_n; ++a) if (parent[a] >= 0) coutlet[parent[a]] += cur_outlet[a];
+        // CSR (symmetric)
+        std::vector<int> deg(C.n, 0);
+        for (int c = 0; c < C.n; ++c)
+            for (int d = 0; d < 6; ++d)
+                if (slot_nbr[6 * c + d] >= 0) { ++deg[c]; ++deg[slot_nbr[6 * c + d]]; }
+        C.row_off.assign(C.n + 1, 0);
+        for (int c = 0; c < C.n; ++c) C.row_off[c + 1] = C.row_off[c] + deg[c];
+        C.col.assign(C.row_off[C.n], 0);
+        C.coef.assign(C.row_off[C.n], 0.0);
+        C.diag.assign(C.n, 0.0);
+        {
+            std::vector<int> fill(C.row_off.begin(), C.row_off.end() - 1);
+            for (int c = 0; c < C.n; ++c)
+                for (int d = 0; d < 6; ++d) {
+                    const int o = slot_nbr[6 * c + d];
+                    if (o < 0) continue;
+                    const double sc = hf * std::ldexp(1.0, nexp[c]), so = hf * std::ldexp(1.0, nexp[o]);
+                    const double k = slot_area[6 * c + d] / (0.5 * (sc + so));
+                    C.col[fill[c]] = o; C.coef[fill[c]++] = k;
+                    C.col[fill[o]] = c; C.coef[fill[o]++] = k;
+                    C.diag[c] += k; C.diag[o] += k;
+                }
+            // deterministic row order: sort each row by column
+            for (int c = 0; c < C.n; ++c) {
+                std::vector<std::pair<int, double>> row;
+                for (int q = C.row_off[c]; q < C.row_off[c + 1]; ++q) row.push_back({C.col[q], C.coef[q]});
+                std::sort(row.begin(), row.end());
+                for (size_t q = 0; q < row.size(); ++q) { C.col[C.row_off[c] + q] = row[q].first; C.coef[C.row_off[c] + q] = row[q].second; }
+            }
+        }
+        for (int c = 0; c < C.n; ++c) {
+            const double sc = hf * std::ldexp(1.0, nexp[c]);
+            C.diag[c] += coutlet[c] / (0.5 * sc);
+        }
+        // faces of the new level for the next coarsening
+        std::vector<Face> nfaces;
+        nfaces.reserve(C.row_off[C.n] / 2);
+        for (int c = 0; c < C.n; ++c)
+            for (int d = 0; d < 6; ++d) {
+                const int o = slot_nbr[6 * c + d];
+                if (o < 0) continue;
+                if (d & 1) nfaces.push_back({c, o, d >> 1, slot_area[6 * c + d]});
+                else nfaces.push_back({o, c, d >> 1, slot_area[6 * c + d]});
+            }
+        faces = std::move(nfaces);
+        cur_exp = nexp;
+        cur_coord = ncoord;
+        cur_act.assign(C.n, 1);
+        cur_outlet = coutlet;
+        cur_n = C.n;
+        T_.mg.push_back(std::move(C));
+        m = mm;
+    }
+    T_.mg.back().parent.assign(T_.mg.back().n, -1);
+    // dense inverse of the coarsest operator (Cholesky); skipped when singular
+    MGLevel& K = T_.mg.back();
+    if (T_.mg.size() > 1 && K.n <= 4096) {
+        const int n = K.n;
+        std::vector<double> A(static_cast<size_t>(n) * n, 0.0);
+        for (int r = 0; r < n; ++r) {
+            A[static_cast<size_t>(r) * n + r] = K.diag[r];
+            for (int q = K.row_off[r]; q < K.row_off[r + 1]; ++q) A[static_cast<size_t>(r) * n + K.col[q]] -= K.coef[q];
+        }
+        // Cholesky A = L L^T in place (lower)
+        bool ok = true;
+        for (int j = 0; j < n && ok; ++j) {
+            double d = A[static_cast<size_t>(j) * n + j];
+            for (int k = 0; k < j; ++k) d -= A[static_cast<size_t>(j) * n + k] * A[static_cast<size_t>(j) * n + k];
+            if (!(d > 1e-14 * K.diag[j])) { ok = false; break; }
+            d = std::sqrt(d);
+            A[static_cast<size_t>(j) * n + j] = d;
+            for (int i = j + 1; i < n; ++i) {
+                double s = A[static_cast<size_t>(i) * n + j];
+                for (int k = 0; k < j; ++k) s -= A[static_cast<size_t>(i) * n + k] * A[static_cast<size_t>(j) * n + k];
+                A[static_cast<size_t>(i) * n + j] = s / d;
+            }
+        }
+        if (ok) {
+            // inverse column by column
+            K.coarse_inv.assign(static_cast<size_t>(n) * n, 0.0);
+            std::vector<double> y(n);
+            for (int c = 0; c < n; ++c) {
+                for (int i = 0; i < n; ++i) {
+                    double s = i == c ? 1.0 : 0.0;
+                    for (int k = 0; k < i; ++k) s -= A[static_cast<size_t>(i) * n + k] * y[k];
+                    y[i] = s / A[static_cast<size_t>(i) * n + i];
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double s = y[i];
+                    for (int k = i + 1; k < n; ++k) s -= A[static_cast<size_t>(k) * n + i] * y[k];
+                    y[i] = s / A[static_cast<size_t>(i) * n + i];
+                }
+                for (int i = 0; i < n; ++i) K.coarse_inv[static_cast<size_t>(i) * n + c] = y[i];
+            }
+        }
+    }
+}
+
+Topology Builder::run() {
+    setup_geometry();
+    setup_domain();
+    refine();
+    classify();
+    components();
+    finalize();
+    build_multigrid();
+    return std::move(T_);
+}
+
+}  // namespace
+
+Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid) {
+    Builder b(geometry, grid);
+    return b.run();
+}
+
+}  // namespace pcfd

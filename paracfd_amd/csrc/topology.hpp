@@ -1,0 +1,104 @@
+// Static preprocessing for one wing/sphere case: 2:1 nested AMR bricks of 8^3 cubic cells,
+// sub-sampled cut cells, compact (coarse/fine and partially open) face records, surface
+// patches and the geometric multigrid hierarchy.  Host only.
+//
+// Plays the role of AmrHierarchy::build_static (src/core/fluid/amr_grid.cpp:103-142),
+// build_amr_embedded_boundary_atlas (amr_eb.cpp:82-375) and
+// build_composite_amr_pressure_system (amr_pressure.cpp:317-445).  The cut-cell geometry
+// is this project's own analytic-solid sub-sampler; the reference's BSP builder
+// (solid_embedded_boundary.cpp:566) needs OCCT and is not reproduced.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/paracfd.h"
+
+namespace pcfd {
+
+constexpr int kBS = 8;          // brick edge in cells
+constexpr int kBC = 512;        // cells per brick
+
+// Domain-boundary bits of a brick (BRICK_XMIN.. in amr_grid.h).
+enum BrickFlag : uint8_t { BF_XMIN = 1, BF_XMAX = 2, BF_YMIN = 4, BF_YMAX = 8, BF_ZMIN = 16, BF_ZMAX = 32 };
+
+// Per-cell mask.  Bits 0-2: the +x/+y/+z face is a structured face (same level, fully open,
+// both cells active); bits 3-5: the same for the -x/-y/-z face; bit 6: active fluid cell;
+// bit 7: special cell (has compact edges, an embedded-boundary patch or a partial volume).
+enum CellMask : uint8_t {
+    CM_PX = 1, CM_PY = 2, CM_PZ = 4, CM_MX = 8, CM_MY = 16, CM_MZ = 32, CM_ACTIVE = 64, CM_SPECIAL = 128
+};
+
+struct MGLevel {
+    int n = 0;
+    std::vector<int> row_off, col;      // CSR of off-diagonal couplings (symmetric)
+    std::vector<double> coef;           // positive coupling: (A x)_a = diag_a x_a - sum coef x_b
+    std::vector<double> diag;
+    std::vector<int> parent;            // index in level m+1, -1 on the coarsest
+    std::vector<int> child_off, child;  // children in level m-1 (for m >= 1)
+    std::vector<int> exponent;          // cell size = h_finest * 2^exponent
+    std::vector<double> coarse_inv;     // dense inverse on the coarsest level (row-major)
+};
+
+struct Topology {
+    // grid
+    double lo[3] = {0, 0, 0};
+    double h0 = 0;
+    int levels = 1;
+    int base_bricks[3] = {0, 0, 0};
+    bool outlet = true;
+    std::vector<double> level_h;
+
+    // leaf, non-empty bricks sorted by level then Morton order
+    int nbricks = 0;
+    std::vector<int> brick_level;
+    std::vector<int> brick_coord;       // 3 per brick
+    std::vector<uint8_t> brick_flags;
+    std::vector<int> brick_nbr;         // 6 per brick: -x,+x,-y,+y,-z,+z same-level neighbour
+    std::vector<int> level_brick_begin; // levels+1
+
+    // cells (dof = brick*512 + i + 8 j + 64 k)
+    int64_t ndof = 0;
+    int64_t n_active = 0;
+    std::vector<uint8_t> mask;
+    std::vector<double> centroid;       // 3 per dof
+
+    // special cells in dof order
+    std::vector<int> special_dof;
+    std::vector<double> special_volume;
+    std::vector<int> special_inc_off;
+    std::vector<int> special_inc;       // e >= 0: lower side of edge e; ~e: upper side
+    std::vector<int> special_patch;
+    std::vector<int> brick_special_off; // nbricks+1
+
+    // compact edges
+    std::vector<int> edge_a, edge_b;
+    std::vector<int8_t> edge_axis;
+    std::vector<double> edge_area, edge_coef, edge_w;
+
+    // embedded-boundary surface patches (one per cut cell with a wall)
+    std::vector<int> patch_dof;
+    std::vector<double> patch_S, patch_x;  // 3 per patch
+    std::vector<int> patch_lsq_off, patch_lsq_dof;
+    std::vector<double> patch_lsq_beta;
+
+    // multigrid: mg[0] is the composite level (only parent/exponent are filled there)
+    std::vector<MGLevel> mg;
+
+    double body_volume = 0;
+    double reference_area = 0;
+    double max_diffusion_coefficient = 0; // max over cells of sum(coef)/V (for diffusion substeps)
+    double finest_h = 0;
+};
+
+struct BuildError : std::exception {
+    std::string message;
+    explicit BuildError(std::string m) : message(std::move(m)) {}
+    const char* what() const noexcept override { return message.c_str(); }
+};
+
+Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid);
+
+}  // namespace pcfd
