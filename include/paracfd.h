@@ -249,6 +249,14 @@ int64_t pcfd_device_bytes(const pcfd_solver* solver);
 pcfd_status pcfd_mg_primitive(pcfd_solver* solver, int op, int level, const double* x, const double* b,
                               double* out, double omega, int sweeps);
 
+/* Measurement hook with no counterpart in the reference (which has no kernel-level timing):
+ * average duration of one launch of a hot kernel on the solver's stream, timed with HIP
+ * events over `reps` launches, and its algorithmic HBM bytes per launch.
+ * kernel: 0 composite operator apply + dot (CG), 1 composite-level V-cycle sweep,
+ *         2 MUSCL transport, 3 one full V-cycle (bytes reported as 0).
+ * Leaves the Krylov loop flags in the "finished" state; call between steps. */
+pcfd_status pcfd_time_kernel(pcfd_solver* solver, int kernel, int reps, double* avg_ms, double* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,7 +88,7 @@ EXPORTED = [
     "pcfd_initialize", "pcfd_step", "pcfd_advect", "pcfd_project", "pcfd_apply_pressure_impulse",
     "pcfd_integrate_loads", "pcfd_set_reference", "pcfd_get_velocity", "pcfd_set_velocity", "pcfd_get_pressure",
     "pcfd_set_pressure", "pcfd_get_faces", "pcfd_set_faces", "pcfd_mark_initialized", "pcfd_device_bytes",
-    "pcfd_mg_primitive",
+    "pcfd_mg_primitive", "pcfd_time_kernel",
 ]
 
 _lib = None
@@ -131,6 +131,7 @@ def load():
         "pcfd_mark_initialized": (C.c_int, [P, C.c_double]),
         "pcfd_device_bytes": (C.c_int64, [P]),
         "pcfd_mg_primitive": (C.c_int, [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int]),
+        "pcfd_time_kernel": (C.c_int, [P, C.c_int, C.c_int, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -67,9 +67,9 @@ def uniform_box(n=16, h=0.05, outlet=False):
 # Configurations of the benchmark table (cells_x chosen as described in the module docstring).
 CASES = {
     "sphere64": dict(kind="sphere", args=dict(cells_x=128), rho=1.0, nu=1e-3, speed=1.0, cfl=0.5, steps=100),
-    "naca2412_2lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=192, levels=2, bands=(0.3,)),
+    "naca2412_2lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=208, levels=2, bands=(0.3,)),
                           rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=500),
-    "naca2412_3lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=256, levels=3, bands=(1.0, 0.25)),
+    "naca2412_3lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=288, levels=3, bands=(1.0, 0.25)),
                           rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=200),
 }
 

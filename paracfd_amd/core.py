@@ -1,0 +1,134 @@
+"""Python mirror of the reference's step interface over the C ABI.
+
+`ExternalAeroCore` follows src/core/fluid/external_aero_core.h:124-182: initialize(), step(),
+pressure_loads(pref), and the download/upload helpers.  Every call goes through
+libparacfd.so (HIP on gfx950); there is no CPU implementation behind it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def default_options(**kw) -> L.Options:
+    o = L.Options()
+    L.load().pcfd_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+class ExternalAeroCore:
+    """One solver instance on the current HIP device (reference: ExternalAeroCore)."""
+
+    def __init__(self, topology: L.Topology, options: L.Options):
+        self._lib = L.load()
+        self.topology = topology
+        self.options = options
+        h = C.c_void_p()
+        L.check(self._lib.pcfd_create(topology.handle, C.byref(options), C.byref(h)))
+        self.handle = h
+        self.n = topology.ndof
+        self.nedges = int(topology.info.compact_edges)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.pcfd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stats(self, fn, *args) -> dict:
+        st = L.StepStats()
+        L.check(fn(self.handle, *args, C.byref(st)))
+        return st.as_dict()
+
+    # reference: ExternalAeroCore::initialize (external_aero_core.cpp:245-302)
+    def initialize(self) -> dict:
+        return self._stats(self._lib.pcfd_initialize)
+
+    # reference: ExternalAeroCore::step (external_aero_core.cpp:304-328)
+    def step(self) -> dict:
+        return self._stats(self._lib.pcfd_step)
+
+    def advect(self, dt=0.0) -> float:
+        used = C.c_double()
+        st = L.StepStats()
+        L.check(self._lib.pcfd_advect(self.handle, dt, C.byref(used), C.byref(st)))
+        return used.value
+
+    def project(self, dt, warm=True) -> dict:
+        return self._stats(self._lib.pcfd_project, C.c_double(dt), C.c_int(1 if warm else 0))
+
+    def apply_pressure_impulse(self, dt):
+        L.check(self._lib.pcfd_apply_pressure_impulse(self.handle, dt))
+
+    def set_reference(self, area, origin=(0.0, 0.0, 0.0)):
+        o = (C.c_double * 3)(*origin)
+        L.check(self._lib.pcfd_set_reference(self.handle, area, o))
+
+    # reference: ExternalAeroCore::pressure_loads (external_aero_core.cpp:330-337)
+    def pressure_loads(self, pref=0.0) -> dict:
+        out = L.Loads()
+        L.check(self._lib.pcfd_integrate_loads(self.handle, pref, C.byref(out)))
+        return out.as_dict()
+
+    # reference: download_cell_momentum_state / upload_state (amr_advection.cu:1301-1309)
+    def download_cell_momentum_state(self):
+        x, y, z = (np.empty(self.n) for _ in range(3))
+        L.check(self._lib.pcfd_get_velocity(self.handle, L.dptr(x), L.dptr(y), L.dptr(z)))
+        return x, y, z
+
+    def upload_state(self, x, y, z):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        L.check(self._lib.pcfd_set_velocity(self.handle, *[L.dptr(a) for a in arrs]))
+
+    def download_pressure(self):
+        p = np.empty(self.n)
+        L.check(self._lib.pcfd_get_pressure(self.handle, L.dptr(p)))
+        return p
+
+    def upload_pressure(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        L.check(self._lib.pcfd_set_pressure(self.handle, L.dptr(p)))
+
+    def download_faces(self):
+        fx, fy, fz = (np.empty(self.n) for _ in range(3))
+        fe = np.empty(max(self.nedges, 1))
+        L.check(self._lib.pcfd_get_faces(self.handle, L.dptr(fx), L.dptr(fy), L.dptr(fz), L.dptr(fe)))
+        return fx, fy, fz, fe[: self.nedges]
+
+    def upload_faces(self, fx, fy, fz, fe):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz)]
+        fe = np.ascontiguousarray(fe if len(fe) else np.zeros(1), dtype=np.float64)
+        L.check(self._lib.pcfd_set_faces(self.handle, *[L.dptr(a) for a in arrs], L.dptr(fe)))
+
+    def mark_initialized(self, time=0.0):
+        L.check(self._lib.pcfd_mark_initialized(self.handle, time))
+
+    def device_bytes(self) -> int:
+        return int(self._lib.pcfd_device_bytes(self.handle))
+
+    def time_kernel(self, kernel, reps=20):
+        """(average ms per launch, algorithmic bytes per launch) of a hot kernel."""
+        ms, by = C.c_double(), C.c_double()
+        L.check(self._lib.pcfd_time_kernel(self.handle, kernel, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def mg_primitive(self, op, level, x=None, b=None, omega=2.0 / 3.0, sweeps=1):
+        n = self.topology.mg_n(level)
+        out = np.empty(n)
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        ba = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        L.check(self._lib.pcfd_mg_primitive(self.handle, op, level, L.dptr(xa), L.dptr(ba), L.dptr(out), omega,
+                                            sweeps))
+        return out

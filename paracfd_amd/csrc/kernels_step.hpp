@@ -1,0 +1,450 @@
+// Cell kernels of one solver step on the brick layout: CFL outflow rate, MUSCL transport,
+// explicit diffusion, face-velocity reconstruction, divergence / RHS, face correction,
+// pressure impulse and the surface-load integral.  R is the field type (double or float);
+// pressure and every reduction stay in FP64.
+//
+// All cell kernels run one 512-lane workgroup per brick and evaluate each face from both
+// sides with the same arithmetic (gather form), so transport and impulse are conservative
+// without atomics and bitwise reproducible.
+#pragma once
+
+#include "device.hpp"
+
+namespace pcfd {
+
+template <class R>
+struct FieldsView {
+    R* v[3];        // cell velocity
+    R* f[3];        // velocity on the +x/+y/+z face of each cell (and the x-max outlet face)
+    R* fe;          // compact-edge face velocity (normal component along the edge axis)
+};
+
+template <class R>
+__device__ __forceinline__ R minmod(R a, R b) {
+    if (a * b <= R(0)) return R(0);
+    return a >= R(0) ? fmin(a, b) : fmax(a, b);
+}
+
+// Upwind MUSCL-minmod face value with the Courant-dependent slope factor
+// 0.5 (1 - min(1, C)) (amr_advection.cu:62-75).
+template <class R>
+__device__ __forceinline__ R muscl_value(R swept, R pred, R va, R vb, R vl, bool has_l, R vu, bool has_u) {
+    if (swept >= R(0)) return has_l ? va + pred * minmod(va - vl, vb - va) : va;
+    return has_u ? vb - pred * minmod(vb - va, vu - vb) : vb;
+}
+
+__device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
+
+// ---------------------------------------------------------------------------------------
+// CFL: max over cells of sum(max(outflow, 0)) / V  (amr_advection.cu:169-186) -> dt
+// ---------------------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> F, double* partials, DevScalars* S,
+                                                      int set_dt) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    double rate = 0.0;
+    if (m & kActive) {
+        const double h = cell_h(M, b), A = h * h;
+        double out = 0.0;
+        for (int axis = 0; axis < 3; ++axis) {
+            if (m & (kPX << axis)) out += fmax(A * double(F.f[axis][a]), 0.0);
+            if (m & (kMX << axis)) out += fmax(-A * double(F.f[axis][neighbor_dof(M, b, q, axis, -1)]), 0.0);
+        }
+        const uint8_t bf = M.brick_flags[b];
+        if ((q & 7) == 0 && (bf & kXMIN)) out += fmax(-A * M.speed, 0.0);
+        if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) out += fmax(A * double(F.f[0][a]), 0.0);
+        double V = h * h * h;
+        if (sp >= 0) {
+            V = M.special_volume[sp];
+            for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
+                const int raw = M.special_inc[k];
+                const int e = raw >= 0 ? raw : ~raw;
+                const double flux = M.earea[e] * double(F.fe[e]);
+                out += fmax(raw >= 0 ? flux : -flux, 0.0);
+            }
+        }
+        rate = out / V;
+    }
+    const double bm = block_max<512>(rate);
+    if (publish_partial(partials, bm, &S->ticket[0], gridDim.x)) {
+        const double r = gather_partials<512>(partials, gridDim.x, true);
+        if (threadIdx.x == 0) {
+            S->rate = r;
+            if (set_dt) {
+                const double lim = fmax(fmax(1e-9, fabs(M.speed) / M.finest_h), r);
+                S->dt = M.cfl / lim;
+            }
+        }
+        reset_ticket(&S->ticket[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Conservative MUSCL transport of the cell momentum (amr_advection.cu:82-112, 113-118)
+// ---------------------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* __restrict__ ox, R* __restrict__ oy,
+                                                R* __restrict__ oz, const DevScalars* S) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    R* out[3] = {ox, oy, oz};
+    if (!(m & kActive)) {
+        for (int c = 0; c < 3; ++c) out[c][a] = F.v[c][a];
+        return;
+    }
+    const R dt = R(S->dt);
+    const R h = R(cell_h(M, b)), A = h * h;
+    R va[3], d[3] = {R(0), R(0), R(0)};
+    for (int c = 0; c < 3; ++c) va[c] = F.v[c][a];
+    for (int axis = 0; axis < 3; ++axis) {
+        if (m & (kPX << axis)) {  // face (a, b1)
+            const int64_t b1 = neighbor_dof(M, b, q, axis, 1);
+            const uint8_t mb = M.mask[b1];
+            const bool has_l = (m & (kMX << axis)) != 0, has_u = (mb & (kPX << axis)) != 0;
+            const int64_t l = has_l ? neighbor_dof(M, b, q, axis, -1) : a;
+            const int64_t u = has_u ? neighbor_dof(M, b1, axis, 1) : b1;
+            const R un = F.f[axis][a];
+            const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
+            for (int c = 0; c < 3; ++c) {
+                const R fv = muscl_value(swept, pred, va[c], F.v[c][b1], F.v[c][l], has_l, F.v[c][u], has_u);
+                d[c] -= swept * fv;
+            }
+        }
+        if (m & (kMX << axis)) {  // face (b0, a)
+            const int64_t b0 = neighbor_dof(M, b, q, axis, -1);
+            const uint8_t m0 = M.mask[b0];
+            const bool has_l = (m0 & (kMX << axis)) != 0, has_u = (m & (kPX << axis)) != 0;
+            const int64_t l = has_l ? neighbor_dof(M, b0, axis, -1) : b0;
+            const int64_t u = has_u ? neighbor_dof(M, b, q, axis, 1) : a;
+            const R un = F.f[axis][b0];
+            const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
+            for (int c = 0; c < 3; ++c) {
+                const R fv = muscl_value(swept, pred, F.v[c][b0], va[c], F.v[c][l], has_l, F.v[c][u], has_u);
+                d[c] += swept * fv;
+            }
+        }
+    }
+    const uint8_t bf = M.brick_flags[b];
+    if ((q & 7) == 0 && (bf & kXMIN)) {
+        const R U = R(M.speed), swept = dt * A * U;
+        if (swept >= R(0)) d[0] += swept * U;
+        else for (int c = 0; c < 3; ++c) d[c] += swept * va[c];
+    }
+    if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) {
+        const R swept = dt * A * F.f[0][a];
+        if (swept >= R(0)) for (int c = 0; c < 3; ++c) d[c] -= swept * va[c];
+        else d[0] -= swept * R(M.speed);
+    }
+    R V = h * h * h;
+    if (sp >= 0) {
+        V = R(M.special_volume[sp]);
+        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
+            const int raw = M.special_inc[k];
+            const int e = raw >= 0 ? raw : ~raw;
+            const R swept = dt * R(M.earea[e]) * F.fe[e];
+            const int64_t donor = swept >= R(0) ? M.ea[e] : M.eb[e];
+            for (int c = 0; c < 3; ++c) {
+                const R flux = swept * F.v[c][donor];
+                d[c] += raw >= 0 ? -flux : flux;
+            }
+        }
+    }
+    const R iv = R(1) / V;
+    for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
+}
+
+// ---------------------------------------------------------------------------------------
+// Explicit diffusion substep (amr_advection.cu:233-241): conductance h on structured faces,
+// open area / distance on compact edges.
+// ---------------------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
+                                                 const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
+                                                 R* __restrict__ oz, double dts) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const R* in[3] = {vx, vy, vz};
+    R* out[3] = {ox, oy, oz};
+    if (!(m & kActive)) {
+        for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
+        return;
+    }
+    const R h = R(cell_h(M, b));
+    const R k = R(dts * M.nu) * h;
+    R va[3], d[3] = {R(0), R(0), R(0)};
+    for (int c = 0; c < 3; ++c) va[c] = in[c][a];
+    for (int axis = 0; axis < 3; ++axis)
+        for (int s = -1; s <= 1; s += 2) {
+            if (!(m & ((s > 0 ? kPX : kMX) << axis))) continue;
+            const int64_t o = neighbor_dof(M, b, q, axis, s);
+            for (int c = 0; c < 3; ++c) d[c] += k * (in[c][o] - va[c]);
+        }
+    R V = h * h * h;
+    if (sp >= 0) {
+        V = R(M.special_volume[sp]);
+        for (int j = M.special_inc_off[sp]; j < M.special_inc_off[sp + 1]; ++j) {
+            const int raw = M.special_inc[j];
+            const int e = raw >= 0 ? raw : ~raw;
+            const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
+            const R ke = R(dts * M.nu * M.ecoef[e]);
+            for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
+        }
+    }
+    const R iv = R(1) / V;
+    for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
+}
+
+// ---------------------------------------------------------------------------------------
+// Face reconstruction (amr_advection.cu:257-293): structured face = mean of the two cells'
+// normal components; x-max outlet face = interior cell (zero gradient); compact edges use
+// the distance weight.
+// ---------------------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(512) void k_reconstruct(MeshView M, FieldsView<R> F) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    if (!(m & kActive)) return;
+    for (int axis = 0; axis < 3; ++axis) {
+        if (!(m & (kPX << axis))) continue;
+        const int64_t o = neighbor_dof(M, b, q, axis, 1);
+        F.f[axis][a] = R(0.5) * (F.v[axis][a] + F.v[axis][o]);
+    }
+    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) F.f[0][a] = F.v[0][a];
+}
+
+template <class R>
+__global__ void k_reconstruct_edges(MeshView M, FieldsView<R> F) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M.nedges) return;
+    const int ax = M.eaxis[e];
+    const R w = R(M.ew[e]);
+    F.fe[e] = (R(1) - w) * F.v[ax][M.ea[e]] + w * F.v[ax][M.eb[e]];
+}
+
+// integrated divergence of the face fluxes around cell a (FP64 accumulation)
+template <class R>
+__device__ __forceinline__ double cell_divergence(const MeshView& M, const FieldsView<R>& F, int b, int q, uint8_t m,
+                                                  int sp, double h) {
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const double A = h * h;
+    double div = 0.0;
+    for (int axis = 0; axis < 3; ++axis) {
+        if (m & (kPX << axis)) div += A * double(F.f[axis][a]);
+        if (m & (kMX << axis)) div -= A * double(F.f[axis][neighbor_dof(M, b, q, axis, -1)]);
+    }
+    const uint8_t bf = M.brick_flags[b];
+    if ((q & 7) == 0 && (bf & kXMIN)) div -= A * M.speed;
+    if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) div += A * double(F.f[0][a]);
+    if (sp >= 0)
+        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
+            const int raw = M.special_inc[k];
+            const int e = raw >= 0 ? raw : ~raw;
+            const double flux = M.earea[e] * double(F.fe[e]);
+            div += raw >= 0 ? flux : -flux;
+        }
+    return div;
+}
+
+// RHS = -(rho/dt) * integrated divergence (amr_pressure.cu:255-284); ||b|| reduction
+template <class R>
+__global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double* __restrict__ rhs, double* partials,
+                                             DevScalars* S) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    double val = 0.0;
+    if (m & kActive) val = -(M.rho / S->dt) * cell_divergence(M, F, b, q, m, sp, cell_h(M, b));
+    rhs[a] = val;
+    const double bs = block_sum<512>(val * val);
+    if (publish_partial(partials, bs, &S->ticket[1], gridDim.x)) {
+        const double t = gather_partials<512>(partials, gridDim.x, false);
+        if (threadIdx.x == 0) {
+            S->bnorm = sqrt(t);
+            S->iter = 0; S->done = 0; S->converged = 0; S->breakdown = 0; S->zero_rhs = 0;
+            if (!(t > 0.0)) { S->done = 1; S->converged = 1; S->zero_rhs = 1; S->relres = 0.0; }
+        }
+        reset_ticket(&S->ticket[1]);
+    }
+}
+
+// max |integrated divergence| / V after the correction (FP64 check)
+template <class R>
+__global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, double* partials, DevScalars* S) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    double val = 0.0;
+    if (m & kActive) {
+        const double h = cell_h(M, b);
+        const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
+        val = fabs(cell_divergence(M, F, b, q, m, sp, h) / V);
+    }
+    const double bm = block_max<512>(val);
+    if (publish_partial(partials, bm, &S->ticket[2], gridDim.x)) {
+        const double t = gather_partials<512>(partials, gridDim.x, true);
+        if (threadIdx.x == 0) S->maxdiv = t;
+        reset_ticket(&S->ticket[2]);
+    }
+}
+
+// face correction u -= (dt/rho) dp/dn (amr_pressure.cu:285-310); outlet p = 0 half a cell away
+template <class R>
+__global__ __launch_bounds__(512) void k_correct(MeshView M, FieldsView<R> F, const double* __restrict__ p,
+                                                 const DevScalars* S) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    if (!(m & kActive)) return;
+    const double s = S->dt / M.rho, h = cell_h(M, b);
+    const double pa = p[a];
+    for (int axis = 0; axis < 3; ++axis) {
+        if (!(m & (kPX << axis))) continue;
+        const int64_t o = neighbor_dof(M, b, q, axis, 1);
+        F.f[axis][a] = R(double(F.f[axis][a]) - s * (p[o] - pa) / h);
+    }
+    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) F.f[0][a] = R(double(F.f[0][a]) + s * pa / (0.5 * h));
+}
+
+template <class R>
+__global__ void k_correct_edges(MeshView M, FieldsView<R> F, const double* __restrict__ p, const DevScalars* S) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M.nedges) return;
+    const double s = S->dt / M.rho;
+    F.fe[e] = R(double(F.fe[e]) - s * (p[M.eb[e]] - p[M.ea[e]]) * (M.ecoef[e] / M.earea[e]));
+}
+
+// wall pressure at a patch centroid from the least-squares stencil
+__device__ __forceinline__ double wall_pressure(const MeshView& M, const double* p, int pch) {
+    const int64_t a = M.patch_dof[pch];
+    double pw = p[a];
+    for (int k = M.lsq_off[pch]; k < M.lsq_off[pch + 1]; ++k) pw += M.lsq_beta[k] * (p[M.lsq_dof[k]] - p[a]);
+    return pw;
+}
+
+// ---------------------------------------------------------------------------------------
+// Pressure impulse on the control volumes (amr_advection.cu:380-405): internal faces carry
+// the face pressure, walls the cell pressure, the outlet p = 0, embedded-boundary patches
+// the reconstructed wall pressure.  u += dm / V.
+// ---------------------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
+                                                 double dt) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    if (!(m & kActive)) return;
+    const double s = dt / M.rho, h = cell_h(M, b), A = h * h;
+    const double pa = p[a];
+    double d[3] = {0.0, 0.0, 0.0};
+    for (int axis = 0; axis < 3; ++axis) {
+        if (m & (kPX << axis)) d[axis] -= s * A * 0.5 * (pa + p[neighbor_dof(M, b, q, axis, 1)]);
+        if (m & (kMX << axis)) d[axis] += s * A * 0.5 * (p[neighbor_dof(M, b, q, axis, -1)] + pa);
+    }
+    const uint8_t bf = M.brick_flags[b];
+    const int i = q & 7, j = (q >> 3) & 7, k = q >> 6;
+    const double bnd = s * A * pa;
+    if (i == 0 && (bf & kXMIN)) d[0] += bnd;
+    if (i == 7 && (bf & kXMAX) && !M.outlet) d[0] -= bnd;
+    if (j == 0 && (bf & kYMIN)) d[1] += bnd;
+    if (j == 7 && (bf & kYMAX)) d[1] -= bnd;
+    if (k == 0 && (bf & kZMIN)) d[2] += bnd;
+    if (k == 7 && (bf & kZMAX)) d[2] -= bnd;
+    double V = h * h * h;
+    if (sp >= 0) {
+        V = M.special_volume[sp];
+        for (int t = M.special_inc_off[sp]; t < M.special_inc_off[sp + 1]; ++t) {
+            const int raw = M.special_inc[t];
+            const int e = raw >= 0 ? raw : ~raw;
+            const double w = M.ew[e];
+            const double imp = s * M.earea[e] * ((1.0 - w) * p[M.ea[e]] + w * p[M.eb[e]]);
+            d[M.eaxis[e]] += raw >= 0 ? -imp : imp;
+        }
+        const int pch = M.special_patch[sp];
+        if (pch >= 0) {
+            const double pw = wall_pressure(M, p, pch);
+            for (int c = 0; c < 3; ++c) d[c] -= s * pw * M.patch_S[3 * pch + c];
+        }
+    }
+    const double iv = 1.0 / V;
+    for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(F.v[c][a]) + d[c] * iv);
+}
+
+// ---------------------------------------------------------------------------------------
+// Surface-load integral (aero_loads.cpp:35-47) on the device in FP64: F = sum (p_w - pref) S,
+// M = sum (x - origin) x F; deterministic block partials.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p, double pref, double ox,
+                                               double oy, double oz, double* partials, DevScalars* S) {
+    const int pch = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (pch < M.npatches) {
+        const double pw = wall_pressure(M, p, pch) - pref;
+        const double Fx = pw * M.patch_S[3 * pch], Fy = pw * M.patch_S[3 * pch + 1], Fz = pw * M.patch_S[3 * pch + 2];
+        const double rx = M.patch_x[3 * pch] - ox, ry = M.patch_x[3 * pch + 1] - oy, rz = M.patch_x[3 * pch + 2] - oz;
+        v[0] = Fx; v[1] = Fy; v[2] = Fz;
+        v[3] = ry * Fz - rz * Fy; v[4] = rz * Fx - rx * Fz; v[5] = rx * Fy - ry * Fx;
+    }
+    double part[6];
+    for (int c = 0; c < 6; ++c) part[c] = block_sum<256>(v[c]);
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c)
+            __hip_atomic_store(partials + 6 * blockIdx.x + c, part[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned t = __hip_atomic_fetch_add(&S->ticket[3], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (int c = 0; c < 6; ++c) {
+        double acc = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += 256)
+            acc += __hip_atomic_load(partials + 6 * i + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        acc = block_sum<256>(acc);
+        if (threadIdx.x == 0) S->loads[c] = acc;
+    }
+    reset_ticket(&S->ticket[3]);
+}
+
+// max |v| over active cells
+template <class R>
+__global__ __launch_bounds__(512) void k_vmax(MeshView M, FieldsView<R> F, double* partials, DevScalars* S) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * 512 + threadIdx.x;
+    double v = 0.0;
+    if (M.mask[a] & kActive) {
+        const double x = fabs(double(F.v[0][a])), y = fabs(double(F.v[1][a])), z = fabs(double(F.v[2][a]));
+        v = fmax(x, fmax(y, z));
+        if (!(v == v) || isinf(v)) v = INFINITY;  // non-finite wins the max
+    }
+    const double bm = block_max<512>(v);
+    if (publish_partial(partials, bm, &S->ticket[4], gridDim.x)) {
+        const double t = gather_partials<512>(partials, gridDim.x, true);
+        if (threadIdx.x == 0) S->vmax = t;
+        reset_ticket(&S->ticket[4]);
+    }
+}
+
+template <class R>
+__global__ void k_init_velocity(MeshView M, FieldsView<R> F) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (a >= M.ndof) return;
+    const bool act = (M.mask[a] & kActive) != 0;
+    F.v[0][a] = act ? R(M.speed) : R(0);
+    F.v[1][a] = R(0);
+    F.v[2][a] = R(0);
+    F.f[0][a] = R(0); F.f[1][a] = R(0); F.f[2][a] = R(0);
+}
+
+}  // namespace pcfd

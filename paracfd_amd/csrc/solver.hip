@@ -1,0 +1,840 @@
+// Device solver for one ParaCFD step on gfx950 and the C ABI around it.
+//
+// Host orchestration mirrors ExternalAeroCore::initialize / step
+// (src/core/fluid/external_aero_core.cpp:245-328) with the cell-momentum branch only:
+// CFL dt -> MUSCL transport -> diffusion -> face reconstruction -> projection (RHS, PCG
+// with the geometric V-cycle, face correction, divergence check) -> pressure impulse.
+// All work is enqueued on one HIP stream owned by the handle; the host reads the device
+// scalars once per `check_interval` Krylov iterations and at the end of each stage.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/paracfd.h"
+#include "device.hpp"
+#include "kernels_solve.hpp"
+#include "kernels_step.hpp"
+#include "topology.hpp"
+
+const pcfd::Topology& pcfd_topology_ref(const pcfd_topology* t);
+
+namespace pcfd {
+
+void set_error(const std::string& m);
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                                   \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+#define LAUNCH(kernel, grid, block, ...)                                                                  \
+    do {                                                                                                  \
+        hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, stream_, __VA_ARGS__);                   \
+        HIP_CHECK(hipGetLastError());                                                                     \
+    } while (0)
+
+static inline unsigned blocks_for(int64_t n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
+
+class Solver {
+public:
+    virtual ~Solver() = default;
+    virtual void initialize(pcfd_step_stats* st) = 0;
+    virtual void step(pcfd_step_stats* st) = 0;
+    virtual void advect(double dt, double* dt_used, pcfd_step_stats* st) = 0;
+    virtual void project(double dt, bool warm, pcfd_step_stats* st) = 0;
+    virtual void impulse(double dt) = 0;
+    virtual void loads(double pref, pcfd_loads* out) = 0;
+    virtual void get_velocity(double* x, double* y, double* z) = 0;
+    virtual void set_velocity(const double* x, const double* y, const double* z) = 0;
+    virtual void get_pressure(double* p) = 0;
+    virtual void set_pressure(const double* p) = 0;
+    virtual void get_faces(double* fx, double* fy, double* fz, double* fe) = 0;
+    virtual void set_faces(const double* fx, const double* fy, const double* fz, const double* fe) = 0;
+    virtual void mg_primitive(int op, int level, const double* x, const double* b, double* out, double omega, int sweeps) = 0;
+    virtual void time_kernel(int kernel, int reps, double* avg_ms, double* bytes) = 0;
+    bool initialized = false;
+    double time = 0.0;
+    double ref_area = 0.0, origin[3] = {0, 0, 0};
+    int64_t device_bytes = 0;
+};
+
+template <class R>
+class SolverImpl final : public Solver {
+public:
+    SolverImpl(const Topology& T, const pcfd_options& o);
+    ~SolverImpl() override;
+    void initialize(pcfd_step_stats* st) override;
+    void step(pcfd_step_stats* st) override;
+    void advect(double dt, double* dt_used, pcfd_step_stats* st) override;
+    void project(double dt, bool warm, pcfd_step_stats* st) override;
+    void impulse(double dt) override;
+    void loads(double pref, pcfd_loads* out) override;
+    void get_velocity(double* x, double* y, double* z) override;
+    void set_velocity(const double* x, const double* y, const double* z) override;
+    void get_pressure(double* p) override;
+    void set_pressure(const double* p) override;
+    void get_faces(double* fx, double* fy, double* fz, double* fe) override;
+    void set_faces(const double* fx, const double* fy, const double* fz, const double* fe) override;
+    void mg_primitive(int op, int level, const double* x, const double* b, double* out, double omega, int sweeps) override;
+    void time_kernel(int kernel, int reps, double* avg_ms, double* bytes) override;
+
+private:
+    template <class T>
+    T* alloc(int64_t count) {
+        if (count <= 0) count = 1;
+        void* p = nullptr;
+        HIP_CHECK(hipMalloc(&p, static_cast<size_t>(count) * sizeof(T)));
+        allocations_.push_back(p);
+        device_bytes += count * static_cast<int64_t>(sizeof(T));
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(static_cast<int64_t>(v.size()));
+        if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    template <class T>
+    T* upload_as(const std::vector<double>& v) {
+        std::vector<T> c(v.begin(), v.end());
+        return upload(c);
+    }
+    void sync_scalars() {
+        HIP_CHECK(hipMemcpyAsync(host_s_, S_, sizeof(DevScalars), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+    void put_dt(double dt) {
+        host_s_->dt = dt;
+        HIP_CHECK(hipMemcpyAsync(&S_->dt, &host_s_->dt, sizeof(double), hipMemcpyHostToDevice, stream_));
+    }
+    FieldsView<R> fields() const {
+        FieldsView<R> F;
+        for (int c = 0; c < 3; ++c) { F.v[c] = v_[c]; F.f[c] = f_[c]; }
+        F.fe = fe_;
+        return F;
+    }
+    void reconstruct();
+    void run_projection(bool warm, pcfd_step_stats* st);
+    template <class VT> void precondition(bool initial);
+    template <class VT> VT* coarse_cycle(int m);
+    template <class VT, class CT> VT* coarse_cycle_t(int m);
+    void precondition_dispatch(bool initial);
+
+    pcfd_options opt_;
+    MeshView M_{};
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocations_;
+    int64_t n_ = 0;
+    int nb_ = 0;
+    int nedges_ = 0;
+    double max_diff_coef_ = 0;
+    // state
+    R* v_[3] = {};
+    R* vt_[3] = {};
+    R* f_[3] = {};
+    R* fe_ = nullptr;
+    double* p_ = nullptr;
+    double* rhs_ = nullptr;
+    double* r_ = nullptr;
+    double* z_ = nullptr;
+    double* d_ = nullptr;
+    double* Ad_ = nullptr;
+    double* rprev_ = nullptr;
+    void* vc0_[2] = {};  // composite-level V-cycle buffers (8 bytes per dof)
+    double* partials_ = nullptr;
+    DevScalars* S_ = nullptr;
+    DevScalars* host_s_ = nullptr;
+    // multigrid levels >= 1
+    struct Level {
+        int n = 0;
+        CsrView csr64{}, csr32{};
+        const int* parent = nullptr;  // parent of this level's cells
+        void* B = nullptr; void* X = nullptr; void* T = nullptr;
+    };
+    std::vector<Level> lv_;
+    const int* parent0_ = nullptr;
+    double* inv64_ = nullptr;
+    float* inv32_ = nullptr;
+    int coarse_n_ = 0;
+    hipEvent_t ev_[4] = {};
+};
+
+template <class R>
+SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
+    if (opt_.pre_smooth < 1 || opt_.post_smooth < 1) throw std::invalid_argument("V-cycle needs at least one pre- and post-smoothing sweep");
+    if (opt_.check_interval < 1) opt_.check_interval = 1;
+    if (!(opt_.rho > 0) || !(opt_.cfl > 0) || !(opt_.rtol > 0) || opt_.max_iterations < 1 || opt_.nu < 0)
+        throw std::invalid_argument("invalid solver options");
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_s_), sizeof(DevScalars), hipHostMallocDefault));
+    std::memset(host_s_, 0, sizeof(DevScalars));
+    n_ = T.ndof;
+    nb_ = T.nbricks;
+    nedges_ = static_cast<int>(T.edge_a.size());
+    max_diff_coef_ = T.max_diffusion_coefficient;
+    // topology
+    M_.nbricks = nb_;
+    M_.ndof = n_;
+    M_.mask = upload(T.mask);
+    M_.brick_level = upload(T.brick_level);
+    M_.brick_flags = upload(T.brick_flags);
+    M_.brick_nbr = upload(T.brick_nbr);
+    for (int l = 0; l < 4; ++l) M_.h[l] = l < static_cast<int>(T.level_h.size()) ? T.level_h[l] : 0.0;
+    M_.outlet = T.outlet ? 1 : 0;
+    M_.brick_special_off = upload(T.brick_special_off);
+    M_.special_volume = upload(T.special_volume);
+    M_.special_inc_off = upload(T.special_inc_off);
+    M_.special_inc = upload(T.special_inc);
+    M_.special_patch = upload(T.special_patch);
+    M_.nedges = nedges_;
+    M_.ea = upload(T.edge_a);
+    M_.eb = upload(T.edge_b);
+    M_.eaxis = upload(T.edge_axis);
+    M_.earea = upload(T.edge_area);
+    M_.ecoef = upload(T.edge_coef);
+    M_.ew = upload(T.edge_w);
+    M_.npatches = static_cast<int>(T.patch_dof.size());
+    M_.patch_dof = upload(T.patch_dof);
+    M_.patch_S = upload(T.patch_S);
+    M_.patch_x = upload(T.patch_x);
+    M_.lsq_off = upload(T.patch_lsq_off);
+    M_.lsq_dof = upload(T.patch_lsq_dof);
+    M_.lsq_beta = upload(T.patch_lsq_beta);
+    M_.speed = opt_.speed;
+    M_.rho = opt_.rho;
+    M_.cfl = opt_.cfl;
+    M_.nu = opt_.nu;
+    M_.rtol = opt_.rtol;
+    M_.finest_h = T.finest_h;
+    M_.max_iterations = opt_.max_iterations;
+    // fields
+    for (int c = 0; c < 3; ++c) {
+        v_[c] = alloc<R>(n_); vt_[c] = alloc<R>(n_); f_[c] = alloc<R>(n_);
+        HIP_CHECK(hipMemset(v_[c], 0, n_ * sizeof(R)));
+        HIP_CHECK(hipMemset(f_[c], 0, n_ * sizeof(R)));
+    }
+    fe_ = alloc<R>(nedges_);
+    HIP_CHECK(hipMemset(fe_, 0, std::max(1, nedges_) * sizeof(R)));
+    p_ = alloc<double>(n_); rhs_ = alloc<double>(n_); r_ = alloc<double>(n_); z_ = alloc<double>(n_);
+    d_ = alloc<double>(n_); Ad_ = alloc<double>(n_);
+    for (double* q : {p_, rhs_, r_, z_, d_, Ad_}) HIP_CHECK(hipMemset(q, 0, n_ * sizeof(double)));
+    if (opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED) {
+        rprev_ = alloc<double>(n_);
+        HIP_CHECK(hipMemset(rprev_, 0, n_ * sizeof(double)));
+    }
+    vc0_[0] = alloc<double>(n_);
+    vc0_[1] = alloc<double>(n_);
+    const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 6 * (M_.npatches / 256 + 1)});
+    partials_ = alloc<double>(npart);
+    S_ = alloc<DevScalars>(1);
+    HIP_CHECK(hipMemset(S_, 0, sizeof(DevScalars)));
+    // multigrid
+    if (T.mg.size() > 1) {
+        parent0_ = upload(T.mg[0].parent);
+        lv_.resize(T.mg.size());
+        for (size_t m = 1; m < T.mg.size(); ++m) {
+            const MGLevel& G = T.mg[m];
+            Level& L = lv_[m];
+            L.n = G.n;
+            CsrView c{};
+            c.n = G.n;
+            c.row_off = upload(G.row_off);
+            c.col = upload(G.col);
+            c.parent = m + 1 < T.mg.size() ? upload(G.parent) : nullptr;
+            c.child_off = upload(G.child_off);
+            c.child = upload(G.child);
+            L.csr64 = c; L.csr32 = c;
+            L.csr64.coef = upload(G.coef);
+            L.csr64.diag = upload(G.diag);
+            L.csr32.coef = upload_as<float>(G.coef);
+            L.csr32.diag = upload_as<float>(G.diag);
+            L.parent = c.parent;
+            L.B = alloc<double>(G.n); L.X = alloc<double>(G.n); L.T = alloc<double>(G.n);
+        }
+        const MGLevel& K = T.mg.back();
+        coarse_n_ = K.n;
+        if (!K.coarse_inv.empty()) {
+            inv64_ = upload(K.coarse_inv);
+            inv32_ = upload_as<float>(K.coarse_inv);
+        }
+    }
+    HIP_CHECK(hipDeviceSynchronize());
+}
+
+template <class R>
+SolverImpl<R>::~SolverImpl() {
+    // teardown errors cannot be reported from a destructor; they are ignored deliberately
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocations_) (void)hipFree(p);
+    for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    if (host_s_) (void)hipHostFree(host_s_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <class R>
+void SolverImpl<R>::reconstruct() {
+    LAUNCH(k_reconstruct<R>, nb_, 512, M_, fields());
+    if (nedges_) LAUNCH(k_reconstruct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields());
+}
+
+// ------------------------------------------------------------------------------------------
+// V-cycle
+// ------------------------------------------------------------------------------------------
+template <class R>
+template <class VT, class CT>
+VT* SolverImpl<R>::coarse_cycle_t(int m) {
+    Level& L = lv_[m];
+    const CsrView& C = std::is_same<CT, double>::value ? L.csr64 : L.csr32;
+    VT* B = static_cast<VT*>(L.B);
+    VT* cur = static_cast<VT*>(L.X);
+    VT* oth = static_cast<VT*>(L.T);
+    const double w = 2.0 / 3.0;
+    const unsigned g = blocks_for(L.n, 256);
+    if (m == static_cast<int>(lv_.size()) - 1) {
+        const CT* inv = nullptr;
+        if constexpr (std::is_same<CT, double>::value) inv = inv64_; else inv = inv32_;
+        if (inv) {
+            LAUNCH((k_dense_solve<VT, CT>), L.n, 256, L.n, inv, B, cur, S_);
+        } else {
+            LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 0, cur, cur, B, w, S_);
+            for (int s = 1; s < 64; ++s) { LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 1, cur, oth, B, w, S_); std::swap(cur, oth); }
+        }
+        return cur;
+    }
+    LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 0, cur, cur, B, w, S_);
+    for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 1, cur, oth, B, w, S_); std::swap(cur, oth); }
+    LAUNCH((k_residual_csr<VT, CT>), g, 256, C, cur, B, oth, S_);
+    Level& N = lv_[m + 1];
+    const CsrView& NC = std::is_same<CT, double>::value ? N.csr64 : N.csr32;
+    LAUNCH((k_restrict<VT>), blocks_for(N.n, 256), 256, NC, oth, static_cast<VT*>(N.B), S_);
+    VT* xc = coarse_cycle_t<VT, CT>(m + 1);
+    LAUNCH((k_prolong<VT>), g, 256, static_cast<int64_t>(L.n), L.parent, cur, xc, S_);
+    for (int s = 0; s < opt_.post_smooth; ++s) { LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 1, cur, oth, B, w, S_); std::swap(cur, oth); }
+    return cur;
+}
+
+template <class R>
+template <class VT>
+VT* SolverImpl<R>::coarse_cycle(int m) {
+    if constexpr (std::is_same<VT, double>::value) return coarse_cycle_t<VT, double>(m);
+    else return coarse_cycle_t<VT, float>(m);
+}
+
+template <class R>
+template <class VT>
+void SolverImpl<R>::precondition(bool initial) {
+    const int ini = initial ? 1 : 0;
+    if (opt_.preconditioner == PCFD_PC_JACOBI) {
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 2, static_cast<const VT*>(nullptr), static_cast<VT*>(nullptr), r_, z_, rprev_, 1.0,
+               partials_, S_, ini);
+        return;
+    }
+    const double w = 2.0 / 3.0;
+    VT* cur = static_cast<VT*>(vc0_[0]);
+    VT* oth = static_cast<VT*>(vc0_[1]);
+    LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w, partials_, S_, ini);
+    for (int s = 1; s < opt_.pre_smooth; ++s) {
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w, partials_, S_, ini);
+        std::swap(cur, oth);
+    }
+    if (lv_.size() > 1) {
+        LAUNCH(k_residual0<VT>, nb_, 512, M_, cur, r_, oth, S_);
+        const CsrView& NC = std::is_same<VT, double>::value ? lv_[1].csr64 : lv_[1].csr32;
+        LAUNCH(k_restrict<VT>, blocks_for(lv_[1].n, 256), 256, NC, oth, static_cast<VT*>(lv_[1].B), S_);
+        VT* xc = coarse_cycle<VT>(1);
+        LAUNCH(k_prolong<VT>, blocks_for(n_, 256), 256, n_, parent0_, cur, xc, S_);
+    }
+    for (int s = 0; s < opt_.post_smooth; ++s) {
+        const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_, ini);
+        std::swap(cur, oth);
+    }
+}
+
+template <class R>
+void SolverImpl<R>::precondition_dispatch(bool initial) {
+    switch (opt_.vcycle_precision) {
+        case PCFD_FP32: precondition<float>(initial); break;
+        case PCFD_BF16: precondition<bf16>(initial); break;
+        default: precondition<double>(initial); break;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// projection: RHS, PCG, correction, divergence check  (amr_pressure.cu:2085-2095)
+// ------------------------------------------------------------------------------------------
+template <class R>
+void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
+    LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+    if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+    precondition_dispatch(true);
+    const unsigned gu = std::min<unsigned>(blocks_for(n_, 256), 2048);
+    LAUNCH(k_cg_direction, gu, 256, n_, z_, d_, S_, 1);
+    int enqueued = 0;
+    for (;;) {
+        for (int k = 0; k < opt_.check_interval; ++k) {
+            LAUNCH(k_cg_apply_dot, nb_, 512, M_, d_, Ad_, partials_, S_);
+            LAUNCH(k_cg_update, gu, 256, n_, p_, r_, rprev_, d_, Ad_, partials_, S_, opt_.max_iterations, opt_.rtol);
+            precondition_dispatch(false);
+            LAUNCH(k_cg_direction, gu, 256, n_, z_, d_, S_, 0);
+        }
+        enqueued += opt_.check_interval;
+        sync_scalars();
+        if (host_s_->done || enqueued > opt_.max_iterations + opt_.check_interval) break;
+    }
+    if (host_s_->zero_rhs) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
+    if (nedges_) LAUNCH(k_correct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), p_, S_);
+    LAUNCH(k_divcheck<R>, nb_, 512, M_, fields(), partials_, S_);
+    sync_scalars();
+    if (st) {
+        st->iterations = host_s_->iter;
+        st->converged = host_s_->converged;
+        st->relative_residual = host_s_->relres;
+        st->max_abs_divergence = host_s_->maxdiv;
+    }
+}
+
+template <class R>
+void SolverImpl<R>::project(double dt, bool warm, pcfd_step_stats* st) {
+    if (!(dt > 0) || !std::isfinite(dt)) throw std::invalid_argument("projection timestep must be positive");
+    if (st) std::memset(st, 0, sizeof(*st));
+    put_dt(dt);
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
+    run_projection(warm, st);
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
+    HIP_CHECK(hipEventSynchronize(ev_[1]));
+    if (st) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        st->project_ms = ms;
+        st->dt = dt;
+        st->physical_time = time;
+    }
+}
+
+template <class R>
+void SolverImpl<R>::impulse(double dt) {
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, dt);
+}
+
+template <class R>
+void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
+    if (dt > 0) put_dt(dt);
+    else LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_, S_, 1);
+    sync_scalars();
+    dt = host_s_->dt;
+    if (dt_used) *dt_used = dt;
+    LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_);
+    for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
+    int subs = 0;
+    if (opt_.nu > 0) {
+        subs = std::max(1, static_cast<int>(std::ceil(dt * opt_.nu * max_diff_coef_ / 0.7)));
+        const double dts = dt / subs;
+        for (int s = 0; s < subs; ++s) {
+            LAUNCH(k_diffuse<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], dts);
+            for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
+        }
+    }
+    reconstruct();
+    if (st) st->diffusion_substeps = subs;
+}
+
+template <class R>
+void SolverImpl<R>::initialize(pcfd_step_stats* st) {
+    LAUNCH(k_init_velocity<R>, blocks_for(n_, 256), 256, M_, fields());
+    reconstruct();
+    const double dt = opt_.cfl * M_.finest_h / std::max(1e-9, std::abs(opt_.speed));
+    HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    pcfd_step_stats local{};
+    project(dt, false, &local);
+    if (local.converged) impulse(dt);
+    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_, S_);
+    sync_scalars();
+    local.max_abs_velocity = host_s_->vmax;
+    initialized = local.converged != 0;
+    time = 0.0;
+    local.physical_time = time;
+    if (st) *st = local;
+}
+
+template <class R>
+void SolverImpl<R>::step(pcfd_step_stats* st) {
+    if (!initialized) throw std::logic_error("step before a converged initialization");
+    pcfd_step_stats local{};
+    HIP_CHECK(hipEventRecord(ev_[2], stream_));
+    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_, S_);
+    LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_, S_, 1);
+    sync_scalars();
+    if (!std::isfinite(host_s_->vmax)) throw std::domain_error("non-finite velocity before timestep");
+    const double dt = host_s_->dt;
+    local.cfl_rate = host_s_->rate;
+    local.max_abs_velocity = host_s_->vmax;
+    advect(dt, nullptr, &local);
+    HIP_CHECK(hipEventRecord(ev_[3], stream_));
+    pcfd_step_stats ps{};
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
+    run_projection(true, &ps);
+    if (ps.converged) { impulse(dt); time += dt; }
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
+    HIP_CHECK(hipEventSynchronize(ev_[1]));
+    float a_ms = 0, p_ms = 0, s_ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&a_ms, ev_[2], ev_[3]));
+    HIP_CHECK(hipEventElapsedTime(&p_ms, ev_[0], ev_[1]));
+    HIP_CHECK(hipEventElapsedTime(&s_ms, ev_[2], ev_[1]));
+    local.dt = dt;
+    local.iterations = ps.iterations;
+    local.converged = ps.converged;
+    local.relative_residual = ps.relative_residual;
+    local.max_abs_divergence = ps.max_abs_divergence;
+    local.advect_ms = a_ms;
+    local.project_ms = p_ms;
+    local.step_ms = s_ms;
+    local.physical_time = time;
+    if (st) *st = local;
+}
+
+template <class R>
+void SolverImpl<R>::loads(double pref, pcfd_loads* out) {
+    std::memset(out, 0, sizeof(*out));
+    if (M_.npatches > 0) {
+        LAUNCH(k_loads, blocks_for(M_.npatches, 256), 256, M_, p_, pref, origin[0], origin[1], origin[2], partials_, S_);
+        sync_scalars();
+        for (int c = 0; c < 3; ++c) { out->force[c] = host_s_->loads[c]; out->moment[c] = host_s_->loads[3 + c]; }
+    }
+    out->drag = out->force[0]; out->side = out->force[1]; out->lift = out->force[2];
+    const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed;
+    if (q > 0 && ref_area > 0) {
+        out->coefficients_valid = 1;
+        out->cd = out->drag / (q * ref_area); out->cs = out->side / (q * ref_area); out->cl = out->lift / (q * ref_area);
+    }
+}
+
+template <class R>
+void SolverImpl<R>::get_velocity(double* x, double* y, double* z) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::vector<R> tmp(n_);
+    double* out[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c) {
+        if (!out[c]) continue;
+        HIP_CHECK(hipMemcpy(tmp.data(), v_[c], n_ * sizeof(R), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n_; ++i) out[c][i] = double(tmp[i]);
+    }
+}
+
+template <class R>
+void SolverImpl<R>::set_velocity(const double* x, const double* y, const double* z) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::vector<R> tmp(n_);
+    const double* in[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c) {
+        if (!in[c]) continue;
+        for (int64_t i = 0; i < n_; ++i) tmp[i] = static_cast<R>(in[c][i]);
+        HIP_CHECK(hipMemcpy(v_[c], tmp.data(), n_ * sizeof(R), hipMemcpyHostToDevice));
+    }
+}
+
+template <class R>
+void SolverImpl<R>::get_pressure(double* p) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    HIP_CHECK(hipMemcpy(p, p_, n_ * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+template <class R>
+void SolverImpl<R>::set_pressure(const double* p) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    HIP_CHECK(hipMemcpy(p_, p, n_ * sizeof(double), hipMemcpyHostToDevice));
+}
+
+template <class R>
+void SolverImpl<R>::get_faces(double* fx, double* fy, double* fz, double* fe) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::vector<R> tmp(std::max<int64_t>(n_, nedges_));
+    double* out[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; ++c) {
+        if (!out[c]) continue;
+        HIP_CHECK(hipMemcpy(tmp.data(), f_[c], n_ * sizeof(R), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n_; ++i) out[c][i] = double(tmp[i]);
+    }
+    if (fe && nedges_) {
+        HIP_CHECK(hipMemcpy(tmp.data(), fe_, nedges_ * sizeof(R), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nedges_; ++i) fe[i] = double(tmp[i]);
+    }
+}
+
+template <class R>
+void SolverImpl<R>::set_faces(const double* fx, const double* fy, const double* fz, const double* fe) {
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::vector<R> tmp(std::max<int64_t>(n_, nedges_));
+    const double* in[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; ++c) {
+        if (!in[c]) continue;
+        for (int64_t i = 0; i < n_; ++i) tmp[i] = static_cast<R>(in[c][i]);
+        HIP_CHECK(hipMemcpy(f_[c], tmp.data(), n_ * sizeof(R), hipMemcpyHostToDevice));
+    }
+    if (fe && nedges_) {
+        for (int i = 0; i < nedges_; ++i) tmp[i] = static_cast<R>(fe[i]);
+        HIP_CHECK(hipMemcpy(fe_, tmp.data(), nedges_ * sizeof(R), hipMemcpyHostToDevice));
+    }
+}
+
+template <class R>
+void SolverImpl<R>::mg_primitive(int op, int level, const double* x, const double* b, double* out, double omega, int sweeps) {
+    if (level < 0 || level >= std::max<int>(1, static_cast<int>(lv_.size()))) throw std::invalid_argument("multigrid level out of range");
+    if (opt_.vcycle_precision != PCFD_FP64) throw std::invalid_argument("multigrid primitives run in FP64 only");
+    const int64_t n = level == 0 ? n_ : lv_[level].n;
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    // clear the loop flags so the kernels run
+    std::memset(host_s_, 0, sizeof(DevScalars));
+    HIP_CHECK(hipMemcpy(S_, host_s_, sizeof(DevScalars), hipMemcpyHostToDevice));
+    double* dx = level == 0 ? static_cast<double*>(vc0_[0]) : static_cast<double*>(lv_[level].X);
+    double* dt = level == 0 ? static_cast<double*>(vc0_[1]) : static_cast<double*>(lv_[level].T);
+    double* db = level == 0 ? r_ : static_cast<double*>(lv_[level].B);
+    std::vector<double> zeros(n, 0.0);
+    HIP_CHECK(hipMemcpy(dx, x ? x : zeros.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(db, b ? b : zeros.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    const double* result = dt;
+    if (op == 0 || op == 1) {
+        // op 0 uses b = 0 and negates: A x = -(0 - A x)
+        if (op == 0) HIP_CHECK(hipMemset(db, 0, n * sizeof(double)));
+        if (level == 0) LAUNCH(k_residual0<double>, nb_, 512, M_, dx, db, dt, S_);
+        else LAUNCH((k_residual_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, dx, db, dt, S_);
+    } else if (op == 2) {
+        double* cur = dx;
+        double* oth = dt;
+        for (int s = 0; s < sweeps; ++s) {
+            if (level == 0) LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, cur, oth, db, z_, static_cast<double*>(nullptr), omega, partials_, S_, 1);
+            else LAUNCH((k_smooth_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, 1, cur, oth, db, omega, S_);
+            std::swap(cur, oth);
+        }
+        result = cur;
+    } else if (op == 3) {
+        if (level != 0) throw std::invalid_argument("V-cycle primitive runs on level 0");
+        precondition<double>(true);
+        result = z_;
+    } else {
+        throw std::invalid_argument("unknown multigrid primitive");
+    }
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    HIP_CHECK(hipMemcpy(out, result, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (op == 0) for (int64_t i = 0; i < n; ++i) out[i] = -out[i];
+}
+
+// Average duration of one launch of a hot kernel on the solver's own stream (HIP events),
+// with its algorithmic HBM bytes per launch (each array touched once, neighbours cached).
+template <class R>
+void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* bytes) {
+    if (reps < 1) throw std::invalid_argument("reps must be positive");
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    std::memset(host_s_, 0, sizeof(DevScalars));
+    host_s_->dt = 1e-3;
+    host_s_->rz = 1.0;
+    HIP_CHECK(hipMemcpy(S_, host_s_, sizeof(DevScalars), hipMemcpyHostToDevice));
+    int64_t nact = 0;
+    {
+        std::vector<uint8_t> m(n_);
+        HIP_CHECK(hipMemcpy(m.data(), M_.mask, n_, hipMemcpyDeviceToHost));
+        for (uint8_t v : m) nact += (v & kActive) ? 1 : 0;
+    }
+    const int vt = opt_.vcycle_precision == PCFD_FP64 ? 8 : (opt_.vcycle_precision == PCFD_FP32 ? 4 : 2);
+    auto launch = [&] {
+        switch (kernel) {
+            case 0: LAUNCH(k_cg_apply_dot, nb_, 512, M_, d_, Ad_, partials_, S_); break;
+            case 1:
+                if (opt_.vcycle_precision == PCFD_FP32)
+                    LAUNCH(k_smooth0<float>, nb_, 512, M_, 1, static_cast<float*>(vc0_[0]), static_cast<float*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_, 1);
+                else if (opt_.vcycle_precision == PCFD_BF16)
+                    LAUNCH(k_smooth0<bf16>, nb_, 512, M_, 1, static_cast<bf16*>(vc0_[0]), static_cast<bf16*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_, 1);
+                else
+                    LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_, 1);
+                break;
+            case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); break;
+            case 3: precondition_dispatch(true); break;
+            default: throw std::invalid_argument("unknown kernel id");
+        }
+    };
+    launch();  // warm
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
+    for (int i = 0; i < reps; ++i) launch();
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
+    HIP_CHECK(hipEventSynchronize(ev_[1]));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+    *avg_ms = ms / reps;
+    double per = 0;
+    switch (kernel) {
+        case 0: per = 8 + 8 + 1; break;                       // d in, Ad out, mask
+        case 1: per = vt + 8 + 1 + vt; break;                  // x in, r in, mask, x out
+        case 2: per = 9.0 * sizeof(R) + 1; break;             // v in (3), faces (3), mask, v out (3)
+        case 3: per = 0; break;
+    }
+    *bytes = per * static_cast<double>(nact);
+    // leave the loop flags as a finished solve so a stray kernel does nothing
+    host_s_->done = 1;
+    HIP_CHECK(hipMemcpy(&S_->done, &host_s_->done, sizeof(int), hipMemcpyHostToDevice));
+}
+
+}  // namespace pcfd
+
+// ==========================================================================================
+// C ABI
+// ==========================================================================================
+struct pcfd_solver {
+    std::unique_ptr<pcfd::Solver> impl;
+};
+
+namespace {
+template <class F>
+pcfd_status guarded(const char* where, F&& f) {
+    try {
+        f();
+        return PCFD_OK;
+    } catch (const pcfd::DeviceError& e) {
+        pcfd::set_error(std::string(where) + ": " + e.what());
+        return PCFD_EDEVICE;
+    } catch (const std::invalid_argument& e) {
+        pcfd::set_error(std::string(where) + ": " + e.what());
+        return PCFD_EINVAL;
+    } catch (const std::domain_error& e) {
+        pcfd::set_error(std::string(where) + ": " + e.what());
+        return PCFD_ENONFINITE;
+    } catch (const std::logic_error& e) {
+        pcfd::set_error(std::string(where) + ": " + e.what());
+        return PCFD_ESTATE;
+    } catch (const std::bad_alloc&) {
+        pcfd::set_error(std::string(where) + ": out of memory");
+        return PCFD_ENOMEM;
+    } catch (const std::exception& e) {
+        pcfd::set_error(std::string(where) + ": " + e.what());
+        return PCFD_EDEVICE;
+    }
+}
+#define NEED(cond, msg)                                   \
+    do {                                                  \
+        if (!(cond)) {                                    \
+            pcfd::set_error(msg);                         \
+            return PCFD_EINVAL;                           \
+        }                                                 \
+    } while (0)
+}  // namespace
+
+extern "C" {
+
+void pcfd_default_options(pcfd_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->rho = 1.0; o->nu = 1e-3; o->speed = 1.0; o->cfl = 0.5; o->smagorinsky_cs = 0.0; o->rtol = 1e-8;
+    o->max_iterations = 200; o->field_precision = PCFD_FP64; o->vcycle_precision = PCFD_FP64;
+    o->preconditioner = PCFD_PC_MG; o->pre_smooth = 2; o->post_smooth = 2; o->check_interval = 1;
+}
+
+pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* options, pcfd_solver** out) {
+    NEED(topology && options && out, "pcfd_create: null argument");
+    *out = nullptr;
+    return guarded("pcfd_create", [&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw pcfd::DeviceError("no HIP device");
+        if (options->smagorinsky_cs != 0.0) throw std::invalid_argument("the Smagorinsky model is not enabled in this build (Cs must be 0)");
+        if (options->preconditioner == PCFD_PC_LEARNED) throw std::invalid_argument("learned preconditioner is not available in this build");
+        auto s = std::make_unique<pcfd_solver>();
+        const pcfd::Topology& T = pcfd_topology_ref(topology);
+        if (options->field_precision == PCFD_FP32) s->impl = std::make_unique<pcfd::SolverImpl<float>>(T, *options);
+        else if (options->field_precision == PCFD_FP64) s->impl = std::make_unique<pcfd::SolverImpl<double>>(T, *options);
+        else throw std::invalid_argument("field precision must be FP64 or FP32");
+        s->impl->ref_area = T.reference_area;
+        *out = s.release();
+    });
+}
+
+void pcfd_destroy(pcfd_solver* s) { delete s; }
+
+pcfd_status pcfd_initialize(pcfd_solver* s, pcfd_step_stats* st) {
+    NEED(s, "pcfd_initialize: null solver");
+    return guarded("pcfd_initialize", [&] { s->impl->initialize(st); });
+}
+pcfd_status pcfd_step(pcfd_solver* s, pcfd_step_stats* st) {
+    NEED(s, "pcfd_step: null solver");
+    return guarded("pcfd_step", [&] { s->impl->step(st); });
+}
+pcfd_status pcfd_advect(pcfd_solver* s, double dt, double* dt_used, pcfd_step_stats* st) {
+    NEED(s, "pcfd_advect: null solver");
+    NEED(std::isfinite(dt), "pcfd_advect: non-finite dt");
+    return guarded("pcfd_advect", [&] { s->impl->advect(dt, dt_used, st); });
+}
+pcfd_status pcfd_project(pcfd_solver* s, double dt, int warm, pcfd_step_stats* st) {
+    NEED(s, "pcfd_project: null solver");
+    NEED(dt > 0 && std::isfinite(dt), "pcfd_project: dt must be positive");
+    return guarded("pcfd_project", [&] { s->impl->project(dt, warm != 0, st); });
+}
+pcfd_status pcfd_apply_pressure_impulse(pcfd_solver* s, double dt) {
+    NEED(s, "pcfd_apply_pressure_impulse: null solver");
+    NEED(dt > 0 && std::isfinite(dt), "pcfd_apply_pressure_impulse: dt must be positive");
+    return guarded("pcfd_apply_pressure_impulse", [&] { s->impl->impulse(dt); });
+}
+pcfd_status pcfd_integrate_loads(pcfd_solver* s, double pref, pcfd_loads* loads) {
+    NEED(s && loads, "pcfd_integrate_loads: null argument");
+    return guarded("pcfd_integrate_loads", [&] { s->impl->loads(pref, loads); });
+}
+pcfd_status pcfd_set_reference(pcfd_solver* s, double area, const double origin[3]) {
+    NEED(s, "pcfd_set_reference: null solver");
+    NEED(area >= 0 && std::isfinite(area), "pcfd_set_reference: invalid area");
+    s->impl->ref_area = area;
+    if (origin) for (int c = 0; c < 3; ++c) s->impl->origin[c] = origin[c];
+    return PCFD_OK;
+}
+pcfd_status pcfd_get_velocity(pcfd_solver* s, double* x, double* y, double* z) {
+    NEED(s, "pcfd_get_velocity: null solver");
+    return guarded("pcfd_get_velocity", [&] { s->impl->get_velocity(x, y, z); });
+}
+pcfd_status pcfd_set_velocity(pcfd_solver* s, const double* x, const double* y, const double* z) {
+    NEED(s, "pcfd_set_velocity: null solver");
+    return guarded("pcfd_set_velocity", [&] { s->impl->set_velocity(x, y, z); });
+}
+pcfd_status pcfd_get_pressure(pcfd_solver* s, double* p) {
+    NEED(s && p, "pcfd_get_pressure: null argument");
+    return guarded("pcfd_get_pressure", [&] { s->impl->get_pressure(p); });
+}
+pcfd_status pcfd_set_pressure(pcfd_solver* s, const double* p) {
+    NEED(s && p, "pcfd_set_pressure: null argument");
+    return guarded("pcfd_set_pressure", [&] { s->impl->set_pressure(p); });
+}
+pcfd_status pcfd_get_faces(pcfd_solver* s, double* fx, double* fy, double* fz, double* fe) {
+    NEED(s, "pcfd_get_faces: null solver");
+    return guarded("pcfd_get_faces", [&] { s->impl->get_faces(fx, fy, fz, fe); });
+}
+pcfd_status pcfd_set_faces(pcfd_solver* s, const double* fx, const double* fy, const double* fz, const double* fe) {
+    NEED(s, "pcfd_set_faces: null solver");
+    return guarded("pcfd_set_faces", [&] { s->impl->set_faces(fx, fy, fz, fe); });
+}
+pcfd_status pcfd_mark_initialized(pcfd_solver* s, double physical_time) {
+    NEED(s, "pcfd_mark_initialized: null solver");
+    s->impl->initialized = true;
+    s->impl->time = physical_time;
+    return PCFD_OK;
+}
+int64_t pcfd_device_bytes(const pcfd_solver* s) { return s ? s->impl->device_bytes : -1; }
+pcfd_status pcfd_mg_primitive(pcfd_solver* s, int op, int level, const double* x, const double* b, double* out,
+                              double omega, int sweeps) {
+    NEED(s && out, "pcfd_mg_primitive: null argument");
+    NEED(sweeps >= 0, "pcfd_mg_primitive: negative sweeps");
+    return guarded("pcfd_mg_primitive", [&] { s->impl->mg_primitive(op, level, x, b, out, omega, sweeps); });
+}
+pcfd_status pcfd_time_kernel(pcfd_solver* s, int kernel, int reps, double* avg_ms, double* bytes_per_launch) {
+    NEED(s && avg_ms && bytes_per_launch, "pcfd_time_kernel: null argument");
+    return guarded("pcfd_time_kernel", [&] { s->impl->time_kernel(kernel, reps, avg_ms, bytes_per_launch); });
+}
+
+}  // extern "C"

@@ -912,9 +912,10 @@ void Builder::finalize() {
     T_.reference_area = geo_->ref_area;
 }
 
-// Geometric multigrid by coarsening toward a uniform grid: at level m every cell smaller
-// than h_finest*2^m is merged with its siblings; coarse couplings are re-discretised as
-// open area / centre distance.  Transfers are piecewise constant (R = P^T).
+// Geometric multigrid: at every level each cell is merged with its 2x2x2 siblings (cells of
+// every AMR level double in size together, so neighbours stay within 2:1); coarse couplings
+// are re-discretised as open area / centre distance.  Transfers are piecewise constant
+// (R = P^T).
 void Builder::build_multigrid() {
     T_.mg.clear();
     MGLevel L0;
@@ -972,14 +973,38 @@ void Builder::build_multigrid() {
         for (int64_t a = 0; a < cur_n; ++a) nact += cur_act[a];
         if (m >= 1 && (nact <= coarse_max || m >= 10)) break;
         const int mm = m + 1;
-        // parent keys
+        // Parent keys: every cell proposes its 2x2x2 parent; a proposal is blocked when a
+        // smaller cell lies inside the parent's footprint (a finer AMR region that has not
+        // caught up yet), and the cell is then carried over unchanged.  Blocked parents never
+        // overlap an existing cell, so the coarse cells tile the fluid without overlap.
+        auto key_of = [](int e, int x, int y, int z) {
+            return (static_cast<uint64_t>(e) << 60) | morton(x, y, z);
+        };
+        int max_e = 0;
+        for (int64_t a = 0; a < cur_n; ++a) if (cur_act[a]) max_e = std::max(max_e, cur_exp[a]);
+        std::vector<uint64_t> blocked;
+        for (int64_t a = 0; a < cur_n; ++a) {
+            if (!cur_act[a]) continue;
+            const int e = cur_exp[a];
+            for (int E = e + 2; E <= max_e + 1; ++E) {
+                const int s = E - e;
+                blocked.push_back(key_of(E, cur_coord[3 * a] >> s, cur_coord[3 * a + 1] >> s, cur_coord[3 * a + 2] >> s));
+            }
+        }
+        std::sort(blocked.begin(), blocked.end());
+        blocked.erase(std::unique(blocked.begin(), blocked.end()), blocked.end());
         std::vector<std::pair<uint64_t, int64_t>> keys;
+        std::vector<uint8_t> grows(cur_n, 0);
         keys.reserve(nact);
         for (int64_t a = 0; a < cur_n; ++a) {
             if (!cur_act[a]) continue;
-            const int e = cur_exp[a], ep = std::max(e, mm), sh = ep - e;
-            const uint64_t key = (static_cast<uint64_t>(ep) << 60) |
-                                 morton(cur_coord[3 * a] >> sh, cur_coord[3 * a + 1] >> sh, cur_coord[3 * a + 2] >> sh);
+            const int e = cur_exp[a];
+            uint64_t key = key_of(e + 1, cur_coord[3 * a] >> 1, cur_coord[3 * a + 1] >> 1, cur_coord[3 * a + 2] >> 1);
+            if (std::binary_search(blocked.begin(), blocked.end(), key)) {
+                key = key_of(e, cur_coord[3 * a], cur_coord[3 * a + 1], cur_coord[3 * a + 2]);
+            } else {
+                grows[a] = 1;
+            }
             keys.push_back({key, a});
         }
         std::sort(keys.begin(), keys.end());
@@ -992,15 +1017,19 @@ void Builder::build_multigrid() {
         for (auto& [key, a] : keys) {
             if (key != last) {
                 ++nc; last = key;
-                const int e = cur_exp[a], ep = std::max(e, mm), sh = ep - e;
-                nexp.push_back(ep);
-                ncoord.push_back(cur_coord[3 * a] >> sh);
-                ncoord.push_back(cur_coord[3 * a + 1] >> sh);
-                ncoord.push_back(cur_coord[3 * a + 2] >> sh);
+                const int s = grows[a] ? 1 : 0;
+                nexp.push_back(cur_exp[a] + s);
+                ncoord.push_back(cur_coord[3 * a] >> s);
+                ncoord.push_back(cur_coord[3 * a + 1] >> s);
+                ncoord.push_back(cur_coord[3 * a + 2] >> s);
             }
             parent[a] = nc;
         }
         C.n = nc + 1;
+        if (C.n >= nact) {  // nothing merged: stop at the current level
+            parent.assign(cur_n, -1);
+            break;
+        }
         C.exponent = nexp;
         // children CSR
         C.child_off.assign(C.n + 1, 0);
