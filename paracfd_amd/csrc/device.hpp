@@ -134,37 +134,24 @@ __device__ __forceinline__ double block_max(double v) {
     return r;
 }
 
-// Publish this block's partial and report whether it is the last block to finish.  The
-// partials are then summed by the last block in block order, so the result does not depend
-// on scheduling.
-__device__ __forceinline__ bool publish_partial(double* partials, double value, unsigned int* ticket,
-                                                unsigned int nblocks) {
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(partials + blockIdx.x, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last != 0;
+// Reductions are two-stage and deterministic: every workgroup stores its partial with a
+// plain store (partials[array * nparts + block]); a one-workgroup finalize kernel launched
+// next on the same stream sums them in block order.  (A "last workgroup finalises" scheme
+// needs an agent-scope release fence per workgroup, which writes back L2 and cost ~0.4 ms
+// per launch at 8k workgroups on gfx950.)
+__device__ __forceinline__ void store_partial(double* partials, double value) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = value;
 }
 
-// Sum (or max) of nblocks partials in fixed order by the last block; result in thread 0.
+// Sum (or max) of n partials in fixed order; result valid in thread 0.
 template <int BLOCK>
-__device__ __forceinline__ double gather_partials(const double* partials, unsigned int nblocks, bool take_max) {
-    double acc = take_max ? 0.0 : 0.0;
-    // each lane accumulates a strided subset in index order, then a fixed tree
-    for (unsigned int i = threadIdx.x; i < nblocks; i += BLOCK) {
-        const double v = __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ double gather_partials(const double* partials, int n, bool take_max) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        const double v = partials[i];
         acc = take_max ? fmax(acc, v) : acc + v;
     }
     return take_max ? block_max<BLOCK>(acc) : block_sum<BLOCK>(acc);
-}
-
-__device__ __forceinline__ void reset_ticket(unsigned int* ticket) {
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace pcfd

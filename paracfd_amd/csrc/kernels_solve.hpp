@@ -73,16 +73,7 @@ __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* _
         ra = rhs[a] - apply_row0<double>(M, x, b, q, m, sp, M.h[M.brick_level[b]], dg);
     }
     r[a] = ra;
-    const double s = block_sum<512>(ra * ra);
-    if (publish_partial(partials, s, &S->ticket[5], gridDim.x)) {
-        const double t = gather_partials<512>(partials, gridDim.x, false);
-        if (threadIdx.x == 0) {
-            S->rr = t;
-            S->relres = sqrt(t) / S->bnorm;
-            if (S->relres <= M.rtol) { S->done = 1; S->converged = 1; }
-        }
-        reset_ticket(&S->ticket[5]);
-    }
+    store_partial(partials, block_sum<512>(ra * ra));
 }
 
 __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* __restrict__ d, double* __restrict__ Ad,
@@ -98,22 +89,12 @@ __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* 
         y = apply_row0<double>(M, d, b, q, m, sp, M.h[M.brick_level[b]], dg);
     }
     Ad[a] = y;
-    const double s = block_sum<512>(y * d[a]);
-    if (publish_partial(partials, s, &S->ticket[6], gridDim.x)) {
-        const double t = gather_partials<512>(partials, gridDim.x, false);
-        if (threadIdx.x == 0) {
-            S->pAp = t;
-            if (!(t > 0.0) || isinf(t)) { S->done = 1; S->breakdown = 1; }
-            else S->alpha = S->rz / t;
-        }
-        reset_ticket(&S->ticket[6]);
-    }
+    store_partial(partials, block_sum<512>(y * d[a]));
 }
 
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ rprev, const double* __restrict__ d,
-                                                   const double* __restrict__ Ad, double* partials, DevScalars* S,
-                                                   int max_iterations, double rtol) {
+                                                   const double* __restrict__ Ad, double* partials, const DevScalars* S) {
     if (S->done) return;
     const double alpha = S->alpha;
     double acc = 0.0;
@@ -125,17 +106,71 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, double* __restrict
         r[i] = rn;
         acc += rn * rn;
     }
-    const double s = block_sum<256>(acc);
-    if (publish_partial(partials, s, &S->ticket[7], gridDim.x)) {
-        const double t = gather_partials<256>(partials, gridDim.x, false);
-        if (threadIdx.x == 0) {
-            S->rr = t;
-            S->iter += 1;
-            S->relres = sqrt(t) / S->bnorm;
+    store_partial(partials, block_sum<256>(acc));
+}
+
+// Scalar finalisation of each reduction (one workgroup, partials summed in block order).
+enum FinalizeOp : int {
+    FIN_RATE = 0,     // max outflow rate -> dt = cfl / max(U/h, rate)
+    FIN_BNORM = 1,    // ||b||, resets the Krylov flags; zero RHS ends the solve
+    FIN_MAXDIV = 2,   // max |div| / V
+    FIN_VMAX = 3,     // max |v|
+    FIN_RESID = 4,    // initial ||r||, convergence test
+    FIN_APPLY = 5,    // pAp -> alpha, breakdown guard
+    FIN_UPDATE = 6,   // ||r||, iteration count, convergence / iteration limit
+    FIN_RZ = 7,       // r.z (and r_prev.z) -> beta, rz
+    FIN_RZ_INITIAL = 8,
+    FIN_LOADS = 9     // six load sums
+};
+
+__global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restrict__ partials, int nparts,
+                                                   DevScalars* S, double cfl, double rate_floor, double rtol,
+                                                   int max_iterations, int flexible) {
+    if ((op == FIN_RESID || op == FIN_APPLY || op == FIN_UPDATE || op == FIN_RZ || op == FIN_RZ_INITIAL) && S->done) return;
+    const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX;
+    const int arrays = op == FIN_LOADS ? 6 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts, nparts, take_max);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case FIN_RATE:
+            S->rate = t[0];
+            S->dt = cfl / fmax(rate_floor, t[0]);
+            break;
+        case FIN_BNORM:
+            S->bnorm = sqrt(t[0]);
+            S->iter = 0; S->done = 0; S->converged = 0; S->breakdown = 0; S->zero_rhs = 0;
+            if (!(t[0] > 0.0)) { S->done = 1; S->converged = 1; S->zero_rhs = 1; S->relres = 0.0; }
+            break;
+        case FIN_MAXDIV: S->maxdiv = t[0]; break;
+        case FIN_VMAX: S->vmax = t[0]; break;
+        case FIN_RESID:
+            S->rr = t[0];
+            S->relres = sqrt(t[0]) / S->bnorm;
             if (S->relres <= rtol) { S->done = 1; S->converged = 1; }
-            else if (S->iter >= max_iterations || !(t == t)) S->done = 1;
-        }
-        reset_ticket(&S->ticket[7]);
+            break;
+        case FIN_APPLY:
+            S->pAp = t[0];
+            if (!(t[0] > 0.0) || isinf(t[0])) { S->done = 1; S->breakdown = 1; }
+            else S->alpha = S->rz / t[0];
+            break;
+        case FIN_UPDATE:
+            S->rr = t[0];
+            S->iter += 1;
+            S->relres = sqrt(t[0]) / S->bnorm;
+            if (S->relres <= rtol) { S->done = 1; S->converged = 1; }
+            else if (S->iter >= max_iterations || !(t[0] == t[0])) S->done = 1;
+            break;
+        case FIN_RZ:
+            // Fletcher-Reeves for a fixed preconditioner; Polak-Ribiere (flexible CG) with r_prev
+            S->beta = (t[0] - t[1]) / S->rz;
+            S->rz = t[0];
+            break;
+        case FIN_RZ_INITIAL: S->rz = t[0]; break;
+        case FIN_LOADS:
+            for (int c = 0; c < 6; ++c) S->loads[c] = t[c];
+            break;
+        default: break;
     }
 }
 
@@ -168,12 +203,12 @@ __device__ __forceinline__ A diag_row0(const MeshView& M, int b, int q, uint8_t 
     return dg;
 }
 
-// partials holds 2 * gridDim.x entries: r.z then r_prev.z
+// mode 2 stores two partial arrays of gridDim.x entries: r.z then r_prev.z
 template <class VT>
 __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
                                                  const double* __restrict__ rhs, double* __restrict__ zout,
                                                  const double* __restrict__ rprev, double omega, double* partials,
-                                                 DevScalars* S, int initial) {
+                                                 const DevScalars* S) {
     if (S->done) return;
     using A = typename Arith<VT>::type;
     const int b = blockIdx.x, q = threadIdx.x;
@@ -199,25 +234,10 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
     zout[a] = z;
     const double s1 = block_sum<512>(rhs[a] * z);
     const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
-    __shared__ int is_last;
     if (threadIdx.x == 0) {
-        __hip_atomic_store(partials + blockIdx.x, s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(partials + gridDim.x + blockIdx.x, s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = __hip_atomic_fetch_add(&S->ticket[8], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = t == gridDim.x - 1;
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    const double t1 = gather_partials<512>(partials, gridDim.x, false);
-    const double t2 = rprev ? gather_partials<512>(partials + gridDim.x, gridDim.x, false) : 0.0;
-    if (threadIdx.x == 0) {
-        // Fletcher-Reeves for fixed preconditioners, Polak-Ribiere (flexible) with r_prev
-        if (!initial) S->beta = (t1 - t2) / S->rz;
-        S->rz = t1;
-    }
-    reset_ticket(&S->ticket[8]);
 }
 
 template <class VT>

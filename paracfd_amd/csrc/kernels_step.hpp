@@ -39,8 +39,7 @@ __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[
 // CFL: max over cells of sum(max(outflow, 0)) / V  (amr_advection.cu:169-186) -> dt
 // ---------------------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> F, double* partials, DevScalars* S,
-                                                      int set_dt) {
+__global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> F, double* partials) {
     const int b = blockIdx.x, q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -68,18 +67,7 @@ __global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> 
         }
         rate = out / V;
     }
-    const double bm = block_max<512>(rate);
-    if (publish_partial(partials, bm, &S->ticket[0], gridDim.x)) {
-        const double r = gather_partials<512>(partials, gridDim.x, true);
-        if (threadIdx.x == 0) {
-            S->rate = r;
-            if (set_dt) {
-                const double lim = fmax(fmax(1e-9, fabs(M.speed) / M.finest_h), r);
-                S->dt = M.cfl / lim;
-            }
-        }
-        reset_ticket(&S->ticket[0]);
-    }
+    store_partial(partials, block_max<512>(rate));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -256,7 +244,7 @@ __device__ __forceinline__ double cell_divergence(const MeshView& M, const Field
 // RHS = -(rho/dt) * integrated divergence (amr_pressure.cu:255-284); ||b|| reduction
 template <class R>
 __global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double* __restrict__ rhs, double* partials,
-                                             DevScalars* S) {
+                                             const DevScalars* S) {
     const int b = blockIdx.x, q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -264,21 +252,12 @@ __global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double
     double val = 0.0;
     if (m & kActive) val = -(M.rho / S->dt) * cell_divergence(M, F, b, q, m, sp, cell_h(M, b));
     rhs[a] = val;
-    const double bs = block_sum<512>(val * val);
-    if (publish_partial(partials, bs, &S->ticket[1], gridDim.x)) {
-        const double t = gather_partials<512>(partials, gridDim.x, false);
-        if (threadIdx.x == 0) {
-            S->bnorm = sqrt(t);
-            S->iter = 0; S->done = 0; S->converged = 0; S->breakdown = 0; S->zero_rhs = 0;
-            if (!(t > 0.0)) { S->done = 1; S->converged = 1; S->zero_rhs = 1; S->relres = 0.0; }
-        }
-        reset_ticket(&S->ticket[1]);
-    }
+    store_partial(partials, block_sum<512>(val * val));
 }
 
 // max |integrated divergence| / V after the correction (FP64 check)
 template <class R>
-__global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, double* partials, DevScalars* S) {
+__global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, double* partials) {
     const int b = blockIdx.x, q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -289,12 +268,7 @@ __global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, d
         const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
         val = fabs(cell_divergence(M, F, b, q, m, sp, h) / V);
     }
-    const double bm = block_max<512>(val);
-    if (publish_partial(partials, bm, &S->ticket[2], gridDim.x)) {
-        const double t = gather_partials<512>(partials, gridDim.x, true);
-        if (threadIdx.x == 0) S->maxdiv = t;
-        reset_ticket(&S->ticket[2]);
-    }
+    store_partial(partials, block_max<512>(val));
 }
 
 // face correction u -= (dt/rho) dp/dn (amr_pressure.cu:285-310); outlet p = 0 half a cell away
@@ -385,7 +359,7 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
 // M = sum (x - origin) x F; deterministic block partials.
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p, double pref, double ox,
-                                               double oy, double oz, double* partials, DevScalars* S) {
+                                               double oy, double oz, double* partials) {
     const int pch = blockIdx.x * blockDim.x + threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (pch < M.npatches) {
@@ -395,45 +369,23 @@ __global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restr
         v[0] = Fx; v[1] = Fy; v[2] = Fz;
         v[3] = ry * Fz - rz * Fy; v[4] = rz * Fx - rx * Fz; v[5] = rx * Fy - ry * Fx;
     }
-    double part[6];
-    for (int c = 0; c < 6; ++c) part[c] = block_sum<256>(v[c]);
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < 6; ++c)
-            __hip_atomic_store(partials + 6 * blockIdx.x + c, part[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = __hip_atomic_fetch_add(&S->ticket[3], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = t == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
     for (int c = 0; c < 6; ++c) {
-        double acc = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += 256)
-            acc += __hip_atomic_load(partials + 6 * i + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        acc = block_sum<256>(acc);
-        if (threadIdx.x == 0) S->loads[c] = acc;
+        const double part = block_sum<256>(v[c]);
+        if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = part;
     }
-    reset_ticket(&S->ticket[3]);
 }
 
-// max |v| over active cells
+// max |v| over active cells; a non-finite component makes the maximum infinite
 template <class R>
-__global__ __launch_bounds__(512) void k_vmax(MeshView M, FieldsView<R> F, double* partials, DevScalars* S) {
+__global__ __launch_bounds__(512) void k_vmax(MeshView M, FieldsView<R> F, double* partials) {
     const int64_t a = static_cast<int64_t>(blockIdx.x) * 512 + threadIdx.x;
     double v = 0.0;
     if (M.mask[a] & kActive) {
-        const double x = fabs(double(F.v[0][a])), y = fabs(double(F.v[1][a])), z = fabs(double(F.v[2][a]));
-        v = fmax(x, fmax(y, z));
-        if (!(v == v) || isinf(v)) v = INFINITY;  // non-finite wins the max
+        const double x = double(F.v[0][a]), y = double(F.v[1][a]), z = double(F.v[2][a]);
+        v = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) v = INFINITY;
     }
-    const double bm = block_max<512>(v);
-    if (publish_partial(partials, bm, &S->ticket[4], gridDim.x)) {
-        const double t = gather_partials<512>(partials, gridDim.x, true);
-        if (threadIdx.x == 0) S->vmax = t;
-        reset_ticket(&S->ticket[4]);
-    }
+    store_partial(partials, block_max<512>(v));
 }
 
 template <class R>

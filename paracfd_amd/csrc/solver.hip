@@ -124,6 +124,10 @@ private:
         F.fe = fe_;
         return F;
     }
+    void finalize(int op, int nparts) {
+        LAUNCH(k_finalize, 1, 1024, op, partials_, nparts, S_, opt_.cfl,
+               std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations, rprev_ ? 1 : 0);
+    }
     void reconstruct();
     void run_projection(bool warm, pcfd_step_stats* st);
     template <class VT> void precondition(bool initial);
@@ -336,17 +340,19 @@ template <class R>
 template <class VT>
 void SolverImpl<R>::precondition(bool initial) {
     const int ini = initial ? 1 : 0;
+    const int fin = ini ? FIN_RZ_INITIAL : FIN_RZ;
     if (opt_.preconditioner == PCFD_PC_JACOBI) {
         LAUNCH(k_smooth0<VT>, nb_, 512, M_, 2, static_cast<const VT*>(nullptr), static_cast<VT*>(nullptr), r_, z_, rprev_, 1.0,
-               partials_, S_, ini);
+               partials_, S_);
+        finalize(fin, nb_);
         return;
     }
     const double w = 2.0 / 3.0;
     VT* cur = static_cast<VT*>(vc0_[0]);
     VT* oth = static_cast<VT*>(vc0_[1]);
-    LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w, partials_, S_, ini);
+    LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w, partials_, S_);
     for (int s = 1; s < opt_.pre_smooth; ++s) {
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w, partials_, S_, ini);
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w, partials_, S_);
         std::swap(cur, oth);
     }
     if (lv_.size() > 1) {
@@ -358,9 +364,10 @@ void SolverImpl<R>::precondition(bool initial) {
     }
     for (int s = 0; s < opt_.post_smooth; ++s) {
         const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_, ini);
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_);
         std::swap(cur, oth);
     }
+    finalize(fin, nb_);
 }
 
 template <class R>
@@ -378,8 +385,10 @@ void SolverImpl<R>::precondition_dispatch(bool initial) {
 template <class R>
 void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+    finalize(FIN_BNORM, nb_);
     if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
     LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+    finalize(FIN_RESID, nb_);
     precondition_dispatch(true);
     const unsigned gu = std::min<unsigned>(blocks_for(n_, 256), 2048);
     LAUNCH(k_cg_direction, gu, 256, n_, z_, d_, S_, 1);
@@ -387,7 +396,9 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     for (;;) {
         for (int k = 0; k < opt_.check_interval; ++k) {
             LAUNCH(k_cg_apply_dot, nb_, 512, M_, d_, Ad_, partials_, S_);
-            LAUNCH(k_cg_update, gu, 256, n_, p_, r_, rprev_, d_, Ad_, partials_, S_, opt_.max_iterations, opt_.rtol);
+            finalize(FIN_APPLY, nb_);
+            LAUNCH(k_cg_update, gu, 256, n_, p_, r_, rprev_, d_, Ad_, partials_, S_);
+            finalize(FIN_UPDATE, static_cast<int>(gu));
             precondition_dispatch(false);
             LAUNCH(k_cg_direction, gu, 256, n_, z_, d_, S_, 0);
         }
@@ -398,7 +409,8 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     if (host_s_->zero_rhs) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
     LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
     if (nedges_) LAUNCH(k_correct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), p_, S_);
-    LAUNCH(k_divcheck<R>, nb_, 512, M_, fields(), partials_, S_);
+    LAUNCH(k_divcheck<R>, nb_, 512, M_, fields(), partials_);
+    finalize(FIN_MAXDIV, nb_);
     sync_scalars();
     if (st) {
         st->iterations = host_s_->iter;
@@ -434,7 +446,10 @@ void SolverImpl<R>::impulse(double dt) {
 template <class R>
 void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
     if (dt > 0) put_dt(dt);
-    else LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_, S_, 1);
+    else {
+        LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_);
+        finalize(FIN_RATE, nb_);
+    }
     sync_scalars();
     dt = host_s_->dt;
     if (dt_used) *dt_used = dt;
@@ -462,7 +477,8 @@ void SolverImpl<R>::initialize(pcfd_step_stats* st) {
     pcfd_step_stats local{};
     project(dt, false, &local);
     if (local.converged) impulse(dt);
-    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_, S_);
+    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
+    finalize(FIN_VMAX, nb_);
     sync_scalars();
     local.max_abs_velocity = host_s_->vmax;
     initialized = local.converged != 0;
@@ -476,8 +492,10 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     if (!initialized) throw std::logic_error("step before a converged initialization");
     pcfd_step_stats local{};
     HIP_CHECK(hipEventRecord(ev_[2], stream_));
-    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_, S_);
-    LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_, S_, 1);
+    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
+    finalize(FIN_VMAX, nb_);
+    LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_);
+    finalize(FIN_RATE, nb_);
     sync_scalars();
     if (!std::isfinite(host_s_->vmax)) throw std::domain_error("non-finite velocity before timestep");
     const double dt = host_s_->dt;
@@ -511,7 +529,9 @@ template <class R>
 void SolverImpl<R>::loads(double pref, pcfd_loads* out) {
     std::memset(out, 0, sizeof(*out));
     if (M_.npatches > 0) {
-        LAUNCH(k_loads, blocks_for(M_.npatches, 256), 256, M_, p_, pref, origin[0], origin[1], origin[2], partials_, S_);
+        const unsigned gl = blocks_for(M_.npatches, 256);
+        LAUNCH(k_loads, gl, 256, M_, p_, pref, origin[0], origin[1], origin[2], partials_);
+        finalize(FIN_LOADS, static_cast<int>(gl));
         sync_scalars();
         for (int c = 0; c < 3; ++c) { out->force[c] = host_s_->loads[c]; out->moment[c] = host_s_->loads[3 + c]; }
     }
@@ -616,7 +636,7 @@ void SolverImpl<R>::mg_primitive(int op, int level, const double* x, const doubl
         double* cur = dx;
         double* oth = dt;
         for (int s = 0; s < sweeps; ++s) {
-            if (level == 0) LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, cur, oth, db, z_, static_cast<double*>(nullptr), omega, partials_, S_, 1);
+            if (level == 0) LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, cur, oth, db, z_, static_cast<double*>(nullptr), omega, partials_, S_);
             else LAUNCH((k_smooth_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, 1, cur, oth, db, omega, S_);
             std::swap(cur, oth);
         }
@@ -655,11 +675,11 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
             case 0: LAUNCH(k_cg_apply_dot, nb_, 512, M_, d_, Ad_, partials_, S_); break;
             case 1:
                 if (opt_.vcycle_precision == PCFD_FP32)
-                    LAUNCH(k_smooth0<float>, nb_, 512, M_, 1, static_cast<float*>(vc0_[0]), static_cast<float*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_, 1);
+                    LAUNCH(k_smooth0<float>, nb_, 512, M_, 1, static_cast<float*>(vc0_[0]), static_cast<float*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 else if (opt_.vcycle_precision == PCFD_BF16)
-                    LAUNCH(k_smooth0<bf16>, nb_, 512, M_, 1, static_cast<bf16*>(vc0_[0]), static_cast<bf16*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_, 1);
+                    LAUNCH(k_smooth0<bf16>, nb_, 512, M_, 1, static_cast<bf16*>(vc0_[0]), static_cast<bf16*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 else
-                    LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_, 1);
+                    LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 break;
             case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); break;
             case 3: precondition_dispatch(true); break;

@@ -1,0 +1,342 @@
+"""Physics contracts that pin the FP64 oracle where no golden vector reaches
+(tools/physics_contract_probe.cpp:245-416 states the same families of checks):
+
+  * composite operator: brute-force dense assembly is symmetric, row sums vanish except
+    2h per outlet face, and matches an independent 7-point assembly on a box
+  * MUSCL transport reduces to textbook 1-D MUSCL-minmod with slope factor 0.5(1-C)
+  * uniform freestream is preserved; zero-dt / zero-flux transport is a no-op
+  * transport conserves momentum up to the boundary fluxes
+  * affine velocity -> exact structured face reconstruction
+  * constant pressure -> zero impulse and zero load; linear pressure -> uniform
+    acceleration -grad(p)/rho in regular cells and body force ~ -V e_c
+  * the projection leaves a discretely divergence-free field (true residual checked)
+  * the V-cycle preconditioner is symmetric (R = P^T, symmetric smoothing)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from paracfd_amd import _lib as L, cases
+
+
+def options(**kw):
+    o = L.Options()
+    o.rho, o.nu, o.speed, o.cfl, o.rtol, o.max_iterations = 1.0, 1e-3, 1.0, 0.5, 1e-8, 300
+    o.pre_smooth, o.post_smooth, o.check_interval = 2, 2, 1
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@pytest.fixture(scope="module")
+def box_topo():
+    g, d = cases.uniform_box(n=8, h=0.125, outlet=True)
+    return L.Topology(g, d)
+
+
+@pytest.fixture(scope="module")
+def sphere_topo():
+    g, d = cases.sphere_box(cells_x=64)
+    return L.Topology(g, d)
+
+
+@pytest.fixture(scope="module")
+def amr_topo():
+    g, d = cases.sphere_box(cells_x=32, levels=2, band=0.3)
+    return L.Topology(g, d)
+
+
+def geometry(t):
+    n = t.ndof
+    idx = np.arange(n)
+    q = idx % 512
+    ijk = np.stack([q & 7, (q >> 3) & 7, q >> 6], axis=1)
+    coord = t.array("brick_coord").reshape(-1, 3)[idx // 512]
+    h = t.array("level_h")[t.array("brick_level")][idx // 512]
+    centre = np.array(t.info.lo[:]) + (coord * 8 + ijk + 0.5) * h[:, None]
+    act = (t.array("mask") & 64) != 0
+    return centre, h, act
+
+
+def test_dense_operator_box(box_topo, oracle_built):
+    """8^3 box with outlet: the composite operator equals an independent 7-point assembly."""
+    orc = Oracle(box_topo, options())
+    A = orc.dense_operator()
+    n = box_topo.ndof
+    h = 0.125
+    ref = np.zeros((n, n))
+    for a in range(n):
+        i, j, k = a & 7, (a >> 3) & 7, a >> 6
+        for (di, dj, dk) in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            ii, jj, kk = i + di, j + dj, k + dk
+            if 0 <= ii < 8 and 0 <= jj < 8 and 0 <= kk < 8:
+                b = ii + 8 * jj + 64 * kk
+                ref[a, a] += h
+                ref[a, b] -= h
+        if i == 7:
+            ref[a, a] += 2 * h  # Dirichlet p = 0 half a cell beyond x-max
+    assert np.array_equal(A, ref)
+
+
+def test_composite_operator_cut_amr(amr_topo, oracle_built):
+    """Symmetric (<Ax, y> = <x, Ay>), positive, and A 1 = 2h on outlet cells, 0 elsewhere."""
+    orc = Oracle(amr_topo, options())
+    centre, h, act = geometry(amr_topo)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        x = rng.uniform(-1, 1, amr_topo.ndof) * act
+        y = rng.uniform(-1, 1, amr_topo.ndof) * act
+        Ax = orc.mg_primitive(0, 0, x=x)
+        Ay = orc.mg_primitive(0, 0, x=y)
+        assert abs(np.dot(Ax, y) - np.dot(x, Ay)) < 1e-13 * np.abs(Ax).sum()
+        assert np.dot(Ax, x) > 0
+    rows = orc.mg_primitive(0, 0, x=act.astype(float))[act]
+    flags = np.repeat(amr_topo.array("brick_flags"), 512)[act]
+    q = np.arange(amr_topo.ndof)[act] % 512
+    outlet = ((q & 7) == 7) & ((flags & 2) != 0)
+    assert np.allclose(rows[outlet], 2 * h[act][outlet], rtol=1e-12)
+    assert np.allclose(rows[~outlet], 0.0, atol=1e-14)
+
+
+def muscl_1d(v, u, dt, h, inflow):
+    """textbook conservative MUSCL-minmod on one line, uniform face velocity u > 0"""
+    n = len(v)
+    C = u * dt / h
+    pred = 0.5 * (1 - min(1.0, C))
+
+    def minmod(a, b):
+        return 0.0 if a * b <= 0 else (min(a, b) if a >= 0 else max(a, b))
+
+    face = np.empty(n + 1)
+    face[0] = inflow
+    for f in range(1, n + 1):
+        i = f - 1
+        if f == n:
+            face[f] = v[i]  # outlet: interior donor
+        elif i == 0:
+            face[f] = v[i]  # no upwind neighbour inside the domain
+        else:
+            face[f] = v[i] + pred * minmod(v[i] - v[i - 1], v[i + 1] - v[i])
+    return v - dt * u / h * (face[1:] - face[:-1])
+
+
+def test_muscl_reduces_to_textbook_1d(oracle_built):
+    g, d = cases.uniform_box(n=16, h=0.0625, outlet=True)
+    t = L.Topology(g, d)
+    orc = Oracle(t, options(nu=0.0))
+    centre, h, act = geometry(t)
+    x, y, z = centre.T
+    U = 1.0
+    vy = np.sin(2 * np.pi * x) * (1 + 0.5 * y) + np.where(x > 0.5, 0.3, 0.0)
+    vz = np.cos(3 * x) * z
+    vx = np.full(t.ndof, U)
+    orc.set_state(vx, vy, vz, np.zeros(t.ndof))
+    fx = np.full(t.ndof, U)
+    orc.set_faces(fx, np.zeros(t.ndof), np.zeros(t.ndof), np.zeros(0))
+    dt = 0.03
+    used = orc.advect(dt)
+    assert used == dt
+    gx, gy, gz, _ = orc.state()
+    for comp, (before, after) in enumerate(((vy, gy), (vz, gz))):
+        for j in range(16):
+            for k in range(16):
+                line = np.where((np.abs(centre[:, 1] - (j + 0.5) * 0.0625) < 1e-12) &
+                                (np.abs(centre[:, 2] - (k + 0.5) * 0.0625) < 1e-12))[0]
+                line = line[np.argsort(centre[line, 0])]
+                ref = muscl_1d(before[line], U, dt, 0.0625, 0.0)
+                assert np.allclose(after[line], ref, rtol=0, atol=1e-14)
+
+
+def test_freestream_preserved(sphere_topo, oracle_built):
+    g, d = cases.uniform_box(n=16, h=0.0625, outlet=True)
+    t = L.Topology(g, d)
+    orc = Oracle(t, options())
+    orc.set_reference(0.0)
+    st = orc.initialize()
+    assert st.converged
+    for _ in range(3):
+        st = orc.step()
+    x, y, z, p = orc.state()
+    assert np.max(np.abs(x - 1.0)) < 1e-12 and np.max(np.abs(y)) < 1e-12 and np.max(np.abs(z)) < 1e-12
+    assert np.max(np.abs(p)) < 1e-10
+
+
+def test_zero_flux_and_zero_pressure_are_noops(sphere_topo, oracle_built):
+    orc = Oracle(sphere_topo, options(nu=0.0))
+    centre, h, act = geometry(sphere_topo)
+    rng = np.random.default_rng(3)
+    v = [rng.uniform(-1, 1, sphere_topo.ndof) * act for _ in range(3)]
+    orc.set_state(*v, np.zeros(sphere_topo.ndof))
+    orc.set_faces(np.zeros(sphere_topo.ndof), np.zeros(sphere_topo.ndof), np.zeros(sphere_topo.ndof),
+                  np.zeros(sphere_topo.info.compact_edges))
+    # zero face fluxes: only the x-min inflow face moves momentum; switch it off with U = 0
+    orc2 = Oracle(sphere_topo, options(nu=0.0, speed=0.0))
+    orc2.set_state(*v, np.zeros(sphere_topo.ndof))
+    orc2.set_faces(np.zeros(sphere_topo.ndof), np.zeros(sphere_topo.ndof), np.zeros(sphere_topo.ndof),
+                   np.zeros(sphere_topo.info.compact_edges))
+    orc2.advect(0.01)
+    after = orc2.state()
+    for c in range(3):
+        assert np.array_equal(after[c], v[c])
+    # constant pressure: the closed control volumes see no net impulse, except the outlet
+    # cells whose x-max face carries the Dirichlet p = 0
+    orc2.set_state(*v, np.full(sphere_topo.ndof, 7.0))
+    orc2.impulse(0.01)
+    after = orc2.state()
+    flags = np.repeat(sphere_topo.array("brick_flags"), 512)
+    q = np.arange(sphere_topo.ndof) % 512
+    outlet = ((q & 7) == 7) & ((flags & 2) != 0)
+    for c in range(3):
+        assert np.max(np.abs(after[c] - v[c])[act & ~outlet]) < 1e-12 * 7.0 / h.min()
+    # p = 7 on the -x face and 0 on the +x face pushes the outlet cells downstream
+    assert np.allclose((after[0] - v[0])[act & outlet], 0.01 * 7.0 / h[act & outlet], rtol=1e-12)
+
+
+def test_transport_conserves_momentum(amr_topo, oracle_built):
+    """sum V dv equals the boundary fluxes only (interior faces cancel)."""
+    t = amr_topo
+    U = 1.0
+    orc = Oracle(t, options(nu=0.0, speed=U))
+    centre, h, act = geometry(t)
+    rng = np.random.default_rng(5)
+    v = [(rng.uniform(-1, 1, t.ndof) + (1.0 if c == 0 else 0.0)) * act for c in range(3)]
+    f = [rng.uniform(0.5, 1.5, t.ndof) * act for _ in range(3)]
+    fe = rng.uniform(-1, 1, t.info.compact_edges)
+    orc.set_state(*v, np.zeros(t.ndof))
+    orc.set_faces(*f, fe)
+    vol = np.repeat(t.array("level_h")[t.array("brick_level")] ** 3, 512)
+    vol[t.array("special_dof")] = t.array("special_volume")
+    before = [np.sum(vol[act] * v[c][act]) for c in range(3)]
+    dt = 1e-3
+    orc.advect(dt)
+    after = orc.state()
+    flags = np.repeat(t.array("brick_flags"), 512)
+    q = np.arange(t.ndof) % 512
+    xmin = act & ((q & 7) == 0) & ((flags & 1) != 0)
+    xmax = act & ((q & 7) == 7) & ((flags & 2) != 0)
+    A = h * h
+    expected = [0.0, 0.0, 0.0]
+    expected[0] += np.sum(dt * A[xmin] * U * U)
+    for c in range(3):
+        expected[c] -= np.sum(dt * A[xmax] * f[0][xmax] * v[c][xmax])
+    for c in range(3):
+        got = np.sum(vol[act] * after[c][act]) - before[c]
+        assert abs(got - expected[c]) < 1e-12 * max(1.0, abs(before[c]))
+
+
+def test_affine_velocity_reconstructs_exactly(sphere_topo, oracle_built):
+    t = sphere_topo
+    orc = Oracle(t, options(nu=0.0, speed=0.0))
+    centre, h, act = geometry(t)
+    x, y, z = centre.T
+    v = [0.3 + 0.7 * x - 0.2 * y + 0.1 * z, -0.5 + 0.4 * y + 0.3 * z, 0.2 - 0.6 * z + 0.9 * x]
+    orc.set_state(*[a * act for a in v], np.zeros(t.ndof))
+    orc.set_faces(np.zeros(t.ndof), np.zeros(t.ndof), np.zeros(t.ndof), np.zeros(t.info.compact_edges))
+    # reconstruction happens at the end of advect; zero faces and U = 0 leave v unchanged first
+    orc.advect(1e-6)
+    fx, fy, fz, _ = orc.faces()
+    mask = t.array("mask")
+    for ax, f in enumerate((fx, fy, fz)):
+        sel = (mask & (1 << ax)) != 0
+        pos = centre.copy()
+        pos[:, ax] += 0.5 * h
+        exact = [0.3 + 0.7 * pos[:, 0] - 0.2 * pos[:, 1] + 0.1 * pos[:, 2], -0.5 + 0.4 * pos[:, 1] + 0.3 * pos[:, 2],
+                 0.2 - 0.6 * pos[:, 2] + 0.9 * pos[:, 0]][ax]
+        assert np.max(np.abs(f[sel] - exact[sel])) < 1e-13
+
+
+def test_linear_pressure_impulse_and_loads(sphere_topo, oracle_built):
+    t = sphere_topo
+    orc = Oracle(t, options(nu=0.0, speed=0.0, rho=2.0))
+    orc.set_reference(t.info.reference_area)
+    centre, h, act = geometry(t)
+    mask = t.array("mask")
+    p = 3.0 + centre[:, 0] - 0.5 * centre[:, 2]
+    zero = np.zeros(t.ndof)
+    orc.set_state(zero, zero, zero, p)
+    dt = 0.01
+    orc.impulse(dt)
+    vx, vy, vz, _ = orc.state()
+    regular = act & ((mask & 128) == 0)
+    flags = np.repeat(t.array("brick_flags"), 512)
+    q = np.arange(t.ndof)
+    ijk = np.stack([q & 7, (q >> 3) & 7, (q >> 6) & 7], axis=1)
+    interior = regular & (mask & 63 == 63)
+    # -dt/rho grad p = -0.005 * (1, 0, -0.5)
+    assert np.allclose(vx[interior], -0.005, atol=1e-13)
+    assert np.allclose(vy[interior], 0.0, atol=1e-13)
+    assert np.allclose(vz[interior], 0.0025, atol=1e-13)
+    # loads: constant pressure gives zero force; linear pressure gives -V grad p
+    orc.set_state(zero, zero, zero, np.full(t.ndof, 5.0))
+    ld = orc.loads()
+    assert np.max(np.abs(ld["force"])) < 1e-12
+    orc.set_state(zero, zero, zero, 3.0 + centre[:, 0])
+    ld = orc.loads()
+    V = t.info.body_volume
+    assert abs(ld["force"][0] + V) < 0.05 * V
+    assert abs(ld["force"][1]) < 0.02 * V and abs(ld["force"][2]) < 0.02 * V
+
+
+def test_projection_is_divergence_free(amr_topo, oracle_built):
+    t = amr_topo
+    orc = Oracle(t, options())
+    st = orc.initialize()
+    assert st.converged and st.relative_residual <= 1e-8
+    for _ in range(3):
+        st = orc.step()
+        assert st.converged
+    div = orc.divergence()
+    centre, h, act = geometry(t)
+    # the divergence left is the solver residual scaled by dt/rho
+    assert np.max(np.abs(div[act])) / np.max(h[act]) ** 2 < 1e-5
+    assert st.max_abs_divergence < 1e-4
+
+
+def test_true_residual_after_solve(amr_topo, oracle_built):
+    t = amr_topo
+    orc = Oracle(t, options())
+    orc.initialize()
+    orc.step()
+    centre, h, act = geometry(t)
+    # re-project the already projected faces: the new RHS is the tiny leftover divergence,
+    # so check the solve directly on a manufactured right-hand side instead
+    rng = np.random.default_rng(9)
+    b = rng.uniform(-1, 1, t.ndof) * act
+    z = orc.mg_primitive(3, 0, b=b)
+    Az = orc.mg_primitive(0, 0, x=z)
+    # one V-cycle reduces the residual substantially
+    assert np.linalg.norm((b - Az)[act]) < 0.7 * np.linalg.norm(b[act])
+
+
+def test_vcycle_is_symmetric(amr_topo, oracle_built):
+    t = amr_topo
+    orc = Oracle(t, options())
+    centre, h, act = geometry(t)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1, 1, t.ndof) * act
+    y = rng.uniform(-1, 1, t.ndof) * act
+    Mx = orc.mg_primitive(3, 0, b=x)
+    My = orc.mg_primitive(3, 0, b=y)
+    assert abs(np.dot(Mx, y) - np.dot(x, My)) < 1e-12 * abs(np.dot(Mx, y))
+    assert np.dot(Mx, x) > 0
+
+
+def test_planted_mistake_in_transport_is_caught(oracle_built):
+    """A dropped Courant factor (slope factor 0.5 instead of 0.5(1-C)) fails the 1-D check."""
+    v = np.sin(np.linspace(0, 3, 16))
+    good = muscl_1d(v, 1.0, 0.03, 0.0625, 0.0)
+    C = 0.03 / 0.0625
+    n = len(v)
+    face = np.empty(n + 1)
+    face[0] = 0.0
+    for f in range(1, n + 1):
+        i = f - 1
+        if f == n or i == 0:
+            face[f] = v[i]
+        else:
+            a, b = v[i] - v[i - 1], v[i + 1] - v[i]
+            face[f] = v[i] + 0.5 * (0.0 if a * b <= 0 else (min(a, b) if a >= 0 else max(a, b)))
+    bad = v - 0.03 / 0.0625 * (face[1:] - face[:-1])
+    assert C > 0.1 and np.max(np.abs(bad - good)) > 1e-4

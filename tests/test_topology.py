@@ -1,0 +1,167 @@
+"""Invariants of the shared brick / cut-cell topology and the multigrid hierarchy
+(host preprocessing, no GPU).  Closed forms: sphere volume and area, discrete closure of
+every control volume, 2:1 nesting, symmetric coarse operators."""
+import math
+
+import numpy as np
+import pytest
+
+from paracfd_amd import _lib as L, cases
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    g, d = cases.sphere_box(cells_x=64)
+    return L.Topology(g, d)
+
+
+@pytest.fixture(scope="module")
+def sphere_amr():
+    g, d = cases.sphere_box(cells_x=32, levels=2, band=0.3)
+    return L.Topology(g, d)
+
+
+@pytest.fixture(scope="module")
+def wing():
+    g, d = cases.naca_wing(digits=2412, alpha_deg=4.0, cells_x=48, levels=2, bands=(0.3,))
+    return L.Topology(g, d)
+
+
+def cell_closure(t):
+    """sum over each cell's open faces of (outward normal * open area) plus its wall area vector"""
+    n = t.ndof
+    mask = t.array("mask")
+    h = t.array("level_h")[t.array("brick_level")]
+    hd = np.repeat(h, 512)
+    flags = np.repeat(t.array("brick_flags"), 512)
+    q = np.arange(n) % 512
+    ijk = [q & 7, (q >> 3) & 7, q >> 6]
+    clo = np.zeros((n, 3))
+    A = hd * hd
+    for ax in range(3):
+        clo[:, ax] += np.where(mask & (1 << ax), A, 0.0)
+        clo[:, ax] -= np.where(mask & (8 << ax), A, 0.0)
+        lo_flag, hi_flag = 1 << (2 * ax), 2 << (2 * ax)
+        clo[:, ax] -= np.where((ijk[ax] == 0) & (flags & lo_flag != 0), A, 0.0)
+        clo[:, ax] += np.where((ijk[ax] == 7) & (flags & hi_flag != 0), A, 0.0)
+    ea, eb, ax_, area = t.array("edge_a"), t.array("edge_b"), t.array("edge_axis"), t.array("edge_area")
+    np.add.at(clo, (ea, ax_), area)
+    np.add.at(clo, (eb, ax_), -area)
+    S = t.array("patch_S").reshape(-1, 3)
+    np.add.at(clo, t.array("patch_dof"), S)
+    act = (mask & 64) != 0
+    return clo[act], A[act]
+
+
+@pytest.mark.parametrize("name", ["sphere", "sphere_amr", "wing"])
+def test_every_control_volume_closes(name, request):
+    t = request.getfixturevalue(name)
+    clo, A = cell_closure(t)
+    assert np.max(np.abs(clo) / A[:, None]) < 1e-12
+
+
+def test_sphere_volume_and_area(sphere):
+    r = 0.25  # 4 cells per radius here; cut cells below 10 % fluid become solid
+    assert abs(sphere.info.body_volume - 4.0 / 3.0 * math.pi * r ** 3) < 0.03 * 4.0 / 3.0 * math.pi * r ** 3
+    S = sphere.array("patch_S").reshape(-1, 3)
+    assert np.allclose(S.sum(axis=0), 0.0, atol=1e-12)
+    area = np.linalg.norm(S, axis=1).sum()
+    assert abs(area - 4 * math.pi * r ** 2) < 0.05 * 4 * math.pi * r ** 2
+    # wall normals point from the fluid into the body (towards the centre)
+    X = sphere.array("patch_x").reshape(-1, 3)
+    outward = (X - np.array([1.0, 1.0, 1.0]))
+    assert np.mean(np.sum(S * outward, axis=1) < 0) > 0.99
+
+
+def test_wing_volume(wing):
+    # area of the NACA 2412 section (closed trailing edge) times span
+    info = wing.info
+    assert abs(info.body_volume - 0.0822 * 2.03) < 0.03 * 0.0822 * 2.03
+    assert info.reference_area == pytest.approx(2.03)
+
+
+@pytest.mark.parametrize("name", ["sphere_amr", "wing"])
+def test_coarse_fine_edges(name, request):
+    t = request.getfixturevalue(name)
+    lev = np.repeat(t.array("brick_level"), 512)
+    h = t.array("level_h")
+    ea, eb = t.array("edge_a"), t.array("edge_b")
+    la, lb = lev[ea], lev[eb]
+    assert np.all(np.abs(la - lb) <= 1)
+    ha, hb = h[la], h[lb]
+    coef, area, w = t.array("edge_coef"), t.array("edge_area"), t.array("edge_w")
+    assert np.allclose(coef, area / (0.5 * (ha + hb)), rtol=1e-14)
+    assert np.allclose(w, ha / (ha + hb), rtol=1e-14)
+    cf = la != lb
+    assert cf.sum() > 0
+    # a fine face of a coarse/fine edge is never larger than the fine cell face
+    assert np.all(area[cf] <= np.minimum(ha, hb)[cf] ** 2 * (1 + 1e-12))
+
+
+def test_structured_faces_are_symmetric(sphere_amr):
+    t = sphere_amr
+    mask = t.array("mask").astype(np.int64)
+    nbr = t.array("brick_nbr").reshape(-1, 6)
+    n = t.ndof
+    idx = np.arange(n)
+    b, q = idx // 512, idx % 512
+    for ax in range(3):
+        has = (mask & (1 << ax)) != 0
+        stride = 8 ** ax
+        c = (q >> (3 * ax)) & 7
+        inside = c < 7
+        other = np.where(inside, idx + stride, nbr[b, 2 * ax + 1] * 512 + q - 7 * stride)
+        sel = has
+        assert np.all(nbr[b[sel & ~inside], 2 * ax + 1] >= 0)
+        assert np.all((mask[other[sel]] & (8 << ax)) != 0)
+        assert np.all((mask[other[sel]] & 64) != 0)
+
+
+@pytest.mark.parametrize("name", ["sphere", "wing"])
+def test_multigrid_levels(name, request):
+    t = request.getfixturevalue(name)
+    prev = t.n_active
+    for m in range(1, t.mg_levels):
+        ro, col, coef, diag = (t.mg_array(m, k) for k in ("row_off", "col", "coef", "diag"))
+        n = diag.size
+        assert n < prev
+        prev = n
+        A = np.zeros((n, n)) if n <= 3000 else None
+        rows = np.repeat(np.arange(n), np.diff(ro))
+        assert np.all(coef > 0)
+        assert np.all(diag >= np.bincount(rows, weights=coef, minlength=n) * (1 - 1e-12))
+        if A is not None:
+            A[rows, col] = -coef
+            A[np.arange(n), np.arange(n)] = diag
+            assert np.allclose(A, A.T, rtol=0, atol=1e-14 * diag.max())
+        parent = t.mg_array(m - 1, "parent")
+        if m == 1:
+            act = (t.array("mask") & 64) != 0
+            assert np.all(parent[act] >= 0) and np.all(parent[~act] == -1)
+            counts = np.bincount(parent[act], minlength=n)
+        else:
+            counts = np.bincount(parent, minlength=n)
+        assert np.all(counts >= 1) and np.all(counts <= 8)
+    inv = t.mg_array(t.mg_levels - 1, "coarse_inv")
+    n = t.mg_n(t.mg_levels - 1)
+    ro, col, coef, diag = (t.mg_array(t.mg_levels - 1, k) for k in ("row_off", "col", "coef", "diag"))
+    A = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(ro))
+    A[rows, col] = -coef
+    A[np.arange(n), np.arange(n)] = diag
+    assert np.allclose(A @ inv.reshape(n, n), np.eye(n), atol=1e-8)
+
+
+def test_refinement_band(wing):
+    """level-1 bricks exist only near the wing, the far field stays at level 0"""
+    lev = wing.array("brick_level")
+    coord = wing.array("brick_coord").reshape(-1, 3)
+    h = wing.array("level_h")
+    lo = np.array(wing.info.lo[:])
+    centre = lo + (coord + 0.5) * 8 * h[lev][:, None]
+    fine = lev == 1
+    assert fine.sum() > 0
+    # every fine brick centre lies within chord + band + brick diagonal of the wing box
+    box_lo, box_hi = np.array([-0.1, -1.1, -0.2]), np.array([1.1, 1.1, 0.2])
+    d = np.maximum(0, np.maximum(box_lo - centre[fine], centre[fine] - box_hi))
+    assert np.all(np.linalg.norm(d, axis=1) < 0.3 + 2 * 8 * h[0] * math.sqrt(3))
