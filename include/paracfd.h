@@ -153,6 +153,8 @@ typedef struct pcfd_options {
     int pre_smooth, post_smooth; /* V-cycle Jacobi sweeps */
     int check_interval;       /* Krylov iterations between host convergence reads (>=1);
                                  results do not depend on it */
+    double smoother_weight;   /* damped-Jacobi weight of the V-cycle sweeps (default 6/7;
+                                 the reference's algebraic MG uses 2/3, amr_pressure.cu:1103) */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {

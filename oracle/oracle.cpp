@@ -363,11 +363,11 @@ struct Oracle {
                     x[r] = s;
                 }
             } else {
-                jacobi(m, x, b, 64, 2.0 / 3.0, true);
+                jacobi(m, x, b, 64, 2.0 / 3.0, true);  // fallback without a factorisation
             }
             return;
         }
-        const double omega = 2.0 / 3.0;
+        const double omega = opt.smoother_weight;
         jacobi(m, x, b, opt.pre_smooth, omega, true);
         std::vector<double> Ax, r(nn), bc(mg[m + 1].n, 0.0), xc;
         apply_level(m, x, Ax);

@@ -56,7 +56,8 @@ class Options(C.Structure):
     _fields_ = [("rho", C.c_double), ("nu", C.c_double), ("speed", C.c_double), ("cfl", C.c_double),
                 ("smagorinsky_cs", C.c_double), ("rtol", C.c_double), ("max_iterations", C.c_int),
                 ("field_precision", C.c_int), ("vcycle_precision", C.c_int), ("preconditioner", C.c_int),
-                ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int)]
+                ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int),
+                ("smoother_weight", C.c_double)]
 
 
 class StepStats(C.Structure):

@@ -69,9 +69,83 @@ struct CsrView {
     const int* parent;  // into the next coarser level
     const int* child_off;
     const int* child;
+    // the same rows in SELL-64: slices of 64 rows, slice s holds its rows slot-major at
+    // sl_off[s] + 64 k + (r & 63) with k < (sl_off[s+1] - sl_off[s]) / 64, padded with
+    // (column r, coefficient 0); coalesced, and only as wide as its longest row
+    const int* sl_off;
+    const int* sl_col;
+    const void* sl_coef;
 };
 
+// LDS tile of one brick with a one-cell halo: 10 x 10 x 10 values
+constexpr int kTileN = 1000;
+__device__ __forceinline__ int tile_index(int i, int j, int k) { return (k + 1) * 100 + (j + 1) * 10 + (i + 1); }
+
+// Fill the tile with val(dof) for the brick's 512 cells (coalesced) and the 6 x 64 face
+// halo cells of the same-level neighbour bricks (zero where there is none).  Every lane of
+// the 512-lane workgroup must call it.
+template <class A, class Fn>
+__device__ __forceinline__ void load_tile(const MeshView& M, int b, A* tile, Fn val) {
+    const int q = threadIdx.x;
+    tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)] = val(static_cast<int64_t>(b) * 512 + q);
+    if (q < 384) {
+        const int face = q >> 6, t = q & 63, axis = face >> 1, up = face & 1;
+        const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+        int c[3];
+        c[a1] = t & 7;
+        c[a2] = t >> 3;
+        c[axis] = up ? 8 : -1;
+        const int nb = M.brick_nbr[6 * b + face];
+        A v = A(0);
+        if (nb >= 0) {
+            int s[3] = {c[0], c[1], c[2]};
+            s[axis] = up ? 0 : 7;
+            v = val(static_cast<int64_t>(nb) * 512 + s[0] + 8 * s[1] + 64 * s[2]);
+        }
+        tile[tile_index(c[0], c[1], c[2])] = v;
+    }
+    __syncthreads();
+}
+
+// structured part of (A y)_a from the tile: faces in the order +x, -x, +y, -y, +z, -z,
+// then the x-max outlet (2h y_a)
+template <class A>
+__device__ __forceinline__ A tile_apply(const MeshView& M, const A* tile, int b, int q, uint8_t m, A h, A ya, A& dg) {
+    const int c = tile_index(q & 7, (q >> 3) & 7, q >> 6);
+    const int step[3] = {1, 10, 100};
+    A acc = A(0), d = A(0);
+    for (int axis = 0; axis < 3; ++axis) {
+        if (m & (kPX << axis)) { acc += h * (ya - tile[c + step[axis]]); d += h; }
+        if (m & (kMX << axis)) { acc += h * (ya - tile[c - step[axis]]); d += h; }
+    }
+    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) { acc += A(2) * h * ya; d += A(2) * h; }
+    dg = d;
+    return acc;
+}
+
+// compact-edge part of (A y)_a for a special cell (global gathers through val)
+template <class A, class Fn>
+__device__ __forceinline__ void compact_apply(const MeshView& M, int sp, Fn val, A ya, A& acc, A& dg) {
+    for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
+        const int raw = M.special_inc[k];
+        const int e = raw >= 0 ? raw : ~raw;
+        const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
+        const A c = A(M.ecoef[e]);
+        acc += c * (ya - val(o));
+        dg += c;
+    }
+}
+
 __device__ __forceinline__ int axis_coord(int q, int axis) { return (q >> (3 * axis)) & 7; }
+
+// XCD-aware brick order: workgroups are dispatched round-robin over the 8 XCDs of the
+// MI355X, so block b runs on XCD b % 8.  Giving every XCD one contiguous range of the
+// Morton-ordered bricks keeps a brick and its neighbours (halo reads) in the same L2.
+__device__ __forceinline__ int brick_of_block() {
+    const int G = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    const int x = b & 7, idx = b >> 3, per = G >> 3, rem = G & 7;
+    return x * per + (x < rem ? x : rem) + idx;
+}
 
 // neighbour dof across face (axis, sign) of local cell q in brick b; the caller has checked
 // the structured-face bit, so a cross-brick neighbour exists

@@ -1,11 +1,14 @@
 // Projection solve kernels: FP64 preconditioned conjugate gradients with device-resident
 // scalars and a device convergence flag, and the geometric V-cycle in FP64 / FP32 / BF16
-// storage (VT).  The composite level uses the brick layout; coarser levels are CSR.
+// storage (VT).  The composite level uses the brick layout with an LDS tile per brick;
+// coarser levels are ELL (slot-major).
 //
-// The Krylov scalars never leave the device: each reduction kernel's last workgroup
-// finalises alpha / beta / the relative residual and raises S->done, and every kernel of
-// the loop returns immediately once it is set, so the host may enqueue several iterations
-// between convergence reads without changing the result.
+// The Krylov scalars never leave the device: a one-workgroup finalize kernel after each
+// reduction computes alpha / beta / the relative residual and raises S->done, and every
+// composite-level kernel of the loop returns immediately once it is set, so the host may
+// enqueue (or replay as a graph) several iterations between convergence reads without
+// changing the result.  Coarse-level kernels do not test the flag: after convergence they
+// only rewrite their own scratch levels, which the next V-cycle overwrites.
 #pragma once
 
 #include "device.hpp"
@@ -29,67 +32,105 @@ __device__ __forceinline__ bf16 from_f<bf16>(float x) {
     const unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
     return bf16{static_cast<uint16_t>(r >> 16)};
 }
+template <class VT>
+__device__ __forceinline__ typename Arith<VT>::type round_vt(typename Arith<VT>::type v) {
+    return to_f(from_f<VT>(v));
+}
 
-// (A x)_a and the diagonal on the composite level (amr_pressure.cu:72-93 structured part,
-// :105-134 compact gather, outlet 2h p at x-max)
-template <class A, class T>
-__device__ __forceinline__ A apply_row0(const MeshView& M, const T* __restrict__ x, int b, int q, uint8_t m, int sp,
-                                        A h, A& diag) {
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const A xa = to_f(x[a]);
-    A acc = A(0), dg = A(0);
+__device__ __forceinline__ double level_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
+
+// composite-level diagonal (sum of face conductances + outlet + compact edges)
+template <class A>
+__device__ __forceinline__ A diag_row0(const MeshView& M, int b, int q, uint8_t m, int sp, A h) {
+    A dg = A(0);
     for (int axis = 0; axis < 3; ++axis) {
-        if (m & (kPX << axis)) { acc += h * (xa - A(to_f(x[neighbor_dof(M, b, q, axis, 1)]))); dg += h; }
-        if (m & (kMX << axis)) { acc += h * (xa - A(to_f(x[neighbor_dof(M, b, q, axis, -1)]))); dg += h; }
+        if (m & (kPX << axis)) dg += h;
+        if (m & (kMX << axis)) dg += h;
     }
-    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) { acc += A(2) * h * xa; dg += A(2) * h; }
+    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) dg += A(2) * h;
     if (sp >= 0)
         for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
             const int raw = M.special_inc[k];
-            const int e = raw >= 0 ? raw : ~raw;
-            const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
-            const A c = A(M.ecoef[e]);
-            acc += c * (xa - A(to_f(x[o])));
-            dg += c;
+            dg += A(M.ecoef[raw >= 0 ? raw : ~raw]);
         }
-    diag = dg;
+    return dg;
+}
+
+// (A y)_a on the composite level (amr_pressure.cu:72-93 structured part, :105-134 compact
+// gather, outlet 2h p at x-max) with y already in the brick's LDS tile; val gives y at any
+// dof for the compact edges.  Returns the row and its diagonal.
+template <class A, class Fn>
+__device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, int b, int q, uint8_t m, int sp, A ya, A& dg) {
+    A acc = tile_apply<A>(M, tile, b, q, m, A(level_h(M, b)), ya, dg);
+    if (sp >= 0) compact_apply<A>(M, sp, val, ya, acc, dg);
     return acc;
 }
 
+#define PCFD_BRICK_PROLOGUE                                        \
+    const int b = brick_of_block(), q = threadIdx.x;               \
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;           \
+    const uint8_t m = M.mask[a];                                   \
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0)
+
 // ---------------------------------------------------------------------------------------
-// PCG (amr_pressure.cu:1267-1353 solve_orthogonal): r = b - A x, alpha, x/r update, beta
+// PCG (amr_pressure.cu:1267-1353 solve_orthogonal)
 // ---------------------------------------------------------------------------------------
+// r = b - A x and the ||r||^2 partial
 __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* __restrict__ x,
                                                      const double* __restrict__ rhs, double* __restrict__ r,
-                                                     double* partials, DevScalars* S) {
+                                                     double* partials, const DevScalars* S) {
     if (S->done) return;
-    const int b = blockIdx.x, q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    __shared__ double tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    auto val = [&](int64_t c) -> double { return x[c]; };
+    load_tile<double>(M, b, tile, val);
     double ra = 0.0;
     if (m & kActive) {
         double dg;
-        ra = rhs[a] - apply_row0<double>(M, x, b, q, m, sp, M.h[M.brick_level[b]], dg);
+        ra = rhs[a] - row0<double>(M, tile, val, b, q, m, sp, x[a], dg);
     }
     r[a] = ra;
     store_partial(partials, block_sum<512>(ra * ra));
 }
 
+// A d and the d.Ad partial (used by the measurement hook)
 __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* __restrict__ d, double* __restrict__ Ad,
-                                                      double* partials, DevScalars* S) {
+                                                      double* partials, const DevScalars* S) {
     if (S->done) return;
-    const int b = blockIdx.x, q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    __shared__ double tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    auto val = [&](int64_t c) -> double { return d[c]; };
+    load_tile<double>(M, b, tile, val);
     double y = 0.0;
     if (m & kActive) {
         double dg;
-        y = apply_row0<double>(M, d, b, q, m, sp, M.h[M.brick_level[b]], dg);
+        y = row0<double>(M, tile, val, b, q, m, sp, d[a], dg);
     }
     Ad[a] = y;
     store_partial(partials, block_sum<512>(y * d[a]));
+}
+
+// CG direction update fused into the operator apply: d_new = z + beta d_old (first
+// iteration: z), Ad = A d_new and the d_new.Ad partial; d is double-buffered
+__global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* __restrict__ z,
+                                                      const double* __restrict__ dold, double* __restrict__ dnew,
+                                                      double* __restrict__ Ad, double* partials, const DevScalars* S,
+                                                      int initial) {
+    if (S->done) return;
+    __shared__ double tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    const double beta = initial ? 0.0 : S->beta;
+    auto dn = [&](int64_t c) -> double { return initial ? z[c] : z[c] + beta * dold[c]; };
+    load_tile<double>(M, b, tile, dn);
+    const double da = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
+    double y = 0.0;
+    if (m & kActive) {
+        double dg;
+        y = row0<double>(M, tile, dn, b, q, m, sp, da, dg);
+    }
+    dnew[a] = da;
+    Ad[a] = y;
+    store_partial(partials, block_sum<512>(y * da));
 }
 
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
@@ -174,36 +215,22 @@ __global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restr
     }
 }
 
-__global__ __launch_bounds__(256) void k_cg_direction(int64_t n, const double* __restrict__ z, double* __restrict__ d,
-                                                      const DevScalars* S, int initial) {
-    if (S->done) return;
-    const double beta = initial ? 0.0 : S->beta;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
-        d[i] = initial ? z[i] : z[i] + beta * d[i];
-}
-
 // ---------------------------------------------------------------------------------------
 // V-cycle, composite level: damped Jacobi (project.cu:76-83 restated with the composite
-// operator).  mode 0: x = w b / D; 1: x += w (b - A x) / D; 2: last sweep, writes z (FP64)
-// and the r.z (and r_prev.z for flexible CG) reductions that finalise beta.
+// operator).  Every kernel reproduces the arithmetic (including the rounding to the
+// storage type VT between stages) of the plain sequence "sweep, sweep, residual, restrict,
+// prolong, sweep, sweep"; the fused ones only remove HBM round trips.
 // ---------------------------------------------------------------------------------------
-template <class A>
-__device__ __forceinline__ A diag_row0(const MeshView& M, int b, int q, uint8_t m, int sp, A h) {
-    A dg = A(0);
-    for (int axis = 0; axis < 3; ++axis) {
-        if (m & (kPX << axis)) dg += h;
-        if (m & (kMX << axis)) dg += h;
-    }
-    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) dg += A(2) * h;
-    if (sp >= 0)
-        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
-            const int raw = M.special_inc[k];
-            dg += A(M.ecoef[raw >= 0 ? raw : ~raw]);
-        }
-    return dg;
+
+// composite-level diagonal in the V-cycle arithmetic type (setup, once)
+template <class DT>
+__global__ __launch_bounds__(512) void k_diag0(MeshView M, DT* __restrict__ diag) {
+    PCFD_BRICK_PROLOGUE;
+    diag[a] = (m & kActive) ? diag_row0<DT>(M, b, q, m, sp, DT(level_h(M, b))) : DT(1);
 }
 
-// mode 2 stores two partial arrays of gridDim.x entries: r.z then r_prev.z
+// One sweep.  mode 0: x = w b / D (from zero); 1: x += w (b - A x) / D; 2: last sweep,
+// writes z (FP64) and the r.z (and r_prev.z) partials (two arrays of gridDim.x entries).
 template <class VT>
 __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
                                                  const double* __restrict__ rhs, double* __restrict__ zout,
@@ -211,26 +238,28 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
                                                  const DevScalars* S) {
     if (S->done) return;
     using A = typename Arith<VT>::type;
-    const int b = blockIdx.x, q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    A val = A(0);
+    __shared__ A tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    const bool from_zero = mode == 0 || xin == nullptr;
+    auto val = [&](int64_t c) -> A { return from_zero ? A(0) : A(to_f(xin[c])); };
+    load_tile<A>(M, b, tile, val);
+    A v = A(0);
     if (m & kActive) {
-        const A h = A(M.h[M.brick_level[b]]);
-        if (mode == 0 || xin == nullptr) {
-            val = A(omega) * A(rhs[a]) / diag_row0<A>(M, b, q, m, sp, h);
+        const A h = A(level_h(M, b));
+        if (from_zero) {
+            v = A(omega) * A(rhs[a]) / diag_row0<A>(M, b, q, m, sp, h);
         } else {
             A dg;
-            const A Ax = apply_row0<A, VT>(M, xin, b, q, m, sp, h, dg);
-            val = A(to_f(xin[a])) + A(omega) * (A(rhs[a]) - Ax) / dg;
+            const A xa = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
+            const A Ax = row0<A>(M, tile, val, b, q, m, sp, xa, dg);
+            v = xa + A(omega) * (A(rhs[a]) - Ax) / dg;
         }
     }
     if (mode != 2) {
-        xout[a] = from_f<VT>(val);
+        xout[a] = from_f<VT>(v);
         return;
     }
-    const double z = double(val);
+    const double z = double(v);
     zout[a] = z;
     const double s1 = block_sum<512>(rhs[a] * z);
     const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
@@ -240,23 +269,131 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
     }
 }
 
+// residual b - A x on the composite level (unfused path and the primitive entry point)
 template <class VT>
 __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
                                                    VT* __restrict__ res, const DevScalars* S) {
     if (S->done) return;
     using A = typename Arith<VT>::type;
-    const int b = blockIdx.x, q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    A val = A(0);
+    __shared__ A tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
+    load_tile<A>(M, b, tile, val);
+    A v = A(0);
     if (m & kActive) {
         A dg;
-        val = A(rhs[a]) - apply_row0<A, VT>(M, x, b, q, m, sp, A(M.h[M.brick_level[b]]), dg);
+        v = A(rhs[a]) - row0<A>(M, tile, val, b, q, m, sp, tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)], dg);
     }
-    res[a] = from_f<VT>(val);
+    res[a] = from_f<VT>(v);
 }
 
+// two pre-smoothing sweeps from x = 0:  x1 = w r / D,  x2 = x1 + w (r - A x1) / D
+template <class VT>
+__global__ __launch_bounds__(512) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
+                                                 const typename Arith<VT>::type* __restrict__ diag, VT* __restrict__ xout,
+                                                 double omega, const DevScalars* S) {
+    if (S->done) return;
+    using A = typename Arith<VT>::type;
+    __shared__ A tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    const A w = A(omega);
+    auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rhs[c]) / diag[c]); };
+    load_tile<A>(M, b, tile, x1);
+    A v = A(0);
+    if (m & kActive) {
+        A dg;
+        const A xa = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
+        const A Ax = row0<A>(M, tile, x1, b, q, m, sp, xa, dg);
+        v = xa + w * (A(rhs[a]) - Ax) / dg;
+    }
+    xout[a] = from_f<VT>(v);
+}
+
+// residual r - A x summed over each 2x2x2 group into its level-1 parent (children of one
+// parent always lie in one brick); cells that were not merged map one-to-one
+template <class VT>
+__global__ __launch_bounds__(512) void k_vc_restrict0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
+                                                      const int* __restrict__ parent0, VT* __restrict__ coarse,
+                                                      const DevScalars* S) {
+    if (S->done) return;
+    using A = typename Arith<VT>::type;
+    __shared__ A tile[kTileN];
+    __shared__ A sres[512];
+    __shared__ int spar[512];
+    PCFD_BRICK_PROLOGUE;
+    auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
+    load_tile<A>(M, b, tile, val);
+    A v = A(0);
+    if (m & kActive) {
+        A dg;
+        v = A(rhs[a]) - row0<A>(M, tile, val, b, q, m, sp, tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)], dg);
+    }
+    sres[q] = round_vt<VT>(v);
+    spar[q] = parent0[a];
+    __syncthreads();
+    if ((q & 0x49) != 0) return;  // leaders have even i, j, k
+    const int kids[8] = {q, q + 1, q + 8, q + 9, q + 64, q + 65, q + 72, q + 73};
+    int P = -1;
+    bool shared = true;
+    for (int t = 0; t < 8; ++t) {
+        const int pc = spar[kids[t]];
+        if (pc < 0) continue;
+        if (P < 0) P = pc;
+        else if (pc != P) shared = false;
+    }
+    if (P < 0) return;
+    if (shared) {
+        A s = A(0);
+        for (int t = 0; t < 8; ++t) if (spar[kids[t]] >= 0) s += sres[kids[t]];
+        coarse[P] = from_f<VT>(s);
+    } else {
+        for (int t = 0; t < 8; ++t) {
+            const int pc = spar[kids[t]];
+            if (pc >= 0) coarse[pc] = from_f<VT>(A(0) + sres[kids[t]]);
+        }
+    }
+}
+
+// prolongation x += e[parent] fused with one post-smoothing sweep; `last` writes z (FP64)
+// and the r.z / r_prev.z partials like k_smooth0 mode 2
+template <class VT>
+__global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, const VT* __restrict__ x,
+                                                         const VT* __restrict__ e1, const int* __restrict__ parent0,
+                                                         const double* __restrict__ rhs, VT* __restrict__ xout,
+                                                         double* __restrict__ zout, const double* __restrict__ rprev,
+                                                         double omega, double* partials, const DevScalars* S) {
+    if (S->done) return;
+    using A = typename Arith<VT>::type;
+    __shared__ A tile[kTileN];
+    PCFD_BRICK_PROLOGUE;
+    auto y = [&](int64_t c) -> A {
+        const int pc = parent0[c];
+        const A xc = A(to_f(x[c]));
+        return pc >= 0 ? round_vt<VT>(xc + A(to_f(e1[pc]))) : xc;
+    };
+    load_tile<A>(M, b, tile, y);
+    A v = A(0);
+    if (m & kActive) {
+        A dg;
+        const A ya = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
+        const A Ay = row0<A>(M, tile, y, b, q, m, sp, ya, dg);
+        v = ya + A(omega) * (A(rhs[a]) - Ay) / dg;
+    }
+    if (!last) {
+        xout[a] = from_f<VT>(v);
+        return;
+    }
+    const double z = double(v);
+    zout[a] = z;
+    const double s1 = block_sum<512>(rhs[a] * z);
+    const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+// unfused transfers (primitive entry point and pre/post counts other than the fused ones)
 // coarse b = sum of the children's residuals (R = P^T, piecewise-constant P)
 template <class VT>
 __global__ void k_restrict(CsrView C, const VT* __restrict__ fine, VT* __restrict__ coarse, const DevScalars* S) {
@@ -280,11 +417,24 @@ __global__ void k_prolong(int64_t n, const int* __restrict__ parent, VT* __restr
     if (pc >= 0) fine[i] = from_f<VT>(A(to_f(fine[i])) + A(to_f(coarse[pc])));
 }
 
-// CSR level: mode 0 first sweep from zero, 1 sweep, residual when res != nullptr
+// ---------------------------------------------------------------------------------------
+// Coarse levels
+// ---------------------------------------------------------------------------------------
+// (A y)_r = D_r y_r - sum coef y_col, entries in CSR order (the SELL slots hold them in
+// order, then zero padding)
+template <class A, class CT, class Fn>
+__device__ __forceinline__ A ell_row(const CsrView& C, int r, Fn y, A yr) {
+    const CT* coef = static_cast<const CT*>(C.sl_coef);
+    const int s0 = C.sl_off[r >> 6], s1 = C.sl_off[(r >> 6) + 1];
+    A Ax = A(static_cast<const CT*>(C.diag)[r]) * yr;
+    for (int s = s0 + (r & 63); s < s1; s += 64) Ax -= A(coef[s]) * y(C.sl_col[s]);
+    return Ax;
+}
+
+// CSR forms (multigrid primitive entry point; same arithmetic as the ELL kernels)
 template <class VT, class CT>
 __global__ void k_smooth_csr(CsrView C, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
                              const VT* __restrict__ rhs, double omega, const DevScalars* S) {
-    if (S->done) return;
     using A = typename Arith<VT>::type;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= C.n) return;
@@ -301,7 +451,6 @@ __global__ void k_smooth_csr(CsrView C, int mode, const VT* __restrict__ xin, VT
 template <class VT, class CT>
 __global__ void k_residual_csr(CsrView C, const VT* __restrict__ x, const VT* __restrict__ rhs, VT* __restrict__ res,
                                const DevScalars* S) {
-    if (S->done) return;
     using A = typename Arith<VT>::type;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= C.n) return;
@@ -312,11 +461,76 @@ __global__ void k_residual_csr(CsrView C, const VT* __restrict__ x, const VT* __
     res[r] = from_f<VT>(A(to_f(rhs[r])) - Ax);
 }
 
+// ELL sweep: mode 0 from zero, 1 damped Jacobi
+template <class VT, class CT>
+__global__ void k_ell_smooth(CsrView C, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
+                             const VT* __restrict__ rhs, double omega) {
+    using A = typename Arith<VT>::type;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C.n) return;
+    const A D = A(static_cast<const CT*>(C.diag)[r]);
+    const A br = A(to_f(rhs[r]));
+    if (mode == 0) { xout[r] = from_f<VT>(A(omega) * br / D); return; }
+    const A xr = A(to_f(xin[r]));
+    const A Ax = ell_row<A, CT>(C, r, [&](int c) -> A { return A(to_f(xin[c])); }, xr);
+    xout[r] = from_f<VT>(xr + A(omega) * (br - Ax) / D);
+}
+
+// two sweeps from zero in one pass (x1 = w b / D at the neighbours on the fly)
+template <class VT, class CT>
+__global__ void k_ell_pre2(CsrView C, const VT* __restrict__ rhs, VT* __restrict__ xout, double omega) {
+    using A = typename Arith<VT>::type;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C.n) return;
+    const CT* diag = static_cast<const CT*>(C.diag);
+    const A w = A(omega);
+    auto x1 = [&](int c) -> A { return round_vt<VT>(w * A(to_f(rhs[c])) / A(diag[c])); };
+    const A xr = x1(r);
+    const A Ax = ell_row<A, CT>(C, r, x1, xr);
+    xout[r] = from_f<VT>(xr + w * (A(to_f(rhs[r])) - Ax) / A(diag[r]));
+}
+
+// residual b - A x, one thread per row
+template <class VT, class CT>
+__global__ void k_ell_residual(CsrView C, const VT* __restrict__ x, const VT* __restrict__ rhs, VT* __restrict__ res) {
+    using A = typename Arith<VT>::type;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C.n) return;
+    const A xr = A(to_f(x[r]));
+    const A Ax = ell_row<A, CT>(C, r, [&](int j) -> A { return A(to_f(x[j])); }, xr);
+    res[r] = from_f<VT>(A(to_f(rhs[r])) - Ax);
+}
+
+// restriction from a coarse level (m >= 1): the children of a coarse cell are a contiguous
+// ascending range of the finer level's rows (Morton order), summed in that order
+template <class VT>
+__global__ void k_restrict_range(CsrView Cc, const VT* __restrict__ fine, VT* __restrict__ coarse) {
+    using A = typename Arith<VT>::type;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Cc.n) return;
+    const int k0 = Cc.child_off[c], k1 = Cc.child_off[c + 1], f0 = Cc.child[k0];
+    A s = A(0);
+    for (int k = k0; k < k1; ++k) s += A(to_f(fine[f0 + (k - k0)]));
+    coarse[c] = from_f<VT>(s);
+}
+
+// prolongation from level m+1 fused with one sweep on level m
+template <class VT, class CT>
+__global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT* __restrict__ e, const VT* __restrict__ rhs,
+                                   VT* __restrict__ xout, double omega) {
+    using A = typename Arith<VT>::type;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C.n) return;
+    auto y = [&](int c) -> A { return round_vt<VT>(A(to_f(x[c])) + A(to_f(e[C.parent[c]]))); };
+    const A yr = y(r);
+    const A Ay = ell_row<A, CT>(C, r, y, yr);
+    xout[r] = from_f<VT>(yr + A(omega) * (A(to_f(rhs[r])) - Ay) / A(static_cast<const CT*>(C.diag)[r]));
+}
+
 // coarsest level: x = A^-1 b with the dense inverse, one workgroup per row
 template <class VT, class CT>
 __global__ __launch_bounds__(256) void k_dense_solve(int n, const CT* __restrict__ inv, const VT* __restrict__ rhs,
                                                      VT* __restrict__ x, const DevScalars* S) {
-    if (S->done) return;
     const int r = blockIdx.x;
     double acc = 0.0;
     for (int c = threadIdx.x; c < n; c += 256) acc += double(inv[static_cast<int64_t>(r) * n + c]) * double(to_f(rhs[c]));
@@ -324,16 +538,6 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const CT* __restrict
     if (threadIdx.x == 0) x[r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
 }
 
-// convert an FP64 vector to the V-cycle type (used by the multigrid primitive entry point)
-template <class VT>
-__global__ void k_convert(int64_t n, const double* __restrict__ in, VT* __restrict__ out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = from_f<VT>(static_cast<typename Arith<VT>::type>(in[i]));
-}
-template <class VT>
-__global__ void k_to_double(int64_t n, const VT* __restrict__ in, double* __restrict__ out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = double(to_f(in[i]));
-}
+#undef PCFD_BRICK_PROLOGUE
 
 }  // namespace pcfd

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -124,6 +125,33 @@ private:
         F.fe = fe_;
         return F;
     }
+    // One PCG iteration: d_new = z + beta d_old with A d_new fused, alpha, x/r update,
+    // preconditioner, beta.
+    void cg_iteration(bool initial, double* dold, double* dnew) {
+        LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0);
+        finalize(FIN_APPLY, nb_);
+        const unsigned gu = std::min<unsigned>(blocks_for(n_, 256), 2048);
+        LAUNCH(k_cg_update, gu, 256, n_, p_, r_, rprev_, dnew, Ad_, partials_, S_);
+        finalize(FIN_UPDATE, static_cast<int>(gu));
+        precondition_dispatch(false);
+    }
+    // `chunk_` (even) iterations alternating the two direction buffers
+    void enqueue_chunk() {
+        for (int k = 0; k < chunk_; ++k) cg_iteration(false, dbuf_[(k + 1) & 1], dbuf_[k & 1]);
+    }
+    void capture_chunk() {
+        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_chunk();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+        HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    }
     void finalize(int op, int nparts) {
         LAUNCH(k_finalize, 1, 1024, op, partials_, nparts, S_, opt_.cfl,
                std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations, rprev_ ? 1 : 0);
@@ -172,6 +200,13 @@ private:
     float* inv32_ = nullptr;
     int coarse_n_ = 0;
     hipEvent_t ev_[4] = {};
+    double* dbuf_[2] = {};
+    void* diag_ = nullptr;        // composite-level diagonal in the V-cycle arithmetic type
+    int chunk_ = 2;
+    bool use_graph_ = true;
+    bool fused_ = true;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
 };
 
 template <class R>
@@ -180,6 +215,8 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     if (opt_.check_interval < 1) opt_.check_interval = 1;
     if (!(opt_.rho > 0) || !(opt_.cfl > 0) || !(opt_.rtol > 0) || opt_.max_iterations < 1 || opt_.nu < 0)
         throw std::invalid_argument("invalid solver options");
+    if (!(opt_.smoother_weight > 0.0 && opt_.smoother_weight < 2.0))
+        throw std::invalid_argument("smoother_weight must lie in (0, 2)");
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_s_), sizeof(DevScalars), hipHostMallocDefault));
@@ -234,6 +271,12 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     p_ = alloc<double>(n_); rhs_ = alloc<double>(n_); r_ = alloc<double>(n_); z_ = alloc<double>(n_);
     d_ = alloc<double>(n_); Ad_ = alloc<double>(n_);
     for (double* q : {p_, rhs_, r_, z_, d_, Ad_}) HIP_CHECK(hipMemset(q, 0, n_ * sizeof(double)));
+    dbuf_[0] = d_;
+    dbuf_[1] = alloc<double>(n_);
+    HIP_CHECK(hipMemset(dbuf_[1], 0, n_ * sizeof(double)));
+    chunk_ = std::max(2, (opt_.check_interval + 1) / 2 * 2);
+    if (const char* g = std::getenv("PCFD_NO_GRAPH")) use_graph_ = g[0] == '0';
+    if (const char* f = std::getenv("PCFD_UNFUSED")) fused_ = f[0] == '0';
     if (opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED) {
         rprev_ = alloc<double>(n_);
         HIP_CHECK(hipMemset(rprev_, 0, n_ * sizeof(double)));
@@ -259,11 +302,41 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
             c.parent = m + 1 < T.mg.size() ? upload(G.parent) : nullptr;
             c.child_off = upload(G.child_off);
             c.child = upload(G.child);
+            // SELL-64 copy: per slice of 64 rows, slot-major, as wide as its longest row
+            const int nsl = (G.n + 63) / 64;
+            std::vector<int> sl_off(nsl + 1, 0);
+            for (int s = 0; s < nsl; ++s) {
+                int K = 0;
+                for (int r = 64 * s; r < std::min(G.n, 64 * s + 64); ++r) K = std::max(K, G.row_off[r + 1] - G.row_off[r]);
+                sl_off[s + 1] = sl_off[s] + 64 * K;
+            }
+            std::vector<int> scol(sl_off[nsl]);
+            std::vector<double> scoef(sl_off[nsl], 0.0);
+            for (int r = 0; r < G.n; ++r) {
+                const int s = r >> 6, K = (sl_off[s + 1] - sl_off[s]) / 64;
+                for (int k = 0; k < K; ++k) {
+                    const int e = G.row_off[r] + k;
+                    const size_t slot = static_cast<size_t>(sl_off[s]) + 64 * k + (r & 63);
+                    const bool real = e < G.row_off[r + 1];
+                    scol[slot] = real ? G.col[e] : r;
+                    scoef[slot] = real ? G.coef[e] : 0.0;
+                }
+            }
+            // children of a coarse cell (m >= 2) are one contiguous ascending range
+            if (m >= 2)
+                for (int cc = 0; cc < G.n; ++cc)
+                    for (int k = G.child_off[cc]; k < G.child_off[cc + 1]; ++k)
+                        if (G.child[k] != G.child[G.child_off[cc]] + (k - G.child_off[cc]))
+                            throw std::logic_error("coarse children are not contiguous");
+            c.sl_off = upload(sl_off);
+            c.sl_col = upload(scol);
             L.csr64 = c; L.csr32 = c;
             L.csr64.coef = upload(G.coef);
             L.csr64.diag = upload(G.diag);
+            L.csr64.sl_coef = upload(scoef);
             L.csr32.coef = upload_as<float>(G.coef);
             L.csr32.diag = upload_as<float>(G.diag);
+            L.csr32.sl_coef = upload_as<float>(scoef);
             L.parent = c.parent;
             L.B = alloc<double>(G.n); L.X = alloc<double>(G.n); L.T = alloc<double>(G.n);
         }
@@ -274,6 +347,13 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
             inv32_ = upload_as<float>(K.coarse_inv);
         }
     }
+    if (opt_.vcycle_precision == PCFD_FP64) {
+        diag_ = alloc<double>(n_);
+        LAUNCH(k_diag0<double>, nb_, 512, M_, static_cast<double*>(diag_));
+    } else {
+        diag_ = alloc<float>(n_);
+        LAUNCH(k_diag0<float>, nb_, 512, M_, static_cast<float*>(diag_));
+    }
     HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -281,6 +361,8 @@ template <class R>
 SolverImpl<R>::~SolverImpl() {
     // teardown errors cannot be reported from a destructor; they are ignored deliberately
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : allocations_) (void)hipFree(p);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     if (host_s_) (void)hipHostFree(host_s_);
@@ -304,28 +386,33 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     VT* B = static_cast<VT*>(L.B);
     VT* cur = static_cast<VT*>(L.X);
     VT* oth = static_cast<VT*>(L.T);
-    const double w = 2.0 / 3.0;
+    const double w = opt_.smoother_weight;
     const unsigned g = blocks_for(L.n, 256);
     if (m == static_cast<int>(lv_.size()) - 1) {
         const CT* inv = nullptr;
         if constexpr (std::is_same<CT, double>::value) inv = inv64_; else inv = inv32_;
         if (inv) {
             LAUNCH((k_dense_solve<VT, CT>), L.n, 256, L.n, inv, B, cur, S_);
-        } else {
-            LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 0, cur, cur, B, w, S_);
-            for (int s = 1; s < 64; ++s) { LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 1, cur, oth, B, w, S_); std::swap(cur, oth); }
+        } else {  // no factorisation (pure Neumann): 64 sweeps with weight 2/3
+            LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 0, cur, cur, B, 2.0 / 3.0);
+            for (int s = 1; s < 64; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, 2.0 / 3.0); std::swap(cur, oth); }
         }
         return cur;
     }
-    LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 0, cur, cur, B, w, S_);
-    for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 1, cur, oth, B, w, S_); std::swap(cur, oth); }
-    LAUNCH((k_residual_csr<VT, CT>), g, 256, C, cur, B, oth, S_);
+    if (opt_.pre_smooth == 2) {
+        LAUNCH((k_ell_pre2<VT, CT>), g, 256, C, B, cur, w);
+    } else {
+        LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 0, cur, cur, B, w);
+        for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, w); std::swap(cur, oth); }
+    }
     Level& N = lv_[m + 1];
     const CsrView& NC = std::is_same<CT, double>::value ? N.csr64 : N.csr32;
-    LAUNCH((k_restrict<VT>), blocks_for(N.n, 256), 256, NC, oth, static_cast<VT*>(N.B), S_);
+    LAUNCH((k_ell_residual<VT, CT>), g, 256, C, cur, B, oth);
+    LAUNCH((k_restrict_range<VT>), blocks_for(N.n, 256), 256, NC, oth, static_cast<VT*>(N.B));
     VT* xc = coarse_cycle_t<VT, CT>(m + 1);
-    LAUNCH((k_prolong<VT>), g, 256, static_cast<int64_t>(L.n), L.parent, cur, xc, S_);
-    for (int s = 0; s < opt_.post_smooth; ++s) { LAUNCH((k_smooth_csr<VT, CT>), g, 256, C, 1, cur, oth, B, w, S_); std::swap(cur, oth); }
+    LAUNCH((k_ell_post_prolong<VT, CT>), g, 256, C, cur, xc, B, oth, w);
+    std::swap(cur, oth);
+    for (int s = 1; s < opt_.post_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, w); std::swap(cur, oth); }
     return cur;
 }
 
@@ -347,9 +434,26 @@ void SolverImpl<R>::precondition(bool initial) {
         finalize(fin, nb_);
         return;
     }
-    const double w = 2.0 / 3.0;
+    const double w = opt_.smoother_weight;
     VT* cur = static_cast<VT*>(vc0_[0]);
     VT* oth = static_cast<VT*>(vc0_[1]);
+    if (opt_.pre_smooth == 2 && lv_.size() > 1 && fused_) {
+        // fused composite level: 2 pre-sweeps | residual + restriction | prolongation + sweep
+        using A = typename Arith<VT>::type;
+        LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w, S_);
+        LAUNCH(k_vc_restrict0<VT>, nb_, 512, M_, cur, r_, parent0_, static_cast<VT*>(lv_[1].B), S_);
+        VT* xc = coarse_cycle<VT>(1);
+        const int last = opt_.post_smooth == 1 ? 1 : 0;
+        LAUNCH(k_vc_post_prolong<VT>, nb_, 512, M_, last, cur, xc, parent0_, r_, oth, z_, rprev_, w, partials_, S_);
+        std::swap(cur, oth);
+        for (int s = 1; s < opt_.post_smooth; ++s) {
+            const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
+            LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_);
+            std::swap(cur, oth);
+        }
+        finalize(fin, nb_);
+        return;
+    }
     LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w, partials_, S_);
     for (int s = 1; s < opt_.pre_smooth; ++s) {
         LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w, partials_, S_);
@@ -390,21 +494,19 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
     finalize(FIN_RESID, nb_);
     precondition_dispatch(true);
-    const unsigned gu = std::min<unsigned>(blocks_for(n_, 256), 2048);
-    LAUNCH(k_cg_direction, gu, 256, n_, z_, d_, S_, 1);
-    int enqueued = 0;
-    for (;;) {
-        for (int k = 0; k < opt_.check_interval; ++k) {
-            LAUNCH(k_cg_apply_dot, nb_, 512, M_, d_, Ad_, partials_, S_);
-            finalize(FIN_APPLY, nb_);
-            LAUNCH(k_cg_update, gu, 256, n_, p_, r_, rprev_, d_, Ad_, partials_, S_);
-            finalize(FIN_UPDATE, static_cast<int>(gu));
-            precondition_dispatch(false);
-            LAUNCH(k_cg_direction, gu, 256, n_, z_, d_, S_, 0);
+    // first iteration (beta = 0) outside the replayed chunk; d is double-buffered
+    cg_iteration(true, dbuf_[0], dbuf_[1]);
+    int enqueued = 1;
+    sync_scalars();
+    while (!host_s_->done && enqueued <= opt_.max_iterations + chunk_) {
+        if (use_graph_) {
+            if (!graph_exec_) capture_chunk();
+            HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+        } else {
+            enqueue_chunk();
         }
-        enqueued += opt_.check_interval;
+        enqueued += chunk_;
         sync_scalars();
-        if (host_s_->done || enqueued > opt_.max_iterations + opt_.check_interval) break;
     }
     if (host_s_->zero_rhs) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
     LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
@@ -758,7 +860,8 @@ void pcfd_default_options(pcfd_options* o) {
     std::memset(o, 0, sizeof(*o));
     o->rho = 1.0; o->nu = 1e-3; o->speed = 1.0; o->cfl = 0.5; o->smagorinsky_cs = 0.0; o->rtol = 1e-8;
     o->max_iterations = 200; o->field_precision = PCFD_FP64; o->vcycle_precision = PCFD_FP64;
-    o->preconditioner = PCFD_PC_MG; o->pre_smooth = 2; o->post_smooth = 2; o->check_interval = 1;
+    o->preconditioner = PCFD_PC_MG; o->pre_smooth = 2; o->post_smooth = 2; o->check_interval = 2;
+    o->smoother_weight = 6.0 / 7.0;
 }
 
 pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* options, pcfd_solver** out) {

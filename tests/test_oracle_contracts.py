@@ -25,6 +25,7 @@ def options(**kw):
     o = L.Options()
     o.rho, o.nu, o.speed, o.cfl, o.rtol, o.max_iterations = 1.0, 1e-3, 1.0, 0.5, 1e-8, 300
     o.pre_smooth, o.post_smooth, o.check_interval = 2, 2, 1
+    o.smoother_weight = 6.0 / 7.0
     for k, v in kw.items():
         setattr(o, k, v)
     return o
