@@ -117,10 +117,14 @@ def main():
 
     # roofline of the dominant kernel, timed live with HIP events on the solver's stream
     mean_its = sum(its) / len(its)
+    # launches per step of each timed kernel family: the CG direction+apply pass once per
+    # iteration; four composite-level sweep-type passes per V-cycle (2 pre-sweeps fused,
+    # residual+restriction, prolongation+sweep, last sweep), one V-cycle per iteration plus
+    # the initial one; one transport pass
     per_step = {
-        0: mean_its,                                             # CG apply + dot
-        1: (mean_its + 1) * (opt.pre_smooth - 1 + opt.post_smooth - 1),  # composite-level Jacobi sweeps
-        2: 1.0,                                                  # MUSCL transport
+        0: mean_its,
+        1: (mean_its + 1) * 4,
+        2: 1.0,
     }
     timings = {}
     for k in per_step:

@@ -1,7 +1,7 @@
 // Projection solve kernels: FP64 preconditioned conjugate gradients with device-resident
 // scalars and a device convergence flag, and the geometric V-cycle in FP64 / FP32 / BF16
 // storage (VT).  The composite level uses the brick layout with an LDS tile per brick;
-// coarser levels are ELL (slot-major).
+// coarser levels are SELL-64 (slices of 64 rows, slot-major).
 //
 // The Krylov scalars never leave the device: a one-workgroup finalize kernel after each
 // reduction computes alpha / beta / the relative residual and raises S->done, and every
