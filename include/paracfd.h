@@ -155,6 +155,9 @@ typedef struct pcfd_options {
                                  results do not depend on it */
     double smoother_weight;   /* damped-Jacobi weight of the V-cycle sweeps (default 6/7;
                                  the reference's algebraic MG uses 2/3, amr_pressure.cu:1103) */
+    int learned_bricks;       /* PCFD_PC_LEARNED: bricks the network corrects, 0 all bricks,
+                                 1 bricks holding special cells and their face neighbours */
+    int learned_capacity;     /* PCFD_PC_LEARNED: stored (residual, error) training pairs */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {
@@ -258,6 +261,48 @@ pcfd_status pcfd_mg_primitive(pcfd_solver* solver, int op, int level, const doub
  *         2 MUSCL transport, 3 one full V-cycle (bytes reported as 0).
  * Leaves the Krylov loop flags in the "finished" state; call between steps. */
 pcfd_status pcfd_time_kernel(pcfd_solver* solver, int kernel, int reps, double* avg_ms, double* bytes_per_launch);
+
+/* ---- Learned preconditioner (PCFD_PC_LEARNED) --------------------------------------------
+ * No counterpart in the reference, whose projection runs plain (GCRO / PCG) Krylov with an
+ * algebraic V-cycle (amr_pressure.cu:1103-1353).  A 4-layer 3^3 conv net per brick (16
+ * channels, BF16 MFMA, FP32 accumulate) corrects the V-cycle inside flexible PCG:
+ * z = M_mg(r) + (s_b/h_b) N(r'/s_b) with r' = r - A M_mg(r) and s_b the brick's RMS of r'.
+ * It is trained to minimise the energy norm of the error it leaves, dz^T A dz - 2 dz^T r'.
+ * A projection whose learned solve stalls or fails restarts from its iterate with plain
+ * MGPCG (pcfd_step_stats.fallback_used).  The network starts disabled; it is enabled by
+ * pcfd_learned_train or pcfd_learned_set_weights. */
+#define PCFD_NN_WEIGHTS 27648 /* 4 layers x 27 taps x 16 cout x 16 cin, [layer][tap][cout][cin] */
+#define PCFD_NN_BIASES 64     /* 4 layers x 16 */
+
+typedef struct pcfd_learned_stats {
+    int pairs;                /* stored training pairs */
+    int train_steps;          /* Adam steps taken so far */
+    double loss_first;        /* mean loss of the first / last step of the last train call */
+    double loss_last;
+    double collect_ms, train_ms;
+} pcfd_learned_stats;
+
+/* Advance a copy of the current state by one transport step, solve its projection with plain
+ * MGPCG from zero, and store the post-V-cycle residuals r'_k = r_k - A M_mg(r_k) of its first
+ * `max_pairs` iterations (with their normalisation sum r'^2/h).  The solver state is left
+ * unchanged.  Residuals beyond learned_capacity replace the oldest. */
+pcfd_status pcfd_learned_collect(pcfd_solver* solver, int max_pairs, pcfd_learned_stats* stats);
+/* `steps` Adam steps (lr, betas 0.9/0.999), each over `batch` stored residuals taken in turn
+ * (loss: energy of the remaining error divided by sum r'^2/h, averaged over the batch);
+ * enables the network. */
+pcfd_status pcfd_learned_train(pcfd_solver* solver, int steps, int batch, double learning_rate,
+                               pcfd_learned_stats* stats);
+/* FP32 master weights [PCFD_NN_WEIGHTS] and biases [PCFD_NN_BIASES]; set enables the network. */
+pcfd_status pcfd_learned_get_weights(pcfd_solver* solver, float* weights, float* biases);
+pcfd_status pcfd_learned_set_weights(pcfd_solver* solver, const float* weights, const float* biases);
+/* 1 enables, 0 disables the network (the solver then runs plain MGPCG in flexible PCG). */
+pcfd_status pcfd_learned_enable(pcfd_solver* solver, int on);
+/* Verification hooks: raw network output y[dofs] and brick scales s[bricks] for an input
+ * residual r (every brick); loss and gradient (weights then biases) of one post-V-cycle
+ * residual over the network's bricks with the training kernels, without an update. */
+pcfd_status pcfd_learned_forward(pcfd_solver* solver, const double* r, float* y, float* s);
+pcfd_status pcfd_learned_gradient(pcfd_solver* solver, const double* residual, double* loss, float* grad_weights,
+                                  float* grad_biases);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,19 @@ class Options(C.Structure):
                 ("smagorinsky_cs", C.c_double), ("rtol", C.c_double), ("max_iterations", C.c_int),
                 ("field_precision", C.c_int), ("vcycle_precision", C.c_int), ("preconditioner", C.c_int),
                 ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int),
-                ("smoother_weight", C.c_double)]
+                ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int)]
+
+
+NN_WEIGHTS = 4 * 27 * 16 * 16  # [layer][tap][cout][cin]
+NN_BIASES = 4 * 16
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("pairs", C.c_int), ("train_steps", C.c_int), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("collect_ms", C.c_double), ("train_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class StepStats(C.Structure):
@@ -89,7 +101,9 @@ EXPORTED = [
     "pcfd_initialize", "pcfd_step", "pcfd_advect", "pcfd_project", "pcfd_apply_pressure_impulse",
     "pcfd_integrate_loads", "pcfd_set_reference", "pcfd_get_velocity", "pcfd_set_velocity", "pcfd_get_pressure",
     "pcfd_set_pressure", "pcfd_get_faces", "pcfd_set_faces", "pcfd_mark_initialized", "pcfd_device_bytes",
-    "pcfd_mg_primitive", "pcfd_time_kernel",
+    "pcfd_mg_primitive", "pcfd_time_kernel", "pcfd_learned_collect", "pcfd_learned_train",
+    "pcfd_learned_get_weights", "pcfd_learned_set_weights", "pcfd_learned_enable", "pcfd_learned_forward",
+    "pcfd_learned_gradient",
 ]
 
 _lib = None
@@ -105,6 +119,7 @@ def load():
     lib = C.CDLL(LIB_PATH)
     P = C.c_void_p
     dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
     sig = {
         "pcfd_last_error": (C.c_char_p, []),
         "pcfd_abi_version": (C.c_int, []),
@@ -133,6 +148,13 @@ def load():
         "pcfd_device_bytes": (C.c_int64, [P]),
         "pcfd_mg_primitive": (C.c_int, [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int]),
         "pcfd_time_kernel": (C.c_int, [P, C.c_int, C.c_int, dp, dp]),
+        "pcfd_learned_collect": (C.c_int, [P, C.c_int, C.POINTER(LearnedStats)]),
+        "pcfd_learned_train": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.POINTER(LearnedStats)]),
+        "pcfd_learned_get_weights": (C.c_int, [P, fp, fp]),
+        "pcfd_learned_set_weights": (C.c_int, [P, fp, fp]),
+        "pcfd_learned_enable": (C.c_int, [P, C.c_int]),
+        "pcfd_learned_forward": (C.c_int, [P, dp, fp, fp]),
+        "pcfd_learned_gradient": (C.c_int, [P, dp, dp, fp, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -159,6 +181,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 class Topology:

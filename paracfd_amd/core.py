@@ -132,3 +132,49 @@ class ExternalAeroCore:
         L.check(self._lib.pcfd_mg_primitive(self.handle, op, level, L.dptr(xa), L.dptr(ba), L.dptr(out), omega,
                                             sweeps))
         return out
+
+    # learned preconditioner (options.preconditioner == PC_LEARNED); no reference counterpart
+    def learned_collect(self, max_pairs=8) -> dict:
+        """Store post-V-cycle residuals from a plain-MGPCG solve of the next step's projection."""
+        st = L.LearnedStats()
+        L.check(self._lib.pcfd_learned_collect(self.handle, max_pairs, C.byref(st)))
+        return st.as_dict()
+
+    def learned_train(self, steps, batch=1, lr=1e-3) -> dict:
+        st = L.LearnedStats()
+        L.check(self._lib.pcfd_learned_train(self.handle, steps, batch, lr, C.byref(st)))
+        return st.as_dict()
+
+    def learned_weights(self):
+        """FP32 master weights [4, 27, 16, 16] ([layer][tap][cout][cin]) and biases [4, 16]."""
+        w = np.empty(L.NN_WEIGHTS, dtype=np.float32)
+        b = np.empty(L.NN_BIASES, dtype=np.float32)
+        L.check(self._lib.pcfd_learned_get_weights(self.handle, L.fptr(w), L.fptr(b)))
+        return w.reshape(4, 27, 16, 16), b.reshape(4, 16)
+
+    def set_learned_weights(self, w, b):
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        b = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+        if w.size != L.NN_WEIGHTS or b.size != L.NN_BIASES:
+            raise ValueError("weights must hold 4*27*16*16 values and biases 4*16")
+        L.check(self._lib.pcfd_learned_set_weights(self.handle, L.fptr(w), L.fptr(b)))
+
+    def learned_enable(self, on=True):
+        L.check(self._lib.pcfd_learned_enable(self.handle, 1 if on else 0))
+
+    def learned_forward(self, r):
+        """Raw network output y (per dof, float32) and per-brick residual scales s."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        y = np.empty(self.n, dtype=np.float32)
+        s = np.empty(self.n // 512, dtype=np.float32)
+        L.check(self._lib.pcfd_learned_forward(self.handle, L.dptr(r), L.fptr(y), L.fptr(s)))
+        return y, s
+
+    def learned_gradient(self, residual):
+        """(loss, dW [4,27,16,16], db [4,16]) of one post-V-cycle residual with the training kernels."""
+        r = np.ascontiguousarray(residual, dtype=np.float64)
+        loss = C.c_double()
+        gw = np.empty(L.NN_WEIGHTS, dtype=np.float32)
+        gb = np.empty(L.NN_BIASES, dtype=np.float32)
+        L.check(self._lib.pcfd_learned_gradient(self.handle, L.dptr(r), C.byref(loss), L.fptr(gw), L.fptr(gb)))
+        return loss.value, gw.reshape(4, 27, 16, 16), gb.reshape(4, 16)

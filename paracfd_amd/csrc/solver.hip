@@ -21,6 +21,7 @@
 #include "device.hpp"
 #include "kernels_solve.hpp"
 #include "kernels_step.hpp"
+#include "learned.hpp"
 #include "topology.hpp"
 
 const pcfd::Topology& pcfd_topology_ref(const pcfd_topology* t);
@@ -64,6 +65,12 @@ public:
     virtual void set_faces(const double* fx, const double* fy, const double* fz, const double* fe) = 0;
     virtual void mg_primitive(int op, int level, const double* x, const double* b, double* out, double omega, int sweeps) = 0;
     virtual void time_kernel(int kernel, int reps, double* avg_ms, double* bytes) = 0;
+    virtual void learned_collect(int max_pairs, pcfd_learned_stats* st) = 0;
+    virtual void learned_train(int steps, int batch, double lr, pcfd_learned_stats* st) = 0;
+    virtual void learned_weights(float* w, float* b, const float* wset, const float* bset) = 0;
+    virtual void learned_enable(bool on) = 0;
+    virtual void learned_forward(const double* r, float* y, float* s) = 0;
+    virtual void learned_gradient(const double* r, double* loss, float* gw, float* gb) = 0;
     bool initialized = false;
     double time = 0.0;
     double ref_area = 0.0, origin[3] = {0, 0, 0};
@@ -89,6 +96,12 @@ public:
     void set_faces(const double* fx, const double* fy, const double* fz, const double* fe) override;
     void mg_primitive(int op, int level, const double* x, const double* b, double* out, double omega, int sweeps) override;
     void time_kernel(int kernel, int reps, double* avg_ms, double* bytes) override;
+    void learned_collect(int max_pairs, pcfd_learned_stats* st) override;
+    void learned_train(int steps, int batch, double lr, pcfd_learned_stats* st) override;
+    void learned_weights(float* w, float* b, const float* wset, const float* bset) override;
+    void learned_enable(bool on) override;
+    void learned_forward(const double* r, float* y, float* s) override;
+    void learned_gradient(const double* r, double* loss, float* gw, float* gb) override;
 
 private:
     template <class T>
@@ -156,6 +169,36 @@ private:
         LAUNCH(k_finalize, 1, 1024, op, partials_, nparts, S_, opt_.cfl,
                std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations, rprev_ ? 1 : 0);
     }
+    // learned preconditioner
+    NNView nnv() const { return NNView{nn_w_, nn_wt_, nn_p_ + kNNW}; }
+    void set_nn_off(bool off) { if (nn_) HIP_CHECK(hipMemsetAsync(nn_off_, off ? 1 : 0, sizeof(int), stream_)); }
+    unsigned nn_grid() const { return static_cast<unsigned>(nn_list_ ? nn_nlist_ : nb_); }
+    // network correction after the V-cycle's last kernel, then the r.z finalisation
+    void pc_tail(int fin) {
+        if (nn_) {
+            LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, nn_slot_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
+            LAUNCH(k_nn_add, nn_grid(), 512, nn_list_, nn_dz_, z_, S_, nn_off_);
+        }
+        finalize(fin, nb_);
+    }
+    // sum_active v^2 / h_b of a device vector (normalisation of a recorded residual)
+    double nn_norm(const double* v) {
+        LAUNCH(k_nn_norm, nb_, 512, M_, v, partials_);
+        std::vector<double> h(nb_);
+        HIP_CHECK(hipMemcpyAsync(h.data(), partials_, nb_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        double s = 0.0;
+        for (double x : h) s += x;
+        return s;
+    }
+    // one training pass (forward, energy gradient, backward) on residual rp into nn_g_ / nn_loss_
+    void nn_train_pass(const double* rp, double inv_norm) {
+        LAUNCH(k_nn_train_fwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_scratch_, nn_dfull_, nn_sb_);
+        LAUNCH(k_nn_energy, nn_grid(), 512, M_, nn_list_, nn_dfull_, rp, nn_gy_, nn_loss_, inv_norm);
+        LAUNCH(k_nn_train_bwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_gy_, nn_sb_, nn_scratch_, nn_g_);
+    }
+    void setup_learned(const Topology& T);
+    bool pcg_loop(bool watch_stall);
     void reconstruct();
     void run_projection(bool warm, pcfd_step_stats* st);
     template <class VT> void precondition(bool initial);
@@ -207,6 +250,33 @@ private:
     bool fused_ = true;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
+    // learned preconditioner state (PCFD_PC_LEARNED)
+    bool nn_ = false;             // created with the learned preconditioner
+    bool nn_on_ = false;          // network enabled (trained or weights set)
+    uint16_t* nn_w_ = nullptr;    // BF16 weights [L][27][cout][cin]
+    uint16_t* nn_wt_ = nullptr;   // BF16 flipped/transposed copy
+    float* nn_p_ = nullptr;       // FP32 master [weights | biases]
+    float* nn_m_ = nullptr;       // Adam moments
+    float* nn_v_ = nullptr;
+    float* nn_g_ = nullptr;       // gradient accumulator
+    int* nn_off_ = nullptr;       // device flag read by k_nn_apply
+    double* nn_loss_ = nullptr;
+    uint16_t* nn_scratch_ = nullptr;  // layer inputs per listed brick (training)
+    int* nn_list_ = nullptr;      // the network's bricks (null: all bricks, XCD-aware order)
+    int* nn_slot_ = nullptr;      // V-cycle partial slot of each listed brick
+    int nn_nlist_ = 0;
+    int64_t nn_cells_ = 0;        // active cells of the network's bricks
+    double* nn_dz_ = nullptr;     // correction per listed brick (512 per position)
+    double* nn_dfull_ = nullptr;  // training: correction as a full-length vector
+    double* nn_gy_ = nullptr;     // training: output gradient per listed brick
+    float* nn_sb_ = nullptr;      // training: brick scales
+    std::vector<double*> nn_rs_;  // recorded post-V-cycle residuals
+    std::vector<double> nn_inv_norm_;
+    int nn_cap_ = 0, nn_count_ = 0, nn_next_ = 0, nn_cursor_ = 0, nn_t_ = 0;
+    R* snap_v_[3] = {};
+    R* snap_f_[3] = {};
+    R* snap_fe_ = nullptr;
+    double* snap_p_ = nullptr;
 };
 
 template <class R>
@@ -354,7 +424,87 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
         diag_ = alloc<float>(n_);
         LAUNCH(k_diag0<float>, nb_, 512, M_, static_cast<float*>(diag_));
     }
+    if (opt_.preconditioner == PCFD_PC_LEARNED) setup_learned(T);
     HIP_CHECK(hipDeviceSynchronize());
+}
+
+// Network buffers, brick selection and the initial weights: He-normal (seed 1234) for
+// layers 0..2, zero for the output layer, so an untrained network adds nothing.
+template <class R>
+void SolverImpl<R>::setup_learned(const Topology& T) {
+    if (lv_.empty()) throw std::invalid_argument("the learned preconditioner needs a multigrid hierarchy");
+    if (opt_.learned_bricks < 0 || opt_.learned_bricks > 1) throw std::invalid_argument("learned_bricks must be 0 or 1");
+    if (opt_.learned_capacity < 0) throw std::invalid_argument("learned_capacity must be non-negative");
+    nn_ = true;
+    nn_cap_ = opt_.learned_capacity > 0 ? opt_.learned_capacity : 16;
+    std::vector<float> p(kNNW + kNNB, 0.f);
+    uint64_t state = 1234;
+    auto next = [&state] {
+        uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    auto uniform = [&] { return (static_cast<double>(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+    for (int l = 0; l + 1 < kNNLayers; ++l) {
+        const int fan_in = kNNTaps * (l == 0 ? kNNIn : kNNC);
+        const double sd = std::sqrt(2.0 / fan_in);
+        for (int t = 0; t < kNNTaps; ++t)
+            for (int o = 0; o < kNNC; ++o)
+                for (int i = 0; i < (l == 0 ? kNNIn : kNNC); ++i) {
+                    const double g = std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform());
+                    p[static_cast<size_t>(l) * kNNWLayer + (t * kNNC + o) * kNNC + i] = static_cast<float>(sd * g);
+                }
+    }
+    nn_p_ = upload(p);
+    nn_w_ = alloc<uint16_t>(kNNW);
+    nn_wt_ = alloc<uint16_t>(kNNW);
+    LAUNCH(k_nn_pack, blocks_for(kNNW, 256), 256, nn_p_, nn_w_, nn_wt_);
+    nn_m_ = alloc<float>(kNNW + kNNB);
+    nn_v_ = alloc<float>(kNNW + kNNB);
+    nn_g_ = alloc<float>(kNNW + kNNB);
+    for (float* q : {nn_m_, nn_v_, nn_g_}) HIP_CHECK(hipMemset(q, 0, (kNNW + kNNB) * sizeof(float)));
+    nn_off_ = alloc<int>(1);
+    HIP_CHECK(hipMemset(nn_off_, 1, sizeof(int)));
+    nn_loss_ = alloc<double>(1);
+    // bricks the network corrects
+    std::vector<int> list;
+    if (opt_.learned_bricks == 1) {
+        std::vector<uint8_t> sel(nb_, 0);
+        for (int b = 0; b < nb_; ++b) {
+            if (T.brick_special_off[b + 1] == T.brick_special_off[b]) continue;
+            sel[b] = 1;
+            for (int f = 0; f < 6; ++f) if (T.brick_nbr[6 * b + f] >= 0) sel[T.brick_nbr[6 * b + f]] = 1;
+        }
+        for (int b = 0; b < nb_; ++b) if (sel[b]) list.push_back(b);
+        if (list.empty()) throw std::invalid_argument("learned_bricks = 1 selects no brick (no special cells)");
+        // block index of brick b in a brick-per-block launch of nb_ blocks (inverse of the
+        // XCD-aware order of brick_of_block), where the V-cycle's last kernel put its partials
+        const int per = nb_ >> 3, rem = nb_ & 7;
+        std::vector<int> slot;
+        for (int b : list) {
+            int x = 7;
+            while (x * per + std::min(x, rem) > b) --x;
+            slot.push_back((b - (x * per + std::min(x, rem))) * 8 + x);
+        }
+        nn_list_ = upload(list);
+        nn_slot_ = upload(slot);
+    } else {
+        for (int b = 0; b < nb_; ++b) list.push_back(b);
+    }
+    nn_nlist_ = static_cast<int>(list.size());
+    nn_cells_ = 0;
+    for (int b : list)
+        for (int q = 0; q < 512; ++q) nn_cells_ += (T.mask[static_cast<size_t>(b) * 512 + q] & kActive) ? 1 : 0;
+    if (nn_cells_ == 0) throw std::invalid_argument("the learned preconditioner's bricks hold no active cell");
+    const int64_t slots = static_cast<int64_t>(nn_list_ ? nn_nlist_ : nb_) * 512;
+    nn_scratch_ = alloc<uint16_t>(static_cast<int64_t>(nn_nlist_) * kNNStore);
+    nn_dz_ = alloc<double>(slots);
+    nn_gy_ = alloc<double>(slots);
+    nn_dfull_ = alloc<double>(n_);
+    nn_sb_ = alloc<float>(nn_nlist_);
+    HIP_CHECK(hipMemset(nn_dz_, 0, slots * sizeof(double)));
+    HIP_CHECK(hipMemset(nn_dfull_, 0, n_ * sizeof(double)));
 }
 
 template <class R>
@@ -451,7 +601,7 @@ void SolverImpl<R>::precondition(bool initial) {
             LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_);
             std::swap(cur, oth);
         }
-        finalize(fin, nb_);
+        pc_tail(fin);
         return;
     }
     LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w, partials_, S_);
@@ -471,7 +621,7 @@ void SolverImpl<R>::precondition(bool initial) {
         LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_);
         std::swap(cur, oth);
     }
-    finalize(fin, nb_);
+    pc_tail(fin);
 }
 
 template <class R>
@@ -486,11 +636,11 @@ void SolverImpl<R>::precondition_dispatch(bool initial) {
 // ------------------------------------------------------------------------------------------
 // projection: RHS, PCG, correction, divergence check  (amr_pressure.cu:2085-2095)
 // ------------------------------------------------------------------------------------------
+// PCG from the current p_ on the RHS in rhs_ (FIN_BNORM already issued).  With
+// `watch_stall`, returns false as soon as the relative residual read at a check has not
+// halved over the last three checks (or is not finite); true otherwise.
 template <class R>
-void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
-    LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
-    finalize(FIN_BNORM, nb_);
-    if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+bool SolverImpl<R>::pcg_loop(bool watch_stall) {
     LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
     finalize(FIN_RESID, nb_);
     precondition_dispatch(true);
@@ -498,6 +648,7 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     cg_iteration(true, dbuf_[0], dbuf_[1]);
     int enqueued = 1;
     sync_scalars();
+    std::vector<double> hist{host_s_->relres};
     while (!host_s_->done && enqueued <= opt_.max_iterations + chunk_) {
         if (use_graph_) {
             if (!graph_exec_) capture_chunk();
@@ -507,6 +658,36 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
         }
         enqueued += chunk_;
         sync_scalars();
+        if (watch_stall && !host_s_->done) {
+            const double rr = host_s_->relres;
+            if (!std::isfinite(rr)) return false;
+            hist.push_back(rr);
+            const size_t k = hist.size();
+            if (k >= 4 && rr > 0.5 * hist[k - 4]) return false;
+        }
+    }
+    return true;
+}
+
+template <class R>
+void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
+    LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+    finalize(FIN_BNORM, nb_);
+    if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    const bool learned = nn_ && nn_on_;
+    set_nn_off(!learned);
+    pcg_loop(learned);
+    int iterations = host_s_->iter;
+    bool fell_back = false;
+    if (learned && !host_s_->converged) {
+        // learned solve stalled or failed: plain MGPCG from its iterate on the same RHS
+        fell_back = true;
+        set_nn_off(true);
+        if (!std::isfinite(host_s_->relres)) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+        LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+        finalize(FIN_BNORM, nb_);
+        pcg_loop(false);
+        iterations += host_s_->iter;
     }
     if (host_s_->zero_rhs) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
     LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
@@ -515,10 +696,11 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     finalize(FIN_MAXDIV, nb_);
     sync_scalars();
     if (st) {
-        st->iterations = host_s_->iter;
+        st->iterations = iterations;
         st->converged = host_s_->converged;
         st->relative_residual = host_s_->relres;
         st->max_abs_divergence = host_s_->maxdiv;
+        st->fallback_used = fell_back ? 1 : 0;
     }
 }
 
@@ -620,6 +802,7 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     local.converged = ps.converged;
     local.relative_residual = ps.relative_residual;
     local.max_abs_divergence = ps.max_abs_divergence;
+    local.fallback_used = ps.fallback_used;
     local.advect_ms = a_ms;
     local.project_ms = p_ms;
     local.step_ms = s_ms;
@@ -785,9 +968,16 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
                 break;
             case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); break;
             case 3: precondition_dispatch(true); break;
+            case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, nn_slot_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
             default: throw std::invalid_argument("unknown kernel id");
         }
     };
+    if (kernel == 4) {
+        if (!nn_) throw std::invalid_argument("kernel 4 needs the learned preconditioner");
+        // a residual with every brick non-zero so that each selected brick runs the network
+        LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+        set_nn_off(false);
+    }
     launch();  // warm
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
     for (int i = 0; i < reps; ++i) launch();
@@ -802,11 +992,205 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         case 1: per = vt + 8 + 1 + vt; break;                  // x in, r in, mask, x out
         case 2: per = 9.0 * sizeof(R) + 1; break;             // v in (3), faces (3), mask, v out (3)
         case 3: per = 0; break;
+        case 4: per = 8 + 1 + 8 + 8 + 8; break;               // r, mask, z, r_prev, dz
+    }
+    if (kernel == 4) {
+        nact = nn_cells_;
+        set_nn_off(!nn_on_);
     }
     *bytes = per * static_cast<double>(nact);
     // leave the loop flags as a finished solve so a stray kernel does nothing
     host_s_->done = 1;
     HIP_CHECK(hipMemcpy(&S_->done, &host_s_->done, sizeof(int), hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------------------------------------
+// learned preconditioner: data, training, hooks
+// ------------------------------------------------------------------------------------------
+template <class R>
+void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
+    if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
+    if (!initialized) throw std::logic_error("collect before a converged initialization");
+    if (max_pairs < 1) throw std::invalid_argument("max_pairs must be positive");
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
+    if (!snap_p_) {
+        for (int c = 0; c < 3; ++c) { snap_v_[c] = alloc<R>(n_); snap_f_[c] = alloc<R>(n_); }
+        snap_fe_ = alloc<R>(nedges_);
+        snap_p_ = alloc<double>(n_);
+    }
+    // snapshot the state: the collection advances a copy and solves its projection
+    for (int c = 0; c < 3; ++c) {
+        HIP_CHECK(hipMemcpyAsync(snap_v_[c], v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(snap_f_[c], f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+    }
+    if (nedges_) HIP_CHECK(hipMemcpyAsync(snap_fe_, fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(snap_p_, p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    auto restore = [&] {
+        for (int c = 0; c < 3; ++c) {
+            HIP_CHECK(hipMemcpyAsync(v_[c], snap_v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+            HIP_CHECK(hipMemcpyAsync(f_[c], snap_f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+        }
+        if (nedges_) HIP_CHECK(hipMemcpyAsync(fe_, snap_fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(p_, snap_p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        set_nn_off(!nn_on_);
+    };
+    int recorded = 0;
+    try {
+        double dt = 0.0;
+        advect(0.0, &dt, nullptr);
+        LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+        finalize(FIN_BNORM, nb_);
+        HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+        set_nn_off(true);
+        LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+        finalize(FIN_RESID, nb_);
+        precondition_dispatch(true);
+        sync_scalars();
+        // r' = r - A M_mg(r) of the current iterate; recorded while the solve runs
+        auto record = [&] {
+            if (host_s_->done || recorded >= max_pairs) return;
+            const int slot = nn_next_;
+            if (slot >= static_cast<int>(nn_rs_.size())) {
+                nn_rs_.push_back(alloc<double>(n_));
+                nn_inv_norm_.push_back(0.0);
+            }
+            double* rs = nn_rs_[slot];
+            LAUNCH(k_cg_residual, nb_, 512, M_, z_, r_, rs, partials_, S_);
+            const double norm = nn_norm(rs);
+            if (!(norm > 0.0) || !std::isfinite(norm)) return;
+            nn_inv_norm_[slot] = 1.0 / norm;
+            nn_next_ = (nn_next_ + 1) % nn_cap_;
+            nn_count_ = std::min(nn_count_ + 1, nn_cap_);
+            ++recorded;
+        };
+        record();
+        cg_iteration(true, dbuf_[0], dbuf_[1]);
+        sync_scalars();
+        record();
+        for (int it = 1; !host_s_->done && recorded < max_pairs && it <= opt_.max_iterations; ++it) {
+            cg_iteration(false, dbuf_[it & 1], dbuf_[(it + 1) & 1]);
+            sync_scalars();
+            record();
+        }
+    } catch (...) {
+        try { restore(); } catch (...) {}
+        throw;
+    }
+    restore();
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
+    HIP_CHECK(hipEventSynchronize(ev_[1]));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        st->pairs = nn_count_;
+        st->train_steps = nn_t_;
+        st->collect_ms = ms;
+    }
+}
+
+template <class R>
+void SolverImpl<R>::learned_train(int steps, int batch, double lr, pcfd_learned_stats* st) {
+    if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
+    if (steps < 1 || batch < 1 || !(lr > 0) || !std::isfinite(lr)) throw std::invalid_argument("steps, batch and learning rate must be positive");
+    if (nn_count_ == 0) throw std::logic_error("no training pairs (pcfd_learned_collect first)");
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
+    double first = 0.0, last = 0.0;
+    for (int s = 0; s < steps; ++s) {
+        HIP_CHECK(hipMemsetAsync(nn_loss_, 0, sizeof(double), stream_));
+        for (int j = 0; j < batch; ++j) {
+            const int idx = nn_cursor_ % nn_count_;
+            nn_cursor_ = (nn_cursor_ + 1) % nn_count_;
+            nn_train_pass(nn_rs_[idx], nn_inv_norm_[idx] / batch);
+        }
+        ++nn_t_;
+        const float c1 = static_cast<float>(1.0 - std::pow(0.9, nn_t_));
+        const float c2 = static_cast<float>(1.0 - std::pow(0.999, nn_t_));
+        LAUNCH(k_nn_adam, blocks_for(kNNW + kNNB, 256), 256, nn_p_, nn_m_, nn_v_, nn_g_, static_cast<float>(lr), 0.9f, 0.999f, 1e-8f, c1, c2);
+        LAUNCH(k_nn_pack, blocks_for(kNNW, 256), 256, nn_p_, nn_w_, nn_wt_);
+        if (s == 0 || s == steps - 1) {
+            double l = 0.0;
+            HIP_CHECK(hipMemcpyAsync(&l, nn_loss_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIP_CHECK(hipStreamSynchronize(stream_));
+            if (!std::isfinite(l)) throw std::domain_error("non-finite training loss");
+            if (s == 0) first = l;
+            last = l;
+        }
+    }
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
+    HIP_CHECK(hipEventSynchronize(ev_[1]));
+    nn_on_ = true;
+    set_nn_off(false);
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        st->pairs = nn_count_;
+        st->train_steps = nn_t_;
+        st->loss_first = first;
+        st->loss_last = last;
+        st->train_ms = ms;
+    }
+}
+
+template <class R>
+void SolverImpl<R>::learned_weights(float* w, float* b, const float* wset, const float* bset) {
+    if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    if (wset || bset) {
+        if (!(wset && bset)) throw std::invalid_argument("weights and biases are set together");
+        for (int i = 0; i < kNNW; ++i) if (!std::isfinite(wset[i])) throw std::invalid_argument("non-finite weight");
+        for (int i = 0; i < kNNB; ++i) if (!std::isfinite(bset[i])) throw std::invalid_argument("non-finite bias");
+        HIP_CHECK(hipMemcpy(nn_p_, wset, kNNW * sizeof(float), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(nn_p_ + kNNW, bset, kNNB * sizeof(float), hipMemcpyHostToDevice));
+        LAUNCH(k_nn_pack, blocks_for(kNNW, 256), 256, nn_p_, nn_w_, nn_wt_);
+        nn_on_ = true;
+        set_nn_off(false);
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        return;
+    }
+    if (w) HIP_CHECK(hipMemcpy(w, nn_p_, kNNW * sizeof(float), hipMemcpyDeviceToHost));
+    if (b) HIP_CHECK(hipMemcpy(b, nn_p_ + kNNW, kNNB * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+template <class R>
+void SolverImpl<R>::learned_enable(bool on) {
+    if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
+    nn_on_ = on;
+    set_nn_off(!on);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <class R>
+void SolverImpl<R>::learned_forward(const double* r, float* y, float* s) {
+    if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    double* dr = static_cast<double*>(vc0_[0]);
+    float* dy = static_cast<float*>(vc0_[1]);
+    float* ds = reinterpret_cast<float*>(partials_);
+    HIP_CHECK(hipMemcpy(dr, r, n_ * sizeof(double), hipMemcpyHostToDevice));
+    LAUNCH(k_nn_forward_raw, nb_, 512, M_, nnv(), dr, dy, ds);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    HIP_CHECK(hipMemcpy(y, dy, n_ * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(s, ds, nb_ * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+template <class R>
+void SolverImpl<R>::learned_gradient(const double* r, double* loss, float* gw, float* gb) {
+    if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    double* dr = static_cast<double*>(vc0_[0]);
+    HIP_CHECK(hipMemcpy(dr, r, n_ * sizeof(double), hipMemcpyHostToDevice));
+    const double norm = nn_norm(dr);
+    if (!(norm > 0.0) || !std::isfinite(norm)) throw std::invalid_argument("residual must be non-zero and finite");
+    HIP_CHECK(hipMemset(nn_g_, 0, (kNNW + kNNB) * sizeof(float)));
+    HIP_CHECK(hipMemset(nn_loss_, 0, sizeof(double)));
+    nn_train_pass(dr, 1.0 / norm);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    HIP_CHECK(hipMemcpy(loss, nn_loss_, sizeof(double), hipMemcpyDeviceToHost));
+    if (gw) HIP_CHECK(hipMemcpy(gw, nn_g_, kNNW * sizeof(float), hipMemcpyDeviceToHost));
+    if (gb) HIP_CHECK(hipMemcpy(gb, nn_g_ + kNNW, kNNB * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemset(nn_g_, 0, (kNNW + kNNB) * sizeof(float)));
 }
 
 }  // namespace pcfd
@@ -862,6 +1246,8 @@ void pcfd_default_options(pcfd_options* o) {
     o->max_iterations = 200; o->field_precision = PCFD_FP64; o->vcycle_precision = PCFD_FP64;
     o->preconditioner = PCFD_PC_MG; o->pre_smooth = 2; o->post_smooth = 2; o->check_interval = 2;
     o->smoother_weight = 6.0 / 7.0;
+    o->learned_bricks = 0;
+    o->learned_capacity = 16;
 }
 
 pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* options, pcfd_solver** out) {
@@ -871,7 +1257,8 @@ pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* optio
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw pcfd::DeviceError("no HIP device");
         if (options->smagorinsky_cs != 0.0) throw std::invalid_argument("the Smagorinsky model is not enabled in this build (Cs must be 0)");
-        if (options->preconditioner == PCFD_PC_LEARNED) throw std::invalid_argument("learned preconditioner is not available in this build");
+        if (options->preconditioner < PCFD_PC_MG || options->preconditioner > PCFD_PC_LEARNED)
+            throw std::invalid_argument("unknown preconditioner");
         auto s = std::make_unique<pcfd_solver>();
         const pcfd::Topology& T = pcfd_topology_ref(topology);
         if (options->field_precision == PCFD_FP32) s->impl = std::make_unique<pcfd::SolverImpl<float>>(T, *options);
@@ -958,6 +1345,35 @@ pcfd_status pcfd_mg_primitive(pcfd_solver* s, int op, int level, const double* x
 pcfd_status pcfd_time_kernel(pcfd_solver* s, int kernel, int reps, double* avg_ms, double* bytes_per_launch) {
     NEED(s && avg_ms && bytes_per_launch, "pcfd_time_kernel: null argument");
     return guarded("pcfd_time_kernel", [&] { s->impl->time_kernel(kernel, reps, avg_ms, bytes_per_launch); });
+}
+pcfd_status pcfd_learned_collect(pcfd_solver* s, int max_pairs, pcfd_learned_stats* st) {
+    NEED(s, "pcfd_learned_collect: null solver");
+    return guarded("pcfd_learned_collect", [&] { s->impl->learned_collect(max_pairs, st); });
+}
+pcfd_status pcfd_learned_train(pcfd_solver* s, int steps, int batch, double learning_rate, pcfd_learned_stats* st) {
+    NEED(s, "pcfd_learned_train: null solver");
+    return guarded("pcfd_learned_train", [&] { s->impl->learned_train(steps, batch, learning_rate, st); });
+}
+pcfd_status pcfd_learned_get_weights(pcfd_solver* s, float* weights, float* biases) {
+    NEED(s && (weights || biases), "pcfd_learned_get_weights: null argument");
+    return guarded("pcfd_learned_get_weights", [&] { s->impl->learned_weights(weights, biases, nullptr, nullptr); });
+}
+pcfd_status pcfd_learned_set_weights(pcfd_solver* s, const float* weights, const float* biases) {
+    NEED(s && weights && biases, "pcfd_learned_set_weights: null argument");
+    return guarded("pcfd_learned_set_weights", [&] { s->impl->learned_weights(nullptr, nullptr, weights, biases); });
+}
+pcfd_status pcfd_learned_enable(pcfd_solver* s, int on) {
+    NEED(s, "pcfd_learned_enable: null solver");
+    return guarded("pcfd_learned_enable", [&] { s->impl->learned_enable(on != 0); });
+}
+pcfd_status pcfd_learned_forward(pcfd_solver* s, const double* r, float* y, float* scales) {
+    NEED(s && r && y && scales, "pcfd_learned_forward: null argument");
+    return guarded("pcfd_learned_forward", [&] { s->impl->learned_forward(r, y, scales); });
+}
+pcfd_status pcfd_learned_gradient(pcfd_solver* s, const double* residual, double* loss, float* grad_weights,
+                                  float* grad_biases) {
+    NEED(s && residual && loss, "pcfd_learned_gradient: null argument");
+    return guarded("pcfd_learned_gradient", [&] { s->impl->learned_gradient(residual, loss, grad_weights, grad_biases); });
 }
 
 }  // extern "C"
