@@ -264,15 +264,15 @@ pcfd_status pcfd_time_kernel(pcfd_solver* solver, int kernel, int reps, double* 
 
 /* ---- Learned preconditioner (PCFD_PC_LEARNED) --------------------------------------------
  * No counterpart in the reference, whose projection runs plain (GCRO / PCG) Krylov with an
- * algebraic V-cycle (amr_pressure.cu:1103-1353).  A 4-layer 3^3 conv net per brick (16
+ * algebraic V-cycle (amr_pressure.cu:1103-1353).  A 4-layer 3^3 conv net per brick (8
  * channels, BF16 MFMA, FP32 accumulate) corrects the V-cycle inside flexible PCG:
  * z = M_mg(r) + (s_b/h_b) N(r'/s_b) with r' = r - A M_mg(r) and s_b the brick's RMS of r'.
  * It is trained to minimise the energy norm of the error it leaves, dz^T A dz - 2 dz^T r'.
  * A projection whose learned solve stalls or fails restarts from its iterate with plain
  * MGPCG (pcfd_step_stats.fallback_used).  The network starts disabled; it is enabled by
  * pcfd_learned_train or pcfd_learned_set_weights. */
-#define PCFD_NN_WEIGHTS 27648 /* 4 layers x 27 taps x 16 cout x 16 cin, [layer][tap][cout][cin] */
-#define PCFD_NN_BIASES 64     /* 4 layers x 16 */
+#define PCFD_NN_WEIGHTS 6912  /* 4 layers x 27 taps x 8 cout x 8 cin, [layer][tap][cout][cin] */
+#define PCFD_NN_BIASES 32     /* 4 layers x 8 */
 
 typedef struct pcfd_learned_stats {
     int pairs;                /* stored training pairs */

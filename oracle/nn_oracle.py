@@ -9,8 +9,9 @@ gradient and the back-propagated activation gradients.  With ``bf16_round=False`
 network is plain float64, which the finite-difference tests use to pin the backward pass.
 
 Network (per 8^3 brick, zero padding at the brick boundary):
-  features (16 channels, 12 used): r'/s_b, 1, special flag, six structured-face mask bits,
-      parities of i, j, k as +-1 (zeros on inactive cells); s_b = RMS of r' over active cells
+  features (8 channels): r'/s_b, 1, special flag, parities of i, j, k as +-1, the fraction of
+      the six structured faces that are open, 0 (zeros on inactive cells); s_b = RMS of r'
+      over active cells
   a1 = act * relu(conv(x, W0) + b0), a2, a3 likewise, y = act * (conv(a3, W3) + b3)[ch 0]
   conv(x, W)[p, o] = sum_t sum_c W[t, o, c] x[p + off(t), c],  off(t) = (t%3-1, t//3%3-1, t//9-1)
   correction dz = (s_b / h_b) y on the network's bricks
@@ -25,7 +26,7 @@ from __future__ import annotations
 import numpy as np
 
 ACTIVE, SPECIAL = 64, 128
-C = 16
+C = 8
 TAPS = [((t // 9) - 1, ((t // 3) % 3) - 1, (t % 3) - 1) for t in range(27)]  # (dz, dy, dx)
 
 
@@ -42,7 +43,7 @@ def _round(x, on):
 
 
 def features(mask, r, bricks, bf16_round=True):
-    """Feature tiles (nb, 10, 10, 10, 16), active flags (nb, 8, 8, 8) and scales s (nb,)."""
+    """Feature tiles (nb, 10, 10, 10, 8), active flags (nb, 8, 8, 8) and scales s (nb,)."""
     nb = len(bricks)
     m = mask.reshape(-1, 512)[bricks].reshape(nb, 8, 8, 8)
     rr = r.reshape(-1, 512)[bricks].reshape(nb, 8, 8, 8)
@@ -62,11 +63,11 @@ def features(mask, r, bricks, bf16_round=True):
         f[..., 0] = rr * inv[:, None, None, None]
     f[..., 1] = 1.0
     f[..., 2] = ((m & SPECIAL) != 0)
-    for bit in range(6):
-        f[..., 3 + bit] = ((m >> bit) & 1) != 0
-    f[..., 9] = np.where(i & 1, 1.0, -1.0)[None]
-    f[..., 10] = np.where(j & 1, 1.0, -1.0)[None]
-    f[..., 11] = np.where(k & 1, 1.0, -1.0)[None]
+    f[..., 3] = np.where(i & 1, 1.0, -1.0)[None]
+    f[..., 4] = np.where(j & 1, 1.0, -1.0)[None]
+    f[..., 5] = np.where(k & 1, 1.0, -1.0)[None]
+    nopen = sum(((m >> bit) & 1).astype(np.float64) for bit in range(6))
+    f[..., 6] = _round((nopen / 6.0).astype(np.float32), bf16_round)
     f *= act[..., None]
     x[:, 1:9, 1:9, 1:9, :] = f
     return x, act, s
@@ -147,8 +148,8 @@ def correction(W, B, mask, rp, level_h, brick_level, bricks=None, bf16_round=Tru
 
 
 def energy_loss_and_grad(W, B, mask, rp, apply_A, level_h, brick_level, bricks=None, bf16_round=True):
-    """Energy loss (dz^T A dz - 2 dz^T r') / norm(r') and its gradients (dW (4,27,16,16),
-    dB (4,16)) for one post-V-cycle residual r'; apply_A(x) is the composite operator."""
+    """Energy loss (dz^T A dz - 2 dz^T r') / norm(r') and its gradients (dW (4,27,8,8),
+    dB (4,8)) for one post-V-cycle residual r'; apply_A(x) is the composite operator."""
     inv_norm = 1.0 / norm(mask, rp, level_h, brick_level)
     dz, (bricks, inputs, act, s, scale) = correction(W, B, mask, rp, level_h, brick_level, bricks, bf16_round)
     Ad = apply_A(dz)

@@ -60,8 +60,9 @@ class Options(C.Structure):
                 ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int)]
 
 
-NN_WEIGHTS = 4 * 27 * 16 * 16  # [layer][tap][cout][cin]
-NN_BIASES = 4 * 16
+NN_C = 8
+NN_WEIGHTS = 4 * 27 * NN_C * NN_C  # [layer][tap][cout][cin]
+NN_BIASES = 4 * NN_C
 
 
 class LearnedStats(C.Structure):

@@ -146,17 +146,17 @@ class ExternalAeroCore:
         return st.as_dict()
 
     def learned_weights(self):
-        """FP32 master weights [4, 27, 16, 16] ([layer][tap][cout][cin]) and biases [4, 16]."""
+        """FP32 master weights [4, 27, 8, 8] ([layer][tap][cout][cin]) and biases [4, 8]."""
         w = np.empty(L.NN_WEIGHTS, dtype=np.float32)
         b = np.empty(L.NN_BIASES, dtype=np.float32)
         L.check(self._lib.pcfd_learned_get_weights(self.handle, L.fptr(w), L.fptr(b)))
-        return w.reshape(4, 27, 16, 16), b.reshape(4, 16)
+        return w.reshape(4, 27, L.NN_C, L.NN_C), b.reshape(4, L.NN_C)
 
     def set_learned_weights(self, w, b):
         w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
         b = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
         if w.size != L.NN_WEIGHTS or b.size != L.NN_BIASES:
-            raise ValueError("weights must hold 4*27*16*16 values and biases 4*16")
+            raise ValueError("weights must hold 4*27*8*8 values and biases 4*8")
         L.check(self._lib.pcfd_learned_set_weights(self.handle, L.fptr(w), L.fptr(b)))
 
     def learned_enable(self, on=True):
@@ -171,10 +171,10 @@ class ExternalAeroCore:
         return y, s
 
     def learned_gradient(self, residual):
-        """(loss, dW [4,27,16,16], db [4,16]) of one post-V-cycle residual with the training kernels."""
+        """(loss, dW [4,27,8,8], db [4,8]) of one post-V-cycle residual with the training kernels."""
         r = np.ascontiguousarray(residual, dtype=np.float64)
         loss = C.c_double()
         gw = np.empty(L.NN_WEIGHTS, dtype=np.float32)
         gb = np.empty(L.NN_BIASES, dtype=np.float32)
         L.check(self._lib.pcfd_learned_gradient(self.handle, L.dptr(r), C.byref(loss), L.fptr(gw), L.fptr(gb)))
-        return loss.value, gw.reshape(4, 27, 16, 16), gb.reshape(4, 16)
+        return loss.value, gw.reshape(4, 27, L.NN_C, L.NN_C), gb.reshape(4, L.NN_C)

@@ -1,4 +1,4 @@
-// Learned preconditioner: a 4-layer 3x3x3 convolutional network (16 channels, BF16 MFMA,
+// Learned preconditioner: a 4-layer 3x3x3 convolutional network (8 channels, BF16 MFMA,
 // FP32 accumulate) applied per 8^3 brick (every AMR level, cell size h_b), correcting the
 // geometric V-cycle inside flexible PCG:
 //     z  = M_mg(r)                      (V-cycle)
@@ -12,14 +12,15 @@
 // gradient 2 (A dz - r') needs one operator apply, no exact solution.  Plain MGPCG is the
 // fallback.
 //
-// Input features per active cell (12 channels, the rest zero): r'/s_b, 1 (active),
-// special-cell flag, the six structured-face bits of the cell mask, and the parities of
-// i, j, k as +-1.  Inactive cells carry zeros.  Layers: 16 -> 16 channels, zero padding at
-// the brick boundary, ReLU after layers 0..2, outputs of inactive cells forced to zero;
-// the last layer's channel 0 is the network output.  Weights [layer][tap][cout][cin] BF16
-// (and a tap-flipped, cin/cout-transposed copy for the input gradient), biases FP32, FP32
-// master weights for Adam.  All network kernels run one 512-lane workgroup (8 waves) per
-// brick, thread q owning cell q.
+// Input features per active cell (8 channels): r'/s_b, 1 (active), special-cell flag, the
+// parities of i, j, k as +-1, the fraction of the six structured faces that are open, and 0.
+// Inactive cells carry zeros.  Layers: 8 -> 8 channels, zero padding at the brick boundary,
+// ReLU after layers 0..2, outputs of inactive cells forced to zero; the last layer's channel
+// 0 is the network output.  Weights [layer][tap][cout][cin] BF16 (and a tap-flipped,
+// cin/cout-transposed copy for the input gradient), biases FP32, FP32 master weights for
+// Adam.  All network kernels run one 512-lane workgroup (8 waves) per brick, thread q owning
+// cell q.  An activation tile is 10^3 positions x 8 channels x 2 bytes = 16 KB, so four
+// bricks share a CU.
 //
 // MFMA v_mfma_f32_16x16x32_bf16 lane layout (checked on gfx950): A[m][k] lane l holds
 // m = l%16, k = 8(l/16)+j; B[k][n]: n = l%16, same k; D[m][n]: m = 4(l/16)+i, n = l%16.
@@ -31,8 +32,7 @@
 namespace pcfd {
 
 constexpr int kNNLayers = 4;
-constexpr int kNNC = 16;
-constexpr int kNNIn = 12;
+constexpr int kNNC = 8;
 constexpr int kNNTaps = 27;
 constexpr int kNNWLayer = kNNTaps * kNNC * kNNC;  // weights per layer
 constexpr int kNNW = kNNLayers * kNNWLayer;        // all weights
@@ -59,7 +59,7 @@ __device__ __forceinline__ int nn_pos(int cell) { return tile_index(cell & 7, (c
 struct NNView {
     const uint16_t* w;    // [L][27][cout][cin]
     const uint16_t* wt;   // [L][27][cin][cout], taps flipped: wt[l][t][i][o] = w[l][26-t][o][i]
-    const float* bias;    // [L][16]
+    const float* bias;    // [L][8]
 };
 
 // LDS of the network kernels (one brick): two activation tiles, one layer's weights, flags
@@ -72,10 +72,10 @@ struct NNShared {
     uint8_t act[512];
 };
 
-// One 3x3x3 convolution 16 -> 16 channels over the brick: 32 M-tiles of 16 cells, 4 per
-// wave.  K = 27 taps x 16 channels, two taps per 32-wide MFMA step (14 steps, the second
-// half of the last one zero).  Output to a tile (BF16) or, when out_tile == nullptr,
-// channel 0 into yout (FP32, 512).  wl: [27][n][k] in LDS.
+// One 3x3x3 convolution 8 -> 8 channels over the brick: 32 M-tiles of 16 cells, 4 per wave.
+// K = 27 taps x 8 channels, four taps per 32-wide MFMA step (7 steps, the last tap slot
+// zero); N = 16 with output channels 8..15 zero.  Output to a tile (BF16) or, when
+// out_tile == nullptr, channel 0 into yout (FP32, 512).  wl: [27][cout][cin] in LDS.
 __device__ __forceinline__ void nn_conv(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wl,
                                         const float* bias, bool relu, const uint8_t* act,
                                         uint16_t* __restrict__ out_tile, float* __restrict__ yout) {
@@ -85,23 +85,24 @@ __device__ __forceinline__ void nn_conv(const uint16_t* __restrict__ in, const u
     int pos[kNNMT];
     for (int mt = 0; mt < kNNMT; ++mt) {
         acc[mt] = nn_floatx4{0.f, 0.f, 0.f, 0.f};
-        pos[mt] = nn_pos((wave * kNNMT + mt) * 16 + n) * kNNC;
+        pos[mt] = nn_pos((wave * kNNMT + mt) * 16 + n);
     }
-    for (int s = 0; s < 14; ++s) {
-        const int t = 2 * s + (g >> 1), half = g & 1;
+    for (int s = 0; s < 7; ++s) {
+        const int t = 4 * s + g;
         const bool valid = t < kNNTaps;
         nn_bf16x8 bfrag = {};
         int off = 0;
         if (valid) {
-            bfrag = *reinterpret_cast<const nn_bf16x8*>(wl + (t * kNNC + n) * kNNC + 8 * half);
-            off = nn_tap_offset(t) * kNNC + 8 * half;
+            if (n < kNNC) bfrag = *reinterpret_cast<const nn_bf16x8*>(wl + (t * kNNC + n) * kNNC);
+            off = nn_tap_offset(t);
         }
         for (int mt = 0; mt < kNNMT; ++mt) {
             nn_bf16x8 afrag = {};
-            if (valid) afrag = *reinterpret_cast<const nn_bf16x8*>(in + pos[mt] + off);
+            if (valid) afrag = *reinterpret_cast<const nn_bf16x8*>(in + (pos[mt] + off) * kNNC);
             acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc[mt], 0, 0, 0);
         }
     }
+    if (n >= kNNC) return;
     const float bn = bias ? bias[n] : 0.f;
     for (int mt = 0; mt < kNNMT; ++mt)
         for (int i = 0; i < 4; ++i) {
@@ -114,26 +115,32 @@ __device__ __forceinline__ void nn_conv(const uint16_t* __restrict__ in, const u
         }
 }
 
-// Weight gradient of one layer, dW[t][cout][cin] += sum_cells g[cell][cout] a[cell+off(t)][cin]
-// (M = cout, N = cin, K = 512 cells in 16 steps); wave w owns taps w, w+8, ...  Bias
-// gradient db[cout] += sum_cells g[cell][cout].
+// Weight gradient of one layer, dW[t][cout][cin] += sum_cells g[cell][cout] a[cell+off(t)][cin],
+// two taps per MFMA: M = cout (rows 8..15 zero), N = 2 taps x 8 cin, K = 512 cells in 16
+// steps; wave w owns tap pairs w, w+8 (14 pairs).  Bias gradient db[cout] += sum_cells g.
 __device__ __forceinline__ void nn_weight_grad(const uint16_t* __restrict__ gt, const uint16_t* __restrict__ at,
                                                float* __restrict__ gw, float* __restrict__ gb) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, n = lane & 15;
-    for (int t = wave; t < kNNTaps; t += 8) {
-        const int off = nn_tap_offset(t);
+    for (int pr = wave; pr < 14; pr += 8) {
+        const int t = 2 * pr + (n >> 3), cin = n & 7;
+        const bool valid = t < kNNTaps;
+        const int off = valid ? nn_tap_offset(t) : 0;
         nn_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         for (int s = 0; s < 16; ++s) {
-            nn_bf16x8 afrag, bfrag;
+            nn_bf16x8 afrag = {}, bfrag = {};
             for (int j = 0; j < 8; ++j) {
                 const int p = nn_pos(32 * s + 8 * g + j);
-                afrag[j] = __builtin_bit_cast(__bf16, gt[p * kNNC + n]);
-                bfrag[j] = __builtin_bit_cast(__bf16, at[(p + off) * kNNC + n]);
+                if (n < kNNC) afrag[j] = __builtin_bit_cast(__bf16, gt[p * kNNC + n]);
+                if (valid) bfrag[j] = __builtin_bit_cast(__bf16, at[(p + off) * kNNC + cin]);
             }
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc, 0, 0, 0);
         }
-        for (int i = 0; i < 4; ++i) atomicAdd(gw + (t * kNNC + 4 * g + i) * kNNC + n, acc[i]);
+        if (valid)
+            for (int i = 0; i < 4; ++i) {
+                const int cout = 4 * g + i;
+                if (cout < kNNC) atomicAdd(gw + (t * kNNC + cout) * kNNC + cin, acc[i]);
+            }
     }
     if (threadIdx.x < kNNC) {
         float s = 0.f;
@@ -146,9 +153,20 @@ __device__ __forceinline__ void nn_zero_tile(uint16_t* tile) {
     for (int i = threadIdx.x; i < kNNTile / 8; i += blockDim.x) reinterpret_cast<uint4*>(tile)[i] = uint4{0u, 0u, 0u, 0u};
 }
 
+// One layer's weights (kNNWLayer / 8 = 216 units of 16 bytes) through registers into LDS:
+// fetch issues the global load, put stores it, so that the next layer's load can be in
+// flight during the current convolution.
+struct NNWRegs { uint4 v; };
+__device__ __forceinline__ void nn_fetch_weights(const uint16_t* __restrict__ src, NNWRegs& R) {
+    if (threadIdx.x < kNNWLayer / 8) R.v = reinterpret_cast<const uint4*>(src)[threadIdx.x];
+}
+__device__ __forceinline__ void nn_put_weights(const NNWRegs& R, uint16_t* dst) {
+    if (threadIdx.x < kNNWLayer / 8) reinterpret_cast<uint4*>(dst)[threadIdx.x] = R.v;
+}
 __device__ __forceinline__ void nn_load_weights(const uint16_t* __restrict__ src, uint16_t* dst) {
-    for (int i = threadIdx.x; i < kNNWLayer / 8; i += blockDim.x)
-        reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+    NNWRegs R;
+    nn_fetch_weights(src, R);
+    nn_put_weights(R, dst);
 }
 
 // Brick scale s_b (RMS of v over active cells; 0 when that is zero) and, when s_b > 0, the
@@ -167,22 +185,13 @@ __device__ __forceinline__ float nn_features(float v, uint8_t m, uint16_t* tile,
     const float s = N > 0.f ? sqrtf(S / N) : 0.f;
     if (act && s > 0.f) {
         constexpr uint16_t one = 0x3f80, minus_one = 0xbf80;
-        uint16_t f[12];
-        f[0] = nn_f2bf(v * (1.f / s));
-        f[1] = one;
-        f[2] = (m & kSpecial) ? one : 0;
-        for (int k = 0; k < 6; ++k) f[3 + k] = ((m >> k) & 1) ? one : 0;
-        f[9] = (q & 1) ? one : minus_one;
-        f[10] = ((q >> 3) & 1) ? one : minus_one;
-        f[11] = ((q >> 6) & 1) ? one : minus_one;
-        uint4 lo, hi;
-        lo.x = f[0] | (unsigned(f[1]) << 16); lo.y = f[2] | (unsigned(f[3]) << 16);
-        lo.z = f[4] | (unsigned(f[5]) << 16); lo.w = f[6] | (unsigned(f[7]) << 16);
-        hi.x = f[8] | (unsigned(f[9]) << 16); hi.y = f[10] | (unsigned(f[11]) << 16);
-        hi.z = 0u; hi.w = 0u;
-        uint4* dst = reinterpret_cast<uint4*>(tile + nn_pos(q) * kNNC);
-        dst[0] = lo;
-        dst[1] = hi;
+        const int open = __popc(static_cast<unsigned>(m) & 63u);
+        uint4 f;
+        f.x = nn_f2bf(v * (1.f / s)) | (unsigned(one) << 16);
+        f.y = ((m & kSpecial) ? one : 0u) | (unsigned((q & 1) ? one : minus_one) << 16);
+        f.z = ((q >> 3) & 1 ? one : minus_one) | (unsigned((q >> 6) & 1 ? one : minus_one) << 16);
+        f.w = nn_f2bf(static_cast<float>(open) / 6.f);
+        reinterpret_cast<uint4*>(tile)[nn_pos(q)] = f;
     }
     __syncthreads();
     return s;
@@ -193,15 +202,15 @@ __device__ __forceinline__ float nn_features(float v, uint8_t m, uint16_t* tile,
 __device__ __forceinline__ void nn_layers(const NNView& W, NNShared& L, uint16_t* store) {
     uint16_t* in = L.T0;
     uint16_t* out = L.T1;
+    NNWRegs R;
+    nn_fetch_weights(W.w, R);
     for (int l = 0; l < kNNLayers; ++l) {
-        nn_load_weights(W.w + static_cast<size_t>(l) * kNNWLayer, L.WL);
+        nn_put_weights(R, L.WL);
+        if (l + 1 < kNNLayers) nn_fetch_weights(W.w + static_cast<size_t>(l + 1) * kNNWLayer, R);
         __syncthreads();
         if (store && l > 0)
-            for (int i = threadIdx.x; i < 512 * 2; i += blockDim.x) {
-                const int cell = i >> 1, c = 8 * (i & 1);
-                *reinterpret_cast<uint4*>(store + (static_cast<size_t>(l - 1) * 512 + cell) * kNNC + c) =
-                    *reinterpret_cast<const uint4*>(in + nn_pos(cell) * kNNC + c);
-            }
+            *reinterpret_cast<uint4*>(store + (static_cast<size_t>(l - 1) * 512 + threadIdx.x) * kNNC) =
+                reinterpret_cast<const uint4*>(in)[nn_pos(threadIdx.x)];
         const bool last = l == kNNLayers - 1;
         nn_conv(in, L.WL, W.bias + l * kNNC, !last, L.act, last ? nullptr : out, last ? L.y : nullptr);
         __syncthreads();
@@ -370,11 +379,8 @@ __global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, cons
         uint16_t* ain = L.T1;
         if (l > 0) {
             ain = gnext;
-            for (int i = threadIdx.x; i < 512 * 2; i += blockDim.x) {
-                const int cell = i >> 1, c = 8 * (i & 1);
-                *reinterpret_cast<uint4*>(ain + nn_pos(cell) * kNNC + c) =
-                    *reinterpret_cast<const uint4*>(store + (static_cast<size_t>(l - 1) * 512 + cell) * kNNC + c);
-            }
+            reinterpret_cast<uint4*>(ain)[nn_pos(q)] =
+                *reinterpret_cast<const uint4*>(store + (static_cast<size_t>(l - 1) * 512 + q) * kNNC);
         }
         __syncthreads();
         nn_weight_grad(gcur, ain, grad + static_cast<size_t>(l) * kNNWLayer, grad + kNNW + l * kNNC);

@@ -447,11 +447,10 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
     };
     auto uniform = [&] { return (static_cast<double>(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
     for (int l = 0; l + 1 < kNNLayers; ++l) {
-        const int fan_in = kNNTaps * (l == 0 ? kNNIn : kNNC);
-        const double sd = std::sqrt(2.0 / fan_in);
+        const double sd = std::sqrt(2.0 / (kNNTaps * kNNC));
         for (int t = 0; t < kNNTaps; ++t)
             for (int o = 0; o < kNNC; ++o)
-                for (int i = 0; i < (l == 0 ? kNNIn : kNNC); ++i) {
+                for (int i = 0; i < kNNC; ++i) {
                     const double g = std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform());
                     p[static_cast<size_t>(l) * kNNWLayer + (t * kNNC + o) * kNNC + i] = static_cast<float>(sd * g);
                 }

@@ -44,9 +44,8 @@ def learned_core(topo, **kw):
 
 def random_weights(seed, scale=0.2):
     rng = np.random.default_rng(seed)
-    W = rng.normal(0.0, scale, size=(4, 27, 16, 16)).astype(np.float32)
-    B = rng.normal(0.0, 0.05, size=(4, 16)).astype(np.float32)
-    W[0, :, :, 12:] = 0.0
+    W = rng.normal(0.0, scale, size=(4, 27, 8, 8)).astype(np.float32)
+    B = rng.normal(0.0, 0.05, size=(4, 8)).astype(np.float32)
     return W, B
 
 
@@ -187,8 +186,8 @@ def test_fallback_on_broken_network(lib):
     print(f"[learned] fallback, dofs={topo.ndof}", flush=True)
     core = learned_core(topo)
     core.initialize()
-    W = np.full((4, 27, 16, 16), 1e30, dtype=np.float32)
-    B = np.zeros((4, 16), dtype=np.float32)
+    W = np.full((4, 27, 8, 8), 1e30, dtype=np.float32)
+    B = np.zeros((4, 8), dtype=np.float32)
     core.set_learned_weights(W, B)
     st = core.step()
     assert st["fallback_used"] == 1

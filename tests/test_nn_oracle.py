@@ -21,9 +21,8 @@ def synthetic_mask(nb, seed):
 
 def random_net(seed, scale=0.1):
     rng = np.random.default_rng(seed)
-    W = rng.normal(0, scale, size=(4, 27, 16, 16))
-    B = rng.normal(0, scale, size=(4, 16))
-    W[0, :, :, 12:] = 0.0
+    W = rng.normal(0, scale, size=(4, 27, 8, 8))
+    B = rng.normal(0, scale, size=(4, 8))
     return W, B
 
 
@@ -38,13 +37,13 @@ def test_bf16_rounding_known_values():
 
 def test_conv_matches_brute_force():
     rng = np.random.default_rng(3)
-    x = np.zeros((1, 10, 10, 10, 16))
-    x[:, 1:9, 1:9, 1:9, :] = rng.normal(size=(1, 8, 8, 8, 16))
-    w = rng.normal(size=(27, 16, 16))
-    b = rng.normal(size=16)
+    x = np.zeros((1, 10, 10, 10, 8))
+    x[:, 1:9, 1:9, 1:9, :] = rng.normal(size=(1, 8, 8, 8, 8))
+    w = rng.normal(size=(27, 8, 8))
+    b = rng.normal(size=8)
     out = N.conv(x, w, b)
     for (k, j, i) in [(0, 0, 0), (7, 7, 7), (3, 0, 5), (4, 6, 1)]:
-        for o in (0, 5, 15):
+        for o in (0, 5, 7):
             acc = b[o]
             for dz in (-1, 0, 1):
                 for dy in (-1, 0, 1):
@@ -53,19 +52,19 @@ def test_conv_matches_brute_force():
                         if not (0 <= kk < 8 and 0 <= jj < 8 and 0 <= ii < 8):
                             continue   # zero padding at the brick boundary
                         t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)
-                        acc += sum(w[t, o, c] * x[0, kk + 1, jj + 1, ii + 1, c] for c in range(16))
+                        acc += sum(w[t, o, c] * x[0, kk + 1, jj + 1, ii + 1, c] for c in range(8))
             assert out[0, k, j, i, o] == pytest.approx(acc, rel=1e-12, abs=1e-12)
 
 
 def test_conv_identity_and_shift():
     rng = np.random.default_rng(4)
-    x = np.zeros((2, 10, 10, 10, 16))
-    x[:, 1:9, 1:9, 1:9, :] = rng.normal(size=(2, 8, 8, 8, 16))
-    w = np.zeros((27, 16, 16))
-    w[13] = np.eye(16)          # centre tap
+    x = np.zeros((2, 10, 10, 10, 8))
+    x[:, 1:9, 1:9, 1:9, :] = rng.normal(size=(2, 8, 8, 8, 8))
+    w = np.zeros((27, 8, 8))
+    w[13] = np.eye(8)          # centre tap
     assert np.array_equal(N.conv(x, w), x[:, 1:9, 1:9, 1:9, :])
-    w = np.zeros((27, 16, 16))
-    w[14] = np.eye(16)          # tap (dz, dy, dx) = (0, 0, +1): out[i] = x[i + 1]
+    w = np.zeros((27, 8, 8))
+    w[14] = np.eye(8)          # tap (dz, dy, dx) = (0, 0, +1): out[i] = x[i + 1]
     out = N.conv(x, w)
     assert np.array_equal(out[..., :7, :], x[:, 1:9, 1:9, 2:9, :])
     assert not out[..., 7, :].any()
@@ -80,7 +79,11 @@ def test_features_scale_invariance():
     assert np.allclose(s2, 7.5 * s1, rtol=1e-6)
     assert np.allclose(x1, x2, atol=1e-6)
     assert not x1[:, 1:9, 1:9, 1:9][~act].any()           # inactive cells carry zeros
-    assert x1[..., 12:].max() == 0.0
+    assert x1[..., 7].max() == 0.0
+    # the open-face fraction is the popcount of the six face bits over 6
+    m = mask.reshape(3, 8, 8, 8)
+    nopen = sum(((m >> bit) & 1) for bit in range(6)) / 6.0
+    assert np.allclose(x1[:, 1:9, 1:9, 1:9, 6][act], nopen[act])
 
 
 def synthetic_operator(n):
@@ -113,7 +116,7 @@ def test_backward_matches_finite_differences(layer):
     # central differences carry ~1e-16 |loss| / eps of round-off: an absolute floor far below
     # what a dropped term or a transposed operand would change
     floor = 1e-8 * max(np.abs(dW[layer]).max(), np.abs(dB[layer]).max())
-    idx = [(layer, t, o, c) for t, o, c in zip(rng.integers(0, 27, 6), rng.integers(0, 16, 6), rng.integers(0, 12, 6))]
+    idx = [(layer, t, o, c) for t, o, c in zip(rng.integers(0, 27, 6), rng.integers(0, 8, 6), rng.integers(0, 7, 6))]
     if layer == 3:
         idx = [(3, t, 0, c) for (_, t, _, c) in idx]
     for ix in idx:
@@ -122,7 +125,7 @@ def test_backward_matches_finite_differences(layer):
         Wm[ix] -= eps
         fd = (loss(Wp, B) - loss(Wm, B)) / (2 * eps)
         assert dW[ix] == pytest.approx(fd, rel=1e-5, abs=floor), ix
-    for o in (0, 7):
+    for o in (0, 5):
         Bp, Bm = B.copy(), B.copy()
         Bp[layer, o] += eps
         Bm[layer, o] -= eps
