@@ -1,11 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark: cell-steps/s of the full solver step (MUSCL transport + MGPCG projection to
-rtol 1e-8 + surface loads) on one MI355X.
+rtol 1e-8 + surface loads) on one MI355X, learned preconditioner vs plain MGPCG.
 
 Default workload: NACA 2412 wing, alpha 4 deg, 2-level static AMR of 8^3 bricks (~4.1 M
-active cells), FP32 fields with the FP64 outer conjugate-gradient solve.  A "step" is one
-call of the solver step on the resident state; the loads integral runs once per step too.
-Prints one JSON line (rank 0).  --gpus above 1 is not supported on this single-GPU build.
+active cells), FP32 fields, FP64 outer conjugate gradients.  Three solvers run the same step
+sequence from the same freestream start:
+  * learned: FP32 V-cycle + the learned network correction (BF16 MFMA) in flexible PCG, after
+    untimed data collection and training on the solver's own MGPCG residuals  -> `value`
+  * plain:   FP32 V-cycle MGPCG (the same arithmetic without the network)
+  * plain_fp64_vcycle: FP64 V-cycle MGPCG (plain FP64-outer MGPCG, the reference's method)
+and an FP64-everywhere solver (fields, V-cycle, CG; it matches the CPU oracle to 1e-10 in the
+parity tests) gives the errors of the learned run's velocity, pressure and CL/CD.
+A "step" is one solver step on the resident state plus the loads integral.  Prints one JSON
+line.  --gpus above 1 is not supported on this single-GPU build.
 """
 from __future__ import annotations
 
@@ -19,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "cell-steps/s (MUSCL+MGPCG to rtol 1e-8), learned vs plain PC, 1 MI355X"
+PEAK_HBM = 8000.0      # GB/s
+PEAK_BF16 = 2500.0     # TFLOP/s dense
 
 
 def log(msg):
@@ -35,11 +44,11 @@ def cpu_baseline(topo, options, budget_s):
         setattr(o, name, getattr(options, name))
     o.field_precision = L.FP64
     o.vcycle_precision = L.FP64
+    o.preconditioner = L.PC_MG
     orc = Oracle(topo, o)
     t0 = time.perf_counter()
     orc.initialize()
-    t_init = time.perf_counter() - t0
-    log(f"cpu oracle initialize {t_init:.1f}s")
+    log(f"cpu oracle initialize {time.perf_counter() - t0:.1f}s")
     steps = 0
     elapsed = 0.0
     while steps == 0 or elapsed < budget_s:
@@ -55,22 +64,79 @@ def cpu_baseline(topo, options, budget_s):
                       f"({elapsed:.1f} s timed)"}
 
 
+def rel_l2(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run(topo, spec, args, name, field, vcycle, pc):
+    """Initialise, pre-steps (with training for the learned solver), warmup, K timed steps."""
+    import torch
+    from paracfd_amd import ExternalAeroCore, _lib as L, default_options
+    opt = default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"], cfl=spec["cfl"], rtol=1e-8,
+                          max_iterations=500, field_precision=field, vcycle_precision=vcycle, preconditioner=pc,
+                          pre_smooth=2, post_smooth=2, check_interval=args.check_interval,
+                          learned_bricks=args.learned_bricks, learned_capacity=args.collects * args.pairs)
+    core = ExternalAeroCore(topo, opt)
+    core.set_reference(topo.info.reference_area)
+    st = core.initialize()
+    log(f"{name}: initialize {st['iterations']} its, converged {st['converged']}")
+    train = None
+    for i in range(2 + args.collects):
+        if pc == L.PC_LEARNED and i >= 2:
+            core.learned_collect(max_pairs=args.pairs)
+        core.step()
+        core.pressure_loads()
+    if pc == L.PC_LEARNED:
+        train = core.learned_train(steps=args.train_steps, batch=4, lr=args.lr)
+        log(f"{name}: trained {train['train_steps']} steps on {train['pairs']} residuals "
+            f"({train['bricks']} bricks), loss {train['loss_first']:.4f} -> {train['loss_last']:.4f}, "
+            f"{train['train_ms']:.0f} ms")
+    for i in range(args.warmup):
+        st = core.step()
+        core.pressure_loads()
+    torch.cuda.synchronize()
+    its, fallbacks, used, conv = [], 0, 0, True
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = core.step()
+        loads = core.pressure_loads()
+        its.append(st["iterations"])
+        fallbacks += st["fallback_used"]
+        used += st["learned_used"]
+        conv = conv and bool(st["converged"])
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    n = topo.n_active
+    res = {"core": core, "value": n * args.steps / seconds, "ms_per_step": 1e3 * seconds / args.steps,
+           "mean_its": sum(its) / len(its), "fallback_steps": fallbacks, "converged": conv, "loads": loads,
+           "train": train, "learned_steps": used, "learned_rejected": bool(st["learned_rejected"])}
+    log(f"{name}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} ms/step, mean its {res['mean_its']:.1f}, "
+        f"fallbacks {fallbacks}, network used in {used} steps (rejected: {res['learned_rejected']}), "
+        f"CL {loads['cl']:.6f} CD {loads['cd']:.6f}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--case", default="naca2412_2lvl")
-    ap.add_argument("--fields", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--vcycle", default="fp64", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--check-interval", type=int, default=2)
+    ap.add_argument("--learned-bricks", type=int, default=2,
+                    help="0 all bricks, 1 bricks with special cells and neighbours, 2 bricks with cut cells and neighbours")
+    ap.add_argument("--collects", type=int, default=4)
+    ap.add_argument("--pairs", type=int, default=16)
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is designed, "
-              "not built; see DESIGN.md)", file=sys.stderr)
+              "not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -78,82 +144,84 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    from paracfd_amd import ExternalAeroCore, _lib as L, cases, default_options
+    from paracfd_amd import _lib as L, cases
 
     g, d, spec = cases.build_case(args.case)
     t0 = time.perf_counter()
     topo = L.Topology(g, d)
     log(f"topology {args.case}: {topo.n_active} active cells, {topo.info.bricks} bricks, "
         f"{topo.mg_levels} MG levels, built in {time.perf_counter() - t0:.1f}s")
-    prec = {"fp64": L.FP64, "fp32": L.FP32, "bf16": L.BF16}
-    opt = default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"], cfl=spec["cfl"], rtol=1e-8,
-                          max_iterations=500, field_precision=prec[args.fields], vcycle_precision=prec[args.vcycle],
-                          preconditioner=L.PC_MG, pre_smooth=2, post_smooth=2, check_interval=args.check_interval)
-    core = ExternalAeroCore(topo, opt)
-    core.set_reference(topo.info.reference_area)
-    st = core.initialize()
-    log(f"initialize: {st['iterations']} PCG iterations, relres {st['relative_residual']:.2e}, "
-        f"converged {st['converged']}")
-    for i in range(args.warmup):
-        st = core.step()
-        core.pressure_loads()
-        log(f"warmup {i}: dt {st['dt']:.4g} its {st['iterations']} step {st['step_ms']:.2f} ms")
-    torch.cuda.synchronize()
-    its = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = core.step()
-        loads = core.pressure_loads()
-        its.append(st["iterations"])
-        if not st["converged"]:
-            log(f"step {i} did not converge: relres {st['relative_residual']:.3e}")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    seconds = t1 - t0
-    n = topo.n_active
-    value = n * args.steps / seconds
-    log(f"timed {args.steps} steps in {seconds:.3f}s: {value:.4g} cell-steps/s, mean its {sum(its) / len(its):.1f}, "
-        f"CL {loads['cl']:.4f} CD {loads['cd']:.4f}")
 
-    # roofline of the dominant kernel, timed live with HIP events on the solver's stream
-    mean_its = sum(its) / len(its)
-    # launches per step of each timed kernel family: the CG direction+apply pass once per
-    # iteration; four composite-level sweep-type passes per V-cycle (2 pre-sweeps fused,
-    # residual+restriction, prolongation+sweep, last sweep), one V-cycle per iteration plus
-    # the initial one; one transport pass
-    per_step = {
-        0: mean_its,
-        1: (mean_its + 1) * 4,
-        2: 1.0,
-    }
+    learned = run(topo, spec, args, "learned", L.FP32, L.FP32, L.PC_LEARNED)
+    plain32 = run(topo, spec, args, "plain", L.FP32, L.FP32, L.PC_MG)
+    plain64 = run(topo, spec, args, "plain_fp64_vcycle", L.FP32, L.FP64, L.PC_MG)
+    ref = run(topo, spec, args, "fp64_everywhere", L.FP64, L.FP64, L.PC_MG)
+
+    # errors of the learned mixed-precision run against the FP64-everywhere run (same steps)
+    import numpy as np
+    vl = np.concatenate(learned["core"].download_cell_momentum_state())
+    vr = np.concatenate(ref["core"].download_cell_momentum_state())
+    pl, pr = learned["core"].download_pressure(), ref["core"].download_pressure()
+    errors = {"velocity_rel_l2": rel_l2(vl, vr), "pressure_rel_l2": rel_l2(pl, pr),
+              "cl_rel": abs(learned["loads"]["cl"] - ref["loads"]["cl"]) / max(abs(ref["loads"]["cl"]), 1e-30),
+              "cd_rel": abs(learned["loads"]["cd"] - ref["loads"]["cd"]) / max(abs(ref["loads"]["cd"]), 1e-30)}
+    log(f"errors vs FP64: {errors}")
+
+    # roofline: per-launch time of each hot kernel on the learned solver's stream (HIP events)
+    core = learned["core"]
+    mean_its = learned["mean_its"]
+    per_step = {0: mean_its, 1: (mean_its + 1) * 4, 2: 1.0, 4: mean_its + 1}
+    names = {0: "k_cg_apply_dot", 1: "k_smooth0<float>", 2: "k_advect<float>", 4: "k_nn_apply"}
     timings = {}
     for k in per_step:
         ms, by = core.time_kernel(k, reps=20)
         timings[k] = (ms, by, ms * per_step[k])
-        log(f"kernel {k}: {ms * 1e3:.1f} us/launch, {by / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic, "
+        log(f"kernel {names[k]}: {ms * 1e3:.1f} us/launch, {by / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic, "
             f"{ms * per_step[k]:.2f} ms/step")
-    names = {0: "k_cg_apply_dot", 1: "k_smooth0", 2: "k_advect"}
-    dom = max(timings, key=lambda k: timings[k][2])
+    hbm = {k: v for k, v in timings.items() if k != 4}
+    dom = max(hbm, key=lambda k: hbm[k][2])
     ms, by, _ = timings[dom]
     achieved = by / (ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": None}
+    roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM, "traffic": None}
+    tr = learned["train"]
+    ms_nn = timings[4][0]
+    flops_nn = tr["bricks"] * 512 * 4 * 27 * 8 * 8 * 2.0   # useful network flops per launch
+    nn_roof = {"bound": "mfma", "kernel": "k_nn_apply", "achieved": flops_nn / (ms_nn * 1e-3) / 1e12,
+               "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": flops_nn / (ms_nn * 1e-3) / 1e12 / PEAK_BF16,
+               "us_per_launch": ms_nn * 1e3, "launches_per_step": per_step[4]}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(topo, opt, args.cpu_budget)
+        from paracfd_amd import default_options
+        cpu = cpu_baseline(topo, default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"],
+                                                 cfl=spec["cfl"], rtol=1e-8, max_iterations=500), args.cpu_budget)
         log(f"cpu baseline {cpu['value']:.4g} cell-steps/s")
 
+    def summary(r):
+        return {"value": r["value"], "ms_per_step": r["ms_per_step"], "mean_pcg_iterations": r["mean_its"],
+                "converged": r["converged"], "cl": r["loads"]["cl"], "cd": r["loads"]["cd"]}
+
+    n = topo.n_active
     out = {
-        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * seconds / args.steps, "higher_is_better": True,
+        "metric": METRIC, "value": learned["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": learned["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 fields + f64 PCG" if args.fields == "fp32" else "f64",
+        "dtype": "f32 fields + f32 V-cycle + bf16 network + f64 PCG",
         "data": "synthetic (analytic geometry, freestream start)",
         "config": {"workload": args.case, "active_cells": n, "amr_levels": topo.info.levels,
-                   "preconditioner": "geometric V-cycle MGPCG", "vcycle": args.vcycle, "rtol": 1e-8,
-                   "mean_pcg_iterations": mean_its, "cl": loads["cl"], "cd": loads["cd"]},
+                   "preconditioner": "V-cycle + learned 4-layer 8-channel conv correction, flexible PCG",
+                   "network_bricks": tr["bricks"], "rtol": 1e-8, "mean_pcg_iterations": mean_its,
+                   "fallback_steps": learned["fallback_steps"], "network_steps": learned["learned_steps"],
+                   "network_rejected_by_validation": learned["learned_rejected"], "cl": learned["loads"]["cl"],
+                   "cd": learned["loads"]["cd"],
+                   "training": {k: tr[k] for k in ("pairs", "train_steps", "loss_first", "loss_last", "train_ms")},
+                   "plain_fp32_vcycle": summary(plain32), "plain_fp64_vcycle": summary(plain64),
+                   "speedup_vs_plain_fp32_vcycle": learned["value"] / plain32["value"],
+                   "speedup_vs_plain_fp64_vcycle": learned["value"] / plain64["value"],
+                   "errors_vs_fp64": errors},
         "roofline": roofline,
+        "roofline_network": nn_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

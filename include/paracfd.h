@@ -155,9 +155,11 @@ typedef struct pcfd_options {
                                  results do not depend on it */
     double smoother_weight;   /* damped-Jacobi weight of the V-cycle sweeps (default 6/7;
                                  the reference's algebraic MG uses 2/3, amr_pressure.cu:1103) */
-    int learned_bricks;       /* PCFD_PC_LEARNED: bricks the network corrects, 0 all bricks,
-                                 1 bricks holding special cells and their face neighbours */
-    int learned_capacity;     /* PCFD_PC_LEARNED: stored (residual, error) training pairs */
+    int learned_bricks;       /* PCFD_PC_LEARNED: bricks the network corrects: 0 all bricks,
+                                 1 bricks holding special cells (cut cells, coarse/fine faces)
+                                 and their face neighbours, 2 bricks holding cut cells and
+                                 their face neighbours */
+    int learned_capacity;     /* PCFD_PC_LEARNED: recorded residuals kept for training */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {
@@ -171,6 +173,9 @@ typedef struct pcfd_step_stats {
     double max_abs_divergence;/* after projection, integrated divergence / volume (FP64) */
     int diffusion_substeps;
     int fallback_used;        /* learned preconditioner fell back to plain MG */
+    int learned_used;         /* the learned network ran in this projection */
+    int learned_rejected;     /* validation after training found the network slower than plain
+                                 MGPCG and switched it off (stays set until the next training) */
     double advect_ms, project_ms, step_ms; /* device-event timings */
 } pcfd_step_stats;
 
@@ -270,7 +275,9 @@ pcfd_status pcfd_time_kernel(pcfd_solver* solver, int kernel, int reps, double* 
  * It is trained to minimise the energy norm of the error it leaves, dz^T A dz - 2 dz^T r'.
  * A projection whose learned solve stalls or fails restarts from its iterate with plain
  * MGPCG (pcfd_step_stats.fallback_used).  The network starts disabled; it is enabled by
- * pcfd_learned_train or pcfd_learned_set_weights. */
+ * pcfd_learned_train or pcfd_learned_set_weights.  After pcfd_learned_train, the next three
+ * steps validate it: if their mean projection time exceeds that of the last (up to four)
+ * projections without the network, the network is switched off (learned_rejected). */
 #define PCFD_NN_WEIGHTS 6912  /* 4 layers x 27 taps x 8 cout x 8 cin, [layer][tap][cout][cin] */
 #define PCFD_NN_BIASES 32     /* 4 layers x 8 */
 
@@ -280,10 +287,13 @@ typedef struct pcfd_learned_stats {
     double loss_first;        /* mean loss of the first / last step of the last train call */
     double loss_last;
     double collect_ms, train_ms;
+    int bricks;               /* bricks the network corrects */
+    int64_t cells;            /* their active cells */
 } pcfd_learned_stats;
 
 /* Advance a copy of the current state by one transport step, solve its projection with plain
- * MGPCG from zero, and store the post-V-cycle residuals r'_k = r_k - A M_mg(r_k) of its first
+ * MGPCG warm-started from the current pressure (as a step does), and store the post-V-cycle
+ * residuals r'_k = r_k - A M_mg(r_k) of its first
  * `max_pairs` iterations (with their normalisation sum r'^2/h).  The solver state is left
  * unchanged.  Residuals beyond learned_capacity replace the oldest. */
 pcfd_status pcfd_learned_collect(pcfd_solver* solver, int max_pairs, pcfd_learned_stats* stats);

@@ -67,7 +67,8 @@ NN_BIASES = 4 * NN_C
 
 class LearnedStats(C.Structure):
     _fields_ = [("pairs", C.c_int), ("train_steps", C.c_int), ("loss_first", C.c_double),
-                ("loss_last", C.c_double), ("collect_ms", C.c_double), ("train_ms", C.c_double)]
+                ("loss_last", C.c_double), ("collect_ms", C.c_double), ("train_ms", C.c_double),
+                ("bricks", C.c_int), ("cells", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -77,6 +78,7 @@ class StepStats(C.Structure):
     _fields_ = [("dt", C.c_double), ("physical_time", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
                 ("relative_residual", C.c_double), ("max_abs_velocity", C.c_double), ("cfl_rate", C.c_double),
                 ("max_abs_divergence", C.c_double), ("diffusion_substeps", C.c_int), ("fallback_used", C.c_int),
+                ("learned_used", C.c_int), ("learned_rejected", C.c_int),
                 ("advect_ms", C.c_double), ("project_ms", C.c_double), ("step_ms", C.c_double)]
 
     def as_dict(self):

@@ -198,6 +198,29 @@ private:
         LAUNCH(k_nn_train_bwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_gy_, nn_sb_, nn_scratch_, nn_g_);
     }
     void setup_learned(const Topology& T);
+    // Validation of a freshly trained network against recent plain projections: the first
+    // kTrial learned projections after training are timed; if their mean exceeds the mean of
+    // the last (up to four) projections without the network, the network is switched off.
+    static constexpr int kTrial = 3;
+    void validate_learned(bool used, float project_ms) {
+        if (!nn_) return;
+        if (!used) {
+            plain_ms_.push_back(project_ms);
+            if (plain_ms_.size() > 4) plain_ms_.erase(plain_ms_.begin());
+            return;
+        }
+        if (trial_left_ <= 0) return;
+        trial_ms_ += project_ms;
+        if (--trial_left_ > 0 || plain_ms_.empty()) return;
+        double plain = 0.0;
+        for (double x : plain_ms_) plain += x;
+        plain /= static_cast<double>(plain_ms_.size());
+        if (trial_ms_ / kTrial > plain) {
+            nn_on_ = false;
+            nn_rejected_ = true;
+            set_nn_off(true);
+        }
+    }
     bool pcg_loop(bool watch_stall);
     void reconstruct();
     void run_projection(bool warm, pcfd_step_stats* st);
@@ -273,6 +296,10 @@ private:
     std::vector<double*> nn_rs_;  // recorded post-V-cycle residuals
     std::vector<double> nn_inv_norm_;
     int nn_cap_ = 0, nn_count_ = 0, nn_next_ = 0, nn_cursor_ = 0, nn_t_ = 0;
+    std::vector<double> plain_ms_;  // recent projection times without the network
+    int trial_left_ = 0;            // learned projections still to validate
+    double trial_ms_ = 0.0;
+    bool nn_rejected_ = false;
     R* snap_v_[3] = {};
     R* snap_f_[3] = {};
     R* snap_fe_ = nullptr;
@@ -433,7 +460,7 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
 template <class R>
 void SolverImpl<R>::setup_learned(const Topology& T) {
     if (lv_.empty()) throw std::invalid_argument("the learned preconditioner needs a multigrid hierarchy");
-    if (opt_.learned_bricks < 0 || opt_.learned_bricks > 1) throw std::invalid_argument("learned_bricks must be 0 or 1");
+    if (opt_.learned_bricks < 0 || opt_.learned_bricks > 2) throw std::invalid_argument("learned_bricks must be 0, 1 or 2");
     if (opt_.learned_capacity < 0) throw std::invalid_argument("learned_capacity must be non-negative");
     nn_ = true;
     nn_cap_ = opt_.learned_capacity > 0 ? opt_.learned_capacity : 16;
@@ -468,15 +495,18 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
     nn_loss_ = alloc<double>(1);
     // bricks the network corrects
     std::vector<int> list;
-    if (opt_.learned_bricks == 1) {
+    if (opt_.learned_bricks == 1 || opt_.learned_bricks == 2) {
         std::vector<uint8_t> sel(nb_, 0);
         for (int b = 0; b < nb_; ++b) {
-            if (T.brick_special_off[b + 1] == T.brick_special_off[b]) continue;
+            bool hit = false;
+            for (int k = T.brick_special_off[b]; k < T.brick_special_off[b + 1] && !hit; ++k)
+                hit = opt_.learned_bricks == 1 || T.special_patch[k] >= 0;
+            if (!hit) continue;
             sel[b] = 1;
             for (int f = 0; f < 6; ++f) if (T.brick_nbr[6 * b + f] >= 0) sel[T.brick_nbr[6 * b + f]] = 1;
         }
         for (int b = 0; b < nb_; ++b) if (sel[b]) list.push_back(b);
-        if (list.empty()) throw std::invalid_argument("learned_bricks = 1 selects no brick (no special cells)");
+        if (list.empty()) throw std::invalid_argument("learned_bricks selects no brick (no special or cut cells)");
         // block index of brick b in a brick-per-block launch of nb_ blocks (inverse of the
         // XCD-aware order of brick_of_block), where the V-cycle's last kernel put its partials
         const int per = nb_ >> 3, rem = nb_ & 7;
@@ -700,6 +730,7 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
         st->relative_residual = host_s_->relres;
         st->max_abs_divergence = host_s_->maxdiv;
         st->fallback_used = fell_back ? 1 : 0;
+        st->learned_used = learned ? 1 : 0;
     }
 }
 
@@ -802,6 +833,9 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     local.relative_residual = ps.relative_residual;
     local.max_abs_divergence = ps.max_abs_divergence;
     local.fallback_used = ps.fallback_used;
+    local.learned_used = ps.learned_used;
+    validate_learned(ps.learned_used != 0, p_ms);
+    local.learned_rejected = nn_rejected_ ? 1 : 0;
     local.advect_ms = a_ms;
     local.project_ms = p_ms;
     local.step_ms = s_ms;
@@ -1039,7 +1073,7 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
         advect(0.0, &dt, nullptr);
         LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
         finalize(FIN_BNORM, nb_);
-        HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+        // warm start from the current pressure, as the step's projection does
         set_nn_off(true);
         LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
         finalize(FIN_RESID, nb_);
@@ -1085,6 +1119,8 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
         st->pairs = nn_count_;
         st->train_steps = nn_t_;
         st->collect_ms = ms;
+        st->bricks = nn_nlist_;
+        st->cells = nn_cells_;
     }
 }
 
@@ -1119,6 +1155,9 @@ void SolverImpl<R>::learned_train(int steps, int batch, double lr, pcfd_learned_
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipEventSynchronize(ev_[1]));
     nn_on_ = true;
+    nn_rejected_ = false;
+    trial_left_ = kTrial;
+    trial_ms_ = 0.0;
     set_nn_off(false);
     if (st) {
         std::memset(st, 0, sizeof(*st));
@@ -1129,6 +1168,8 @@ void SolverImpl<R>::learned_train(int steps, int batch, double lr, pcfd_learned_
         st->loss_first = first;
         st->loss_last = last;
         st->train_ms = ms;
+        st->bricks = nn_nlist_;
+        st->cells = nn_cells_;
     }
 }
 
@@ -1156,6 +1197,8 @@ template <class R>
 void SolverImpl<R>::learned_enable(bool on) {
     if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
     nn_on_ = on;
+    trial_left_ = 0;  // an explicit choice ends any validation in progress
+    nn_rejected_ = false;
     set_nn_off(!on);
     HIP_CHECK(hipStreamSynchronize(stream_));
 }
