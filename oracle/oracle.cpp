@@ -76,6 +76,7 @@ typedef struct oracle_topology {
     const double* const* mg_coef;
     const double* const* mg_diag;
     const double* coarse_inv;        /* may be NULL */
+    const double* centroid;          /* 3 per dof: fluid centroid (Smagorinsky gradients) */
 } oracle_topology;
 
 }
@@ -98,7 +99,11 @@ struct Oracle {
     std::vector<double> earea, ecoef, ew;
     std::vector<int> patch_dof, lsq_off, lsq_dof;
     std::vector<double> patch_S, patch_x, lsq_beta;
+    std::vector<double> centroid;
     double finest_h = 0, max_diff_coef = 0;
+    // Smagorinsky least-squares gradient stencils of the special cells (built on first use)
+    std::vector<int> sg_off, sg_nbr;
+    std::vector<double> sg_wd, sg_inv;
     struct Level { int64_t n; std::vector<int> parent, row_off, col; std::vector<double> coef, diag; };
     std::vector<Level> mg;
     std::vector<double> coarse_inv;
@@ -215,13 +220,170 @@ struct Oracle {
             vx[a] += d[0][a] * iv; vy[a] += d[1][a] * iv; vz[a] += d[2][a] * iv;
         }
     }
-    int diffusion_substeps(double dt) const {
+    // ---- diffusion and the Smagorinsky model ------------------------------------------------
+    // Symmetric 3x3 pseudo-inverse: cyclic Jacobi eigen-decomposition (Golub & Van Loan,
+    // Alg. 8.4.3), keeping the modes whose eigenvalue exceeds 1e-4 of the largest (the rank
+    // rule of amr_advection.cu:822-831).
+    static void pinv3(const double* A, double* out) {
+        double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) a[r][c] = A[3 * r + c];
+        for (int sweep = 0; sweep < 64; ++sweep) {
+            const double off = std::abs(a[0][1]) + std::abs(a[0][2]) + std::abs(a[1][2]);
+            const double dia = std::abs(a[0][0]) + std::abs(a[1][1]) + std::abs(a[2][2]);
+            if (!(off > 1e-300) || off <= 1e-17 * dia) break;
+            for (int p = 0; p < 2; ++p)
+                for (int q = p + 1; q < 3; ++q) {
+                    if (a[p][q] == 0.0) continue;
+                    const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                    const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
+                    const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                    for (int k = 0; k < 3; ++k) {  // A <- A J
+                        const double akp = a[k][p], akq = a[k][q];
+                        a[k][p] = c * akp - s * akq;
+                        a[k][q] = s * akp + c * akq;
+                    }
+                    for (int k = 0; k < 3; ++k) {  // A <- J^T A
+                        const double apk = a[p][k], aqk = a[q][k];
+                        a[p][k] = c * apk - s * aqk;
+                        a[q][k] = s * apk + c * aqk;
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        const double vkp = v[k][p], vkq = v[k][q];
+                        v[k][p] = c * vkp - s * vkq;
+                        v[k][q] = s * vkp + c * vkq;
+                    }
+                }
+        }
+        const double lmax = std::max({std::abs(a[0][0]), std::abs(a[1][1]), std::abs(a[2][2])});
+        for (int k = 0; k < 9; ++k) out[k] = 0.0;
+        for (int m = 0; m < 3; ++m) {
+            const double l = a[m][m];
+            if (!(l > 1e-4 * lmax)) continue;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) out[3 * r + c] += v[r][m] * v[c][m] / l;
+        }
+    }
+    // Least-squares gradient stencils of the special cells (amr_advection.cu:834):
+    // neighbours across the open structured faces (area h^2) and the compact edges (open
+    // area), weight area / |x_b - x_a|^2; the normal matrix is pseudo-inverted.
+    void build_gradient_stencils() {
+        const int ns = static_cast<int>(special_volume.size());
+        sg_off.assign(ns + 1, 0);
+        sg_nbr.clear(); sg_wd.clear(); sg_inv.assign(9 * static_cast<size_t>(ns), 0.0);
+        std::vector<std::vector<int>> inc(ns);
+        for (size_t e = 0; e < ea.size(); ++e) {
+            if (special_of[ea[e]] >= 0) inc[special_of[ea[e]]].push_back(static_cast<int>(e));
+            if (special_of[eb[e]] >= 0) inc[special_of[eb[e]]].push_back(~static_cast<int>(e));
+        }
+        std::vector<int64_t> dof_of(ns);
+        for (int64_t a = 0; a < n; ++a) if (special_of[a] >= 0) dof_of[special_of[a]] = a;
+        for (int sp = 0; sp < ns; ++sp) {
+            const int64_t a = dof_of[sp];
+            double N[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            auto add = [&](int64_t b, double area) {
+                double d[3];
+                for (int c = 0; c < 3; ++c) d[c] = centroid[3 * b + c] - centroid[3 * a + c];
+                const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (!(area > 0) || !(d2 > 1e-20)) throw std::runtime_error("invalid gradient stencil geometry");
+                const double w = area / d2;
+                sg_nbr.push_back(static_cast<int>(b));
+                for (int c = 0; c < 3; ++c) sg_wd.push_back(w * d[c]);
+                for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) N[3 * r + c] += w * d[r] * d[c];
+            };
+            const double h = h_of(a);
+            for (int axis = 0; axis < 3; ++axis)
+                for (int sgn = -1; sgn <= 1; sgn += 2) {
+                    const int64_t b = snbr(a, axis, sgn);
+                    if (b >= 0) add(b, h * h);
+                }
+            for (int raw : inc[sp]) {
+                const int e = raw >= 0 ? raw : ~raw;
+                add(raw >= 0 ? eb[e] : ea[e], earea[e]);
+            }
+            sg_off[sp + 1] = static_cast<int>(sg_nbr.size());
+            pinv3(N, &sg_inv[9 * static_cast<size_t>(sp)]);
+        }
+    }
+    static double strain_rate(const double* g) {  // sqrt(2 S:S), S = sym(grad v)
+        const double sxx = g[0], syy = g[4], szz = g[8];
+        const double sxy = 0.5 * (g[1] + g[3]), sxz = 0.5 * (g[2] + g[6]), syz = 0.5 * (g[5] + g[7]);
+        return std::sqrt(std::max(0.0, 2.0 * (sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz))));
+    }
+    // Cell viscosity nu_mol + (Cs V^(1/3))^2 |S| (amr_advection.cu:198-204 regular cells,
+    // central or one-sided differences; :219-223 special cells, least squares)
+    void viscosity(std::vector<double>& nu) {
+        nu.assign(n, 0.0);
+        const double cs = opt.smagorinsky_cs;
+        if (cs > 0 && sg_off.empty()) build_gradient_stencils();
+        for (int64_t a = 0; a < n; ++a) {
+            if (!active(a)) continue;
+            nu[a] = opt.nu;
+            if (!(cs > 0)) continue;
+            double g[9];
+            const int sp = special_of[a];
+            if (sp < 0) {
+                const double h = h_of(a);
+                for (int c = 0; c < 3; ++c)
+                    for (int axis = 0; axis < 3; ++axis) {
+                        const int64_t lo = snbr(a, axis, -1), up = snbr(a, axis, 1);
+                        double v = 0.0;
+                        if (lo >= 0 && up >= 0) v = (comp(c, up) - comp(c, lo)) / (2.0 * h);
+                        else if (up >= 0) v = (comp(c, up) - comp(c, a)) / h;
+                        else if (lo >= 0) v = (comp(c, a) - comp(c, lo)) / h;
+                        g[3 * c + axis] = v;
+                    }
+            } else {
+                double rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int k = sg_off[sp]; k < sg_off[sp + 1]; ++k)
+                    for (int c = 0; c < 3; ++c) {
+                        const double dv = comp(c, sg_nbr[k]) - comp(c, a);
+                        for (int axis = 0; axis < 3; ++axis) rhs[3 * c + axis] += sg_wd[3 * k + axis] * dv;
+                    }
+                const double* M = &sg_inv[9 * static_cast<size_t>(sp)];
+                for (int c = 0; c < 3; ++c)
+                    for (int r = 0; r < 3; ++r)
+                        g[3 * c + r] = M[3 * r] * rhs[3 * c] + M[3 * r + 1] * rhs[3 * c + 1] + M[3 * r + 2] * rhs[3 * c + 2];
+            }
+            const double L = cs * std::cbrt(vol(a));
+            nu[a] += L * L * strain_rate(g);
+        }
+    }
+    // max over cells of sum_faces conductance (nu_a + nu_b)/2 / V (amr_advection.cu:1343)
+    double diffusion_rate(const std::vector<double>& nu) const {
+        std::vector<double> rate(n, 0.0);
+        for (int64_t a = 0; a < n; ++a) {
+            if (!active(a)) continue;
+            const double h = h_of(a);
+            for (int axis = 0; axis < 3; ++axis) {
+                const int64_t b = snbr(a, axis, 1);
+                if (b < 0) continue;
+                const double k = h * 0.5 * (nu[a] + nu[b]);
+                rate[a] += k; rate[b] += k;
+            }
+        }
+        for (size_t e = 0; e < ea.size(); ++e) {
+            const double k = ecoef[e] * 0.5 * (nu[ea[e]] + nu[eb[e]]);
+            rate[ea[e]] += k; rate[eb[e]] += k;
+        }
+        double mx = 0.0;
+        for (int64_t a = 0; a < n; ++a) if (active(a)) mx = std::max(mx, rate[a] / vol(a));
+        return mx;
+    }
+    int diffusion_substeps(double dt) {
+        if (opt.smagorinsky_cs > 0) {
+            viscosity(nu_field);
+            return std::max(1, static_cast<int>(std::ceil(dt * diffusion_rate(nu_field) / 0.7)));
+        }
         if (!(opt.nu > 0)) return 0;
         return std::max(1, static_cast<int>(std::ceil(dt * opt.nu * max_diff_coef / 0.7)));
     }
+    std::vector<double> nu_field;
+    // explicit diffusion; with Cs > 0 the face coefficient is (nu_a + nu_b)/2 of the field
+    // computed once per step from the transported velocity (amr_advection.cu:1342-1344)
     void diffuse(double dt) {
         const int s = diffusion_substeps(dt);
         if (s == 0) return;
+        const bool var = opt.smagorinsky_cs > 0;
         const double dts = dt / s;
         for (int it = 0; it < s; ++it) {
             std::vector<double> d[3] = {std::vector<double>(n, 0.0), std::vector<double>(n, 0.0), std::vector<double>(n, 0.0)};
@@ -231,7 +393,7 @@ struct Oracle {
                 for (int axis = 0; axis < 3; ++axis) {
                     const int64_t b = snbr(a, axis, 1);
                     if (b < 0) continue;
-                    const double k = dts * opt.nu * h;
+                    const double k = dts * (var ? 0.5 * (nu_field[a] + nu_field[b]) : opt.nu) * h;
                     for (int c = 0; c < 3; ++c) {
                         const double m = k * (comp(c, b) - comp(c, a));
                         d[c][a] += m; d[c][b] -= m;
@@ -239,7 +401,7 @@ struct Oracle {
                 }
             }
             for (size_t e = 0; e < ea.size(); ++e) {
-                const double k = dts * opt.nu * ecoef[e];
+                const double k = dts * (var ? 0.5 * (nu_field[ea[e]] + nu_field[eb[e]]) : opt.nu) * ecoef[e];
                 for (int c = 0; c < 3; ++c) {
                     const double m = k * (comp(c, eb[e]) - comp(c, ea[e]));
                     d[c][ea[e]] += m; d[c][eb[e]] -= m;
@@ -555,6 +717,8 @@ void* oracle_create(const oracle_topology* t, const pcfd_options* o) {
         s->lsq_beta.assign(t->patch_lsq_beta, t->patch_lsq_beta + s->lsq_off.back());
         s->finest_h = t->finest_h;
         s->max_diff_coef = t->max_diffusion_coefficient;
+        if (t->centroid) s->centroid.assign(t->centroid, t->centroid + 3 * t->ndof);
+        if (o->smagorinsky_cs > 0 && s->centroid.empty()) throw std::invalid_argument("the Smagorinsky model needs cell centroids");
         s->mg.resize(t->mg_levels);
         for (int m = 0; m < t->mg_levels; ++m) {
             auto& L = s->mg[m];
@@ -735,6 +899,19 @@ int oracle_dense_operator(void* h, double* A) {
     return PCFD_OK;
 }
 double oracle_outflow_rate(void* h) { return static_cast<Oracle*>(h)->outflow_rate(); }
+/* Cell viscosity nu_mol + (Cs V^(1/3))^2 |S| of the current velocity (0 on inactive cells). */
+int oracle_viscosity(void* h, double* nu) {
+    try {
+        auto* s = static_cast<Oracle*>(h);
+        std::vector<double> v;
+        s->viscosity(v);
+        std::memcpy(nu, v.data(), v.size() * sizeof(double));
+        return PCFD_OK;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return PCFD_EINVAL;
+    }
+}
 int oracle_divergence(void* h, double* div) {
     auto* s = static_cast<Oracle*>(h);
     std::vector<double> d;

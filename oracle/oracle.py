@@ -40,7 +40,8 @@ class OracleTopology(C.Structure):
                 ("patch_lsq_dof", C.c_void_p), ("patch_lsq_beta", C.c_void_p), ("finest_h", C.c_double),
                 ("max_diffusion_coefficient", C.c_double), ("mg_levels", C.c_int), ("mg_n", C.c_void_p),
                 ("mg_parent", C.c_void_p), ("mg_row_off", C.c_void_p), ("mg_col", C.c_void_p),
-                ("mg_coef", C.c_void_p), ("mg_diag", C.c_void_p), ("coarse_inv", C.c_void_p)]
+                ("mg_coef", C.c_void_p), ("mg_diag", C.c_void_p), ("coarse_inv", C.c_void_p),
+                ("centroid", C.c_void_p)]
 
 
 def _ptr(a):
@@ -70,7 +71,7 @@ class Oracle:
             "oracle_get_faces": [P, dp, dp, dp, dp], "oracle_set_faces": [P, dp, dp, dp, dp],
             "oracle_mark_initialized": [P, C.c_double],
             "oracle_mg_primitive": [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int],
-            "oracle_dense_operator": [P, dp], "oracle_divergence": [P, dp],
+            "oracle_dense_operator": [P, dp], "oracle_divergence": [P, dp], "oracle_viscosity": [P, dp],
         }.items():
             f = getattr(lib, name)
             f.restype = C.c_int
@@ -86,7 +87,8 @@ class Oracle:
         keep = {}
         for name in ["level_h", "brick_level", "brick_flags", "brick_nbr", "mask", "special_dof", "special_volume",
                      "special_patch", "edge_a", "edge_b", "edge_axis", "edge_area", "edge_coef", "edge_w",
-                     "patch_dof", "patch_S", "patch_x", "patch_lsq_off", "patch_lsq_dof", "patch_lsq_beta"]:
+                     "patch_dof", "patch_S", "patch_x", "patch_lsq_off", "patch_lsq_dof", "patch_lsq_beta",
+                     "centroid"]:
             keep[name] = np.ascontiguousarray(t.array(name))
         nl = t.mg_levels
         mg_n = np.array([t.mg_n(m) for m in range(nl)], dtype=np.int64)
@@ -224,6 +226,14 @@ class Oracle:
         d = np.empty(self.n)
         self.lib.oracle_divergence(self.h, d.ctypes.data_as(C.POINTER(C.c_double)))
         return d
+
+    def viscosity(self):
+        """Cell viscosity nu_mol + (Cs V^(1/3))^2 |S| of the current velocity."""
+        nu = np.empty(self.n)
+        rc = self.lib.oracle_viscosity(self.h, nu.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return nu
 
     def outflow_rate(self):
         return self.lib.oracle_outflow_rate(self.h)

@@ -21,7 +21,7 @@ constexpr uint8_t kXMIN = 1, kXMAX = 2, kYMIN = 4, kYMAX = 8, kZMIN = 16, kZMAX 
 // Scalars the Krylov loop keeps on the device (no host round trip per iteration).
 struct DevScalars {
     double rz, rz_prev, pAp, alpha, beta, rr, bnorm, relres;
-    double dt, rate, maxdiv, vmax, rpz;
+    double dt, rate, maxdiv, vmax, diff_rate;
     double loads[6];
     int iter, done, converged, breakdown, zero_rhs, pad0;
     unsigned int ticket[16];

@@ -161,14 +161,15 @@ enum FinalizeOp : int {
     FIN_UPDATE = 6,   // ||r||, iteration count, convergence / iteration limit
     FIN_RZ = 7,       // r.z (and r_prev.z) -> beta, rz
     FIN_RZ_INITIAL = 8,
-    FIN_LOADS = 9     // six load sums
+    FIN_LOADS = 9,    // six load sums
+    FIN_DIFFRATE = 10 // max explicit diffusion rate (Smagorinsky substeps)
 };
 
 __global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restrict__ partials, int nparts,
                                                    DevScalars* S, double cfl, double rate_floor, double rtol,
                                                    int max_iterations, int flexible) {
     if ((op == FIN_RESID || op == FIN_APPLY || op == FIN_UPDATE || op == FIN_RZ || op == FIN_RZ_INITIAL) && S->done) return;
-    const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX;
+    const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE;
     const int arrays = op == FIN_LOADS ? 6 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
     double t[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts, nparts, take_max);
@@ -211,6 +212,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restr
         case FIN_LOADS:
             for (int c = 0; c < 6; ++c) S->loads[c] = t[c];
             break;
+        case FIN_DIFFRATE: S->diff_rate = t[0]; break;
         default: break;
     }
 }

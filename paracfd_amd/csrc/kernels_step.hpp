@@ -190,6 +190,144 @@ __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict
 }
 
 // ---------------------------------------------------------------------------------------
+// Smagorinsky model (amr_advection.cu:198-252, 1339-1345): cell viscosity
+// nu = nu_mol + (Cs V^(1/3))^2 sqrt(2 S:S) from the transported velocity, once per step;
+// regular cells differentiate across their open structured faces (central, or one-sided
+// where one neighbour is missing), special cells use a least-squares gradient whose
+// pseudo-inverted normal matrix and weighted displacements are precomputed per special cell
+// (stencil CSR sg_off / sg_nbr / sg_wd, inverses sg_inv).
+// ---------------------------------------------------------------------------------------
+struct SmagView {
+    const int* off;       // nspecial + 1
+    const int* nbr;       // neighbour dof per stencil entry
+    const double* wd;     // 3 per entry: weight * displacement
+    const double* inv;    // 9 per special cell
+};
+
+__device__ __forceinline__ double strain_rate(const double* g) {
+    const double sxx = g[0], syy = g[4], szz = g[8];
+    const double sxy = 0.5 * (g[1] + g[3]), sxz = 0.5 * (g[2] + g[6]), syz = 0.5 * (g[5] + g[7]);
+    return sqrt(fmax(0.0, 2.0 * (sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz))));
+}
+
+template <class R>
+__global__ __launch_bounds__(512) void k_viscosity(MeshView M, FieldsView<R> F, SmagView G, double nu_mol, double cs,
+                                                   R* __restrict__ nu) {
+    const int b = brick_of_block(), q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    if (!(m & kActive)) {
+        nu[a] = R(0);
+        return;
+    }
+    const double h = cell_h(M, b);
+    double g[9];
+    if (sp < 0) {
+        for (int axis = 0; axis < 3; ++axis) {
+            const int64_t lo = (m & (kMX << axis)) ? neighbor_dof(M, b, q, axis, -1) : -1;
+            const int64_t up = (m & (kPX << axis)) ? neighbor_dof(M, b, q, axis, 1) : -1;
+            for (int c = 0; c < 3; ++c) {
+                const R* v = F.v[c];
+                double d = 0.0;
+                if (lo >= 0 && up >= 0) d = (double(v[up]) - double(v[lo])) / (2.0 * h);
+                else if (up >= 0) d = (double(v[up]) - double(v[a])) / h;
+                else if (lo >= 0) d = (double(v[a]) - double(v[lo])) / h;
+                g[3 * c + axis] = d;
+            }
+        }
+    } else {
+        double rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = G.off[sp]; k < G.off[sp + 1]; ++k) {
+            const int64_t o = G.nbr[k];
+            for (int c = 0; c < 3; ++c) {
+                const double dv = double(F.v[c][o]) - double(F.v[c][a]);
+                for (int axis = 0; axis < 3; ++axis) rhs[3 * c + axis] += G.wd[3 * k + axis] * dv;
+            }
+        }
+        const double* I = G.inv + 9 * static_cast<int64_t>(sp);
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) g[3 * c + r] = I[3 * r] * rhs[3 * c] + I[3 * r + 1] * rhs[3 * c + 1] + I[3 * r + 2] * rhs[3 * c + 2];
+    }
+    const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
+    const double L = cs * cbrt(V);
+    nu[a] = R(nu_mol + L * L * strain_rate(g));
+}
+
+// per-cell explicit diffusion rate sum_faces conductance (nu_a + nu_b)/2 / V, block maxima
+template <class R>
+__global__ __launch_bounds__(512) void k_diff_rate(MeshView M, const R* __restrict__ nu, double* partials) {
+    const int b = brick_of_block(), q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    double rate = 0.0;
+    if (m & kActive) {
+        const double h = cell_h(M, b), na = double(nu[a]);
+        double s = 0.0;
+        for (int axis = 0; axis < 3; ++axis)
+            for (int sg = -1; sg <= 1; sg += 2) {
+                if (!(m & ((sg > 0 ? kPX : kMX) << axis))) continue;
+                s += h * 0.5 * (na + double(nu[neighbor_dof(M, b, q, axis, sg)]));
+            }
+        double V = h * h * h;
+        if (sp >= 0) {
+            V = M.special_volume[sp];
+            for (int j = M.special_inc_off[sp]; j < M.special_inc_off[sp + 1]; ++j) {
+                const int raw = M.special_inc[j];
+                const int e = raw >= 0 ? raw : ~raw;
+                const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
+                s += M.ecoef[e] * 0.5 * (na + double(nu[o]));
+            }
+        }
+        rate = s / V;
+    }
+    store_partial(partials, block_max<512>(rate));
+}
+
+// one explicit diffusion substep with the cell viscosity field (face coefficient mean of
+// the two cells' viscosities)
+template <class R>
+__global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
+                                                     const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
+                                                     R* __restrict__ oz, const R* __restrict__ nu, double dts) {
+    const int b = brick_of_block(), q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const R* in[3] = {vx, vy, vz};
+    R* out[3] = {ox, oy, oz};
+    if (!(m & kActive)) {
+        for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
+        return;
+    }
+    const R h = R(cell_h(M, b));
+    const R half = R(0.5), na = nu[a];
+    R va[3], d[3] = {R(0), R(0), R(0)};
+    for (int c = 0; c < 3; ++c) va[c] = in[c][a];
+    for (int axis = 0; axis < 3; ++axis)
+        for (int s = -1; s <= 1; s += 2) {
+            if (!(m & ((s > 0 ? kPX : kMX) << axis))) continue;
+            const int64_t o = neighbor_dof(M, b, q, axis, s);
+            const R k = R(dts) * (half * (na + nu[o])) * h;
+            for (int c = 0; c < 3; ++c) d[c] += k * (in[c][o] - va[c]);
+        }
+    R V = h * h * h;
+    if (sp >= 0) {
+        V = R(M.special_volume[sp]);
+        for (int j = M.special_inc_off[sp]; j < M.special_inc_off[sp + 1]; ++j) {
+            const int raw = M.special_inc[j];
+            const int e = raw >= 0 ? raw : ~raw;
+            const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
+            const R ke = R(dts) * (half * (na + nu[o])) * R(M.ecoef[e]);
+            for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
+        }
+    }
+    const R iv = R(1) / V;
+    for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
+}
+
+// ---------------------------------------------------------------------------------------
 // Face reconstruction (amr_advection.cu:257-293): structured face = mean of the two cells'
 // normal components; x-max outlet face = interior cell (zero gradient); compact edges use
 // the distance weight.

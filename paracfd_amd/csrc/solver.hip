@@ -22,6 +22,7 @@
 #include "kernels_solve.hpp"
 #include "kernels_step.hpp"
 #include "learned.hpp"
+#include "lsq3.hpp"
 #include "topology.hpp"
 
 const pcfd::Topology& pcfd_topology_ref(const pcfd_topology* t);
@@ -198,6 +199,7 @@ private:
         LAUNCH(k_nn_train_bwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_gy_, nn_sb_, nn_scratch_, nn_g_);
     }
     void setup_learned(const Topology& T);
+    void setup_smagorinsky(const Topology& T);
     // Validation of a freshly trained network against recent plain projections: the first
     // kTrial learned projections after training are timed; if their mean exceeds the mean of
     // the last (up to four) projections without the network, the network is switched off.
@@ -273,6 +275,9 @@ private:
     bool fused_ = true;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
+    // Smagorinsky model (smagorinsky_cs > 0)
+    R* nu_ = nullptr;
+    SmagView smag_{};
     // learned preconditioner state (PCFD_PC_LEARNED)
     bool nn_ = false;             // created with the learned preconditioner
     bool nn_on_ = false;          // network enabled (trained or weights set)
@@ -314,6 +319,8 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
         throw std::invalid_argument("invalid solver options");
     if (!(opt_.smoother_weight > 0.0 && opt_.smoother_weight < 2.0))
         throw std::invalid_argument("smoother_weight must lie in (0, 2)");
+    if (!(opt_.smagorinsky_cs >= 0.0) || !std::isfinite(opt_.smagorinsky_cs))
+        throw std::invalid_argument("smagorinsky_cs must be a non-negative number");
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_s_), sizeof(DevScalars), hipHostMallocDefault));
@@ -452,7 +459,60 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
         LAUNCH(k_diag0<float>, nb_, 512, M_, static_cast<float*>(diag_));
     }
     if (opt_.preconditioner == PCFD_PC_LEARNED) setup_learned(T);
+    if (opt_.smagorinsky_cs > 0.0) setup_smagorinsky(T);
     HIP_CHECK(hipDeviceSynchronize());
+}
+
+// Least-squares gradient stencils of the special cells (amr_advection.cu:834): neighbours
+// across the open structured faces (area h^2) and the compact edges (open area), weight
+// area / |x_b - x_a|^2 on the fluid centroids; the 3x3 normal matrix is pseudo-inverted.
+template <class R>
+void SolverImpl<R>::setup_smagorinsky(const Topology& T) {
+    const int ns = static_cast<int>(T.special_dof.size());
+    std::vector<int> off(ns + 1, 0), nbr;
+    std::vector<double> wd, inv(9 * static_cast<size_t>(ns), 0.0);
+    for (int sp = 0; sp < ns; ++sp) {
+        const int64_t a = T.special_dof[sp];
+        const int b = static_cast<int>(a / 512), q = static_cast<int>(a % 512);
+        const double h = T.level_h[T.brick_level[b]];
+        double N[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        auto add = [&](int64_t o, double area) {
+            double d[3];
+            for (int c = 0; c < 3; ++c) d[c] = T.centroid[3 * o + c] - T.centroid[3 * a + c];
+            const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(area > 0) || !(d2 > 1e-20)) throw std::logic_error("invalid gradient stencil geometry");
+            const double w = area / d2;
+            nbr.push_back(static_cast<int>(o));
+            for (int c = 0; c < 3; ++c) wd.push_back(w * d[c]);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) N[3 * r + c] += w * d[r] * d[c];
+        };
+        const uint8_t m = T.mask[a];
+        for (int axis = 0; axis < 3; ++axis)
+            for (int sg = -1; sg <= 1; sg += 2) {
+                if (!(m & ((sg > 0 ? kPX : kMX) << axis))) continue;
+                int c[3] = {q & 7, (q >> 3) & 7, q >> 6};
+                int bb = b;
+                c[axis] += sg;
+                if (c[axis] < 0 || c[axis] > 7) {
+                    bb = T.brick_nbr[6 * b + 2 * axis + (sg > 0 ? 1 : 0)];
+                    c[axis] = sg > 0 ? 0 : 7;
+                }
+                add(static_cast<int64_t>(bb) * 512 + c[0] + 8 * c[1] + 64 * c[2], h * h);
+            }
+        for (int k = T.special_inc_off[sp]; k < T.special_inc_off[sp + 1]; ++k) {
+            const int raw = T.special_inc[k];
+            const int e = raw >= 0 ? raw : ~raw;
+            add(raw >= 0 ? T.edge_b[e] : T.edge_a[e], T.edge_area[e]);
+        }
+        off[sp + 1] = static_cast<int>(nbr.size());
+        symmetric_pinv3(N, &inv[9 * static_cast<size_t>(sp)]);
+    }
+    smag_.off = upload(off);
+    smag_.nbr = upload(nbr);
+    smag_.wd = upload(wd);
+    smag_.inv = upload(inv);
+    nu_ = alloc<R>(n_);
 }
 
 // Network buffers, brick selection and the initial weights: He-normal (seed 1234) for
@@ -770,7 +830,21 @@ void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
     LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_);
     for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
     int subs = 0;
-    if (opt_.nu > 0) {
+    if (opt_.smagorinsky_cs > 0) {
+        // cell viscosity from the transported velocity, substeps from the largest rate
+        LAUNCH(k_viscosity<R>, nb_, 512, M_, fields(), smag_, opt_.nu, opt_.smagorinsky_cs, nu_);
+        LAUNCH(k_diff_rate<R>, nb_, 512, M_, nu_, partials_);
+        finalize(FIN_DIFFRATE, nb_);
+        sync_scalars();
+        const double rate = host_s_->diff_rate;
+        if (!std::isfinite(rate)) throw std::domain_error("non-finite eddy viscosity");
+        subs = std::max(1, static_cast<int>(std::ceil(dt * rate / 0.7)));
+        const double dts = dt / subs;
+        for (int s = 0; s < subs; ++s) {
+            LAUNCH(k_diffuse_var<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], nu_, dts);
+            for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
+        }
+    } else if (opt_.nu > 0) {
         subs = std::max(1, static_cast<int>(std::ceil(dt * opt_.nu * max_diff_coef_ / 0.7)));
         const double dts = dt / subs;
         for (int s = 0; s < subs; ++s) {
@@ -1298,7 +1372,6 @@ pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* optio
     return guarded("pcfd_create", [&] {
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw pcfd::DeviceError("no HIP device");
-        if (options->smagorinsky_cs != 0.0) throw std::invalid_argument("the Smagorinsky model is not enabled in this build (Cs must be 0)");
         if (options->preconditioner < PCFD_PC_MG || options->preconditioner > PCFD_PC_LEARNED)
             throw std::invalid_argument("unknown preconditioner");
         auto s = std::make_unique<pcfd_solver>();

@@ -168,6 +168,28 @@ def test_fp64_step_matches_oracle(lib, oracle_built, case, kw, steps):
             assert rel_l2(g, o) <= 1e-10
 
 
+@pytest.mark.parametrize("case,kw", [
+    ("sphere", dict(cells_x=32, levels=2, band=0.3)),
+    ("naca", dict(digits=2412, alpha_deg=4.0, cells_x=24, levels=2, bands=(0.3,))),
+])
+def test_fp64_smagorinsky_step_matches_oracle(lib, oracle_built, case, kw):
+    """Eddy viscosity (central / one-sided differences on regular cells, least squares on
+    special cells), the diffusion substep count from the largest rate and the variable-
+    viscosity diffusion, through whole steps, at the FP64 bar."""
+    topo = make(case, **kw)
+    gpu, orc = pair(topo, opts(smagorinsky_cs=0.17, nu=1e-4))
+    gi, oi, hist = run_both(gpu, orc, 6)
+    for gs, os_ in hist:
+        assert gs["diffusion_substeps"] == os_.diffusion_substeps
+        assert gs["iterations"] == os_.iterations
+    act = (topo.array("mask") & 64) != 0
+    gx, gy, gz = gpu.download_cell_momentum_state()
+    ox, oy, oz, op = orc.state()
+    for g, o in ((gx, ox), (gy, oy), (gz, oz)):
+        assert rel_l2(g[act], o[act]) <= 1e-10
+    assert rel_l2(gpu.download_pressure()[act], op[act]) <= 1e-10
+
+
 def test_fp32_fields_track_oracle(lib, oracle_built):
     from paracfd_amd import _lib as L
     topo = make("sphere", cells_x=64)

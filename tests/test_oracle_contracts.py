@@ -341,3 +341,91 @@ def test_planted_mistake_in_transport_is_caught(oracle_built):
             face[f] = v[i] + 0.5 * (0.0 if a * b <= 0 else (min(a, b) if a >= 0 else max(a, b)))
     bad = v - 0.03 / 0.0625 * (face[1:] - face[:-1])
     assert C > 0.1 and np.max(np.abs(bad - good)) > 1e-4
+
+
+def structured_neighbours(t):
+    """(n, 6) structured neighbour dof (-1 where the face bit is clear): -x,+x,-y,+y,-z,+z."""
+    n = t.ndof
+    mask = t.array("mask").astype(np.int64)
+    nbr = t.array("brick_nbr").reshape(-1, 6)
+    idx = np.arange(n)
+    b, q = idx // 512, idx % 512
+    ijk = np.stack([q & 7, (q >> 3) & 7, q >> 6], axis=1)
+    out = -np.ones((n, 6), dtype=np.int64)
+    for axis in range(3):
+        for k, sgn in enumerate((-1, 1)):
+            bit = (1 << axis) if sgn > 0 else (8 << axis)
+            has = (mask & bit) != 0
+            c = ijk.copy()
+            c[:, axis] += sgn
+            bb = b.copy()
+            cross = (c[:, axis] < 0) | (c[:, axis] > 7)
+            bb[cross] = nbr[b[cross], 2 * axis + (1 if sgn > 0 else 0)]
+            c[cross, axis] = np.where(sgn > 0, 0, 7)
+            dof = bb * 512 + c[:, 0] + 8 * c[:, 1] + 64 * c[:, 2]
+            out[has, 2 * axis + k] = dof[has]
+    return out
+
+
+def smagorinsky_expected(G, cs, nu_mol, V):
+    S = 0.5 * (G + G.T)
+    return nu_mol + (cs * np.cbrt(V)) ** 2 * math.sqrt(2.0 * float((S * S).sum()))
+
+
+@pytest.mark.parametrize("kind", ["shear", "rotation"])
+def test_smagorinsky_viscosity_of_linear_fields(amr_topo, oracle_built, kind):
+    """nu = nu_mol + (Cs V^(1/3))^2 sqrt(2 S:S) is exact for a linear velocity on regular cells
+    whose neighbours sit at their cell centres (central and one-sided differences) and on
+    special cells with a full-rank least-squares stencil; a rigid rotation gives nu_mol."""
+    t = amr_topo
+    cs, nu_mol = 0.17, 2e-3
+    orc = Oracle(t, options(nu=nu_mol, smagorinsky_cs=cs))
+    centre, h, act = geometry(t)
+    cen = t.array("centroid").reshape(-1, 3)
+    if kind == "shear":
+        G = np.array([[0.3, 0.8, -0.2], [0.1, -0.5, 0.4], [0.6, 0.2, 0.2]])
+    else:
+        G = np.array([[0.0, -0.7, 0.3], [0.7, 0.0, -0.5], [-0.3, 0.5, 0.0]])
+    v = (cen @ G.T + np.array([0.4, -0.1, 0.2])) * act[:, None]
+    orc.set_state(v[:, 0], v[:, 1], v[:, 2], np.zeros(t.ndof))
+    nu = orc.viscosity()
+    assert np.all(nu[~act] == 0.0)
+    mask = t.array("mask")
+    special = (mask & 128) != 0
+    full = np.all(np.abs(cen - centre) < 1e-12, axis=1)
+    nb = structured_neighbours(t)
+    nb_ok = np.all((nb < 0) | full[np.maximum(nb, 0)], axis=1)
+    regular = act & ~special & nb_ok
+    want_reg = smagorinsky_expected(G, cs, nu_mol, h[regular] ** 3)
+    assert regular.sum() > 1000
+    assert np.max(np.abs(nu[regular] - want_reg)) < 1e-12
+    # special cells: least squares reproduces linear fields wherever the stencil has rank 3
+    sdof = t.array("special_dof")
+    want_sp = np.array([smagorinsky_expected(G, cs, nu_mol, V) for V in t.array("special_volume")])
+    close = np.abs(nu[sdof] - want_sp) < 1e-10 * np.maximum(1.0, np.abs(want_sp))
+    assert close.mean() > 0.95
+
+
+def test_smagorinsky_off_is_molecular(amr_topo, oracle_built):
+    t = amr_topo
+    orc = Oracle(t, options(nu=1e-3, smagorinsky_cs=0.0))
+    centre, h, act = geometry(t)
+    orc.set_state(*(np.sin(centre.T * 3.0) * act), np.zeros(t.ndof))
+    nu = orc.viscosity()
+    assert np.all(nu[act] == 1e-3) and np.all(nu[~act] == 0.0)
+
+
+def test_smagorinsky_step_adds_dissipation(amr_topo, oracle_built):
+    """A sheared field loses more kinetic energy per step with the eddy viscosity."""
+    t = amr_topo
+    centre, h, act = geometry(t)
+    ke = []
+    for cs in (0.0, 0.2):
+        orc = Oracle(t, options(nu=1e-3, smagorinsky_cs=cs, speed=0.0))
+        v = np.stack([np.sin(4.0 * centre[:, 1]), np.cos(3.0 * centre[:, 2]), 0.0 * centre[:, 0]], axis=1) * act[:, None]
+        orc.set_state(v[:, 0], v[:, 1], v[:, 2], np.zeros(t.ndof))
+        orc.set_faces(np.zeros(t.ndof), np.zeros(t.ndof), np.zeros(t.ndof), np.zeros(t.info.compact_edges))
+        orc.advect(2e-3)
+        x, y, z, _ = orc.state()
+        ke.append(float((x * x + y * y + z * z).sum()))
+    assert ke[1] < ke[0]
