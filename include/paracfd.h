@@ -95,7 +95,18 @@ typedef struct pcfd_topology_info {
     double finest_h;          /* cell size of the finest level */
     double max_diffusion_coefficient; /* max over cells of sum(open area / distance) / volume */
     int outlet;               /* 1 when x-max is a Dirichlet outlet */
+    int instances;            /* independent cases in this topology (1 unless merged) */
 } pcfd_topology_info;
+
+/* One instance of a merged topology: its contiguous ranges in the merged numbering. */
+typedef struct pcfd_instance_info {
+    int64_t brick_begin, brick_end;
+    int64_t dof_begin, dof_end;   /* = 512 * bricks */
+    int64_t patch_begin, patch_end;
+    int64_t active_cells;
+    double reference_area;
+    double body_volume;
+} pcfd_instance_info;
 
 /* Topology arrays exported for host-side checks (oracle, tests). */
 typedef enum pcfd_array_id {
@@ -204,6 +215,15 @@ int64_t pcfd_topology_array_size(const pcfd_topology* topology, int array_id);
 /* Copy an exported array into dst (bytes must equal count * element size). */
 pcfd_status pcfd_topology_export(const pcfd_topology* topology, int array_id, void* dst, size_t bytes);
 
+/* Batching (no counterpart in the reference, which runs one ExternalAeroCore per case):
+ * concatenate `count` single-case topologies with the same AMR levels and cell sizes into
+ * one batch topology.  A solver created on it advances every instance in the same kernel
+ * launches and replayed graphs; each instance keeps its own timestep, Krylov scalars,
+ * convergence and loads, and computes exactly what it computes alone.  The parts may be
+ * freed afterwards. */
+pcfd_status pcfd_topology_merge(const pcfd_topology* const* parts, int count, pcfd_topology** out);
+pcfd_status pcfd_topology_instance(const pcfd_topology* topology, int instance, pcfd_instance_info* info);
+
 /* Device solver on the current HIP device.  Replaces ExternalAeroCore's device objects
  * (DeviceAmrFields, DeviceCompositeAmrProjection, DeviceCompositeCellMomentumTransport;
  * external_aero_core.cpp:77-200). The topology may be freed after creation. */
@@ -234,7 +254,18 @@ pcfd_status pcfd_apply_pressure_impulse(pcfd_solver* solver, double dt);
  * aero_loads.cpp:35-47), evaluated on the device in FP64. S_ref and the moment origin come
  * from pcfd_set_reference. */
 pcfd_status pcfd_integrate_loads(pcfd_solver* solver, double pref, pcfd_loads* loads);
+/* Sets S_ref and the moment origin of every instance. */
 pcfd_status pcfd_set_reference(pcfd_solver* solver, double area, const double origin[3]);
+
+/* Batches (a solver on a merged topology, pcfd_topology_merge).  pcfd_step / pcfd_initialize
+ * / pcfd_project advance every instance; their stats then report the worst instance
+ * (iterations, residual, divergence), all-instance convergence and the smallest dt, and
+ * pcfd_instance_stats gives each instance's own stats of the last call.  Each instance
+ * advances its own physical time only when its projection converged.  Loads are per
+ * instance (pcfd_integrate_loads is for single cases).  No counterpart in the reference. */
+pcfd_status pcfd_instance_stats(const pcfd_solver* solver, int instance, pcfd_step_stats* stats);
+pcfd_status pcfd_integrate_loads_instance(pcfd_solver* solver, int instance, double pref, pcfd_loads* loads);
+pcfd_status pcfd_set_reference_instance(pcfd_solver* solver, int instance, double area, const double origin[3]);
 
 /* State I/O (upload_state/download_state amr_advection.cu:1301-1309, upload_pressure
  * amr_pressure.cu:2098-2108, download_pressure).  Arrays are FP64 host arrays of length

@@ -49,7 +49,13 @@ class TopologyInfo(C.Structure):
                 ("mg_levels", C.c_int64), ("mg_cells", C.c_int64 * 12), ("levels", C.c_int), ("h0", C.c_double),
                 ("lo", C.c_double * 3), ("base_bricks", C.c_int * 3), ("body_volume", C.c_double),
                 ("reference_area", C.c_double), ("finest_h", C.c_double),
-                ("max_diffusion_coefficient", C.c_double), ("outlet", C.c_int)]
+                ("max_diffusion_coefficient", C.c_double), ("outlet", C.c_int), ("instances", C.c_int)]
+
+
+class InstanceInfo(C.Structure):
+    _fields_ = [("brick_begin", C.c_int64), ("brick_end", C.c_int64), ("dof_begin", C.c_int64),
+                ("dof_end", C.c_int64), ("patch_begin", C.c_int64), ("patch_end", C.c_int64),
+                ("active_cells", C.c_int64), ("reference_area", C.c_double), ("body_volume", C.c_double)]
 
 
 class Options(C.Structure):
@@ -106,7 +112,8 @@ EXPORTED = [
     "pcfd_set_pressure", "pcfd_get_faces", "pcfd_set_faces", "pcfd_mark_initialized", "pcfd_device_bytes",
     "pcfd_mg_primitive", "pcfd_time_kernel", "pcfd_learned_collect", "pcfd_learned_train",
     "pcfd_learned_get_weights", "pcfd_learned_set_weights", "pcfd_learned_enable", "pcfd_learned_forward",
-    "pcfd_learned_gradient",
+    "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
+    "pcfd_integrate_loads_instance", "pcfd_set_reference_instance",
 ]
 
 _lib = None
@@ -119,6 +126,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"paracfd_amd: {LIB_PATH} is not built; run __graft_entry__.build()")
+    # PyTorch ships its own HIP runtime.  When both are used in one process, torch's runtime
+    # must be loaded first so that libparacfd binds to the same one: two HIP runtimes in one
+    # process cannot both open the device.  Without torch, the ROCm runtime is used alone.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     P = C.c_void_p
     dp = C.POINTER(C.c_double)
@@ -158,6 +172,11 @@ def load():
         "pcfd_learned_enable": (C.c_int, [P, C.c_int]),
         "pcfd_learned_forward": (C.c_int, [P, dp, fp, fp]),
         "pcfd_learned_gradient": (C.c_int, [P, dp, dp, fp, fp]),
+        "pcfd_topology_merge": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
+        "pcfd_topology_instance": (C.c_int, [P, C.c_int, C.POINTER(InstanceInfo)]),
+        "pcfd_instance_stats": (C.c_int, [P, C.c_int, C.POINTER(StepStats)]),
+        "pcfd_integrate_loads_instance": (C.c_int, [P, C.c_int, C.c_double, C.POINTER(Loads)]),
+        "pcfd_set_reference_instance": (C.c_int, [P, C.c_int, C.c_double, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -196,15 +215,36 @@ def fptr(a):
 class Topology:
     """Host topology (bricks, cut cells, compact edges, patches, multigrid hierarchy)."""
 
-    def __init__(self, geometry: GeometryDesc, grid: GridDesc):
+    def __init__(self, geometry: GeometryDesc = None, grid: GridDesc = None, _handle=None):
         lib = load()
         self._lib = lib
-        h = C.c_void_p()
-        check(lib.pcfd_topology_build(C.byref(geometry), C.byref(grid), C.byref(h)))
+        if _handle is None:
+            h = C.c_void_p()
+            check(lib.pcfd_topology_build(C.byref(geometry), C.byref(grid), C.byref(h)))
+        else:
+            h = _handle
         self.handle = h
         self.info = TopologyInfo()
         check(lib.pcfd_topology_get_info(h, C.byref(self.info)))
         self._cache = {}
+
+    @classmethod
+    def merge(cls, parts):
+        """One batch topology of independent single cases (pcfd_topology_merge)."""
+        lib = load()
+        arr = (C.c_void_p * len(parts))(*[p.handle.value for p in parts])
+        h = C.c_void_p()
+        check(lib.pcfd_topology_merge(arr, len(parts), C.byref(h)))
+        return cls(_handle=h)
+
+    @property
+    def instances(self):
+        return int(self.info.instances)
+
+    def instance(self, i):
+        info = InstanceInfo()
+        check(self._lib.pcfd_topology_instance(self.handle, i, C.byref(info)))
+        return {name: getattr(info, name) for name, _ in InstanceInfo._fields_}
 
     def close(self):
         if self.handle:

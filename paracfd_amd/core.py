@@ -133,6 +133,21 @@ class ExternalAeroCore:
                                             sweeps))
         return out
 
+    # batches (merged topologies); no reference counterpart
+    def instance_stats(self, i) -> dict:
+        st = L.StepStats()
+        L.check(self._lib.pcfd_instance_stats(self.handle, i, C.byref(st)))
+        return st.as_dict()
+
+    def instance_loads(self, i, pref=0.0) -> dict:
+        out = L.Loads()
+        L.check(self._lib.pcfd_integrate_loads_instance(self.handle, i, pref, C.byref(out)))
+        return out.as_dict()
+
+    def set_instance_reference(self, i, area, origin=(0.0, 0.0, 0.0)):
+        o = (C.c_double * 3)(*origin)
+        L.check(self._lib.pcfd_set_reference_instance(self.handle, i, area, o))
+
     # learned preconditioner (options.preconditioner == PC_LEARNED); no reference counterpart
     def learned_collect(self, max_pairs=8) -> dict:
         """Store post-V-cycle residuals from a plain-MGPCG solve of the next step's projection."""

@@ -1,6 +1,7 @@
 // C ABI: error reporting and topology construction / export.
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/paracfd.h"
 #include "topology.hpp"
@@ -60,6 +61,43 @@ pcfd_status pcfd_topology_get_info(const pcfd_topology* topology, pcfd_topology_
     info->finest_h = t.finest_h;
     info->max_diffusion_coefficient = t.max_diffusion_coefficient;
     info->outlet = t.outlet ? 1 : 0;
+    info->instances = t.instances();
+    return PCFD_OK;
+}
+
+pcfd_status pcfd_topology_merge(const pcfd_topology* const* parts, int count, pcfd_topology** out) {
+    if (!parts || count < 1 || !out) { pcfd::set_error("pcfd_topology_merge: invalid argument"); return PCFD_EINVAL; }
+    *out = nullptr;
+    try {
+        std::vector<const pcfd::Topology*> v;
+        for (int i = 0; i < count; ++i) {
+            if (!parts[i]) { pcfd::set_error("pcfd_topology_merge: null part"); return PCFD_EINVAL; }
+            v.push_back(&parts[i]->t);
+        }
+        *out = new pcfd_topology{pcfd::merge_topologies(v)};
+        return PCFD_OK;
+    } catch (const pcfd::BuildError& e) {
+        pcfd::set_error(std::string("pcfd_topology_merge: ") + e.what());
+        return PCFD_EINVAL;
+    } catch (const std::bad_alloc&) {
+        pcfd::set_error("pcfd_topology_merge: out of host memory");
+        return PCFD_ENOMEM;
+    }
+}
+
+pcfd_status pcfd_topology_instance(const pcfd_topology* topology, int instance, pcfd_instance_info* info) {
+    if (!topology || !info) { pcfd::set_error("pcfd_topology_instance: null argument"); return PCFD_EINVAL; }
+    const pcfd::Topology& t = topology->t;
+    if (instance < 0 || instance >= t.instances()) { pcfd::set_error("pcfd_topology_instance: no such instance"); return PCFD_EINVAL; }
+    info->brick_begin = t.inst_brick_off[instance];
+    info->brick_end = t.inst_brick_off[instance + 1];
+    info->dof_begin = info->brick_begin * 512;
+    info->dof_end = info->brick_end * 512;
+    info->patch_begin = t.inst_patch_off[instance];
+    info->patch_end = t.inst_patch_off[instance + 1];
+    info->active_cells = t.inst_active[instance];
+    info->reference_area = t.inst_reference_area[instance];
+    info->body_volume = t.inst_body_volume[instance];
     return PCFD_OK;
 }
 

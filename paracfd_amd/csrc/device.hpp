@@ -18,18 +18,23 @@ constexpr int kBrickThreads = 512;
 constexpr uint8_t kPX = 1, kMX = 8, kActive = 64, kSpecial = 128;
 constexpr uint8_t kXMIN = 1, kXMAX = 2, kYMIN = 4, kYMAX = 8, kZMIN = 16, kZMAX = 32;
 
-// Scalars the Krylov loop keeps on the device (no host round trip per iteration).
+// Scalars the Krylov loop keeps on the device (no host round trip per iteration), one
+// record per instance of a batch; record [ninst] is the batch aggregate (done when every
+// instance is done), read by the kernels that do not know their instance.
 struct DevScalars {
     double rz, rz_prev, pAp, alpha, beta, rr, bnorm, relres;
     double dt, rate, maxdiv, vmax, diff_rate;
+    double dts;       // diffusion substep of this instance
     double loads[6];
-    int iter, done, converged, breakdown, zero_rhs, pad0;
+    int iter, done, converged, breakdown, zero_rhs, subs;
     unsigned int ticket[16];
 };
 
 struct MeshView {
     int nbricks;
     int64_t ndof;
+    int ninst;               // instances of the batch (1 for a single solver)
+    const int* inst;         // instance of each brick
     const uint8_t* mask;
     const int* brick_level;
     const uint8_t* brick_flags;
@@ -209,12 +214,13 @@ __device__ __forceinline__ double block_max(double v) {
 }
 
 // Reductions are two-stage and deterministic: every workgroup stores its partial with a
-// plain store (partials[array * nparts + block]); a one-workgroup finalize kernel launched
-// next on the same stream sums them in block order.  (A "last workgroup finalises" scheme
-// needs an agent-scope release fence per workgroup, which writes back L2 and cost ~0.4 ms
-// per launch at 8k workgroups on gfx950.)
-__device__ __forceinline__ void store_partial(double* partials, double value) {
-    if (threadIdx.x == 0) partials[blockIdx.x] = value;
+// plain store at its brick's slot (partials[array * nparts + brick]); a finalize kernel
+// launched next on the same stream, one workgroup per instance, sums the slots of that
+// instance's bricks in brick order, so a batched instance reduces exactly as it does alone.
+// (A "last workgroup finalises" scheme needs an agent-scope release fence per workgroup,
+// which writes back L2 and cost ~0.4 ms per launch at 8k workgroups on gfx950.)
+__device__ __forceinline__ void store_partial(double* partials, int slot, double value) {
+    if (threadIdx.x == 0) partials[slot] = value;
 }
 
 // Sum (or max) of n partials in fixed order; result valid in thread 0.

@@ -3,12 +3,13 @@
 // storage (VT).  The composite level uses the brick layout with an LDS tile per brick;
 // coarser levels are SELL-64 (slices of 64 rows, slot-major).
 //
-// The Krylov scalars never leave the device: a one-workgroup finalize kernel after each
-// reduction computes alpha / beta / the relative residual and raises S->done, and every
-// composite-level kernel of the loop returns immediately once it is set, so the host may
-// enqueue (or replay as a graph) several iterations between convergence reads without
-// changing the result.  Coarse-level kernels do not test the flag: after convergence they
-// only rewrite their own scratch levels, which the next V-cycle overwrites.
+// The Krylov scalars never leave the device: a finalize kernel after each reduction (one
+// workgroup per instance of a batch) computes alpha / beta / the relative residual and
+// raises the instance's done flag, and every composite-level kernel of the loop returns
+// immediately for the bricks of a finished instance, so the host may enqueue (or replay as
+// a graph) several iterations between convergence reads without changing the result.
+// Coarse-level kernels test the batch aggregate flag only: until every instance is done
+// they also rewrite the finished instances' scratch levels, which nothing reads.
 #pragma once
 
 #include "device.hpp"
@@ -79,7 +80,7 @@ __device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, int 
 __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* __restrict__ x,
                                                      const double* __restrict__ rhs, double* __restrict__ r,
                                                      double* partials, const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     __shared__ double tile[kTileN];
     PCFD_BRICK_PROLOGUE;
     auto val = [&](int64_t c) -> double { return x[c]; };
@@ -90,13 +91,13 @@ __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* _
         ra = rhs[a] - row0<double>(M, tile, val, b, q, m, sp, x[a], dg);
     }
     r[a] = ra;
-    store_partial(partials, block_sum<512>(ra * ra));
+    store_partial(partials, b, block_sum<512>(ra * ra));
 }
 
 // A d and the d.Ad partial (used by the measurement hook)
 __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* __restrict__ d, double* __restrict__ Ad,
                                                       double* partials, const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     __shared__ double tile[kTileN];
     PCFD_BRICK_PROLOGUE;
     auto val = [&](int64_t c) -> double { return d[c]; };
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* 
         y = row0<double>(M, tile, val, b, q, m, sp, d[a], dg);
     }
     Ad[a] = y;
-    store_partial(partials, block_sum<512>(y * d[a]));
+    store_partial(partials, b, block_sum<512>(y * d[a]));
 }
 
 // CG direction update fused into the operator apply: d_new = z + beta d_old (first
@@ -116,10 +117,10 @@ __global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* 
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
                                                       double* __restrict__ Ad, double* partials, const DevScalars* S,
                                                       int initial) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     __shared__ double tile[kTileN];
     PCFD_BRICK_PROLOGUE;
-    const double beta = initial ? 0.0 : S->beta;
+    const double beta = initial ? 0.0 : S[M.inst[b]].beta;
     auto dn = [&](int64_t c) -> double { return initial ? z[c] : z[c] + beta * dold[c]; };
     load_tile<double>(M, b, tile, dn);
     const double da = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
@@ -130,24 +131,25 @@ __global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* 
     }
     dnew[a] = da;
     Ad[a] = y;
-    store_partial(partials, block_sum<512>(y * da));
+    store_partial(partials, b, block_sum<512>(y * da));
 }
 
-__global__ __launch_bounds__(256) void k_cg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
+// x += alpha d, r -= alpha Ad (r_prev keeps the old r for flexible CG) and the ||r||^2
+// partial, one brick per workgroup with the brick's instance's alpha
+__global__ __launch_bounds__(512) void k_cg_update(MeshView M, double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ rprev, const double* __restrict__ d,
                                                    const double* __restrict__ Ad, double* partials, const DevScalars* S) {
-    if (S->done) return;
-    const double alpha = S->alpha;
-    double acc = 0.0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
-        x[i] += alpha * d[i];
-        const double ri = r[i];
-        if (rprev) rprev[i] = ri;
-        const double rn = ri - alpha * Ad[i];
-        r[i] = rn;
-        acc += rn * rn;
-    }
-    store_partial(partials, block_sum<256>(acc));
+    const int b = brick_of_block();
+    const DevScalars& I = S[M.inst[b]];
+    if (I.done) return;
+    const double alpha = I.alpha;
+    const int64_t i = static_cast<int64_t>(b) * 512 + threadIdx.x;
+    x[i] += alpha * d[i];
+    const double ri = r[i];
+    if (rprev) rprev[i] = ri;
+    const double rn = ri - alpha * Ad[i];
+    r[i] = rn;
+    store_partial(partials, b, block_sum<512>(rn * rn));
 }
 
 // Scalar finalisation of each reduction (one workgroup, partials summed in block order).
@@ -165,14 +167,13 @@ enum FinalizeOp : int {
     FIN_DIFFRATE = 10 // max explicit diffusion rate (Smagorinsky substeps)
 };
 
-__global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restrict__ partials, int nparts,
-                                                   DevScalars* S, double cfl, double rate_floor, double rtol,
-                                                   int max_iterations, int flexible) {
-    if ((op == FIN_RESID || op == FIN_APPLY || op == FIN_UPDATE || op == FIN_RZ || op == FIN_RZ_INITIAL) && S->done) return;
+__device__ __forceinline__ void finalize_one(int op, const double* __restrict__ partials, int nparts, int beg, int end,
+                                             DevScalars* S, double cfl, double rate_floor, double rtol,
+                                             int max_iterations, int flexible) {
     const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE;
     const int arrays = op == FIN_LOADS ? 6 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
     double t[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts, nparts, take_max);
+    for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts + beg, end - beg, take_max);
     if (threadIdx.x != 0) return;
     switch (op) {
         case FIN_RATE:
@@ -217,6 +218,32 @@ __global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restr
     }
 }
 
+// One workgroup per instance: partial slots [seg[i], seg[i+1]) of each array (arrays are
+// nparts apart; seg == nullptr: all nparts slots), scalars of instance i in S[i].  With
+// `agg`, the last workgroup to finish sets agg->done when every instance is done.
+__global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restrict__ partials, int nparts,
+                                                   const int* __restrict__ seg, DevScalars* Sbase, DevScalars* agg,
+                                                   double cfl, double rate_floor, double rtol, int max_iterations,
+                                                   int flexible) {
+    const int inst = blockIdx.x;
+    DevScalars* S = Sbase + inst;
+    const bool krylov = op == FIN_RESID || op == FIN_APPLY || op == FIN_UPDATE || op == FIN_RZ || op == FIN_RZ_INITIAL;
+    if (!(krylov && S->done)) finalize_one(op, partials, nparts, seg ? seg[inst] : 0, seg ? seg[inst + 1] : nparts, S,
+                                           cfl, rate_floor, rtol, max_iterations, flexible);
+    if (agg && threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int ticket = atomicAdd(&agg->ticket[0], 1u);
+        if (ticket == gridDim.x - 1) {  // every instance's update is visible
+            __threadfence();
+            int all = 1;
+            for (unsigned int i = 0; i < gridDim.x; ++i)
+                all &= __hip_atomic_load(&Sbase[i].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1 : 0;
+            agg->done = all;
+            agg->ticket[0] = 0u;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // V-cycle, composite level: damped Jacobi (project.cu:76-83 restated with the composite
 // operator).  Every kernel reproduces the arithmetic (including the rounding to the
@@ -238,7 +265,7 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
                                                  const double* __restrict__ rhs, double* __restrict__ zout,
                                                  const double* __restrict__ rprev, double omega, double* partials,
                                                  const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
     __shared__ A tile[kTileN];
     PCFD_BRICK_PROLOGUE;
@@ -266,8 +293,8 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
     const double s1 = block_sum<512>(rhs[a] * z);
     const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = s1;
-        partials[gridDim.x + blockIdx.x] = s2;
+        partials[b] = s1;
+        partials[M.nbricks + b] = s2;
     }
 }
 
@@ -275,7 +302,7 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
 template <class VT>
 __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
                                                    VT* __restrict__ res, const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
     __shared__ A tile[kTileN];
     PCFD_BRICK_PROLOGUE;
@@ -294,7 +321,7 @@ template <class VT>
 __global__ __launch_bounds__(512) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
                                                  const typename Arith<VT>::type* __restrict__ diag, VT* __restrict__ xout,
                                                  double omega, const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
     __shared__ A tile[kTileN];
     PCFD_BRICK_PROLOGUE;
@@ -317,7 +344,7 @@ template <class VT>
 __global__ __launch_bounds__(512) void k_vc_restrict0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
                                                       const int* __restrict__ parent0, VT* __restrict__ coarse,
                                                       const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
     __shared__ A tile[kTileN];
     __shared__ A sres[512];
@@ -364,7 +391,7 @@ __global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, c
                                                          const double* __restrict__ rhs, VT* __restrict__ xout,
                                                          double* __restrict__ zout, const double* __restrict__ rprev,
                                                          double omega, double* partials, const DevScalars* S) {
-    if (S->done) return;
+    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
     __shared__ A tile[kTileN];
     PCFD_BRICK_PROLOGUE;
@@ -390,8 +417,8 @@ __global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, c
     const double s1 = block_sum<512>(rhs[a] * z);
     const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = s1;
-        partials[gridDim.x + blockIdx.x] = s2;
+        partials[b] = s1;
+        partials[M.nbricks + b] = s2;
     }
 }
 
@@ -529,15 +556,20 @@ __global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT
     xout[r] = from_f<VT>(yr + A(omega) * (A(to_f(rhs[r])) - Ay) / A(static_cast<const CT*>(C.diag)[r]));
 }
 
-// coarsest level: x = A^-1 b with the dense inverse, one workgroup per row
+// coarsest level of each instance: x = A^-1 b with its dense inverse; block (row, k) solves
+// row `row` of dense block k (cells [off[k], off[k] + n[k]) of the level, inverse at ioff[k])
 template <class VT, class CT>
-__global__ __launch_bounds__(256) void k_dense_solve(int n, const CT* __restrict__ inv, const VT* __restrict__ rhs,
-                                                     VT* __restrict__ x, const DevScalars* S) {
-    const int r = blockIdx.x;
+__global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off, const int* __restrict__ nrow,
+                                                     const int64_t* __restrict__ ioff, const CT* __restrict__ inv,
+                                                     const VT* __restrict__ rhs, VT* __restrict__ x, const DevScalars* S) {
+    const int k = blockIdx.y, r = blockIdx.x, n = nrow[k];
+    if (r >= n) return;
+    const CT* row = inv + ioff[k] + static_cast<int64_t>(r) * n;
+    const VT* b = rhs + off[k];
     double acc = 0.0;
-    for (int c = threadIdx.x; c < n; c += 256) acc += double(inv[static_cast<int64_t>(r) * n + c]) * double(to_f(rhs[c]));
+    for (int c = threadIdx.x; c < n; c += 256) acc += double(row[c]) * double(to_f(b[c]));
     acc = block_sum<256>(acc);
-    if (threadIdx.x == 0) x[r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
+    if (threadIdx.x == 0) x[off[k] + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
 }
 
 #undef PCFD_BRICK_PROLOGUE

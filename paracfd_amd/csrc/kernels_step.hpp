@@ -67,7 +67,7 @@ __global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> 
         }
         rate = out / V;
     }
-    store_partial(partials, block_max<512>(rate));
+    store_partial(partials, b, block_max<512>(rate));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
         for (int c = 0; c < 3; ++c) out[c][a] = F.v[c][a];
         return;
     }
-    const R dt = R(S->dt);
+    const R dt = R(S[M.inst[b]].dt);
     const R h = R(cell_h(M, b)), A = h * h;
     R va[3], d[3] = {R(0), R(0), R(0)};
     for (int c = 0; c < 3; ++c) va[c] = F.v[c][a];
@@ -153,14 +153,16 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
 template <class R>
 __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
                                                  const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
-                                                 R* __restrict__ oz, double dts) {
+                                                 R* __restrict__ oz, const DevScalars* S, int step) {
     const int b = brick_of_block(), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
     const R* in[3] = {vx, vy, vz};
     R* out[3] = {ox, oy, oz};
-    if (!(m & kActive)) {
+    const DevScalars& I = S[M.inst[b]];
+    const double dts = I.dts;
+    if (!(m & kActive) || step >= I.subs) {  // this instance needs fewer substeps
         for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
         return;
     }
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(512) void k_diff_rate(MeshView M, const R* __restri
         }
         rate = s / V;
     }
-    store_partial(partials, block_max<512>(rate));
+    store_partial(partials, b, block_max<512>(rate));
 }
 
 // one explicit diffusion substep with the cell viscosity field (face coefficient mean of
@@ -290,14 +292,17 @@ __global__ __launch_bounds__(512) void k_diff_rate(MeshView M, const R* __restri
 template <class R>
 __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
                                                      const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
-                                                     R* __restrict__ oz, const R* __restrict__ nu, double dts) {
+                                                     R* __restrict__ oz, const R* __restrict__ nu, const DevScalars* S,
+                                                     int step) {
     const int b = brick_of_block(), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
     const R* in[3] = {vx, vy, vz};
     R* out[3] = {ox, oy, oz};
-    if (!(m & kActive)) {
+    const DevScalars& I = S[M.inst[b]];
+    const double dts = I.dts;
+    if (!(m & kActive) || step >= I.subs) {
         for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
         return;
     }
@@ -388,9 +393,9 @@ __global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
     double val = 0.0;
-    if (m & kActive) val = -(M.rho / S->dt) * cell_divergence(M, F, b, q, m, sp, cell_h(M, b));
+    if (m & kActive) val = -(M.rho / S[M.inst[b]].dt) * cell_divergence(M, F, b, q, m, sp, cell_h(M, b));
     rhs[a] = val;
-    store_partial(partials, block_sum<512>(val * val));
+    store_partial(partials, b, block_sum<512>(val * val));
 }
 
 // max |integrated divergence| / V after the correction (FP64 check)
@@ -406,7 +411,7 @@ __global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, d
         const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
         val = fabs(cell_divergence(M, F, b, q, m, sp, h) / V);
     }
-    store_partial(partials, block_max<512>(val));
+    store_partial(partials, b, block_max<512>(val));
 }
 
 // face correction u -= (dt/rho) dp/dn (amr_pressure.cu:285-310); outlet p = 0 half a cell away
@@ -417,7 +422,7 @@ __global__ __launch_bounds__(512) void k_correct(MeshView M, FieldsView<R> F, co
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     if (!(m & kActive)) return;
-    const double s = S->dt / M.rho, h = cell_h(M, b);
+    const double s = S[M.inst[b]].dt / M.rho, h = cell_h(M, b);
     const double pa = p[a];
     for (int axis = 0; axis < 3; ++axis) {
         if (!(m & (kPX << axis))) continue;
@@ -431,7 +436,7 @@ template <class R>
 __global__ void k_correct_edges(MeshView M, FieldsView<R> F, const double* __restrict__ p, const DevScalars* S) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= M.nedges) return;
-    const double s = S->dt / M.rho;
+    const double s = S[M.inst[M.ea[e] >> 9]].dt / M.rho;
     F.fe[e] = R(double(F.fe[e]) - s * (p[M.eb[e]] - p[M.ea[e]]) * (M.ecoef[e] / M.earea[e]));
 }
 
@@ -448,14 +453,19 @@ __device__ __forceinline__ double wall_pressure(const MeshView& M, const double*
 // the face pressure, walls the cell pressure, the outlet p = 0, embedded-boundary patches
 // the reconstructed wall pressure.  u += dm / V.
 // ---------------------------------------------------------------------------------------
+// dt: the given value when positive, else the instance's own step dt; with only_converged,
+// instances whose projection did not converge are left unchanged.
 template <class R>
 __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
-                                                 double dt) {
+                                                 const DevScalars* S, double dt_given, int only_converged) {
     const int b = brick_of_block(), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
+    const DevScalars& I = S[M.inst[b]];
+    if (only_converged && !I.converged) return;  // block-uniform
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
     if (!(m & kActive)) return;
+    const double dt = dt_given > 0.0 ? dt_given : I.dt;
     const double s = dt / M.rho, h = cell_h(M, b), A = h * h;
     const double pa = p[a];
     double d[3] = {0.0, 0.0, 0.0};
@@ -497,10 +507,10 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
 // M = sum (x - origin) x F; deterministic block partials.
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p, double pref, double ox,
-                                               double oy, double oz, double* partials) {
-    const int pch = blockIdx.x * blockDim.x + threadIdx.x;
+                                               double oy, double oz, int pbegin, int pend, double* partials) {
+    const int pch = pbegin + blockIdx.x * blockDim.x + threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
-    if (pch < M.npatches) {
+    if (pch < pend) {
         const double pw = wall_pressure(M, p, pch) - pref;
         const double Fx = pw * M.patch_S[3 * pch], Fy = pw * M.patch_S[3 * pch + 1], Fz = pw * M.patch_S[3 * pch + 2];
         const double rx = M.patch_x[3 * pch] - ox, ry = M.patch_x[3 * pch + 1] - oy, rz = M.patch_x[3 * pch + 2] - oz;
@@ -523,7 +533,28 @@ __global__ __launch_bounds__(512) void k_vmax(MeshView M, FieldsView<R> F, doubl
         v = fmax(fabs(x), fmax(fabs(y), fabs(z)));
         if (!isfinite(x) || !isfinite(y) || !isfinite(z)) v = INFINITY;
     }
-    store_partial(partials, block_max<512>(v));
+    store_partial(partials, static_cast<int>(blockIdx.x), block_max<512>(v));
+}
+
+// the same timestep for every instance (an explicit dt given by the caller)
+__global__ void k_set_dt(DevScalars* S, int ninst, double dt) {
+    const int i = threadIdx.x;
+    if (i < ninst) S[i].dt = dt;
+}
+
+// diffusion substeps of each instance from its timestep: rate nu * max(sum conductance / V)
+// (constant viscosity) or the reduced Smagorinsky rate; substeps ceil(dt rate / 0.7)
+// (amr_advection.cu:1343), dts = dt / substeps; no diffusion when the rate is zero
+__global__ void k_diffusion_substeps(DevScalars* S, int ninst, double nu, const double* __restrict__ maxdiff,
+                                     int use_rate) {
+    const int i = threadIdx.x;
+    if (i >= ninst) return;
+    const double dt = S[i].dt;
+    int subs = 0;
+    if (use_rate) subs = max(1, static_cast<int>(ceil(dt * S[i].diff_rate / 0.7)));
+    else if (nu > 0.0) subs = max(1, static_cast<int>(ceil(dt * nu * maxdiff[i] / 0.7)));
+    S[i].subs = subs;
+    S[i].dts = subs > 0 ? dt / subs : 0.0;
 }
 
 template <class R>

@@ -218,28 +218,27 @@ __device__ __forceinline__ void nn_layers(const NNView& W, NNShared& L, uint16_t
     }
 }
 
-// brick, partial slot and list position of this workgroup: with a list, block p handles
-// list[p] and adds to the partial slot[p]; without one, the XCD-aware brick order
-struct NNBrick { int b, slot, pos; };
-__device__ __forceinline__ NNBrick nn_brick(const int* list, const int* slot) {
-    if (list) return NNBrick{list[blockIdx.x], slot ? slot[blockIdx.x] : static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.x)};
+// brick and list position of this workgroup: with a list, block p handles list[p];
+// without one, the XCD-aware brick order (position = brick)
+struct NNBrick { int b, pos; };
+__device__ __forceinline__ NNBrick nn_brick(const int* list) {
+    if (list) return NNBrick{list[blockIdx.x], static_cast<int>(blockIdx.x)};
     const int b = brick_of_block();
-    return NNBrick{b, static_cast<int>(blockIdx.x), b};
+    return NNBrick{b, b};
 }
 
 // Inference inside PCG, after the V-cycle's last kernel wrote z and the r.z / r_prev.z
-// partials (grid of one block per brick in the XCD-aware order): r' = r - A z on the brick,
-// the network, dz = (s_b / h_b) y into dz[pos] (applied to z by k_nn_add, so that no block
-// reads a z that another block has changed), and the partials of the brick's V-cycle
-// workgroup increased by r.dz and r_prev.dz.
+// partials (at brick slots): r' = r - A z on the brick, the network, dz = (s_b / h_b) y into
+// dz[pos] (applied to z by k_nn_add, so that no block reads a z that another block has
+// changed), and the brick's partials increased by r.dz and r_prev.dz.
 __global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, const int* __restrict__ list,
-                                                  const int* __restrict__ slot, const double* __restrict__ r,
+                                                  const double* __restrict__ r,
                                                   const double* __restrict__ z, double* __restrict__ dz,
                                                   const double* __restrict__ rprev, double* partials,
                                                   int nparts, const DevScalars* S, const int* __restrict__ off) {
     if (S->done || *off) return;
     __shared__ __attribute__((aligned(16))) NNShared L;
-    const NNBrick B = nn_brick(list, slot);
+    const NNBrick B = nn_brick(list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -266,8 +265,8 @@ __global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, const in
     const double t1 = block_sum<512>(r[a] * d);
     const double t2 = rprev ? block_sum<512>(rprev[a] * d) : 0.0;
     if (threadIdx.x == 0) {
-        partials[B.slot] += t1;
-        partials[nparts + B.slot] += t2;
+        partials[B.b] += t1;
+        partials[nparts + B.b] += t2;
     }
 }
 
@@ -275,7 +274,7 @@ __global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, const in
 __global__ __launch_bounds__(512) void k_nn_add(const int* __restrict__ list, const double* __restrict__ dz,
                                                 double* __restrict__ z, const DevScalars* S, const int* __restrict__ off) {
     if (S && (S->done || *off)) return;
-    const NNBrick B = nn_brick(list, nullptr);
+    const NNBrick B = nn_brick(list);
     z[static_cast<int64_t>(B.b) * 512 + threadIdx.x] += dz[static_cast<int64_t>(B.pos) * 512 + threadIdx.x];
 }
 
@@ -303,7 +302,7 @@ __global__ __launch_bounds__(512) void k_nn_train_fwd(MeshView M, NNView W, cons
                                                       const double* __restrict__ rp, uint16_t* __restrict__ scratch,
                                                       double* __restrict__ dfull, float* __restrict__ sb) {
     __shared__ __attribute__((aligned(16))) NNShared L;
-    const NNBrick B = nn_brick(list, nullptr);
+    const NNBrick B = nn_brick(list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     nn_zero_tile(L.T0);
@@ -322,7 +321,7 @@ __global__ __launch_bounds__(512) void k_nn_energy(MeshView M, const int* __rest
                                                    const double* __restrict__ rp, double* __restrict__ gy,
                                                    double* __restrict__ loss, double inv_norm) {
     __shared__ double tile[kTileN];
-    const NNBrick B = nn_brick(list, nullptr);
+    const NNBrick B = nn_brick(list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -351,7 +350,7 @@ __global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, cons
                                                       float* __restrict__ grad) {
     __shared__ __attribute__((aligned(16))) NNShared L;
     __shared__ __attribute__((aligned(16))) uint16_t T2[kNNTile];
-    const NNBrick B = nn_brick(list, nullptr);
+    const NNBrick B = nn_brick(list);
     const float s = sb[blockIdx.x];
     if (!(s > 0.f)) return;  // block-uniform
     const int q = threadIdx.x;

@@ -72,9 +72,13 @@ public:
     virtual void learned_enable(bool on) = 0;
     virtual void learned_forward(const double* r, float* y, float* s) = 0;
     virtual void learned_gradient(const double* r, double* loss, float* gw, float* gb) = 0;
+    // batches: instance -1 sets every instance
+    virtual void set_reference(int instance, double area, const double* origin) = 0;
+    virtual void instance_stats(int instance, pcfd_step_stats* st) const = 0;
+    virtual void instance_loads(int instance, double pref, pcfd_loads* out) = 0;
+    virtual int instances() const = 0;
     bool initialized = false;
     double time = 0.0;
-    double ref_area = 0.0, origin[3] = {0, 0, 0};
     int64_t device_bytes = 0;
 };
 
@@ -103,6 +107,16 @@ public:
     void learned_enable(bool on) override;
     void learned_forward(const double* r, float* y, float* s) override;
     void learned_gradient(const double* r, double* loss, float* gw, float* gb) override;
+    void set_reference(int instance, double area, const double* origin) override {
+        for (int i = 0; i < ninst_; ++i) {
+            if (instance >= 0 && i != instance) continue;
+            inst_ref_area_[i] = area;
+            if (origin) for (int c = 0; c < 3; ++c) inst_origin_[3 * static_cast<size_t>(i) + c] = origin[c];
+        }
+    }
+    void instance_stats(int instance, pcfd_step_stats* st) const override { *st = inst_stats_.at(instance); }
+    void instance_loads(int instance, double pref, pcfd_loads* out) override { loads_instance(instance, pref, out); }
+    int instances() const override { return ninst_; }
 
 private:
     template <class T>
@@ -125,13 +139,15 @@ private:
         std::vector<T> c(v.begin(), v.end());
         return upload(c);
     }
+    // instance scalars S_[0..ninst_) and the batch aggregate agg_ = S_ + ninst_
     void sync_scalars() {
-        HIP_CHECK(hipMemcpyAsync(host_s_, S_, sizeof(DevScalars), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipMemcpyAsync(host_s_, S_, (ninst_ + 1) * sizeof(DevScalars), hipMemcpyDeviceToHost, stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
+    const DevScalars& hagg() const { return host_s_[ninst_]; }
     void put_dt(double dt) {
-        host_s_->dt = dt;
-        HIP_CHECK(hipMemcpyAsync(&S_->dt, &host_s_->dt, sizeof(double), hipMemcpyHostToDevice, stream_));
+        for (int i = 0; i < ninst_; ++i) host_s_[i].dt = dt;
+        LAUNCH(k_set_dt, 1, 64, S_, ninst_, dt);
     }
     FieldsView<R> fields() const {
         FieldsView<R> F;
@@ -144,9 +160,8 @@ private:
     void cg_iteration(bool initial, double* dold, double* dnew) {
         LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0);
         finalize(FIN_APPLY, nb_);
-        const unsigned gu = std::min<unsigned>(blocks_for(n_, 256), 2048);
-        LAUNCH(k_cg_update, gu, 256, n_, p_, r_, rprev_, dnew, Ad_, partials_, S_);
-        finalize(FIN_UPDATE, static_cast<int>(gu));
+        LAUNCH(k_cg_update, nb_, 512, M_, p_, r_, rprev_, dnew, Ad_, partials_, S_);
+        finalize(FIN_UPDATE, nb_);
         precondition_dispatch(false);
     }
     // `chunk_` (even) iterations alternating the two direction buffers
@@ -166,9 +181,22 @@ private:
         HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
         HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
     }
+    // reduction over brick-indexed partials, one finalize workgroup per instance
     void finalize(int op, int nparts) {
-        LAUNCH(k_finalize, 1, 1024, op, partials_, nparts, S_, opt_.cfl,
+        LAUNCH(k_finalize, ninst_, 1024, op, partials_, nparts, seg_, S_, agg_, opt_.cfl,
                std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations, rprev_ ? 1 : 0);
+    }
+    // dense coarse solves of the instances whose coarsest multigrid level is m
+    template <class VT, class CT>
+    bool dense_blocks(int m, VT* B, VT* X) {
+        const DenseLevel& D = dense_[m];
+        if (D.count == 0) return false;
+        const CT* inv = nullptr;
+        if constexpr (std::is_same<CT, double>::value) inv = dense_inv64_; else inv = dense_inv32_;
+        hipLaunchKernelGGL((k_dense_solve<VT, CT>), dim3(D.max_n, D.count), dim3(256), 0, stream_, D.off, D.n, D.ioff,
+                           inv, B, X, agg_);
+        HIP_CHECK(hipGetLastError());
+        return true;
     }
     // learned preconditioner
     NNView nnv() const { return NNView{nn_w_, nn_wt_, nn_p_ + kNNW}; }
@@ -177,7 +205,7 @@ private:
     // network correction after the V-cycle's last kernel, then the r.z finalisation
     void pc_tail(int fin) {
         if (nn_) {
-            LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, nn_slot_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
+            LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
             LAUNCH(k_nn_add, nn_grid(), 512, nn_list_, nn_dz_, z_, S_, nn_off_);
         }
         finalize(fin, nb_);
@@ -224,6 +252,9 @@ private:
         }
     }
     bool pcg_loop(bool watch_stall);
+    void aggregate_stats(pcfd_step_stats* st) const;
+    void advect_core();
+    void loads_instance(int i, double pref, pcfd_loads* out);
     void reconstruct();
     void run_projection(bool warm, pcfd_step_stats* st);
     template <class VT> void precondition(bool initial);
@@ -253,7 +284,18 @@ private:
     double* rprev_ = nullptr;
     void* vc0_[2] = {};  // composite-level V-cycle buffers (8 bytes per dof)
     double* partials_ = nullptr;
-    DevScalars* S_ = nullptr;
+    DevScalars* S_ = nullptr;     // ninst_ instance records, then the batch aggregate
+    DevScalars* agg_ = nullptr;
+    int ninst_ = 1;
+    int* seg_ = nullptr;          // brick offsets of the instances (partial segments)
+    std::vector<int> inst_brick_off_, inst_patch_off_;
+    std::vector<double> inst_maxdiff_, inst_time_, inst_ref_area_, inst_origin_;
+    std::vector<pcfd_step_stats> inst_stats_;
+    double* inst_maxdiff_dev_ = nullptr;
+    struct DenseLevel { int count = 0, max_n = 0; const int* off = nullptr; const int* n = nullptr; const int64_t* ioff = nullptr; };
+    std::vector<DenseLevel> dense_;
+    double* dense_inv64_ = nullptr;
+    float* dense_inv32_ = nullptr;
     DevScalars* host_s_ = nullptr;
     // multigrid levels >= 1
     struct Level {
@@ -264,9 +306,6 @@ private:
     };
     std::vector<Level> lv_;
     const int* parent0_ = nullptr;
-    double* inv64_ = nullptr;
-    float* inv32_ = nullptr;
-    int coarse_n_ = 0;
     hipEvent_t ev_[4] = {};
     double* dbuf_[2] = {};
     void* diag_ = nullptr;        // composite-level diagonal in the V-cycle arithmetic type
@@ -291,7 +330,6 @@ private:
     double* nn_loss_ = nullptr;
     uint16_t* nn_scratch_ = nullptr;  // layer inputs per listed brick (training)
     int* nn_list_ = nullptr;      // the network's bricks (null: all bricks, XCD-aware order)
-    int* nn_slot_ = nullptr;      // V-cycle partial slot of each listed brick
     int nn_nlist_ = 0;
     int64_t nn_cells_ = 0;        // active cells of the network's bricks
     double* nn_dz_ = nullptr;     // correction per listed brick (512 per position)
@@ -323,8 +361,8 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
         throw std::invalid_argument("smagorinsky_cs must be a non-negative number");
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
-    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_s_), sizeof(DevScalars), hipHostMallocDefault));
-    std::memset(host_s_, 0, sizeof(DevScalars));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_s_), (T.instances() + 1) * sizeof(DevScalars), hipHostMallocDefault));
+    std::memset(host_s_, 0, (T.instances() + 1) * sizeof(DevScalars));
     n_ = T.ndof;
     nb_ = T.nbricks;
     nedges_ = static_cast<int>(T.edge_a.size());
@@ -389,8 +427,30 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     vc0_[1] = alloc<double>(n_);
     const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 6 * (M_.npatches / 256 + 1)});
     partials_ = alloc<double>(npart);
-    S_ = alloc<DevScalars>(1);
-    HIP_CHECK(hipMemset(S_, 0, sizeof(DevScalars)));
+    // instances of a batch
+    ninst_ = T.instances();
+    if (ninst_ < 1) throw std::invalid_argument("topology has no instance");
+    if (ninst_ > 1 && opt_.preconditioner == PCFD_PC_LEARNED)
+        throw std::invalid_argument("the learned preconditioner runs on single cases, not batches");
+    inst_brick_off_ = T.inst_brick_off;
+    inst_patch_off_ = T.inst_patch_off;
+    inst_maxdiff_ = T.inst_max_diffusion;
+    inst_time_.assign(ninst_, 0.0);
+    inst_ref_area_ = T.inst_reference_area;
+    inst_origin_.assign(3 * static_cast<size_t>(ninst_), 0.0);
+    inst_stats_.assign(ninst_, pcfd_step_stats{});
+    {
+        std::vector<int> inst(nb_);
+        for (int i = 0; i < ninst_; ++i)
+            for (int b = T.inst_brick_off[i]; b < T.inst_brick_off[i + 1]; ++b) inst[b] = i;
+        M_.inst = upload(inst);
+        M_.ninst = ninst_;
+    }
+    seg_ = upload(T.inst_brick_off);
+    inst_maxdiff_dev_ = upload(T.inst_max_diffusion);
+    S_ = alloc<DevScalars>(ninst_ + 1);
+    agg_ = S_ + ninst_;
+    HIP_CHECK(hipMemset(S_, 0, (ninst_ + 1) * sizeof(DevScalars)));
     // multigrid
     if (T.mg.size() > 1) {
         parent0_ = upload(T.mg[0].parent);
@@ -444,11 +504,28 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
             L.parent = c.parent;
             L.B = alloc<double>(G.n); L.X = alloc<double>(G.n); L.T = alloc<double>(G.n);
         }
-        const MGLevel& K = T.mg.back();
-        coarse_n_ = K.n;
-        if (!K.coarse_inv.empty()) {
-            inv64_ = upload(K.coarse_inv);
-            inv32_ = upload_as<float>(K.coarse_inv);
+        // dense coarse blocks per level (the coarsest level of each instance)
+        dense_.assign(T.mg.size(), DenseLevel{});
+        if (!T.dense_inv.empty()) {
+            dense_inv64_ = upload(T.dense_inv);
+            dense_inv32_ = upload_as<float>(T.dense_inv);
+        }
+        for (size_t m = 1; m < T.mg.size(); ++m) {
+            std::vector<int> off, n;
+            std::vector<int64_t> ioff;
+            for (int i = 0; i < ninst_; ++i)
+                if (T.dense_level[i] == static_cast<int>(m)) {
+                    off.push_back(T.dense_off[i]);
+                    n.push_back(T.dense_n[i]);
+                    ioff.push_back(T.dense_inv_off[i]);
+                }
+            if (off.empty()) continue;
+            DenseLevel& D = dense_[m];
+            D.count = static_cast<int>(off.size());
+            D.max_n = *std::max_element(n.begin(), n.end());
+            D.off = upload(off);
+            D.n = upload(n);
+            D.ioff = upload(ioff);
         }
     }
     if (opt_.vcycle_precision == PCFD_FP64) {
@@ -567,17 +644,7 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
         }
         for (int b = 0; b < nb_; ++b) if (sel[b]) list.push_back(b);
         if (list.empty()) throw std::invalid_argument("learned_bricks selects no brick (no special or cut cells)");
-        // block index of brick b in a brick-per-block launch of nb_ blocks (inverse of the
-        // XCD-aware order of brick_of_block), where the V-cycle's last kernel put its partials
-        const int per = nb_ >> 3, rem = nb_ & 7;
-        std::vector<int> slot;
-        for (int b : list) {
-            int x = 7;
-            while (x * per + std::min(x, rem) > b) --x;
-            slot.push_back((b - (x * per + std::min(x, rem))) * 8 + x);
-        }
         nn_list_ = upload(list);
-        nn_slot_ = upload(slot);
     } else {
         for (int b = 0; b < nb_; ++b) list.push_back(b);
     }
@@ -628,11 +695,7 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     const double w = opt_.smoother_weight;
     const unsigned g = blocks_for(L.n, 256);
     if (m == static_cast<int>(lv_.size()) - 1) {
-        const CT* inv = nullptr;
-        if constexpr (std::is_same<CT, double>::value) inv = inv64_; else inv = inv32_;
-        if (inv) {
-            LAUNCH((k_dense_solve<VT, CT>), L.n, 256, L.n, inv, B, cur, S_);
-        } else {  // no factorisation (pure Neumann): 64 sweeps with weight 2/3
+        if (!dense_blocks<VT, CT>(m, B, cur)) {  // no factorisation (pure Neumann): 64 sweeps with weight 2/3
             LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 0, cur, cur, B, 2.0 / 3.0);
             for (int s = 1; s < 64; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, 2.0 / 3.0); std::swap(cur, oth); }
         }
@@ -652,6 +715,9 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     LAUNCH((k_ell_post_prolong<VT, CT>), g, 256, C, cur, xc, B, oth, w);
     std::swap(cur, oth);
     for (int s = 1; s < opt_.post_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, w); std::swap(cur, oth); }
+    // instances of a batch whose hierarchy ends at this level: their cells take the exact
+    // coarse solve (what their own V-cycle does at its last level)
+    dense_blocks<VT, CT>(m, B, cur);
     return cur;
 }
 
@@ -701,9 +767,9 @@ void SolverImpl<R>::precondition(bool initial) {
     if (lv_.size() > 1) {
         LAUNCH(k_residual0<VT>, nb_, 512, M_, cur, r_, oth, S_);
         const CsrView& NC = std::is_same<VT, double>::value ? lv_[1].csr64 : lv_[1].csr32;
-        LAUNCH(k_restrict<VT>, blocks_for(lv_[1].n, 256), 256, NC, oth, static_cast<VT*>(lv_[1].B), S_);
+        LAUNCH(k_restrict<VT>, blocks_for(lv_[1].n, 256), 256, NC, oth, static_cast<VT*>(lv_[1].B), agg_);
         VT* xc = coarse_cycle<VT>(1);
-        LAUNCH(k_prolong<VT>, blocks_for(n_, 256), 256, n_, parent0_, cur, xc, S_);
+        LAUNCH(k_prolong<VT>, blocks_for(n_, 256), 256, n_, parent0_, cur, xc, agg_);
     }
     for (int s = 0; s < opt_.post_smooth; ++s) {
         const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
@@ -737,8 +803,8 @@ bool SolverImpl<R>::pcg_loop(bool watch_stall) {
     cg_iteration(true, dbuf_[0], dbuf_[1]);
     int enqueued = 1;
     sync_scalars();
-    std::vector<double> hist{host_s_->relres};
-    while (!host_s_->done && enqueued <= opt_.max_iterations + chunk_) {
+    std::vector<double> hist{host_s_[0].relres};
+    while (!hagg().done && enqueued <= opt_.max_iterations + chunk_) {
         if (use_graph_) {
             if (!graph_exec_) capture_chunk();
             HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
@@ -747,8 +813,8 @@ bool SolverImpl<R>::pcg_loop(bool watch_stall) {
         }
         enqueued += chunk_;
         sync_scalars();
-        if (watch_stall && !host_s_->done) {
-            const double rr = host_s_->relres;
+        if (watch_stall && !host_s_[0].done) {  // the learned solver runs single cases only
+            const double rr = host_s_[0].relres;
             if (!std::isfinite(rr)) return false;
             hist.push_back(rr);
             const size_t k = hist.size();
@@ -756,6 +822,26 @@ bool SolverImpl<R>::pcg_loop(bool watch_stall) {
         }
     }
     return true;
+}
+
+// stats of the batch from the per-instance records: the worst instance for iterations,
+// residual, divergence and rates, all instances for convergence, the smallest timestep
+template <class R>
+void SolverImpl<R>::aggregate_stats(pcfd_step_stats* st) const {
+    pcfd_step_stats a = inst_stats_[0];
+    for (int i = 1; i < ninst_; ++i) {
+        const pcfd_step_stats& s = inst_stats_[i];
+        a.iterations = std::max(a.iterations, s.iterations);
+        a.converged = a.converged && s.converged;
+        a.relative_residual = std::max(a.relative_residual, s.relative_residual);
+        a.max_abs_divergence = std::max(a.max_abs_divergence, s.max_abs_divergence);
+        a.max_abs_velocity = std::max(a.max_abs_velocity, s.max_abs_velocity);
+        a.cfl_rate = std::max(a.cfl_rate, s.cfl_rate);
+        a.diffusion_substeps = std::max(a.diffusion_substeps, s.diffusion_substeps);
+        a.dt = std::min(a.dt, s.dt);
+        a.fallback_used |= s.fallback_used;
+    }
+    *st = a;
 }
 
 template <class R>
@@ -766,30 +852,39 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     const bool learned = nn_ && nn_on_;
     set_nn_off(!learned);
     pcg_loop(learned);
-    int iterations = host_s_->iter;
+    int iterations = host_s_[0].iter;
     bool fell_back = false;
-    if (learned && !host_s_->converged) {
+    if (learned && !host_s_[0].converged) {
         // learned solve stalled or failed: plain MGPCG from its iterate on the same RHS
         fell_back = true;
         set_nn_off(true);
-        if (!std::isfinite(host_s_->relres)) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+        if (!std::isfinite(host_s_[0].relres)) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
         LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
         finalize(FIN_BNORM, nb_);
         pcg_loop(false);
-        iterations += host_s_->iter;
+        iterations += host_s_[0].iter;
     }
-    if (host_s_->zero_rhs) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    for (int i = 0; i < ninst_; ++i)
+        if (host_s_[i].zero_rhs) {
+            const int64_t d0 = static_cast<int64_t>(inst_brick_off_[i]) * 512, d1 = static_cast<int64_t>(inst_brick_off_[i + 1]) * 512;
+            HIP_CHECK(hipMemsetAsync(p_ + d0, 0, (d1 - d0) * sizeof(double), stream_));
+        }
     LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
     if (nedges_) LAUNCH(k_correct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), p_, S_);
     LAUNCH(k_divcheck<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_MAXDIV, nb_);
     sync_scalars();
+    for (int i = 0; i < ninst_; ++i) {
+        pcfd_step_stats& s = inst_stats_[i];
+        s.iterations = ninst_ == 1 ? iterations : host_s_[i].iter;
+        s.converged = host_s_[i].converged;
+        s.relative_residual = host_s_[i].relres;
+        s.max_abs_divergence = host_s_[i].maxdiv;
+        s.fallback_used = fell_back ? 1 : 0;
+        s.learned_used = learned ? 1 : 0;
+    }
     if (st) {
-        st->iterations = iterations;
-        st->converged = host_s_->converged;
-        st->relative_residual = host_s_->relres;
-        st->max_abs_divergence = host_s_->maxdiv;
-        st->fallback_used = fell_back ? 1 : 0;
+        aggregate_stats(st);
         st->learned_used = learned ? 1 : 0;
     }
 }
@@ -797,24 +892,25 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
 template <class R>
 void SolverImpl<R>::project(double dt, bool warm, pcfd_step_stats* st) {
     if (!(dt > 0) || !std::isfinite(dt)) throw std::invalid_argument("projection timestep must be positive");
-    if (st) std::memset(st, 0, sizeof(*st));
+    for (auto& s : inst_stats_) std::memset(&s, 0, sizeof(s));
     put_dt(dt);
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
-    run_projection(warm, st);
+    run_projection(warm, nullptr);
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipEventSynchronize(ev_[1]));
-    if (st) {
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-        st->project_ms = ms;
-        st->dt = dt;
-        st->physical_time = time;
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+    for (int i = 0; i < ninst_; ++i) {
+        inst_stats_[i].project_ms = ms;
+        inst_stats_[i].dt = dt;
+        inst_stats_[i].physical_time = inst_time_[i];
     }
+    if (st) aggregate_stats(st);
 }
 
 template <class R>
 void SolverImpl<R>::impulse(double dt) {
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, dt);
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, S_, dt, 0);
 }
 
 template <class R>
@@ -824,36 +920,39 @@ void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
         LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_);
         finalize(FIN_RATE, nb_);
     }
-    sync_scalars();
-    dt = host_s_->dt;
-    if (dt_used) *dt_used = dt;
+    advect_core();
+    if (dt_used) *dt_used = host_s_[0].dt;
+    if (st) st->diffusion_substeps = inst_stats_[0].diffusion_substeps;
+}
+
+// transport with each instance's dt (already in the device scalars), diffusion substeps,
+// face reconstruction
+template <class R>
+void SolverImpl<R>::advect_core() {
     LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_);
     for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
-    int subs = 0;
-    if (opt_.smagorinsky_cs > 0) {
+    const bool smag = opt_.smagorinsky_cs > 0;
+    if (smag) {
         // cell viscosity from the transported velocity, substeps from the largest rate
         LAUNCH(k_viscosity<R>, nb_, 512, M_, fields(), smag_, opt_.nu, opt_.smagorinsky_cs, nu_);
         LAUNCH(k_diff_rate<R>, nb_, 512, M_, nu_, partials_);
         finalize(FIN_DIFFRATE, nb_);
-        sync_scalars();
-        const double rate = host_s_->diff_rate;
-        if (!std::isfinite(rate)) throw std::domain_error("non-finite eddy viscosity");
-        subs = std::max(1, static_cast<int>(std::ceil(dt * rate / 0.7)));
-        const double dts = dt / subs;
-        for (int s = 0; s < subs; ++s) {
-            LAUNCH(k_diffuse_var<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], nu_, dts);
-            for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
-        }
-    } else if (opt_.nu > 0) {
-        subs = std::max(1, static_cast<int>(std::ceil(dt * opt_.nu * max_diff_coef_ / 0.7)));
-        const double dts = dt / subs;
-        for (int s = 0; s < subs; ++s) {
-            LAUNCH(k_diffuse<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], dts);
-            for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
-        }
+    }
+    LAUNCH(k_diffusion_substeps, 1, 64, S_, ninst_, opt_.nu, inst_maxdiff_dev_, smag ? 1 : 0);
+    sync_scalars();
+    int subs = 0;
+    for (int i = 0; i < ninst_; ++i) {
+        if (smag && !std::isfinite(host_s_[i].diff_rate)) throw std::domain_error("non-finite eddy viscosity");
+        subs = std::max(subs, host_s_[i].subs);
+        inst_stats_[i].diffusion_substeps = host_s_[i].subs;
+        inst_stats_[i].dt = host_s_[i].dt;
+    }
+    for (int s = 0; s < subs; ++s) {
+        if (smag) LAUNCH(k_diffuse_var<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], nu_, S_, s);
+        else LAUNCH(k_diffuse<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], S_, s);
+        for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
     }
     reconstruct();
-    if (st) st->diffusion_substeps = subs;
 }
 
 template <class R>
@@ -862,77 +961,91 @@ void SolverImpl<R>::initialize(pcfd_step_stats* st) {
     reconstruct();
     const double dt = opt_.cfl * M_.finest_h / std::max(1e-9, std::abs(opt_.speed));
     HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
-    pcfd_step_stats local{};
-    project(dt, false, &local);
-    if (local.converged) impulse(dt);
+    project(dt, false, nullptr);
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, S_, dt, 1);  // converged instances only
     LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_VMAX, nb_);
     sync_scalars();
-    local.max_abs_velocity = host_s_->vmax;
-    initialized = local.converged != 0;
+    bool all = true;
+    for (int i = 0; i < ninst_; ++i) {
+        inst_stats_[i].max_abs_velocity = host_s_[i].vmax;
+        inst_stats_[i].physical_time = inst_time_[i] = 0.0;
+        all = all && inst_stats_[i].converged;
+    }
+    initialized = all;
     time = 0.0;
-    local.physical_time = time;
-    if (st) *st = local;
+    if (st) aggregate_stats(st);
 }
 
 template <class R>
 void SolverImpl<R>::step(pcfd_step_stats* st) {
     if (!initialized) throw std::logic_error("step before a converged initialization");
-    pcfd_step_stats local{};
+    for (auto& s : inst_stats_) std::memset(&s, 0, sizeof(s));
     HIP_CHECK(hipEventRecord(ev_[2], stream_));
     LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_VMAX, nb_);
     LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_RATE, nb_);
     sync_scalars();
-    if (!std::isfinite(host_s_->vmax)) throw std::domain_error("non-finite velocity before timestep");
-    const double dt = host_s_->dt;
-    local.cfl_rate = host_s_->rate;
-    local.max_abs_velocity = host_s_->vmax;
-    advect(dt, nullptr, &local);
+    for (int i = 0; i < ninst_; ++i) {
+        if (!std::isfinite(host_s_[i].vmax)) throw std::domain_error("non-finite velocity before timestep");
+        inst_stats_[i].cfl_rate = host_s_[i].rate;
+        inst_stats_[i].max_abs_velocity = host_s_[i].vmax;
+    }
+    advect_core();
     HIP_CHECK(hipEventRecord(ev_[3], stream_));
-    pcfd_step_stats ps{};
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
-    run_projection(true, &ps);
-    if (ps.converged) { impulse(dt); time += dt; }
+    run_projection(true, nullptr);
+    // impulse and time advance for the instances whose projection converged
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, S_, 0.0, 1);
+    for (int i = 0; i < ninst_; ++i)
+        if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
+    time = inst_time_[0];
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipEventSynchronize(ev_[1]));
     float a_ms = 0, p_ms = 0, s_ms = 0;
     HIP_CHECK(hipEventElapsedTime(&a_ms, ev_[2], ev_[3]));
     HIP_CHECK(hipEventElapsedTime(&p_ms, ev_[0], ev_[1]));
     HIP_CHECK(hipEventElapsedTime(&s_ms, ev_[2], ev_[1]));
-    local.dt = dt;
-    local.iterations = ps.iterations;
-    local.converged = ps.converged;
-    local.relative_residual = ps.relative_residual;
-    local.max_abs_divergence = ps.max_abs_divergence;
-    local.fallback_used = ps.fallback_used;
-    local.learned_used = ps.learned_used;
-    validate_learned(ps.learned_used != 0, p_ms);
-    local.learned_rejected = nn_rejected_ ? 1 : 0;
-    local.advect_ms = a_ms;
-    local.project_ms = p_ms;
-    local.step_ms = s_ms;
-    local.physical_time = time;
-    if (st) *st = local;
+    const bool used = inst_stats_[0].learned_used != 0;
+    validate_learned(used, p_ms);
+    for (int i = 0; i < ninst_; ++i) {
+        pcfd_step_stats& s = inst_stats_[i];
+        s.learned_rejected = nn_rejected_ ? 1 : 0;
+        s.advect_ms = a_ms;
+        s.project_ms = p_ms;
+        s.step_ms = s_ms;
+        s.physical_time = inst_time_[i];
+    }
+    if (st) aggregate_stats(st);
+}
+
+// surface loads of one instance over its patch range (FP64, deterministic)
+template <class R>
+void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
+    std::memset(out, 0, sizeof(*out));
+    const int p0 = inst_patch_off_[i], p1 = inst_patch_off_[i + 1];
+    if (p1 > p0) {
+        const unsigned gl = blocks_for(p1 - p0, 256);
+        const double* o = &inst_origin_[3 * static_cast<size_t>(i)];
+        LAUNCH(k_loads, gl, 256, M_, p_, pref, o[0], o[1], o[2], p0, p1, partials_);
+        LAUNCH(k_finalize, 1, 1024, FIN_LOADS, partials_, static_cast<int>(gl), static_cast<const int*>(nullptr), S_ + i,
+               static_cast<DevScalars*>(nullptr), opt_.cfl, 1.0, opt_.rtol, opt_.max_iterations, 0);
+        sync_scalars();
+        for (int c = 0; c < 3; ++c) { out->force[c] = host_s_[i].loads[c]; out->moment[c] = host_s_[i].loads[3 + c]; }
+    }
+    out->drag = out->force[0]; out->side = out->force[1]; out->lift = out->force[2];
+    const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed, area = inst_ref_area_[i];
+    if (q > 0 && area > 0) {
+        out->coefficients_valid = 1;
+        out->cd = out->drag / (q * area); out->cs = out->side / (q * area); out->cl = out->lift / (q * area);
+    }
 }
 
 template <class R>
 void SolverImpl<R>::loads(double pref, pcfd_loads* out) {
-    std::memset(out, 0, sizeof(*out));
-    if (M_.npatches > 0) {
-        const unsigned gl = blocks_for(M_.npatches, 256);
-        LAUNCH(k_loads, gl, 256, M_, p_, pref, origin[0], origin[1], origin[2], partials_);
-        finalize(FIN_LOADS, static_cast<int>(gl));
-        sync_scalars();
-        for (int c = 0; c < 3; ++c) { out->force[c] = host_s_->loads[c]; out->moment[c] = host_s_->loads[3 + c]; }
-    }
-    out->drag = out->force[0]; out->side = out->force[1]; out->lift = out->force[2];
-    const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed;
-    if (q > 0 && ref_area > 0) {
-        out->coefficients_valid = 1;
-        out->cd = out->drag / (q * ref_area); out->cs = out->side / (q * ref_area); out->cl = out->lift / (q * ref_area);
-    }
+    if (ninst_ != 1) throw std::invalid_argument("a batch reports loads per instance (pcfd_integrate_loads_instance)");
+    loads_instance(0, pref, out);
 }
 
 template <class R>
@@ -1007,11 +1120,12 @@ template <class R>
 void SolverImpl<R>::mg_primitive(int op, int level, const double* x, const double* b, double* out, double omega, int sweeps) {
     if (level < 0 || level >= std::max<int>(1, static_cast<int>(lv_.size()))) throw std::invalid_argument("multigrid level out of range");
     if (opt_.vcycle_precision != PCFD_FP64) throw std::invalid_argument("multigrid primitives run in FP64 only");
+    if (ninst_ != 1) throw std::invalid_argument("multigrid primitives run on single cases");
     const int64_t n = level == 0 ? n_ : lv_[level].n;
     HIP_CHECK(hipStreamSynchronize(stream_));
     // clear the loop flags so the kernels run
-    std::memset(host_s_, 0, sizeof(DevScalars));
-    HIP_CHECK(hipMemcpy(S_, host_s_, sizeof(DevScalars), hipMemcpyHostToDevice));
+    std::memset(host_s_, 0, (ninst_ + 1) * sizeof(DevScalars));
+    HIP_CHECK(hipMemcpy(S_, host_s_, (ninst_ + 1) * sizeof(DevScalars), hipMemcpyHostToDevice));
     double* dx = level == 0 ? static_cast<double*>(vc0_[0]) : static_cast<double*>(lv_[level].X);
     double* dt = level == 0 ? static_cast<double*>(vc0_[1]) : static_cast<double*>(lv_[level].T);
     double* db = level == 0 ? r_ : static_cast<double*>(lv_[level].B);
@@ -1051,10 +1165,12 @@ template <class R>
 void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* bytes) {
     if (reps < 1) throw std::invalid_argument("reps must be positive");
     HIP_CHECK(hipStreamSynchronize(stream_));
-    std::memset(host_s_, 0, sizeof(DevScalars));
-    host_s_->dt = 1e-3;
-    host_s_->rz = 1.0;
-    HIP_CHECK(hipMemcpy(S_, host_s_, sizeof(DevScalars), hipMemcpyHostToDevice));
+    std::memset(host_s_, 0, (ninst_ + 1) * sizeof(DevScalars));
+    for (int i = 0; i < ninst_; ++i) {
+        host_s_[i].dt = 1e-3;
+        host_s_[i].rz = 1.0;
+    }
+    HIP_CHECK(hipMemcpy(S_, host_s_, (ninst_ + 1) * sizeof(DevScalars), hipMemcpyHostToDevice));
     int64_t nact = 0;
     {
         std::vector<uint8_t> m(n_);
@@ -1075,7 +1191,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
                 break;
             case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); break;
             case 3: precondition_dispatch(true); break;
-            case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, nn_slot_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
+            case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
             default: throw std::invalid_argument("unknown kernel id");
         }
     };
@@ -1107,8 +1223,8 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
     }
     *bytes = per * static_cast<double>(nact);
     // leave the loop flags as a finished solve so a stray kernel does nothing
-    host_s_->done = 1;
-    HIP_CHECK(hipMemcpy(&S_->done, &host_s_->done, sizeof(int), hipMemcpyHostToDevice));
+    for (int i = 0; i <= ninst_; ++i) host_s_[i].done = 1;
+    HIP_CHECK(hipMemcpy(S_, host_s_, (ninst_ + 1) * sizeof(DevScalars), hipMemcpyHostToDevice));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1155,7 +1271,7 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
         sync_scalars();
         // r' = r - A M_mg(r) of the current iterate; recorded while the solve runs
         auto record = [&] {
-            if (host_s_->done || recorded >= max_pairs) return;
+            if (host_s_[0].done || recorded >= max_pairs) return;
             const int slot = nn_next_;
             if (slot >= static_cast<int>(nn_rs_.size())) {
                 nn_rs_.push_back(alloc<double>(n_));
@@ -1174,7 +1290,7 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
         cg_iteration(true, dbuf_[0], dbuf_[1]);
         sync_scalars();
         record();
-        for (int it = 1; !host_s_->done && recorded < max_pairs && it <= opt_.max_iterations; ++it) {
+        for (int it = 1; !host_s_[0].done && recorded < max_pairs && it <= opt_.max_iterations; ++it) {
             cg_iteration(false, dbuf_[it & 1], dbuf_[(it + 1) & 1]);
             sync_scalars();
             record();
@@ -1379,7 +1495,6 @@ pcfd_status pcfd_create(const pcfd_topology* topology, const pcfd_options* optio
         if (options->field_precision == PCFD_FP32) s->impl = std::make_unique<pcfd::SolverImpl<float>>(T, *options);
         else if (options->field_precision == PCFD_FP64) s->impl = std::make_unique<pcfd::SolverImpl<double>>(T, *options);
         else throw std::invalid_argument("field precision must be FP64 or FP32");
-        s->impl->ref_area = T.reference_area;
         *out = s.release();
     });
 }
@@ -1416,9 +1531,26 @@ pcfd_status pcfd_integrate_loads(pcfd_solver* s, double pref, pcfd_loads* loads)
 pcfd_status pcfd_set_reference(pcfd_solver* s, double area, const double origin[3]) {
     NEED(s, "pcfd_set_reference: null solver");
     NEED(area >= 0 && std::isfinite(area), "pcfd_set_reference: invalid area");
-    s->impl->ref_area = area;
-    if (origin) for (int c = 0; c < 3; ++c) s->impl->origin[c] = origin[c];
+    s->impl->set_reference(-1, area, origin);
     return PCFD_OK;
+}
+pcfd_status pcfd_set_reference_instance(pcfd_solver* s, int instance, double area, const double origin[3]) {
+    NEED(s, "pcfd_set_reference_instance: null solver");
+    NEED(area >= 0 && std::isfinite(area), "pcfd_set_reference_instance: invalid area");
+    NEED(instance >= 0 && instance < s->impl->instances(), "pcfd_set_reference_instance: no such instance");
+    s->impl->set_reference(instance, area, origin);
+    return PCFD_OK;
+}
+pcfd_status pcfd_instance_stats(const pcfd_solver* s, int instance, pcfd_step_stats* stats) {
+    NEED(s && stats, "pcfd_instance_stats: null argument");
+    NEED(instance >= 0 && instance < s->impl->instances(), "pcfd_instance_stats: no such instance");
+    s->impl->instance_stats(instance, stats);
+    return PCFD_OK;
+}
+pcfd_status pcfd_integrate_loads_instance(pcfd_solver* s, int instance, double pref, pcfd_loads* loads) {
+    NEED(s && loads, "pcfd_integrate_loads_instance: null argument");
+    NEED(instance >= 0 && instance < s->impl->instances(), "pcfd_integrate_loads_instance: no such instance");
+    return guarded("pcfd_integrate_loads_instance", [&] { s->impl->instance_loads(instance, pref, loads); });
 }
 pcfd_status pcfd_get_velocity(pcfd_solver* s, double* x, double* y, double* z) {
     NEED(s, "pcfd_get_velocity: null solver");

@@ -1168,9 +1168,161 @@ Topology Builder::run() {
 
 }  // namespace
 
+// instance tables of a single case
+static void set_single_instance(Topology& T) {
+    T.inst_brick_off = {0, T.nbricks};
+    T.inst_patch_off = {0, static_cast<int>(T.patch_dof.size())};
+    T.inst_active = {T.n_active};
+    T.inst_reference_area = {T.reference_area};
+    T.inst_body_volume = {T.body_volume};
+    T.inst_max_diffusion = {T.max_diffusion_coefficient};
+    T.mg_inst_off.assign(T.mg.size(), {});
+    for (size_t m = 1; m < T.mg.size(); ++m) T.mg_inst_off[m] = {0, T.mg[m].n};
+    const int last = static_cast<int>(T.mg.size()) - 1;
+    if (last >= 1 && !T.mg.back().coarse_inv.empty()) {
+        T.dense_level = {last};
+        T.dense_off = {0};
+        T.dense_n = {T.mg.back().n};
+        T.dense_inv_off = {0};
+        T.dense_inv = T.mg.back().coarse_inv;
+    } else {
+        T.dense_level = {-1};
+        T.dense_off = {0};
+        T.dense_n = {0};
+        T.dense_inv_off = {0};
+    }
+}
+
 Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid) {
     Builder b(geometry, grid);
-    return b.run();
+    Topology T = b.run();
+    set_single_instance(T);
+    return T;
+}
+
+Topology merge_topologies(const std::vector<const Topology*>& parts) {
+    if (parts.empty()) throw BuildError("merge: no topology");
+    const Topology& P0 = *parts[0];
+    Topology T;
+    for (int a = 0; a < 3; ++a) { T.lo[a] = P0.lo[a]; T.base_bricks[a] = P0.base_bricks[a]; }
+    T.h0 = P0.h0;
+    T.levels = P0.levels;
+    T.outlet = P0.outlet;
+    T.level_h = P0.level_h;
+    T.finest_h = P0.finest_h;
+    size_t depth = 0;
+    for (const Topology* p : parts) {
+        if (p->levels != P0.levels || p->level_h != P0.level_h || p->outlet != P0.outlet)
+            throw BuildError("merge: instances need the same AMR levels, cell sizes and outlet");
+        if (p->instances() != 1) throw BuildError("merge: parts must be single cases");
+        if (p->dense_level[0] < 0) throw BuildError("merge: every instance needs a dense coarse inverse");
+        depth = std::max(depth, p->mg.size());
+    }
+    T.mg.resize(depth);
+    T.mg_inst_off.assign(depth, {});
+    for (size_t m = 1; m < depth; ++m) T.mg_inst_off[m].push_back(0);
+    T.inst_brick_off.push_back(0);
+    T.inst_patch_off.push_back(0);
+    T.level_brick_begin.assign(T.levels + 1, 0);
+    int64_t dof0 = 0;
+    int brick0 = 0, special0 = 0, edge0 = 0, patch0 = 0, inc0 = 0, lsq0 = 0;
+    std::vector<int> cell0(depth, 0), nnz0(depth, 0), child0(depth, 0);
+    auto shift = [](int v, int64_t by) { return v >= 0 ? static_cast<int>(v + by) : v; };
+    for (size_t i = 0; i < parts.size(); ++i) {
+        const Topology& P = *parts[i];
+        // bricks and cells
+        T.brick_level.insert(T.brick_level.end(), P.brick_level.begin(), P.brick_level.end());
+        T.brick_coord.insert(T.brick_coord.end(), P.brick_coord.begin(), P.brick_coord.end());
+        T.brick_flags.insert(T.brick_flags.end(), P.brick_flags.begin(), P.brick_flags.end());
+        for (int v : P.brick_nbr) T.brick_nbr.push_back(shift(v, brick0));
+        T.mask.insert(T.mask.end(), P.mask.begin(), P.mask.end());
+        T.centroid.insert(T.centroid.end(), P.centroid.begin(), P.centroid.end());
+        // special cells
+        for (int v : P.special_dof) T.special_dof.push_back(static_cast<int>(v + dof0));
+        T.special_volume.insert(T.special_volume.end(), P.special_volume.begin(), P.special_volume.end());
+        if (T.special_inc_off.empty()) T.special_inc_off.push_back(0);
+        for (size_t k = 1; k < P.special_inc_off.size(); ++k) T.special_inc_off.push_back(P.special_inc_off[k] + inc0);
+        for (int raw : P.special_inc) T.special_inc.push_back(raw >= 0 ? raw + edge0 : ~(~raw + edge0));
+        for (int v : P.special_patch) T.special_patch.push_back(shift(v, patch0));
+        if (T.brick_special_off.empty()) T.brick_special_off.push_back(0);
+        for (size_t b = 1; b < P.brick_special_off.size(); ++b) T.brick_special_off.push_back(P.brick_special_off[b] + special0);
+        // compact edges
+        for (int v : P.edge_a) T.edge_a.push_back(static_cast<int>(v + dof0));
+        for (int v : P.edge_b) T.edge_b.push_back(static_cast<int>(v + dof0));
+        T.edge_axis.insert(T.edge_axis.end(), P.edge_axis.begin(), P.edge_axis.end());
+        T.edge_area.insert(T.edge_area.end(), P.edge_area.begin(), P.edge_area.end());
+        T.edge_coef.insert(T.edge_coef.end(), P.edge_coef.begin(), P.edge_coef.end());
+        T.edge_w.insert(T.edge_w.end(), P.edge_w.begin(), P.edge_w.end());
+        // surface patches
+        for (int v : P.patch_dof) T.patch_dof.push_back(static_cast<int>(v + dof0));
+        T.patch_S.insert(T.patch_S.end(), P.patch_S.begin(), P.patch_S.end());
+        T.patch_x.insert(T.patch_x.end(), P.patch_x.begin(), P.patch_x.end());
+        if (T.patch_lsq_off.empty()) T.patch_lsq_off.push_back(0);
+        for (size_t k = 1; k < P.patch_lsq_off.size(); ++k) T.patch_lsq_off.push_back(P.patch_lsq_off[k] + lsq0);
+        for (int v : P.patch_lsq_dof) T.patch_lsq_dof.push_back(static_cast<int>(v + dof0));
+        T.patch_lsq_beta.insert(T.patch_lsq_beta.end(), P.patch_lsq_beta.begin(), P.patch_lsq_beta.end());
+        // multigrid: level 0 carries the parent map and the cell exponents
+        const int pdepth = static_cast<int>(P.mg.size());
+        for (int v : P.mg[0].parent) T.mg[0].parent.push_back(pdepth > 1 ? shift(v, cell0[1]) : -1);
+        T.mg[0].exponent.insert(T.mg[0].exponent.end(), P.mg[0].exponent.begin(), P.mg[0].exponent.end());
+        for (size_t m = 1; m < depth; ++m) {
+            MGLevel& D = T.mg[m];
+            if (static_cast<int>(m) < pdepth) {
+                const MGLevel& S = P.mg[m];
+                if (D.row_off.empty()) D.row_off.push_back(0);
+                for (int r = 0; r < S.n; ++r) D.row_off.push_back(S.row_off[r + 1] + nnz0[m]);
+                for (int v : S.col) D.col.push_back(v + cell0[m]);
+                D.coef.insert(D.coef.end(), S.coef.begin(), S.coef.end());
+                D.diag.insert(D.diag.end(), S.diag.begin(), S.diag.end());
+                for (int v : S.parent) D.parent.push_back(static_cast<int>(m) + 1 < pdepth ? shift(v, cell0[m + 1]) : -1);
+                if (D.child_off.empty()) D.child_off.push_back(0);
+                for (int c = 0; c < S.n; ++c) D.child_off.push_back(S.child_off[c + 1] + child0[m]);
+                const int below = m == 1 ? 0 : cell0[m - 1];
+                for (int v : S.child) D.child.push_back(m == 1 ? static_cast<int>(v + dof0) : v + below);
+                D.exponent.insert(D.exponent.end(), S.exponent.begin(), S.exponent.end());
+                D.n += S.n;
+                nnz0[m] += static_cast<int>(S.col.size());
+                child0[m] += static_cast<int>(S.child.size());
+            }
+            T.mg_inst_off[m].push_back(D.n);
+        }
+        // dense coarse block of this instance
+        T.dense_level.push_back(P.dense_level[0]);
+        T.dense_off.push_back(T.mg_inst_off[P.dense_level[0]][i]);
+        T.dense_n.push_back(P.dense_n[0]);
+        T.dense_inv_off.push_back(static_cast<int64_t>(T.dense_inv.size()));
+        T.dense_inv.insert(T.dense_inv.end(), P.dense_inv.begin(), P.dense_inv.end());
+        // per-instance facts
+        T.inst_active.push_back(P.n_active);
+        T.inst_reference_area.push_back(P.reference_area);
+        T.inst_body_volume.push_back(P.body_volume);
+        T.inst_max_diffusion.push_back(P.max_diffusion_coefficient);
+        T.n_active += P.n_active;
+        T.body_volume += P.body_volume;
+        T.max_diffusion_coefficient = std::max(T.max_diffusion_coefficient, P.max_diffusion_coefficient);
+        // advance the offsets
+        for (size_t m = 1; m < depth; ++m) cell0[m] = T.mg_inst_off[m].back();
+        dof0 += P.ndof;
+        brick0 += P.nbricks;
+        special0 += static_cast<int>(P.special_dof.size());
+        edge0 += static_cast<int>(P.edge_a.size());
+        patch0 += static_cast<int>(P.patch_dof.size());
+        inc0 += static_cast<int>(P.special_inc.size());
+        lsq0 += static_cast<int>(P.patch_lsq_dof.size());
+        T.inst_brick_off.push_back(brick0);
+        T.inst_patch_off.push_back(patch0);
+    }
+    T.nbricks = brick0;
+    T.ndof = dof0;
+    T.mg[0].n = static_cast<int>(dof0);
+    T.reference_area = P0.reference_area;
+    if (T.special_inc_off.empty()) T.special_inc_off.push_back(0);
+    if (T.patch_lsq_off.empty()) T.patch_lsq_off.push_back(0);
+    for (size_t m = 1; m < depth; ++m) {
+        if (T.mg[m].row_off.empty()) T.mg[m].row_off.push_back(0);
+        if (T.mg[m].child_off.empty()) T.mg[m].child_off.push_back(0);
+    }
+    return T;
 }
 
 }  // namespace pcfd

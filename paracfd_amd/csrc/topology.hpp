@@ -91,6 +91,21 @@ struct Topology {
     double reference_area = 0;
     double max_diffusion_coefficient = 0; // max over cells of sum(coef)/V (for diffusion substeps)
     double finest_h = 0;
+
+    // Batch of independent instances (a single case is a batch of one).  Instance i owns the
+    // bricks [inst_brick_off[i], inst_brick_off[i+1]) and, contiguously, their dofs, special
+    // cells, compact edges and patches; no face, edge or stencil crosses instances.
+    std::vector<int> inst_brick_off;        // ninst+1
+    std::vector<int> inst_patch_off;        // ninst+1
+    std::vector<int64_t> inst_active;       // active cells per instance
+    std::vector<double> inst_reference_area, inst_body_volume, inst_max_diffusion;
+    // multigrid level m >= 1: instance i's cells are [mg_inst_off[m][i], mg_inst_off[m][i+1])
+    std::vector<std::vector<int>> mg_inst_off;
+    // coarsest level of each instance with its dense inverse (level -1: none)
+    std::vector<int> dense_level, dense_off, dense_n;
+    std::vector<int64_t> dense_inv_off;
+    std::vector<double> dense_inv;
+    int instances() const { return static_cast<int>(inst_brick_off.size()) - 1; }
 };
 
 struct BuildError : std::exception {
@@ -100,5 +115,9 @@ struct BuildError : std::exception {
 };
 
 Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid);
+
+// Concatenate independent topologies into one batch (same AMR levels and cell sizes; each
+// part needs its dense coarse inverse).  Every index is offset into the merged numbering.
+Topology merge_topologies(const std::vector<const Topology*>& parts);
 
 }  // namespace pcfd
