@@ -265,6 +265,8 @@ pcfd_status pcfd_set_reference(pcfd_solver* solver, double area, const double or
  * instance (pcfd_integrate_loads is for single cases).  No counterpart in the reference. */
 pcfd_status pcfd_instance_stats(const pcfd_solver* solver, int instance, pcfd_step_stats* stats);
 pcfd_status pcfd_integrate_loads_instance(pcfd_solver* solver, int instance, double pref, pcfd_loads* loads);
+/* Loads of every instance (loads[count], count == instances) with one device synchronisation. */
+pcfd_status pcfd_integrate_loads_batch(pcfd_solver* solver, double pref, pcfd_loads* loads, int count);
 pcfd_status pcfd_set_reference_instance(pcfd_solver* solver, int instance, double area, const double origin[3]);
 
 /* State I/O (upload_state/download_state amr_advection.cu:1301-1309, upload_pressure

@@ -113,7 +113,7 @@ EXPORTED = [
     "pcfd_mg_primitive", "pcfd_time_kernel", "pcfd_learned_collect", "pcfd_learned_train",
     "pcfd_learned_get_weights", "pcfd_learned_set_weights", "pcfd_learned_enable", "pcfd_learned_forward",
     "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
-    "pcfd_integrate_loads_instance", "pcfd_set_reference_instance",
+    "pcfd_integrate_loads_instance", "pcfd_set_reference_instance", "pcfd_integrate_loads_batch",
 ]
 
 _lib = None
@@ -177,6 +177,7 @@ def load():
         "pcfd_instance_stats": (C.c_int, [P, C.c_int, C.POINTER(StepStats)]),
         "pcfd_integrate_loads_instance": (C.c_int, [P, C.c_int, C.c_double, C.POINTER(Loads)]),
         "pcfd_set_reference_instance": (C.c_int, [P, C.c_int, C.c_double, dp]),
+        "pcfd_integrate_loads_batch": (C.c_int, [P, C.c_double, C.POINTER(Loads), C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

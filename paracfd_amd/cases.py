@@ -74,6 +74,29 @@ CASES = {
 }
 
 
+# Angle-of-attack sweeps run as one batch of instances (one merged topology).  The 4412 sweep:
+# 16 wings at alpha = -4 + 1.2 k deg, 2-level AMR with the 0.3-chord band, cells_x = 200
+# (~3.6 M active cells per wing, ~58 M in total).
+SWEEPS = {
+    "naca4412_sweep16": dict(args=dict(digits=4412, cells_x=200, levels=2, bands=(0.3,)),
+                             alphas=tuple(-4.0 + 1.2 * k for k in range(16)),
+                             rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=300),
+}
+
+
+def build_sweep(name, **overrides):
+    """Topologies of every instance of a sweep (built one after another; each build is
+    multithreaded) and the sweep's parameters."""
+    spec = dict(SWEEPS[name])
+    args = dict(spec["args"])
+    args.update(overrides)
+    parts = []
+    for a in spec["alphas"]:
+        g, d = naca_wing(alpha_deg=a, **args)
+        parts.append(L.Topology(g, d))
+    return parts, spec
+
+
 def build_case(name, **overrides):
     spec = dict(CASES[name])
     args = dict(spec["args"])

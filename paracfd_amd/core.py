@@ -144,6 +144,12 @@ class ExternalAeroCore:
         L.check(self._lib.pcfd_integrate_loads_instance(self.handle, i, pref, C.byref(out)))
         return out.as_dict()
 
+    def batch_loads(self, pref=0.0) -> list:
+        n = self.topology.instances
+        out = (L.Loads * n)()
+        L.check(self._lib.pcfd_integrate_loads_batch(self.handle, pref, out, n))
+        return [out[i].as_dict() for i in range(n)]
+
     def set_instance_reference(self, i, area, origin=(0.0, 0.0, 0.0)):
         o = (C.c_double * 3)(*origin)
         L.check(self._lib.pcfd_set_reference_instance(self.handle, i, area, o))

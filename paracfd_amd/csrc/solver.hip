@@ -76,6 +76,7 @@ public:
     virtual void set_reference(int instance, double area, const double* origin) = 0;
     virtual void instance_stats(int instance, pcfd_step_stats* st) const = 0;
     virtual void instance_loads(int instance, double pref, pcfd_loads* out) = 0;
+    virtual void batch_loads(double pref, pcfd_loads* out) = 0;  // every instance, one sync
     virtual int instances() const = 0;
     bool initialized = false;
     double time = 0.0;
@@ -116,6 +117,7 @@ public:
     }
     void instance_stats(int instance, pcfd_step_stats* st) const override { *st = inst_stats_.at(instance); }
     void instance_loads(int instance, double pref, pcfd_loads* out) override { loads_instance(instance, pref, out); }
+    void batch_loads(double pref, pcfd_loads* out) override;
     int instances() const override { return ninst_; }
 
 private:
@@ -255,6 +257,8 @@ private:
     void aggregate_stats(pcfd_step_stats* st) const;
     void advect_core();
     void loads_instance(int i, double pref, pcfd_loads* out);
+    void enqueue_loads(int i, double pref);
+    void read_loads(int i, pcfd_loads* out) const;
     void reconstruct();
     void run_projection(bool warm, pcfd_step_stats* st);
     template <class VT> void precondition(bool initial);
@@ -1020,18 +1024,24 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     if (st) aggregate_stats(st);
 }
 
-// surface loads of one instance over its patch range (FP64, deterministic)
+// surface loads of one instance over its patch range (FP64, deterministic): the patch
+// partials and their reduction into the instance's scalar record, enqueued on the stream
 template <class R>
-void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
-    std::memset(out, 0, sizeof(*out));
+void SolverImpl<R>::enqueue_loads(int i, double pref) {
     const int p0 = inst_patch_off_[i], p1 = inst_patch_off_[i + 1];
-    if (p1 > p0) {
-        const unsigned gl = blocks_for(p1 - p0, 256);
-        const double* o = &inst_origin_[3 * static_cast<size_t>(i)];
-        LAUNCH(k_loads, gl, 256, M_, p_, pref, o[0], o[1], o[2], p0, p1, partials_);
-        LAUNCH(k_finalize, 1, 1024, FIN_LOADS, partials_, static_cast<int>(gl), static_cast<const int*>(nullptr), S_ + i,
-               static_cast<DevScalars*>(nullptr), opt_.cfl, 1.0, opt_.rtol, opt_.max_iterations, 0);
-        sync_scalars();
+    if (p1 <= p0) return;
+    const unsigned gl = blocks_for(p1 - p0, 256);
+    const double* o = &inst_origin_[3 * static_cast<size_t>(i)];
+    LAUNCH(k_loads, gl, 256, M_, p_, pref, o[0], o[1], o[2], p0, p1, partials_);
+    LAUNCH(k_finalize, 1, 1024, FIN_LOADS, partials_, static_cast<int>(gl), static_cast<const int*>(nullptr), S_ + i,
+           static_cast<DevScalars*>(nullptr), opt_.cfl, 1.0, opt_.rtol, opt_.max_iterations, 0);
+}
+
+// forces, moments and coefficients of instance i from its record (after sync_scalars)
+template <class R>
+void SolverImpl<R>::read_loads(int i, pcfd_loads* out) const {
+    std::memset(out, 0, sizeof(*out));
+    if (inst_patch_off_[i + 1] > inst_patch_off_[i]) {
         for (int c = 0; c < 3; ++c) { out->force[c] = host_s_[i].loads[c]; out->moment[c] = host_s_[i].loads[3 + c]; }
     }
     out->drag = out->force[0]; out->side = out->force[1]; out->lift = out->force[2];
@@ -1040,6 +1050,20 @@ void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
         out->coefficients_valid = 1;
         out->cd = out->drag / (q * area); out->cs = out->side / (q * area); out->cl = out->lift / (q * area);
     }
+}
+
+template <class R>
+void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
+    enqueue_loads(i, pref);
+    sync_scalars();
+    read_loads(i, out);
+}
+
+template <class R>
+void SolverImpl<R>::batch_loads(double pref, pcfd_loads* out) {
+    for (int i = 0; i < ninst_; ++i) enqueue_loads(i, pref);
+    sync_scalars();
+    for (int i = 0; i < ninst_; ++i) read_loads(i, out + i);
 }
 
 template <class R>
@@ -1547,6 +1571,12 @@ pcfd_status pcfd_instance_stats(const pcfd_solver* s, int instance, pcfd_step_st
     s->impl->instance_stats(instance, stats);
     return PCFD_OK;
 }
+pcfd_status pcfd_integrate_loads_batch(pcfd_solver* s, double pref, pcfd_loads* loads, int count) {
+    NEED(s && loads, "pcfd_integrate_loads_batch: null argument");
+    NEED(count == s->impl->instances(), "pcfd_integrate_loads_batch: count must equal the number of instances");
+    return guarded("pcfd_integrate_loads_batch", [&] { s->impl->batch_loads(pref, loads); });
+}
+
 pcfd_status pcfd_integrate_loads_instance(pcfd_solver* s, int instance, double pref, pcfd_loads* loads) {
     NEED(s && loads, "pcfd_integrate_loads_instance: null argument");
     NEED(instance >= 0 && instance < s->impl->instances(), "pcfd_integrate_loads_instance: no such instance");

@@ -130,3 +130,10 @@ def test_batch_matches_individual_runs(parts, merged):
         bl, sl_ = batch.instance_loads(i), c.pressure_loads()
         for k in ("cl", "cd"):
             assert bl[k] == pytest.approx(sl_[k], rel=1e-11, abs=1e-14)
+    # all instances' loads in one call are the per-instance integrals
+    every = batch.batch_loads()
+    assert len(every) == merged.instances
+    for i in range(merged.instances):
+        one = batch.instance_loads(i)
+        for k in ("cl", "cd", "drag", "lift"):
+            assert every[i][k] == one[k]
