@@ -92,6 +92,8 @@ def run(topo, spec, args, name, field, vcycle, pc):
         log(f"{name}: trained {train['train_steps']} steps on {train['pairs']} residuals "
             f"({train['bricks']} bricks), loss {train['loss_first']:.4f} -> {train['loss_last']:.4f}, "
             f"{train['train_ms']:.0f} ms")
+    tune = core.autotune(repetitions=2) if args.autotune else core.tuning()
+    log(f"{name}: launch shapes {tune}")
     for i in range(args.warmup):
         st = core.step()
         core.pressure_loads()
@@ -110,7 +112,7 @@ def run(topo, spec, args, name, field, vcycle, pc):
     n = topo.n_active
     res = {"core": core, "value": n * args.steps / seconds, "ms_per_step": 1e3 * seconds / args.steps,
            "mean_its": sum(its) / len(its), "fallback_steps": fallbacks, "converged": conv, "loads": loads,
-           "train": train, "learned_steps": used, "learned_rejected": bool(st["learned_rejected"])}
+           "train": train, "learned_steps": used, "learned_rejected": bool(st["learned_rejected"]), "tuning": tune}
     log(f"{name}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} ms/step, mean its {res['mean_its']:.1f}, "
         f"fallbacks {fallbacks}, network used in {used} steps (rejected: {res['learned_rejected']}), "
         f"CL {loads['cl']:.6f} CD {loads['cd']:.6f}")
@@ -138,7 +140,9 @@ def run_sweep(args):
 
     def advance(core, topo, loads_fn):
         core.initialize()
-        for _ in range(2 + args.warmup):
+        core.step()
+        tune = core.autotune(repetitions=2) if args.autotune else core.tuning()
+        for _ in range(1 + args.warmup):
             core.step()
             loads_fn()
         torch.cuda.synchronize()
@@ -149,13 +153,14 @@ def run_sweep(args):
             loads = loads_fn()
             its.append(st["iterations"])
         torch.cuda.synchronize()
-        return time.perf_counter() - t, its, loads
+        return time.perf_counter() - t, its, loads, tune
 
     batch = ExternalAeroCore(merged, opt)
     for i, p in enumerate(parts):
         batch.set_instance_reference(i, p.info.reference_area)
     log(f"batch solver: {batch.device_bytes() / 2**30:.1f} GiB of device memory")
-    tb, its_b, loads_b = advance(batch, merged, batch.batch_loads)
+    tb, its_b, loads_b, tune_b = advance(batch, merged, batch.batch_loads)
+    log(f"batch launch shapes {tune_b}")
     per_inst = [batch.instance_stats(i) for i in range(len(parts))]
     value = n_total * args.steps / tb
     log(f"batch: {value:.4g} cell-steps/s, {1e3 * tb / args.steps:.1f} ms/step, mean its {np.mean(its_b):.1f}")
@@ -166,7 +171,7 @@ def run_sweep(args):
     for i, p in enumerate(parts):
         core = ExternalAeroCore(p, opt)
         core.set_reference(p.info.reference_area)
-        t, its, loads = advance(core, p, core.pressure_loads)
+        t, its, loads, _ = advance(core, p, core.pressure_loads)
         core.close()
         seq_s += t
         seq_loads.append(loads)
@@ -185,6 +190,7 @@ def run_sweep(args):
         "config": {"workload": args.case, "instances": len(parts), "active_cells": n_total,
                    "alphas": list(spec["alphas"]), "rtol": 1e-8, "mean_pcg_iterations_batch": float(np.mean(its_b)),
                    "sequential_value": seq_value, "speedup_vs_sequential": value / seq_value,
+                   "launch_shapes": tune_b,
                    "max_rel_load_difference_batch_vs_sequential": dl,
                    "instances_detail": [{"alpha": a, "cells": p.n_active, "cl": l["cl"], "cd": l["cd"],
                                          "iterations": s["iterations"], "dt": s["dt"], "mean_its_alone": si}
@@ -211,6 +217,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes instead of autotuning them before the timed steps")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -298,6 +306,7 @@ def main():
                    "network_rejected_by_validation": learned["learned_rejected"], "cl": learned["loads"]["cl"],
                    "cd": learned["loads"]["cd"],
                    "training": {k: tr[k] for k in ("pairs", "train_steps", "loss_first", "loss_last", "train_ms")},
+                   "launch_shapes": learned["tuning"],
                    "plain_fp32_vcycle": summary(plain32), "plain_fp64_vcycle": summary(plain64),
                    "speedup_vs_plain_fp32_vcycle": learned["value"] / plain32["value"],
                    "speedup_vs_plain_fp64_vcycle": learned["value"] / plain64["value"],

@@ -269,6 +269,27 @@ pcfd_status pcfd_integrate_loads_instance(pcfd_solver* solver, int instance, dou
 pcfd_status pcfd_integrate_loads_batch(pcfd_solver* solver, double pref, pcfd_loads* loads, int count);
 pcfd_status pcfd_set_reference_instance(pcfd_solver* solver, int instance, double area, const double origin[3]);
 
+/* Launch-shape tuning.  No counterpart in the reference (its kernels have fixed launch
+ * shapes).  The tunables decide which workgroup computes which brick or coarse-level row, or
+ * whether the composite-level V-cycle runs as fused kernels; none changes the arithmetic, so
+ * every configuration gives bit-identical results.  pcfd_autotune times `repetitions` full
+ * steps per candidate from one snapshot of the state (greedy: XCD chunk, then coarse
+ * workgroup size, then fused or plain V-cycle), keeps the fastest configuration and restores
+ * the state and physical time, so the solution is unchanged.  Needs a converged
+ * initialization. */
+typedef struct pcfd_tuning {
+    int xcd_chunk_log2;     /* bricks go to the 8 XCDs in chunks of 2^k (-1: one range per XCD) */
+    int coarse_block;       /* workgroup size of the coarse-level kernels: 64, 128, 256 or 512 */
+    int fused_vcycle;       /* 1: fused composite-level V-cycle kernels, 0: the plain sequence */
+    int candidates;         /* configurations timed by the last pcfd_autotune */
+    double step_ms;         /* fastest step of the chosen configuration (last pcfd_autotune) */
+    double default_step_ms; /* fastest step of the configuration it started from */
+} pcfd_tuning;
+pcfd_status pcfd_autotune(pcfd_solver* solver, int repetitions, pcfd_tuning* result);
+pcfd_status pcfd_get_tuning(const pcfd_solver* solver, pcfd_tuning* tuning);
+/* Applies the first three fields of a tuning (e.g. one stored from an earlier autotune). */
+pcfd_status pcfd_set_tuning(pcfd_solver* solver, const pcfd_tuning* tuning);
+
 /* State I/O (upload_state/download_state amr_advection.cu:1301-1309, upload_pressure
  * amr_pressure.cu:2098-2108, download_pressure).  Arrays are FP64 host arrays of length
  * dofs (velocity: three arrays).  Face velocities: +x/+y/+z face of every dof, then one

@@ -80,6 +80,14 @@ class LearnedStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("xcd_chunk_log2", C.c_int), ("coarse_block", C.c_int), ("fused_vcycle", C.c_int),
+                ("candidates", C.c_int), ("step_ms", C.c_double), ("default_step_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class StepStats(C.Structure):
     _fields_ = [("dt", C.c_double), ("physical_time", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
                 ("relative_residual", C.c_double), ("max_abs_velocity", C.c_double), ("cfl_rate", C.c_double),
@@ -114,6 +122,7 @@ EXPORTED = [
     "pcfd_learned_get_weights", "pcfd_learned_set_weights", "pcfd_learned_enable", "pcfd_learned_forward",
     "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
     "pcfd_integrate_loads_instance", "pcfd_set_reference_instance", "pcfd_integrate_loads_batch",
+    "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning",
 ]
 
 _lib = None
@@ -178,6 +187,9 @@ def load():
         "pcfd_integrate_loads_instance": (C.c_int, [P, C.c_int, C.c_double, C.POINTER(Loads)]),
         "pcfd_set_reference_instance": (C.c_int, [P, C.c_int, C.c_double, dp]),
         "pcfd_integrate_loads_batch": (C.c_int, [P, C.c_double, C.POINTER(Loads), C.c_int]),
+        "pcfd_autotune": (C.c_int, [P, C.c_int, C.POINTER(Tuning)]),
+        "pcfd_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "pcfd_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

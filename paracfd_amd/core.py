@@ -144,6 +144,21 @@ class ExternalAeroCore:
         L.check(self._lib.pcfd_integrate_loads_instance(self.handle, i, pref, C.byref(out)))
         return out.as_dict()
 
+    # launch-shape tuning; no reference counterpart
+    def autotune(self, repetitions=2) -> dict:
+        out = L.Tuning()
+        L.check(self._lib.pcfd_autotune(self.handle, repetitions, C.byref(out)))
+        return out.as_dict()
+
+    def tuning(self) -> dict:
+        out = L.Tuning()
+        L.check(self._lib.pcfd_get_tuning(self.handle, C.byref(out)))
+        return out.as_dict()
+
+    def set_tuning(self, xcd_chunk_log2=-1, coarse_block=256, fused_vcycle=1):
+        t = L.Tuning(xcd_chunk_log2, coarse_block, fused_vcycle, 0, 0.0, 0.0)
+        L.check(self._lib.pcfd_set_tuning(self.handle, C.byref(t)))
+
     def batch_loads(self, pref=0.0) -> list:
         n = self.topology.instances
         out = (L.Loads * n)()

@@ -30,7 +30,27 @@ struct DevScalars {
     unsigned int ticket[16];
 };
 
+// Per-brick record, 64 bytes, read by a brick kernel with scalar loads before any cell
+// data: neighbours for the halo, the special-cell ranks, instance, cell size, boundary bits.
+struct alignas(64) BrickMeta {
+    int nbr[6];            // same-level neighbour brick per face (-x, +x, -y, +y, -z, +z) or -1
+    int special_off;       // rank of the brick's first special cell
+    int inst;              // instance of the batch
+    double h;              // cell size of the brick's level
+    uint16_t flags;        // domain-boundary bits (kXMIN .. kZMAX)
+    uint16_t level;
+    uint16_t wave_pre[8];  // special cells of the brick in the waves before wave w
+    int pad;
+};
+static_assert(sizeof(BrickMeta) == 64, "BrickMeta is one 64-byte record");
+
 struct MeshView {
+    const BrickMeta* meta;
+    const int* sp_other;     // per special incidence (special_inc order): the other cell's dof
+    const double* sp_coef;   // and the edge conductance
+    const int* sp4_other;    // the first four incidences of special cell s at 4 s + j (padding:
+    const double* sp4_coef;  //   the cell itself with conductance 0)
+    int xcd_chunk;           // log2 of the bricks per XCD chunk of the block -> brick order (-1: one range per XCD)
     int nbricks;
     int64_t ndof;
     int ninst;               // instances of the batch (1 for a single solver)
@@ -86,29 +106,106 @@ struct CsrView {
 constexpr int kTileN = 1000;
 __device__ __forceinline__ int tile_index(int i, int j, int k) { return (k + 1) * 100 + (j + 1) * 10 + (i + 1); }
 
-// Fill the tile with val(dof) for the brick's 512 cells (coalesced) and the 6 x 64 face
-// halo cells of the same-level neighbour bricks (zero where there is none).  Every lane of
-// the 512-lane workgroup must call it.
+// Values of one lane for a brick tile: its own cell and, for lanes q < 384, one face-halo
+// cell of a same-level neighbour brick (zero where there is none).  The loads are issued
+// here and consumed by stage_tile, so a kernel can issue its other per-cell loads (and test
+// its done flag) while they are in flight.  Wave w < 6 serves face w, so the neighbour index
+// is a scalar read of the brick record.
+template <class A>
+struct TileFetch {
+    A own, halo;
+    int hidx;
+};
+
 template <class A, class Fn>
-__device__ __forceinline__ void load_tile(const MeshView& M, int b, A* tile, Fn val) {
+__device__ __forceinline__ TileFetch<A> fetch_tile(const MeshView& M, int b, Fn val) {
     const int q = threadIdx.x;
-    tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)] = val(static_cast<int64_t>(b) * 512 + q);
+    TileFetch<A> f;
+    f.own = val(static_cast<int64_t>(b) * 512 + q);
+    f.halo = A(0);
+    f.hidx = 0;
     if (q < 384) {
-        const int face = q >> 6, t = q & 63, axis = face >> 1, up = face & 1;
+        const int face = __builtin_amdgcn_readfirstlane(q >> 6), t = q & 63, axis = face >> 1, up = face & 1;
         const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
         int c[3];
         c[a1] = t & 7;
         c[a2] = t >> 3;
         c[axis] = up ? 8 : -1;
-        const int nb = M.brick_nbr[6 * b + face];
-        A v = A(0);
+        f.hidx = tile_index(c[0], c[1], c[2]);
+        const int nb = M.meta[b].nbr[face];
         if (nb >= 0) {
-            int s[3] = {c[0], c[1], c[2]};
-            s[axis] = up ? 0 : 7;
-            v = val(static_cast<int64_t>(nb) * 512 + s[0] + 8 * s[1] + 64 * s[2]);
+            c[axis] = up ? 0 : 7;
+            f.halo = val(static_cast<int64_t>(nb) * 512 + c[0] + 8 * c[1] + 64 * c[2]);
         }
-        tile[tile_index(c[0], c[1], c[2])] = v;
     }
+    return f;
+}
+
+// Write the fetched values into the tile and synchronise.  Every lane must call it.
+template <class A>
+__device__ __forceinline__ void stage_tile(A* tile, const TileFetch<A>& f) {
+    const int q = threadIdx.x;
+    tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)] = f.own;
+    if (q < 384) tile[f.hidx] = f.halo;
+    __syncthreads();
+}
+
+// The first four compact-edge terms of a special cell, fetched before the tile barrier (the
+// other cells' values and the conductances), and its incidence range for the rest.
+template <class A>
+struct CompactFetch {
+    A c[4], v[4];
+    int k0, k1;
+};
+
+template <class A, class Fn>
+__device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int sp, int64_t a, Fn val) {
+    CompactFetch<A> f;
+    f.k0 = f.k1 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { f.c[j] = A(0); f.v[j] = A(0); }
+    if (sp >= 0) {
+        f.k0 = M.special_inc_off[sp];
+        f.k1 = M.special_inc_off[sp + 1];
+        int o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            o[j] = M.sp4_other[4 * static_cast<int64_t>(sp) + j];
+            f.c[j] = A(M.sp4_coef[4 * static_cast<int64_t>(sp) + j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f.v[j] = val(static_cast<int64_t>(o[j]));
+    }
+    return f;
+}
+
+// compact-edge part of (A y)_a from a fetch (in incidence order; incidences past the fourth
+// are gathered here)
+template <class A, class Fn>
+__device__ __forceinline__ void compact_apply_fetched(const MeshView& M, const CompactFetch<A>& f, Fn val, A ya, A& acc,
+                                                      A& dg) {
+    const int n = f.k1 - f.k0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < n) {
+            acc += f.c[j] * (ya - f.v[j]);
+            dg += f.c[j];
+        }
+    for (int k = f.k0 + 4; k < f.k1; ++k) {
+        const A c = A(M.sp_coef[k]);
+        acc += c * (ya - val(static_cast<int64_t>(M.sp_other[k])));
+        dg += c;
+    }
+}
+
+// Fill the tile with val(dof) for the brick's 512 cells and the 6 x 64 face halo cells
+// (zero where there is no neighbour).  Every lane of the 512-lane workgroup must call it.
+template <class A, class Fn>
+__device__ __forceinline__ void load_tile(const MeshView& M, int b, A* tile, Fn val) {
+    const TileFetch<A> f = fetch_tile<A>(M, b, val);
+    const int q = threadIdx.x;
+    tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)] = f.own;
+    if (q < 384) tile[f.hidx] = f.halo;
     __syncthreads();
 }
 
@@ -123,33 +220,55 @@ __device__ __forceinline__ A tile_apply(const MeshView& M, const A* tile, int b,
         if (m & (kPX << axis)) { acc += h * (ya - tile[c + step[axis]]); d += h; }
         if (m & (kMX << axis)) { acc += h * (ya - tile[c - step[axis]]); d += h; }
     }
-    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) { acc += A(2) * h * ya; d += A(2) * h; }
+    if (M.outlet && (q & 7) == 7 && (M.meta[b].flags & kXMAX)) { acc += A(2) * h * ya; d += A(2) * h; }
     dg = d;
     return acc;
 }
 
-// compact-edge part of (A y)_a for a special cell (global gathers through val)
+// compact-edge part of (A y)_a for a special cell: its incidences in groups of four, the
+// other cells and conductances of a group loaded together, then their values, summed in
+// incidence order
 template <class A, class Fn>
 __device__ __forceinline__ void compact_apply(const MeshView& M, int sp, Fn val, A ya, A& acc, A& dg) {
-    for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
-        const int raw = M.special_inc[k];
-        const int e = raw >= 0 ? raw : ~raw;
-        const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
-        const A c = A(M.ecoef[e]);
-        acc += c * (ya - val(o));
-        dg += c;
+    const int k0 = M.special_inc_off[sp], k1 = M.special_inc_off[sp + 1];
+    for (int k = k0; k < k1; k += 4) {
+        int o[4];
+        A c[4], v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool ok = k + j < k1;
+            o[j] = ok ? M.sp_other[k + j] : 0;
+            c[j] = ok ? A(M.sp_coef[k + j]) : A(0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = k + j < k1 ? val(static_cast<int64_t>(o[j])) : A(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1) {
+                acc += c[j] * (ya - v[j]);
+                dg += c[j];
+            }
     }
 }
 
 __device__ __forceinline__ int axis_coord(int q, int axis) { return (q >> (3 * axis)) & 7; }
 
 // XCD-aware brick order: workgroups are dispatched round-robin over the 8 XCDs of the
-// MI355X, so block b runs on XCD b % 8.  Giving every XCD one contiguous range of the
-// Morton-ordered bricks keeps a brick and its neighbours (halo reads) in the same L2.
-__device__ __forceinline__ int brick_of_block() {
-    const int G = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
-    const int x = b & 7, idx = b >> 3, per = G >> 3, rem = G & 7;
-    return x * per + (x < rem ? x : rem) + idx;
+// MI355X, so block g runs on XCD g % 8.  The bricks (Morton order) are dealt to the XCDs in
+// chunks of 2^lc consecutive bricks, so a brick and most of its neighbours (halo reads)
+// share one L2 while bricks of one region (the cut cells of a body) spread over all XCDs;
+// lc < 0 gives every XCD one contiguous range.  Bricks past the last whole round of 8
+// chunks are split into per-XCD ranges.
+__device__ __forceinline__ int brick_of_block(int lc = -1) {
+    const int G = static_cast<int>(gridDim.x), g = static_cast<int>(blockIdx.x);
+    const int full = lc >= 0 ? (G >> (lc + 3)) << (lc + 3) : 0;
+    if (g < full) {
+        const int x = g & 7, idx = g >> 3;
+        return ((((idx >> lc) << 3) + x) << lc) + (idx & ((1 << lc) - 1));
+    }
+    const int t = g - full, T = G - full;
+    const int x = t & 7, idx = t >> 3, per = T >> 3, rem = T & 7;
+    return full + x * per + (x < rem ? x : rem) + idx;
 }
 
 // neighbour dof across face (axis, sign) of local cell q in brick b; the caller has checked
@@ -166,17 +285,16 @@ __device__ __forceinline__ int64_t neighbor_dof(const MeshView& M, int64_t a, in
     return neighbor_dof(M, static_cast<int>(a >> 9), static_cast<int>(a & 511), axis, sign);
 }
 
-// Rank of a special cell inside its brick: ballot per wave, wave counts through LDS.
-// Block-collective: every lane of the 512-lane workgroup must call it.
+// Rank of a special cell inside its brick: the brick's first rank, the specials of the
+// earlier waves (brick record) and a ballot of this wave.  No barrier; all lanes of the wave
+// must call it.
 __device__ __forceinline__ int special_rank(const MeshView& M, int b, int q, bool sp) {
-    __shared__ int wave_count[8];
+    const int wave = __builtin_amdgcn_readfirstlane(q >> 6), lane = q & 63;
+    // wave_pre read as the 32-bit word holding it (a scalar load)
+    const uint32_t pair = reinterpret_cast<const uint32_t*>(M.meta[b].wave_pre)[wave >> 1];
+    const int base = M.meta[b].special_off + static_cast<int>((pair >> (16 * (wave & 1))) & 0xffffu);
     const unsigned long long bal = __ballot(sp);
-    const int wave = q >> 6, lane = q & 63;
-    if (lane == 0) wave_count[wave] = __popcll(bal);
-    __syncthreads();
     if (!sp) return -1;
-    int base = M.brick_special_off[b];
-    for (int w = 0; w < wave; ++w) base += wave_count[w];
     const unsigned long long below = lane == 0 ? 0ull : (bal & ((~0ull) >> (64 - lane)));
     return base + __popcll(below);
 }

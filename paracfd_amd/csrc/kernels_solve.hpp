@@ -38,7 +38,7 @@ __device__ __forceinline__ typename Arith<VT>::type round_vt(typename Arith<VT>:
     return to_f(from_f<VT>(v));
 }
 
-__device__ __forceinline__ double level_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
+__device__ __forceinline__ double level_h(const MeshView& M, int b) { return M.meta[b].h; }
 
 // composite-level diagonal (sum of face conductances + outlet + compact edges)
 template <class A>
@@ -48,12 +48,9 @@ __device__ __forceinline__ A diag_row0(const MeshView& M, int b, int q, uint8_t 
         if (m & (kPX << axis)) dg += h;
         if (m & (kMX << axis)) dg += h;
     }
-    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) dg += A(2) * h;
+    if (M.outlet && (q & 7) == 7 && (M.meta[b].flags & kXMAX)) dg += A(2) * h;
     if (sp >= 0)
-        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
-            const int raw = M.special_inc[k];
-            dg += A(M.ecoef[raw >= 0 ? raw : ~raw]);
-        }
+        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) dg += A(M.sp_coef[k]);
     return dg;
 }
 
@@ -67,11 +64,31 @@ __device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, int 
     return acc;
 }
 
-#define PCFD_BRICK_PROLOGUE                                        \
-    const int b = brick_of_block(), q = threadIdx.x;               \
+// Brick kernels issue every independent load first (brick record: one scalar load; cell,
+// halo and right-hand-side values: vector loads) and only then test the instance's done
+// flag and stage the tile, so one workgroup waits for about two memory latencies instead of
+// one per dependent step.
+// the same with the compact-edge terms fetched before the tile barrier
+template <class A, class Fn>
+__device__ __forceinline__ A row0f(const MeshView& M, const A* tile, const CompactFetch<A>& cf, Fn val, int b, int q,
+                                   uint8_t m, A ya, A& dg) {
+    A acc = tile_apply<A>(M, tile, b, q, m, A(level_h(M, b)), ya, dg);
+    if (m & kSpecial) compact_apply_fetched<A>(M, cf, val, ya, acc, dg);
+    return acc;
+}
+
+#define PCFD_BRICK_BEGIN                                           \
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;               \
     const int64_t a = static_cast<int64_t>(b) * 512 + q;           \
-    const uint8_t m = M.mask[a];                                   \
+    const uint8_t m = M.mask[a]
+
+#define PCFD_BRICK_PROLOGUE \
+    PCFD_BRICK_BEGIN;       \
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0)
+
+__device__ __forceinline__ bool brick_done(const MeshView& M, const DevScalars* S, int b) {
+    return S[M.meta[b].inst].done != 0;
+}
 
 // ---------------------------------------------------------------------------------------
 // PCG (amr_pressure.cu:1267-1353 solve_orthogonal)
@@ -80,15 +97,20 @@ __device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, int 
 __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* __restrict__ x,
                                                      const double* __restrict__ rhs, double* __restrict__ r,
                                                      double* partials, const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
-    __shared__ double tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
+    PCFD_BRICK_BEGIN;
     auto val = [&](int64_t c) -> double { return x[c]; };
-    load_tile<double>(M, b, tile, val);
+    const TileFetch<double> tf = fetch_tile<double>(M, b, val);
+    const double rb = rhs[a];
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, a, val);
+    if (done) return;
+    __shared__ double tile[kTileN];
+    stage_tile<double>(tile, tf);
     double ra = 0.0;
     if (m & kActive) {
         double dg;
-        ra = rhs[a] - row0<double>(M, tile, val, b, q, m, sp, x[a], dg);
+        ra = rb - row0f<double>(M, tile, cf, val, b, q, m, tf.own, dg);
     }
     r[a] = ra;
     store_partial(partials, b, block_sum<512>(ra * ra));
@@ -97,18 +119,22 @@ __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* _
 // A d and the d.Ad partial (used by the measurement hook)
 __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* __restrict__ d, double* __restrict__ Ad,
                                                       double* partials, const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
-    __shared__ double tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
+    PCFD_BRICK_BEGIN;
     auto val = [&](int64_t c) -> double { return d[c]; };
-    load_tile<double>(M, b, tile, val);
+    const TileFetch<double> tf = fetch_tile<double>(M, b, val);
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, a, val);
+    if (done) return;
+    __shared__ double tile[kTileN];
+    stage_tile<double>(tile, tf);
     double y = 0.0;
     if (m & kActive) {
         double dg;
-        y = row0<double>(M, tile, val, b, q, m, sp, d[a], dg);
+        y = row0f<double>(M, tile, cf, val, b, q, m, tf.own, dg);
     }
     Ad[a] = y;
-    store_partial(partials, b, block_sum<512>(y * d[a]));
+    store_partial(partials, b, block_sum<512>(y * tf.own));
 }
 
 // CG direction update fused into the operator apply: d_new = z + beta d_old (first
@@ -117,17 +143,22 @@ __global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* 
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
                                                       double* __restrict__ Ad, double* partials, const DevScalars* S,
                                                       int initial) {
-    if (S[M.inst[brick_of_block()]].done) return;
-    __shared__ double tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
-    const double beta = initial ? 0.0 : S[M.inst[b]].beta;
+    PCFD_BRICK_BEGIN;
+    const DevScalars& I = S[M.meta[b].inst];
+    const double beta = initial ? 0.0 : I.beta;
     auto dn = [&](int64_t c) -> double { return initial ? z[c] : z[c] + beta * dold[c]; };
-    load_tile<double>(M, b, tile, dn);
-    const double da = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
+    const TileFetch<double> tf = fetch_tile<double>(M, b, dn);
+    const bool done = I.done != 0;
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, a, dn);
+    if (done) return;
+    __shared__ double tile[kTileN];
+    stage_tile<double>(tile, tf);
+    const double da = tf.own;
     double y = 0.0;
     if (m & kActive) {
         double dg;
-        y = row0<double>(M, tile, dn, b, q, m, sp, da, dg);
+        y = row0f<double>(M, tile, cf, dn, b, q, m, da, dg);
     }
     dnew[a] = da;
     Ad[a] = y;
@@ -139,15 +170,15 @@ __global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* 
 __global__ __launch_bounds__(512) void k_cg_update(MeshView M, double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ rprev, const double* __restrict__ d,
                                                    const double* __restrict__ Ad, double* partials, const DevScalars* S) {
-    const int b = brick_of_block();
-    const DevScalars& I = S[M.inst[b]];
+    const int b = brick_of_block(M.xcd_chunk);
+    const int64_t i = static_cast<int64_t>(b) * 512 + threadIdx.x;
+    const double xi = x[i], di = d[i], ri = r[i], adi = Ad[i];
+    const DevScalars& I = S[M.meta[b].inst];
     if (I.done) return;
     const double alpha = I.alpha;
-    const int64_t i = static_cast<int64_t>(b) * 512 + threadIdx.x;
-    x[i] += alpha * d[i];
-    const double ri = r[i];
+    x[i] = xi + alpha * di;
     if (rprev) rprev[i] = ri;
-    const double rn = ri - alpha * Ad[i];
+    const double rn = ri - alpha * adi;
     r[i] = rn;
     store_partial(partials, b, block_sum<512>(rn * rn));
 }
@@ -265,23 +296,29 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
                                                  const double* __restrict__ rhs, double* __restrict__ zout,
                                                  const double* __restrict__ rprev, double omega, double* partials,
                                                  const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
-    __shared__ A tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
+    PCFD_BRICK_BEGIN;
     const bool from_zero = mode == 0 || xin == nullptr;
     auto val = [&](int64_t c) -> A { return from_zero ? A(0) : A(to_f(xin[c])); };
-    load_tile<A>(M, b, tile, val);
+    const TileFetch<A> tf = fetch_tile<A>(M, b, val);
+    const double rb = rhs[a];
+    const double rp = (mode == 2 && rprev) ? rprev[a] : 0.0;
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, val);
+    if (done) return;
+    __shared__ A tile[kTileN];
+    stage_tile<A>(tile, tf);
     A v = A(0);
     if (m & kActive) {
         const A h = A(level_h(M, b));
         if (from_zero) {
-            v = A(omega) * A(rhs[a]) / diag_row0<A>(M, b, q, m, sp, h);
+            v = A(omega) * A(rb) / diag_row0<A>(M, b, q, m, sp, h);
         } else {
             A dg;
-            const A xa = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
-            const A Ax = row0<A>(M, tile, val, b, q, m, sp, xa, dg);
-            v = xa + A(omega) * (A(rhs[a]) - Ax) / dg;
+            const A xa = tf.own;
+            const A Ax = row0f<A>(M, tile, cf, val, b, q, m, xa, dg);
+            v = xa + A(omega) * (A(rb) - Ax) / dg;
         }
     }
     if (mode != 2) {
@@ -290,8 +327,8 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
     }
     const double z = double(v);
     zout[a] = z;
-    const double s1 = block_sum<512>(rhs[a] * z);
-    const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
+    const double s1 = block_sum<512>(rb * z);
+    const double s2 = rprev ? block_sum<512>(rp * z) : 0.0;
     if (threadIdx.x == 0) {
         partials[b] = s1;
         partials[M.nbricks + b] = s2;
@@ -302,16 +339,21 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
 template <class VT>
 __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
                                                    VT* __restrict__ res, const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
-    __shared__ A tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
+    PCFD_BRICK_BEGIN;
     auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
-    load_tile<A>(M, b, tile, val);
+    const TileFetch<A> tf = fetch_tile<A>(M, b, val);
+    const double rb = rhs[a];
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, val);
+    if (done) return;
+    __shared__ A tile[kTileN];
+    stage_tile<A>(tile, tf);
     A v = A(0);
     if (m & kActive) {
         A dg;
-        v = A(rhs[a]) - row0<A>(M, tile, val, b, q, m, sp, tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)], dg);
+        v = A(rb) - row0f<A>(M, tile, cf, val, b, q, m, tf.own, dg);
     }
     res[a] = from_f<VT>(v);
 }
@@ -321,19 +363,24 @@ template <class VT>
 __global__ __launch_bounds__(512) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
                                                  const typename Arith<VT>::type* __restrict__ diag, VT* __restrict__ xout,
                                                  double omega, const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
-    __shared__ A tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
+    PCFD_BRICK_BEGIN;
     const A w = A(omega);
     auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rhs[c]) / diag[c]); };
-    load_tile<A>(M, b, tile, x1);
+    const TileFetch<A> tf = fetch_tile<A>(M, b, x1);
+    const double rb = rhs[a];
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, x1);
+    if (done) return;
+    __shared__ A tile[kTileN];
+    stage_tile<A>(tile, tf);
     A v = A(0);
     if (m & kActive) {
         A dg;
-        const A xa = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
-        const A Ax = row0<A>(M, tile, x1, b, q, m, sp, xa, dg);
-        v = xa + w * (A(rhs[a]) - Ax) / dg;
+        const A xa = tf.own;
+        const A Ax = row0f<A>(M, tile, cf, x1, b, q, m, xa, dg);
+        v = xa + w * (A(rb) - Ax) / dg;
     }
     xout[a] = from_f<VT>(v);
 }
@@ -344,21 +391,27 @@ template <class VT>
 __global__ __launch_bounds__(512) void k_vc_restrict0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
                                                       const int* __restrict__ parent0, VT* __restrict__ coarse,
                                                       const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
+    PCFD_BRICK_BEGIN;
+    auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
+    const TileFetch<A> tf = fetch_tile<A>(M, b, val);
+    const double rb = rhs[a];
+    const int pa = parent0[a];
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, val);
+    if (done) return;
     __shared__ A tile[kTileN];
     __shared__ A sres[512];
     __shared__ int spar[512];
-    PCFD_BRICK_PROLOGUE;
-    auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
-    load_tile<A>(M, b, tile, val);
+    stage_tile<A>(tile, tf);
     A v = A(0);
     if (m & kActive) {
         A dg;
-        v = A(rhs[a]) - row0<A>(M, tile, val, b, q, m, sp, tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)], dg);
+        v = A(rb) - row0f<A>(M, tile, cf, val, b, q, m, tf.own, dg);
     }
     sres[q] = round_vt<VT>(v);
-    spar[q] = parent0[a];
+    spar[q] = pa;
     __syncthreads();
     if ((q & 0x49) != 0) return;  // leaders have even i, j, k
     const int kids[8] = {q, q + 1, q + 8, q + 9, q + 64, q + 65, q + 72, q + 73};
@@ -391,22 +444,28 @@ __global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, c
                                                          const double* __restrict__ rhs, VT* __restrict__ xout,
                                                          double* __restrict__ zout, const double* __restrict__ rprev,
                                                          double omega, double* partials, const DevScalars* S) {
-    if (S[M.inst[brick_of_block()]].done) return;
     using A = typename Arith<VT>::type;
-    __shared__ A tile[kTileN];
-    PCFD_BRICK_PROLOGUE;
+    PCFD_BRICK_BEGIN;
     auto y = [&](int64_t c) -> A {
         const int pc = parent0[c];
         const A xc = A(to_f(x[c]));
         return pc >= 0 ? round_vt<VT>(xc + A(to_f(e1[pc]))) : xc;
     };
-    load_tile<A>(M, b, tile, y);
+    const TileFetch<A> tf = fetch_tile<A>(M, b, y);
+    const double rb = rhs[a];
+    const double rp = (last && rprev) ? rprev[a] : 0.0;
+    const bool done = brick_done(M, S, b);
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, y);
+    if (done) return;
+    __shared__ A tile[kTileN];
+    stage_tile<A>(tile, tf);
     A v = A(0);
     if (m & kActive) {
         A dg;
-        const A ya = tile[tile_index(q & 7, (q >> 3) & 7, q >> 6)];
-        const A Ay = row0<A>(M, tile, y, b, q, m, sp, ya, dg);
-        v = ya + A(omega) * (A(rhs[a]) - Ay) / dg;
+        const A ya = tf.own;
+        const A Ay = row0f<A>(M, tile, cf, y, b, q, m, ya, dg);
+        v = ya + A(omega) * (A(rb) - Ay) / dg;
     }
     if (!last) {
         xout[a] = from_f<VT>(v);
@@ -414,8 +473,8 @@ __global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, c
     }
     const double z = double(v);
     zout[a] = z;
-    const double s1 = block_sum<512>(rhs[a] * z);
-    const double s2 = rprev ? block_sum<512>(rprev[a] * z) : 0.0;
+    const double s1 = block_sum<512>(rb * z);
+    const double s2 = rprev ? block_sum<512>(rp * z) : 0.0;
     if (threadIdx.x == 0) {
         partials[b] = s1;
         partials[M.nbricks + b] = s2;
@@ -573,5 +632,6 @@ __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off
 }
 
 #undef PCFD_BRICK_PROLOGUE
+#undef PCFD_BRICK_BEGIN
 
 }  // namespace pcfd

@@ -40,7 +40,7 @@ __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[
 // ---------------------------------------------------------------------------------------
 template <class R>
 __global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> F, double* partials) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> 
 template <class R>
 __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* __restrict__ ox, R* __restrict__ oy,
                                                 R* __restrict__ oz, const DevScalars* S) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -154,7 +154,7 @@ template <class R>
 __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
                                                  const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
                                                  R* __restrict__ oz, const DevScalars* S, int step) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -215,7 +215,7 @@ __device__ __forceinline__ double strain_rate(const double* g) {
 template <class R>
 __global__ __launch_bounds__(512) void k_viscosity(MeshView M, FieldsView<R> F, SmagView G, double nu_mol, double cs,
                                                    R* __restrict__ nu) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(512) void k_viscosity(MeshView M, FieldsView<R> F, 
 // per-cell explicit diffusion rate sum_faces conductance (nu_a + nu_b)/2 / V, block maxima
 template <class R>
 __global__ __launch_bounds__(512) void k_diff_rate(MeshView M, const R* __restrict__ nu, double* partials) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __rest
                                                      const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
                                                      R* __restrict__ oz, const R* __restrict__ nu, const DevScalars* S,
                                                      int step) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __rest
 // ---------------------------------------------------------------------------------------
 template <class R>
 __global__ __launch_bounds__(512) void k_reconstruct(MeshView M, FieldsView<R> F) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     if (!(m & kActive)) return;
@@ -388,7 +388,7 @@ __device__ __forceinline__ double cell_divergence(const MeshView& M, const Field
 template <class R>
 __global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double* __restrict__ rhs, double* partials,
                                              const DevScalars* S) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double
 // max |integrated divergence| / V after the correction (FP64 check)
 template <class R>
 __global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, double* partials) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, d
 template <class R>
 __global__ __launch_bounds__(512) void k_correct(MeshView M, FieldsView<R> F, const double* __restrict__ p,
                                                  const DevScalars* S) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     if (!(m & kActive)) return;
@@ -458,7 +458,7 @@ __device__ __forceinline__ double wall_pressure(const MeshView& M, const double*
 template <class R>
 __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
                                                  const DevScalars* S, double dt_given, int only_converged) {
-    const int b = brick_of_block(), q = threadIdx.x;
+    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const DevScalars& I = S[M.inst[b]];

@@ -77,6 +77,9 @@ public:
     virtual void instance_stats(int instance, pcfd_step_stats* st) const = 0;
     virtual void instance_loads(int instance, double pref, pcfd_loads* out) = 0;
     virtual void batch_loads(double pref, pcfd_loads* out) = 0;  // every instance, one sync
+    virtual void autotune(int repetitions, pcfd_tuning* out) = 0;
+    virtual void get_tuning(pcfd_tuning* out) const = 0;
+    virtual void set_tuning(const pcfd_tuning& t) = 0;
     virtual int instances() const = 0;
     bool initialized = false;
     double time = 0.0;
@@ -118,6 +121,9 @@ public:
     void instance_stats(int instance, pcfd_step_stats* st) const override { *st = inst_stats_.at(instance); }
     void instance_loads(int instance, double pref, pcfd_loads* out) override { loads_instance(instance, pref, out); }
     void batch_loads(double pref, pcfd_loads* out) override;
+    void autotune(int repetitions, pcfd_tuning* out) override;
+    void get_tuning(pcfd_tuning* out) const override { *out = tuning_; }
+    void set_tuning(const pcfd_tuning& t) override;
     int instances() const override { return ninst_; }
 
 private:
@@ -316,6 +322,17 @@ private:
     int chunk_ = 2;
     bool use_graph_ = true;
     bool fused_ = true;
+    int coarse_block_ = 256;   // workgroup size of the coarse-level (SELL) kernels
+    pcfd_tuning tuning_{};
+    bool tuning_run_ = false;  // inside autotune: steps leave no learned-validation record
+    void snapshot_state();
+    void restore_state();
+    void drop_graph() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
+    }
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     // Smagorinsky model (smagorinsky_cs > 0)
@@ -423,6 +440,9 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     chunk_ = std::max(2, (opt_.check_interval + 1) / 2 * 2);
     if (const char* g = std::getenv("PCFD_NO_GRAPH")) use_graph_ = g[0] == '0';
     if (const char* f = std::getenv("PCFD_UNFUSED")) fused_ = f[0] == '0';
+    tuning_.xcd_chunk_log2 = -1;
+    tuning_.coarse_block = coarse_block_;
+    tuning_.fused_vcycle = fused_ ? 1 : 0;
     if (opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED) {
         rprev_ = alloc<double>(n_);
         HIP_CHECK(hipMemset(rprev_, 0, n_ * sizeof(double)));
@@ -449,6 +469,51 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
             for (int b = T.inst_brick_off[i]; b < T.inst_brick_off[i + 1]; ++b) inst[b] = i;
         M_.inst = upload(inst);
         M_.ninst = ninst_;
+        // per-brick records and the special incidences resolved to (other cell, conductance)
+        std::vector<BrickMeta> meta(nb_);
+        for (int b = 0; b < nb_; ++b) {
+            BrickMeta& r = meta[b];
+            std::memset(&r, 0, sizeof(r));
+            for (int f = 0; f < 6; ++f) r.nbr[f] = T.brick_nbr[6 * static_cast<size_t>(b) + f];
+            r.special_off = T.brick_special_off[b];
+            r.inst = inst[b];
+            r.level = static_cast<uint16_t>(T.brick_level[b]);
+            r.h = T.level_h.at(r.level);
+            r.flags = T.brick_flags[b];
+            int run = 0;
+            for (int w = 0; w < 8; ++w) {
+                r.wave_pre[w] = static_cast<uint16_t>(run);
+                for (int q = 64 * w; q < 64 * w + 64; ++q)
+                    if (T.mask[512 * static_cast<size_t>(b) + q] & kSpecial) ++run;
+            }
+            if (T.brick_special_off[b] + run != T.brick_special_off[b + 1])
+                throw std::runtime_error("topology: special cells of a brick do not match its special range");
+        }
+        M_.meta = upload(meta);
+        std::vector<int> other(T.special_inc.size());
+        std::vector<double> coef(T.special_inc.size());
+        for (size_t k = 0; k < T.special_inc.size(); ++k) {
+            const int raw = T.special_inc[k];
+            const int e = raw >= 0 ? raw : ~raw;
+            other[k] = raw >= 0 ? T.edge_b[e] : T.edge_a[e];
+            coef[k] = T.edge_coef[e];
+        }
+        const size_t nsp = T.special_inc_off.empty() ? 0 : T.special_inc_off.size() - 1;
+        std::vector<int> other4(4 * nsp);
+        std::vector<double> coef4(4 * nsp, 0.0);
+        for (size_t sp = 0; sp < nsp; ++sp) {
+            const int k0 = T.special_inc_off[sp], k1 = T.special_inc_off[sp + 1];
+            for (int j = 0; j < 4; ++j) {
+                const bool in = k0 + j < k1;
+                other4[4 * sp + j] = in ? other[k0 + j] : T.special_dof[sp];
+                coef4[4 * sp + j] = in ? coef[k0 + j] : 0.0;
+            }
+        }
+        M_.sp4_other = upload(other4);
+        M_.sp4_coef = upload(coef4);
+        M_.xcd_chunk = -1;
+        M_.sp_other = upload(other);
+        M_.sp_coef = upload(coef);
     }
     seg_ = upload(T.inst_brick_off);
     inst_maxdiff_dev_ = upload(T.inst_max_diffusion);
@@ -697,28 +762,29 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     VT* cur = static_cast<VT*>(L.X);
     VT* oth = static_cast<VT*>(L.T);
     const double w = opt_.smoother_weight;
-    const unsigned g = blocks_for(L.n, 256);
+    const int cb = coarse_block_;
+    const unsigned g = blocks_for(L.n, cb);
     if (m == static_cast<int>(lv_.size()) - 1) {
         if (!dense_blocks<VT, CT>(m, B, cur)) {  // no factorisation (pure Neumann): 64 sweeps with weight 2/3
-            LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 0, cur, cur, B, 2.0 / 3.0);
-            for (int s = 1; s < 64; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, 2.0 / 3.0); std::swap(cur, oth); }
+            LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 0, cur, cur, B, 2.0 / 3.0);
+            for (int s = 1; s < 64; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, 2.0 / 3.0); std::swap(cur, oth); }
         }
         return cur;
     }
     if (opt_.pre_smooth == 2) {
-        LAUNCH((k_ell_pre2<VT, CT>), g, 256, C, B, cur, w);
+        LAUNCH((k_ell_pre2<VT, CT>), g, cb, C, B, cur, w);
     } else {
-        LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 0, cur, cur, B, w);
-        for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, w); std::swap(cur, oth); }
+        LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 0, cur, cur, B, w);
+        for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, w); std::swap(cur, oth); }
     }
     Level& N = lv_[m + 1];
     const CsrView& NC = std::is_same<CT, double>::value ? N.csr64 : N.csr32;
-    LAUNCH((k_ell_residual<VT, CT>), g, 256, C, cur, B, oth);
-    LAUNCH((k_restrict_range<VT>), blocks_for(N.n, 256), 256, NC, oth, static_cast<VT*>(N.B));
+    LAUNCH((k_ell_residual<VT, CT>), g, cb, C, cur, B, oth);
+    LAUNCH((k_restrict_range<VT>), blocks_for(N.n, cb), cb, NC, oth, static_cast<VT*>(N.B));
     VT* xc = coarse_cycle_t<VT, CT>(m + 1);
-    LAUNCH((k_ell_post_prolong<VT, CT>), g, 256, C, cur, xc, B, oth, w);
+    LAUNCH((k_ell_post_prolong<VT, CT>), g, cb, C, cur, xc, B, oth, w);
     std::swap(cur, oth);
-    for (int s = 1; s < opt_.post_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, 256, C, 1, cur, oth, B, w); std::swap(cur, oth); }
+    for (int s = 1; s < opt_.post_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, w); std::swap(cur, oth); }
     // instances of a batch whose hierarchy ends at this level: their cells take the exact
     // coarse solve (what their own V-cycle does at its last level)
     dense_blocks<VT, CT>(m, B, cur);
@@ -1012,7 +1078,7 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     HIP_CHECK(hipEventElapsedTime(&p_ms, ev_[0], ev_[1]));
     HIP_CHECK(hipEventElapsedTime(&s_ms, ev_[2], ev_[1]));
     const bool used = inst_stats_[0].learned_used != 0;
-    validate_learned(used, p_ms);
+    if (!tuning_run_) validate_learned(used, p_ms);
     for (int i = 0; i < ninst_; ++i) {
         pcfd_step_stats& s = inst_stats_[i];
         s.learned_rejected = nn_rejected_ ? 1 : 0;
@@ -1251,6 +1317,123 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
     HIP_CHECK(hipMemcpy(S_, host_s_, (ninst_ + 1) * sizeof(DevScalars), hipMemcpyHostToDevice));
 }
 
+// copy of the evolving state (velocity, faces, pressure) on the solver's stream
+template <class R>
+void SolverImpl<R>::snapshot_state() {
+    if (!snap_p_) {
+        for (int c = 0; c < 3; ++c) { snap_v_[c] = alloc<R>(n_); snap_f_[c] = alloc<R>(n_); }
+        snap_fe_ = alloc<R>(nedges_);
+        snap_p_ = alloc<double>(n_);
+    }
+    for (int c = 0; c < 3; ++c) {
+        HIP_CHECK(hipMemcpyAsync(snap_v_[c], v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(snap_f_[c], f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+    }
+    if (nedges_) HIP_CHECK(hipMemcpyAsync(snap_fe_, fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(snap_p_, p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+}
+
+template <class R>
+void SolverImpl<R>::restore_state() {
+    for (int c = 0; c < 3; ++c) {
+        HIP_CHECK(hipMemcpyAsync(v_[c], snap_v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(f_[c], snap_f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+    }
+    if (nedges_) HIP_CHECK(hipMemcpyAsync(fe_, snap_fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(p_, snap_p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+}
+
+// ------------------------------------------------------------------------------------------
+// launch-shape autotuner
+// ------------------------------------------------------------------------------------------
+// The tunables change which workgroup computes which brick or row (XCD chunking of the
+// brick order, coarse-level workgroup size) or fuse kernels whose arithmetic is the same,
+// never the arithmetic: every configuration gives bit-identical results.
+template <class R>
+void SolverImpl<R>::set_tuning(const pcfd_tuning& t) {
+    if (t.xcd_chunk_log2 < -1 || t.xcd_chunk_log2 > 16) throw std::invalid_argument("xcd_chunk_log2 must lie in [-1, 16]");
+    if (t.coarse_block != 64 && t.coarse_block != 128 && t.coarse_block != 256 && t.coarse_block != 512)
+        throw std::invalid_argument("coarse_block must be 64, 128, 256 or 512");
+    M_.xcd_chunk = t.xcd_chunk_log2;
+    coarse_block_ = t.coarse_block;
+    fused_ = t.fused_vcycle != 0;
+    tuning_.xcd_chunk_log2 = t.xcd_chunk_log2;
+    tuning_.coarse_block = t.coarse_block;
+    tuning_.fused_vcycle = fused_ ? 1 : 0;
+    drop_graph();  // kernel arguments are baked into the captured chunk
+}
+
+// Greedy search, one parameter at a time (XCD chunk, then coarse workgroup size, then
+// fused or plain composite-level V-cycle): each candidate runs `repetitions` full steps
+// from the same snapshot of the state and keeps its fastest step time.  The state, the
+// physical time and the scalars are restored afterwards, so tuning leaves the solution
+// where it was.
+template <class R>
+void SolverImpl<R>::autotune(int repetitions, pcfd_tuning* out) {
+    if (!initialized) throw std::logic_error("autotune after a converged initialization");
+    if (repetitions < 1) throw std::invalid_argument("repetitions must be positive");
+    const std::vector<double> time0 = inst_time_;
+    const double t0 = time;
+    snapshot_state();
+    tuning_run_ = true;
+    int timed = 0;
+    auto measure = [&](const pcfd_tuning& t) {
+        set_tuning(t);
+        double best = 1e300;
+        for (int r = 0; r < repetitions; ++r) {
+            restore_state();
+            pcfd_step_stats st{};
+            step(&st);
+            best = std::min(best, st.step_ms);
+        }
+        ++timed;
+        return best;
+    };
+    pcfd_tuning cur = tuning_;
+    double best = 0.0;
+    try {
+        const double def = measure(cur);
+        best = def;
+        for (int lc : {-1, 0, 2, 4, 6}) {
+            if (lc == cur.xcd_chunk_log2) continue;
+            pcfd_tuning t = cur;
+            t.xcd_chunk_log2 = lc;
+            const double ms = measure(t);
+            if (ms < best) { best = ms; cur = t; }
+        }
+        for (int cb : {64, 128, 256, 512}) {
+            if (cb == cur.coarse_block) continue;
+            pcfd_tuning t = cur;
+            t.coarse_block = cb;
+            const double ms = measure(t);
+            if (ms < best) { best = ms; cur = t; }
+        }
+        if (lv_.size() > 1 && opt_.pre_smooth == 2) {
+            pcfd_tuning t = cur;
+            t.fused_vcycle = cur.fused_vcycle ? 0 : 1;
+            const double ms = measure(t);
+            if (ms < best) { best = ms; cur = t; }
+        }
+        cur.default_step_ms = def;
+    } catch (...) {
+        tuning_run_ = false;
+        restore_state();
+        inst_time_ = time0;
+        time = t0;
+        throw;
+    }
+    tuning_run_ = false;
+    set_tuning(cur);
+    restore_state();
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    inst_time_ = time0;
+    time = t0;
+    tuning_.candidates = timed;
+    tuning_.step_ms = best;
+    tuning_.default_step_ms = cur.default_step_ms;
+    if (out) *out = tuning_;
+}
+
 // ------------------------------------------------------------------------------------------
 // learned preconditioner: data, training, hooks
 // ------------------------------------------------------------------------------------------
@@ -1260,25 +1443,10 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
     if (!initialized) throw std::logic_error("collect before a converged initialization");
     if (max_pairs < 1) throw std::invalid_argument("max_pairs must be positive");
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
-    if (!snap_p_) {
-        for (int c = 0; c < 3; ++c) { snap_v_[c] = alloc<R>(n_); snap_f_[c] = alloc<R>(n_); }
-        snap_fe_ = alloc<R>(nedges_);
-        snap_p_ = alloc<double>(n_);
-    }
-    // snapshot the state: the collection advances a copy and solves its projection
-    for (int c = 0; c < 3; ++c) {
-        HIP_CHECK(hipMemcpyAsync(snap_v_[c], v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
-        HIP_CHECK(hipMemcpyAsync(snap_f_[c], f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
-    }
-    if (nedges_) HIP_CHECK(hipMemcpyAsync(snap_fe_, fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
-    HIP_CHECK(hipMemcpyAsync(snap_p_, p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    // the collection advances a copy of the state and solves its projection
+    snapshot_state();
     auto restore = [&] {
-        for (int c = 0; c < 3; ++c) {
-            HIP_CHECK(hipMemcpyAsync(v_[c], snap_v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
-            HIP_CHECK(hipMemcpyAsync(f_[c], snap_f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
-        }
-        if (nedges_) HIP_CHECK(hipMemcpyAsync(fe_, snap_fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
-        HIP_CHECK(hipMemcpyAsync(p_, snap_p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        restore_state();
         set_nn_off(!nn_on_);
     };
     int recorded = 0;
@@ -1571,6 +1739,21 @@ pcfd_status pcfd_instance_stats(const pcfd_solver* s, int instance, pcfd_step_st
     s->impl->instance_stats(instance, stats);
     return PCFD_OK;
 }
+pcfd_status pcfd_autotune(pcfd_solver* s, int repetitions, pcfd_tuning* out) {
+    NEED(s, "pcfd_autotune: null solver");
+    return guarded("pcfd_autotune", [&] { s->impl->autotune(repetitions, out); });
+}
+
+pcfd_status pcfd_get_tuning(const pcfd_solver* s, pcfd_tuning* out) {
+    NEED(s && out, "pcfd_get_tuning: null argument");
+    return guarded("pcfd_get_tuning", [&] { s->impl->get_tuning(out); });
+}
+
+pcfd_status pcfd_set_tuning(pcfd_solver* s, const pcfd_tuning* t) {
+    NEED(s && t, "pcfd_set_tuning: null argument");
+    return guarded("pcfd_set_tuning", [&] { s->impl->set_tuning(*t); });
+}
+
 pcfd_status pcfd_integrate_loads_batch(pcfd_solver* s, double pref, pcfd_loads* loads, int count) {
     NEED(s && loads, "pcfd_integrate_loads_batch: null argument");
     NEED(count == s->impl->instances(), "pcfd_integrate_loads_batch: count must equal the number of instances");
