@@ -549,13 +549,15 @@ __global__ void k_residual_csr(CsrView C, const VT* __restrict__ x, const VT* __
     res[r] = from_f<VT>(A(to_f(rhs[r])) - Ax);
 }
 
+// Coarse-level rows.  Each operation is a per-row device function shared by the one-launch-
+// per-operation kernels below and the one-workgroup tail kernel, so both give bit-identical
+// results.
+
 // ELL sweep: mode 0 from zero, 1 damped Jacobi
 template <class VT, class CT>
-__global__ void k_ell_smooth(CsrView C, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
-                             const VT* __restrict__ rhs, double omega) {
+__device__ __forceinline__ void ell_smooth_row(const CsrView& C, int r, int mode, const VT* __restrict__ xin,
+                                               VT* __restrict__ xout, const VT* __restrict__ rhs, double omega) {
     using A = typename Arith<VT>::type;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C.n) return;
     const A D = A(static_cast<const CT*>(C.diag)[r]);
     const A br = A(to_f(rhs[r]));
     if (mode == 0) { xout[r] = from_f<VT>(A(omega) * br / D); return; }
@@ -566,10 +568,9 @@ __global__ void k_ell_smooth(CsrView C, int mode, const VT* __restrict__ xin, VT
 
 // two sweeps from zero in one pass (x1 = w b / D at the neighbours on the fly)
 template <class VT, class CT>
-__global__ void k_ell_pre2(CsrView C, const VT* __restrict__ rhs, VT* __restrict__ xout, double omega) {
+__device__ __forceinline__ void ell_pre2_row(const CsrView& C, int r, const VT* __restrict__ rhs, VT* __restrict__ xout,
+                                             double omega) {
     using A = typename Arith<VT>::type;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C.n) return;
     const CT* diag = static_cast<const CT*>(C.diag);
     const A w = A(omega);
     auto x1 = [&](int c) -> A { return round_vt<VT>(w * A(to_f(rhs[c])) / A(diag[c])); };
@@ -578,12 +579,11 @@ __global__ void k_ell_pre2(CsrView C, const VT* __restrict__ rhs, VT* __restrict
     xout[r] = from_f<VT>(xr + w * (A(to_f(rhs[r])) - Ax) / A(diag[r]));
 }
 
-// residual b - A x, one thread per row
+// residual b - A x
 template <class VT, class CT>
-__global__ void k_ell_residual(CsrView C, const VT* __restrict__ x, const VT* __restrict__ rhs, VT* __restrict__ res) {
+__device__ __forceinline__ void ell_residual_row(const CsrView& C, int r, const VT* __restrict__ x,
+                                                 const VT* __restrict__ rhs, VT* __restrict__ res) {
     using A = typename Arith<VT>::type;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C.n) return;
     const A xr = A(to_f(x[r]));
     const A Ax = ell_row<A, CT>(C, r, [&](int j) -> A { return A(to_f(x[j])); }, xr);
     res[r] = from_f<VT>(A(to_f(rhs[r])) - Ax);
@@ -592,10 +592,9 @@ __global__ void k_ell_residual(CsrView C, const VT* __restrict__ x, const VT* __
 // restriction from a coarse level (m >= 1): the children of a coarse cell are a contiguous
 // ascending range of the finer level's rows (Morton order), summed in that order
 template <class VT>
-__global__ void k_restrict_range(CsrView Cc, const VT* __restrict__ fine, VT* __restrict__ coarse) {
+__device__ __forceinline__ void restrict_range_row(const CsrView& Cc, int c, const VT* __restrict__ fine,
+                                                   VT* __restrict__ coarse) {
     using A = typename Arith<VT>::type;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Cc.n) return;
     const int k0 = Cc.child_off[c], k1 = Cc.child_off[c + 1], f0 = Cc.child[k0];
     A s = A(0);
     for (int k = k0; k < k1; ++k) s += A(to_f(fine[f0 + (k - k0)]));
@@ -604,31 +603,174 @@ __global__ void k_restrict_range(CsrView Cc, const VT* __restrict__ fine, VT* __
 
 // prolongation from level m+1 fused with one sweep on level m
 template <class VT, class CT>
-__global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT* __restrict__ e, const VT* __restrict__ rhs,
-                                   VT* __restrict__ xout, double omega) {
+__device__ __forceinline__ void ell_post_prolong_row(const CsrView& C, int r, const VT* __restrict__ x,
+                                                     const VT* __restrict__ e, const VT* __restrict__ rhs,
+                                                     VT* __restrict__ xout, double omega) {
     using A = typename Arith<VT>::type;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= C.n) return;
     auto y = [&](int c) -> A { return round_vt<VT>(A(to_f(x[c])) + A(to_f(e[C.parent[c]]))); };
     const A yr = y(r);
     const A Ay = ell_row<A, CT>(C, r, y, yr);
     xout[r] = from_f<VT>(yr + A(omega) * (A(to_f(rhs[r])) - Ay) / A(static_cast<const CT*>(C.diag)[r]));
 }
 
-// coarsest level of each instance: x = A^-1 b with its dense inverse; block (row, k) solves
-// row `row` of dense block k (cells [off[k], off[k] + n[k]) of the level, inverse at ioff[k])
+template <class VT, class CT>
+__global__ void k_ell_smooth(CsrView C, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
+                             const VT* __restrict__ rhs, double omega) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < C.n) ell_smooth_row<VT, CT>(C, r, mode, xin, xout, rhs, omega);
+}
+
+template <class VT, class CT>
+__global__ void k_ell_pre2(CsrView C, const VT* __restrict__ rhs, VT* __restrict__ xout, double omega) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < C.n) ell_pre2_row<VT, CT>(C, r, rhs, xout, omega);
+}
+
+template <class VT, class CT>
+__global__ void k_ell_residual(CsrView C, const VT* __restrict__ x, const VT* __restrict__ rhs, VT* __restrict__ res) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < C.n) ell_residual_row<VT, CT>(C, r, x, rhs, res);
+}
+
+template <class VT>
+__global__ void k_restrict_range(CsrView Cc, const VT* __restrict__ fine, VT* __restrict__ coarse) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < Cc.n) restrict_range_row<VT>(Cc, c, fine, coarse);
+}
+
+template <class VT, class CT>
+__global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT* __restrict__ e, const VT* __restrict__ rhs,
+                                   VT* __restrict__ xout, double omega) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < C.n) ell_post_prolong_row<VT, CT>(C, r, x, e, rhs, xout, omega);
+}
+
+// one row of a dense block solve: x[off + r] = sum_c inv[ioff + c n + r] b[off + c], summed
+// in column order by one lane (the inverse of the SPD block is symmetric, so reading its
+// column r lets neighbouring lanes read neighbouring entries)
+template <class VT, class CT>
+__device__ __forceinline__ void dense_row(int off, int n, int64_t ioff, int r, const CT* __restrict__ inv,
+                                          const VT* __restrict__ rhs, VT* __restrict__ x) {
+    const CT* col = inv + ioff + r;
+    const VT* b = rhs + off;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < n; ++c) acc += double(col[static_cast<int64_t>(c) * n]) * double(to_f(b[c]));
+    x[off + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
+}
+
+// coarsest level of each instance: x = A^-1 b with its dense inverse; block (x, k) solves
+// rows [256 x, 256 x + 256) of dense block k (cells [off[k], off[k] + n[k]) of the level,
+// inverse at ioff[k])
 template <class VT, class CT>
 __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off, const int* __restrict__ nrow,
                                                      const int64_t* __restrict__ ioff, const CT* __restrict__ inv,
                                                      const VT* __restrict__ rhs, VT* __restrict__ x, const DevScalars* S) {
-    const int k = blockIdx.y, r = blockIdx.x, n = nrow[k];
-    if (r >= n) return;
-    const CT* row = inv + ioff[k] + static_cast<int64_t>(r) * n;
-    const VT* b = rhs + off[k];
-    double acc = 0.0;
-    for (int c = threadIdx.x; c < n; c += 256) acc += double(row[c]) * double(to_f(b[c]));
-    acc = block_sum<256>(acc);
-    if (threadIdx.x == 0) x[off[k] + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
+    const int k = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+    if (r < nrow[k]) dense_row<VT, CT>(off[k], nrow[k], ioff[k], r, inv, rhs, x);
+}
+
+// ---------------------------------------------------------------------------------------
+// Coarse tail: the V-cycle of every level from the first small one down, in one workgroup of
+// 1024 lanes with barriers between the operations instead of one launch per operation (the
+// small levels cost a launch each, not their work).  Same operations in the same order as
+// the launches of the host's coarse cycle, with the same per-row functions.
+// ---------------------------------------------------------------------------------------
+constexpr int kTailMax = 8;
+struct TailLevel {
+    CsrView C;                 // operator, parent map, child ranges
+    void* B; void* X; void* T;
+    int dcount;                // dense blocks of the instances whose hierarchy ends here
+    const int* doff;
+    const int* dn;
+    const int64_t* dioff;
+};
+struct TailParams {
+    TailLevel lv[kTailMax];
+    int nlev;                  // levels in the tail; the last is the coarsest
+    const void* inv;           // dense inverses (CT)
+    double omega;
+    int pre, post;
+    const DevScalars* agg;
+};
+
+template <class VT, class CT>
+__device__ __forceinline__ void tail_dense(const TailLevel& L, const CT* inv, const VT* B, VT* X) {
+    for (int k = 0; k < L.dcount; ++k) {
+        const int n = L.dn[k], off = L.doff[k];
+        const int64_t io = L.dioff[k];
+        for (int r = threadIdx.x; r < n; r += 1024) dense_row<VT, CT>(off, n, io, r, inv, B, X);
+    }
+    __syncthreads();
+}
+
+template <class VT, class CT>
+__global__ __launch_bounds__(1024) void k_coarse_tail(TailParams P) {
+    if (P.agg->done) return;
+    const int tid = threadIdx.x;
+    const double w = P.omega;
+    const CT* inv = static_cast<const CT*>(P.inv);
+    unsigned sel = 0;  // bit l: level l's current iterate is in T (else X)
+    auto cur_of = [&](int l) -> VT* { return static_cast<VT*>(((sel >> l) & 1u) ? P.lv[l].T : P.lv[l].X); };
+    auto oth_of = [&](int l) -> VT* { return static_cast<VT*>(((sel >> l) & 1u) ? P.lv[l].X : P.lv[l].T); };
+    const int last = P.nlev - 1;
+    for (int l = 0; l < last; ++l) {
+        const CsrView& C = P.lv[l].C;
+        const VT* B = static_cast<const VT*>(P.lv[l].B);
+        if (P.pre == 2) {
+    #pragma unroll 4
+        for (int r = tid; r < C.n; r += 1024) ell_pre2_row<VT, CT>(C, r, B, cur_of(l), w);
+            __syncthreads();
+        } else {
+    #pragma unroll 4
+        for (int r = tid; r < C.n; r += 1024) ell_smooth_row<VT, CT>(C, r, 0, cur_of(l), cur_of(l), B, w);
+            __syncthreads();
+            for (int s = 1; s < P.pre; ++s) {
+        #pragma unroll 4
+        for (int r = tid; r < C.n; r += 1024) ell_smooth_row<VT, CT>(C, r, 1, cur_of(l), oth_of(l), B, w);
+                __syncthreads();
+                sel ^= 1u << l;
+            }
+        }
+#pragma unroll 4
+        for (int r = tid; r < C.n; r += 1024) ell_residual_row<VT, CT>(C, r, cur_of(l), B, oth_of(l));
+        __syncthreads();
+        const CsrView& NC = P.lv[l + 1].C;
+        for (int c = tid; c < NC.n; c += 1024) restrict_range_row<VT>(NC, c, oth_of(l), static_cast<VT*>(P.lv[l + 1].B));
+        __syncthreads();
+    }
+    {
+        const TailLevel& L = P.lv[last];
+        const VT* B = static_cast<const VT*>(L.B);
+        if (L.dcount > 0) {
+            tail_dense<VT, CT>(L, inv, B, cur_of(last));
+        } else {  // no factorisation (pure Neumann): 64 sweeps with weight 2/3
+            for (int r = tid; r < L.C.n; r += 1024) ell_smooth_row<VT, CT>(L.C, r, 0, cur_of(last), cur_of(last), B, 2.0 / 3.0);
+            __syncthreads();
+            for (int s = 1; s < 64; ++s) {
+                for (int r = tid; r < L.C.n; r += 1024)
+                    ell_smooth_row<VT, CT>(L.C, r, 1, cur_of(last), oth_of(last), B, 2.0 / 3.0);
+                __syncthreads();
+                sel ^= 1u << last;
+            }
+        }
+    }
+    for (int l = last - 1; l >= 0; --l) {
+        const CsrView& C = P.lv[l].C;
+        const VT* B = static_cast<const VT*>(P.lv[l].B);
+        const VT* xc = cur_of(l + 1);
+#pragma unroll 4
+        for (int r = tid; r < C.n; r += 1024) ell_post_prolong_row<VT, CT>(C, r, cur_of(l), xc, B, oth_of(l), w);
+        __syncthreads();
+        sel ^= 1u << l;
+        for (int s = 1; s < P.post; ++s) {
+    #pragma unroll 4
+        for (int r = tid; r < C.n; r += 1024) ell_smooth_row<VT, CT>(C, r, 1, cur_of(l), oth_of(l), B, w);
+            __syncthreads();
+            sel ^= 1u << l;
+        }
+        if (P.lv[l].dcount > 0) tail_dense<VT, CT>(P.lv[l], inv, B, cur_of(l));
+    }
 }
 
 #undef PCFD_BRICK_PROLOGUE

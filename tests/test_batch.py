@@ -137,3 +137,52 @@ def test_batch_matches_individual_runs(parts, merged):
         one = batch.instance_loads(i)
         for k in ("cl", "cd", "drag", "lift"):
             assert every[i][k] == one[k]
+
+
+@pytest.fixture(scope="module")
+def mixed():
+    """A sphere box and a wing on the same cell size (h = 1/16) whose multigrid hierarchies end
+    at different levels, so the batch has a dense coarse block above its coarsest level."""
+    g, d = cases.sphere_box(cells_x=64, levels=2)
+    s = L.Topology(g, d)
+    g, d = cases.naca_wing(digits=4412, alpha_deg=6.0, cells_x=144, levels=2, bands=(0.3,))
+    w = L.Topology(g, d)
+    return [s, w], L.Topology.merge([s, w])
+
+
+def test_mixed_depth_merge(mixed):
+    parts, merged = mixed
+    assert parts[0].mg_levels != parts[1].mg_levels
+    assert merged.mg_levels == max(p.mg_levels for p in parts)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tail", [0, 4096, 1 << 22])
+def test_mixed_depth_batch_matches_individual_runs(mixed, tail):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paracfd_amd import ExternalAeroCore, default_options
+    parts, merged = mixed
+    o = default_options(rho=1.0, nu=1e-4, speed=1.0, cfl=0.5, rtol=1e-8, max_iterations=300,
+                        field_precision=L.FP64, vcycle_precision=L.FP64)
+    print(f"[batch] mixed depths, coarse tail rows {tail}", flush=True)
+    batch = ExternalAeroCore(merged, o)
+    batch.set_tuning(xcd_chunk_log2=4, coarse_block=128, fused_vcycle=1, coarse_tail_rows=tail)
+    singles = [ExternalAeroCore(p, o) for p in parts]
+    batch.initialize()
+    for c in singles:
+        c.initialize()
+    for step in range(3):
+        batch.step()
+        ss = [c.step() for c in singles]
+        for i, s in enumerate(ss):
+            assert batch.instance_stats(i)["iterations"] == s["iterations"], (step, i)
+    bx, _, _ = batch.download_cell_momentum_state()
+    bp = batch.download_pressure()
+    for i, c in enumerate(singles):
+        inst = merged.instance(i)
+        sl = slice(inst["dof_begin"], inst["dof_end"])
+        x, _, _ = c.download_cell_momentum_state()
+        for a, b in ((bx[sl], x), (bp[sl], c.download_pressure())):
+            assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))

@@ -31,16 +31,18 @@ def state(core):
 def test_tunings_give_identical_results(topo, field, vcycle):
     from paracfd_amd import ExternalAeroCore
     print(f"[tuning] identical results field={field} vcycle={vcycle}", flush=True)
-    cores = [ExternalAeroCore(topo, options(field, vcycle)) for _ in range(3)]
-    cores[1].set_tuning(xcd_chunk_log2=2, coarse_block=64, fused_vcycle=0)
-    cores[2].set_tuning(xcd_chunk_log2=0, coarse_block=512, fused_vcycle=1)
+    cores = [ExternalAeroCore(topo, options(field, vcycle)) for _ in range(4)]
+    cores[1].set_tuning(xcd_chunk_log2=2, coarse_block=64, fused_vcycle=0, coarse_tail_rows=2048)
+    cores[2].set_tuning(xcd_chunk_log2=0, coarse_block=512, fused_vcycle=1, coarse_tail_rows=1 << 22)
+    cores[3].set_tuning(xcd_chunk_log2=4, coarse_block=128, fused_vcycle=1, coarse_tail_rows=200)
     assert cores[1].tuning()["coarse_block"] == 64
-    its = [[], [], []]
+    assert cores[2].tuning()["coarse_tail_rows"] == 1 << 22
+    its = [[], [], [], []]
     for i, c in enumerate(cores):
         c.initialize()
         for _ in range(3):
             its[i].append(c.step()["iterations"])
-    assert its[0] == its[1] == its[2]
+    assert its[0] == its[1] == its[2] == its[3]
     ref = state(cores[0])
     for c in cores[1:]:
         assert np.array_equal(state(c), ref)
