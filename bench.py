@@ -64,6 +64,19 @@ def cpu_baseline(topo, options, budget_s):
                       f"({elapsed:.1f} s timed)"}
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("workload") != workload:
+        return None
+    return rec["hbm_bytes_per_launch"]
+
+
 def rel_l2(a, b):
     import numpy as np
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -202,6 +215,29 @@ def run_sweep(args):
     print(json.dumps(out), flush=True)
 
 
+def kernel_probe(args):
+    """A series of launches of one hot kernel (no other launch of it in the process) and plain
+    device copies of known size, for rocprofv3 --pmc runs: the counters of the copies give the
+    scale factor that turns FETCH_SIZE / WRITE_SIZE of the kernel into HBM bytes."""
+    import torch
+    from paracfd_amd import ExternalAeroCore, _lib as L, cases, default_options
+    g, d, spec = cases.build_case(args.case)
+    topo = L.Topology(g, d)
+    core = ExternalAeroCore(topo, default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"], cfl=spec["cfl"],
+                                                  field_precision=L.FP32, vcycle_precision=L.FP32))
+    core.set_tuning(xcd_chunk_log2=args.probe_chunk)
+    ms, by = core.time_kernel(args.kernel_probe, reps=args.probe_reps)
+    nbytes = 256 * 2**20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_id": args.kernel_probe, "workload": args.case, "active_cells": topo.n_active,
+                      "us_per_launch": ms * 1e3, "algorithmic_bytes": by, "launches": args.probe_reps + 1,
+                      "copy_bytes": nbytes, "copies": 3, "xcd_chunk_log2": args.probe_chunk}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +255,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes instead of autotuning them before the timed steps")
+    ap.add_argument("--kernel-probe", type=int, default=None,
+                    help="only launch hot kernel K (0 apply, 1 V-cycle sweep, 2 transport) and calibration copies")
+    ap.add_argument("--probe-reps", type=int, default=20)
+    ap.add_argument("--probe-chunk", type=int, default=4)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -233,6 +273,9 @@ def main():
 
     from paracfd_amd import _lib as L, cases
 
+    if args.kernel_probe is not None:
+        kernel_probe(args)
+        return
     if args.case in cases.SWEEPS:
         run_sweep(args)
         return
@@ -273,7 +316,9 @@ def main():
     ms, by, _ = timings[dom]
     achieved = by / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM, "traffic": None}
+                "frac": achieved / PEAK_HBM, "traffic": measured_traffic(names[dom], args.case),
+                "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel on this "
+                                  "workload, scaled by the counters of a device copy of known size"}
     tr = learned["train"]
     ms_nn = timings[4][0]
     flops_nn = tr["bricks"] * 512 * 4 * 27 * 8 * 8 * 2.0   # useful network flops per launch

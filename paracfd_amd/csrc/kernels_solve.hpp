@@ -515,7 +515,23 @@ __device__ __forceinline__ A ell_row(const CsrView& C, int r, Fn y, A yr) {
     const CT* coef = static_cast<const CT*>(C.sl_coef);
     const int s0 = C.sl_off[r >> 6], s1 = C.sl_off[(r >> 6) + 1];
     A Ax = A(static_cast<const CT*>(C.diag)[r]) * yr;
-    for (int s = s0 + (r & 63); s < s1; s += 64) Ax -= A(coef[s]) * y(C.sl_col[s]);
+    int s = s0 + (r & 63);
+    // four slots per round: their columns, coefficients and values are loaded before any is
+    // used, then subtracted in slot order
+    for (; s + 192 < s1; s += 256) {
+        int c[4];
+        A k[4], v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] = C.sl_col[s + 64 * j];
+            k[j] = A(coef[s + 64 * j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = y(c[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Ax -= k[j] * v[j];
+    }
+    for (; s < s1; s += 64) Ax -= A(coef[s]) * y(C.sl_col[s]);
     return Ax;
 }
 
@@ -645,29 +661,32 @@ __global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT
     if (r < C.n) ell_post_prolong_row<VT, CT>(C, r, x, e, rhs, xout, omega);
 }
 
-// one row of a dense block solve: x[off + r] = sum_c inv[ioff + c n + r] b[off + c], summed
-// in column order by one lane (the inverse of the SPD block is symmetric, so reading its
-// column r lets neighbouring lanes read neighbouring entries)
+// one row of a dense block solve, x[off + r] = sum_c inv[ioff + r n + c] b[off + c], by one
+// wave: lane l sums the columns l, l + 64, ... in order, then the wave's butterfly.  The
+// whole wave takes the same r (rows r >= n write nothing).
 template <class VT, class CT>
 __device__ __forceinline__ void dense_row(int off, int n, int64_t ioff, int r, const CT* __restrict__ inv,
                                           const VT* __restrict__ rhs, VT* __restrict__ x) {
-    const CT* col = inv + ioff + r;
-    const VT* b = rhs + off;
+    const int lane = threadIdx.x & 63;
     double acc = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < n; ++c) acc += double(col[static_cast<int64_t>(c) * n]) * double(to_f(b[c]));
-    x[off + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
+    if (r < n) {
+        const CT* row = inv + ioff + static_cast<int64_t>(r) * n;
+        const VT* b = rhs + off;
+        for (int c = lane; c < n; c += 64) acc += double(row[c]) * double(to_f(b[c]));
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (r < n && lane == 0) x[off + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
 }
 
-// coarsest level of each instance: x = A^-1 b with its dense inverse; block (x, k) solves
-// rows [256 x, 256 x + 256) of dense block k (cells [off[k], off[k] + n[k]) of the level,
+// coarsest level of each instance: x = A^-1 b with its dense inverse; wave w of block
+// (x, k) solves row 4 x + w of dense block k (cells [off[k], off[k] + n[k]) of the level,
 // inverse at ioff[k])
 template <class VT, class CT>
 __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off, const int* __restrict__ nrow,
                                                      const int64_t* __restrict__ ioff, const CT* __restrict__ inv,
                                                      const VT* __restrict__ rhs, VT* __restrict__ x, const DevScalars* S) {
-    const int k = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
-    if (r < nrow[k]) dense_row<VT, CT>(off[k], nrow[k], ioff[k], r, inv, rhs, x);
+    const int k = blockIdx.y, r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    dense_row<VT, CT>(off[k], nrow[k], ioff[k], r, inv, rhs, x);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -694,12 +713,14 @@ struct TailParams {
     const DevScalars* agg;
 };
 
+// dense blocks of a tail level: one row per wave, 16 rows at a time
 template <class VT, class CT>
 __device__ __forceinline__ void tail_dense(const TailLevel& L, const CT* inv, const VT* B, VT* X) {
+    const int wave = threadIdx.x >> 6;
     for (int k = 0; k < L.dcount; ++k) {
         const int n = L.dn[k], off = L.doff[k];
         const int64_t io = L.dioff[k];
-        for (int r = threadIdx.x; r < n; r += 1024) dense_row<VT, CT>(off, n, io, r, inv, B, X);
+        for (int r = wave; r < n; r += 16) dense_row<VT, CT>(off, n, io, r, inv, B, X);
     }
     __syncthreads();
 }

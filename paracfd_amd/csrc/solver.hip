@@ -201,7 +201,7 @@ private:
         if (D.count == 0) return false;
         const CT* inv = nullptr;
         if constexpr (std::is_same<CT, double>::value) inv = dense_inv64_; else inv = dense_inv32_;
-        hipLaunchKernelGGL((k_dense_solve<VT, CT>), dim3(blocks_for(D.max_n, 256), D.count), dim3(256), 0, stream_, D.off, D.n, D.ioff,
+        hipLaunchKernelGGL((k_dense_solve<VT, CT>), dim3(blocks_for(D.max_n, 4), D.count), dim3(256), 0, stream_, D.off, D.n, D.ioff,
                            inv, B, X, agg_);
         HIP_CHECK(hipGetLastError());
         return true;
@@ -1458,7 +1458,7 @@ void SolverImpl<R>::autotune(int repetitions, pcfd_tuning* out) {
             if (ms < best) { best = ms; cur = t; }
         }
         int last_start = -2;
-        for (int rows : {0, 2048, 8192, 32768, 131072}) {
+        for (int rows : {0, 2048, 8192}) {
             if (rows == cur.coarse_tail_rows) continue;
             tail_rows_ = rows;
             const int start = tail_start();

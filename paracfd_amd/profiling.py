@@ -1,0 +1,59 @@
+"""Summaries of rocprofv3 result databases (SQLite, `<name>_results.db`).
+
+    python -m paracfd_amd.profiling trace DB            per-kernel calls, mean and total duration
+    python -m paracfd_amd.profiling counters DB [TEXT]  per-kernel mean of each collected counter
+                                                        (kernels whose name contains TEXT)
+"""
+from __future__ import annotations
+
+import collections
+import sqlite3
+import sys
+
+
+def _short(name: str, width: int = 72) -> str:
+    return name if len(name) <= width else name[:width - 3] + "..."
+
+
+def kernel_table(db: str) -> list[tuple[str, int, float, float]]:
+    """(kernel, calls, mean us, total ms) sorted by total time."""
+    con = sqlite3.connect(db)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    name_col = "kernel_name" if "kernel_name" in cols else "name"
+    agg = collections.defaultdict(list)
+    for name, start, end in con.execute(f"select {name_col}, start, end from kernels"):
+        agg[name].append((end - start) / 1000.0)
+    rows = [(k, len(v), sum(v) / len(v), sum(v) / 1000.0) for k, v in agg.items()]
+    return sorted(rows, key=lambda r: -r[3])
+
+
+def counter_table(db: str, text: str = "") -> dict[str, dict[str, tuple[float, int]]]:
+    """kernel -> counter -> (mean per dispatch, dispatches)."""
+    con = sqlite3.connect(db)
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for kernel, counter, value in con.execute("select kernel_name, counter_name, value from counters_collection"):
+        if text in kernel:
+            agg[kernel][counter].append(value)
+    return {k: {c: (sum(v) / len(v), len(v)) for c, v in cs.items()} for k, cs in agg.items()}
+
+
+def main(argv: list[str]) -> int:
+    if len(argv) < 2 or argv[0] not in ("trace", "counters"):
+        print(__doc__, file=sys.stderr)
+        return 2
+    if argv[0] == "trace":
+        rows = kernel_table(argv[1])
+        total = sum(r[3] for r in rows) or 1.0
+        print(f"{'kernel':72s} {'calls':>6s} {'avg_us':>9s} {'total_ms':>9s} {'pct':>6s}")
+        for name, calls, avg, tot in rows:
+            print(f"{_short(name):72s} {calls:6d} {avg:9.1f} {tot:9.2f} {100 * tot / total:6.1f}")
+    else:
+        for kernel, cs in counter_table(argv[1], argv[2] if len(argv) > 2 else "").items():
+            print(_short(kernel, 110))
+            for counter, (mean, n) in sorted(cs.items()):
+                print(f"    {counter:32s} mean {mean:.6g} (dispatches {n})")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
