@@ -55,12 +55,13 @@ typedef struct pcfd_geometry_desc {
     double center[3];         /* sphere centre; wing: leading edge of the root section (pitch pivot) */
     double radius;            /* sphere radius */
     int naca_digits;          /* 4-digit code, e.g. 2412 (wing) */
-    double chord;             /* wing root chord [m] */
+    double chord;             /* wing root chord [m] (paraglider: 0 = aspect ratio 5 from the span) */
     double span;              /* wing span along y [m], centred on center[1] */
     double alpha_deg;         /* pitch about +y through center; positive = nose up */
-    double sweep_deg;         /* paraglider: leading-edge sweep */
-    double anhedral_deg;      /* paraglider: total arc anhedral angle */
-    double reflex;            /* paraglider: reflex camber magnitude (fraction of chord) */
+    double sweep_deg;         /* paraglider: leading-edge sweep added to the elliptic quarter-chord line */
+    double anhedral_deg;      /* paraglider: turn of each half of the circular span arc (tip droop angle) */
+    double reflex;            /* paraglider: upturn of the camber line at the trailing edge (chords),
+                                 quadratic over the last 30 % of the chord */
 } pcfd_geometry_desc;
 
 typedef struct pcfd_grid_desc {

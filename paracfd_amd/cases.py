@@ -6,6 +6,8 @@ is chosen so the active-cell counts land near the stated ~4 M (2-level) and ~20 
 """
 from __future__ import annotations
 
+import math
+
 from . import _lib as L
 
 
@@ -49,6 +51,38 @@ def naca_wing(digits=2412, alpha_deg=4.0, cells_x=128, levels=2, bands=(0.3, 0.0
     return g, d
 
 
+def paraglider(span=8.0, alpha_deg=6.0, anhedral_deg=25.0, sweep_deg=8.0, digits=2412, reflex=0.02, chord=0.0,
+               cells_x=160, levels=3, bands=(1.0, 0.25), subsamples=4, min_volume_fraction=0.1):
+    """Synthetic paraglider-like wing: elliptic planform of aspect ratio 5 (chord 0) on a circular
+    span arc drooping anhedral_deg at each tip, leading-edge sweep, NACA section with a reflexed
+    (upturned) trailing edge, pitched alpha about +y through the root leading edge at the origin.
+    Domain: 3 root chords upstream, 6 downstream, 1.5 spans across, from 2.5 chords below the
+    tips to 2.5 chords above the root."""
+    g = L.GeometryDesc()
+    g.kind = L.GEOM_PARAGLIDER
+    g.center[:] = (0.0, 0.0, 0.0)
+    g.naca_digits = digits
+    g.chord = chord
+    g.span = span
+    g.alpha_deg = alpha_deg
+    g.sweep_deg = sweep_deg
+    g.anhedral_deg = anhedral_deg
+    g.reflex = reflex
+    c0 = chord if chord > 0 else 4.0 * span / (5.0 * math.pi)
+    half = math.radians(anhedral_deg)
+    drop = (0.5 * span / half) * (1 - math.cos(half)) if half > 0 else 0.0
+    d = L.GridDesc()
+    d.lo[:] = (-3.0 * c0, -0.75 * span, -drop - 2.5 * c0)
+    d.hi[:] = (6.0 * c0, 0.75 * span, 2.5 * c0)
+    d.base_cells_x = cells_x
+    d.levels = levels
+    d.refine_band[:] = tuple(b * c0 for b in bands) + (0.0,) * (4 - len(bands))
+    d.min_volume_fraction = min_volume_fraction
+    d.subsamples = subsamples
+    d.outlet_xmax = 1
+    return g, d
+
+
 def uniform_box(n=16, h=0.05, outlet=False):
     """Empty n^3 box (legacy MAC golden-vector grid when n=16, h=0.05, no outlet)."""
     g = L.GeometryDesc()
@@ -71,6 +105,8 @@ CASES = {
                           rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=500),
     "naca2412_3lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=288, levels=3, bands=(1.0, 0.25)),
                           rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=200),
+    "paraglider_3lvl": dict(kind="paraglider", args=dict(alpha_deg=6.0, cells_x=160, levels=3, bands=(1.0, 0.25)),
+                            rho=1.0, nu=1.5e-5, speed=10.0, cfl=0.5, steps=200),
 }
 
 
@@ -103,6 +139,8 @@ def build_case(name, **overrides):
     args.update(overrides)
     if spec["kind"] == "sphere":
         g, d = sphere_box(**args)
+    elif spec["kind"] == "paraglider":
+        g, d = paraglider(**args)
     else:
         g, d = naca_wing(**args)
     return g, d, spec

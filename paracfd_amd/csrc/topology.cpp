@@ -238,6 +238,151 @@ private:
     double le_[3], chord_, half_span_, ca_, sa_;
 };
 
+// Synthetic paraglider-like wing.  The span coordinate s in [-b/2, b/2] runs along a circular
+// arc in the y-z plane that droops to the tips: each half of the arc turns through
+// anhedral_deg (the tip's span tangent droops anhedral_deg below horizontal), so its radius is
+// R = (b/2) / anhedral.  The chord is elliptic,
+// c(s) = c0 sqrt(1 - (2s/b)^2), with root chord c0 (aspect ratio 5, c0 = 4 b / (5 pi), when the
+// given chord is 0), truncated at |2s/b| = 0.98; the leading edge follows the ellipse's
+// quarter-chord line plus
+// |s| tan(sweep).  Sections are the NACA 4-digit section of naca_digits with the trailing 30 %
+// of the camber line turned up by reflex * ((x - 0.7) / 0.3)^2 (reflex: upturn at the
+// trailing edge in chords), set normal to the arc, then the whole wing is pitched by alpha
+// about +y through center (the root leading edge).  The point-in-solid test inverts the arc
+// exactly: radial distance and angle about the arc centre give (s, section normal offset).
+class Paraglider final : public Geometry {
+public:
+    Paraglider(const pcfd_geometry_desc& g) {
+        b_ = g.span;
+        c0_ = g.chord > 0 ? g.chord : 4.0 * b_ / (5.0 * kPi);
+        const double area = 0.25 * kPi * c0_ * b_;  // planform area along the arc
+        const double half = g.anhedral_deg * kPi / 180.0;
+        flat_ = !(half > 1e-6);
+        R_ = flat_ ? 0.0 : 0.5 * b_ / half;
+        tsw_ = std::tan(g.sweep_deg * kPi / 180.0);
+        for (int a = 0; a < 3; ++a) le_[a] = g.center[a];
+        const double al = g.alpha_deg * kPi / 180.0;
+        ca_ = std::cos(al); sa_ = std::sin(al);
+        // section: NACA mean line + reflex upturn, thickness normal to the camber line
+        const int digits = g.naca_digits;
+        const double m = (digits / 1000) / 100.0, p = ((digits / 100) % 10) / 10.0, t = (digits % 100) / 100.0;
+        const double rf = g.reflex;
+        const int n = 400;
+        std::vector<double> xu(n + 1), zu(n + 1), xl(n + 1), zl(n + 1);
+        for (int i = 0; i <= n; ++i) {
+            const double beta = kPi * i / n, x = 0.5 * (1 - std::cos(beta));
+            const double yt = 5 * t * (0.2969 * std::sqrt(x) - 0.1260 * x - 0.3516 * x * x + 0.2843 * x * x * x -
+                                       0.1036 * x * x * x * x);
+            double yc = 0, slope = 0;
+            if (m > 0) {
+                if (x < p) { yc = m / (p * p) * (2 * p * x - x * x); slope = 2 * m / (p * p) * (p - x); }
+                else { yc = m / ((1 - p) * (1 - p)) * ((1 - 2 * p) + 2 * p * x - x * x); slope = 2 * m / ((1 - p) * (1 - p)) * (p - x); }
+            }
+            if (x > 0.7) {
+                const double u = (x - 0.7) / 0.3;
+                yc += rf * u * u;
+                slope += rf * 2 * u / 0.3;
+            }
+            const double th = std::atan(slope);
+            xu[i] = x - yt * std::sin(th); zu[i] = yc + yt * std::cos(th);
+            xl[i] = x + yt * std::sin(th); zl[i] = yc - yt * std::cos(th);
+        }
+        std::vector<double> px, pz;
+        for (int i = n; i >= 0; --i) { px.push_back(xu[i]); pz.push_back(zu[i]); }
+        for (int i = 1; i < n; ++i) { px.push_back(xl[i]); pz.push_back(zl[i]); }
+        poly_.set(px, pz);
+        zmin_ = *std::min_element(pz.begin(), pz.end());
+        zmax_ = *std::max_element(pz.begin(), pz.end());
+        ref_area = area;
+        // bounding box from a dense sample of the surface
+        std::vector<double> pts;
+        surface(0.02 * c0_, pts);
+        aabb_lo[0] = aabb_lo[1] = aabb_lo[2] = 1e300;
+        aabb_hi[0] = aabb_hi[1] = aabb_hi[2] = -1e300;
+        for (size_t i = 0; i < pts.size(); i += 3)
+            for (int a = 0; a < 3; ++a) {
+                aabb_lo[a] = std::min(aabb_lo[a], pts[i + a] - 0.02 * c0_);
+                aabb_hi[a] = std::max(aabb_hi[a], pts[i + a] + 0.02 * c0_);
+            }
+    }
+    double chord(double s) const {
+        const double e = 2.0 * s / b_;
+        return c0_ * std::sqrt(std::max(0.0, 1.0 - e * e));
+    }
+    double x_le(double s) const { return 0.25 * (c0_ - chord(s)) + std::abs(s) * tsw_; }
+    // wing frame (before pitch) -> world
+    void to_world(double xw, double yw, double zw, double w[3]) const {
+        w[0] = le_[0] + xw * ca_ + zw * sa_;
+        w[1] = le_[1] + yw;
+        w[2] = le_[2] - xw * sa_ + zw * ca_;
+    }
+    // section point (xi along the chord, zeta normal, in chords) at span station s
+    void section_point(double s, double xi, double zeta, double w[3]) const {
+        const double c = chord(s);
+        double y, z;
+        if (flat_) { y = s; z = c * zeta; }
+        else {
+            const double th = s / R_, rad = R_ + c * zeta;
+            y = rad * std::sin(th);
+            z = rad * std::cos(th) - R_;
+        }
+        to_world(x_le(s) + c * xi, y, z, w);
+    }
+    bool solid(double x, double y, double z) const override {
+        const double dx = x - le_[0], dz = z - le_[2], yw = y - le_[1];
+        const double xw = dx * ca_ - dz * sa_, zw = dx * sa_ + dz * ca_;
+        double s, off;
+        if (flat_) { s = yw; off = zw; }
+        else {
+            const double zc = zw + R_;
+            s = R_ * std::atan2(yw, zc);
+            off = std::hypot(yw, zc) - R_;
+        }
+        if (std::abs(s) > 0.49 * b_) return false;
+        const double c = chord(s);
+        if (!(c > 0)) return false;
+        return poly_.inside((xw - x_le(s)) / c, off / c);
+    }
+    void surface(double spacing, std::vector<double>& pts) const override {
+        const auto& px = poly_.xs(); const auto& pz = poly_.zs();
+        const int n = static_cast<int>(px.size());
+        const double smax = 0.49 * b_;
+        const int ns = std::max(2, static_cast<int>(std::ceil(2 * smax / spacing)) + 1);
+        for (int j = 0; j < ns; ++j) {
+            const double s = -smax + 2 * smax * j / (ns - 1), c = chord(s);
+            const double ds = spacing / std::max(c, 1e-9);
+            for (int e = 0; e < n; ++e) {
+                const int f = (e + 1) % n;
+                const double len = std::hypot(px[f] - px[e], pz[f] - pz[e]);
+                const int k = std::max(1, static_cast<int>(std::ceil(len / ds)));
+                for (int q = 0; q < k; ++q) {
+                    const double u = static_cast<double>(q) / k;
+                    double w[3];
+                    section_point(s, px[e] + u * (px[f] - px[e]), pz[e] + u * (pz[f] - pz[e]), w);
+                    pts.insert(pts.end(), w, w + 3);
+                }
+            }
+        }
+        // tip caps
+        for (double sgn : {-1.0, 1.0}) {
+            const double c = chord(sgn * smax), d = spacing / std::max(c, 1e-9);
+            for (double xi = poly_.xmin(); xi <= poly_.xmax(); xi += d)
+                for (double ze = zmin_; ze <= zmax_; ze += d)
+                    if (poly_.inside(xi, ze)) {
+                        double w[3];
+                        section_point(sgn * smax, xi, ze, w);
+                        pts.insert(pts.end(), w, w + 3);
+                    }
+        }
+    }
+
+private:
+    Polygon2 poly_;
+    double b_, c0_, R_, tsw_, ca_, sa_, zmin_, zmax_;
+    double le_[3];
+    bool flat_;
+};
+
 // Uniform hash grid of surface points for "is any point within r of this box" queries.
 class PointGrid {
 public:
@@ -373,6 +518,12 @@ void Builder::setup_geometry() {
             if (!(gd_.chord > 0) || !(gd_.span > 0) || gd_.naca_digits <= 0 || gd_.naca_digits > 9999)
                 throw BuildError("invalid NACA wing description");
             geo_ = std::make_unique<NacaWing>(gd_); break;
+        case PCFD_GEOM_PARAGLIDER:
+            if (!(gd_.span > 0) || !(gd_.chord >= 0) || gd_.naca_digits <= 0 || gd_.naca_digits > 9999 ||
+                !(gd_.anhedral_deg >= 0 && gd_.anhedral_deg < 90) || !(std::abs(gd_.sweep_deg) < 60) ||
+                !(std::abs(gd_.reflex) < 0.2))
+                throw BuildError("invalid paraglider description");
+            geo_ = std::make_unique<Paraglider>(gd_); break;
         default: throw BuildError("unsupported geometry kind");
     }
 }

@@ -53,7 +53,7 @@ def cell_closure(t):
     return clo[act], A[act]
 
 
-@pytest.mark.parametrize("name", ["sphere", "sphere_amr", "wing"])
+@pytest.mark.parametrize("name", ["sphere", "sphere_amr", "wing", "glider"])
 def test_every_control_volume_closes(name, request):
     t = request.getfixturevalue(name)
     clo, A = cell_closure(t)
@@ -117,7 +117,7 @@ def test_structured_faces_are_symmetric(sphere_amr):
         assert np.all((mask[other[sel]] & 64) != 0)
 
 
-@pytest.mark.parametrize("name", ["sphere", "wing"])
+@pytest.mark.parametrize("name", ["sphere", "wing", "glider"])
 def test_multigrid_levels(name, request):
     t = request.getfixturevalue(name)
     prev = t.n_active
@@ -165,3 +165,50 @@ def test_refinement_band(wing):
     box_lo, box_hi = np.array([-0.1, -1.1, -0.2]), np.array([1.1, 1.1, 0.2])
     d = np.maximum(0, np.maximum(box_lo - centre[fine], centre[fine] - box_hi))
     assert np.all(np.linalg.norm(d, axis=1) < 0.3 + 2 * 8 * h[0] * math.sqrt(3))
+
+
+@pytest.fixture(scope="module")
+def glider():
+    g, d = cases.paraglider(cells_x=96, levels=3)
+    return L.Topology(g, d)
+
+
+def naca_section_area(digits, n=4000):
+    """Area of the closed-trailing-edge NACA 4-digit section in chord^2 (thickness integral;
+    camber does not change it to first order)."""
+    t = (digits % 100) / 100.0
+    x = 0.5 * (1 - np.cos(np.linspace(0, math.pi, n)))
+    yt = 5 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1036 * x ** 4)
+    return float(np.trapezoid(2 * yt, x))
+
+
+def test_paraglider_volume_and_area(glider):
+    # elliptic chord c(s) = c0 sqrt(1 - (2s/b)^2) up to |2s/b| = 0.98:
+    # volume = A_section * int c^2 ds = A_section * c0^2 b (0.98 - (8/3) 0.49^3)
+    b = 8.0
+    c0 = 4 * b / (5 * math.pi)
+    vol = naca_section_area(2412) * c0 * c0 * b * (0.98 - 8.0 / 3.0 * 0.49 ** 3)
+    assert glider.info.body_volume == pytest.approx(vol, rel=0.04)
+    assert glider.info.reference_area == pytest.approx(math.pi / 4 * c0 * b)  # = b^2 / 5
+
+
+def test_paraglider_arc_sweep_and_symmetry(glider):
+    x = glider.array("patch_x").reshape(-1, 3)
+    S = glider.array("patch_S").reshape(-1, 3)
+    b = 8.0
+    c0 = 4 * b / (5 * math.pi)
+    half = math.radians(25.0)
+    R = 0.5 * b / half
+    # the wing tips droop to z ~ -R (1 - cos 25 deg) at |y| ~ R sin(0.98 * 25 deg) (pitch 6 deg
+    # moves them by less than the tip chord)
+    tip_y = R * math.sin(0.98 * half)
+    assert np.max(np.abs(x[:, 1])) == pytest.approx(tip_y, abs=0.1)
+    tips = np.abs(x[:, 1]) > tip_y - 0.2
+    assert np.mean(x[tips, 2]) == pytest.approx(-R * (1 - math.cos(0.98 * half)), abs=0.3)
+    # swept: the tips sit downstream of the root leading edge
+    assert np.mean(x[tips, 0]) > 0.5 * tip_y * math.tan(math.radians(8.0))
+    # mirror symmetry in y: closed surface (sum of wall area vectors ~ 0), equal halves
+    assert np.linalg.norm(S.sum(axis=0)) < 1e-6 * np.abs(S).sum()
+    left, right = x[:, 1] < 0, x[:, 1] > 0
+    assert np.abs(S[left]).sum() == pytest.approx(np.abs(S[right]).sum(), rel=1e-3)
+    assert S[left, 1].sum() == pytest.approx(-S[right, 1].sum(), rel=1e-3)
