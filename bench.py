@@ -105,6 +105,13 @@ def run(topo, spec, args, name, field, vcycle, pc):
         log(f"{name}: trained {train['train_steps']} steps on {train['pairs']} residuals "
             f"({train['bricks']} bricks), loss {train['loss_first']:.4f} -> {train['loss_last']:.4f}, "
             f"{train['train_ms']:.0f} ms")
+    # every solver runs the same steps; for the learned one they are the validation that
+    # alternates learned and plain projections (6 steps)
+    for i in range(6):
+        st = core.step()
+        core.pressure_loads()
+    if pc == L.PC_LEARNED:
+        log(f"{name}: network {'rejected' if st['learned_rejected'] else 'kept'} by the validation")
     tune = core.autotune(repetitions=2) if args.autotune else core.tuning()
     log(f"{name}: launch shapes {tune}")
     for i in range(args.warmup):

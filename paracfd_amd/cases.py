@@ -105,7 +105,7 @@ CASES = {
                           rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=500),
     "naca2412_3lvl": dict(kind="naca", args=dict(digits=2412, alpha_deg=4.0, cells_x=288, levels=3, bands=(1.0, 0.25)),
                           rho=1.0, nu=1e-5, speed=1.0, cfl=0.5, steps=200),
-    "paraglider_3lvl": dict(kind="paraglider", args=dict(alpha_deg=6.0, cells_x=160, levels=3, bands=(1.0, 0.25)),
+    "paraglider_3lvl": dict(kind="paraglider", args=dict(alpha_deg=6.0, cells_x=400, levels=3, bands=(1.0, 0.25)),
                             rho=1.0, nu=1.5e-5, speed=10.0, cfl=0.5, steps=200),
 }
 

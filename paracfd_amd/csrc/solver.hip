@@ -236,24 +236,16 @@ private:
     }
     void setup_learned(const Topology& T);
     void setup_smagorinsky(const Topology& T);
-    // Validation of a freshly trained network against recent plain projections: the first
-    // kTrial learned projections after training are timed; if their mean exceeds the mean of
-    // the last (up to four) projections without the network, the network is switched off.
+    // Validation of a freshly trained network: the next 2 kTrial projections alternate learned
+    // and plain MGPCG (learned first), so both see the same stage of the flow; if the learned
+    // ones take longer in total, the network is switched off.
     static constexpr int kTrial = 3;
+    bool trial_plain() const { return trial_left_ > 0 && (trial_left_ & 1); }
     void validate_learned(bool used, float project_ms) {
-        if (!nn_) return;
-        if (!used) {
-            plain_ms_.push_back(project_ms);
-            if (plain_ms_.size() > 4) plain_ms_.erase(plain_ms_.begin());
-            return;
-        }
-        if (trial_left_ <= 0) return;
-        trial_ms_ += project_ms;
-        if (--trial_left_ > 0 || plain_ms_.empty()) return;
-        double plain = 0.0;
-        for (double x : plain_ms_) plain += x;
-        plain /= static_cast<double>(plain_ms_.size());
-        if (trial_ms_ / kTrial > plain) {
+        if (!nn_ || trial_left_ <= 0) return;
+        (used ? trial_ms_ : trial_plain_ms_) += project_ms;
+        if (--trial_left_ > 0) return;
+        if (trial_ms_ > trial_plain_ms_) {
             nn_on_ = false;
             nn_rejected_ = true;
             set_nn_off(true);
@@ -369,9 +361,9 @@ private:
     std::vector<double*> nn_rs_;  // recorded post-V-cycle residuals
     std::vector<double> nn_inv_norm_;
     int nn_cap_ = 0, nn_count_ = 0, nn_next_ = 0, nn_cursor_ = 0, nn_t_ = 0;
-    std::vector<double> plain_ms_;  // recent projection times without the network
-    int trial_left_ = 0;            // learned projections still to validate
-    double trial_ms_ = 0.0;
+    int trial_left_ = 0;            // projections of the validation still to run
+    double trial_ms_ = 0.0;         // their learned and plain projection times
+    double trial_plain_ms_ = 0.0;
     bool nn_rejected_ = false;
     R* snap_v_[3] = {};
     R* snap_f_[3] = {};
@@ -958,7 +950,7 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
     finalize(FIN_BNORM, nb_);
     if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
-    const bool learned = nn_ && nn_on_;
+    const bool learned = nn_ && nn_on_ && !trial_plain();
     set_nn_off(!learned);
     pcg_loop(learned);
     int iterations = host_s_[0].iter;
@@ -1594,8 +1586,8 @@ void SolverImpl<R>::learned_train(int steps, int batch, double lr, pcfd_learned_
     HIP_CHECK(hipEventSynchronize(ev_[1]));
     nn_on_ = true;
     nn_rejected_ = false;
-    trial_left_ = kTrial;
-    trial_ms_ = 0.0;
+    trial_left_ = 2 * kTrial;
+    trial_ms_ = trial_plain_ms_ = 0.0;
     set_nn_off(false);
     if (st) {
         std::memset(st, 0, sizeof(*st));

@@ -195,3 +195,28 @@ def test_fallback_on_broken_network(lib):
     core.learned_enable(False)
     st = core.step()
     assert st["fallback_used"] == 0 and st["converged"] == 1
+
+
+def test_validation_rejects_a_network_that_does_not_help(lib):
+    """After training, plain and learned projections alternate (learned first); a network whose
+    correction is negligible (learning rate 1e-30 leaves the zero output layer at ~0) does the
+    same iterations plus its own cost, so the validation switches it off."""
+    topo = sphere(cells_x=32, levels=2)
+    print(f"[learned] validation, dofs={topo.ndof}", flush=True)
+    core = learned_core(topo)
+    core.initialize()
+    core.step()
+    core.learned_collect(max_pairs=8)
+    core.step()
+    core.learned_train(steps=2, batch=1, lr=1e-30)
+    used, its = [], []
+    for _ in range(6):
+        st = core.step()
+        used.append(st["learned_used"])
+        its.append(st["iterations"])
+        assert st["converged"] == 1
+    assert used == [1, 0, 1, 0, 1, 0]
+    print(f"[learned] validation iterations {its}", flush=True)
+    assert st["learned_rejected"] == 1
+    st = core.step()
+    assert st["learned_used"] == 0 and st["learned_rejected"] == 1
