@@ -1111,7 +1111,7 @@ void Builder::build_multigrid() {
 
     T_.mg.push_back(std::move(L0));
     const double hf = T_.finest_h;
-    const int coarse_max = 1024;
+    const int coarse_max = 2048;  // the coarsest level is solved exactly (dense inverse)
     int m = 0;
     std::vector<int> cur_exp = T_.mg[0].exponent;
     std::vector<int> cur_coord = coord;
@@ -1286,22 +1286,29 @@ void Builder::build_multigrid() {
             }
         }
         if (ok) {
-            // inverse column by column
+            // inverse column by column (columns are independent: host threads)
             K.coarse_inv.assign(static_cast<size_t>(n) * n, 0.0);
-            std::vector<double> y(n);
-            for (int c = 0; c < n; ++c) {
-                for (int i = 0; i < n; ++i) {
-                    double s = i == c ? 1.0 : 0.0;
-                    for (int k = 0; k < i; ++k) s -= A[static_cast<size_t>(i) * n + k] * y[k];
-                    y[i] = s / A[static_cast<size_t>(i) * n + i];
+            auto columns = [&](int c0, int c1) {
+                std::vector<double> y(n);
+                for (int c = c0; c < c1; ++c) {
+                    for (int i = 0; i < n; ++i) {
+                        double s = i == c ? 1.0 : 0.0;
+                        for (int k = 0; k < i; ++k) s -= A[static_cast<size_t>(i) * n + k] * y[k];
+                        y[i] = s / A[static_cast<size_t>(i) * n + i];
+                    }
+                    for (int i = n - 1; i >= 0; --i) {
+                        double s = y[i];
+                        for (int k = i + 1; k < n; ++k) s -= A[static_cast<size_t>(k) * n + i] * y[k];
+                        y[i] = s / A[static_cast<size_t>(i) * n + i];
+                    }
+                    for (int i = 0; i < n; ++i) K.coarse_inv[static_cast<size_t>(i) * n + c] = y[i];
                 }
-                for (int i = n - 1; i >= 0; --i) {
-                    double s = y[i];
-                    for (int k = i + 1; k < n; ++k) s -= A[static_cast<size_t>(k) * n + i] * y[k];
-                    y[i] = s / A[static_cast<size_t>(i) * n + i];
-                }
-                for (int i = 0; i < n; ++i) K.coarse_inv[static_cast<size_t>(i) * n + c] = y[i];
-            }
+            };
+            const unsigned hw = std::thread::hardware_concurrency();
+            const int nt = std::max(1, std::min(static_cast<int>(hw ? hw : 1u), std::min(16, n / 64)));
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nt; ++t) pool.emplace_back(columns, n * t / nt, n * (t + 1) / nt);
+            for (auto& th : pool) th.join();
         }
     }
 }
