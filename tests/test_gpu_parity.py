@@ -39,6 +39,8 @@ def make(case, **kw):
         g, d = cases.sphere_box(**kw)
     elif case == "naca":
         g, d = cases.naca_wing(**kw)
+    elif case == "paraglider":
+        g, d = cases.paraglider(**kw)
     else:
         g, d = cases.uniform_box(**kw)
     return L.Topology(g, d)
@@ -144,6 +146,7 @@ def run_both(gpu, orc, steps):
     ("sphere", dict(cells_x=64), 20),
     ("sphere", dict(cells_x=32, levels=2, band=0.3), 10),
     ("naca", dict(digits=2412, alpha_deg=4.0, cells_x=24, levels=2, bands=(0.3,)), 5),
+    ("paraglider", dict(cells_x=48, levels=2, bands=(0.5,)), 4),
 ])
 def test_fp64_step_matches_oracle(lib, oracle_built, case, kw, steps):
     topo = make(case, **kw)
