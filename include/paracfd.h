@@ -275,23 +275,20 @@ pcfd_status pcfd_set_reference_instance(pcfd_solver* solver, int instance, doubl
  * whether the composite-level V-cycle runs as fused kernels; none changes the arithmetic, so
  * every configuration gives bit-identical results.  pcfd_autotune times `repetitions` full
  * steps per candidate from one snapshot of the state (greedy: XCD chunk, then coarse
- * workgroup size, then fused or plain V-cycle, then the one-workgroup coarse tail), keeps the
- * fastest configuration and restores
+ * workgroup size, then fused or plain V-cycle), keeps the fastest configuration and restores
  * the state and physical time, so the solution is unchanged.  Needs a converged
  * initialization. */
 typedef struct pcfd_tuning {
     int xcd_chunk_log2;     /* bricks go to the 8 XCDs in chunks of 2^k (-1: one range per XCD) */
     int coarse_block;       /* workgroup size of the coarse-level kernels: 64, 128, 256 or 512 */
     int fused_vcycle;       /* 1: fused composite-level V-cycle kernels, 0: the plain sequence */
-    int coarse_tail_rows;   /* the V-cycle from the first level with at most this many rows runs in
-                               one workgroup (one launch instead of one per operation); 0: off */
     int candidates;         /* configurations timed by the last pcfd_autotune */
     double step_ms;         /* fastest step of the chosen configuration (last pcfd_autotune) */
     double default_step_ms; /* fastest step of the configuration it started from */
 } pcfd_tuning;
 pcfd_status pcfd_autotune(pcfd_solver* solver, int repetitions, pcfd_tuning* result);
 pcfd_status pcfd_get_tuning(const pcfd_solver* solver, pcfd_tuning* tuning);
-/* Applies the first four fields of a tuning (e.g. one stored from an earlier autotune). */
+/* Applies the first three fields of a tuning (e.g. one stored from an earlier autotune). */
 pcfd_status pcfd_set_tuning(pcfd_solver* solver, const pcfd_tuning* tuning);
 
 /* State I/O (upload_state/download_state amr_advection.cu:1301-1309, upload_pressure

@@ -565,9 +565,7 @@ __global__ void k_residual_csr(CsrView C, const VT* __restrict__ x, const VT* __
     res[r] = from_f<VT>(A(to_f(rhs[r])) - Ax);
 }
 
-// Coarse-level rows.  Each operation is a per-row device function shared by the one-launch-
-// per-operation kernels below and the one-workgroup tail kernel, so both give bit-identical
-// results.
+// Coarse-level rows: one lane per row, each operation a per-row device function.
 
 // ELL sweep: mode 0 from zero, 1 damped Jacobi
 template <class VT, class CT>
@@ -687,111 +685,6 @@ __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off
                                                      const VT* __restrict__ rhs, VT* __restrict__ x, const DevScalars* S) {
     const int k = blockIdx.y, r = blockIdx.x * 4 + (threadIdx.x >> 6);
     dense_row<VT, CT>(off[k], nrow[k], ioff[k], r, inv, rhs, x);
-}
-
-// ---------------------------------------------------------------------------------------
-// Coarse tail: the V-cycle of every level from the first small one down, in one workgroup of
-// 1024 lanes with barriers between the operations instead of one launch per operation (the
-// small levels cost a launch each, not their work).  Same operations in the same order as
-// the launches of the host's coarse cycle, with the same per-row functions.
-// ---------------------------------------------------------------------------------------
-constexpr int kTailMax = 8;
-struct TailLevel {
-    CsrView C;                 // operator, parent map, child ranges
-    void* B; void* X; void* T;
-    int dcount;                // dense blocks of the instances whose hierarchy ends here
-    const int* doff;
-    const int* dn;
-    const int64_t* dioff;
-};
-struct TailParams {
-    TailLevel lv[kTailMax];
-    int nlev;                  // levels in the tail; the last is the coarsest
-    const void* inv;           // dense inverses (CT)
-    double omega;
-    int pre, post;
-    const DevScalars* agg;
-};
-
-// dense blocks of a tail level: one row per wave, 16 rows at a time
-template <class VT, class CT>
-__device__ __forceinline__ void tail_dense(const TailLevel& L, const CT* inv, const VT* B, VT* X) {
-    const int wave = threadIdx.x >> 6;
-    for (int k = 0; k < L.dcount; ++k) {
-        const int n = L.dn[k], off = L.doff[k];
-        const int64_t io = L.dioff[k];
-        for (int r = wave; r < n; r += 16) dense_row<VT, CT>(off, n, io, r, inv, B, X);
-    }
-    __syncthreads();
-}
-
-template <class VT, class CT>
-__global__ __launch_bounds__(1024) void k_coarse_tail(TailParams P) {
-    if (P.agg->done) return;
-    const int tid = threadIdx.x;
-    const double w = P.omega;
-    const CT* inv = static_cast<const CT*>(P.inv);
-    unsigned sel = 0;  // bit l: level l's current iterate is in T (else X)
-    auto cur_of = [&](int l) -> VT* { return static_cast<VT*>(((sel >> l) & 1u) ? P.lv[l].T : P.lv[l].X); };
-    auto oth_of = [&](int l) -> VT* { return static_cast<VT*>(((sel >> l) & 1u) ? P.lv[l].X : P.lv[l].T); };
-    const int last = P.nlev - 1;
-    for (int l = 0; l < last; ++l) {
-        const CsrView& C = P.lv[l].C;
-        const VT* B = static_cast<const VT*>(P.lv[l].B);
-        if (P.pre == 2) {
-    #pragma unroll 4
-        for (int r = tid; r < C.n; r += 1024) ell_pre2_row<VT, CT>(C, r, B, cur_of(l), w);
-            __syncthreads();
-        } else {
-    #pragma unroll 4
-        for (int r = tid; r < C.n; r += 1024) ell_smooth_row<VT, CT>(C, r, 0, cur_of(l), cur_of(l), B, w);
-            __syncthreads();
-            for (int s = 1; s < P.pre; ++s) {
-        #pragma unroll 4
-        for (int r = tid; r < C.n; r += 1024) ell_smooth_row<VT, CT>(C, r, 1, cur_of(l), oth_of(l), B, w);
-                __syncthreads();
-                sel ^= 1u << l;
-            }
-        }
-#pragma unroll 4
-        for (int r = tid; r < C.n; r += 1024) ell_residual_row<VT, CT>(C, r, cur_of(l), B, oth_of(l));
-        __syncthreads();
-        const CsrView& NC = P.lv[l + 1].C;
-        for (int c = tid; c < NC.n; c += 1024) restrict_range_row<VT>(NC, c, oth_of(l), static_cast<VT*>(P.lv[l + 1].B));
-        __syncthreads();
-    }
-    {
-        const TailLevel& L = P.lv[last];
-        const VT* B = static_cast<const VT*>(L.B);
-        if (L.dcount > 0) {
-            tail_dense<VT, CT>(L, inv, B, cur_of(last));
-        } else {  // no factorisation (pure Neumann): 64 sweeps with weight 2/3
-            for (int r = tid; r < L.C.n; r += 1024) ell_smooth_row<VT, CT>(L.C, r, 0, cur_of(last), cur_of(last), B, 2.0 / 3.0);
-            __syncthreads();
-            for (int s = 1; s < 64; ++s) {
-                for (int r = tid; r < L.C.n; r += 1024)
-                    ell_smooth_row<VT, CT>(L.C, r, 1, cur_of(last), oth_of(last), B, 2.0 / 3.0);
-                __syncthreads();
-                sel ^= 1u << last;
-            }
-        }
-    }
-    for (int l = last - 1; l >= 0; --l) {
-        const CsrView& C = P.lv[l].C;
-        const VT* B = static_cast<const VT*>(P.lv[l].B);
-        const VT* xc = cur_of(l + 1);
-#pragma unroll 4
-        for (int r = tid; r < C.n; r += 1024) ell_post_prolong_row<VT, CT>(C, r, cur_of(l), xc, B, oth_of(l), w);
-        __syncthreads();
-        sel ^= 1u << l;
-        for (int s = 1; s < P.post; ++s) {
-    #pragma unroll 4
-        for (int r = tid; r < C.n; r += 1024) ell_smooth_row<VT, CT>(C, r, 1, cur_of(l), oth_of(l), B, w);
-            __syncthreads();
-            sel ^= 1u << l;
-        }
-        if (P.lv[l].dcount > 0) tail_dense<VT, CT>(P.lv[l], inv, B, cur_of(l));
-    }
 }
 
 #undef PCFD_BRICK_PROLOGUE

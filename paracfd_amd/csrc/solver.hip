@@ -315,15 +315,6 @@ private:
     bool use_graph_ = true;
     bool fused_ = true;
     int coarse_block_ = 256;   // workgroup size of the coarse-level (SELL) kernels
-    int tail_rows_ = 0;        // levels from the first with at most this many rows run in one workgroup (0: off)
-    int tail_start() const {
-        if (tail_rows_ <= 0) return -1;
-        const int L = static_cast<int>(lv_.size());
-        for (int m = 1; m < L; ++m)
-            if (lv_[m].n <= tail_rows_ && L - m <= kTailMax) return m;
-        return -1;
-    }
-    template <class VT, class CT> VT* launch_tail(int m0);
     pcfd_tuning tuning_{};
     bool tuning_run_ = false;  // inside autotune: steps leave no learned-validation record
     void snapshot_state();
@@ -444,7 +435,6 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     tuning_.xcd_chunk_log2 = -1;
     tuning_.coarse_block = coarse_block_;
     tuning_.fused_vcycle = fused_ ? 1 : 0;
-    tuning_.coarse_tail_rows = 0;
     if (opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED) {
         rprev_ = alloc<double>(n_);
         HIP_CHECK(hipMemset(rprev_, 0, n_ * sizeof(double)));
@@ -758,7 +748,6 @@ void SolverImpl<R>::reconstruct() {
 template <class R>
 template <class VT, class CT>
 VT* SolverImpl<R>::coarse_cycle_t(int m) {
-    if (m == tail_start()) return launch_tail<VT, CT>(m);
     Level& L = lv_[m];
     const CsrView& C = std::is_same<CT, double>::value ? L.csr64 : L.csr32;
     VT* B = static_cast<VT*>(L.B);
@@ -792,34 +781,6 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     // coarse solve (what their own V-cycle does at its last level)
     dense_blocks<VT, CT>(m, B, cur);
     return cur;
-}
-
-// the levels m0.. in one workgroup; returns the buffer that holds level m0's result (the
-// kernel's swap sequence replayed on the host)
-template <class R>
-template <class VT, class CT>
-VT* SolverImpl<R>::launch_tail(int m0) {
-    TailParams P{};
-    P.nlev = static_cast<int>(lv_.size()) - m0;
-    for (int i = 0; i < P.nlev; ++i) {
-        Level& L = lv_[m0 + i];
-        P.lv[i].C = std::is_same<CT, double>::value ? L.csr64 : L.csr32;
-        P.lv[i].B = L.B; P.lv[i].X = L.X; P.lv[i].T = L.T;
-        const DenseLevel& D = dense_[m0 + i];
-        P.lv[i].dcount = D.count; P.lv[i].doff = D.off; P.lv[i].dn = D.n; P.lv[i].dioff = D.ioff;
-    }
-    if constexpr (std::is_same<CT, double>::value) P.inv = dense_inv64_; else P.inv = dense_inv32_;
-    P.omega = opt_.smoother_weight;
-    P.pre = opt_.pre_smooth;
-    P.post = opt_.post_smooth;
-    P.agg = agg_;
-    hipLaunchKernelGGL((k_coarse_tail<VT, CT>), dim3(1), dim3(1024), 0, stream_, P);
-    HIP_CHECK(hipGetLastError());
-    int swaps;
-    if (P.nlev == 1) swaps = P.lv[0].dcount > 0 ? 0 : 63;
-    else swaps = (opt_.pre_smooth == 2 ? 0 : std::max(0, opt_.pre_smooth - 1)) + 1 + std::max(0, opt_.post_smooth - 1);
-    Level& L0 = lv_[m0];
-    return static_cast<VT*>((swaps & 1) ? L0.T : L0.X);
 }
 
 template <class R>
@@ -1385,16 +1346,12 @@ void SolverImpl<R>::set_tuning(const pcfd_tuning& t) {
     if (t.xcd_chunk_log2 < -1 || t.xcd_chunk_log2 > 16) throw std::invalid_argument("xcd_chunk_log2 must lie in [-1, 16]");
     if (t.coarse_block != 64 && t.coarse_block != 128 && t.coarse_block != 256 && t.coarse_block != 512)
         throw std::invalid_argument("coarse_block must be 64, 128, 256 or 512");
-    if (t.coarse_tail_rows < 0 || t.coarse_tail_rows > (1 << 22))
-        throw std::invalid_argument("coarse_tail_rows must lie in [0, 2^22]");
     M_.xcd_chunk = t.xcd_chunk_log2;
     coarse_block_ = t.coarse_block;
     fused_ = t.fused_vcycle != 0;
-    tail_rows_ = t.coarse_tail_rows;
     tuning_.xcd_chunk_log2 = t.xcd_chunk_log2;
     tuning_.coarse_block = t.coarse_block;
     tuning_.fused_vcycle = fused_ ? 1 : 0;
-    tuning_.coarse_tail_rows = tail_rows_;
     drop_graph();  // kernel arguments are baked into the captured chunk
 }
 
@@ -1446,19 +1403,6 @@ void SolverImpl<R>::autotune(int repetitions, pcfd_tuning* out) {
         if (lv_.size() > 1 && opt_.pre_smooth == 2) {
             pcfd_tuning t = cur;
             t.fused_vcycle = cur.fused_vcycle ? 0 : 1;
-            const double ms = measure(t);
-            if (ms < best) { best = ms; cur = t; }
-        }
-        int last_start = -2;
-        for (int rows : {0, 2048, 8192}) {
-            if (rows == cur.coarse_tail_rows) continue;
-            tail_rows_ = rows;
-            const int start = tail_start();
-            tail_rows_ = cur.coarse_tail_rows;
-            if (start == last_start || (rows > 0 && start < 0)) continue;  // same split as a timed one
-            last_start = start;
-            pcfd_tuning t = cur;
-            t.coarse_tail_rows = rows;
             const double ms = measure(t);
             if (ms < best) { best = ms; cur = t; }
         }

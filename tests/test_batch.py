@@ -157,8 +157,8 @@ def test_mixed_depth_merge(mixed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tail", [0, 4096, 1 << 22])
-def test_mixed_depth_batch_matches_individual_runs(mixed, tail):
+@pytest.mark.parametrize("shape", [(-1, 256, 1), (4, 128, 0)])
+def test_mixed_depth_batch_matches_individual_runs(mixed, shape):
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
@@ -166,9 +166,9 @@ def test_mixed_depth_batch_matches_individual_runs(mixed, tail):
     parts, merged = mixed
     o = default_options(rho=1.0, nu=1e-4, speed=1.0, cfl=0.5, rtol=1e-8, max_iterations=300,
                         field_precision=L.FP64, vcycle_precision=L.FP64)
-    print(f"[batch] mixed depths, coarse tail rows {tail}", flush=True)
+    print(f"[batch] mixed depths, launch shape {shape}", flush=True)
     batch = ExternalAeroCore(merged, o)
-    batch.set_tuning(xcd_chunk_log2=4, coarse_block=128, fused_vcycle=1, coarse_tail_rows=tail)
+    batch.set_tuning(xcd_chunk_log2=shape[0], coarse_block=shape[1], fused_vcycle=shape[2])
     singles = [ExternalAeroCore(p, o) for p in parts]
     batch.initialize()
     for c in singles:
