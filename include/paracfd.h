@@ -282,13 +282,15 @@ typedef struct pcfd_tuning {
     int xcd_chunk_log2;     /* bricks go to the 8 XCDs in chunks of 2^k (-1: one range per XCD) */
     int coarse_block;       /* workgroup size of the coarse-level kernels: 64, 128, 256 or 512 */
     int fused_vcycle;       /* 1: fused composite-level V-cycle kernels, 0: the plain sequence */
+    int graph_replay;       /* 1: PCG iterations replayed from a captured HIP graph, 0: launched one
+                               by one (not searched by pcfd_autotune) */
     int candidates;         /* configurations timed by the last pcfd_autotune */
     double step_ms;         /* fastest step of the chosen configuration (last pcfd_autotune) */
     double default_step_ms; /* fastest step of the configuration it started from */
 } pcfd_tuning;
 pcfd_status pcfd_autotune(pcfd_solver* solver, int repetitions, pcfd_tuning* result);
 pcfd_status pcfd_get_tuning(const pcfd_solver* solver, pcfd_tuning* tuning);
-/* Applies the first three fields of a tuning (e.g. one stored from an earlier autotune). */
+/* Applies the first four fields of a tuning (e.g. one stored from an earlier autotune). */
 pcfd_status pcfd_set_tuning(pcfd_solver* solver, const pcfd_tuning* tuning);
 
 /* State I/O (upload_state/download_state amr_advection.cu:1301-1309, upload_pressure

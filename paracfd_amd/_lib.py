@@ -82,7 +82,8 @@ class LearnedStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("xcd_chunk_log2", C.c_int), ("coarse_block", C.c_int), ("fused_vcycle", C.c_int),
-                ("candidates", C.c_int), ("step_ms", C.c_double), ("default_step_ms", C.c_double)]
+                ("graph_replay", C.c_int), ("candidates", C.c_int), ("step_ms", C.c_double),
+                ("default_step_ms", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -155,8 +155,8 @@ class ExternalAeroCore:
         L.check(self._lib.pcfd_get_tuning(self.handle, C.byref(out)))
         return out.as_dict()
 
-    def set_tuning(self, xcd_chunk_log2=-1, coarse_block=256, fused_vcycle=1):
-        t = L.Tuning(xcd_chunk_log2, coarse_block, fused_vcycle, 0, 0.0, 0.0)
+    def set_tuning(self, xcd_chunk_log2=-1, coarse_block=256, fused_vcycle=1, graph_replay=1):
+        t = L.Tuning(xcd_chunk_log2, coarse_block, fused_vcycle, graph_replay, 0, 0.0, 0.0)
         L.check(self._lib.pcfd_set_tuning(self.handle, C.byref(t)))
 
     def batch_loads(self, pref=0.0) -> list:

@@ -159,7 +159,7 @@ struct CompactFetch {
 };
 
 template <class A, class Fn>
-__device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int sp, int64_t a, Fn val) {
+__device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int sp, Fn val) {
     CompactFetch<A> f;
     f.k0 = f.k1 = 0;
 #pragma unroll

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* _
     const double rb = rhs[a];
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<double> cf = fetch_compact<double>(M, sp, a, val);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, val);
     if (done) return;
     __shared__ double tile[kTileN];
     stage_tile<double>(tile, tf);
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* 
     const TileFetch<double> tf = fetch_tile<double>(M, b, val);
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<double> cf = fetch_compact<double>(M, sp, a, val);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, val);
     if (done) return;
     __shared__ double tile[kTileN];
     stage_tile<double>(tile, tf);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* 
     const TileFetch<double> tf = fetch_tile<double>(M, b, dn);
     const bool done = I.done != 0;
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<double> cf = fetch_compact<double>(M, sp, a, dn);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, dn);
     if (done) return;
     __shared__ double tile[kTileN];
     stage_tile<double>(tile, tf);
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT*
     const double rp = (mode == 2 && rprev) ? rprev[a] : 0.0;
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, val);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, val);
     if (done) return;
     __shared__ A tile[kTileN];
     stage_tile<A>(tile, tf);
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restr
     const double rb = rhs[a];
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, val);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, val);
     if (done) return;
     __shared__ A tile[kTileN];
     stage_tile<A>(tile, tf);
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(512) void k_vc_pre2(MeshView M, const double* __res
     const double rb = rhs[a];
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, x1);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, x1);
     if (done) return;
     __shared__ A tile[kTileN];
     stage_tile<A>(tile, tf);
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(512) void k_vc_restrict0(MeshView M, const VT* __re
     const int pa = parent0[a];
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, val);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, val);
     if (done) return;
     __shared__ A tile[kTileN];
     __shared__ A sres[512];
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, c
     const double rp = (last && rprev) ? rprev[a] : 0.0;
     const bool done = brick_done(M, S, b);
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, a, y);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, y);
     if (done) return;
     __shared__ A tile[kTileN];
     stage_tile<A>(tile, tf);
@@ -538,7 +538,7 @@ __device__ __forceinline__ A ell_row(const CsrView& C, int r, Fn y, A yr) {
 // CSR forms (multigrid primitive entry point; same arithmetic as the ELL kernels)
 template <class VT, class CT>
 __global__ void k_smooth_csr(CsrView C, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
-                             const VT* __restrict__ rhs, double omega, const DevScalars* S) {
+                             const VT* __restrict__ rhs, double omega) {
     using A = typename Arith<VT>::type;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= C.n) return;
@@ -553,8 +553,7 @@ __global__ void k_smooth_csr(CsrView C, int mode, const VT* __restrict__ xin, VT
 }
 
 template <class VT, class CT>
-__global__ void k_residual_csr(CsrView C, const VT* __restrict__ x, const VT* __restrict__ rhs, VT* __restrict__ res,
-                               const DevScalars* S) {
+__global__ void k_residual_csr(CsrView C, const VT* __restrict__ x, const VT* __restrict__ rhs, VT* __restrict__ res) {
     using A = typename Arith<VT>::type;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= C.n) return;
@@ -682,7 +681,7 @@ __device__ __forceinline__ void dense_row(int off, int n, int64_t ioff, int r, c
 template <class VT, class CT>
 __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off, const int* __restrict__ nrow,
                                                      const int64_t* __restrict__ ioff, const CT* __restrict__ inv,
-                                                     const VT* __restrict__ rhs, VT* __restrict__ x, const DevScalars* S) {
+                                                     const VT* __restrict__ rhs, VT* __restrict__ x) {
     const int k = blockIdx.y, r = blockIdx.x * 4 + (threadIdx.x >> 6);
     dense_row<VT, CT>(off[k], nrow[k], ioff[k], r, inv, rhs, x);
 }

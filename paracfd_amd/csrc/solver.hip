@@ -202,7 +202,7 @@ private:
         const CT* inv = nullptr;
         if constexpr (std::is_same<CT, double>::value) inv = dense_inv64_; else inv = dense_inv32_;
         hipLaunchKernelGGL((k_dense_solve<VT, CT>), dim3(blocks_for(D.max_n, 4), D.count), dim3(256), 0, stream_, D.off, D.n, D.ioff,
-                           inv, B, X, agg_);
+                           inv, B, X);
         HIP_CHECK(hipGetLastError());
         return true;
     }
@@ -430,11 +430,10 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     dbuf_[1] = alloc<double>(n_);
     HIP_CHECK(hipMemset(dbuf_[1], 0, n_ * sizeof(double)));
     chunk_ = std::max(2, (opt_.check_interval + 1) / 2 * 2);
-    if (const char* g = std::getenv("PCFD_NO_GRAPH")) use_graph_ = g[0] == '0';
-    if (const char* f = std::getenv("PCFD_UNFUSED")) fused_ = f[0] == '0';
     tuning_.xcd_chunk_log2 = -1;
     tuning_.coarse_block = coarse_block_;
     tuning_.fused_vcycle = fused_ ? 1 : 0;
+    tuning_.graph_replay = use_graph_ ? 1 : 0;
     if (opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED) {
         rprev_ = alloc<double>(n_);
         HIP_CHECK(hipMemset(rprev_, 0, n_ * sizeof(double)));
@@ -1219,13 +1218,13 @@ void SolverImpl<R>::mg_primitive(int op, int level, const double* x, const doubl
         // op 0 uses b = 0 and negates: A x = -(0 - A x)
         if (op == 0) HIP_CHECK(hipMemset(db, 0, n * sizeof(double)));
         if (level == 0) LAUNCH(k_residual0<double>, nb_, 512, M_, dx, db, dt, S_);
-        else LAUNCH((k_residual_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, dx, db, dt, S_);
+        else LAUNCH((k_residual_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, dx, db, dt);
     } else if (op == 2) {
         double* cur = dx;
         double* oth = dt;
         for (int s = 0; s < sweeps; ++s) {
             if (level == 0) LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, cur, oth, db, z_, static_cast<double*>(nullptr), omega, partials_, S_);
-            else LAUNCH((k_smooth_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, 1, cur, oth, db, omega, S_);
+            else LAUNCH((k_smooth_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, 1, cur, oth, db, omega);
             std::swap(cur, oth);
         }
         result = cur;
@@ -1349,9 +1348,11 @@ void SolverImpl<R>::set_tuning(const pcfd_tuning& t) {
     M_.xcd_chunk = t.xcd_chunk_log2;
     coarse_block_ = t.coarse_block;
     fused_ = t.fused_vcycle != 0;
+    use_graph_ = t.graph_replay != 0;
     tuning_.xcd_chunk_log2 = t.xcd_chunk_log2;
     tuning_.coarse_block = t.coarse_block;
     tuning_.fused_vcycle = fused_ ? 1 : 0;
+    tuning_.graph_replay = use_graph_ ? 1 : 0;
     drop_graph();  // kernel arguments are baked into the captured chunk
 }
 

@@ -213,17 +213,9 @@ def test_fused_graph_path_matches_plain_launches(lib, prec):
     from paracfd_amd import ExternalAeroCore
     topo = make("sphere", cells_x=32, levels=2, band=0.3)
     res = []
-    for env in ({}, {"PCFD_UNFUSED": "1", "PCFD_NO_GRAPH": "1"}):
-        old = {k: os.environ.get(k) for k in ("PCFD_UNFUSED", "PCFD_NO_GRAPH")}
-        os.environ.update(env)
-        try:
-            core = ExternalAeroCore(topo, opts(vcycle_precision=prec, check_interval=4))
-        finally:
-            for k, v in old.items():
-                if v is None:
-                    os.environ.pop(k, None)
-                else:
-                    os.environ[k] = v
+    for fused in (1, 0):
+        core = ExternalAeroCore(topo, opts(vcycle_precision=prec, check_interval=4))
+        core.set_tuning(fused_vcycle=fused, graph_replay=fused)
         core.initialize()
         its = [core.step()["iterations"] for _ in range(3)]
         res.append((its, core.download_pressure(), core.download_cell_momentum_state()[0]))
