@@ -658,32 +658,21 @@ __global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT
     if (r < C.n) ell_post_prolong_row<VT, CT>(C, r, x, e, rhs, xout, omega);
 }
 
-// one row of a dense block solve, x[off + r] = sum_c inv[ioff + r n + c] b[off + c], by one
-// wave: lane l sums the columns l, l + 64, ... in order, then the wave's butterfly.  The
-// whole wave takes the same r (rows r >= n write nothing).
-template <class VT, class CT>
-__device__ __forceinline__ void dense_row(int off, int n, int64_t ioff, int r, const CT* __restrict__ inv,
-                                          const VT* __restrict__ rhs, VT* __restrict__ x) {
-    const int lane = threadIdx.x & 63;
-    double acc = 0.0;
-    if (r < n) {
-        const CT* row = inv + ioff + static_cast<int64_t>(r) * n;
-        const VT* b = rhs + off;
-        for (int c = lane; c < n; c += 64) acc += double(row[c]) * double(to_f(b[c]));
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (r < n && lane == 0) x[off + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
-}
-
-// coarsest level of each instance: x = A^-1 b with its dense inverse; wave w of block
-// (x, k) solves row 4 x + w of dense block k (cells [off[k], off[k] + n[k]) of the level,
-// inverse at ioff[k])
+// coarsest level of each instance: x = A^-1 b with its dense inverse; block (row, k) solves
+// row `row` of dense block k (cells [off[k], off[k] + n[k]) of the level, inverse at ioff[k]):
+// lane t sums the columns t, t + 256, ..., then the block sum
 template <class VT, class CT>
 __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off, const int* __restrict__ nrow,
                                                      const int64_t* __restrict__ ioff, const CT* __restrict__ inv,
                                                      const VT* __restrict__ rhs, VT* __restrict__ x) {
-    const int k = blockIdx.y, r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    dense_row<VT, CT>(off[k], nrow[k], ioff[k], r, inv, rhs, x);
+    const int k = blockIdx.y, r = blockIdx.x, n = nrow[k];
+    if (r >= n) return;  // block-uniform
+    const CT* row = inv + ioff[k] + static_cast<int64_t>(r) * n;
+    const VT* b = rhs + off[k];
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < n; c += 256) acc += double(row[c]) * double(to_f(b[c]));
+    acc = block_sum<256>(acc);
+    if (threadIdx.x == 0) x[off[k] + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
 }
 
 #undef PCFD_BRICK_PROLOGUE

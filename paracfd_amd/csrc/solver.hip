@@ -201,7 +201,7 @@ private:
         if (D.count == 0) return false;
         const CT* inv = nullptr;
         if constexpr (std::is_same<CT, double>::value) inv = dense_inv64_; else inv = dense_inv32_;
-        hipLaunchKernelGGL((k_dense_solve<VT, CT>), dim3(blocks_for(D.max_n, 4), D.count), dim3(256), 0, stream_, D.off, D.n, D.ioff,
+        hipLaunchKernelGGL((k_dense_solve<VT, CT>), dim3(D.max_n, D.count), dim3(256), 0, stream_, D.off, D.n, D.ioff,
                            inv, B, X);
         HIP_CHECK(hipGetLastError());
         return true;

@@ -186,3 +186,42 @@ def test_mixed_depth_batch_matches_individual_runs(mixed, shape):
         x, _, _ = c.download_cell_momentum_state()
         for a, b in ((bx[sl], x), (bp[sl], c.download_pressure())):
             assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
+
+
+@pytest.mark.gpu
+def test_batch_with_an_instance_without_body():
+    """An empty box (no body: zero right-hand side, no surface patches) batched with a wing:
+    the empty instance's projection ends at once with zero pressure and zero loads while the
+    wing advances exactly as it does alone."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paracfd_amd import ExternalAeroCore, default_options
+    gw, dw = cases.naca_wing(digits=4412, alpha_deg=4.0, cells_x=24, levels=2, bands=(0.3,))
+    w = L.Topology(gw, dw)
+    ge = L.GeometryDesc()
+    ge.kind = L.GEOM_NONE
+    e = L.Topology(ge, dw)
+    merged = L.Topology.merge([e, w])
+    assert e.info.patches == 0
+    o = default_options(rho=1.0, nu=1e-4, speed=1.0, cfl=0.5, rtol=1e-8, max_iterations=300,
+                        field_precision=L.FP64, vcycle_precision=L.FP64)
+    print(f"[batch] empty instance + wing, {merged.info.dofs} dofs", flush=True)
+    batch, single = ExternalAeroCore(merged, o), ExternalAeroCore(w, o)
+    batch.set_instance_reference(1, w.info.reference_area)
+    single.set_reference(w.info.reference_area)
+    batch.initialize()
+    single.initialize()
+    for _ in range(3):
+        bs, ss = batch.step(), single.step()
+        assert bs["converged"] == 1
+        assert batch.instance_stats(0)["iterations"] == 0
+        assert batch.instance_stats(1)["iterations"] == ss["iterations"]
+    e0, w0 = merged.instance(0), merged.instance(1)
+    bp = batch.download_pressure()
+    assert np.all(bp[e0["dof_begin"]:e0["dof_end"]] == 0.0)
+    sl = slice(w0["dof_begin"], w0["dof_end"])
+    assert np.max(np.abs(bp[sl] - single.download_pressure())) <= 1e-12 * np.max(np.abs(bp[sl]))
+    loads = batch.batch_loads()
+    assert loads[0]["drag"] == 0.0 and loads[0]["lift"] == 0.0 and loads[0]["coefficients_valid"] == 0
+    assert loads[1]["cd"] == pytest.approx(single.pressure_loads()["cd"], rel=1e-11)

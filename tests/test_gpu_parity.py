@@ -171,6 +171,37 @@ def test_fp64_step_matches_oracle(lib, oracle_built, case, kw, steps):
             assert rel_l2(g, o) <= 1e-10
 
 
+def test_single_brick_box_matches_oracle(lib, oracle_built):
+    """The smallest topology (one 8^3 brick, no body, outlet at x-max) from a perturbed
+    velocity field: every neighbour of the brick is a domain boundary and the hierarchy is one
+    coarse level solved exactly."""
+    topo = make("box", n=8, h=0.125, outlet=True)
+    assert topo.info.bricks == 1
+    gpu, orc = pair(topo, opts())
+    gpu.initialize()
+    orc.initialize()
+    rng = np.random.default_rng(7)
+    act = ((topo.array("mask") & 64) != 0).astype(float)
+    v = [1.0 * act + 0.2 * rng.standard_normal(topo.ndof) * act for _ in range(3)]
+    v[1] -= 1.0 * act
+    v[2] -= 1.0 * act
+    gpu.upload_state(*v)
+    orc.set_state(*v)
+    gpu.mark_initialized(0.0)
+    orc.mark_initialized(0.0)
+    for _ in range(3):
+        gs, os_ = gpu.step(), orc.step()
+        assert gs["converged"] and os_.converged
+        assert gs["iterations"] == os_.iterations and gs["iterations"] > 0
+        assert abs(gs["dt"] - os_.dt) <= 1e-12 * os_.dt
+    a = act > 0
+    gx, gy, gz = gpu.download_cell_momentum_state()
+    ox, oy, oz, op = orc.state()
+    for g, o in ((gx, ox), (gy, oy), (gz, oz)):
+        assert rel_l2(g[a], o[a]) <= 1e-10
+    assert rel_l2(gpu.download_pressure()[a], op[a]) <= 1e-10
+
+
 @pytest.mark.parametrize("case,kw", [
     ("sphere", dict(cells_x=32, levels=2, band=0.3)),
     ("naca", dict(digits=2412, alpha_deg=4.0, cells_x=24, levels=2, bands=(0.3,))),
