@@ -22,6 +22,19 @@ def build(force=False):
     subprocess.run(["make", "-C", HERE, "-s"] + (["-B"] if force else []), check=True)
 
 
+REFERENCE_SRC = "/root/reference/src"
+
+
+def build_reference():
+    """Compile the reference's own composite-operator code into _ref/libref.so (test
+    infrastructure).  Only where the reference sources are present; elsewhere the tests that
+    use it skip.  Returns whether the library was built."""
+    if not os.path.isdir(os.path.join(REFERENCE_SRC, "core", "fluid")):
+        return False
+    subprocess.run(["make", "-C", HERE, "-s", "-B", "ref", f"REF={REFERENCE_SRC}"], check=True)
+    return True
+
+
 def _need(path):
     if not os.path.exists(path):
         build()
