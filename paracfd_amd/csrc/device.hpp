@@ -94,6 +94,7 @@ struct CsrView {
     const int* parent;  // into the next coarser level
     const int* child_off;
     const int* child;
+    const int* child_first;  // child[child_off[c]] (or -1 without children): one load, no chain
     // the same rows in SELL-64: slices of 64 rows, slice s holds its rows slot-major at
     // sl_off[s] + 64 k + (r & 63) with k < (sl_off[s+1] - sl_off[s]) / 64, padded with
     // (column r, coefficient 0); coalesced, and only as wide as its longest row
@@ -259,8 +260,7 @@ __device__ __forceinline__ int axis_coord(int q, int axis) { return (q >> (3 * a
 // share one L2 while bricks of one region (the cut cells of a body) spread over all XCDs;
 // lc < 0 gives every XCD one contiguous range.  Bricks past the last whole round of 8
 // chunks are split into per-XCD ranges.
-__device__ __forceinline__ int brick_of_block(int lc = -1) {
-    const int G = static_cast<int>(gridDim.x), g = static_cast<int>(blockIdx.x);
+__device__ __forceinline__ int brick_of(int lc, int g, int G) {
     const int full = lc >= 0 ? (G >> (lc + 3)) << (lc + 3) : 0;
     if (g < full) {
         const int x = g & 7, idx = g >> 3;
@@ -269,6 +269,10 @@ __device__ __forceinline__ int brick_of_block(int lc = -1) {
     const int t = g - full, T = G - full;
     const int x = t & 7, idx = t >> 3, per = T >> 3, rem = T & 7;
     return full + x * per + (x < rem ? x : rem) + idx;
+}
+
+__device__ __forceinline__ int brick_of_block(int lc = -1) {
+    return brick_of(lc, static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
 }
 
 // neighbour dof across face (axis, sign) of local cell q in brick b; the caller has checked
@@ -341,13 +345,20 @@ __device__ __forceinline__ void store_partial(double* partials, int slot, double
     if (threadIdx.x == 0) partials[slot] = value;
 }
 
-// Sum (or max) of n partials in fixed order; result valid in thread 0.
+// Sum (or max) of n partials in fixed order; result valid in thread 0.  Thread t folds
+// slots t, t + BLOCK, t + 2 BLOCK, ... in that order; the slots are loaded 16 at a time
+// before any is folded, so a batch costs one memory latency, not sixteen.
 template <int BLOCK>
 __device__ __forceinline__ double gather_partials(const double* partials, int n, bool take_max) {
+    constexpr int U = 16;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += BLOCK) {
-        const double v = partials[i];
-        acc = take_max ? fmax(acc, v) : acc + v;
+    for (int base = threadIdx.x; base < n; base += U * BLOCK) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = base + u * BLOCK < n ? partials[base + u * BLOCK] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u * BLOCK < n) acc = take_max ? fmax(acc, v[u]) : acc + v[u];
     }
     return take_max ? block_max<BLOCK>(acc) : block_sum<BLOCK>(acc);
 }

@@ -198,14 +198,20 @@ enum FinalizeOp : int {
     FIN_DIFFRATE = 10 // max explicit diffusion rate (Smagorinsky substeps)
 };
 
-__device__ __forceinline__ void finalize_one(int op, const double* __restrict__ partials, int nparts, int beg, int end,
-                                             DevScalars* S, double cfl, double rate_floor, double rtol,
-                                             int max_iterations, int flexible) {
+// The instance's scalars that an op reads (done flag, rz, bnorm, iteration count) are loaded
+// before the partials are gathered, so the kernel waits for one latency for both.  Thread 0
+// applies the op unless `skip`; returns the instance's done flag after the op (thread 0).
+__device__ __forceinline__ int finalize_one(int op, const double* __restrict__ partials, int nparts, int beg, int end,
+                                            DevScalars* S, bool krylov, double cfl, double rate_floor, double rtol,
+                                            int max_iterations, int flexible) {
+    int done = S->done;
+    const double rz = S->rz, bnorm = S->bnorm;
+    const int iter = S->iter;
     const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE;
     const int arrays = op == FIN_LOADS ? 6 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
     double t[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts + beg, end - beg, take_max);
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0 || (krylov && done)) return done;
     switch (op) {
         case FIN_RATE:
             S->rate = t[0];
@@ -213,31 +219,37 @@ __device__ __forceinline__ void finalize_one(int op, const double* __restrict__ 
             break;
         case FIN_BNORM:
             S->bnorm = sqrt(t[0]);
-            S->iter = 0; S->done = 0; S->converged = 0; S->breakdown = 0; S->zero_rhs = 0;
-            if (!(t[0] > 0.0)) { S->done = 1; S->converged = 1; S->zero_rhs = 1; S->relres = 0.0; }
+            S->iter = 0; S->converged = 0; S->breakdown = 0; S->zero_rhs = 0;
+            done = 0;
+            if (!(t[0] > 0.0)) { done = 1; S->converged = 1; S->zero_rhs = 1; S->relres = 0.0; }
+            S->done = done;
             break;
         case FIN_MAXDIV: S->maxdiv = t[0]; break;
         case FIN_VMAX: S->vmax = t[0]; break;
-        case FIN_RESID:
+        case FIN_RESID: {
             S->rr = t[0];
-            S->relres = sqrt(t[0]) / S->bnorm;
-            if (S->relres <= rtol) { S->done = 1; S->converged = 1; }
+            const double rel = sqrt(t[0]) / bnorm;
+            S->relres = rel;
+            if (rel <= rtol) { done = 1; S->done = 1; S->converged = 1; }
             break;
+        }
         case FIN_APPLY:
             S->pAp = t[0];
-            if (!(t[0] > 0.0) || isinf(t[0])) { S->done = 1; S->breakdown = 1; }
-            else S->alpha = S->rz / t[0];
+            if (!(t[0] > 0.0) || isinf(t[0])) { done = 1; S->done = 1; S->breakdown = 1; }
+            else S->alpha = rz / t[0];
             break;
-        case FIN_UPDATE:
+        case FIN_UPDATE: {
             S->rr = t[0];
-            S->iter += 1;
-            S->relres = sqrt(t[0]) / S->bnorm;
-            if (S->relres <= rtol) { S->done = 1; S->converged = 1; }
-            else if (S->iter >= max_iterations || !(t[0] == t[0])) S->done = 1;
+            S->iter = iter + 1;
+            const double rel = sqrt(t[0]) / bnorm;
+            S->relres = rel;
+            if (rel <= rtol) { done = 1; S->done = 1; S->converged = 1; }
+            else if (iter + 1 >= max_iterations || !(t[0] == t[0])) { done = 1; S->done = 1; }
             break;
+        }
         case FIN_RZ:
             // Fletcher-Reeves for a fixed preconditioner; Polak-Ribiere (flexible CG) with r_prev
-            S->beta = (t[0] - t[1]) / S->rz;
+            S->beta = (t[0] - t[1]) / rz;
             S->rz = t[0];
             break;
         case FIN_RZ_INITIAL: S->rz = t[0]; break;
@@ -247,11 +259,13 @@ __device__ __forceinline__ void finalize_one(int op, const double* __restrict__ 
         case FIN_DIFFRATE: S->diff_rate = t[0]; break;
         default: break;
     }
+    return done;
 }
 
 // One workgroup per instance: partial slots [seg[i], seg[i+1]) of each array (arrays are
 // nparts apart; seg == nullptr: all nparts slots), scalars of instance i in S[i].  With
-// `agg`, the last workgroup to finish sets agg->done when every instance is done.
+// `agg`, agg->done is set when every instance is done: directly for one instance, else by
+// the last workgroup to finish.
 __global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restrict__ partials, int nparts,
                                                    const int* __restrict__ seg, DevScalars* Sbase, DevScalars* agg,
                                                    double cfl, double rate_floor, double rtol, int max_iterations,
@@ -259,9 +273,13 @@ __global__ __launch_bounds__(1024) void k_finalize(int op, const double* __restr
     const int inst = blockIdx.x;
     DevScalars* S = Sbase + inst;
     const bool krylov = op == FIN_RESID || op == FIN_APPLY || op == FIN_UPDATE || op == FIN_RZ || op == FIN_RZ_INITIAL;
-    if (!(krylov && S->done)) finalize_one(op, partials, nparts, seg ? seg[inst] : 0, seg ? seg[inst + 1] : nparts, S,
-                                           cfl, rate_floor, rtol, max_iterations, flexible);
+    const int done = finalize_one(op, partials, nparts, seg ? seg[inst] : 0, seg ? seg[inst + 1] : nparts, S, krylov,
+                                  cfl, rate_floor, rtol, max_iterations, flexible);
     if (agg && threadIdx.x == 0) {
+        if (gridDim.x == 1) {
+            agg->done = done;
+            return;
+        }
         __threadfence();
         const unsigned int ticket = atomicAdd(&agg->ticket[0], 1u);
         if (ticket == gridDim.x - 1) {  // every instance's update is visible
@@ -608,9 +626,17 @@ template <class VT>
 __device__ __forceinline__ void restrict_range_row(const CsrView& Cc, int c, const VT* __restrict__ fine,
                                                    VT* __restrict__ coarse) {
     using A = typename Arith<VT>::type;
-    const int k0 = Cc.child_off[c], k1 = Cc.child_off[c + 1], f0 = Cc.child[k0];
+    const int k0 = Cc.child_off[c], k1 = Cc.child_off[c + 1], f0 = Cc.child_first[c];
     A s = A(0);
-    for (int k = k0; k < k1; ++k) s += A(to_f(fine[f0 + (k - k0)]));
+    // eight children per round, all loaded before the first is added
+    for (int k = k0; k < k1; k += 8) {
+        A v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = k + j < k1 ? A(to_f(fine[f0 + (k + j - k0)])) : A(0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k + j < k1) s += v[j];
+    }
     coarse[c] = from_f<VT>(s);
 }
 
@@ -660,7 +686,7 @@ __global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT
 
 // coarsest level of each instance: x = A^-1 b with its dense inverse; block (row, k) solves
 // row `row` of dense block k (cells [off[k], off[k] + n[k]) of the level, inverse at ioff[k]):
-// lane t sums the columns t, t + 256, ..., then the block sum
+// lane t sums the columns t, t + 256, ... in that order, then the block sum
 template <class VT, class CT>
 __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off, const int* __restrict__ nrow,
                                                      const int64_t* __restrict__ ioff, const CT* __restrict__ inv,
@@ -670,7 +696,19 @@ __global__ __launch_bounds__(256) void k_dense_solve(const int* __restrict__ off
     const CT* row = inv + ioff[k] + static_cast<int64_t>(r) * n;
     const VT* b = rhs + off[k];
     double acc = 0.0;
-    for (int c = threadIdx.x; c < n; c += 256) acc += double(row[c]) * double(to_f(b[c]));
+    // eight columns per lane and round, loaded before they are accumulated in column order
+    for (int c0 = threadIdx.x; c0 < n; c0 += 8 * 256) {
+        double w[8], v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int c = c0 + 256 * j;
+            w[j] = c < n ? double(row[c]) : 0.0;
+            v[j] = c < n ? double(to_f(b[c])) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (c0 + 256 * j < n) acc += w[j] * v[j];
+    }
     acc = block_sum<256>(acc);
     if (threadIdx.x == 0) x[off[k] + r] = from_f<VT>(static_cast<typename Arith<VT>::type>(acc));
 }

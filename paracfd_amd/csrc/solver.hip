@@ -191,7 +191,7 @@ private:
     }
     // reduction over brick-indexed partials, one finalize workgroup per instance
     void finalize(int op, int nparts) {
-        LAUNCH(k_finalize, ninst_, 1024, op, partials_, nparts, seg_, S_, agg_, opt_.cfl,
+        LAUNCH(k_finalize, ninst_, 1024, op, partials_, nparts, ninst_ > 1 ? seg_ : nullptr, S_, agg_, opt_.cfl,
                std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations, rprev_ ? 1 : 0);
     }
     // dense coarse solves of the instances whose coarsest multigrid level is m
@@ -526,6 +526,12 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
             c.parent = m + 1 < T.mg.size() ? upload(G.parent) : nullptr;
             c.child_off = upload(G.child_off);
             c.child = upload(G.child);
+            {
+                std::vector<int> first(G.n, -1);
+                for (int cc = 0; cc < G.n; ++cc)
+                    if (G.child_off[cc + 1] > G.child_off[cc]) first[cc] = G.child[G.child_off[cc]];
+                c.child_first = upload(first);
+            }
             // SELL-64 copy: per slice of 64 rows, slot-major, as wide as its longest row
             const int nsl = (G.n + 63) / 64;
             std::vector<int> sl_off(nsl + 1, 0);
