@@ -281,7 +281,8 @@ pcfd_status pcfd_set_reference_instance(pcfd_solver* solver, int instance, doubl
 typedef struct pcfd_tuning {
     int xcd_chunk_log2;     /* bricks go to the 8 XCDs in chunks of 2^k (-1: one range per XCD) */
     int coarse_block;       /* workgroup size of the coarse-level kernels: 64, 128, 256 or 512 */
-    int fused_vcycle;       /* 1: fused composite-level V-cycle kernels, 0: the plain sequence */
+    int fused_vcycle;       /* 1: fused V-cycle kernels (composite level; residual + restriction on
+                               coarse levels), 0: the plain sequence */
     int graph_replay;       /* 1: PCG iterations replayed from a captured HIP graph, 0: launched one
                                by one (not searched by pcfd_autotune) */
     int candidates;         /* configurations timed by the last pcfd_autotune */

@@ -677,6 +677,36 @@ __global__ void k_restrict_range(CsrView Cc, const VT* __restrict__ fine, VT* __
     if (c < Cc.n) restrict_range_row<VT>(Cc, c, fine, coarse);
 }
 
+// residual of level m fused with its restriction to level m+1: a workgroup takes
+// kRestrictGroup consecutive rows of level m+1, whose children form one window of at most 256
+// consecutive rows of level m (checked on the host); each lane computes one row's residual
+// (rounded to VT, as stored by the unfused pair), then each coarse row sums its children in
+// order from LDS.  Rows of the window without a parent in the group are computed and unused.
+constexpr int kRestrictGroup = 32;
+template <class VT, class CT>
+__global__ __launch_bounds__(256) void k_ell_residual_restrict(CsrView C, CsrView NC, const VT* __restrict__ x,
+                                                               const VT* __restrict__ rhs, VT* __restrict__ coarse) {
+    using A = typename Arith<VT>::type;
+    __shared__ A res[256];
+    const int c0 = blockIdx.x * kRestrictGroup, c1 = min(c0 + kRestrictGroup, NC.n);
+    const int f0 = NC.child_first[c0];
+    const int fend = NC.child_first[c1 - 1] + (NC.child_off[c1] - NC.child_off[c1 - 1]);
+    const int t = threadIdx.x, r = f0 + t;
+    if (r < fend) {
+        const A xr = A(to_f(x[r]));
+        const A Ax = ell_row<A, CT>(C, r, [&](int j) -> A { return A(to_f(x[j])); }, xr);
+        res[t] = round_vt<VT>(A(to_f(rhs[r])) - Ax);
+    }
+    __syncthreads();
+    if (t < c1 - c0) {
+        const int c = c0 + t;
+        const int n = NC.child_off[c + 1] - NC.child_off[c], k = NC.child_first[c] - f0;
+        A s = A(0);
+        for (int j = 0; j < n; ++j) s += res[k + j];
+        coarse[c] = from_f<VT>(s);
+    }
+}
+
 template <class VT, class CT>
 __global__ void k_ell_post_prolong(CsrView C, const VT* __restrict__ x, const VT* __restrict__ e, const VT* __restrict__ rhs,
                                    VT* __restrict__ xout, double omega) {

@@ -305,6 +305,10 @@ private:
         CsrView csr64{}, csr32{};
         const int* parent = nullptr;  // parent of this level's cells
         void* B = nullptr; void* X = nullptr; void* T = nullptr;
+        // the residual of this level and its restriction to the next run as one kernel
+        // (each group of kRestrictGroup next-level rows has its children in one window of at
+        // most 256 rows of this level)
+        bool fused_restrict = false;
     };
     std::vector<Level> lv_;
     const int* parent0_ = nullptr;
@@ -570,6 +574,24 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
             L.parent = c.parent;
             L.B = alloc<double>(G.n); L.X = alloc<double>(G.n); L.T = alloc<double>(G.n);
         }
+        for (size_t m = 1; m + 1 < T.mg.size(); ++m) {
+            const MGLevel& N = T.mg[m + 1];
+            bool ok = N.n > 0;
+            for (int c0 = 0; ok && c0 < N.n; c0 += kRestrictGroup) {
+                const int c1 = std::min(c0 + kRestrictGroup, N.n);
+                int prev_end = -1, first = -1;
+                for (int c = c0; ok && c < c1; ++c) {
+                    const int k0 = N.child_off[c], k1 = N.child_off[c + 1];
+                    if (k1 <= k0) { ok = false; break; }
+                    const int f = N.child[k0];
+                    if (c == c0) first = f;
+                    if (f < prev_end) ok = false;
+                    prev_end = f + (k1 - k0);
+                }
+                if (ok && prev_end - first > 256) ok = false;
+            }
+            lv_[m].fused_restrict = ok;
+        }
         // dense coarse blocks per level (the coarsest level of each instance)
         dense_.assign(T.mg.size(), DenseLevel{});
         if (!T.dense_inv.empty()) {
@@ -776,8 +798,12 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     }
     Level& N = lv_[m + 1];
     const CsrView& NC = std::is_same<CT, double>::value ? N.csr64 : N.csr32;
-    LAUNCH((k_ell_residual<VT, CT>), g, cb, C, cur, B, oth);
-    LAUNCH((k_restrict_range<VT>), blocks_for(N.n, cb), cb, NC, oth, static_cast<VT*>(N.B));
+    if (L.fused_restrict && fused_) {
+        LAUNCH((k_ell_residual_restrict<VT, CT>), blocks_for(N.n, kRestrictGroup), 256, C, NC, cur, B, static_cast<VT*>(N.B));
+    } else {
+        LAUNCH((k_ell_residual<VT, CT>), g, cb, C, cur, B, oth);
+        LAUNCH((k_restrict_range<VT>), blocks_for(N.n, cb), cb, NC, oth, static_cast<VT*>(N.B));
+    }
     VT* xc = coarse_cycle_t<VT, CT>(m + 1);
     LAUNCH((k_ell_post_prolong<VT, CT>), g, cb, C, cur, xc, B, oth, w);
     std::swap(cur, oth);
