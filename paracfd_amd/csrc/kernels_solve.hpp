@@ -378,7 +378,7 @@ __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restr
 
 // two pre-smoothing sweeps from x = 0:  x1 = w r / D,  x2 = x1 + w (r - A x1) / D
 template <class VT>
-__global__ __launch_bounds__(512) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
+__global__ __launch_bounds__(512, 8) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
                                                  const typename Arith<VT>::type* __restrict__ diag, VT* __restrict__ xout,
                                                  double omega, const DevScalars* S) {
     using A = typename Arith<VT>::type;
@@ -401,6 +401,46 @@ __global__ __launch_bounds__(512) void k_vc_pre2(MeshView M, const double* __res
         v = xa + w * (A(rb) - Ax) / dg;
     }
     xout[a] = from_f<VT>(v);
+}
+
+// The CG update fused into the first V-cycle pass: x += alpha d, r = rin - alpha Ad (written
+// to rout, never in place: neighbour bricks read rin for their halo) and the ||r||^2
+// partial, then the two pre-smoothing sweeps from zero on the new r, whose halo and
+// compact-neighbour values are formed on the fly with the same arithmetic.  The results are
+// those of k_cg_update followed by k_vc_pre2.
+template <class VT>
+__global__ __launch_bounds__(512, 8) void k_vc_update_pre2(MeshView M, double* __restrict__ x, const double* __restrict__ rin,
+                                                        double* __restrict__ rout, const double* __restrict__ d,
+                                                        const double* __restrict__ Ad,
+                                                        const typename Arith<VT>::type* __restrict__ diag,
+                                                        VT* __restrict__ xout, double omega, double* partials,
+                                                        const DevScalars* S) {
+    using A = typename Arith<VT>::type;
+    PCFD_BRICK_BEGIN;
+    const DevScalars& I = S[M.meta[b].inst];
+    const double alpha = I.alpha;
+    const A w = A(omega);
+    auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rin[c] - alpha * Ad[c]) / diag[c]); };
+    const TileFetch<A> tf = fetch_tile<A>(M, b, x1);
+    const double xi = x[a], di = d[a], ri = rin[a], adi = Ad[a];
+    const bool done = I.done != 0;
+    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const CompactFetch<A> cf = fetch_compact<A>(M, sp, x1);
+    if (done) return;
+    const double rn = ri - alpha * adi;
+    x[a] = xi + alpha * di;
+    rout[a] = rn;
+    __shared__ A tile[kTileN];
+    stage_tile<A>(tile, tf);
+    A v = A(0);
+    if (m & kActive) {
+        A dg;
+        const A xa = tf.own;
+        const A Ax = row0f<A>(M, tile, cf, x1, b, q, m, xa, dg);
+        v = xa + w * (A(rn) - Ax) / dg;
+    }
+    xout[a] = from_f<VT>(v);
+    store_partial(partials, b, block_sum<512>(rn * rn));
 }
 
 // residual r - A x summed over each 2x2x2 group into its level-1 parent (children of one

@@ -164,13 +164,28 @@ private:
         return F;
     }
     // One PCG iteration: d_new = z + beta d_old with A d_new fused, alpha, x/r update,
-    // preconditioner, beta.
+    // preconditioner, beta.  With the fused V-cycle the x/r update runs inside its first pass.
     void cg_iteration(bool initial, double* dold, double* dnew) {
         LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0);
         finalize(FIN_APPLY, nb_);
+        if (update_fused()) {
+            double* rin = r_;
+            r_ = r_ == rbuf_[0] ? rbuf_[1] : rbuf_[0];
+            if (flexible_) rprev_ = rin;
+            precondition_dispatch(false, dnew, rin);
+            return;
+        }
         LAUNCH(k_cg_update, nb_, 512, M_, p_, r_, rprev_, dnew, Ad_, partials_, S_);
         finalize(FIN_UPDATE, nb_);
         precondition_dispatch(false);
+    }
+    bool update_fused() const {
+        return opt_.preconditioner != PCFD_PC_JACOBI && opt_.pre_smooth == 2 && lv_.size() > 1 && fused_;
+    }
+    // residual roles at the start of a PCG solve
+    void reset_residual_roles() {
+        r_ = rbuf_[0];
+        if (flexible_) rprev_ = rbuf_[1];
     }
     // `chunk_` (even) iterations alternating the two direction buffers
     void enqueue_chunk() {
@@ -259,10 +274,12 @@ private:
     void read_loads(int i, pcfd_loads* out) const;
     void reconstruct();
     void run_projection(bool warm, pcfd_step_stats* st);
-    template <class VT> void precondition(bool initial);
+    template <class VT> void precondition(bool initial, const double* upd_d, const double* upd_rin);
     template <class VT> VT* coarse_cycle(int m);
     template <class VT, class CT> VT* coarse_cycle_t(int m);
-    void precondition_dispatch(bool initial);
+    // upd_d != nullptr: the CG update x += alpha d, r = upd_rin - alpha Ad (into r_) runs in the
+    // first V-cycle pass, followed by the FIN_UPDATE reduction
+    void precondition_dispatch(bool initial, const double* upd_d = nullptr, const double* upd_rin = nullptr);
 
     pcfd_options opt_;
     MeshView M_{};
@@ -284,6 +301,12 @@ private:
     double* d_ = nullptr;
     double* Ad_ = nullptr;
     double* rprev_ = nullptr;
+    // The residual lives in two buffers: with the CG update fused into the first V-cycle pass
+    // each iteration reads r from one and writes the new r to the other (no in-place update
+    // that a neighbour brick's halo read could see), and for flexible CG the old one is r_prev.
+    // r_ / rprev_ name the current roles at enqueue time; every PCG solve starts from rbuf_[0].
+    double* rbuf_[2] = {};
+    bool flexible_ = false;
     void* vc0_[2] = {};  // composite-level V-cycle buffers (8 bytes per dof)
     double* partials_ = nullptr;
     DevScalars* S_ = nullptr;     // ninst_ instance records, then the batch aggregate
@@ -438,10 +461,11 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     tuning_.coarse_block = coarse_block_;
     tuning_.fused_vcycle = fused_ ? 1 : 0;
     tuning_.graph_replay = use_graph_ ? 1 : 0;
-    if (opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED) {
-        rprev_ = alloc<double>(n_);
-        HIP_CHECK(hipMemset(rprev_, 0, n_ * sizeof(double)));
-    }
+    flexible_ = opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED;
+    rbuf_[0] = r_;
+    rbuf_[1] = alloc<double>(n_);
+    HIP_CHECK(hipMemset(rbuf_[1], 0, n_ * sizeof(double)));
+    if (flexible_) rprev_ = rbuf_[1];
     vc0_[0] = alloc<double>(n_);
     vc0_[1] = alloc<double>(n_);
     const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 6 * (M_.npatches / 256 + 1)});
@@ -823,7 +847,8 @@ VT* SolverImpl<R>::coarse_cycle(int m) {
 
 template <class R>
 template <class VT>
-void SolverImpl<R>::precondition(bool initial) {
+void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double* upd_rin) {
+    if (upd_d && !update_fused()) throw std::logic_error("fused CG update without the fused V-cycle");
     const int ini = initial ? 1 : 0;
     const int fin = ini ? FIN_RZ_INITIAL : FIN_RZ;
     if (opt_.preconditioner == PCFD_PC_JACOBI) {
@@ -838,7 +863,13 @@ void SolverImpl<R>::precondition(bool initial) {
     if (opt_.pre_smooth == 2 && lv_.size() > 1 && fused_) {
         // fused composite level: 2 pre-sweeps | residual + restriction | prolongation + sweep
         using A = typename Arith<VT>::type;
-        LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w, S_);
+        if (upd_d) {
+            LAUNCH(k_vc_update_pre2<VT>, nb_, 512, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur, w,
+                   partials_, S_);
+            finalize(FIN_UPDATE, nb_);
+        } else {
+            LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w, S_);
+        }
         LAUNCH(k_vc_restrict0<VT>, nb_, 512, M_, cur, r_, parent0_, static_cast<VT*>(lv_[1].B), S_);
         VT* xc = coarse_cycle<VT>(1);
         const int last = opt_.post_smooth == 1 ? 1 : 0;
@@ -873,11 +904,11 @@ void SolverImpl<R>::precondition(bool initial) {
 }
 
 template <class R>
-void SolverImpl<R>::precondition_dispatch(bool initial) {
+void SolverImpl<R>::precondition_dispatch(bool initial, const double* upd_d, const double* upd_rin) {
     switch (opt_.vcycle_precision) {
-        case PCFD_FP32: precondition<float>(initial); break;
-        case PCFD_BF16: precondition<bf16>(initial); break;
-        default: precondition<double>(initial); break;
+        case PCFD_FP32: precondition<float>(initial, upd_d, upd_rin); break;
+        case PCFD_BF16: precondition<bf16>(initial, upd_d, upd_rin); break;
+        default: precondition<double>(initial, upd_d, upd_rin); break;
     }
 }
 
@@ -889,6 +920,7 @@ void SolverImpl<R>::precondition_dispatch(bool initial) {
 // halved over the last three checks (or is not finite); true otherwise.
 template <class R>
 bool SolverImpl<R>::pcg_loop(bool watch_stall) {
+    reset_residual_roles();
     LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
     finalize(FIN_RESID, nb_);
     precondition_dispatch(true);
@@ -1262,7 +1294,7 @@ void SolverImpl<R>::mg_primitive(int op, int level, const double* x, const doubl
         result = cur;
     } else if (op == 3) {
         if (level != 0) throw std::invalid_argument("V-cycle primitive runs on level 0");
-        precondition<double>(true);
+        precondition<double>(true, nullptr, nullptr);
         result = z_;
     } else {
         throw std::invalid_argument("unknown multigrid primitive");
@@ -1482,6 +1514,7 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
         finalize(FIN_BNORM, nb_);
         // warm start from the current pressure, as the step's projection does
         set_nn_off(true);
+        reset_residual_roles();
         LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
         finalize(FIN_RESID, nb_);
         precondition_dispatch(true);
