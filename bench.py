@@ -107,10 +107,17 @@ def run(topo, spec, args, name, field, vcycle, pc):
             f"{train['train_ms']:.0f} ms")
     # every solver runs the same steps; for the learned one they are the validation that
     # alternates learned and plain projections (6 steps)
+    trial = []
     for i in range(6):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
         st = core.step()
         core.pressure_loads()
+        torch.cuda.synchronize()
+        trial.append(f"{'learned' if st['learned_used'] else 'plain'} {st['iterations']} its "
+                     f"{1e3 * (time.perf_counter() - t):.1f} ms")
     if pc == L.PC_LEARNED:
+        log(f"{name}: validation steps: " + "; ".join(trial))
         log(f"{name}: network {'rejected' if st['learned_rejected'] else 'kept'} by the validation")
     tune = core.autotune(repetitions=2) if args.autotune else core.tuning()
     log(f"{name}: launch shapes {tune}")
