@@ -191,7 +191,7 @@ def run_sweep(args):
     per_inst = [batch.instance_stats(i) for i in range(len(parts))]
     value = n_total * args.steps / tb
     log(f"batch: {value:.4g} cell-steps/s, {1e3 * tb / args.steps:.1f} ms/step, mean its {np.mean(its_b):.1f}")
-    ms_apply, by_apply = batch.time_kernel(0, reps=20)
+    ms_apply, by_apply = batch.time_kernel(6, reps=20)   # the dominant kernel of a PCG iteration
     batch.close()
 
     seq_s, seq_loads, seq_its = 0.0, [], []
@@ -222,7 +222,7 @@ def run_sweep(args):
                    "instances_detail": [{"alpha": a, "cells": p.n_active, "cl": l["cl"], "cd": l["cd"],
                                          "iterations": s["iterations"], "dt": s["dt"], "mean_its_alone": si}
                                         for a, p, l, s, si in zip(spec["alphas"], parts, loads_b, per_inst, seq_its)]},
-        "roofline": {"bound": "hbm", "kernel": "k_cg_apply_dot", "achieved": achieved, "peak": PEAK_HBM,
+        "roofline": {"bound": "hbm", "kernel": "k_vc_update_pre2<float>", "achieved": achieved, "peak": PEAK_HBM,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None},
         "cpu_baseline": None,
     }
@@ -317,8 +317,11 @@ def main():
     # roofline: per-launch time of each hot kernel on the learned solver's stream (HIP events)
     core = learned["core"]
     mean_its = learned["mean_its"]
-    per_step = {0: mean_its, 1: (mean_its + 1) * 4, 2: 1.0, 4: mean_its + 1}
-    names = {0: "k_cg_apply_dot", 1: "k_smooth0<float>", 2: "k_advect<float>", 4: "k_nn_apply"}
+    # launches per step: one direction update and one fused x/r update + pre-smoothing per PCG
+    # iteration, one last post-sweep (k_smooth0) per V-cycle, one network correction per V-cycle
+    per_step = {5: mean_its, 6: mean_its, 1: mean_its, 2: 1.0, 4: mean_its + 1}
+    names = {5: "k_cg_dir_apply", 6: "k_vc_update_pre2<float>", 1: "k_smooth0<float>", 2: "k_advect<float>",
+             4: "k_nn_apply"}
     timings = {}
     for k in per_step:
         ms, by = core.time_kernel(k, reps=20)

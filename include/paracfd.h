@@ -321,7 +321,10 @@ pcfd_status pcfd_mg_primitive(pcfd_solver* solver, int op, int level, const doub
  * average duration of one launch of a hot kernel on the solver's stream, timed with HIP
  * events over `reps` launches, and its algorithmic HBM bytes per launch.
  * kernel: 0 composite operator apply + dot (CG), 1 composite-level V-cycle sweep,
- *         2 MUSCL transport, 3 one full V-cycle (bytes reported as 0).
+ *         2 MUSCL transport, 3 one full V-cycle (bytes reported as 0), 4 learned correction
+ *         (learned solvers only), 5 CG direction update + operator apply + dot
+ *         (k_cg_dir_apply), 6 CG x/r update + two pre-smoothing sweeps (k_vc_update_pre2; its
+ *         x is a scratch vector, so the state is unchanged).
  * Leaves the Krylov loop flags in the "finished" state; call between steps. */
 pcfd_status pcfd_time_kernel(pcfd_solver* solver, int kernel, int reps, double* avg_ms, double* bytes_per_launch);
 

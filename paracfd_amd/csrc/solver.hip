@@ -1337,6 +1337,18 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
             case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); break;
             case 3: precondition_dispatch(true); break;
             case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
+            case 5: LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;
+            case 6:  // alpha = 0 and x = z (scratch): the pressure and the residual are untouched
+                if (opt_.vcycle_precision == PCFD_FP32)
+                    LAUNCH(k_vc_update_pre2<float>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
+                           static_cast<const float*>(diag_), static_cast<float*>(vc0_[0]), 0.6, partials_, S_);
+                else if (opt_.vcycle_precision == PCFD_BF16)
+                    LAUNCH(k_vc_update_pre2<bf16>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
+                           static_cast<const float*>(diag_), static_cast<bf16*>(vc0_[0]), 0.6, partials_, S_);
+                else
+                    LAUNCH(k_vc_update_pre2<double>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
+                           static_cast<const double*>(diag_), static_cast<double*>(vc0_[0]), 0.6, partials_, S_);
+                break;
             default: throw std::invalid_argument("unknown kernel id");
         }
     };
@@ -1361,6 +1373,8 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         case 2: per = 9.0 * sizeof(R) + 1; break;             // v in (3), faces (3), mask, v out (3)
         case 3: per = 0; break;
         case 4: per = 8 + 1 + 8 + 8 + 8; break;               // r, mask, z, r_prev, dz
+        case 5: per = 8 + 8 + 8 + 8 + 1; break;               // z, d_old in; d_new, Ad out; mask
+        case 6: per = 16 + 8 + 8 + 8 + 8 + 1 + 2.0 * vt; break;  // x in/out, d, r, Ad, r' out, mask, diag, x1 out
     }
     if (kernel == 4) {
         nact = nn_cells_;

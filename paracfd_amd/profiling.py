@@ -1,6 +1,8 @@
 """Summaries of rocprofv3 result databases (SQLite, `<name>_results.db`).
 
     python -m paracfd_amd.profiling trace DB            per-kernel calls, mean and total duration
+    python -m paracfd_amd.profiling last DB N TEXT      mean duration of the last N dispatches of each
+                                                        kernel whose name contains TEXT
     python -m paracfd_amd.profiling counters DB [TEXT]  per-kernel mean of each collected counter
                                                         (kernels whose name contains TEXT)
 """
@@ -27,6 +29,18 @@ def kernel_table(db: str) -> list[tuple[str, int, float, float]]:
     return sorted(rows, key=lambda r: -r[3])
 
 
+def last_dispatches(db: str, n: int, text: str) -> dict[str, tuple[float, int]]:
+    """kernel -> (mean us of its last n dispatches in start order, dispatches used)."""
+    con = sqlite3.connect(db)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    name_col = "kernel_name" if "kernel_name" in cols else "name"
+    agg = collections.defaultdict(list)
+    for name, start, end in con.execute(f"select {name_col}, start, end from kernels order by start"):
+        if text in name:
+            agg[name].append((end - start) / 1000.0)
+    return {k: (sum(v[-n:]) / len(v[-n:]), len(v[-n:])) for k, v in agg.items()}
+
+
 def counter_table(db: str, text: str = "") -> dict[str, dict[str, tuple[float, int]]]:
     """kernel -> counter -> (mean per dispatch, dispatches)."""
     con = sqlite3.connect(db)
@@ -38,9 +52,13 @@ def counter_table(db: str, text: str = "") -> dict[str, dict[str, tuple[float, i
 
 
 def main(argv: list[str]) -> int:
-    if len(argv) < 2 or argv[0] not in ("trace", "counters"):
+    if len(argv) < 2 or argv[0] not in ("trace", "counters", "last") or (argv[0] == "last" and len(argv) < 4):
         print(__doc__, file=sys.stderr)
         return 2
+    if argv[0] == "last":
+        for name, (mean, n) in last_dispatches(argv[1], int(argv[2]), argv[3]).items():
+            print(f"{_short(name):72s} last {n:4d} dispatches: {mean:9.2f} us")
+        return 0
     if argv[0] == "trace":
         rows = kernel_table(argv[1])
         total = sum(r[3] for r in rows) or 1.0

@@ -70,3 +70,24 @@ def test_autotune_keeps_the_solution(topo):
     assert np.array_equal(state(tuned), state(plain))
     with pytest.raises(RuntimeError):
         tuned.set_tuning(coarse_block=100)
+
+
+@pytest.mark.parametrize("field,vcycle", [(L.FP64, L.FP64), (L.FP32, L.FP32)])
+def test_kernel_timing_hooks_leave_the_state(topo, field, vcycle):
+    """The measurement launches (bench roofline) touch only scratch vectors: the state is unchanged
+    and the next steps match a solver that was never measured, bit for bit."""
+    from paracfd_amd import ExternalAeroCore
+    print(f"[tuning] timing hooks field={field} vcycle={vcycle}", flush=True)
+    timed, plain = (ExternalAeroCore(topo, options(field, vcycle)) for _ in range(2))
+    for c in (timed, plain):
+        c.initialize()
+        c.step()
+    before = state(timed)
+    for k in (0, 1, 2, 3, 5, 6):
+        ms, by = timed.time_kernel(k, reps=2)
+        assert ms > 0 and (by > 0 or k == 3)
+    assert np.array_equal(state(timed), before)
+    for _ in range(2):
+        a, b = timed.step(), plain.step()
+        assert a["iterations"] == b["iterations"]
+    assert np.array_equal(state(timed), state(plain))
