@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=16)
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--vcycle", choices=("fp32", "bf16"), default="fp32",
+                    help="V-cycle storage of the learned and the plain mixed-precision solvers")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -299,8 +301,9 @@ def main():
     log(f"topology {args.case}: {topo.n_active} active cells, {topo.info.bricks} bricks, "
         f"{topo.mg_levels} MG levels, built in {time.perf_counter() - t0:.1f}s")
 
-    learned = run(topo, spec, args, "learned", L.FP32, L.FP32, L.PC_LEARNED)
-    plain32 = run(topo, spec, args, "plain", L.FP32, L.FP32, L.PC_MG)
+    vc = L.BF16 if args.vcycle == "bf16" else L.FP32
+    learned = run(topo, spec, args, "learned", L.FP32, vc, L.PC_LEARNED)
+    plain32 = run(topo, spec, args, "plain", L.FP32, vc, L.PC_MG)
     plain64 = run(topo, spec, args, "plain_fp64_vcycle", L.FP32, L.FP64, L.PC_MG)
     ref = run(topo, spec, args, "fp64_everywhere", L.FP64, L.FP64, L.PC_MG)
 
@@ -320,7 +323,8 @@ def main():
     # launches per step: one direction update and one fused x/r update + pre-smoothing per PCG
     # iteration, one last post-sweep (k_smooth0) per V-cycle, one network correction per V-cycle
     per_step = {5: mean_its, 6: mean_its, 1: mean_its, 2: 1.0, 4: mean_its + 1}
-    names = {5: "k_cg_dir_apply", 6: "k_vc_update_pre2<float>", 1: "k_smooth0<float>", 2: "k_advect<float>",
+    vtn = "float" if args.vcycle == "fp32" else "bf16"
+    names = {5: "k_cg_dir_apply", 6: f"k_vc_update_pre2<{vtn}>", 1: f"k_smooth0<{vtn}>", 2: "k_advect<float>",
              4: "k_nn_apply"}
     timings = {}
     for k in per_step:
@@ -359,7 +363,7 @@ def main():
         "metric": METRIC, "value": learned["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": learned["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 fields + f32 V-cycle + bf16 network + f64 PCG",
+        "dtype": f"f32 fields + {args.vcycle} V-cycle + bf16 network + f64 PCG",
         "data": "synthetic (analytic geometry, freestream start)",
         "config": {"workload": args.case, "active_cells": n, "amr_levels": topo.info.levels,
                    "preconditioner": "V-cycle + learned 4-layer 8-channel conv correction, flexible PCG",

@@ -1374,7 +1374,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         case 3: per = 0; break;
         case 4: per = 8 + 1 + 8 + 8 + 8; break;               // r, mask, z, r_prev, dz
         case 5: per = 8 + 8 + 8 + 8 + 1; break;               // z, d_old in; d_new, Ad out; mask
-        case 6: per = 16 + 8 + 8 + 8 + 8 + 1 + 2.0 * vt; break;  // x in/out, d, r, Ad, r' out, mask, diag, x1 out
+        case 6: per = 16 + 8 + 8 + 8 + 8 + 1 + (vt == 8 ? 8 : 4) + vt; break;  // x in/out, d, r, Ad, r' out, mask, diag, x1 out
     }
     if (kernel == 4) {
         nact = nn_cells_;
