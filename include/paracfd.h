@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PCFD_ABI_VERSION 1
+#define PCFD_ABI_VERSION 2
 
 typedef enum pcfd_status {
     PCFD_OK = 0,
@@ -172,6 +172,12 @@ typedef struct pcfd_options {
                                  and their face neighbours, 2 bricks holding cut cells and
                                  their face neighbours */
     int learned_capacity;     /* PCFD_PC_LEARNED: recorded residuals kept for training */
+    int smooth_wall;          /* 1: smooth-wall shear model on the surface patches after the
+                                 diffusion of every step (Spalding's law, implicit per wall cell;
+                                 reference: ExternalAeroExecutionOptions::smooth_fabric_wall,
+                                 external_aero_core.h:112, on by default there;
+                                 DeviceCompositeCellMomentumTransport::apply_smooth_fabric_wall_model,
+                                 amr_advection.cu:1347-1365); 0 (default here): off */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {
@@ -197,6 +203,17 @@ typedef struct pcfd_loads {
     double drag, side, lift;  /* force x, y, z */
     double cd, cs, cl;        /* coefficients, valid when q*S_ref > 0 */
     int coefficients_valid;
+    /* wall shear of the smooth-wall model's last application (reference:
+     * accumulate_viscous_loads, aero_loads.cpp:49-56); zero and viscous_valid = 0 while the
+     * model is off or has not run */
+    int viscous_valid;
+    double viscous_force[3], viscous_moment[3];
+    double viscous_drag, viscous_side, viscous_lift;
+    double cd_viscous, cs_viscous, cl_viscous;
+    /* pressure + viscous (reference: ExternalAeroCore::aerodynamic_loads) */
+    double total_force[3], total_moment[3];
+    double total_drag, total_side, total_lift;
+    double total_cd, total_cs, total_cl;
 } pcfd_loads;
 
 /* Last error message of the calling thread (reference: exception what()). */

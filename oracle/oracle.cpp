@@ -12,7 +12,10 @@
 //   pcg               solve_orthogonal            amr_pressure.cu:1267-1353
 //   correct           structured/special_flux_correction                     amr_pressure.cu:285-330
 //   impulse           pressure_*_cell_impulse     amr_advection.cu:380-405, CPU twin :979-1053
-//   loads             compute_pressure_loads      aero_loads.cpp:35-47
+//   smooth_wall       apply_composite_cell_smooth_fabric_wall_cpu  amr_advection.cu:920-942,
+//                     friction velocity amr_pressure.h:41-53
+//   loads             compute_pressure_loads      aero_loads.cpp:35-47,
+//                     accumulate_viscous_loads    aero_loads.cpp:49-56
 // The V-cycle is this project's geometric multigrid (the reference uses aggregation AMG,
 // amr_pressure.cu:1103-1154); its primitives are pinned by the reference golden vectors.
 //
@@ -619,6 +622,81 @@ struct Oracle {
         for (int q = lsq_off[pch]; q < lsq_off[pch + 1]; ++q) pw += lsq_beta[q] * (p[lsq_dof[q]] - p[a]);
         return pw;
     }
+    // Smooth-wall law (Spalding, kappa 0.41, B 5.2) inverted for the friction velocity by 36
+    // bisections of u+ in [0, 80] with u+ y+(u+) = u_t y / nu (amr_pressure.h:41-53)
+    static double friction_velocity(double ut, double y, double nu) {
+        if (!(ut > 0.0) || !(y > 0.0) || !(nu > 0.0)) return 0.0;
+        const double re = ut * y / nu, kappa = 0.41, c = std::exp(-kappa * 5.2);
+        double lo = 0.0, hi = 80.0;
+        for (int it = 0; it < 36; ++it) {
+            const double up = 0.5 * (lo + hi), x = kappa * up, x2 = x * x;
+            const double rem = x < 0.5 ? x2 * x2 * (1.0 / 24.0 + x * (1.0 / 120.0 + x * (1.0 / 720.0)))
+                                       : std::exp(x) - 1.0 - x - 0.5 * x2 - x2 * x * (1.0 / 6.0);
+            if (up * (up + c * rem) < re) lo = up;
+            else hi = up;
+        }
+        return ut / std::max(0.5 * (lo + hi), 1e-12);
+    }
+    // wall shear, implicit per wall cell (amr_advection.cu:920-942): per patch k = A u_tau^2 /
+    // |v_t| from the current velocity, (I + dt/V sum k (I - n n^T)) v' = v by Cholesky, patch
+    // force rho k (v' - (v'.n) n).  Patches of a cell in patch order.
+    std::vector<double> wall_force;
+    bool wall_applied = false;
+    void smooth_wall(double dt) {
+        const size_t np = patch_dof.size();
+        wall_force.assign(3 * np, 0.0);
+        std::vector<int> order(np);
+        for (size_t q = 0; q < np; ++q) order[q] = static_cast<int>(q);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return patch_dof[x] < patch_dof[y]; });
+        std::vector<double> kp(np, 0.0);
+        for (size_t j0 = 0; j0 < np;) {
+            const int64_t a = patch_dof[order[j0]];
+            size_t j1 = j0;
+            while (j1 < np && patch_dof[order[j1]] == a) ++j1;
+            const double v[3] = {vx[a], vy[a], vz[a]};
+            double K[6] = {0, 0, 0, 0, 0, 0};
+            for (size_t j = j0; j < j1; ++j) {
+                const int q = order[j];
+                const double* S = &patch_S[3 * q];
+                const double area = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+                const double nn[3] = {S[0] / area, S[1] / area, S[2] / area};
+                double y = 0.0;
+                for (int c = 0; c < 3; ++c) y += (centroid[3 * a + c] - patch_x[3 * q + c]) * S[c] / area;
+                y = std::abs(y);
+                const double un = v[0] * nn[0] + v[1] * nn[1] + v[2] * nn[2];
+                const double t[3] = {v[0] - un * nn[0], v[1] - un * nn[1], v[2] - un * nn[2]};
+                const double speed = std::sqrt(std::max(0.0, t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+                double k = 0.0;
+                if (speed > 0.0) {
+                    const double ut = friction_velocity(speed, y, opt.nu);
+                    k = area * ut * ut / speed;
+                }
+                kp[q] = k;
+                K[0] += k * (1.0 - nn[0] * nn[0]); K[1] -= k * nn[0] * nn[1]; K[2] -= k * nn[0] * nn[2];
+                K[3] += k * (1.0 - nn[1] * nn[1]); K[4] -= k * nn[1] * nn[2]; K[5] += k * (1.0 - nn[2] * nn[2]);
+            }
+            const double s = dt / vol(a);
+            const double a00 = 1.0 + s * K[0], a01 = s * K[1], a02 = s * K[2], a11 = 1.0 + s * K[3], a12 = s * K[4],
+                         a22 = 1.0 + s * K[5];
+            const double l00 = std::sqrt(a00), l10 = a01 / l00, l20 = a02 / l00;
+            const double l11 = std::sqrt(a11 - l10 * l10), l21 = (a12 - l20 * l10) / l11;
+            const double l22 = std::sqrt(a22 - l20 * l20 - l21 * l21);
+            const double y0 = v[0] / l00, y1 = (v[1] - l10 * y0) / l11, y2 = (v[2] - l20 * y0 - l21 * y1) / l22;
+            const double x2 = y2 / l22, x1 = (y1 - l21 * x2) / l11, x0 = (y0 - l10 * x1 - l20 * x2) / l00;
+            vx[a] = x0; vy[a] = x1; vz[a] = x2;
+            for (size_t j = j0; j < j1; ++j) {
+                const int q = order[j];
+                const double* S = &patch_S[3 * q];
+                const double area = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+                const double nn[3] = {S[0] / area, S[1] / area, S[2] / area};
+                const double un = x0 * nn[0] + x1 * nn[1] + x2 * nn[2];
+                const double xv[3] = {x0, x1, x2};
+                for (int c = 0; c < 3; ++c) wall_force[3 * q + c] = opt.rho * kp[q] * (xv[c] - un * nn[c]);
+            }
+            j0 = j1;
+        }
+        wall_applied = true;
+    }
     void impulse(double dt) {
         const double s = dt / opt.rho;
         std::vector<double> d[3] = {std::vector<double>(n, 0.0), std::vector<double>(n, 0.0), std::vector<double>(n, 0.0)};
@@ -661,19 +739,35 @@ struct Oracle {
         std::memset(out, 0, sizeof(*out));
         for (size_t pch = 0; pch < patch_dof.size(); ++pch) {
             const double pw = wall_pressure(static_cast<int>(pch)) - pref;
-            double F[3];
-            for (int c = 0; c < 3; ++c) { F[c] = pw * patch_S[3 * pch + c]; out->force[c] += F[c]; }
             double r[3];
             for (int c = 0; c < 3; ++c) r[c] = patch_x[3 * pch + c] - origin[c];
-            out->moment[0] += r[1] * F[2] - r[2] * F[1];
-            out->moment[1] += r[2] * F[0] - r[0] * F[2];
-            out->moment[2] += r[0] * F[1] - r[1] * F[0];
+            for (int part = 0; part < 2; ++part) {
+                double F[3];
+                for (int c = 0; c < 3; ++c)
+                    F[c] = part == 0 ? pw * patch_S[3 * pch + c] : (wall_force.empty() ? 0.0 : wall_force[3 * pch + c]);
+                double* f = part == 0 ? out->force : out->viscous_force;
+                double* m = part == 0 ? out->moment : out->viscous_moment;
+                for (int c = 0; c < 3; ++c) f[c] += F[c];
+                m[0] += r[1] * F[2] - r[2] * F[1];
+                m[1] += r[2] * F[0] - r[0] * F[2];
+                m[2] += r[0] * F[1] - r[1] * F[0];
+            }
+        }
+        out->viscous_valid = wall_applied ? 1 : 0;
+        for (int c = 0; c < 3; ++c) {
+            out->total_force[c] = out->force[c] + out->viscous_force[c];
+            out->total_moment[c] = out->moment[c] + out->viscous_moment[c];
         }
         out->drag = out->force[0]; out->side = out->force[1]; out->lift = out->force[2];
+        out->viscous_drag = out->viscous_force[0]; out->viscous_side = out->viscous_force[1]; out->viscous_lift = out->viscous_force[2];
+        out->total_drag = out->total_force[0]; out->total_side = out->total_force[1]; out->total_lift = out->total_force[2];
         const double q = 0.5 * opt.rho * opt.speed * opt.speed;
         if (q > 0 && ref_area > 0) {
+            const double qs = q * ref_area;
             out->coefficients_valid = 1;
-            out->cd = out->drag / (q * ref_area); out->cs = out->side / (q * ref_area); out->cl = out->lift / (q * ref_area);
+            out->cd = out->drag / qs; out->cs = out->side / qs; out->cl = out->lift / qs;
+            out->cd_viscous = out->viscous_drag / qs; out->cs_viscous = out->viscous_side / qs; out->cl_viscous = out->viscous_lift / qs;
+            out->total_cd = out->total_drag / qs; out->total_cs = out->total_side / qs; out->total_cl = out->total_lift / qs;
         }
     }
     double max_abs_velocity() const {
@@ -719,6 +813,7 @@ void* oracle_create(const oracle_topology* t, const pcfd_options* o) {
         s->max_diff_coef = t->max_diffusion_coefficient;
         if (t->centroid) s->centroid.assign(t->centroid, t->centroid + 3 * t->ndof);
         if (o->smagorinsky_cs > 0 && s->centroid.empty()) throw std::invalid_argument("the Smagorinsky model needs cell centroids");
+        if (o->smooth_wall && s->centroid.empty()) throw std::invalid_argument("the smooth-wall model needs cell centroids");
         s->mg.resize(t->mg_levels);
         for (int m = 0; m < t->mg_levels; ++m) {
             auto& L = s->mg[m];
@@ -790,6 +885,7 @@ int oracle_advect(void* h, double dt, double* dt_used, int* substeps) {
     s->advect(dt);
     const int subs = s->diffusion_substeps(dt);
     s->diffuse(dt);
+    if (s->opt.smooth_wall) s->smooth_wall(dt);
     s->reconstruct();
     if (dt_used) *dt_used = dt;
     if (substeps) *substeps = subs;
@@ -816,6 +912,7 @@ int oracle_step(void* h, pcfd_step_stats* st) {
     s->advect(dt);
     const int subs = s->diffusion_substeps(dt);
     s->diffuse(dt);
+    if (s->opt.smooth_wall) s->smooth_wall(dt);
     s->reconstruct();
     double md = 0;
     auto r = s->project(dt, true, &md);
@@ -826,6 +923,16 @@ int oracle_step(void* h, pcfd_step_stats* st) {
 }
 
 int oracle_loads(void* h, double pref, pcfd_loads* out) { static_cast<Oracle*>(h)->loads(pref, out); return PCFD_OK; }
+
+// the smooth-wall pieces on their own, for the tests that pin them
+double oracle_wall_friction(double ut, double y, double nu) { return Oracle::friction_velocity(ut, y, nu); }
+int oracle_smooth_wall(void* h, double dt, double* force) {
+    auto* s = static_cast<Oracle*>(h);
+    if (s->centroid.empty()) { g_error = "the smooth-wall model needs cell centroids"; return PCFD_EINVAL; }
+    s->smooth_wall(dt);
+    if (force) std::copy(s->wall_force.begin(), s->wall_force.end(), force);
+    return PCFD_OK;
+}
 
 int oracle_get_state(void* h, double* x, double* y, double* z, double* p) {
     auto* s = static_cast<Oracle*>(h);

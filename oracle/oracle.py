@@ -85,6 +85,7 @@ class Oracle:
             "oracle_mark_initialized": [P, C.c_double],
             "oracle_mg_primitive": [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int],
             "oracle_dense_operator": [P, dp], "oracle_divergence": [P, dp], "oracle_viscosity": [P, dp],
+            "oracle_smooth_wall": [P, C.c_double, dp],
         }.items():
             f = getattr(lib, name)
             f.restype = C.c_int
@@ -94,6 +95,8 @@ class Oracle:
         lib.oracle_outflow_rate.argtypes = [P]
         lib.oracle_time_steps.restype = C.c_double
         lib.oracle_time_steps.argtypes = [P, C.c_int]
+        lib.oracle_wall_friction.restype = C.c_double
+        lib.oracle_wall_friction.argtypes = [C.c_double, C.c_double, C.c_double]
 
         t = topo
         self.topo = topo
@@ -190,6 +193,19 @@ class Oracle:
         out = self.L.Loads()
         self.lib.oracle_loads(self.h, pref, C.byref(out))
         return out.as_dict()
+
+    def wall_friction(self, ut, y, nu):
+        """Friction velocity of the smooth-wall law for tangential speed ut at wall distance y."""
+        return self.lib.oracle_wall_friction(ut, y, nu)
+
+    def smooth_wall(self, dt):
+        """Apply only the smooth-wall model to the current velocity; returns the patch forces [npatch, 3]."""
+        npatch = int(self.topo.info.patches)
+        f = np.zeros(3 * max(npatch, 1))
+        rc = self.lib.oracle_smooth_wall(self.h, dt, f.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return f[:3 * npatch].reshape(npatch, 3)
 
     def state(self):
         x, y, z, p = (np.empty(self.n) for _ in range(4))

@@ -63,7 +63,8 @@ class Options(C.Structure):
                 ("smagorinsky_cs", C.c_double), ("rtol", C.c_double), ("max_iterations", C.c_int),
                 ("field_precision", C.c_int), ("vcycle_precision", C.c_int), ("preconditioner", C.c_int),
                 ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int),
-                ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int)]
+                ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int),
+                ("smooth_wall", C.c_int)]
 
 
 NN_C = 8
@@ -103,12 +104,18 @@ class StepStats(C.Structure):
 class Loads(C.Structure):
     _fields_ = [("force", C.c_double * 3), ("moment", C.c_double * 3), ("drag", C.c_double), ("side", C.c_double),
                 ("lift", C.c_double), ("cd", C.c_double), ("cs", C.c_double), ("cl", C.c_double),
-                ("coefficients_valid", C.c_int)]
+                ("coefficients_valid", C.c_int), ("viscous_valid", C.c_int),
+                ("viscous_force", C.c_double * 3), ("viscous_moment", C.c_double * 3),
+                ("viscous_drag", C.c_double), ("viscous_side", C.c_double), ("viscous_lift", C.c_double),
+                ("cd_viscous", C.c_double), ("cs_viscous", C.c_double), ("cl_viscous", C.c_double),
+                ("total_force", C.c_double * 3), ("total_moment", C.c_double * 3),
+                ("total_drag", C.c_double), ("total_side", C.c_double), ("total_lift", C.c_double),
+                ("total_cd", C.c_double), ("total_cs", C.c_double), ("total_cl", C.c_double)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
-        d["force"] = list(self.force)
-        d["moment"] = list(self.moment)
+        for name in ("force", "moment", "viscous_force", "viscous_moment", "total_force", "total_moment"):
+            d[name] = list(getattr(self, name))
         return d
 
 

@@ -25,7 +25,7 @@ struct DevScalars {
     double rz, rz_prev, pAp, alpha, beta, rr, bnorm, relres;
     double dt, rate, maxdiv, vmax, diff_rate;
     double dts;       // diffusion substep of this instance
-    double loads[6];
+    double loads[12];  // pressure force, moment; wall-shear force, moment
     int iter, done, converged, breakdown, zero_rhs, subs;
     unsigned int ticket[16];
 };

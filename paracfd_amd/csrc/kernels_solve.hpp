@@ -194,7 +194,7 @@ enum FinalizeOp : int {
     FIN_UPDATE = 6,   // ||r||, iteration count, convergence / iteration limit
     FIN_RZ = 7,       // r.z (and r_prev.z) -> beta, rz
     FIN_RZ_INITIAL = 8,
-    FIN_LOADS = 9,    // six load sums
+    FIN_LOADS = 9,    // twelve load sums (pressure and wall-shear force and moment)
     FIN_DIFFRATE = 10 // max explicit diffusion rate (Smagorinsky substeps)
 };
 
@@ -208,8 +208,8 @@ __device__ __forceinline__ int finalize_one(int op, const double* __restrict__ p
     const double rz = S->rz, bnorm = S->bnorm;
     const int iter = S->iter;
     const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE;
-    const int arrays = op == FIN_LOADS ? 6 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
-    double t[6] = {0, 0, 0, 0, 0, 0};
+    const int arrays = op == FIN_LOADS ? 12 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
+    double t[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts + beg, end - beg, take_max);
     if (threadIdx.x != 0 || (krylov && done)) return done;
     switch (op) {
@@ -254,7 +254,7 @@ __device__ __forceinline__ int finalize_one(int op, const double* __restrict__ p
             break;
         case FIN_RZ_INITIAL: S->rz = t[0]; break;
         case FIN_LOADS:
-            for (int c = 0; c < 6; ++c) S->loads[c] = t[c];
+            for (int c = 0; c < 12; ++c) S->loads[c] = t[c];
             break;
         case FIN_DIFFRATE: S->diff_rate = t[0]; break;
         default: break;

@@ -506,20 +506,104 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
 // Surface-load integral (aero_loads.cpp:35-47) on the device in FP64: F = sum (p_w - pref) S,
 // M = sum (x - origin) x F; deterministic block partials.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p, double pref, double ox,
+// pressure force and moment of each patch, then the wall-shear force of the smooth-wall model
+// (wall_force, zero while the model is off) and its moment: 12 partial arrays
+__global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p,
+                                               const double* __restrict__ wall_force, double pref, double ox,
                                                double oy, double oz, int pbegin, int pend, double* partials) {
     const int pch = pbegin + blockIdx.x * blockDim.x + threadIdx.x;
-    double v[6] = {0, 0, 0, 0, 0, 0};
+    double v[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (pch < pend) {
         const double pw = wall_pressure(M, p, pch) - pref;
-        const double Fx = pw * M.patch_S[3 * pch], Fy = pw * M.patch_S[3 * pch + 1], Fz = pw * M.patch_S[3 * pch + 2];
         const double rx = M.patch_x[3 * pch] - ox, ry = M.patch_x[3 * pch + 1] - oy, rz = M.patch_x[3 * pch + 2] - oz;
-        v[0] = Fx; v[1] = Fy; v[2] = Fz;
-        v[3] = ry * Fz - rz * Fy; v[4] = rz * Fx - rx * Fz; v[5] = rx * Fy - ry * Fx;
+        for (int part = 0; part < 2; ++part) {
+            double F[3];
+            for (int c = 0; c < 3; ++c) F[c] = part == 0 ? pw * M.patch_S[3 * pch + c] : wall_force[3 * pch + c];
+            double* o = v + 6 * part;
+            o[0] = F[0]; o[1] = F[1]; o[2] = F[2];
+            o[3] = ry * F[2] - rz * F[1]; o[4] = rz * F[0] - rx * F[2]; o[5] = rx * F[1] - ry * F[0];
+        }
     }
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < 12; ++c) {
         const double part = block_sum<256>(v[c]);
         if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = part;
+    }
+}
+
+// Smooth-wall law (Spalding): y+ = u+ + e^(-kappa B) (e^(kappa u+) - 1 - kappa u+ - (kappa u+)^2/2
+// - (kappa u+)^3/6), kappa = 0.41, B = 5.2.  With Re_y = u_t y / nu = u+ y+ the friction
+// velocity is u_t / u+; u+ by 36 bisections of [0, 80] (amr_pressure.h:41-53).  Below
+// kappa u+ = 0.5 the bracket is its Taylor series x^4/24 + x^5/120 + x^6/720 (no cancellation).
+__host__ __device__ inline double smooth_wall_friction_velocity(double ut, double y, double nu) {
+    if (!(ut > 0.0) || !(y > 0.0) || !(nu > 0.0)) return 0.0;
+    const double re = ut * y / nu, kappa = 0.41, c = exp(-kappa * 5.2);
+    double lo = 0.0, hi = 80.0;
+    for (int it = 0; it < 36; ++it) {
+        const double up = 0.5 * (lo + hi), x = kappa * up, x2 = x * x;
+        const double rem = x < 0.5 ? x2 * x2 * (1.0 / 24.0 + x * (1.0 / 120.0 + x * (1.0 / 720.0)))
+                                   : exp(x) - 1.0 - x - 0.5 * x2 - x2 * x * (1.0 / 6.0);
+        const double yp = up + c * rem;
+        if (up * yp < re) lo = up;
+        else hi = up;
+    }
+    return ut / fmax(0.5 * (lo + hi), 1e-12);
+}
+
+// Wall shear of one wall cell (all its patches), implicit in the new velocity
+// (amr_advection.cu:421-437, CPU twin :920-942): per patch with unit normal n, area A and wall
+// distance y, the tangential speed |v_t| of the current velocity gives k = A u_tau^2 / |v_t|;
+// (I + dt/V sum k (I - n n^T)) v' = v is solved by a 3x3 Cholesky factorisation, and the
+// patch force is rho k (v' - (v'.n) n).  One lane per wall cell, patches in a fixed order, so
+// the result does not depend on scheduling (the reference accumulates with atomics).
+template <class R>
+__global__ __launch_bounds__(256) void k_smooth_wall(MeshView M, FieldsView<R> F, int nwall,
+                                                     const int* __restrict__ wall_cell, const int* __restrict__ wall_off,
+                                                     const int* __restrict__ wall_patch, const double* __restrict__ wall_vol,
+                                                     const double* __restrict__ wall_dist, double nu, double rho,
+                                                     double* __restrict__ wall_k, double* __restrict__ wall_force,
+                                                     const DevScalars* S) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwall) return;
+    const int a = wall_cell[w];
+    const double dt = S[M.meta[a >> 9].inst].dt;
+    const double v[3] = {double(F.v[0][a]), double(F.v[1][a]), double(F.v[2][a])};
+    double K[6] = {0, 0, 0, 0, 0, 0};  // symmetric: 00 01 02 11 12 22
+    for (int j = wall_off[w]; j < wall_off[w + 1]; ++j) {
+        const int pch = wall_patch[j];
+        const double Sx = M.patch_S[3 * pch], Sy = M.patch_S[3 * pch + 1], Sz = M.patch_S[3 * pch + 2];
+        const double area = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+        const double n[3] = {Sx / area, Sy / area, Sz / area};
+        const double un = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+        const double t[3] = {v[0] - un * n[0], v[1] - un * n[1], v[2] - un * n[2]};
+        const double speed = sqrt(fmax(0.0, t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+        double k = 0.0;
+        if (speed > 0.0) {
+            const double ut = smooth_wall_friction_velocity(speed, wall_dist[pch], nu);
+            k = area * ut * ut / speed;
+        }
+        wall_k[pch] = k;
+        K[0] += k * (1.0 - n[0] * n[0]); K[1] -= k * n[0] * n[1]; K[2] -= k * n[0] * n[2];
+        K[3] += k * (1.0 - n[1] * n[1]); K[4] -= k * n[1] * n[2]; K[5] += k * (1.0 - n[2] * n[2]);
+    }
+    const double s = dt / wall_vol[w];
+    const double a00 = 1.0 + s * K[0], a01 = s * K[1], a02 = s * K[2], a11 = 1.0 + s * K[3], a12 = s * K[4],
+                 a22 = 1.0 + s * K[5];
+    const double l00 = sqrt(a00), l10 = a01 / l00, l20 = a02 / l00;
+    const double l11 = sqrt(a11 - l10 * l10), l21 = (a12 - l20 * l10) / l11;
+    const double l22 = sqrt(a22 - l20 * l20 - l21 * l21);
+    const double y0 = v[0] / l00, y1 = (v[1] - l10 * y0) / l11, y2 = (v[2] - l20 * y0 - l21 * y1) / l22;
+    const double x2 = y2 / l22, x1 = (y1 - l21 * x2) / l11, x0 = (y0 - l10 * x1 - l20 * x2) / l00;
+    F.v[0][a] = R(x0); F.v[1][a] = R(x1); F.v[2][a] = R(x2);
+    // the force from the stored (storage-precision) velocity, as the reference reads it back
+    const double vn[3] = {double(R(x0)), double(R(x1)), double(R(x2))};
+    for (int j = wall_off[w]; j < wall_off[w + 1]; ++j) {
+        const int pch = wall_patch[j];
+        const double Sx = M.patch_S[3 * pch], Sy = M.patch_S[3 * pch + 1], Sz = M.patch_S[3 * pch + 2];
+        const double area = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+        const double n[3] = {Sx / area, Sy / area, Sz / area};
+        const double un = vn[0] * n[0] + vn[1] * n[1] + vn[2] * n[2];
+        const double k = wall_k[pch];
+        for (int c = 0; c < 3; ++c) wall_force[3 * pch + c] = rho * k * (vn[c] - un * n[c]);
     }
 }
 

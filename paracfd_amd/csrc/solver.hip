@@ -251,6 +251,7 @@ private:
     }
     void setup_learned(const Topology& T);
     void setup_smagorinsky(const Topology& T);
+    void setup_smooth_wall(const Topology& T);
     // Validation of a freshly trained network: the next 2 kTrial projections alternate learned
     // and plain MGPCG (learned first), so both see the same stage of the flow; if the learned
     // ones take longer in total, the network is switched off.
@@ -387,6 +388,19 @@ private:
     R* snap_f_[3] = {};
     R* snap_fe_ = nullptr;
     double* snap_p_ = nullptr;
+    double* snap_wall_force_ = nullptr;
+    // smooth-wall model: wall cells (cells holding surface patches), their patches (CSR over a
+    // patch permutation), volumes, per-patch wall distance; the coefficient and force of every
+    // patch from the last application (force zero until the model runs)
+    int nwall_ = 0;
+    int* wall_cell_ = nullptr;
+    int* wall_off_ = nullptr;
+    int* wall_patch_ = nullptr;
+    double* wall_vol_ = nullptr;
+    double* wall_dist_ = nullptr;
+    double* wall_k_ = nullptr;
+    double* wall_force_ = nullptr;
+    std::vector<uint8_t> wall_applied_, snap_wall_applied_;  // per instance
 };
 
 template <class R>
@@ -468,7 +482,7 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     if (flexible_) rprev_ = rbuf_[1];
     vc0_[0] = alloc<double>(n_);
     vc0_[1] = alloc<double>(n_);
-    const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 6 * (M_.npatches / 256 + 1)});
+    const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 12 * (M_.npatches / 256 + 1)});
     partials_ = alloc<double>(npart);
     // instances of a batch
     ninst_ = T.instances();
@@ -649,6 +663,7 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     }
     if (opt_.preconditioner == PCFD_PC_LEARNED) setup_learned(T);
     if (opt_.smagorinsky_cs > 0.0) setup_smagorinsky(T);
+    setup_smooth_wall(T);
     HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -773,6 +788,52 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
     nn_sb_ = alloc<float>(nn_nlist_);
     HIP_CHECK(hipMemset(nn_dz_, 0, slots * sizeof(double)));
     HIP_CHECK(hipMemset(nn_dfull_, 0, n_ * sizeof(double)));
+}
+
+// Smooth-wall model: the wall cells in dof order with their patches, the cells' volumes and
+// each patch's wall distance |(x_cell - x_patch) . n| (amr_advection.cu:934).  The patch force
+// array exists (zero) with the model off, so the load reduction always reads it.
+template <class R>
+void SolverImpl<R>::setup_smooth_wall(const Topology& T) {
+    const int np = static_cast<int>(T.patch_dof.size());
+    wall_force_ = alloc<double>(3 * static_cast<int64_t>(np));
+    HIP_CHECK(hipMemset(wall_force_, 0, std::max(1, 3 * np) * sizeof(double)));
+    wall_applied_.assign(ninst_, 0);
+    if (!opt_.smooth_wall || np == 0) return;
+    if (!(opt_.nu > 0.0)) throw std::invalid_argument("the smooth-wall model needs a positive molecular viscosity");
+    std::vector<int> order(np);
+    for (int q = 0; q < np; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return T.patch_dof[x] < T.patch_dof[y]; });
+    std::vector<int> special(n_, -1);
+    for (size_t sp = 0; sp < T.special_dof.size(); ++sp) special[T.special_dof[sp]] = static_cast<int>(sp);
+    std::vector<int> cells, off{0};
+    std::vector<double> vol, dist(np, 0.0);
+    for (int j = 0; j < np; ++j) {
+        const int q = order[j], a = T.patch_dof[q];
+        if (cells.empty() || cells.back() != a) {
+            if (!cells.empty()) off.push_back(j);
+            cells.push_back(a);
+            const double h = T.level_h.at(T.brick_level[a >> 9]);
+            vol.push_back(special[a] >= 0 ? T.special_volume[special[a]] : h * h * h);
+            if (!(vol.back() > 0.0)) throw std::invalid_argument("smooth-wall cell without volume");
+        }
+        double S2 = 0.0, d = 0.0;
+        for (int c = 0; c < 3; ++c) S2 += T.patch_S[3 * q + c] * T.patch_S[3 * q + c];
+        const double area = std::sqrt(S2);
+        if (!(area > 0.0)) throw std::invalid_argument("smooth-wall patch without area");
+        for (int c = 0; c < 3; ++c) d += (T.centroid[3 * static_cast<size_t>(a) + c] - T.patch_x[3 * q + c]) * T.patch_S[3 * q + c] / area;
+        dist[q] = std::abs(d);
+        if (!(dist[q] > std::max(1e-14, std::cbrt(vol.back()) * 1e-10)))
+            throw std::invalid_argument("smooth-wall patch at zero distance from its cell centroid");
+    }
+    off.push_back(np);
+    nwall_ = static_cast<int>(cells.size());
+    wall_cell_ = upload(cells);
+    wall_off_ = upload(off);
+    wall_patch_ = upload(order);
+    wall_vol_ = upload(vol);
+    wall_dist_ = upload(dist);
+    wall_k_ = alloc<double>(np);
 }
 
 template <class R>
@@ -1077,6 +1138,12 @@ void SolverImpl<R>::advect_core() {
         else LAUNCH(k_diffuse<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], S_, s);
         for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
     }
+    // wall shear after the diffusion, before the face reconstruction (external_aero_core.cpp:319)
+    if (nwall_) {
+        LAUNCH(k_smooth_wall<R>, blocks_for(nwall_, 256), 256, M_, fields(), nwall_, wall_cell_, wall_off_, wall_patch_,
+               wall_vol_, wall_dist_, opt_.nu, opt_.rho, wall_k_, wall_force_, S_);
+        std::fill(wall_applied_.begin(), wall_applied_.end(), 1);
+    }
     reconstruct();
 }
 
@@ -1153,7 +1220,7 @@ void SolverImpl<R>::enqueue_loads(int i, double pref) {
     if (p1 <= p0) return;
     const unsigned gl = blocks_for(p1 - p0, 256);
     const double* o = &inst_origin_[3 * static_cast<size_t>(i)];
-    LAUNCH(k_loads, gl, 256, M_, p_, pref, o[0], o[1], o[2], p0, p1, partials_);
+    LAUNCH(k_loads, gl, 256, M_, p_, wall_force_, pref, o[0], o[1], o[2], p0, p1, partials_);
     LAUNCH(k_finalize, 1, 1024, FIN_LOADS, partials_, static_cast<int>(gl), static_cast<const int*>(nullptr), S_ + i,
            static_cast<DevScalars*>(nullptr), opt_.cfl, 1.0, opt_.rtol, opt_.max_iterations, 0);
 }
@@ -1163,13 +1230,28 @@ template <class R>
 void SolverImpl<R>::read_loads(int i, pcfd_loads* out) const {
     std::memset(out, 0, sizeof(*out));
     if (inst_patch_off_[i + 1] > inst_patch_off_[i]) {
-        for (int c = 0; c < 3; ++c) { out->force[c] = host_s_[i].loads[c]; out->moment[c] = host_s_[i].loads[3 + c]; }
+        for (int c = 0; c < 3; ++c) {
+            out->force[c] = host_s_[i].loads[c];
+            out->moment[c] = host_s_[i].loads[3 + c];
+            out->viscous_force[c] = host_s_[i].loads[6 + c];
+            out->viscous_moment[c] = host_s_[i].loads[9 + c];
+        }
+    }
+    out->viscous_valid = wall_applied_[i];
+    for (int c = 0; c < 3; ++c) {
+        out->total_force[c] = out->force[c] + out->viscous_force[c];
+        out->total_moment[c] = out->moment[c] + out->viscous_moment[c];
     }
     out->drag = out->force[0]; out->side = out->force[1]; out->lift = out->force[2];
+    out->viscous_drag = out->viscous_force[0]; out->viscous_side = out->viscous_force[1]; out->viscous_lift = out->viscous_force[2];
+    out->total_drag = out->total_force[0]; out->total_side = out->total_force[1]; out->total_lift = out->total_force[2];
     const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed, area = inst_ref_area_[i];
     if (q > 0 && area > 0) {
+        const double qs = q * area;
         out->coefficients_valid = 1;
-        out->cd = out->drag / (q * area); out->cs = out->side / (q * area); out->cl = out->lift / (q * area);
+        out->cd = out->drag / qs; out->cs = out->side / qs; out->cl = out->lift / qs;
+        out->cd_viscous = out->viscous_drag / qs; out->cs_viscous = out->viscous_side / qs; out->cl_viscous = out->viscous_lift / qs;
+        out->total_cd = out->total_drag / qs; out->total_cs = out->total_side / qs; out->total_cl = out->total_lift / qs;
     }
 }
 
@@ -1400,6 +1482,12 @@ void SolverImpl<R>::snapshot_state() {
     }
     if (nedges_) HIP_CHECK(hipMemcpyAsync(snap_fe_, fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
     HIP_CHECK(hipMemcpyAsync(snap_p_, p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (M_.npatches) {
+        if (!snap_wall_force_) snap_wall_force_ = alloc<double>(3 * static_cast<int64_t>(M_.npatches));
+        HIP_CHECK(hipMemcpyAsync(snap_wall_force_, wall_force_, 3 * static_cast<size_t>(M_.npatches) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream_));
+    }
+    snap_wall_applied_ = wall_applied_;
 }
 
 template <class R>
@@ -1410,6 +1498,10 @@ void SolverImpl<R>::restore_state() {
     }
     if (nedges_) HIP_CHECK(hipMemcpyAsync(fe_, snap_fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
     HIP_CHECK(hipMemcpyAsync(p_, snap_p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (M_.npatches)
+        HIP_CHECK(hipMemcpyAsync(wall_force_, snap_wall_force_, 3 * static_cast<size_t>(M_.npatches) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream_));
+    wall_applied_ = snap_wall_applied_;
 }
 
 // ------------------------------------------------------------------------------------------

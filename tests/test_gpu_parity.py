@@ -268,3 +268,46 @@ def test_mixed_precision_vcycle_converges(lib, oracle_built, prec):
     ox, _, _, op = orc.state()
     assert rel_l2(gx[act], ox[act]) <= 1e-6
     assert rel_l2(gpu.download_pressure()[act], op[act]) <= 1e-5
+
+
+@pytest.mark.parametrize("case,kw", [
+    ("sphere", dict(cells_x=32, levels=2, band=0.3)),
+    ("naca", dict(digits=2412, alpha_deg=4.0, cells_x=24, levels=2, bands=(0.3,))),
+])
+def test_fp64_smooth_wall_step_matches_oracle(lib, oracle_built, case, kw):
+    """Smooth-wall shear (Spalding friction velocity, implicit 3x3 update per wall cell) in
+    every step, and the viscous loads it reports, at the FP64 bar."""
+    topo = make(case, **kw)
+    gpu, orc = pair(topo, opts(smooth_wall=1, nu=2e-4, speed=2.0))
+    before = gpu.pressure_loads()
+    assert before["viscous_valid"] == 0 and before["viscous_force"] == [0.0, 0.0, 0.0]
+    gi, oi, hist = run_both(gpu, orc, 5)
+    for gs, os_ in hist:
+        assert gs["iterations"] == os_.iterations
+        assert abs(gs["dt"] - os_.dt) <= 1e-12 * os_.dt
+    act = (topo.array("mask") & 64) != 0
+    gx, gy, gz = gpu.download_cell_momentum_state()
+    ox, oy, oz, op = orc.state()
+    for g, o in ((gx, ox), (gy, oy), (gz, oz)):
+        assert rel_l2(g[act], o[act]) <= 1e-10
+    assert rel_l2(gpu.download_pressure()[act], op[act]) <= 1e-10
+    gl, ol = gpu.pressure_loads(), orc.loads()
+    assert gl["viscous_valid"] == 1 and ol["viscous_valid"] == 1
+    assert ol["viscous_drag"] > 0
+    for key in ("viscous_force", "viscous_moment", "total_force"):
+        ref = np.array(ol[key])
+        assert np.linalg.norm(np.array(gl[key]) - ref) <= 1e-10 * np.linalg.norm(ref), key
+    assert gl["total_cd"] == pytest.approx(ol["total_cd"], rel=1e-10)
+
+
+def test_fp32_smooth_wall_tracks_oracle(lib, oracle_built):
+    from paracfd_amd import _lib as L
+    topo = make("sphere", cells_x=32, levels=2, band=0.3)
+    gpu, orc = pair(topo, opts(smooth_wall=1, nu=2e-4, speed=2.0, field_precision=L.FP32, vcycle_precision=L.FP32))
+    run_both(gpu, orc, 5)
+    act = (topo.array("mask") & 64) != 0
+    gx, _, _ = gpu.download_cell_momentum_state()
+    ox, _, _, _ = orc.state()
+    assert rel_l2(gx[act], ox[act]) <= 1e-4
+    gl, ol = gpu.pressure_loads(), orc.loads()
+    assert abs(gl["viscous_drag"] - ol["viscous_drag"]) <= 1e-3 * abs(ol["viscous_drag"])

@@ -1,0 +1,135 @@
+"""The oracle's smooth-wall model (amr_advection.cu:920-942, friction velocity
+amr_pressure.h:41-53), pinned by what the mathematics fixes:
+
+  * the friction velocity inverts Spalding's law: u+ from a chosen y+ round-trips
+  * viscous sublayer: u_tau^2 -> nu u_t / y (laminar wall shear) as Re_y -> 0
+  * log layer: u+ -> ln(y+) / kappa + B with kappa = 0.41, B = 5.2 for large Re_y
+  * the implicit update conserves momentum: rho V (v - v') / dt equals the cell's patch forces
+  * the normal velocity of a one-patch cell is unchanged and its tangential velocity is
+    divided by 1 + (dt / V) k, with k = A u_tau^2 / |v_t| from the old velocity
+A planted mistake (a wrong von Karman constant in the check) fails the log-layer test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from paracfd_amd import _lib as L, cases
+
+KAPPA, B = 0.41, 5.2
+
+
+def spalding_yplus(up):
+    x = KAPPA * up
+    return up + math.exp(-KAPPA * B) * (math.expm1(x) - x - 0.5 * x * x - x ** 3 / 6.0)
+
+
+def options(**kw):
+    o = L.Options()
+    o.rho, o.nu, o.speed, o.cfl, o.rtol, o.max_iterations = 1.2, 1.5e-5, 10.0, 0.5, 1e-8, 300
+    o.pre_smooth, o.post_smooth, o.check_interval = 2, 2, 1
+    o.smoother_weight = 6.0 / 7.0
+    o.smooth_wall = 1
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    g, d = cases.sphere_box(cells_x=32, levels=2, band=0.3)
+    topo = L.Topology(g, d)
+    return topo, Oracle(topo, options())
+
+
+def test_friction_velocity_inverts_spalding(sphere):
+    _, orc = sphere
+    nu, y = 1.5e-5, 2e-3
+    for up in (0.05, 0.3, 1.0, 3.0, 8.0, 15.0, 25.0, 40.0):
+        yp = spalding_yplus(up)
+        utau = yp * nu / y            # y+ = u_tau y / nu
+        ut = up * utau                # u+ = u_t / u_tau
+        got = orc.wall_friction(ut, y, nu)
+        assert got == pytest.approx(utau, rel=4e-8 / min(up, 1.0) + 1e-9), up
+
+
+def test_friction_velocity_viscous_sublayer(sphere):
+    _, orc = sphere
+    nu, y, ut = 1e-3, 1e-3, 1e-3      # Re_y = 1e-3
+    got = orc.wall_friction(ut, y, nu)
+    assert got ** 2 == pytest.approx(nu * ut / y, rel=2e-4)
+    assert orc.wall_friction(0.0, y, nu) == 0.0 and orc.wall_friction(ut, 0.0, nu) == 0.0
+
+
+@pytest.mark.parametrize("kappa,fails", [(KAPPA, False), (0.38, True)])
+def test_friction_velocity_log_layer(sphere, kappa, fails):
+    _, orc = sphere
+    nu, y = 1.5e-5, 0.05
+    worst = 0.0
+    for ut in (5.0, 20.0, 60.0):
+        re = ut * y / nu
+        up = ut / orc.wall_friction(ut, y, nu)
+        yp = re / up
+        worst = max(worst, abs(up - (math.log(yp) / kappa + B)))
+    assert (worst > 0.2) == fails, worst
+
+
+def test_implicit_wall_update_balances_momentum(sphere):
+    topo, orc = sphere
+    n = topo.ndof
+    act = (topo.array("mask") & 64) != 0
+    rng = np.random.default_rng(5)
+    v = [np.where(act, c + 0.2 * rng.standard_normal(n), 0.0) for c in (9.0, 1.5, -2.0)]
+    orc.set_state(*v, np.zeros(n))
+    dt, rho = 2e-3, 1.2
+    F = orc.smooth_wall(dt)
+    nx, ny, nz, _ = orc.state()
+    new = np.stack([nx, ny, nz], axis=1)
+    old = np.stack(v, axis=1)
+    pdof = topo.array("patch_dof")
+    S = topo.array("patch_S").reshape(-1, 3)
+    xs = topo.array("patch_x").reshape(-1, 3)
+    cen = topo.array("centroid").reshape(-1, 3)
+    sdof = topo.array("special_dof")
+    svol = dict(zip(sdof.tolist(), topo.array("special_volume").tolist()))
+    cells, counts = np.unique(pdof, return_counts=True)
+    assert len(cells) > 50
+    # momentum: rho V (v - v') / dt = sum of the cell's patch forces
+    for a in cells:
+        V = svol[int(a)]
+        lhs = rho * V * (old[a] - new[a]) / dt
+        rhs = F[pdof == a].sum(axis=0)
+        assert np.allclose(lhs, rhs, rtol=1e-9, atol=1e-12 * np.abs(lhs).max())
+    # one-patch cells: closed form of the implicit solve
+    nu = 1.5e-5
+    for q in np.flatnonzero(np.isin(pdof, cells[counts == 1]))[:200]:
+        a = int(pdof[q])
+        area = np.linalg.norm(S[q])
+        nrm = S[q] / area
+        y = abs(np.dot(cen[a] - xs[q], nrm))
+        vt = old[a] - np.dot(old[a], nrm) * nrm
+        k = area * orc.wall_friction(np.linalg.norm(vt), y, nu) ** 2 / np.linalg.norm(vt)
+        expect = np.dot(old[a], nrm) * nrm + vt / (1.0 + dt / svol[a] * k)
+        assert np.allclose(new[a], expect, rtol=1e-12, atol=1e-12)
+        assert np.dot(new[a], nrm) == pytest.approx(np.dot(old[a], nrm), rel=1e-12, abs=1e-12)
+    # only wall cells change
+    others = np.setdiff1d(np.flatnonzero(act), cells)
+    assert np.array_equal(new[others], old[others])
+
+
+def test_viscous_loads_sum_the_patch_forces(sphere):
+    topo, _ = sphere
+    orc = Oracle(topo, options())
+    orc.set_reference(topo.info.reference_area)
+    orc.initialize()
+    before = orc.loads()
+    assert before["viscous_valid"] == 0 and before["viscous_force"] == [0.0, 0.0, 0.0]
+    orc.step()
+    ld = orc.loads()
+    assert ld["viscous_valid"] == 1
+    # wall shear drags the body downstream; totals are pressure + viscous
+    assert ld["viscous_drag"] > 0
+    assert ld["total_drag"] == pytest.approx(ld["drag"] + ld["viscous_drag"], rel=1e-14)
+    q = 0.5 * 1.2 * 10.0 ** 2 * topo.info.reference_area
+    assert ld["cd_viscous"] == pytest.approx(ld["viscous_drag"] / q, rel=1e-14)
