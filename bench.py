@@ -89,7 +89,8 @@ def run(topo, spec, args, name, field, vcycle, pc):
     opt = default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"], cfl=spec["cfl"], rtol=1e-8,
                           max_iterations=500, field_precision=field, vcycle_precision=vcycle, preconditioner=pc,
                           pre_smooth=2, post_smooth=2, check_interval=args.check_interval,
-                          learned_bricks=args.learned_bricks, learned_capacity=args.collects * args.pairs)
+                          learned_bricks=args.learned_bricks, learned_capacity=args.collects * args.pairs,
+                          smooth_wall=1 if args.smooth_wall else 0)
     core = ExternalAeroCore(topo, opt)
     core.set_reference(topo.info.reference_area)
     st = core.initialize()
@@ -163,7 +164,8 @@ def run_sweep(args):
         f"{merged.info.bricks} bricks, built and merged in {time.perf_counter() - t0:.1f}s")
     opt = default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"], cfl=spec["cfl"], rtol=1e-8,
                           max_iterations=500, field_precision=L.FP32, vcycle_precision=L.FP32,
-                          preconditioner=L.PC_MG, pre_smooth=2, post_smooth=2, check_interval=args.check_interval)
+                          preconditioner=L.PC_MG, pre_smooth=2, post_smooth=2, check_interval=args.check_interval,
+                          smooth_wall=1 if args.smooth_wall else 0)
 
     def advance(core, topo, loads_fn):
         core.initialize()
@@ -265,6 +267,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=16)
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--smooth-wall", action="store_true",
+                    help="apply the smooth-wall shear model in every step (the reference's default) and "
+                         "report viscous and total loads")
     ap.add_argument("--vcycle", choices=("fp32", "bf16"), default="fp32",
                     help="V-cycle storage of the learned and the plain mixed-precision solvers")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
@@ -351,7 +356,8 @@ def main():
     if not args.no_cpu_baseline:
         from paracfd_amd import default_options
         cpu = cpu_baseline(topo, default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"],
-                                                 cfl=spec["cfl"], rtol=1e-8, max_iterations=500), args.cpu_budget)
+                                                 cfl=spec["cfl"], rtol=1e-8, max_iterations=500,
+                                                 smooth_wall=1 if args.smooth_wall else 0), args.cpu_budget)
         log(f"cpu baseline {cpu['value']:.4g} cell-steps/s")
 
     def summary(r):
@@ -370,7 +376,9 @@ def main():
                    "network_bricks": tr["bricks"], "rtol": 1e-8, "mean_pcg_iterations": mean_its,
                    "fallback_steps": learned["fallback_steps"], "network_steps": learned["learned_steps"],
                    "network_rejected_by_validation": learned["learned_rejected"], "cl": learned["loads"]["cl"],
-                   "cd": learned["loads"]["cd"],
+                   "cd": learned["loads"]["cd"], "smooth_wall": bool(args.smooth_wall),
+                   "cd_viscous": learned["loads"]["cd_viscous"], "total_cd": learned["loads"]["total_cd"],
+                   "total_cl": learned["loads"]["total_cl"],
                    "training": {k: tr[k] for k in ("pairs", "train_steps", "loss_first", "loss_last", "train_ms")},
                    "launch_shapes": learned["tuning"],
                    "plain_fp32_vcycle": summary(plain32), "plain_fp64_vcycle": summary(plain64),

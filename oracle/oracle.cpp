@@ -663,6 +663,8 @@ struct Oracle {
                 double y = 0.0;
                 for (int c = 0; c < 3; ++c) y += (centroid[3 * a + c] - patch_x[3 * q + c]) * S[c] / area;
                 y = std::abs(y);
+                // fluid centroid on the patch's plane: the slab estimate V / (2A)
+                if (!(y > std::max(1e-14, std::cbrt(vol(a)) * 1e-10))) y = 0.5 * vol(a) / area;
                 const double un = v[0] * nn[0] + v[1] * nn[1] + v[2] * nn[2];
                 const double t[3] = {v[0] - un * nn[0], v[1] - un * nn[1], v[2] - un * nn[2]};
                 const double speed = std::sqrt(std::max(0.0, t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));

@@ -791,7 +791,8 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
 }
 
 // Smooth-wall model: the wall cells in dof order with their patches, the cells' volumes and
-// each patch's wall distance |(x_cell - x_patch) . n| (amr_advection.cu:934).  The patch force
+// each patch's wall distance |(x_cell - x_patch) . n| (amr_advection.cu:934), or V / (2A) where
+// that vanishes.  The patch force
 // array exists (zero) with the model off, so the load reduction always reads it.
 template <class R>
 void SolverImpl<R>::setup_smooth_wall(const Topology& T) {
@@ -823,8 +824,10 @@ void SolverImpl<R>::setup_smooth_wall(const Topology& T) {
         if (!(area > 0.0)) throw std::invalid_argument("smooth-wall patch without area");
         for (int c = 0; c < 3; ++c) d += (T.centroid[3 * static_cast<size_t>(a) + c] - T.patch_x[3 * q + c]) * T.patch_S[3 * q + c] / area;
         dist[q] = std::abs(d);
-        if (!(dist[q] > std::max(1e-14, std::cbrt(vol.back()) * 1e-10)))
-            throw std::invalid_argument("smooth-wall patch at zero distance from its cell centroid");
+        // a fluid centroid on the patch's plane (fluid on both sides of a thin edge at coarse
+        // resolution) has no distance; the slab estimate V / (2A) takes its place (the
+        // reference rejects such a side, amr_advection.cu:934)
+        if (!(dist[q] > std::max(1e-14, std::cbrt(vol.back()) * 1e-10))) dist[q] = 0.5 * vol.back() / area;
     }
     off.push_back(np);
     nwall_ = static_cast<int>(cells.size());

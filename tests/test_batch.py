@@ -93,14 +93,15 @@ def test_merge_rejects_mismatched_levels():
 
 
 @pytest.mark.gpu
-def test_batch_matches_individual_runs(parts, merged):
+@pytest.mark.parametrize("wall", [0, 1])
+def test_batch_matches_individual_runs(parts, merged, wall):
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from paracfd_amd import ExternalAeroCore, default_options
     o = default_options(rho=1.0, nu=1e-4, speed=1.0, cfl=0.5, rtol=1e-8, max_iterations=300,
-                        field_precision=L.FP64, vcycle_precision=L.FP64)
-    print(f"[batch] {merged.instances} instances, {merged.info.dofs} dofs", flush=True)
+                        field_precision=L.FP64, vcycle_precision=L.FP64, smooth_wall=wall)
+    print(f"[batch] {merged.instances} instances, {merged.info.dofs} dofs, smooth wall {wall}", flush=True)
     batch = ExternalAeroCore(merged, o)
     singles = [ExternalAeroCore(p, o) for p in parts]
     for i, p in enumerate(parts):
@@ -128,8 +129,10 @@ def test_batch_matches_individual_runs(parts, merged):
         for a, b in ((bx[sl], x), (by[sl], y), (bz[sl], z), (bp[sl], c.download_pressure())):
             assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
         bl, sl_ = batch.instance_loads(i), c.pressure_loads()
-        for k in ("cl", "cd"):
+        for k in ("cl", "cd", "cd_viscous", "cl_viscous", "total_cd"):
             assert bl[k] == pytest.approx(sl_[k], rel=1e-11, abs=1e-14)
+        assert bl["viscous_valid"] == sl_["viscous_valid"] == wall
+        assert (sl_["cd_viscous"] > 0) == bool(wall)
     # all instances' loads in one call are the per-instance integrals
     every = batch.batch_loads()
     assert len(every) == merged.instances

@@ -108,6 +108,8 @@ def test_implicit_wall_update_balances_momentum(sphere):
         area = np.linalg.norm(S[q])
         nrm = S[q] / area
         y = abs(np.dot(cen[a] - xs[q], nrm))
+        if y <= max(1e-14, np.cbrt(svol[a]) * 1e-10):
+            y = 0.5 * svol[a] / area
         vt = old[a] - np.dot(old[a], nrm) * nrm
         k = area * orc.wall_friction(np.linalg.norm(vt), y, nu) ** 2 / np.linalg.norm(vt)
         expect = np.dot(old[a], nrm) * nrm + vt / (1.0 + dt / svol[a] * k)
@@ -133,3 +135,23 @@ def test_viscous_loads_sum_the_patch_forces(sphere):
     assert ld["total_drag"] == pytest.approx(ld["drag"] + ld["viscous_drag"], rel=1e-14)
     q = 0.5 * 1.2 * 10.0 ** 2 * topo.info.reference_area
     assert ld["cd_viscous"] == pytest.approx(ld["viscous_drag"] / q, rel=1e-14)
+
+
+def test_patch_on_the_centroid_plane_uses_the_slab_distance():
+    """A coarse wing has a cut cell whose fluid centroid lies on its patch's plane (fluid on both
+    sides of the thin trailing edge); its wall distance becomes V / (2A) and the update stays finite."""
+    g, d = cases.naca_wing(digits=4412, alpha_deg=-4.0, cells_x=24, levels=2, bands=(0.3,))
+    topo = L.Topology(g, d)
+    pd = topo.array("patch_dof")
+    S = topo.array("patch_S").reshape(-1, 3)
+    nrm = S / np.linalg.norm(S, axis=1)[:, None]
+    cen = topo.array("centroid").reshape(-1, 3)
+    dist = np.abs(np.einsum("ij,ij->i", cen[pd] - topo.array("patch_x").reshape(-1, 3), nrm))
+    assert dist.min() == 0.0
+    orc = Oracle(topo, options())
+    n = topo.ndof
+    act = (topo.array("mask") & 64) != 0
+    orc.set_state(np.where(act, 9.0, 0.0), np.where(act, 1.0, 0.0), np.zeros(n), np.zeros(n))
+    F = orc.smooth_wall(1e-3)
+    assert np.all(np.isfinite(F)) and np.all(np.isfinite(orc.state()[0]))
+    assert np.abs(F[dist == 0.0]).max() > 0.0
