@@ -326,6 +326,17 @@ pcfd_status pcfd_set_faces(pcfd_solver* solver, const double* fx, const double* 
 pcfd_status pcfd_mark_initialized(pcfd_solver* solver, double physical_time);
 int64_t pcfd_device_bytes(const pcfd_solver* solver);
 
+/* Field of one instance on the plane x[axis] = coord, as an image for a viewer (no counterpart
+ * on the AMR path of the reference, which samples its AMR state on the host,
+ * core/fluid/amr_sampling.h, and colours a uniform MAC state on the device,
+ * gui/slice_field.h:20-31).  out[i + nu j] is the value at the centre of pixel (i, j) of the
+ * window lo..hi in the plane's two other axes (in increasing order): the cell-centred value of
+ * the finest leaf cell containing the point, NaN where it is solid or outside the grid.
+ * field: 0 |v|, 1 u, 2 v, 3 w, 4 pressure, 5 Cp = p / (rho U^2 / 2).  out is host memory
+ * (nu * nv floats); only the image crosses PCIe. */
+pcfd_status pcfd_sample_slice(pcfd_solver* solver, int instance, int field, int axis, double coord, const double lo[2],
+                              const double hi[2], int nu, int nv, float* out);
+
 /* Multigrid primitives on level `level` (0 = composite level), host arrays in, host array out.
  * op: 0 apply A x, 1 residual b - A x, 2 damped-Jacobi sweeps on x (omega, sweeps),
  *     3 one full V-cycle z = M^-1 b.  The uniform legacy primitives project.cu:286-328

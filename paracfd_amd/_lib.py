@@ -130,7 +130,7 @@ EXPORTED = [
     "pcfd_learned_get_weights", "pcfd_learned_set_weights", "pcfd_learned_enable", "pcfd_learned_forward",
     "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
     "pcfd_integrate_loads_instance", "pcfd_set_reference_instance", "pcfd_integrate_loads_batch",
-    "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning",
+    "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning", "pcfd_sample_slice",
 ]
 
 _lib = None
@@ -198,6 +198,7 @@ def load():
         "pcfd_autotune": (C.c_int, [P, C.c_int, C.POINTER(Tuning)]),
         "pcfd_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pcfd_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "pcfd_sample_slice": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_int, C.c_int, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

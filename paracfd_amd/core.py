@@ -169,6 +169,18 @@ class ExternalAeroCore:
         o = (C.c_double * 3)(*origin)
         L.check(self._lib.pcfd_set_reference_instance(self.handle, i, area, o))
 
+    SLICE_FIELDS = {"speed": 0, "u": 1, "v": 2, "w": 3, "p": 4, "cp": 5}
+
+    def sample_slice(self, field, axis, coord, lo, hi, nu, nv, instance=0) -> np.ndarray:
+        """Image [nv, nu] of a field on the plane x[axis] = coord over the window lo..hi of the
+        two other axes: the finest leaf cell's value at each pixel centre, NaN where solid."""
+        f = self.SLICE_FIELDS[field] if isinstance(field, str) else int(field)
+        out = np.empty(nu * nv, dtype=np.float32)
+        lo2 = (C.c_double * 2)(*lo)
+        hi2 = (C.c_double * 2)(*hi)
+        L.check(self._lib.pcfd_sample_slice(self.handle, instance, f, axis, coord, lo2, hi2, nu, nv, L.fptr(out)))
+        return out.reshape(nv, nu)
+
     # learned preconditioner (options.preconditioner == PC_LEARNED); no reference counterpart
     def learned_collect(self, max_pairs=8) -> dict:
         """Store post-V-cycle residuals from a plain-MGPCG solve of the next step's projection."""

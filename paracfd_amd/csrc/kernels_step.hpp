@@ -530,6 +530,65 @@ __global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restr
     }
 }
 
+// Field sampling on an axis-aligned plane for a viewer (gui/slice_field.h:20-31 fields; the
+// reference samples its uniform MAC state there and the AMR state on the host,
+// core/fluid/amr_sampling.h).  Pixel (i, j) of an nu x nv image is the point at the centre of
+// the pixel's rectangle in the plane; its value is the cell-centred value of the finest leaf
+// cell containing it (no interpolation), NaN where the point is solid or outside the grid.
+// The leaf is found from per-level maps of brick coordinates to stored bricks, finest first.
+struct SliceMap {
+    const int* map;      // per level: dims x*y*z brick ids (-1: none), level l at off[l]
+    int64_t off[8];
+    int dims[8][3];
+    int levels;
+    double lo[3];
+    double h[8];
+};
+
+template <class R>
+__global__ __launch_bounds__(256) void k_slice(MeshView M, FieldsView<R> F, const double* __restrict__ p, SliceMap G,
+                                               int field, int axis, double coord, double u0, double u1, double v0,
+                                               double v1, int nu, int nv, double qinf, float* __restrict__ out) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= nu * nv) return;
+    const int i = pix % nu, j = pix / nu;
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    double x[3];
+    x[axis] = coord;
+    x[a1] = u0 + (i + 0.5) * (u1 - u0) / nu;
+    x[a2] = v0 + (j + 0.5) * (v1 - v0) / nv;
+    float val = __builtin_nanf("");
+    for (int l = G.levels - 1; l >= 0; --l) {
+        int c[3], bc[3];
+        bool in = true;
+        for (int d = 0; d < 3; ++d) {
+            const double t = floor((x[d] - G.lo[d]) / G.h[l]);
+            in = in && t >= 0.0 && t < 8.0 * G.dims[l][d];
+            c[d] = in ? static_cast<int>(t) : 0;
+            bc[d] = c[d] >> 3;
+        }
+        if (!in) continue;
+        const int b = G.map[G.off[l] + (static_cast<int64_t>(bc[2]) * G.dims[l][1] + bc[1]) * G.dims[l][0] + bc[0]];
+        if (b < 0) continue;
+        const int64_t a = static_cast<int64_t>(b) * 512 + (c[0] & 7) + 8 * (c[1] & 7) + 64 * (c[2] & 7);
+        if (M.mask[a] & kActive) {
+            const double vx = double(F.v[0][a]), vy = double(F.v[1][a]), vz = double(F.v[2][a]);
+            double r = 0.0;
+            switch (field) {
+                case 0: r = sqrt(vx * vx + vy * vy + vz * vz); break;
+                case 1: r = vx; break;
+                case 2: r = vy; break;
+                case 3: r = vz; break;
+                case 4: r = p[a]; break;
+                default: r = p[a] / qinf; break;
+            }
+            val = static_cast<float>(r);
+        }
+        break;
+    }
+    out[pix] = val;
+}
+
 // Smooth-wall law (Spalding): y+ = u+ + e^(-kappa B) (e^(kappa u+) - 1 - kappa u+ - (kappa u+)^2/2
 // - (kappa u+)^3/6), kappa = 0.41, B = 5.2.  With Re_y = u_t y / nu = u+ y+ the friction
 // velocity is u_t / u+; u+ by 36 bisections of [0, 80] (amr_pressure.h:41-53).  Below

@@ -81,6 +81,8 @@ public:
     virtual void get_tuning(pcfd_tuning* out) const = 0;
     virtual void set_tuning(const pcfd_tuning& t) = 0;
     virtual int instances() const = 0;
+    virtual void sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
+                              int nu, int nv, float* out) = 0;
     bool initialized = false;
     double time = 0.0;
     int64_t device_bytes = 0;
@@ -125,6 +127,8 @@ public:
     void get_tuning(pcfd_tuning* out) const override { *out = tuning_; }
     void set_tuning(const pcfd_tuning& t) override;
     int instances() const override { return ninst_; }
+    void sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi, int nu,
+                      int nv, float* out) override;
 
 private:
     template <class T>
@@ -401,6 +405,15 @@ private:
     double* wall_k_ = nullptr;
     double* wall_force_ = nullptr;
     std::vector<uint8_t> wall_applied_, snap_wall_applied_;  // per instance
+    // slice sampling: brick maps per instance (built on first use) and the image buffer
+    std::vector<int> brick_level_h_, brick_coord_h_;
+    int base_bricks_[3] = {0, 0, 0};
+    double grid_lo_[3] = {0, 0, 0};
+    std::vector<double> level_h_h_;
+    std::vector<int*> slice_map_;
+    std::vector<SliceMap> slice_geo_;
+    float* slice_buf_ = nullptr;
+    int64_t slice_cap_ = 0;
 };
 
 template <class R>
@@ -664,6 +677,12 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     if (opt_.preconditioner == PCFD_PC_LEARNED) setup_learned(T);
     if (opt_.smagorinsky_cs > 0.0) setup_smagorinsky(T);
     setup_smooth_wall(T);
+    brick_level_h_ = T.brick_level;
+    brick_coord_h_ = T.brick_coord;
+    for (int a = 0; a < 3; ++a) { base_bricks_[a] = T.base_bricks[a]; grid_lo_[a] = T.lo[a]; }
+    level_h_h_ = T.level_h;
+    slice_map_.assign(ninst_, nullptr);
+    slice_geo_.assign(ninst_, SliceMap{});
     HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -1507,6 +1526,58 @@ void SolverImpl<R>::restore_state() {
     wall_applied_ = snap_wall_applied_;
 }
 
+// Field of one instance on a plane (an image of nu x nv values, NaN where solid or outside).
+// The brick maps of the instance are built on the first call; the image is copied to the host.
+template <class R>
+void SolverImpl<R>::sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
+                                 int nu, int nv, float* out) {
+    if (instance < 0 || instance >= ninst_) throw std::invalid_argument("sample_slice: no such instance");
+    if (field < 0 || field > 5) throw std::invalid_argument("sample_slice: field must be 0..5");
+    if (axis < 0 || axis > 2) throw std::invalid_argument("sample_slice: axis must be 0, 1 or 2");
+    if (nu < 1 || nv < 1 || static_cast<int64_t>(nu) * nv > (int64_t(1) << 26))
+        throw std::invalid_argument("sample_slice: image size must be 1..2^26 pixels");
+    if (!std::isfinite(coord) || !(lo[0] < hi[0]) || !(lo[1] < hi[1]) || !std::isfinite(hi[0] - lo[0]) ||
+        !std::isfinite(hi[1] - lo[1]))
+        throw std::invalid_argument("sample_slice: invalid plane or window");
+    const int levels = static_cast<int>(level_h_h_.size());
+    if (levels > 8) throw std::invalid_argument("sample_slice: at most 8 AMR levels");
+    const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed;
+    if (field == 5 && !(q > 0.0)) throw std::invalid_argument("sample_slice: Cp needs a positive dynamic pressure");
+    if (!slice_map_[instance]) {
+        SliceMap G{};
+        G.levels = levels;
+        int64_t total = 0;
+        for (int l = 0; l < levels; ++l) {
+            G.off[l] = total;
+            for (int d = 0; d < 3; ++d) G.dims[l][d] = base_bricks_[d] << l;
+            total += static_cast<int64_t>(G.dims[l][0]) * G.dims[l][1] * G.dims[l][2];
+            G.h[l] = level_h_h_[l];
+        }
+        for (int d = 0; d < 3; ++d) G.lo[d] = grid_lo_[d];
+        std::vector<int> map(total, -1);
+        for (int b = inst_brick_off_[instance]; b < inst_brick_off_[instance + 1]; ++b) {
+            const int l = brick_level_h_[b];
+            const int* c = &brick_coord_h_[3 * static_cast<size_t>(b)];
+            for (int d = 0; d < 3; ++d)
+                if (c[d] < 0 || c[d] >= G.dims[l][d]) throw std::logic_error("sample_slice: brick outside its level grid");
+            map[G.off[l] + (static_cast<int64_t>(c[2]) * G.dims[l][1] + c[1]) * G.dims[l][0] + c[0]] = b;
+        }
+        slice_map_[instance] = upload(map);
+        slice_geo_[instance] = G;
+    }
+    SliceMap G = slice_geo_[instance];
+    G.map = slice_map_[instance];
+    const int64_t npix = static_cast<int64_t>(nu) * nv;
+    if (npix > slice_cap_) {
+        slice_buf_ = alloc<float>(npix);
+        slice_cap_ = npix;
+    }
+    LAUNCH(k_slice<R>, blocks_for(npix, 256), 256, M_, fields(), p_, G, field, axis, coord, lo[0], hi[0], lo[1], hi[1],
+           nu, nv, field == 5 ? q : 1.0, slice_buf_);
+    HIP_CHECK(hipMemcpyAsync(out, slice_buf_, npix * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
 // ------------------------------------------------------------------------------------------
 // launch-shape autotuner
 // ------------------------------------------------------------------------------------------
@@ -1925,6 +1996,12 @@ pcfd_status pcfd_integrate_loads_batch(pcfd_solver* s, double pref, pcfd_loads* 
     NEED(s && loads, "pcfd_integrate_loads_batch: null argument");
     NEED(count == s->impl->instances(), "pcfd_integrate_loads_batch: count must equal the number of instances");
     return guarded("pcfd_integrate_loads_batch", [&] { s->impl->batch_loads(pref, loads); });
+}
+
+pcfd_status pcfd_sample_slice(pcfd_solver* s, int instance, int field, int axis, double coord, const double lo[2],
+                              const double hi[2], int nu, int nv, float* out) {
+    NEED(s && lo && hi && out, "pcfd_sample_slice: null argument");
+    return guarded("pcfd_sample_slice", [&] { s->impl->sample_slice(instance, field, axis, coord, lo, hi, nu, nv, out); });
 }
 
 pcfd_status pcfd_integrate_loads_instance(pcfd_solver* s, int instance, double pref, pcfd_loads* loads) {
