@@ -1271,44 +1271,67 @@ void Builder::build_multigrid() {
             A[static_cast<size_t>(r) * n + r] = K.diag[r];
             for (int q = K.row_off[r]; q < K.row_off[r + 1]; ++q) A[static_cast<size_t>(r) * n + K.col[q]] -= K.coef[q];
         }
-        // Cholesky A = L L^T in place (lower)
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nt = std::max(1, std::min(static_cast<int>(hw ? hw : 1u), std::min(16, n / 64)));
+        auto on_threads = [&](int count, const std::function<void(int, int)>& body) {
+            if (nt == 1 || count < 2 * nt) { body(0, count); return; }
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nt; ++t) pool.emplace_back(body, count * t / nt, count * (t + 1) / nt);
+            for (auto& th : pool) th.join();
+        };
+        auto at = [&](int r, int c) -> double& { return A[static_cast<size_t>(r) * n + c]; };
+        // Cholesky A = L L^T in place (lower), by blocks of 64 columns: each entry is
+        // A_ij - sum_{k<j} L_ik L_jk (k ascending) divided by L_jj, as in the unblocked order;
+        // the rows below a block are independent and run on host threads
         bool ok = true;
-        for (int j = 0; j < n && ok; ++j) {
-            double d = A[static_cast<size_t>(j) * n + j];
-            for (int k = 0; k < j; ++k) d -= A[static_cast<size_t>(j) * n + k] * A[static_cast<size_t>(j) * n + k];
-            if (!(d > 1e-14 * K.diag[j])) { ok = false; break; }
-            d = std::sqrt(d);
-            A[static_cast<size_t>(j) * n + j] = d;
-            for (int i = j + 1; i < n; ++i) {
-                double s = A[static_cast<size_t>(i) * n + j];
-                for (int k = 0; k < j; ++k) s -= A[static_cast<size_t>(i) * n + k] * A[static_cast<size_t>(j) * n + k];
-                A[static_cast<size_t>(i) * n + j] = s / d;
+        constexpr int kNB = 64;
+        for (int j0 = 0; j0 < n && ok; j0 += kNB) {
+            const int j1 = std::min(n, j0 + kNB);
+            for (int j = j0; j < j1 && ok; ++j) {  // the diagonal block
+                double d = at(j, j);
+                for (int k = 0; k < j; ++k) d -= at(j, k) * at(j, k);
+                if (!(d > 1e-14 * K.diag[j])) { ok = false; break; }
+                d = std::sqrt(d);
+                at(j, j) = d;
+                for (int i = j + 1; i < j1; ++i) {
+                    double v = at(i, j);
+                    for (int k = 0; k < j; ++k) v -= at(i, k) * at(j, k);
+                    at(i, j) = v / d;
+                }
             }
+            if (!ok || j1 == n) break;
+            on_threads(n - j1, [&](int r0, int r1) {
+                for (int i = j1 + r0; i < j1 + r1; ++i)
+                    for (int j = j0; j < j1; ++j) {
+                        double v = at(i, j);
+                        for (int k = 0; k < j; ++k) v -= at(i, k) * at(j, k);
+                        at(i, j) = v / at(j, j);
+                    }
+            });
         }
         if (ok) {
-            // inverse column by column (columns are independent: host threads)
+            // inverse column by column (columns are independent: host threads); the backward
+            // substitution reads L^T row by row
+            std::vector<double> U(static_cast<size_t>(n) * n, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k <= i; ++k) U[static_cast<size_t>(k) * n + i] = at(i, k);
             K.coarse_inv.assign(static_cast<size_t>(n) * n, 0.0);
-            auto columns = [&](int c0, int c1) {
+            on_threads(n, [&](int c0, int c1) {
                 std::vector<double> y(n);
                 for (int c = c0; c < c1; ++c) {
                     for (int i = 0; i < n; ++i) {
-                        double s = i == c ? 1.0 : 0.0;
-                        for (int k = 0; k < i; ++k) s -= A[static_cast<size_t>(i) * n + k] * y[k];
-                        y[i] = s / A[static_cast<size_t>(i) * n + i];
+                        double v = i == c ? 1.0 : 0.0;
+                        for (int k = 0; k < i; ++k) v -= A[static_cast<size_t>(i) * n + k] * y[k];
+                        y[i] = v / A[static_cast<size_t>(i) * n + i];
                     }
                     for (int i = n - 1; i >= 0; --i) {
-                        double s = y[i];
-                        for (int k = i + 1; k < n; ++k) s -= A[static_cast<size_t>(k) * n + i] * y[k];
-                        y[i] = s / A[static_cast<size_t>(i) * n + i];
+                        double v = y[i];
+                        for (int k = i + 1; k < n; ++k) v -= U[static_cast<size_t>(i) * n + k] * y[k];
+                        y[i] = v / U[static_cast<size_t>(i) * n + i];
                     }
                     for (int i = 0; i < n; ++i) K.coarse_inv[static_cast<size_t>(i) * n + c] = y[i];
                 }
-            };
-            const unsigned hw = std::thread::hardware_concurrency();
-            const int nt = std::max(1, std::min(static_cast<int>(hw ? hw : 1u), std::min(16, n / 64)));
-            std::vector<std::thread> pool;
-            for (int t = 0; t < nt; ++t) pool.emplace_back(columns, n * t / nt, n * (t + 1) / nt);
-            for (auto& th : pool) th.join();
+            });
         }
     }
 }
