@@ -311,3 +311,12 @@ def test_fp32_smooth_wall_tracks_oracle(lib, oracle_built):
     assert rel_l2(gx[act], ox[act]) <= 1e-4
     gl, ol = gpu.pressure_loads(), orc.loads()
     assert abs(gl["viscous_drag"] - ol["viscous_drag"]) <= 1e-3 * abs(ol["viscous_drag"])
+
+
+def test_smooth_wall_needs_a_molecular_viscosity(lib):
+    from paracfd_amd import ExternalAeroCore, _lib as L
+    topo = make("sphere", cells_x=32)
+    with pytest.raises(L.PcfdError, match="viscosity"):
+        ExternalAeroCore(topo, opts(smooth_wall=1, nu=0.0))
+    core = ExternalAeroCore(topo, opts(smooth_wall=0, nu=0.0))   # without the model nu = 0 is allowed
+    assert core.initialize()["converged"]

@@ -162,3 +162,23 @@ def test_device_slice_of_a_batch_instance():
     a = core.sample_slice("u", 1, 1.0, lo, hi, 120, 80, instance=0)
     b = core.sample_slice("u", 1, 1.0, lo, hi, 120, 80, instance=1)
     assert not np.array_equal(a, b)   # two angles of attack: each instance answers with its own field
+
+
+@pytest.mark.gpu
+def test_device_slice_rejects_bad_requests():
+    _gpu()
+    from paracfd_amd import ExternalAeroCore, default_options
+    g, d = cases.uniform_box(n=8, h=0.125, outlet=True)
+    topo = L.Topology(g, d)
+    core = ExternalAeroCore(topo, default_options())
+    core.initialize()
+    good = core.sample_slice("p", 2, 0.5, (0.0, 0.0), (1.0, 1.0), 8, 8)
+    assert good.shape == (8, 8) and np.isfinite(good).all()
+    for kw in (dict(field=6), dict(axis=3), dict(lo=(1.0, 0.0)), dict(nu=0), dict(instance=1), dict(coord=np.nan)):
+        args = dict(field=4, axis=2, coord=0.5, lo=(0.0, 0.0), hi=(1.0, 1.0), nu=8, nv=8, instance=0)
+        args.update(kw)
+        with pytest.raises(L.PcfdError):
+            core.sample_slice(**args)
+    # a window reaching outside the grid gives NaN there
+    img = core.sample_slice("u", 2, 0.5, (-1.0, 0.0), (1.0, 1.0), 16, 4)
+    assert np.isnan(img[:, :8]).all() and np.isfinite(img[:, 8:]).all()
