@@ -199,11 +199,13 @@ __device__ __forceinline__ float nn_features(float v, uint8_t m, uint16_t* tile,
 
 // Layers 0..3 from the feature tile T0 (T1 zeroed, halos of both zero) into L.y.  When
 // `store` is set, the inputs of layers 1..3 are written there (training).
-__device__ __forceinline__ void nn_layers(const NNView& W, NNShared& L, uint16_t* store) {
+__device__ __forceinline__ void nn_layers(const NNView& W, NNShared& L, uint16_t* store,
+                                          const NNWRegs* first = nullptr) {
     uint16_t* in = L.T0;
     uint16_t* out = L.T1;
     NNWRegs R;
-    nn_fetch_weights(W.w, R);
+    if (first) R = *first;
+    else nn_fetch_weights(W.w, R);
     for (int l = 0; l < kNNLayers; ++l) {
         nn_put_weights(R, L.WL);
         if (l + 1 < kNNLayers) nn_fetch_weights(W.w + static_cast<size_t>(l + 1) * kNNWLayer, R);
@@ -236,20 +238,28 @@ __global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, const in
                                                   const double* __restrict__ z, double* __restrict__ dz,
                                                   const double* __restrict__ rprev, double* partials,
                                                   int nparts, const DevScalars* S, const int* __restrict__ off) {
-    if (S->done || *off) return;
+    // every independent load first (brick, mask, z tile and halo, r, compact neighbours, the
+    // first layer's weights), then the flags, then the tile barrier
     __shared__ __attribute__((aligned(16))) NNShared L;
     const NNBrick B = nn_brick(list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, B.b, q, (m & kSpecial) != 0);
     double* ztile = reinterpret_cast<double*>(L.T1);   // the z tile lives in T1 until the layers
     auto zv = [&](int64_t c) -> double { return z[c]; };
-    load_tile<double>(M, B.b, ztile, zv);
+    const TileFetch<double> tf = fetch_tile<double>(M, B.b, zv);
+    const double ra = r[a];
+    NNWRegs W0;
+    nn_fetch_weights(W.w, W0);
+    const bool skip = S->done || *off;
+    const int sp = special_rank(M, B.b, q, (m & kSpecial) != 0);
+    const CompactFetch<double> cf = fetch_compact<double>(M, sp, zv);
+    if (skip) return;
+    stage_tile<double>(ztile, tf);
     double rp = 0.0;
     if (m & kActive) {
         double dg;
-        rp = r[a] - row0<double>(M, ztile, zv, B.b, q, m, sp, ztile[nn_pos(q)], dg);
+        rp = ra - row0f<double>(M, ztile, cf, zv, B.b, q, m, tf.own, dg);
     }
     __syncthreads();
     nn_zero_tile(L.T0);
@@ -258,7 +268,7 @@ __global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, const in
     const float s = nn_features(static_cast<float>(rp), m, L.T0, L);
     double d = 0.0;
     if (s > 0.f) {  // block-uniform
-        nn_layers(W, L, nullptr);
+        nn_layers(W, L, nullptr, &W0);
         if (m & kActive) d = double(s) / level_h(M, B.b) * double(L.y[q]);
     }
     dz[static_cast<int64_t>(B.pos) * 512 + q] = d;
