@@ -64,6 +64,38 @@ def cpu_baseline(topo, options, budget_s):
                       f"({elapsed:.1f} s timed)"}
 
 
+def device_ceilings():
+    """Practical ceilings measured on this GPU: a 1 GiB device-to-device copy (read + write bytes
+    per second) and a 8192^3 BF16 GEMM (hipBLASLt through torch.matmul), HIP events."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    copy_gbs = 10 * 2 * 4 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    m = 8192
+    x = torch.randn(m, m, dtype=torch.bfloat16, device="cuda")
+    y = torch.randn(m, m, dtype=torch.bfloat16, device="cuda")
+    torch.matmul(x, y)
+    e0.record()
+    for _ in range(10):
+        torch.matmul(x, y)
+    e1.record()
+    e1.synchronize()
+    gemm = 10 * 2.0 * m ** 3 / (e0.elapsed_time(e1) * 1e-3) / 1e12
+    del x, y
+    torch.cuda.empty_cache()
+    return {"copy_GBps": copy_gbs, "bf16_gemm_TFLOPs": gemm,
+            "note": "1 GiB device copy (read + write bytes) and an 8192^3 BF16 GEMM via torch.matmul, HIP events"}
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed PMC summary, or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -345,12 +377,16 @@ def main():
                 "frac": achieved / PEAK_HBM, "traffic": measured_traffic(names[dom], args.case),
                 "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel on this "
                                   "workload, scaled by the counters of a device copy of known size"}
+    ceilings = device_ceilings()
+    log(f"device ceilings: copy {ceilings['copy_GBps']:.0f} GB/s, BF16 GEMM {ceilings['bf16_gemm_TFLOPs']:.0f} TFLOP/s")
+    roofline["frac_of_measured_copy"] = achieved / ceilings["copy_GBps"]
     tr = learned["train"]
     ms_nn = timings[4][0]
     flops_nn = tr["bricks"] * 512 * 4 * 27 * 8 * 8 * 2.0   # useful network flops per launch
     nn_roof = {"bound": "mfma", "kernel": "k_nn_apply", "achieved": flops_nn / (ms_nn * 1e-3) / 1e12,
                "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": flops_nn / (ms_nn * 1e-3) / 1e12 / PEAK_BF16,
                "us_per_launch": ms_nn * 1e3, "launches_per_step": per_step[4]}
+    nn_roof["frac_of_measured_gemm"] = nn_roof["achieved"] / ceilings["bf16_gemm_TFLOPs"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -387,6 +423,7 @@ def main():
                    "errors_vs_fp64": errors},
         "roofline": roofline,
         "roofline_network": nn_roof,
+        "device_ceilings": ceilings,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
