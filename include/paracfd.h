@@ -165,8 +165,10 @@ typedef struct pcfd_options {
     int pre_smooth, post_smooth; /* V-cycle Jacobi sweeps */
     int check_interval;       /* Krylov iterations between host convergence reads (>=1);
                                  results do not depend on it */
-    double smoother_weight;   /* damped-Jacobi weight of the V-cycle sweeps (default 6/7;
-                                 the reference's algebraic MG uses 2/3, amr_pressure.cu:1103) */
+    double smoother_weight;   /* damped-Jacobi weight of the first sweep of each pair of V-cycle
+                                 sweeps (sweep s uses smoother_weight for even s, smoother_weight2
+                                 for odd s); default 1/(1.2 + 0.8/sqrt 2) = 0.5664 (the reference's
+                                 algebraic MG uses 2/3 for every sweep, amr_pressure.cu:1103) */
     int learned_bricks;       /* PCFD_PC_LEARNED: bricks the network corrects: 0 all bricks,
                                  1 bricks holding special cells (cut cells, coarse/fine faces)
                                  and their face neighbours, 2 bricks holding cut cells and
@@ -178,6 +180,11 @@ typedef struct pcfd_options {
                                  external_aero_core.h:112, on by default there;
                                  DeviceCompositeCellMomentumTransport::apply_smooth_fabric_wall_model,
                                  amr_advection.cu:1347-1365); 0 (default here): off */
+    double smoother_weight2;  /* weight of the second sweep of each pair; default
+                                 1/(1.2 - 0.8/sqrt 2) = 1.5765: with smoother_weight the roots of the
+                                 degree-2 Chebyshev polynomial on [0.4, 2] of D^-1 A.  Set both equal
+                                 for plain damped Jacobi.  The pair must satisfy
+                                 |(1 - w1 l)(1 - w2 l)| < 1 on (0, 2] */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {

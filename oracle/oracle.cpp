@@ -502,18 +502,21 @@ struct Oracle {
     }
     const std::vector<double>& level_diag(int m) const { return m == 0 ? diag0 : mg[m].diag; }
     bool level_active(int m, int64_t r) const { return m == 0 ? active(r) : true; }
-    void jacobi(int m, std::vector<double>& x, const std::vector<double>& b, int sweeps, double omega, bool zero_start) const {
+    // damped Jacobi sweeps; sweep s has weight omega for even s and omega2 for odd s
+    void jacobi(int m, std::vector<double>& x, const std::vector<double>& b, int sweeps, double omega, bool zero_start,
+                double omega2 = -1.0) const {
         const std::vector<double>& D = level_diag(m);
         const int64_t nn = static_cast<int64_t>(b.size());
         std::vector<double> Ax;
         for (int s = 0; s < sweeps; ++s) {
+            const double w = (s & 1) && omega2 > 0.0 ? omega2 : omega;
             if (s == 0 && zero_start) {
-                for (int64_t r = 0; r < nn; ++r) x[r] = (level_active(m, r) && D[r] > 0) ? omega * b[r] / D[r] : 0.0;
+                for (int64_t r = 0; r < nn; ++r) x[r] = (level_active(m, r) && D[r] > 0) ? w * b[r] / D[r] : 0.0;
                 continue;
             }
             apply_level(m, x, Ax);
             for (int64_t r = 0; r < nn; ++r)
-                if (level_active(m, r) && D[r] > 0) x[r] += omega * (b[r] - Ax[r]) / D[r];
+                if (level_active(m, r) && D[r] > 0) x[r] += w * (b[r] - Ax[r]) / D[r];
         }
     }
     void vcycle(int m, const std::vector<double>& b, std::vector<double>& x) const {
@@ -532,15 +535,14 @@ struct Oracle {
             }
             return;
         }
-        const double omega = opt.smoother_weight;
-        jacobi(m, x, b, opt.pre_smooth, omega, true);
+        jacobi(m, x, b, opt.pre_smooth, opt.smoother_weight, true, opt.smoother_weight2);
         std::vector<double> Ax, r(nn), bc(mg[m + 1].n, 0.0), xc;
         apply_level(m, x, Ax);
         for (int64_t i = 0; i < nn; ++i) r[i] = level_active(m, i) ? b[i] - Ax[i] : 0.0;
         for (int64_t i = 0; i < nn; ++i) if (mg[m].parent[i] >= 0) bc[mg[m].parent[i]] += r[i];
         vcycle(m + 1, bc, xc);
         for (int64_t i = 0; i < nn; ++i) if (mg[m].parent[i] >= 0) x[i] += xc[mg[m].parent[i]];
-        jacobi(m, x, b, opt.post_smooth, omega, false);
+        jacobi(m, x, b, opt.post_smooth, opt.smoother_weight, false, opt.smoother_weight2);
     }
     void precondition(const std::vector<double>& r, std::vector<double>& z) const {
         if (opt.preconditioner == PCFD_PC_JACOBI || mg.size() < 2) {

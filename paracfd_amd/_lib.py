@@ -64,7 +64,7 @@ class Options(C.Structure):
                 ("field_precision", C.c_int), ("vcycle_precision", C.c_int), ("preconditioner", C.c_int),
                 ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int),
                 ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int),
-                ("smooth_wall", C.c_int)]
+                ("smooth_wall", C.c_int), ("smoother_weight2", C.c_double)]
 
 
 NN_C = 8

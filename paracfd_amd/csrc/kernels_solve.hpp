@@ -376,14 +376,14 @@ __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restr
     res[a] = from_f<VT>(v);
 }
 
-// two pre-smoothing sweeps from x = 0:  x1 = w r / D,  x2 = x1 + w (r - A x1) / D
+// two pre-smoothing sweeps from x = 0:  x1 = w1 r / D,  x2 = x1 + w2 (r - A x1) / D
 template <class VT>
 __global__ __launch_bounds__(512, 8) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
                                                  const typename Arith<VT>::type* __restrict__ diag, VT* __restrict__ xout,
-                                                 double omega, const DevScalars* S) {
+                                                 double omega, double omega2, const DevScalars* S) {
     using A = typename Arith<VT>::type;
     PCFD_BRICK_BEGIN;
-    const A w = A(omega);
+    const A w = A(omega), w2 = A(omega2);
     auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rhs[c]) / diag[c]); };
     const TileFetch<A> tf = fetch_tile<A>(M, b, x1);
     const double rb = rhs[a];
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(512, 8) void k_vc_pre2(MeshView M, const double* __
         A dg;
         const A xa = tf.own;
         const A Ax = row0f<A>(M, tile, cf, x1, b, q, m, xa, dg);
-        v = xa + w * (A(rb) - Ax) / dg;
+        v = xa + w2 * (A(rb) - Ax) / dg;
     }
     xout[a] = from_f<VT>(v);
 }
@@ -413,13 +413,13 @@ __global__ __launch_bounds__(512, 8) void k_vc_update_pre2(MeshView M, double* _
                                                         double* __restrict__ rout, const double* __restrict__ d,
                                                         const double* __restrict__ Ad,
                                                         const typename Arith<VT>::type* __restrict__ diag,
-                                                        VT* __restrict__ xout, double omega, double* partials,
-                                                        const DevScalars* S) {
+                                                        VT* __restrict__ xout, double omega, double omega2,
+                                                        double* partials, const DevScalars* S) {
     using A = typename Arith<VT>::type;
     PCFD_BRICK_BEGIN;
     const DevScalars& I = S[M.meta[b].inst];
     const double alpha = I.alpha;
-    const A w = A(omega);
+    const A w = A(omega), w2 = A(omega2);
     auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rin[c] - alpha * Ad[c]) / diag[c]); };
     const TileFetch<A> tf = fetch_tile<A>(M, b, x1);
     const double xi = x[a], di = d[a], ri = rin[a], adi = Ad[a];
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(512, 8) void k_vc_update_pre2(MeshView M, double* _
         A dg;
         const A xa = tf.own;
         const A Ax = row0f<A>(M, tile, cf, x1, b, q, m, xa, dg);
-        v = xa + w * (A(rn) - Ax) / dg;
+        v = xa + w2 * (A(rn) - Ax) / dg;
     }
     xout[a] = from_f<VT>(v);
     store_partial(partials, b, block_sum<512>(rn * rn));
@@ -637,17 +637,17 @@ __device__ __forceinline__ void ell_smooth_row(const CsrView& C, int r, int mode
     xout[r] = from_f<VT>(xr + A(omega) * (br - Ax) / D);
 }
 
-// two sweeps from zero in one pass (x1 = w b / D at the neighbours on the fly)
+// two sweeps from zero in one pass (x1 = w1 b / D at the neighbours on the fly, then weight w2)
 template <class VT, class CT>
 __device__ __forceinline__ void ell_pre2_row(const CsrView& C, int r, const VT* __restrict__ rhs, VT* __restrict__ xout,
-                                             double omega) {
+                                             double omega, double omega2) {
     using A = typename Arith<VT>::type;
     const CT* diag = static_cast<const CT*>(C.diag);
     const A w = A(omega);
     auto x1 = [&](int c) -> A { return round_vt<VT>(w * A(to_f(rhs[c])) / A(diag[c])); };
     const A xr = x1(r);
     const A Ax = ell_row<A, CT>(C, r, x1, xr);
-    xout[r] = from_f<VT>(xr + w * (A(to_f(rhs[r])) - Ax) / A(diag[r]));
+    xout[r] = from_f<VT>(xr + A(omega2) * (A(to_f(rhs[r])) - Ax) / A(diag[r]));
 }
 
 // residual b - A x
@@ -700,9 +700,9 @@ __global__ void k_ell_smooth(CsrView C, int mode, const VT* __restrict__ xin, VT
 }
 
 template <class VT, class CT>
-__global__ void k_ell_pre2(CsrView C, const VT* __restrict__ rhs, VT* __restrict__ xout, double omega) {
+__global__ void k_ell_pre2(CsrView C, const VT* __restrict__ rhs, VT* __restrict__ xout, double omega, double omega2) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < C.n) ell_pre2_row<VT, CT>(C, r, rhs, xout, omega);
+    if (r < C.n) ell_pre2_row<VT, CT>(C, r, rhs, xout, omega, omega2);
 }
 
 template <class VT, class CT>

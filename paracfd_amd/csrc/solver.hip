@@ -422,8 +422,18 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     if (opt_.check_interval < 1) opt_.check_interval = 1;
     if (!(opt_.rho > 0) || !(opt_.cfl > 0) || !(opt_.rtol > 0) || opt_.max_iterations < 1 || opt_.nu < 0)
         throw std::invalid_argument("invalid solver options");
-    if (!(opt_.smoother_weight > 0.0 && opt_.smoother_weight < 2.0))
-        throw std::invalid_argument("smoother_weight must lie in (0, 2)");
+    if (!(opt_.smoother_weight > 0.0) || !(opt_.smoother_weight2 > 0.0))
+        throw std::invalid_argument("smoother weights must be positive");
+    {
+        // the pair of sweeps must damp every mode of D^-1 A (eigenvalues in (0, 2]):
+        // |(1 - w1 l)(1 - w2 l)| < 1 on (0, 2]
+        double worst = 0.0;
+        for (int k = 1; k <= 2000; ++k) {
+            const double l = 2.0 * k / 2000.0;
+            worst = std::max(worst, std::abs((1.0 - opt_.smoother_weight * l) * (1.0 - opt_.smoother_weight2 * l)));
+        }
+        if (!(worst < 1.0)) throw std::invalid_argument("smoother weights do not damp every mode of D^-1 A");
+    }
     if (!(opt_.smagorinsky_cs >= 0.0) || !std::isfinite(opt_.smagorinsky_cs))
         throw std::invalid_argument("smagorinsky_cs must be a non-negative number");
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -887,7 +897,8 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     VT* B = static_cast<VT*>(L.B);
     VT* cur = static_cast<VT*>(L.X);
     VT* oth = static_cast<VT*>(L.T);
-    const double w = opt_.smoother_weight;
+    // sweep s of a smoothing phase has weight w[s & 1] (the two Chebyshev-placed weights by default)
+    const double w[2] = {opt_.smoother_weight, opt_.smoother_weight2};
     const int cb = coarse_block_;
     const unsigned g = blocks_for(L.n, cb);
     if (m == static_cast<int>(lv_.size()) - 1) {
@@ -898,10 +909,10 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
         return cur;
     }
     if (opt_.pre_smooth == 2) {
-        LAUNCH((k_ell_pre2<VT, CT>), g, cb, C, B, cur, w);
+        LAUNCH((k_ell_pre2<VT, CT>), g, cb, C, B, cur, w[0], w[1]);
     } else {
-        LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 0, cur, cur, B, w);
-        for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, w); std::swap(cur, oth); }
+        LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 0, cur, cur, B, w[0]);
+        for (int s = 1; s < opt_.pre_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, w[s & 1]); std::swap(cur, oth); }
     }
     Level& N = lv_[m + 1];
     const CsrView& NC = std::is_same<CT, double>::value ? N.csr64 : N.csr32;
@@ -912,9 +923,9 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
         LAUNCH((k_restrict_range<VT>), blocks_for(N.n, cb), cb, NC, oth, static_cast<VT*>(N.B));
     }
     VT* xc = coarse_cycle_t<VT, CT>(m + 1);
-    LAUNCH((k_ell_post_prolong<VT, CT>), g, cb, C, cur, xc, B, oth, w);
+    LAUNCH((k_ell_post_prolong<VT, CT>), g, cb, C, cur, xc, B, oth, w[0]);
     std::swap(cur, oth);
-    for (int s = 1; s < opt_.post_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, w); std::swap(cur, oth); }
+    for (int s = 1; s < opt_.post_smooth; ++s) { LAUNCH((k_ell_smooth<VT, CT>), g, cb, C, 1, cur, oth, B, w[s & 1]); std::swap(cur, oth); }
     // instances of a batch whose hierarchy ends at this level: their cells take the exact
     // coarse solve (what their own V-cycle does at its last level)
     dense_blocks<VT, CT>(m, B, cur);
@@ -940,35 +951,35 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
         finalize(fin, nb_);
         return;
     }
-    const double w = opt_.smoother_weight;
+    const double w[2] = {opt_.smoother_weight, opt_.smoother_weight2};
     VT* cur = static_cast<VT*>(vc0_[0]);
     VT* oth = static_cast<VT*>(vc0_[1]);
     if (opt_.pre_smooth == 2 && lv_.size() > 1 && fused_) {
         // fused composite level: 2 pre-sweeps | residual + restriction | prolongation + sweep
         using A = typename Arith<VT>::type;
         if (upd_d) {
-            LAUNCH(k_vc_update_pre2<VT>, nb_, 512, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur, w,
-                   partials_, S_);
+            LAUNCH(k_vc_update_pre2<VT>, nb_, 512, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur, w[0],
+                   w[1], partials_, S_);
             finalize(FIN_UPDATE, nb_);
         } else {
-            LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w, S_);
+            LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w[0], w[1], S_);
         }
         LAUNCH(k_vc_restrict0<VT>, nb_, 512, M_, cur, r_, parent0_, static_cast<VT*>(lv_[1].B), S_);
         VT* xc = coarse_cycle<VT>(1);
         const int last = opt_.post_smooth == 1 ? 1 : 0;
-        LAUNCH(k_vc_post_prolong<VT>, nb_, 512, M_, last, cur, xc, parent0_, r_, oth, z_, rprev_, w, partials_, S_);
+        LAUNCH(k_vc_post_prolong<VT>, nb_, 512, M_, last, cur, xc, parent0_, r_, oth, z_, rprev_, w[0], partials_, S_);
         std::swap(cur, oth);
         for (int s = 1; s < opt_.post_smooth; ++s) {
             const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-            LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_);
+            LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
             std::swap(cur, oth);
         }
         pc_tail(fin);
         return;
     }
-    LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w, partials_, S_);
+    LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w[0], partials_, S_);
     for (int s = 1; s < opt_.pre_smooth; ++s) {
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w, partials_, S_);
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
         std::swap(cur, oth);
     }
     if (lv_.size() > 1) {
@@ -980,7 +991,7 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
     }
     for (int s = 0; s < opt_.post_smooth; ++s) {
         const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w, partials_, S_);
+        LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
         std::swap(cur, oth);
     }
     pc_tail(fin);
@@ -1445,13 +1456,13 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
             case 6:  // alpha = 0 and x = z (scratch): the pressure and the residual are untouched
                 if (opt_.vcycle_precision == PCFD_FP32)
                     LAUNCH(k_vc_update_pre2<float>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
-                           static_cast<const float*>(diag_), static_cast<float*>(vc0_[0]), 0.6, partials_, S_);
+                           static_cast<const float*>(diag_), static_cast<float*>(vc0_[0]), 0.6, 0.6, partials_, S_);
                 else if (opt_.vcycle_precision == PCFD_BF16)
                     LAUNCH(k_vc_update_pre2<bf16>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
-                           static_cast<const float*>(diag_), static_cast<bf16*>(vc0_[0]), 0.6, partials_, S_);
+                           static_cast<const float*>(diag_), static_cast<bf16*>(vc0_[0]), 0.6, 0.6, partials_, S_);
                 else
                     LAUNCH(k_vc_update_pre2<double>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
-                           static_cast<const double*>(diag_), static_cast<double*>(vc0_[0]), 0.6, partials_, S_);
+                           static_cast<const double*>(diag_), static_cast<double*>(vc0_[0]), 0.6, 0.6, partials_, S_);
                 break;
             default: throw std::invalid_argument("unknown kernel id");
         }
@@ -1907,7 +1918,10 @@ void pcfd_default_options(pcfd_options* o) {
     o->rho = 1.0; o->nu = 1e-3; o->speed = 1.0; o->cfl = 0.5; o->smagorinsky_cs = 0.0; o->rtol = 1e-8;
     o->max_iterations = 200; o->field_precision = PCFD_FP64; o->vcycle_precision = PCFD_FP64;
     o->preconditioner = PCFD_PC_MG; o->pre_smooth = 2; o->post_smooth = 2; o->check_interval = 2;
-    o->smoother_weight = 6.0 / 7.0;
+    // two damped-Jacobi weights placed at the roots of the degree-2 Chebyshev polynomial on
+    // [0.4, 2], the upper part of the spectrum of D^-1 A that the sweeps must damp
+    o->smoother_weight = 1.0 / (1.2 + 0.8 / std::sqrt(2.0));
+    o->smoother_weight2 = 1.0 / (1.2 - 0.8 / std::sqrt(2.0));
     o->learned_bricks = 0;
     o->learned_capacity = 16;
 }

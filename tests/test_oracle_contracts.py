@@ -311,9 +311,12 @@ def test_true_residual_after_solve(amr_topo, oracle_built):
     assert np.linalg.norm((b - Az)[act]) < 0.7 * np.linalg.norm(b[act])
 
 
-def test_vcycle_is_symmetric(amr_topo, oracle_built):
+@pytest.mark.parametrize("w1,w2", [(6.0 / 7.0, 6.0 / 7.0), (1.0 / (1.2 + 0.8 / math.sqrt(2)), 1.0 / (1.2 - 0.8 / math.sqrt(2)))])
+def test_vcycle_is_symmetric(amr_topo, oracle_built, w1, w2):
+    """Also with two different sweep weights: the pre- and post-sweeps are the same polynomial in
+    D^-1 A, whose factors commute, so the V-cycle stays symmetric positive definite."""
     t = amr_topo
-    orc = Oracle(t, options())
+    orc = Oracle(t, options(smoother_weight=w1, smoother_weight2=w2))
     centre, h, act = geometry(t)
     rng = np.random.default_rng(11)
     x = rng.uniform(-1, 1, t.ndof) * act
@@ -322,6 +325,7 @@ def test_vcycle_is_symmetric(amr_topo, oracle_built):
     My = orc.mg_primitive(3, 0, b=y)
     assert abs(np.dot(Mx, y) - np.dot(x, My)) < 1e-12 * abs(np.dot(Mx, y))
     assert np.dot(Mx, x) > 0
+    assert np.dot(My, y) > 0
 
 
 def test_planted_mistake_in_transport_is_caught(oracle_built):
