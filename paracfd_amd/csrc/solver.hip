@@ -191,22 +191,42 @@ private:
         r_ = rbuf_[0];
         if (flexible_) rprev_ = rbuf_[1];
     }
-    // `chunk_` (even) iterations alternating the two direction buffers
-    void enqueue_chunk() {
-        for (int k = 0; k < chunk_; ++k) cg_iteration(false, dbuf_[(k + 1) & 1], dbuf_[k & 1]);
-    }
-    void capture_chunk() {
-        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueue_chunk();
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(stream_, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
+    // Iteration j of the loop after the first one reads d from dbuf[(j+1)&1] and writes
+    // dbuf[j&1]; with the fused update it reads r from rbuf[(j+1)&1] and writes rbuf[j&1]
+    // (the previous r is r_prev).  The roles are set from j, so any run of iterations can be
+    // enqueued or captured on its own.
+    void set_roles_for(int j) {
+        if (update_fused()) {
+            r_ = rbuf_[(j + 1) & 1];
+            if (flexible_) rprev_ = rbuf_[j & 1];
+        } else {
+            reset_residual_roles();
         }
-        HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
-        HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    }
+    void enqueue_iterations(int count, int j0) {
+        for (int k = 0; k < count; ++k) {
+            const int j = j0 + k;
+            set_roles_for(j);
+            cg_iteration(false, dbuf_[(j + 1) & 1], dbuf_[j & 1]);
+        }
+    }
+    // captured graphs of one or `chunk_` iterations starting at an even or odd j
+    hipGraphExec_t iteration_graph(int count, int parity) {
+        const int kind = count == 1 ? 0 : 1;
+        if (!gexec_[kind][parity]) {
+            HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_iterations(count, parity);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(stream_, &graphs_[kind][parity]));
+            HIP_CHECK(hipGraphInstantiate(&gexec_[kind][parity], graphs_[kind][parity], nullptr, nullptr, 0));
+        }
+        return gexec_[kind][parity];
     }
     // reduction over brick-indexed partials, one finalize workgroup per instance
     void finalize(int op, int nparts) {
@@ -352,13 +372,16 @@ private:
     void snapshot_state();
     void restore_state();
     void drop_graph() {
-        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-        if (graph_) (void)hipGraphDestroy(graph_);
-        graph_exec_ = nullptr;
-        graph_ = nullptr;
+        for (int k = 0; k < 2; ++k)
+            for (int p = 0; p < 2; ++p) {
+                if (gexec_[k][p]) (void)hipGraphExecDestroy(gexec_[k][p]);
+                if (graphs_[k][p]) (void)hipGraphDestroy(graphs_[k][p]);
+                gexec_[k][p] = nullptr;
+                graphs_[k][p] = nullptr;
+            }
     }
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graphs_[2][2] = {};
+    hipGraphExec_t gexec_[2][2] = {};
     // Smagorinsky model (smagorinsky_cs > 0)
     R* nu_ = nullptr;
     SmagView smag_{};
@@ -872,8 +895,7 @@ template <class R>
 SolverImpl<R>::~SolverImpl() {
     // teardown errors cannot be reported from a destructor; they are ignored deliberately
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    drop_graph();
     for (void* p : allocations_) (void)hipFree(p);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     if (host_s_) (void)hipHostFree(host_s_);
@@ -1018,19 +1040,36 @@ bool SolverImpl<R>::pcg_loop(bool watch_stall) {
     LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
     finalize(FIN_RESID, nb_);
     precondition_dispatch(true);
-    // first iteration (beta = 0) outside the replayed chunk; d is double-buffered
+    // first iteration (beta = 0) outside the replayed runs; d is double-buffered
     cg_iteration(true, dbuf_[0], dbuf_[1]);
-    int enqueued = 1;
+    int enqueued = 1, j = 0;
     sync_scalars();
     std::vector<double> hist{host_s_[0].relres};
+    // Runs of chunk_ iterations between host reads; when every unfinished instance is
+    // predicted to converge in the next iteration (geometric rate of its last two reads) a
+    // single iteration runs instead, so that no whole iteration is spent after convergence.
+    std::vector<double> last(ninst_);
+    for (int i = 0; i < ninst_; ++i) last[i] = host_s_[i].relres;
+    int last_count = 0;
     while (!hagg().done && enqueued <= opt_.max_iterations + chunk_) {
-        if (use_graph_) {
-            if (!graph_exec_) capture_chunk();
-            HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
-        } else {
-            enqueue_chunk();
+        int count = chunk_;
+        if (last_count > 0) {
+            bool one = true;
+            for (int i = 0; i < ninst_ && one; ++i) {
+                if (host_s_[i].done) continue;
+                const double now = host_s_[i].relres, rate = std::pow(now / last[i], 1.0 / last_count);
+                if (!(now > 0.0) || !(rate > 0.0 && rate < 1.0) || !std::isfinite(rate)) { one = false; break; }
+                one = now * rate <= opt_.rtol;
+            }
+            if (one) count = 1;
         }
-        enqueued += chunk_;
+        for (int i = 0; i < ninst_; ++i) last[i] = host_s_[i].relres;
+        if (use_graph_) HIP_CHECK(hipGraphLaunch(iteration_graph(count, j & 1), stream_));
+        else enqueue_iterations(count, j);
+        j += count;
+        set_roles_for(j);   // a replayed graph leaves the host-side roles where its capture left them
+        enqueued += count;
+        last_count = count;
         sync_scalars();
         if (watch_stall && !host_s_[0].done) {  // the learned solver runs single cases only
             const double rr = host_s_[0].relres;
