@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PCFD_ABI_VERSION 2
+#define PCFD_ABI_VERSION 3
 
 typedef enum pcfd_status {
     PCFD_OK = 0,
@@ -136,6 +136,22 @@ typedef enum pcfd_array_id {
     PCFD_ARR_PATCH_LSQ_BETA = 22,  /* double[lsq] p_wall = p + sum beta (p_nbr - p) */
     PCFD_ARR_CELL_CENTROID = 23,   /* double[dofs*3] fluid centroid (cell centre for full cells) */
     PCFD_ARR_LEVEL_H = 24,         /* double[levels] */
+    /* non-orthogonal correction (after pcfd_topology_build_nonorthogonal; size -1 before) */
+    PCFD_ARR_NF_LOWER = 25,        /* int32[faces] lower cell along the face's axis */
+    PCFD_ARR_NF_UPPER = 26,        /* int32[faces] upper cell */
+    PCFD_ARR_NF_EDGE = 27,         /* int32[faces] compact edge, or -1-axis for a structured face */
+    PCFD_ARR_NF_AREA = 28,         /* double[faces] open area */
+    PCFD_ARR_NF_DELTA = 29,        /* double[faces] 1/d_axis minus the orthogonal factor */
+    PCFD_ARR_NF_K = 30,            /* double[faces*3] correction vector e_axis - d/d_axis */
+    PCFD_ARR_NF_W = 31,            /* double[faces] gradient weight of the upper cell */
+    PCFD_ARR_NF_LNODE = 32,        /* int32[faces] gradient node of the lower cell or -1 */
+    PCFD_ARR_NF_UNODE = 33,        /* int32[faces] gradient node of the upper cell or -1 */
+    PCFD_ARR_NG_DOF = 34,          /* int32[nodes] cell of each gradient node */
+    PCFD_ARR_NG_OFF = 35,          /* int32[nodes+1] */
+    PCFD_ARR_NG_NBR = 36,          /* int32[entries] stencil cells */
+    PCFD_ARR_NG_W = 37,            /* double[entries*3] g = sum w (p_nbr - p_node) */
+    PCFD_ARR_NG_RANK = 38,         /* uint8[nodes] rank of the stencil's normal matrix */
+    PCFD_ARR_NG_RING = 39,         /* uint8[nodes] 1 + rings added for a deficient stencil */
     PCFD_ARR_MG_ROW_OFF = 100,     /* + 8*m: int32[n_m+1] CSR of multigrid level m (m >= 1) */
     PCFD_ARR_MG_COL = 101,         /* + 8*m: int32[nnz] */
     PCFD_ARR_MG_COEF = 102,        /* + 8*m: double[nnz] off-diagonal couplings (positive) */
@@ -185,6 +201,17 @@ typedef struct pcfd_options {
                                  degree-2 Chebyshev polynomial on [0.4, 2] of D^-1 A.  Set both equal
                                  for plain damped Jacobi.  The pair must satisfy
                                  |(1 - w1 l)(1 - w2 l)| < 1 on (0, 2] */
+    int nonorthogonal;        /* 1: the projection solves the full operator with the
+                                 non-orthogonal correction (centroid two-point factors plus
+                                 the least-squares correction term, amr_pressure.cu:831-834)
+                                 by flexible GMRES right-preconditioned with the V-cycle of the
+                                 orthogonal operator, and the face correction carries the same
+                                 derivative (amr_pressure.cu:2047-2069).  The reference uses
+                                 GCRO-LGMRES with recycling for the same system
+                                 (amr_pressure.cu:1390-1698).  Single cases with the MG
+                                 preconditioner only.  0 (default): the orthogonal operator and
+                                 PCG */
+    int gmres_restart;        /* Krylov vectors per GMRES cycle (default 24, as the reference) */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {
@@ -235,6 +262,12 @@ pcfd_status pcfd_topology_build(const pcfd_geometry_desc* geometry, const pcfd_g
                                 pcfd_topology** out);
 void pcfd_topology_free(pcfd_topology* topology);
 pcfd_status pcfd_topology_get_info(const pcfd_topology* topology, pcfd_topology_info* info);
+/* Build the non-orthogonal correction of the pressure operator into the topology
+ * (finalize_nonorthogonal_pressure_topology, amr_pressure.cpp:63-190): per-face centroid
+ * factors and correction vectors, least-squares gradient stencils with ring extension of
+ * rank-deficient ones, and the check that every correction is represented.  Idempotent;
+ * single-instance topologies only.  Solvers created with options.nonorthogonal = 1 call it. */
+pcfd_status pcfd_topology_build_nonorthogonal(pcfd_topology* topology);
 /* Element count of an exported array (see pcfd_array_id), or -1. */
 int64_t pcfd_topology_array_size(const pcfd_topology* topology, int array_id);
 /* Copy an exported array into dst (bytes must equal count * element size). */

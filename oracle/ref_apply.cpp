@@ -8,9 +8,9 @@
 // level become that level's BrickMetadata (same 8^3 brick numbering, dof = level_offset +
 // brick * 512 + i + 8 j + 64 k, amr_pressure.cpp:213-216), a cell whose +axis face is not a
 // structured face gets the cut bit of that axis, and every compact edge becomes an
-// `embedded` connection with open area A and gradient factor coef / A.  No pressure-gradient
-// nodes are given, so the non-orthogonal terms of apply_cpu vanish and it evaluates the
-// orthogonal operator that the projection solves.
+// `embedded` connection with open area A and gradient factor coef / A.  pcfdref_operator gives
+// no pressure-gradient nodes, so the non-orthogonal terms of apply_cpu vanish and it evaluates
+// the orthogonal operator; pcfdref_full_operator adds the non-orthogonal records.
 #include "core/fluid/amr_grid.h"
 #include "core/fluid/amr_pressure.h"
 
@@ -88,7 +88,87 @@ void build(const Inputs& in, AmrHierarchy& H, CompositeAmrPressureSystem& S) {
     }
 }
 
+// The non-orthogonal records of this project's topology (nf_* faces, ng_* gradient nodes)
+// placed into the reference's structures: a compact face becomes the `embedded` connection of
+// its edge with factor 1/d_axis, correction vector, weight and nodes; a structured face a
+// RegularPressureCorrection; the stencils the pressure_gradient_* arrays.  apply_cpu then
+// evaluates the full operator (amr_pressure.cpp:516-519).
+struct NonOrthInputs {
+    int nfaces;
+    const int* lower;
+    const int* upper;
+    const int* edge;
+    const double* area;
+    const double* delta;
+    const double* k;
+    const double* w;
+    const int* lnode;
+    const int* unode;
+    int nnodes;
+    const int* gdof;
+    const int* goff;
+    const int* gnbr;
+    const double* gw;
+};
+
+void add_nonorth(const NonOrthInputs& n, CompositeAmrPressureSystem& S) {
+    for (int f = 0; f < n.nfaces; ++f) {
+        const Vec3d k{n.k[3 * f], n.k[3 * f + 1], n.k[3 * f + 2]};
+        if (n.edge[f] >= 0) {
+            CoarseFinePressureConnection& c = S.embedded[n.edge[f]];
+            c.coarse_dof = n.lower[f];
+            c.fine_dof = n.upper[f];
+            c.direction = 1;
+            c.orthogonal_gradient_factor += n.delta[f];
+            c.nonorthogonal_correction = k;
+            c.upper_gradient_weight = n.w[f];
+            c.lower_gradient_node = n.lnode[f];
+            c.upper_gradient_node = n.unode[f];
+        } else {
+            RegularPressureCorrection r;
+            r.lower_dof = n.lower[f];
+            r.upper_dof = n.upper[f];
+            r.axis = static_cast<std::int8_t>(-1 - n.edge[f]);
+            r.open_area = n.area[f];
+            r.two_point_delta = n.delta[f];
+            r.nonorthogonal_correction = k;
+            r.upper_gradient_weight = n.w[f];
+            r.lower_gradient_node = n.lnode[f];
+            r.upper_gradient_node = n.unode[f];
+            S.regular_pressure_corrections.push_back(r);
+        }
+    }
+    S.pressure_gradient_dof.assign(n.gdof, n.gdof + n.nnodes);
+    S.pressure_gradient_offset.assign(n.goff, n.goff + n.nnodes + 1);
+    const int entries = n.goff[n.nnodes];
+    S.pressure_gradient_neighbour.assign(n.gnbr, n.gnbr + entries);
+    S.pressure_gradient_weight.resize(entries);
+    for (int q = 0; q < entries; ++q) S.pressure_gradient_weight[q] = Vec3d{n.gw[3 * q], n.gw[3 * q + 1], n.gw[3 * q + 2]};
+}
+
 }  // namespace
+
+extern "C" __attribute__((visibility("default"))) int pcfdref_full_operator(
+    int nlevels, const double* level_h, const int* level_first, const int* brick_nbr, const std::uint8_t* brick_flags,
+    int outlet, const std::uint8_t* mask, int nedges, const int* ea, const int* eb, const double* earea,
+    const double* ecoef, int nfaces, const int* flower, const int* fupper, const int* fedge, const double* farea,
+    const double* fdelta, const double* fk, const double* fw, const int* flnode, const int* funode, int nnodes,
+    const int* gdof, const int* goff, const int* gnbr, const double* gw, const double* p, double* ap) {
+    try {
+        const Inputs in{nlevels, level_h, level_first, brick_nbr, brick_flags, outlet, mask, nedges, ea, eb, earea, ecoef};
+        const NonOrthInputs no{nfaces, flower, fupper, fedge, farea, fdelta, fk, fw, flnode, funode, nnodes, gdof, goff, gnbr, gw};
+        AmrHierarchy H;
+        CompositeAmrPressureSystem S;
+        build(in, H, S);
+        add_nonorth(no, S);
+        std::vector<double> pv(p, p + S.storage_size), out;
+        S.apply_cpu(pv, out);
+        for (int a = 0; a < S.storage_size; ++a) ap[a] = out[a];
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
 
 extern "C" __attribute__((visibility("default"))) int pcfdref_operator(
     int nlevels, const double* level_h, const int* level_first, const int* brick_nbr, const std::uint8_t* brick_flags,

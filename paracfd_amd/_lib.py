@@ -26,6 +26,10 @@ ARR = dict(
     edge_w=(16, np.float64), patch_dof=(17, np.int32), patch_S=(18, np.float64), patch_x=(19, np.float64),
     patch_lsq_off=(20, np.int32), patch_lsq_dof=(21, np.int32), patch_lsq_beta=(22, np.float64),
     centroid=(23, np.float64), level_h=(24, np.float64),
+    nf_lower=(25, np.int32), nf_upper=(26, np.int32), nf_edge=(27, np.int32), nf_area=(28, np.float64),
+    nf_delta=(29, np.float64), nf_k=(30, np.float64), nf_w=(31, np.float64), nf_lnode=(32, np.int32),
+    nf_unode=(33, np.int32), ng_dof=(34, np.int32), ng_off=(35, np.int32), ng_nbr=(36, np.int32),
+    ng_w=(37, np.float64), ng_rank=(38, np.uint8), ng_ring=(39, np.uint8),
 )
 MG_ARR = dict(row_off=(0, np.int32), col=(1, np.int32), coef=(2, np.float64), diag=(3, np.float64),
               parent=(4, np.int32), coarse_inv=(5, np.float64))
@@ -64,7 +68,8 @@ class Options(C.Structure):
                 ("field_precision", C.c_int), ("vcycle_precision", C.c_int), ("preconditioner", C.c_int),
                 ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int),
                 ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int),
-                ("smooth_wall", C.c_int), ("smoother_weight2", C.c_double)]
+                ("smooth_wall", C.c_int), ("smoother_weight2", C.c_double),
+                ("nonorthogonal", C.c_int), ("gmres_restart", C.c_int)]
 
 
 NN_C = 8
@@ -122,7 +127,7 @@ class Loads(C.Structure):
 # every symbol the public header declares
 EXPORTED = [
     "pcfd_last_error", "pcfd_abi_version", "pcfd_topology_build", "pcfd_topology_free", "pcfd_topology_get_info",
-    "pcfd_topology_array_size", "pcfd_topology_export", "pcfd_create", "pcfd_destroy", "pcfd_default_options",
+    "pcfd_topology_array_size", "pcfd_topology_export", "pcfd_topology_build_nonorthogonal", "pcfd_create", "pcfd_destroy", "pcfd_default_options",
     "pcfd_initialize", "pcfd_step", "pcfd_advect", "pcfd_project", "pcfd_apply_pressure_impulse",
     "pcfd_integrate_loads", "pcfd_set_reference", "pcfd_get_velocity", "pcfd_set_velocity", "pcfd_get_pressure",
     "pcfd_set_pressure", "pcfd_get_faces", "pcfd_set_faces", "pcfd_mark_initialized", "pcfd_device_bytes",
@@ -162,6 +167,7 @@ def load():
         "pcfd_topology_get_info": (C.c_int, [P, C.POINTER(TopologyInfo)]),
         "pcfd_topology_array_size": (C.c_int64, [P, C.c_int]),
         "pcfd_topology_export": (C.c_int, [P, C.c_int, P, C.c_size_t]),
+        "pcfd_topology_build_nonorthogonal": (C.c_int, [P]),
         "pcfd_create": (C.c_int, [P, C.POINTER(Options), C.POINTER(P)]),
         "pcfd_destroy": (None, [P]),
         "pcfd_default_options": (None, [C.POINTER(Options)]),
@@ -286,6 +292,14 @@ class Topology:
         out = np.empty(n, dtype=dtype)
         check(self._lib.pcfd_topology_export(self.handle, aid, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
+
+    def build_nonorthogonal(self):
+        """Build the non-orthogonal correction (pcfd_topology_build_nonorthogonal); its arrays
+        nf_* (faces) and ng_* (gradient nodes) are exported afterwards."""
+        check(self._lib.pcfd_topology_build_nonorthogonal(self.handle))
+        for k in [k for k in self._cache if isinstance(k, str) and k[:3] in ("nf_", "ng_")]:
+            del self._cache[k]
+        return self
 
     def array(self, name):
         if name not in self._cache:

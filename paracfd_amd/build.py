@@ -28,7 +28,7 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("topology.cpp", "capi_topology.cpp"):
+    for src in ("topology.cpp", "nonorth.cpp", "capi_topology.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -20,6 +20,20 @@ extern "C" {
 const char* pcfd_last_error(void) { return pcfd::g_last_error.c_str(); }
 int pcfd_abi_version(void) { return PCFD_ABI_VERSION; }
 
+pcfd_status pcfd_topology_build_nonorthogonal(pcfd_topology* topology) {
+    if (!topology) { pcfd::set_error("pcfd_topology_build_nonorthogonal: null topology"); return PCFD_EINVAL; }
+    try {
+        pcfd::build_nonorthogonal(topology->t);
+        return PCFD_OK;
+    } catch (const pcfd::BuildError& e) {
+        pcfd::set_error(e.what());
+        return PCFD_EINVAL;
+    } catch (const std::bad_alloc&) {
+        pcfd::set_error("pcfd_topology_build_nonorthogonal: out of host memory");
+        return PCFD_ENOMEM;
+    }
+}
+
 pcfd_status pcfd_topology_build(const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid, pcfd_topology** out) {
     if (!geometry || !grid || !out) { pcfd::set_error("pcfd_topology_build: null argument"); return PCFD_EINVAL; }
     *out = nullptr;
@@ -138,6 +152,28 @@ bool lookup(const pcfd::Topology& t, int id, ArrayRef& r) {
         case PCFD_ARR_CELL_CENTROID: return set(t.centroid);
         case PCFD_ARR_LEVEL_H: return set(t.level_h);
         default: break;
+    }
+    if (id >= PCFD_ARR_NF_LOWER && id <= PCFD_ARR_NG_RING) {
+        const auto& N = t.nonorth;
+        if (!N.built) return false;
+        switch (id) {
+            case PCFD_ARR_NF_LOWER: return set(N.f_lower);
+            case PCFD_ARR_NF_UPPER: return set(N.f_upper);
+            case PCFD_ARR_NF_EDGE: return set(N.f_edge);
+            case PCFD_ARR_NF_AREA: return set(N.f_area);
+            case PCFD_ARR_NF_DELTA: return set(N.f_delta);
+            case PCFD_ARR_NF_K: return set(N.f_k);
+            case PCFD_ARR_NF_W: return set(N.f_w);
+            case PCFD_ARR_NF_LNODE: return set(N.f_lnode);
+            case PCFD_ARR_NF_UNODE: return set(N.f_unode);
+            case PCFD_ARR_NG_DOF: return set(N.g_dof);
+            case PCFD_ARR_NG_OFF: return set(N.g_off);
+            case PCFD_ARR_NG_NBR: return set(N.g_nbr);
+            case PCFD_ARR_NG_W: return set(N.g_w);
+            case PCFD_ARR_NG_RANK: return set(N.g_rank);
+            case PCFD_ARR_NG_RING: return set(N.g_ring);
+            default: return false;
+        }
     }
     if (id >= PCFD_ARR_MG_ROW_OFF) {
         const int m = (id - PCFD_ARR_MG_ROW_OFF) / 8, kind = (id - PCFD_ARR_MG_ROW_OFF) % 8;

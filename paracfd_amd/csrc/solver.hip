@@ -1961,6 +1961,8 @@ void pcfd_default_options(pcfd_options* o) {
     // [0.4, 2], the upper part of the spectrum of D^-1 A that the sweeps must damp
     o->smoother_weight = 1.0 / (1.2 + 0.8 / std::sqrt(2.0));
     o->smoother_weight2 = 1.0 / (1.2 - 0.8 / std::sqrt(2.0));
+    o->nonorthogonal = 0;
+    o->gmres_restart = 24;
     o->learned_bricks = 0;
     o->learned_capacity = 16;
 }

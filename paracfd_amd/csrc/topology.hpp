@@ -106,7 +106,29 @@ struct Topology {
     std::vector<int64_t> dense_inv_off;
     std::vector<double> dense_inv;
     int instances() const { return static_cast<int>(inst_brick_off.size()) - 1; }
+
+    // Non-orthogonal pressure correction (built on request by build_nonorthogonal).  Face f
+    // between cells lower < upper along `axis` adds to the composite operator the flux
+    //   F = -area (delta (p_upper - p_lower) + k . g_f),  g_f = (1-w) g[lnode] + w g[unode],
+    // out[lower] += F, out[upper] -= F, where g[n] = sum_q gw[q] (p[gnbr[q]] - p[gdof[n]]) is
+    // the weighted least-squares gradient at node n.  nf_edge: compact edge e, or -1-axis for
+    // a structured face (its face velocity is f[axis][lower]).
+    struct NonOrth {
+        bool built = false;
+        std::vector<int> f_lower, f_upper, f_edge, f_lnode, f_unode;
+        std::vector<double> f_area, f_delta, f_k, f_w;   // f_k: 3 per face
+        std::vector<int> g_dof, g_off, g_nbr;
+        std::vector<double> g_w;                        // 3 per stencil entry
+        std::vector<uint8_t> g_rank, g_ring;
+        std::vector<int> inc_dof, inc_off, inc_face;    // per touched dof: f (lower side) or ~f
+        int64_t regular_faces = 0, compact_faces = 0, below_resolution = 0;
+    } nonorth;
 };
+
+// Build the non-orthogonal correction of the pressure operator (idempotent; throws
+// BuildError for a face whose centroids do not bracket it or a correction the gradient
+// stencils cannot represent).  Single-instance topologies only.
+void build_nonorthogonal(Topology& T);
 
 struct BuildError : std::exception {
     std::string message;

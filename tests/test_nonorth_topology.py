@@ -1,0 +1,139 @@
+"""Non-orthogonal correction of the pressure operator: topology records and the oracle's full
+operator (oracle/nonorth.py).
+
+Pinned by what the mathematics fixes and by the reference's code:
+  * the least-squares gradient of every full-rank node reproduces a linear field exactly
+  * for a linear pressure, the full normal derivative of every correction face,
+    (orthogonal factor + delta)(p_u - p_l) + k . g_f, equals the exact derivative G_axis;
+    a planted mistake (the k term dropped) misses it by O(1)
+  * closed forms on coarse/fine faces between full cells: w = h_lower / (h_lower + h_upper) and
+    lateral |k| = 1/3 (fine centres sit h_f / 2 off the coarse centre, d_axis = 3 h_f / 2)
+  * the full operator of the oracle equals the reference's CompositeAmrPressureSystem::apply_cpu
+    (amr_pressure.cpp:498-521) fed with the same records (oracle/_ref, where the reference
+    sources were present at build time), and its orthogonal part equals the reference's
+    orthogonal operator
+  * the full operator annihilates nothing it should not: A 1 = outlet term only (row sums)"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import nonorth as NO
+from paracfd_amd import _lib as L, cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+
+def _case(name):
+    if name == "sphere":
+        g, d = cases.sphere_box(cells_x=32, levels=2, band=0.3)
+    else:
+        g, d = cases.naca_wing(digits=2412, alpha_deg=4.0, cells_x=24, levels=2, bands=(0.3,))
+    return L.Topology(g, d).build_nonorthogonal()
+
+
+@pytest.fixture(scope="module", params=["sphere", "wing"])
+def topo(request):
+    return _case(request.param)
+
+
+def _linear(topo, G=(0.3, -1.7, 2.2)):
+    G = np.array(G)
+    return topo.array("centroid").reshape(-1, 3) @ G, G
+
+
+def test_gradient_is_exact_for_linear_fields(topo):
+    p, G = _linear(topo)
+    g = (NO.gradient_matrix(topo) @ p).reshape(-1, 3)
+    rank = topo.array("ng_rank")
+    assert (rank == 3).all()
+    assert np.abs(g - G).max() < 1e-10
+
+
+@pytest.mark.parametrize("planted", [False, True])
+def test_full_face_derivative_is_exact_for_linear_fields(topo, planted):
+    p, G = _linear(topo)
+    lo, up, edge = topo.array("nf_lower"), topo.array("nf_upper"), topo.array("nf_edge")
+    h = topo.array("level_h")[topo.array("brick_level")]
+    base = np.where(edge >= 0, topo.array("edge_coef")[np.maximum(edge, 0)] / topo.array("edge_area")[np.maximum(edge, 0)],
+                    1.0 / h[lo >> 9])
+    axis = np.where(edge >= 0, topo.array("edge_axis")[np.maximum(edge, 0)], -1 - edge)
+    extra = NO.face_extra_matrix(topo) @ p
+    if planted:
+        extra = topo.array("nf_delta") * (p[up] - p[lo])
+    dpdn = base * (p[up] - p[lo]) + extra
+    err = np.abs(dpdn - G[axis]).max()
+    assert (err > 1e-2) if planted else (err < 1e-9), err
+
+
+def test_coarse_fine_closed_forms(topo):
+    lo, up, edge = topo.array("nf_lower"), topo.array("nf_upper"), topo.array("nf_edge")
+    lev = topo.array("brick_level")
+    h = topo.array("level_h")
+    special = np.zeros(topo.ndof, bool)
+    special[topo.array("special_dof")] = True
+    vol = np.zeros(topo.ndof)
+    vol[topo.array("special_dof")] = topo.array("special_volume")
+    partial = special & (vol > 0) & (vol < (h[lev] ** 3)[np.arange(topo.ndof) >> 9] * (1 - 1e-12))
+    cf = (edge >= 0) & (lev[lo >> 9] != lev[up >> 9]) & ~partial[lo] & ~partial[up]
+    assert cf.sum() > 100
+    hl, hu = h[lev[lo[cf] >> 9]], h[lev[up[cf] >> 9]]
+    assert np.allclose(topo.array("nf_w")[cf], hl / (hl + hu), rtol=0, atol=1e-12)
+    k = topo.array("nf_k").reshape(-1, 3)[cf]
+    axis = topo.array("edge_axis")[edge[cf]]
+    lat = np.abs(k[np.arange(k.shape[0])[:, None], (axis[:, None] + np.array([1, 2])[None, :]) % 3])
+    assert np.allclose(lat, 1.0 / 3.0, atol=1e-12)
+    assert np.abs(k[np.arange(k.shape[0]), axis]).max() < 1e-15
+
+
+def test_row_sums_are_the_outlet_term(topo):
+    A = NO.full_matrix(topo)
+    act = (topo.array("mask") & 64) != 0
+    ones = act.astype(float)
+    r = A @ ones
+    A0 = NO.orthogonal_matrix(topo)
+    assert np.abs(r - A0 @ ones).max() < 1e-12 * np.abs(A0.diagonal()).max()
+
+
+def _ref():
+    if not os.path.exists(LIB):
+        pytest.skip("reference operator library not built (needs the reference sources)")
+    return C.CDLL(LIB)
+
+
+def _ptr(a, ct):
+    return np.ascontiguousarray(a).ctypes.data_as(C.POINTER(ct))
+
+
+def test_full_operator_matches_reference_apply_cpu(topo):
+    lib = _ref()
+    t = topo
+    lev = t.array("brick_level")
+    nl = int(t.info.levels)
+    first = np.searchsorted(lev, np.arange(nl + 1)).astype(np.int32)
+    rng = np.random.default_rng(3)
+    act = (t.array("mask") & 64) != 0
+    p = np.where(act, rng.standard_normal(t.ndof), 0.0)
+    ap = np.zeros(t.ndof)
+    keep = []
+    args = [C.c_int(nl), _ptr(t.array("level_h"), C.c_double), _ptr(first, C.c_int), _ptr(t.array("brick_nbr"), C.c_int),
+            _ptr(t.array("brick_flags"), C.c_uint8), C.c_int(int(t.info.outlet)), _ptr(t.array("mask"), C.c_uint8),
+            C.c_int(t.array("edge_a").size), _ptr(t.array("edge_a"), C.c_int), _ptr(t.array("edge_b"), C.c_int),
+            _ptr(t.array("edge_area"), C.c_double), _ptr(t.array("edge_coef"), C.c_double),
+            C.c_int(t.array("nf_lower").size)]
+    for name, ct in [("nf_lower", C.c_int), ("nf_upper", C.c_int), ("nf_edge", C.c_int), ("nf_area", C.c_double),
+                     ("nf_delta", C.c_double), ("nf_k", C.c_double), ("nf_w", C.c_double), ("nf_lnode", C.c_int),
+                     ("nf_unode", C.c_int)]:
+        args.append(_ptr(t.array(name), ct))
+    args.append(C.c_int(t.array("ng_dof").size))
+    for name, ct in [("ng_dof", C.c_int), ("ng_off", C.c_int), ("ng_nbr", C.c_int), ("ng_w", C.c_double)]:
+        args.append(_ptr(t.array(name), ct))
+    args += [_ptr(p, C.c_double), ap.ctypes.data_as(C.POINTER(C.c_double))]
+    assert lib.pcfdref_full_operator(*args) == 0
+    mine = NO.full_matrix(t) @ p
+    scale = np.abs(ap).max()
+    assert np.abs(mine - ap).max() <= 1e-12 * scale
+    # the correction is not negligible on these cases
+    assert np.abs(mine - NO.orthogonal_matrix(t) @ p).max() > 1e-3 * scale
