@@ -125,11 +125,9 @@ struct Geometry {
 
 }  // namespace
 
-void build_nonorthogonal(Topology& T) {
-    auto& N = T.nonorth;
-    if (N.built) return;
+Topology::NonOrth compute_nonorthogonal(const Topology& T) {
     if (T.instances() > 1) throw BuildError("non-orthogonal correction: single-instance topologies only");
-    N = Topology::NonOrth{};
+    Topology::NonOrth N;
     const Geometry G(T);
     const double resolution = 16.0 * static_cast<double>(FLT_EPSILON);
 
@@ -329,6 +327,11 @@ void build_nonorthogonal(Topology& T) {
     }
     if (!inc.empty()) N.inc_off.push_back(static_cast<int>(inc.size()));
     N.built = true;
+    return N;
+}
+
+void build_nonorthogonal(Topology& T) {
+    if (!T.nonorth.built) T.nonorth = compute_nonorthogonal(T);
 }
 
 }  // namespace pcfd

@@ -22,6 +22,7 @@
 #include "kernels_solve.hpp"
 #include "kernels_step.hpp"
 #include "learned.hpp"
+#include "kernels_nonorth.hpp"
 #include "lsq3.hpp"
 #include "topology.hpp"
 
@@ -276,6 +277,15 @@ private:
     void setup_learned(const Topology& T);
     void setup_smagorinsky(const Topology& T);
     void setup_smooth_wall(const Topology& T);
+    // non-orthogonal operator and its flexible GMRES outer solve
+    void setup_nonorth(const Topology& T);
+    void full_residual(double* r);
+    void mdot(const double* Vb, int nv, const double* w, double* host_out);
+    void fgmres_solve();
+    bool nonorth_ = false;
+    NonOrthView NV_{};
+    double *gm_V_ = nullptr, *gm_Z_ = nullptr, *gm_part_ = nullptr, *gm_dev_ = nullptr;
+    int gm_m_ = 0, gm_its_ = 0;
     // Validation of a freshly trained network: the next 2 kTrial projections alternate learned
     // and plain MGPCG (learned first), so both see the same stage of the flow; if the learned
     // ones take longer in total, the network is switched off.
@@ -710,6 +720,7 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     if (opt_.preconditioner == PCFD_PC_LEARNED) setup_learned(T);
     if (opt_.smagorinsky_cs > 0.0) setup_smagorinsky(T);
     setup_smooth_wall(T);
+    if (opt_.nonorthogonal) setup_nonorth(T);
     brick_level_h_ = T.brick_level;
     brick_coord_h_ = T.brick_coord;
     for (int a = 0; a < 3; ++a) { base_bricks_[a] = T.base_bricks[a]; grid_lo_[a] = T.lo[a]; }
@@ -1102,6 +1113,150 @@ void SolverImpl<R>::aggregate_stats(pcfd_step_stats* st) const {
     *st = a;
 }
 
+// ------------------------------------------------------------------------------------------
+// non-orthogonal operator: records to the device, full residual, flexible GMRES
+// ------------------------------------------------------------------------------------------
+template <class R>
+void SolverImpl<R>::setup_nonorth(const Topology& T) {
+    if (ninst_ != 1) throw std::invalid_argument("the non-orthogonal operator needs a single case (no batch)");
+    if (opt_.preconditioner != PCFD_PC_MG)
+        throw std::invalid_argument("the non-orthogonal operator is solved with the MG preconditioner only");
+    const Topology::NonOrth local = T.nonorth.built ? Topology::NonOrth{} : compute_nonorthogonal(T);
+    const Topology::NonOrth& N = T.nonorth.built ? T.nonorth : local;
+    nonorth_ = true;
+    NV_.nf = static_cast<int>(N.f_lower.size());
+    NV_.nn = static_cast<int>(N.g_dof.size());
+    NV_.ninc = static_cast<int>(N.inc_dof.size());
+    if (NV_.nf > 0) {
+        NV_.lower = upload(N.f_lower); NV_.upper = upload(N.f_upper); NV_.edge = upload(N.f_edge);
+        NV_.lnode = upload(N.f_lnode); NV_.unode = upload(N.f_unode);
+        NV_.area = upload(N.f_area); NV_.delta = upload(N.f_delta); NV_.k = upload(N.f_k); NV_.w = upload(N.f_w);
+        NV_.inc_dof = upload(N.inc_dof); NV_.inc_off = upload(N.inc_off); NV_.inc_face = upload(N.inc_face);
+    }
+    if (NV_.nn > 0) {
+        NV_.gdof = upload(N.g_dof); NV_.goff = upload(N.g_off); NV_.gnbr = upload(N.g_nbr); NV_.gw = upload(N.g_w);
+        NV_.grad = alloc<double>(3 * static_cast<int64_t>(NV_.nn));
+    }
+    gm_m_ = std::min(std::max(opt_.gmres_restart, 2), 64);
+    gm_V_ = alloc<double>(static_cast<int64_t>(gm_m_ + 1) * n_);
+    gm_Z_ = alloc<double>(static_cast<int64_t>(gm_m_) * n_);
+    HIP_CHECK(hipMemset(gm_V_, 0, static_cast<size_t>(gm_m_ + 1) * n_ * sizeof(double)));
+    HIP_CHECK(hipMemset(gm_Z_, 0, static_cast<size_t>(gm_m_) * n_ * sizeof(double)));
+    gm_part_ = alloc<double>(static_cast<int64_t>(gm_m_ + 1) * kGmresBlocks);
+    gm_dev_ = alloc<double>(gm_m_ + 1);
+}
+
+// r = rhs - A p with the full operator
+template <class R>
+void SolverImpl<R>::full_residual(double* r) {
+    LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r, partials_, S_);
+    if (NV_.nf > 0) {
+        if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, p_);
+        LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, p_, r, -1.0);
+    }
+}
+
+// host_out[v] = V[v] . w over active cells, v < nv (consecutive vectors of length n_)
+template <class R>
+void SolverImpl<R>::mdot(const double* Vb, int nv, const double* w, double* host_out) {
+    LAUNCH(k_mdot, dim3(kGmresBlocks, nv), 256, n_, Vb, n_, w, M_.mask, gm_part_);
+    LAUNCH(k_mdot_final, nv, 256, kGmresBlocks, gm_part_, gm_dev_);
+    HIP_CHECK(hipMemcpyAsync(host_out, gm_dev_, nv * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// Flexible GMRES(m), right-preconditioned by the V-cycle of the orthogonal operator (which
+// may differ between applications: FP32/BF16 storage), two-pass classical Gram-Schmidt,
+// Givens rotations on the host.  Converged when the true residual ||rhs - A p|| <= rtol ||rhs||,
+// checked at the end of every cycle (the reference's criterion, amr_pressure.cu:1430-1440).
+template <class R>
+void SolverImpl<R>::fgmres_solve() {
+    const int m = gm_m_;
+    const int64_t n = n_;
+    double* V = gm_V_;
+    double* Z = gm_Z_;
+    const unsigned G = kGmresBlocks;
+    double bb = 0;
+    mdot(rhs_, 1, rhs_, &bb);
+    const double bnorm = std::sqrt(bb);
+    gm_its_ = 0;
+    if (!(bnorm > 0)) {
+        LAUNCH(k_no_status, 1, 64, S_, 0, 1, 0.0);
+        return;
+    }
+    double* r = V;  // the cycle's first basis vector is built in place
+    full_residual(r);
+    double rr = 0;
+    mdot(r, 1, r, &rr);
+    double res = std::sqrt(rr);
+    bool converged = res <= opt_.rtol * bnorm;
+    std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+    double* const r_save = r_;
+    double* const z_save = z_;
+    while (!converged && gm_its_ < opt_.max_iterations && std::isfinite(res)) {
+        LAUNCH(k_scale, G, 256, n, r, 1.0 / res, V, M_.mask);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = res;
+        int j = 0;
+        for (; j < m && gm_its_ < opt_.max_iterations; ++j) {
+            double* vj = V + j * n;
+            double* zj = Z + j * n;
+            double* w = V + (j + 1) * n;
+            r_ = vj;
+            z_ = zj;
+            precondition_dispatch(true);
+            r_ = r_save;
+            z_ = z_save;
+            LAUNCH(k_cg_dir_apply, nb_, 512, M_, zj, zj, dbuf_[0], w, partials_, S_, 1);
+            if (NV_.nf > 0) {
+                if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, zj);
+                LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, zj, w, 1.0);
+            }
+            mdot(V, j + 1, w, h.data());
+            HIP_CHECK(hipMemcpyAsync(gm_dev_, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
+            LAUNCH(k_gs_update, G, 256, n, V, n, j + 1, gm_dev_, w, M_.mask);
+            mdot(V, j + 1, w, h2.data());
+            HIP_CHECK(hipMemcpyAsync(gm_dev_, h2.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
+            LAUNCH(k_gs_update, G, 256, n, V, n, j + 1, gm_dev_, w, M_.mask);
+            double ww = 0;
+            mdot(w, 1, w, &ww);
+            const double hn = std::sqrt(ww);
+            for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
+            H[(j + 1) * m + j] = hn;
+            for (int i = 0; i < j; ++i) {
+                const double a = H[i * m + j], b = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * a + sn[i] * b;
+                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = H[j * m + j], b = H[(j + 1) * m + j];
+            const double rho = std::hypot(a, b);
+            cs[j] = rho > 0 ? a / rho : 1.0;
+            sn[j] = rho > 0 ? b / rho : 0.0;
+            H[j * m + j] = rho;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++gm_its_;
+            const double est = std::fabs(g[j + 1]);
+            if (!(hn > 0) || !std::isfinite(est) || est <= opt_.rtol * bnorm) { ++j; break; }
+            LAUNCH(k_scale, G, 256, n, w, 1.0 / hn, w, M_.mask);
+        }
+        // y = H^-1 g (upper triangular), p += Z y
+        for (int i = j - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int k = i + 1; k < j; ++k) s -= H[i * m + k] * y[k];
+            y[i] = H[i * m + i] != 0.0 ? s / H[i * m + i] : 0.0;
+        }
+        HIP_CHECK(hipMemcpyAsync(gm_dev_, y.data(), j * sizeof(double), hipMemcpyHostToDevice, stream_));
+        LAUNCH(k_combine, G, 256, n, Z, n, j, gm_dev_, p_, M_.mask);
+        full_residual(r);
+        mdot(r, 1, r, &rr);
+        res = std::sqrt(rr);
+        converged = res <= opt_.rtol * bnorm;
+    }
+    LAUNCH(k_no_status, 1, 64, S_, gm_its_, converged ? 1 : 0, res / bnorm);
+}
+
 template <class R>
 void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
@@ -1109,8 +1264,14 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
     const bool learned = nn_ && nn_on_ && !trial_plain();
     set_nn_off(!learned);
-    pcg_loop(learned);
-    int iterations = host_s_[0].iter;
+    if (nonorth_) {
+        sync_scalars();
+        gm_its_ = 0;
+        if (!host_s_[0].zero_rhs) fgmres_solve();
+    } else {
+        pcg_loop(learned);
+    }
+    int iterations = nonorth_ ? gm_its_ : host_s_[0].iter;
     bool fell_back = false;
     if (learned && !host_s_[0].converged) {
         // learned solve stalled or failed: plain MGPCG from its iterate on the same RHS
@@ -1129,6 +1290,10 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
         }
     LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
     if (nedges_) LAUNCH(k_correct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), p_, S_);
+    if (nonorth_ && NV_.nf > 0) {
+        if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, p_);
+        LAUNCH(k_no_correct<R>, blocks_for(NV_.nf, 256), 256, NV_, fields(), p_, S_, opt_.rho);
+    }
     LAUNCH(k_divcheck<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_MAXDIV, nb_);
     sync_scalars();

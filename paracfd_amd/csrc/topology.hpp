@@ -129,6 +129,7 @@ struct Topology {
 // BuildError for a face whose centroids do not bracket it or a correction the gradient
 // stencils cannot represent).  Single-instance topologies only.
 void build_nonorthogonal(Topology& T);
+Topology::NonOrth compute_nonorthogonal(const Topology& T);
 
 struct BuildError : std::exception {
     std::string message;
