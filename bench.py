@@ -10,7 +10,9 @@ sequence from the same freestream start:
   * plain:   FP32 V-cycle MGPCG (the same arithmetic without the network)
   * plain_fp64_vcycle: FP64 V-cycle MGPCG (plain FP64-outer MGPCG, the reference's method)
 and an FP64-everywhere solver (fields, V-cycle, CG; it matches the CPU oracle to 1e-10 in the
-parity tests) gives the errors of the learned run's velocity, pressure and CL/CD.
+parity tests) gives the errors of the learned run's velocity, pressure and CL/CD.  A fifth run,
+nonorthogonal_fgmres, solves the full operator with the non-orthogonal correction (flexible
+GMRES, FP32 V-cycle; the reference's default semantics) and reports its throughput beside.
 A "step" is one solver step on the resident state plus the loads integral.  Prints one JSON
 line.  --gpus above 1 is not supported on this single-GPU build.
 """
@@ -114,7 +116,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run(topo, spec, args, name, field, vcycle, pc):
+def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0):
     """Initialise, pre-steps (with training for the learned solver), warmup, K timed steps."""
     import torch
     from paracfd_amd import ExternalAeroCore, _lib as L, default_options
@@ -122,7 +124,7 @@ def run(topo, spec, args, name, field, vcycle, pc):
                           max_iterations=500, field_precision=field, vcycle_precision=vcycle, preconditioner=pc,
                           pre_smooth=2, post_smooth=2, check_interval=args.check_interval,
                           learned_bricks=args.learned_bricks, learned_capacity=args.collects * args.pairs,
-                          smooth_wall=1 if args.smooth_wall else 0)
+                          smooth_wall=1 if args.smooth_wall else 0, nonorthogonal=nonorthogonal)
     core = ExternalAeroCore(topo, opt)
     core.set_reference(topo.info.reference_area)
     st = core.initialize()
@@ -306,6 +308,8 @@ def main():
                     help="V-cycle storage of the learned and the plain mixed-precision solvers")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-nonorthogonal", dest="nonorthogonal", action="store_false",
+                    help="skip the run with the non-orthogonal operator (flexible GMRES, FP32 V-cycle)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes instead of autotuning them before the timed steps")
     ap.add_argument("--kernel-probe", type=int, default=None,
@@ -342,6 +346,7 @@ def main():
     learned = run(topo, spec, args, "learned", L.FP32, vc, L.PC_LEARNED)
     plain32 = run(topo, spec, args, "plain", L.FP32, vc, L.PC_MG)
     plain64 = run(topo, spec, args, "plain_fp64_vcycle", L.FP32, L.FP64, L.PC_MG)
+    nonorth = run(topo, spec, args, "nonorthogonal_fgmres", L.FP32, vc, L.PC_MG, nonorthogonal=1) if args.nonorthogonal else None
     ref = run(topo, spec, args, "fp64_everywhere", L.FP64, L.FP64, L.PC_MG)
 
     # errors of the learned mixed-precision run against the FP64-everywhere run (same steps)
@@ -418,6 +423,7 @@ def main():
                    "training": {k: tr[k] for k in ("pairs", "train_steps", "loss_first", "loss_last", "train_ms")},
                    "launch_shapes": learned["tuning"],
                    "plain_fp32_vcycle": summary(plain32), "plain_fp64_vcycle": summary(plain64),
+                   "nonorthogonal_fgmres": summary(nonorth) if nonorth else None,
                    "speedup_vs_plain_fp32_vcycle": learned["value"] / plain32["value"],
                    "speedup_vs_plain_fp64_vcycle": learned["value"] / plain64["value"],
                    "errors_vs_fp64": errors},
