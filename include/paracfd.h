@@ -211,7 +211,8 @@ typedef struct pcfd_options {
                                  (amr_pressure.cu:1390-1698).  Single cases with the MG
                                  preconditioner only.  0 (default): the orthogonal operator and
                                  PCG */
-    int gmres_restart;        /* Krylov vectors per GMRES cycle (default 24, as the reference) */
+    int gmres_restart;        /* Krylov vectors per GMRES cycle (default 24, as the reference;
+                                 clamped to 2..30) */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {

@@ -107,16 +107,31 @@ __device__ __forceinline__ double block_reduce_sum(double v) {
     return r;  // valid in thread 0
 }
 
-// partials[v * gridDim.x + block] = sum over active i of V[v][i] * w[i], v = blockIdx.y
-__global__ __launch_bounds__(256) void k_mdot(int64_t n, const double* __restrict__ Vb, int64_t ldv,
+constexpr int kGmresMaxDots = 32;
+
+// partials[v * gridDim.x + block] = sum over active i of V[v][i] * w[i] for v < nv: w and the
+// mask are read once, the nv sums stay in registers (nv <= kGmresMaxDots)
+__global__ __launch_bounds__(256) void k_mdot(int64_t n, const double* __restrict__ Vb, int64_t ldv, int nv,
                                               const double* __restrict__ w, const uint8_t* __restrict__ mask,
                                               double* __restrict__ partials) {
-    const double* v = Vb + static_cast<int64_t>(blockIdx.y) * ldv;
-    double s = 0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
-        if (mask[i] & kActive) s += v[i] * w[i];
-    s = block_reduce_sum<256>(s);
-    if (threadIdx.x == 0) partials[static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    double acc[kGmresMaxDots];
+#pragma unroll
+    for (int v = 0; v < kGmresMaxDots; ++v) acc[v] = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
+        if (!(mask[i] & kActive)) continue;
+        const double wi = w[i];
+#pragma unroll
+        for (int v = 0; v < kGmresMaxDots; ++v)
+            if (v < nv) acc[v] += Vb[v * ldv + i] * wi;
+    }
+#pragma unroll
+    for (int v = 0; v < kGmresMaxDots; ++v) {
+        if (v < nv) {  // nv is uniform over the block, so the barriers below are too
+            const double s = block_reduce_sum<256>(acc[v]);
+            if (threadIdx.x == 0) partials[static_cast<int64_t>(v) * gridDim.x + blockIdx.x] = s;
+            __syncthreads();  // the reduction's shared slots are reused by the next v
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mdot_final(int nparts, const double* __restrict__ partials, double* __restrict__ out) {
@@ -126,16 +141,20 @@ __global__ __launch_bounds__(256) void k_mdot_final(int nparts, const double* __
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-// w -= sum_v h[v] V[v] on active cells
+// w -= sum_v h[v] V[v] on active cells; partials[block] = that block's sum of the new w^2
 __global__ __launch_bounds__(256) void k_gs_update(int64_t n, const double* __restrict__ Vb, int64_t ldv, int nv,
                                                    const double* __restrict__ h, double* __restrict__ w,
-                                                   const uint8_t* __restrict__ mask) {
+                                                   const uint8_t* __restrict__ mask, double* __restrict__ partials) {
+    double ss = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
         if (!(mask[i] & kActive)) { w[i] = 0.0; continue; }
         double s = w[i];
         for (int v = 0; v < nv; ++v) s -= h[v] * Vb[v * ldv + i];
         w[i] = s;
+        ss += s * s;
     }
+    ss = block_reduce_sum<256>(ss);
+    if (threadIdx.x == 0) partials[blockIdx.x] = ss;
 }
 
 // dst = a * src on active cells, 0 elsewhere

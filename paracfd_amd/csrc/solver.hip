@@ -1137,13 +1137,13 @@ void SolverImpl<R>::setup_nonorth(const Topology& T) {
         NV_.gdof = upload(N.g_dof); NV_.goff = upload(N.g_off); NV_.gnbr = upload(N.g_nbr); NV_.gw = upload(N.g_w);
         NV_.grad = alloc<double>(3 * static_cast<int64_t>(NV_.nn));
     }
-    gm_m_ = std::min(std::max(opt_.gmres_restart, 2), 64);
+    gm_m_ = std::min(std::max(opt_.gmres_restart, 2), kGmresMaxDots - 2);
     gm_V_ = alloc<double>(static_cast<int64_t>(gm_m_ + 1) * n_);
     gm_Z_ = alloc<double>(static_cast<int64_t>(gm_m_) * n_);
     HIP_CHECK(hipMemset(gm_V_, 0, static_cast<size_t>(gm_m_ + 1) * n_ * sizeof(double)));
     HIP_CHECK(hipMemset(gm_Z_, 0, static_cast<size_t>(gm_m_) * n_ * sizeof(double)));
-    gm_part_ = alloc<double>(static_cast<int64_t>(gm_m_ + 1) * kGmresBlocks);
-    gm_dev_ = alloc<double>(gm_m_ + 1);
+    gm_part_ = alloc<double>(static_cast<int64_t>(gm_m_ + 2) * kGmresBlocks);
+    gm_dev_ = alloc<double>(gm_m_ + 2);
 }
 
 // r = rhs - A p with the full operator
@@ -1159,7 +1159,8 @@ void SolverImpl<R>::full_residual(double* r) {
 // host_out[v] = V[v] . w over active cells, v < nv (consecutive vectors of length n_)
 template <class R>
 void SolverImpl<R>::mdot(const double* Vb, int nv, const double* w, double* host_out) {
-    LAUNCH(k_mdot, dim3(kGmresBlocks, nv), 256, n_, Vb, n_, w, M_.mask, gm_part_);
+    if (nv < 1 || nv > kGmresMaxDots) throw std::logic_error("GMRES dot count out of range");
+    LAUNCH(k_mdot, kGmresBlocks, 256, n_, Vb, n_, nv, w, M_.mask, gm_part_);
     LAUNCH(k_mdot_final, nv, 256, kGmresBlocks, gm_part_, gm_dev_);
     HIP_CHECK(hipMemcpyAsync(host_out, gm_dev_, nv * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1190,7 +1191,7 @@ void SolverImpl<R>::fgmres_solve() {
     mdot(r, 1, r, &rr);
     double res = std::sqrt(rr);
     bool converged = res <= opt_.rtol * bnorm;
-    std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+    std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 1);
     double* const r_save = r_;
     double* const z_save = z_;
     while (!converged && gm_its_ < opt_.max_iterations && std::isfinite(res)) {
@@ -1212,15 +1213,28 @@ void SolverImpl<R>::fgmres_solve() {
                 if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, zj);
                 LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, zj, w, 1.0);
             }
-            mdot(V, j + 1, w, h.data());
-            HIP_CHECK(hipMemcpyAsync(gm_dev_, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
-            LAUNCH(k_gs_update, G, 256, n, V, n, j + 1, gm_dev_, w, M_.mask);
-            mdot(V, j + 1, w, h2.data());
-            HIP_CHECK(hipMemcpyAsync(gm_dev_, h2.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
-            LAUNCH(k_gs_update, G, 256, n, V, n, j + 1, gm_dev_, w, M_.mask);
-            double ww = 0;
-            mdot(w, 1, w, &ww);
-            const double hn = std::sqrt(ww);
+            // classical Gram-Schmidt; w = V[j+1] follows V[j], so one pass gives V[i].w and w.w.
+            // With a V-cycle close to A^-1, A M^-1 v lies nearly in the span and the projection
+            // always removes most of w, so Kahan's 0.7 test would re-orthogonalise every time
+            // (half the cycle's memory traffic).  A second pass runs only when w almost vanished;
+            // the true residual at the end of every cycle keeps the stopping test honest.
+            mdot(V, j + 2, w, h.data());
+            double before = std::sqrt(std::max(h[j + 1], 0.0)), hn = 0;
+            auto project_out = [&](const double* coef) {
+                HIP_CHECK(hipMemcpyAsync(gm_dev_, coef, (j + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
+                LAUNCH(k_gs_update, G, 256, n, V, n, j + 1, gm_dev_, w, M_.mask, gm_part_);
+                LAUNCH(k_mdot_final, 1, 256, kGmresBlocks, gm_part_, gm_dev_);
+                double ww = 0;
+                HIP_CHECK(hipMemcpyAsync(&ww, gm_dev_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+                HIP_CHECK(hipStreamSynchronize(stream_));
+                return std::sqrt(ww);
+            };
+            hn = project_out(h.data());
+            std::fill(h2.begin(), h2.end(), 0.0);
+            if (hn < 1e-6 * before) {
+                mdot(V, j + 1, w, h2.data());
+                hn = project_out(h2.data());
+            }
             for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
             H[(j + 1) * m + j] = hn;
             for (int i = 0; i < j; ++i) {
