@@ -282,7 +282,11 @@ private:
     void full_residual(double* r);
     void mdot(const double* Vb, int nv, const double* w, double* host_out);
     void fgmres_solve();
+    void fgmres_direction(int j);
+    void enqueue_fgmres_direction(int j);
     bool nonorth_ = false;
+    std::vector<hipGraph_t> gm_graph_;
+    std::vector<hipGraphExec_t> gm_exec_;
     NonOrthView NV_{};
     double *gm_V_ = nullptr, *gm_Z_ = nullptr, *gm_part_ = nullptr, *gm_dev_ = nullptr;
     int gm_m_ = 0, gm_its_ = 0;
@@ -382,6 +386,12 @@ private:
     void snapshot_state();
     void restore_state();
     void drop_graph() {
+        for (size_t j = 0; j < gm_exec_.size(); ++j) {
+            if (gm_exec_[j]) (void)hipGraphExecDestroy(gm_exec_[j]);
+            if (gm_graph_[j]) (void)hipGraphDestroy(gm_graph_[j]);
+            gm_exec_[j] = nullptr;
+            gm_graph_[j] = nullptr;
+        }
         for (int k = 0; k < 2; ++k)
             for (int p = 0; p < 2; ++p) {
                 if (gexec_[k][p]) (void)hipGraphExecDestroy(gexec_[k][p]);
@@ -1144,6 +1154,59 @@ void SolverImpl<R>::setup_nonorth(const Topology& T) {
     HIP_CHECK(hipMemset(gm_Z_, 0, static_cast<size_t>(gm_m_) * n_ * sizeof(double)));
     gm_part_ = alloc<double>(static_cast<int64_t>(gm_m_ + 2) * kGmresBlocks);
     gm_dev_ = alloc<double>(gm_m_ + 2);
+    gm_graph_.assign(gm_m_, nullptr);
+    gm_exec_.assign(gm_m_, nullptr);
+}
+
+// Z[j] = M^-1 V[j] (the V-cycle reads r_ and writes z_), V[j+1] = A_full Z[j]
+template <class R>
+void SolverImpl<R>::enqueue_fgmres_direction(int j) {
+    const int64_t n = n_;
+    double* vj = gm_V_ + j * n;
+    double* zj = gm_Z_ + j * n;
+    double* w = gm_V_ + (j + 1) * n;
+    double* const r_save = r_;
+    double* const z_save = z_;
+    r_ = vj;
+    z_ = zj;
+    try {
+        precondition_dispatch(true);
+    } catch (...) {
+        r_ = r_save;
+        z_ = z_save;
+        throw;
+    }
+    r_ = r_save;
+    z_ = z_save;
+    LAUNCH(k_cg_dir_apply, nb_, 512, M_, zj, zj, dbuf_[0], w, partials_, S_, 1);
+    if (NV_.nf > 0) {
+        if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, zj);
+        LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, zj, w, 1.0);
+    }
+}
+
+// the same from a graph captured once per basis index (launch shapes are baked in; the
+// graphs are dropped with the PCG graphs when the tuning changes)
+template <class R>
+void SolverImpl<R>::fgmres_direction(int j) {
+    if (!use_graph_) {
+        enqueue_fgmres_direction(j);
+        return;
+    }
+    if (!gm_exec_[j]) {
+        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_fgmres_direction(j);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(stream_, &gm_graph_[j]));
+        HIP_CHECK(hipGraphInstantiate(&gm_exec_[j], gm_graph_[j], nullptr, nullptr, 0));
+    }
+    HIP_CHECK(hipGraphLaunch(gm_exec_[j], stream_));
 }
 
 // r = rhs - A p with the full operator
@@ -1192,27 +1255,14 @@ void SolverImpl<R>::fgmres_solve() {
     double res = std::sqrt(rr);
     bool converged = res <= opt_.rtol * bnorm;
     std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 1);
-    double* const r_save = r_;
-    double* const z_save = z_;
     while (!converged && gm_its_ < opt_.max_iterations && std::isfinite(res)) {
         LAUNCH(k_scale, G, 256, n, r, 1.0 / res, V, M_.mask);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = res;
         int j = 0;
         for (; j < m && gm_its_ < opt_.max_iterations; ++j) {
-            double* vj = V + j * n;
-            double* zj = Z + j * n;
             double* w = V + (j + 1) * n;
-            r_ = vj;
-            z_ = zj;
-            precondition_dispatch(true);
-            r_ = r_save;
-            z_ = z_save;
-            LAUNCH(k_cg_dir_apply, nb_, 512, M_, zj, zj, dbuf_[0], w, partials_, S_, 1);
-            if (NV_.nf > 0) {
-                if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, zj);
-                LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, zj, w, 1.0);
-            }
+            fgmres_direction(j);
             // classical Gram-Schmidt; w = V[j+1] follows V[j], so one pass gives V[i].w and w.w.
             // With a V-cycle close to A^-1, A M^-1 v lies nearly in the span and the projection
             // always removes most of w, so Kahan's 0.7 test would re-orthogonalise every time
