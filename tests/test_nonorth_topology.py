@@ -137,3 +137,36 @@ def test_full_operator_matches_reference_apply_cpu(topo):
     assert np.abs(mine - ap).max() <= 1e-12 * scale
     # the correction is not negligible on these cases
     assert np.abs(mine - NO.orthogonal_matrix(t) @ p).max() > 1e-3 * scale
+
+
+def test_records_exist_only_after_the_build_and_batches_are_refused():
+    g, d = cases.naca_wing(digits=2412, alpha_deg=4.0, cells_x=12, levels=2, bands=(0.3,))
+    t = L.Topology(g, d)
+    with pytest.raises(KeyError):
+        t.array("nf_lower")
+    t.build_nonorthogonal()
+    n = t.array("nf_lower").size
+    t.build_nonorthogonal()          # idempotent
+    assert t.array("nf_lower").size == n
+    assert t.array("ng_off").size == t.array("ng_dof").size + 1
+    merged = L.Topology.merge([L.Topology(g, d), L.Topology(g, d)])
+    with pytest.raises(L.PcfdError, match="single-instance"):
+        merged.build_nonorthogonal()
+
+
+def test_records_are_structurally_sound(topo):
+    lo, up, edge = topo.array("nf_lower"), topo.array("nf_upper"), topo.array("nf_edge")
+    act = (topo.array("mask") & 64) != 0
+    assert act[lo].all() and act[up].all()
+    w = topo.array("nf_w")
+    assert ((w > 0) & (w < 1)).all()
+    # every compact edge between active cells has exactly one record; structured records once each
+    ea, eb = topo.array("edge_a"), topo.array("edge_b")
+    comp = edge[edge >= 0]
+    assert np.array_equal(np.sort(comp), np.flatnonzero(act[ea] & act[eb]))
+    reg = np.stack([lo[edge < 0], -1 - edge[edge < 0]], axis=1)
+    assert np.unique(reg, axis=0).shape[0] == reg.shape[0]
+    # k has no component along its face's axis (1 - d_a * (1/d_a): zero to rounding)
+    k = topo.array("nf_k").reshape(-1, 3)
+    axis = np.where(edge >= 0, topo.array("edge_axis")[np.maximum(edge, 0)], -1 - edge)
+    assert np.abs(k[np.arange(k.shape[0]), axis]).max() <= 4e-16
