@@ -2,9 +2,9 @@
 """Benchmark: cell-steps/s of the full solver step (MUSCL transport + MGPCG projection to
 rtol 1e-8 + surface loads) on one MI355X, learned preconditioner vs plain MGPCG.
 
-Default workload: NACA 2412 wing, alpha 4 deg, 2-level static AMR of 8^3 bricks (~4.1 M
-active cells), FP32 fields, FP64 outer conjugate gradients.  Three solvers run the same step
-sequence from the same freestream start:
+Default workload: NACA 2412 wing, alpha 4 deg, 3-level static AMR of 8^3 bricks (19,840,144
+active cells, refinement bands 1.0 and 0.25 chord), FP32 fields, FP64 outer conjugate
+gradients.  Three solvers run the same step sequence from the same freestream start:
   * learned: FP32 V-cycle + the learned network correction (BF16 MFMA) in flexible PCG, after
     untimed data collection and training on the solver's own MGPCG residuals  -> `value`
   * plain:   FP32 V-cycle MGPCG (the same arithmetic without the network)
@@ -14,7 +14,11 @@ parity tests) gives the errors of the learned run's velocity, pressure and CL/CD
 nonorthogonal_fgmres, solves the full operator with the non-orthogonal correction (flexible
 GMRES, FP32 V-cycle; the reference's default semantics) and reports its throughput beside.
 A "step" is one solver step on the resident state plus the loads integral.  Prints one JSON
-line.  --gpus above 1 is not supported on this single-GPU build.
+line.  `roofline` is the hot kernel with the largest time per step, timed in-step: HIP events
+around each of its launches on the solver's stream during steps that follow the timed ones
+(pcfd_probe_start/stop; the iterations are then launched one by one instead of replayed from
+graphs).  The timed steps and the probe steps sit in roctx ranges ("timed:<solver>",
+"probe:<solver>") for rocprofv3 --marker-trace.  --gpus above 1 is not supported on this single-GPU build.
 """
 from __future__ import annotations
 
@@ -34,6 +38,51 @@ PEAK_BF16 = 2500.0     # TFLOP/s dense
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class _Roctx:
+    """roctx ranges for rocprofv3 --marker-trace (no-ops when the library is absent)."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+
+    def push(self, name):
+        if self.lib is not None:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib is not None:
+            self.lib.roctxRangePop()
+
+
+ROCTX = _Roctx()
+
+
+def host_cpu():
+    """Model name and thread counts of the host the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = None
+    return {"cpu_model": model, "host_logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
 def cpu_baseline(topo, options, budget_s):
@@ -61,9 +110,11 @@ def cpu_baseline(topo, options, budget_s):
             break
     orc.close()
     n = topo.n_active
-    return {"value": n * steps / elapsed, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} step(s) after initialization of the same {n}-cell case, serial FP64 oracle "
-                      f"({elapsed:.1f} s timed)"}
+    out = {"value": n * steps / elapsed, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{steps} step(s) after initialization of the same {n}-cell case, serial FP64 oracle "
+                     f"({elapsed:.1f} s timed)"}
+    out.update(host_cpu())
+    return out
 
 
 def device_ceilings():
@@ -103,12 +154,13 @@ def measured_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(kernel)
+            entries = json.load(f).get("entries", [])
     except (OSError, ValueError):
         return None
-    if not rec or rec.get("workload") != workload:
-        return None
-    return rec["hbm_bytes_per_launch"]
+    for rec in entries:
+        if rec.get("kernel") == kernel and rec.get("workload") == workload:
+            return rec["hbm_bytes_per_launch"]
+    return None
 
 
 def rel_l2(a, b):
@@ -161,6 +213,7 @@ def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0):
         core.pressure_loads()
     torch.cuda.synchronize()
     its, fallbacks, used, conv = [], 0, 0, True
+    ROCTX.push(f"timed:{name}")
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = core.step()
@@ -171,10 +224,12 @@ def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0):
         conv = conv and bool(st["converged"])
     torch.cuda.synchronize()
     seconds = time.perf_counter() - t0
+    ROCTX.pop()
     n = topo.n_active
     res = {"core": core, "value": n * args.steps / seconds, "ms_per_step": 1e3 * seconds / args.steps,
            "mean_its": sum(its) / len(its), "fallback_steps": fallbacks, "converged": conv, "loads": loads,
-           "train": train, "learned_steps": used, "learned_rejected": bool(st["learned_rejected"]), "tuning": tune}
+           "train": train, "learned_steps": used, "learned_rejected": bool(st["learned_rejected"]), "tuning": tune,
+           "substeps": st["diffusion_substeps"]}
     log(f"{name}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} ms/step, mean its {res['mean_its']:.1f}, "
         f"fallbacks {fallbacks}, network used in {used} steps (rejected: {res['learned_rejected']}), "
         f"CL {loads['cl']:.6f} CD {loads['cd']:.6f}")
@@ -293,7 +348,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--case", default="naca2412_2lvl")
+    ap.add_argument("--case", default="naca2412_3lvl")
+    ap.add_argument("--probe-steps", type=int, default=10,
+                    help="steps after the timed ones during which the hot kernels are timed in-step")
     ap.add_argument("--check-interval", type=int, default=2)
     ap.add_argument("--learned-bricks", type=int, default=2,
                     help="0 all bricks, 1 bricks with special cells and neighbours, 2 bricks with cut cells and neighbours")
@@ -359,38 +416,70 @@ def main():
               "cd_rel": abs(learned["loads"]["cd"] - ref["loads"]["cd"]) / max(abs(ref["loads"]["cd"]), 1e-30)}
     log(f"errors vs FP64: {errors}")
 
-    # roofline: per-launch time of each hot kernel on the learned solver's stream (HIP events)
+    # roofline: in-step duration of each hot kernel on the learned solver's stream (HIP events
+    # around every launch during probe steps that follow the timed ones), and the same kernels
+    # launched back to back in isolation for comparison
     core = learned["core"]
     mean_its = learned["mean_its"]
-    # launches per step: one direction update and one fused x/r update + pre-smoothing per PCG
-    # iteration, one last post-sweep (k_smooth0) per V-cycle, one network correction per V-cycle
-    per_step = {5: mean_its, 6: mean_its, 1: mean_its, 2: 1.0, 4: mean_its + 1}
     vtn = "float" if args.vcycle == "fp32" else "bf16"
     names = {5: "k_cg_dir_apply", 6: f"k_vc_update_pre2<{vtn}>", 1: f"k_smooth0<{vtn}>", 2: "k_advect<float>",
              4: "k_nn_apply"}
+    probe_ids = (1, 2, 5, 6)
+    torch.cuda.synchronize()
+    ROCTX.push("probe:learned")
+    core.probe_start(probe_ids)
+    probe_its = []
+    for _ in range(args.probe_steps):
+        probe_its.append(core.step()["iterations"])
+        core.pressure_loads()
+    probed = core.probe_stop()
+    torch.cuda.synchronize()
+    ROCTX.pop()
     timings = {}
-    for k in per_step:
-        ms, by = core.time_kernel(k, reps=20)
-        timings[k] = (ms, by, ms * per_step[k])
-        log(f"kernel {names[k]}: {ms * 1e3:.1f} us/launch, {by / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic, "
-            f"{ms * per_step[k]:.2f} ms/step")
+    for k in probe_ids + (4,):
+        ms_iso, by = core.time_kernel(k, reps=20)
+        launches, total = probed.get(k, (0, 0.0))
+        ms = total / launches if launches else ms_iso
+        per_step = launches / max(args.probe_steps, 1) if k != 4 else mean_its + 1
+        timings[k] = {"ms": ms, "ms_isolated": ms_iso, "bytes": by, "launches": launches,
+                      "launches_per_step": per_step, "ms_per_step": ms * per_step}
+        log(f"kernel {names[k]}: in-step {ms * 1e3:.1f} us/launch over {launches} launches "
+            f"({per_step:.1f}/step, {ms * per_step:.2f} ms/step), isolated {ms_iso * 1e3:.1f} us, "
+            f"{by / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
     hbm = {k: v for k, v in timings.items() if k != 4}
-    dom = max(hbm, key=lambda k: hbm[k][2])
-    ms, by, _ = timings[dom]
-    achieved = by / (ms * 1e-3) / 1e9
+    dom = max(hbm, key=lambda k: hbm[k]["ms_per_step"])
+    t = timings[dom]
+    achieved = t["bytes"] / (t["ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM, "traffic": measured_traffic(names[dom], args.case),
+                "algorithmic_bytes_per_launch": t["bytes"], "us_per_launch_in_step": t["ms"] * 1e3,
+                "launches_timed": t["launches"], "launches_per_step": t["launches_per_step"],
+                "us_per_launch_isolated": t["ms_isolated"] * 1e3,
+                "timing": f"HIP events around each launch on the solver's stream during {args.probe_steps} steps "
+                          "after the timed ones (iterations launched one by one)",
                 "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel on this "
                                   "workload, scaled by the counters of a device copy of known size"}
+    # whole step: algorithmic bytes per active cell of one step (each pass touching its arrays
+    # once; FP32 fields, FP32 V-cycle) over the measured step time
+    subs = learned["substeps"]
+    n_patch = topo.info.patches
+    b_cell = (18 + 43 + 24 * subs + 26 + 30 + mean_its * (98 + 114) + 34 + 38 + 16 + 80.0 * n_patch / topo.n_active)
+    step_gbs = b_cell * topo.n_active / (learned["ms_per_step"] * 1e-3) / 1e9
+    step_roofline = {"bytes_per_cell": b_cell, "achieved": step_gbs, "peak": PEAK_HBM, "unit": "GB/s",
+                     "frac": step_gbs / PEAK_HBM,
+                     "model": "18 CFL + 43 transport + 24 per diffusion substep + 26 reconstruction + 30 RHS + "
+                              "iterations x (98 PCG + 114 FP32 V-cycle) + 34 correction + 38 impulse + 16 divergence "
+                              "check + 80 per surface patch, bytes per active cell"}
+    log(f"step bytes model: {b_cell:.0f} B/cell -> {step_gbs:.0f} GB/s over the measured step ({step_gbs / PEAK_HBM:.3f} of peak)")
     ceilings = device_ceilings()
     log(f"device ceilings: copy {ceilings['copy_GBps']:.0f} GB/s, BF16 GEMM {ceilings['bf16_gemm_TFLOPs']:.0f} TFLOP/s")
     roofline["frac_of_measured_copy"] = achieved / ceilings["copy_GBps"]
     tr = learned["train"]
-    ms_nn = timings[4][0]
+    ms_nn = timings[4]["ms"]
     flops_nn = tr["bricks"] * 512 * 4 * 27 * 8 * 8 * 2.0   # useful network flops per launch
     nn_roof = {"bound": "mfma", "kernel": "k_nn_apply", "achieved": flops_nn / (ms_nn * 1e-3) / 1e12,
                "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": flops_nn / (ms_nn * 1e-3) / 1e12 / PEAK_BF16,
-               "us_per_launch": ms_nn * 1e3, "launches_per_step": per_step[4]}
+               "us_per_launch": ms_nn * 1e3, "launches_per_step": timings[4]["launches_per_step"]}
     nn_roof["frac_of_measured_gemm"] = nn_roof["achieved"] / ceilings["bf16_gemm_TFLOPs"]
 
     cpu = None
@@ -428,6 +517,7 @@ def main():
                    "speedup_vs_plain_fp64_vcycle": learned["value"] / plain64["value"],
                    "errors_vs_fp64": errors},
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "roofline_network": nn_roof,
         "device_ceilings": ceilings,
         "cpu_baseline": cpu,

@@ -397,6 +397,16 @@ pcfd_status pcfd_mg_primitive(pcfd_solver* solver, int op, int level, const doub
  * Leaves the Krylov loop flags in the "finished" state; call between steps. */
 pcfd_status pcfd_time_kernel(pcfd_solver* solver, int kernel, int reps, double* avg_ms, double* bytes_per_launch);
 
+/* In-step launch timing, no counterpart in the reference: between pcfd_probe_start and
+ * pcfd_probe_stop every launch of the kernels in `kernel_mask` (bit k for kernel id k of
+ * pcfd_time_kernel; ids 1, 2, 5 and 6) made by the calls in between (pcfd_step, ...) is
+ * bracketed by a pair of HIP events on the solver's stream.  While probing, the Krylov
+ * iterations are launched one by one instead of replayed from captured graphs (same
+ * arithmetic, same results).  pcfd_probe_stop returns, per kernel id 0..7, the number of
+ * timed launches and their summed duration in ms (arrays of 8, either may be NULL). */
+pcfd_status pcfd_probe_start(pcfd_solver* solver, unsigned kernel_mask);
+pcfd_status pcfd_probe_stop(pcfd_solver* solver, int64_t* launches, double* total_ms);
+
 /* ---- Learned preconditioner (PCFD_PC_LEARNED) --------------------------------------------
  * No counterpart in the reference, whose projection runs plain (GCRO / PCG) Krylov with an
  * algebraic V-cycle (amr_pressure.cu:1103-1353).  A 4-layer 3^3 conv net per brick (8

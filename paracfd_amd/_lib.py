@@ -135,7 +135,8 @@ EXPORTED = [
     "pcfd_learned_get_weights", "pcfd_learned_set_weights", "pcfd_learned_enable", "pcfd_learned_forward",
     "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
     "pcfd_integrate_loads_instance", "pcfd_set_reference_instance", "pcfd_integrate_loads_batch",
-    "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning", "pcfd_sample_slice",
+    "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning", "pcfd_sample_slice", "pcfd_probe_start",
+    "pcfd_probe_stop",
 ]
 
 _lib = None
@@ -188,6 +189,8 @@ def load():
         "pcfd_device_bytes": (C.c_int64, [P]),
         "pcfd_mg_primitive": (C.c_int, [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int]),
         "pcfd_time_kernel": (C.c_int, [P, C.c_int, C.c_int, dp, dp]),
+        "pcfd_probe_start": (C.c_int, [P, C.c_uint]),
+        "pcfd_probe_stop": (C.c_int, [P, C.POINTER(C.c_int64), dp]),
         "pcfd_learned_collect": (C.c_int, [P, C.c_int, C.POINTER(LearnedStats)]),
         "pcfd_learned_train": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.POINTER(LearnedStats)]),
         "pcfd_learned_get_weights": (C.c_int, [P, fp, fp]),

@@ -124,6 +124,21 @@ class ExternalAeroCore:
         L.check(self._lib.pcfd_time_kernel(self.handle, kernel, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def probe_start(self, kernels):
+        """Time every launch of the given hot kernels (ids of time_kernel: 1, 2, 5, 6) in the
+        calls that follow, with HIP events on the solver's stream."""
+        mask = 0
+        for k in kernels:
+            mask |= 1 << int(k)
+        L.check(self._lib.pcfd_probe_start(self.handle, mask))
+
+    def probe_stop(self):
+        """{kernel id: (launches, total ms)} of the launches timed since probe_start."""
+        n = (C.c_int64 * 8)()
+        ms = (C.c_double * 8)()
+        L.check(self._lib.pcfd_probe_stop(self.handle, n, ms))
+        return {k: (int(n[k]), float(ms[k])) for k in range(8) if n[k] > 0}
+
     def mg_primitive(self, op, level, x=None, b=None, omega=2.0 / 3.0, sweeps=1):
         n = self.topology.mg_n(level)
         out = np.empty(n)

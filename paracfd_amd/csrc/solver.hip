@@ -84,6 +84,8 @@ public:
     virtual int instances() const = 0;
     virtual void sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
                               int nu, int nv, float* out) = 0;
+    virtual void probe_start(unsigned mask) = 0;
+    virtual void probe_stop(int64_t* launches, double* total_ms) = 0;
     bool initialized = false;
     double time = 0.0;
     int64_t device_bytes = 0;
@@ -168,10 +170,72 @@ private:
         F.fe = fe_;
         return F;
     }
+    // In-step launch timing (pcfd_probe_start / pcfd_probe_stop): while kernel `id` is in
+    // probe_mask_, every launch of it is bracketed by a pair of HIP events on the solver's
+    // stream; the pairs are read back when the pool is full and when probing stops.
+    static constexpr int kProbeKernels = 8, kProbePool = 512;
+    unsigned probe_mask_ = 0;
+    bool probe_saved_graph_ = true;
+    std::vector<hipEvent_t> probe_ev_;
+    std::vector<int> probe_ev_id_;   // kernel of each event pair
+    int probe_used_ = 0;             // pairs recorded since the last harvest
+    int64_t probe_launches_[kProbeKernels] = {};
+    double probe_ms_[kProbeKernels] = {};
+    void probe_harvest() {
+        if (!probe_used_) return;
+        HIP_CHECK(hipEventSynchronize(probe_ev_[2 * probe_used_ - 1]));
+        for (int k = 0; k < probe_used_; ++k) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, probe_ev_[2 * k], probe_ev_[2 * k + 1]));
+            probe_ms_[probe_ev_id_[k]] += ms;
+            ++probe_launches_[probe_ev_id_[k]];
+        }
+        probe_used_ = 0;
+    }
+    template <class F>
+    void probed(int id, F&& launch) {
+        if (!(probe_mask_ & (1u << id))) {
+            launch();
+            return;
+        }
+        if (probe_used_ == kProbePool) probe_harvest();
+        HIP_CHECK(hipEventRecord(probe_ev_[2 * probe_used_], stream_));
+        launch();
+        HIP_CHECK(hipEventRecord(probe_ev_[2 * probe_used_ + 1], stream_));
+        probe_ev_id_[probe_used_++] = id;
+    }
+public:
+    void probe_start(unsigned mask) override {
+        if (mask == 0 || (mask & ~((1u << 1) | (1u << 2) | (1u << 5) | (1u << 6))))
+            throw std::invalid_argument("probe mask: kernels 1, 2, 5 and 6 can be probed");
+        if (probe_mask_) throw std::invalid_argument("a probe is already running");
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        if (probe_ev_.empty()) {
+            probe_ev_.assign(2 * kProbePool, nullptr);
+            probe_ev_id_.assign(kProbePool, 0);
+            for (auto& e : probe_ev_) HIP_CHECK(hipEventCreate(&e));
+        }
+        for (int k = 0; k < kProbeKernels; ++k) { probe_launches_[k] = 0; probe_ms_[k] = 0.0; }
+        probe_used_ = 0;
+        probe_saved_graph_ = use_graph_;
+        use_graph_ = false;  // the iterations are launched one by one while probing
+        probe_mask_ = mask;
+    }
+    void probe_stop(int64_t* launches, double* total_ms) override {
+        if (!probe_mask_) throw std::invalid_argument("no probe is running");
+        probe_harvest();
+        probe_mask_ = 0;
+        use_graph_ = probe_saved_graph_;
+        for (int k = 0; k < kProbeKernels; ++k) {
+            if (launches) launches[k] = probe_launches_[k];
+            if (total_ms) total_ms[k] = probe_ms_[k];
+        }
+    }
+private:
     // One PCG iteration: d_new = z + beta d_old with A d_new fused, alpha, x/r update,
     // preconditioner, beta.  With the fused V-cycle the x/r update runs inside its first pass.
     void cg_iteration(bool initial, double* dold, double* dnew) {
-        LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0);
+        probed(5, [&] { LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0); });
         finalize(FIN_APPLY, nb_);
         if (update_fused()) {
             double* rin = r_;
@@ -919,6 +983,7 @@ SolverImpl<R>::~SolverImpl() {
     drop_graph();
     for (void* p : allocations_) (void)hipFree(p);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    for (auto& e : probe_ev_) if (e) (void)hipEventDestroy(e);
     if (host_s_) (void)hipHostFree(host_s_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1001,8 +1066,10 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
         // fused composite level: 2 pre-sweeps | residual + restriction | prolongation + sweep
         using A = typename Arith<VT>::type;
         if (upd_d) {
-            LAUNCH(k_vc_update_pre2<VT>, nb_, 512, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur, w[0],
-                   w[1], partials_, S_);
+            probed(6, [&] {
+                LAUNCH(k_vc_update_pre2<VT>, nb_, 512, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur,
+                       w[0], w[1], partials_, S_);
+            });
             finalize(FIN_UPDATE, nb_);
         } else {
             LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w[0], w[1], S_);
@@ -1014,7 +1081,7 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
         std::swap(cur, oth);
         for (int s = 1; s < opt_.post_smooth; ++s) {
             const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-            LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
+            probed(1, [&] { LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_); });
             std::swap(cur, oth);
         }
         pc_tail(fin);
@@ -1416,7 +1483,7 @@ void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
 // face reconstruction
 template <class R>
 void SolverImpl<R>::advect_core() {
-    LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_);
+    probed(2, [&] { LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); });
     for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
     const bool smag = opt_.smagorinsky_cs > 0;
     if (smag) {
@@ -2333,6 +2400,14 @@ pcfd_status pcfd_mg_primitive(pcfd_solver* s, int op, int level, const double* x
 pcfd_status pcfd_time_kernel(pcfd_solver* s, int kernel, int reps, double* avg_ms, double* bytes_per_launch) {
     NEED(s && avg_ms && bytes_per_launch, "pcfd_time_kernel: null argument");
     return guarded("pcfd_time_kernel", [&] { s->impl->time_kernel(kernel, reps, avg_ms, bytes_per_launch); });
+}
+pcfd_status pcfd_probe_start(pcfd_solver* s, unsigned kernel_mask) {
+    NEED(s, "pcfd_probe_start: null solver");
+    return guarded("pcfd_probe_start", [&] { s->impl->probe_start(kernel_mask); });
+}
+pcfd_status pcfd_probe_stop(pcfd_solver* s, int64_t* launches, double* total_ms) {
+    NEED(s, "pcfd_probe_stop: null solver");
+    return guarded("pcfd_probe_stop", [&] { s->impl->probe_stop(launches, total_ms); });
 }
 pcfd_status pcfd_learned_collect(pcfd_solver* s, int max_pairs, pcfd_learned_stats* st) {
     NEED(s, "pcfd_learned_collect: null solver");
