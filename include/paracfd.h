@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PCFD_ABI_VERSION 3
+#define PCFD_ABI_VERSION 4
 
 typedef enum pcfd_status {
     PCFD_OK = 0,
@@ -127,13 +127,17 @@ typedef enum pcfd_array_id {
     PCFD_ARR_EDGE_AXIS = 13,       /* int8[edges] */
     PCFD_ARR_EDGE_AREA = 14,       /* double[edges] open area */
     PCFD_ARR_EDGE_COEF = 15,       /* double[edges] open area / centre distance */
-    PCFD_ARR_EDGE_W = 16,          /* double[edges] interpolation weight of the upper cell */
+    PCFD_ARR_EDGE_W = 16,          /* double[edges] interpolation weight of the upper cell at the face
+                                      centroid (composite_face_interpolation_weight,
+                                      amr_advection.cu:732-753) */
     PCFD_ARR_PATCH_DOF = 17,       /* int32[patches] */
     PCFD_ARR_PATCH_S = 18,         /* double[patches*3] area vector, fluid -> solid */
     PCFD_ARR_PATCH_X = 19,         /* double[patches*3] patch centroid */
-    PCFD_ARR_PATCH_LSQ_OFF = 20,   /* int32[patches+1] */
-    PCFD_ARR_PATCH_LSQ_DOF = 21,   /* int32[lsq] */
-    PCFD_ARR_PATCH_LSQ_BETA = 22,  /* double[lsq] p_wall = p + sum beta (p_nbr - p) */
+    PCFD_ARR_PATCH_D = 20,         /* double[patches*9] first moment of the wall about the cell
+                                      centroid, int_W (x - c) (x) n dA, row-major: the wall force of
+                                      a pressure p + g.(x - c) is p S + D^T g */
+    PCFD_ARR_EDGE_X = 21,          /* double[edges*3] centroid of the face's open part */
+    PCFD_ARR_EDGE_DCOEF = 22,      /* double[edges] diffusion conductance */
     PCFD_ARR_CELL_CENTROID = 23,   /* double[dofs*3] fluid centroid (cell centre for full cells) */
     PCFD_ARR_LEVEL_H = 24,         /* double[levels] */
     /* non-orthogonal correction (after pcfd_topology_build_nonorthogonal; size -1 before) */

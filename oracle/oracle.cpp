@@ -66,9 +66,9 @@ typedef struct oracle_topology {
     const int* patch_dof;
     const double* patch_S;
     const double* patch_x;
-    const int* patch_lsq_off;
-    const int* patch_lsq_dof;
-    const double* patch_lsq_beta;
+    const double* patch_D;           /* 9 per patch: int_W (x - c) (x) n dA */
+    const double* edge_x;            /* 3 per edge: face centroid of the open part */
+    const double* edge_dcoef;        /* diffusion conductance */
     double finest_h;
     double max_diffusion_coefficient;
     int mg_levels;
@@ -95,13 +95,16 @@ struct Oracle {
     std::vector<int> brick_level, brick_nbr;
     std::vector<uint8_t> brick_flags, mask;
     std::vector<int> special_of;     // dof -> special or -1
+    std::vector<int64_t> sdof;       // special -> dof
     std::vector<double> special_volume;
     std::vector<int> special_patch;
     std::vector<int> ea, eb;
     std::vector<int8_t> eaxis;
     std::vector<double> earea, ecoef, ew;
-    std::vector<int> patch_dof, lsq_off, lsq_dof;
-    std::vector<double> patch_S, patch_x, lsq_beta;
+    std::vector<int> patch_dof;
+    std::vector<double> patch_S, patch_x, patch_D;
+    std::vector<double> edcoef;      // diffusion conductance per edge
+    std::vector<double> edisp;       // 6 per edge: x_f - c_a, x_f - c_b
     std::vector<double> centroid;
     double finest_h = 0, max_diff_coef = 0;
     // Smagorinsky least-squares gradient stencils of the special cells (built on first use)
@@ -118,6 +121,15 @@ struct Oracle {
     double time = 0;
     bool initialized = false;
     std::vector<double> diag0;  // composite diagonal
+    // Planted mistake for the contract tests (oracle_plant): 0 none; 1 CFL rate without the
+    // x-min inflow face; 2 structured diffusion conductance h^2 instead of h; 3 diffusion
+    // substeps rounded down; 4 RHS without the 1/dt; 5 outlet correction over h instead of h/2;
+    // 6 wall force p_a S + g.(x_p - c_a) S (the point form at the patch point, without the wall
+    // moment); 7 compact face pressure without the skew term; 8 compact reconstruction without
+    // the transverse correction; 9 compact transport donor-only; 10 compact transport slope
+    // without the (1 - C) factor; 11 gradient stencil weight area/|d| instead of area/|d|^2;
+    // 12 wall traction in the impulse without D^T g (the loads keep it).
+    int defect = 0;
 
     double h_of(int64_t a) const { return level_h[brick_level[a / BC]]; }
     double vol(int64_t a) const {
@@ -164,7 +176,7 @@ struct Oracle {
                 out[a] += std::max(f, 0.0);
                 out[b] += std::max(-f, 0.0);
             }
-            if (at_xmin(a)) out[a] += std::max(-A * opt.speed, 0.0);
+            if (at_xmin(a) && defect != 1) out[a] += std::max(-A * opt.speed, 0.0);
             if (outlet && at_xmax(a)) out[a] += std::max(A * fx[a], 0.0);
         }
         for (size_t e = 0; e < ea.size(); ++e) {
@@ -208,11 +220,29 @@ struct Oracle {
                 else d[0][a] -= swept * opt.speed;
             }
         }
+        // compact faces: the donor's value reconstructed to the face centroid with its
+        // least-squares gradient, scaled by 1 - C (C = min(1, |swept| / V_donor)) and clamped
+        // to the range of the two cell values (compact_muscl_cell_face_value,
+        // amr_advection.cu:94-105); gradients of the velocity before the transport
+        std::vector<double> g;
+        velocity_gradients(g);
         for (size_t e = 0; e < ea.size(); ++e) {
             const double swept = dt * earea[e] * fe[e];
-            const int64_t donor = swept >= 0 ? ea[e] : eb[e];
+            const bool lower = swept >= 0;
+            const int64_t donor = lower ? ea[e] : eb[e];
+            const int sp = special_of[donor];
+            const double C = std::min(1.0, std::abs(swept) / vol(donor));
+            const double* dsp = &edisp[6 * e + (lower ? 0 : 3)];
             for (int c = 0; c < 3; ++c) {
-                const double f = swept * comp(c, donor);
+                const double va = comp(c, ea[e]), vb = comp(c, eb[e]);
+                double fv = lower ? va : vb;
+                if (sp >= 0 && defect != 9) {
+                    const double* gc = &g[9 * static_cast<size_t>(sp) + 3 * c];
+                    const double slope = defect == 10 ? 1.0 : 1.0 - C;
+                    const double rec = fv + slope * (gc[0] * dsp[0] + gc[1] * dsp[1] + gc[2] * dsp[2]);
+                    fv = std::max(std::min(va, vb), std::min(std::max(va, vb), rec));
+                }
+                const double f = swept * fv;
                 d[c][ea[e]] -= f;
                 d[c][eb[e]] += f;
             }
@@ -278,17 +308,15 @@ struct Oracle {
             if (special_of[ea[e]] >= 0) inc[special_of[ea[e]]].push_back(static_cast<int>(e));
             if (special_of[eb[e]] >= 0) inc[special_of[eb[e]]].push_back(~static_cast<int>(e));
         }
-        std::vector<int64_t> dof_of(ns);
-        for (int64_t a = 0; a < n; ++a) if (special_of[a] >= 0) dof_of[special_of[a]] = a;
         for (int sp = 0; sp < ns; ++sp) {
-            const int64_t a = dof_of[sp];
+            const int64_t a = sdof[sp];
             double N[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             auto add = [&](int64_t b, double area) {
                 double d[3];
                 for (int c = 0; c < 3; ++c) d[c] = centroid[3 * b + c] - centroid[3 * a + c];
                 const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (!(area > 0) || !(d2 > 1e-20)) throw std::runtime_error("invalid gradient stencil geometry");
-                const double w = area / d2;
+                const double w = defect == 11 ? area / std::sqrt(d2) : area / d2;
                 sg_nbr.push_back(static_cast<int>(b));
                 for (int c = 0; c < 3; ++c) sg_wd.push_back(w * d[c]);
                 for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) N[3 * r + c] += w * d[r] * d[c];
@@ -307,6 +335,41 @@ struct Oracle {
             pinv3(N, &sg_inv[9 * static_cast<size_t>(sp)]);
         }
     }
+    // Least-squares gradient at every special cell of the current velocity (9 per cell,
+    // d v_c / d x_r at 3 c + r; accumulate_special_cell_gradient + solve_special_cell_gradient,
+    // amr_advection.cu:206-216) or of the pressure (3 per cell; the same stencil applied to p,
+    // as reconstruct_scalar_pressure_gradient applies the pressure topology's, :400-405).
+    void velocity_gradients(std::vector<double>& g) const {
+        const int ns = static_cast<int>(special_volume.size());
+        g.assign(9 * static_cast<size_t>(ns), 0.0);
+        for (int sp = 0; sp < ns; ++sp) {
+            const int64_t a = sdof[sp];
+            double rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int k = sg_off[sp]; k < sg_off[sp + 1]; ++k)
+                for (int c = 0; c < 3; ++c) {
+                    const double dv = comp(c, sg_nbr[k]) - comp(c, a);
+                    for (int r = 0; r < 3; ++r) rhs[3 * c + r] += sg_wd[3 * k + r] * dv;
+                }
+            const double* M = &sg_inv[9 * static_cast<size_t>(sp)];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r)
+                    g[9 * static_cast<size_t>(sp) + 3 * c + r] = M[3 * r] * rhs[3 * c] + M[3 * r + 1] * rhs[3 * c + 1] + M[3 * r + 2] * rhs[3 * c + 2];
+        }
+    }
+    void pressure_gradients(std::vector<double>& g) const {
+        const int ns = static_cast<int>(special_volume.size());
+        g.assign(3 * static_cast<size_t>(ns), 0.0);
+        for (int sp = 0; sp < ns; ++sp) {
+            const int64_t a = sdof[sp];
+            double rhs[3] = {0, 0, 0};
+            for (int k = sg_off[sp]; k < sg_off[sp + 1]; ++k) {
+                const double dp = p[sg_nbr[k]] - p[a];
+                for (int r = 0; r < 3; ++r) rhs[r] += sg_wd[3 * k + r] * dp;
+            }
+            const double* M = &sg_inv[9 * static_cast<size_t>(sp)];
+            for (int r = 0; r < 3; ++r) g[3 * static_cast<size_t>(sp) + r] = M[3 * r] * rhs[0] + M[3 * r + 1] * rhs[1] + M[3 * r + 2] * rhs[2];
+        }
+    }
     static double strain_rate(const double* g) {  // sqrt(2 S:S), S = sym(grad v)
         const double sxx = g[0], syy = g[4], szz = g[8];
         const double sxy = 0.5 * (g[1] + g[3]), sxz = 0.5 * (g[2] + g[6]), syz = 0.5 * (g[5] + g[7]);
@@ -317,7 +380,6 @@ struct Oracle {
     void viscosity(std::vector<double>& nu) {
         nu.assign(n, 0.0);
         const double cs = opt.smagorinsky_cs;
-        if (cs > 0 && sg_off.empty()) build_gradient_stencils();
         for (int64_t a = 0; a < n; ++a) {
             if (!active(a)) continue;
             nu[a] = opt.nu;
@@ -365,7 +427,7 @@ struct Oracle {
             }
         }
         for (size_t e = 0; e < ea.size(); ++e) {
-            const double k = ecoef[e] * 0.5 * (nu[ea[e]] + nu[eb[e]]);
+            const double k = edcoef[e] * 0.5 * (nu[ea[e]] + nu[eb[e]]);
             rate[ea[e]] += k; rate[eb[e]] += k;
         }
         double mx = 0.0;
@@ -378,6 +440,7 @@ struct Oracle {
             return std::max(1, static_cast<int>(std::ceil(dt * diffusion_rate(nu_field) / 0.7)));
         }
         if (!(opt.nu > 0)) return 0;
+        if (defect == 3) return std::max(1, static_cast<int>(std::floor(dt * opt.nu * max_diff_coef / 0.7)));
         return std::max(1, static_cast<int>(std::ceil(dt * opt.nu * max_diff_coef / 0.7)));
     }
     std::vector<double> nu_field;
@@ -396,7 +459,7 @@ struct Oracle {
                 for (int axis = 0; axis < 3; ++axis) {
                     const int64_t b = snbr(a, axis, 1);
                     if (b < 0) continue;
-                    const double k = dts * (var ? 0.5 * (nu_field[a] + nu_field[b]) : opt.nu) * h;
+                    const double k = dts * (var ? 0.5 * (nu_field[a] + nu_field[b]) : opt.nu) * (defect == 2 ? h * h : h);
                     for (int c = 0; c < 3; ++c) {
                         const double m = k * (comp(c, b) - comp(c, a));
                         d[c][a] += m; d[c][b] -= m;
@@ -404,7 +467,7 @@ struct Oracle {
                 }
             }
             for (size_t e = 0; e < ea.size(); ++e) {
-                const double k = dts * (var ? 0.5 * (nu_field[ea[e]] + nu_field[eb[e]]) : opt.nu) * ecoef[e];
+                const double k = dts * (var ? 0.5 * (nu_field[ea[e]] + nu_field[eb[e]]) : opt.nu) * edcoef[e];
                 for (int c = 0; c < 3; ++c) {
                     const double m = k * (comp(c, eb[e]) - comp(c, ea[e]));
                     d[c][ea[e]] += m; d[c][eb[e]] -= m;
@@ -426,9 +489,25 @@ struct Oracle {
             }
             if (outlet && at_xmax(a)) fx[a] = vx[a];
         }
+        // compact faces: the weighted mean plus the transverse correction both cells' gradients
+        // support, minmod(g_a . s, g_b . s) with the skew s = x_f - ((1-w) c_a + w c_b)
+        // (reconstruct_compact_cell_flux_kernel, amr_advection.cu:268-293)
+        std::vector<double> g;
+        velocity_gradients(g);
         for (size_t e = 0; e < ea.size(); ++e) {
             const int ax = eaxis[e];
-            fe[e] = (1.0 - ew[e]) * comp(ax, ea[e]) + ew[e] * comp(ax, eb[e]);
+            const double w = ew[e];
+            const double* da = &edisp[6 * e];
+            const double* db = da + 3;
+            const double skew[3] = {(1.0 - w) * da[0] + w * db[0], (1.0 - w) * da[1] + w * db[1], (1.0 - w) * da[2] + w * db[2]};
+            double corr[2] = {0.0, 0.0};
+            for (int side = 0; side < 2; ++side) {
+                const int sp = special_of[side ? eb[e] : ea[e]];
+                if (sp < 0) continue;
+                const double* gc = &g[9 * static_cast<size_t>(sp) + 3 * ax];
+                corr[side] = gc[0] * skew[0] + gc[1] * skew[1] + gc[2] * skew[2];
+            }
+            fe[e] = (1.0 - w) * comp(ax, ea[e]) + w * comp(ax, eb[e]) + (defect == 8 ? 0.0 : minmod(corr[0], corr[1]));
         }
     }
 
@@ -598,7 +677,7 @@ struct Oracle {
                 const int64_t b = snbr(a, axis, 1);
                 if (b >= 0) face(axis, a) -= s * (p[b] - p[a]) / h;
             }
-            if (outlet && at_xmax(a)) fx[a] += s * p[a] / (0.5 * h);
+            if (outlet && at_xmax(a)) fx[a] += s * p[a] / (defect == 5 ? h : 0.5 * h);
         }
         for (size_t e = 0; e < ea.size(); ++e) fe[e] -= s * (p[eb[e]] - p[ea[e]]) * (ecoef[e] / earea[e]);
     }
@@ -609,20 +688,38 @@ struct Oracle {
         for (int64_t a = 0; a < n; ++a) if (active(a)) m = std::max(m, std::abs(div[a] / vol(a)));
         return m;
     }
-    SolveResult project(double dt, bool warm, double* maxdiv) {
-        std::vector<double> div, b(n, 0.0);
+    // RHS = -(rho/dt) * integrated divergence (normalize_divergence_rhs, amr_pressure.cu:255-284)
+    void rhs(double dt, std::vector<double>& b) const {
+        std::vector<double> div;
         integrated_divergence(div);
-        for (int64_t a = 0; a < n; ++a) b[a] = active(a) ? -(opt.rho / dt) * div[a] : 0.0;
+        b.assign(n, 0.0);
+        const double scale = defect == 4 ? opt.rho : opt.rho / dt;
+        for (int64_t a = 0; a < n; ++a) b[a] = active(a) ? -scale * div[a] : 0.0;
+    }
+    SolveResult project(double dt, bool warm, double* maxdiv) {
+        std::vector<double> b;
+        rhs(dt, b);
         SolveResult r = pcg(b, p, warm);
         correct(dt);
         if (maxdiv) *maxdiv = max_abs_divergence();
         return r;
     }
-    double wall_pressure(int pch) const {
+    // wall force of patch pch for the pressure p_a + g.(x - c_a) on the wall: p_a S + D^T g
+    // (D = int_W (x - c_a) (x) n dA; for a planar patch with centroid x_p this is the
+    // reference's (p_a + g.(x_p - c_a)) S, amr_advection.cu:391-399, aero_loads.cpp:38-40)
+    void wall_force_p(int pch, const std::vector<double>& gp, double pref, double* F) const {
         const int64_t a = patch_dof[pch];
-        double pw = p[a];
-        for (int q = lsq_off[pch]; q < lsq_off[pch + 1]; ++q) pw += lsq_beta[q] * (p[lsq_dof[q]] - p[a]);
-        return pw;
+        const int sp = special_of[a];
+        const double* g = &gp[3 * static_cast<size_t>(sp)];
+        const double* D = &patch_D[9 * static_cast<size_t>(pch)];
+        if (defect == 6) {
+            double pw = p[a] - pref;
+            for (int r = 0; r < 3; ++r) pw += g[r] * (patch_x[3 * pch + r] - centroid[3 * a + r]);
+            for (int c = 0; c < 3; ++c) F[c] = pw * patch_S[3 * pch + c];
+            return;
+        }
+        for (int c = 0; c < 3; ++c)
+            F[c] = (p[a] - pref) * patch_S[3 * pch + c] + D[c] * g[0] + D[3 + c] * g[1] + D[6 + c] * g[2];
     }
     // Smooth-wall law (Spalding, kappa 0.41, B 5.2) inverted for the friction velocity by 36
     // bisections of u+ in [0, 80] with u+ y+(u+) = u_t y / nu (amr_pressure.h:41-53)
@@ -724,14 +821,29 @@ struct Oracle {
             if (k == 0 && (f & BF_ZMIN)) d[2][a] += bnd;
             if (k == 7 && (f & BF_ZMAX)) d[2][a] -= bnd;
         }
+        // compact faces: the weighted face pressure plus skew . ((1-w) g_a + w g_b)
+        // (pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390); walls: p S + D^T g
+        std::vector<double> gp;
+        pressure_gradients(gp);
         for (size_t e = 0; e < ea.size(); ++e) {
-            const double pf = (1.0 - ew[e]) * p[ea[e]] + ew[e] * p[eb[e]];
+            const double w = ew[e];
+            double pf = (1.0 - w) * p[ea[e]] + w * p[eb[e]];
+            const int sa = special_of[ea[e]], sb = special_of[eb[e]];
+            if (sa >= 0 && sb >= 0 && defect != 7) {
+                const double* da = &edisp[6 * e];
+                const double* db = da + 3;
+                for (int c = 0; c < 3; ++c)
+                    pf += ((1.0 - w) * da[c] + w * db[c]) * ((1.0 - w) * gp[3 * static_cast<size_t>(sa) + c] + w * gp[3 * static_cast<size_t>(sb) + c]);
+            }
             const double imp = s * earea[e] * pf;
             d[eaxis[e]][ea[e]] -= imp; d[eaxis[e]][eb[e]] += imp;
         }
         for (size_t pch = 0; pch < patch_dof.size(); ++pch) {
-            const double pw = wall_pressure(static_cast<int>(pch));
-            for (int c = 0; c < 3; ++c) d[c][patch_dof[pch]] -= s * pw * patch_S[3 * pch + c];
+            double F[3];
+            wall_force_p(static_cast<int>(pch), gp, 0.0, F);
+            if (defect == 12)
+                for (int c = 0; c < 3; ++c) F[c] = p[patch_dof[pch]] * patch_S[3 * pch + c];
+            for (int c = 0; c < 3; ++c) d[c][patch_dof[pch]] -= s * F[c];
         }
         for (int64_t a = 0; a < n; ++a) {
             if (!active(a)) continue;
@@ -741,14 +853,17 @@ struct Oracle {
     }
     void loads(double pref, pcfd_loads* out) const {
         std::memset(out, 0, sizeof(*out));
+        std::vector<double> gp;
+        pressure_gradients(gp);
         for (size_t pch = 0; pch < patch_dof.size(); ++pch) {
-            const double pw = wall_pressure(static_cast<int>(pch)) - pref;
+            double Fp[3];
+            wall_force_p(static_cast<int>(pch), gp, pref, Fp);
             double r[3];
             for (int c = 0; c < 3; ++c) r[c] = patch_x[3 * pch + c] - origin[c];
             for (int part = 0; part < 2; ++part) {
                 double F[3];
                 for (int c = 0; c < 3; ++c)
-                    F[c] = part == 0 ? pw * patch_S[3 * pch + c] : (wall_force.empty() ? 0.0 : wall_force[3 * pch + c]);
+                    F[c] = part == 0 ? Fp[c] : (wall_force.empty() ? 0.0 : wall_force[3 * pch + c]);
                 double* f = part == 0 ? out->force : out->viscous_force;
                 double* m = part == 0 ? out->moment : out->viscous_moment;
                 for (int c = 0; c < 3; ++c) f[c] += F[c];
@@ -799,6 +914,7 @@ void* oracle_create(const oracle_topology* t, const pcfd_options* o) {
         s->mask.assign(t->mask, t->mask + t->ndof);
         s->special_of.assign(t->ndof, -1);
         for (int64_t i = 0; i < t->nspecial; ++i) s->special_of[t->special_dof[i]] = static_cast<int>(i);
+        s->sdof.assign(t->special_dof, t->special_dof + t->nspecial);
         s->special_volume.assign(t->special_volume, t->special_volume + t->nspecial);
         s->special_patch.assign(t->special_patch, t->special_patch + t->nspecial);
         s->ea.assign(t->edge_a, t->edge_a + t->nedges);
@@ -810,14 +926,19 @@ void* oracle_create(const oracle_topology* t, const pcfd_options* o) {
         s->patch_dof.assign(t->patch_dof, t->patch_dof + t->npatches);
         s->patch_S.assign(t->patch_S, t->patch_S + 3 * t->npatches);
         s->patch_x.assign(t->patch_x, t->patch_x + 3 * t->npatches);
-        s->lsq_off.assign(t->patch_lsq_off, t->patch_lsq_off + t->npatches + 1);
-        s->lsq_dof.assign(t->patch_lsq_dof, t->patch_lsq_dof + s->lsq_off.back());
-        s->lsq_beta.assign(t->patch_lsq_beta, t->patch_lsq_beta + s->lsq_off.back());
+        s->patch_D.assign(t->patch_D, t->patch_D + 9 * t->npatches);
+        s->edcoef.assign(t->edge_dcoef, t->edge_dcoef + t->nedges);
         s->finest_h = t->finest_h;
         s->max_diff_coef = t->max_diffusion_coefficient;
-        if (t->centroid) s->centroid.assign(t->centroid, t->centroid + 3 * t->ndof);
-        if (o->smagorinsky_cs > 0 && s->centroid.empty()) throw std::invalid_argument("the Smagorinsky model needs cell centroids");
-        if (o->smooth_wall && s->centroid.empty()) throw std::invalid_argument("the smooth-wall model needs cell centroids");
+        if (!t->centroid) throw std::invalid_argument("the oracle needs the cell centroids");
+        s->centroid.assign(t->centroid, t->centroid + 3 * t->ndof);
+        s->edisp.assign(6 * t->nedges, 0.0);
+        for (int64_t e = 0; e < t->nedges; ++e)
+            for (int c = 0; c < 3; ++c) {
+                s->edisp[6 * e + c] = t->edge_x[3 * e + c] - s->centroid[3 * static_cast<int64_t>(t->edge_a[e]) + c];
+                s->edisp[6 * e + 3 + c] = t->edge_x[3 * e + c] - s->centroid[3 * static_cast<int64_t>(t->edge_b[e]) + c];
+            }
+        s->build_gradient_stencils();
         s->mg.resize(t->mg_levels);
         for (int m = 0; m < t->mg_levels; ++m) {
             auto& L = s->mg[m];
@@ -1022,6 +1143,32 @@ int oracle_viscosity(void* h, double* nu) {
         g_error = e.what();
         return PCFD_EINVAL;
     }
+}
+/* Plant mistake `defect` (see Oracle::defect; 0 removes it) for the contract tests. */
+int oracle_plant(void* h, int defect) {
+    auto* s = static_cast<Oracle*>(h);
+    s->defect = defect;
+    s->build_gradient_stencils();
+    return PCFD_OK;
+}
+/* Face correction u -= (dt/rho) dp/dn with the current pressure (no solve). */
+int oracle_correct(void* h, double dt) { static_cast<Oracle*>(h)->correct(dt); return PCFD_OK; }
+/* The projection's right-hand side -(rho/dt) * integrated divergence of the current faces. */
+int oracle_rhs(void* h, double dt, double* b) {
+    auto* s = static_cast<Oracle*>(h);
+    std::vector<double> v;
+    s->rhs(dt, v);
+    std::memcpy(b, v.data(), v.size() * sizeof(double));
+    return PCFD_OK;
+}
+/* Least-squares gradients at the special cells of the current velocity (9 per cell) and
+ * pressure (3 per cell). */
+int oracle_gradients(void* h, double* gv, double* gp) {
+    auto* s = static_cast<Oracle*>(h);
+    std::vector<double> g;
+    if (gv) { s->velocity_gradients(g); std::memcpy(gv, g.data(), g.size() * sizeof(double)); }
+    if (gp) { s->pressure_gradients(g); std::memcpy(gp, g.data(), g.size() * sizeof(double)); }
+    return PCFD_OK;
 }
 int oracle_divergence(void* h, double* div) {
     auto* s = static_cast<Oracle*>(h);

@@ -49,8 +49,8 @@ class OracleTopology(C.Structure):
                 ("special_volume", C.c_void_p), ("special_patch", C.c_void_p), ("edge_a", C.c_void_p),
                 ("edge_b", C.c_void_p), ("edge_axis", C.c_void_p), ("edge_area", C.c_void_p),
                 ("edge_coef", C.c_void_p), ("edge_w", C.c_void_p), ("patch_dof", C.c_void_p),
-                ("patch_S", C.c_void_p), ("patch_x", C.c_void_p), ("patch_lsq_off", C.c_void_p),
-                ("patch_lsq_dof", C.c_void_p), ("patch_lsq_beta", C.c_void_p), ("finest_h", C.c_double),
+                ("patch_S", C.c_void_p), ("patch_x", C.c_void_p), ("patch_D", C.c_void_p),
+                ("edge_x", C.c_void_p), ("edge_dcoef", C.c_void_p), ("finest_h", C.c_double),
                 ("max_diffusion_coefficient", C.c_double), ("mg_levels", C.c_int), ("mg_n", C.c_void_p),
                 ("mg_parent", C.c_void_p), ("mg_row_off", C.c_void_p), ("mg_col", C.c_void_p),
                 ("mg_coef", C.c_void_p), ("mg_diag", C.c_void_p), ("coarse_inv", C.c_void_p),
@@ -85,7 +85,8 @@ class Oracle:
             "oracle_mark_initialized": [P, C.c_double],
             "oracle_mg_primitive": [P, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int],
             "oracle_dense_operator": [P, dp], "oracle_divergence": [P, dp], "oracle_viscosity": [P, dp],
-            "oracle_smooth_wall": [P, C.c_double, dp],
+            "oracle_smooth_wall": [P, C.c_double, dp], "oracle_gradients": [P, dp, dp],
+            "oracle_plant": [P, C.c_int], "oracle_rhs": [P, C.c_double, dp], "oracle_correct": [P, C.c_double],
         }.items():
             f = getattr(lib, name)
             f.restype = C.c_int
@@ -103,8 +104,7 @@ class Oracle:
         keep = {}
         for name in ["level_h", "brick_level", "brick_flags", "brick_nbr", "mask", "special_dof", "special_volume",
                      "special_patch", "edge_a", "edge_b", "edge_axis", "edge_area", "edge_coef", "edge_w",
-                     "patch_dof", "patch_S", "patch_x", "patch_lsq_off", "patch_lsq_dof", "patch_lsq_beta",
-                     "centroid"]:
+                     "patch_dof", "patch_S", "patch_x", "patch_D", "edge_x", "edge_dcoef", "centroid"]:
             keep[name] = np.ascontiguousarray(t.array(name))
         nl = t.mg_levels
         mg_n = np.array([t.mg_n(m) for m in range(nl)], dtype=np.int64)
@@ -177,6 +177,7 @@ class Oracle:
         used = C.c_double()
         subs = C.c_int()
         self.lib.oracle_advect(self.h, dt, C.byref(used), C.byref(subs))
+        self.last_substeps = subs.value
         return used.value
 
     def project(self, dt, warm=True):
@@ -263,6 +264,30 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(self.lib.oracle_last_error().decode())
         return nu
+
+    def gradients(self):
+        """(velocity gradients [nspecial, 3, 3] with [s, c, r] = d v_c / d x_r, pressure
+        gradients [nspecial, 3]) at the special cells, least squares on the current state."""
+        ns = int(self.topo.info.special_cells)
+        gv = np.zeros(9 * max(ns, 1))
+        gp = np.zeros(3 * max(ns, 1))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self.lib.oracle_gradients(self.h, dp(gv), dp(gp))
+        return gv[:9 * ns].reshape(ns, 3, 3), gp[:3 * ns].reshape(ns, 3)
+
+    def plant(self, defect):
+        """Plant mistake `defect` in this oracle (contract tests show that it fails them); 0 removes it."""
+        self.lib.oracle_plant(self.h, int(defect))
+
+    def correct(self, dt):
+        """Face correction u -= (dt/rho) dp/dn with the current pressure (no solve)."""
+        self.lib.oracle_correct(self.h, dt)
+
+    def rhs(self, dt):
+        """-(rho/dt) times the integrated divergence of the current faces."""
+        b = np.empty(self.n)
+        self.lib.oracle_rhs(self.h, dt, b.ctypes.data_as(C.POINTER(C.c_double)))
+        return b
 
     def outflow_rate(self):
         return self.lib.oracle_outflow_rate(self.h)

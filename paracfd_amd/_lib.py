@@ -24,7 +24,7 @@ ARR = dict(
     special_inc=(8, np.int32), special_patch=(9, np.int32), brick_special_off=(10, np.int32), edge_a=(11, np.int32),
     edge_b=(12, np.int32), edge_axis=(13, np.int8), edge_area=(14, np.float64), edge_coef=(15, np.float64),
     edge_w=(16, np.float64), patch_dof=(17, np.int32), patch_S=(18, np.float64), patch_x=(19, np.float64),
-    patch_lsq_off=(20, np.int32), patch_lsq_dof=(21, np.int32), patch_lsq_beta=(22, np.float64),
+    patch_D=(20, np.float64), edge_x=(21, np.float64), edge_dcoef=(22, np.float64),
     centroid=(23, np.float64), level_h=(24, np.float64),
     nf_lower=(25, np.int32), nf_upper=(26, np.int32), nf_edge=(27, np.int32), nf_area=(28, np.float64),
     nf_delta=(29, np.float64), nf_k=(30, np.float64), nf_w=(31, np.float64), nf_lnode=(32, np.int32),

@@ -146,9 +146,9 @@ bool lookup(const pcfd::Topology& t, int id, ArrayRef& r) {
         case PCFD_ARR_PATCH_DOF: return set(t.patch_dof);
         case PCFD_ARR_PATCH_S: return set(t.patch_S);
         case PCFD_ARR_PATCH_X: return set(t.patch_x);
-        case PCFD_ARR_PATCH_LSQ_OFF: return set(t.patch_lsq_off);
-        case PCFD_ARR_PATCH_LSQ_DOF: return set(t.patch_lsq_dof);
-        case PCFD_ARR_PATCH_LSQ_BETA: return set(t.patch_lsq_beta);
+        case PCFD_ARR_PATCH_D: return set(t.patch_D);
+        case PCFD_ARR_EDGE_X: return set(t.edge_x);
+        case PCFD_ARR_EDGE_DCOEF: return set(t.edge_dcoef);
         case PCFD_ARR_CELL_CENTROID: return set(t.centroid);
         case PCFD_ARR_LEVEL_H: return set(t.level_h);
         default: break;

@@ -73,13 +73,17 @@ struct MeshView {
     const double* earea;
     const double* ecoef;
     const double* ew;
+    const double* edcoef;    // diffusion conductance of each edge
+    const double* edisp;     // 6 per edge: face centroid minus the centroid of a, then of b
+    const int* esp;          // 2 per edge: special ranks of a and b
+    const int* special_dof;  // dof of each special cell
+    int nspecial;
     int npatches;
     const int* patch_dof;
+    const int* patch_sp;     // special rank of the patch's cell
     const double* patch_S;
     const double* patch_x;
-    const int* lsq_off;
-    const int* lsq_dof;
-    const double* lsq_beta;
+    const double* patch_D;   // 9 per patch: int_W (x - c) (x) n dA, row-major
     double speed, rho, cfl, nu, rtol;
     double finest_h;
     int max_iterations;

@@ -35,6 +35,59 @@ __device__ __forceinline__ R muscl_value(R swept, R pred, R va, R vb, R vl, bool
 
 __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
 
+// Least-squares gradient stencils of the special cells: neighbours across every open face
+// (structured faces, area h^2, and compact edges, open area), weight area / |d|^2 on the
+// fluid centroids, the pseudo-inverted 3x3 normal matrix (amr_advection.cu:822-865).
+// g = inv * sum_k wd_k (f(nbr_k) - f(cell)).
+struct SmagView {
+    const int* off;       // nspecial + 1
+    const int* nbr;       // neighbour dof per stencil entry
+    const double* wd;     // 3 per entry: weight * displacement
+    const double* inv;    // 9 per special cell
+};
+
+// velocity gradient at every special cell, 9 per cell (d v_c / d x_r at 3 c + r), one lane per
+// cell (accumulate_special_cell_gradient + solve_special_cell_gradient, amr_advection.cu:206-216)
+template <class R>
+__global__ __launch_bounds__(256) void k_grad_v(MeshView M, FieldsView<R> F, SmagView G, R* __restrict__ g) {
+    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sp >= M.nspecial) return;
+    const int64_t a = M.special_dof[sp];
+    const double va[3] = {double(F.v[0][a]), double(F.v[1][a]), double(F.v[2][a])};
+    double rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = G.off[sp]; k < G.off[sp + 1]; ++k) {
+        const int64_t o = G.nbr[k];
+        const double w0 = G.wd[3 * k], w1 = G.wd[3 * k + 1], w2 = G.wd[3 * k + 2];
+        for (int c = 0; c < 3; ++c) {
+            const double dv = double(F.v[c][o]) - va[c];
+            rhs[3 * c] += w0 * dv; rhs[3 * c + 1] += w1 * dv; rhs[3 * c + 2] += w2 * dv;
+        }
+    }
+    const double* I = G.inv + 9 * static_cast<int64_t>(sp);
+    R* out = g + 9 * static_cast<int64_t>(sp);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            out[3 * c + r] = R(I[3 * r] * rhs[3 * c] + I[3 * r + 1] * rhs[3 * c + 1] + I[3 * r + 2] * rhs[3 * c + 2]);
+}
+
+// pressure gradient at every special cell, 3 per cell, with the same stencils
+// (reconstruct_scalar_pressure_gradient_kernel, amr_advection.cu:400-405)
+__global__ __launch_bounds__(256) void k_grad_p(MeshView M, SmagView G, const double* __restrict__ p,
+                                                double* __restrict__ g) {
+    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sp >= M.nspecial) return;
+    const int64_t a = M.special_dof[sp];
+    const double pa = p[a];
+    double rhs[3] = {0, 0, 0};
+    for (int k = G.off[sp]; k < G.off[sp + 1]; ++k) {
+        const double dp = p[G.nbr[k]] - pa;
+        rhs[0] += G.wd[3 * k] * dp; rhs[1] += G.wd[3 * k + 1] * dp; rhs[2] += G.wd[3 * k + 2] * dp;
+    }
+    const double* I = G.inv + 9 * static_cast<int64_t>(sp);
+    for (int r = 0; r < 3; ++r)
+        g[3 * static_cast<int64_t>(sp) + r] = I[3 * r] * rhs[0] + I[3 * r + 1] * rhs[1] + I[3 * r + 2] * rhs[2];
+}
+
 // ---------------------------------------------------------------------------------------
 // CFL: max over cells of sum(max(outflow, 0)) / V  (amr_advection.cu:169-186) -> dt
 // ---------------------------------------------------------------------------------------
@@ -73,9 +126,13 @@ __global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> 
 // ---------------------------------------------------------------------------------------
 // Conservative MUSCL transport of the cell momentum (amr_advection.cu:82-112, 113-118)
 // ---------------------------------------------------------------------------------------
+// Compact faces (coarse/fine, partially open, and the fully open faces of embedded nodes) carry
+// the donor's value reconstructed to the face centroid with its least-squares gradient g
+// (velocity before the transport), scaled by 1 - C, C = min(1, |swept| / V_donor), and clamped
+// to the two cells' values (compact_muscl_cell_face_value, amr_advection.cu:94-105).
 template <class R>
 __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* __restrict__ ox, R* __restrict__ oy,
-                                                R* __restrict__ oz, const DevScalars* S) {
+                                                R* __restrict__ oz, const DevScalars* S, const R* __restrict__ G) {
     const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -135,9 +192,24 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             const int raw = M.special_inc[k];
             const int e = raw >= 0 ? raw : ~raw;
             const R swept = dt * R(M.earea[e]) * F.fe[e];
-            const int64_t donor = swept >= R(0) ? M.ea[e] : M.eb[e];
+            const bool lower = swept >= R(0);
+            const int64_t ia = M.ea[e], ib = M.eb[e];
+            const int ds = M.esp[2 * e + (lower ? 0 : 1)];
+            R C = R(1), dx[3] = {R(0), R(0), R(0)};
+            if (ds >= 0) {
+                C = fmin(R(1), fabs(swept) / R(M.special_volume[ds]));
+                const double* dd = M.edisp + 6 * static_cast<int64_t>(e) + (lower ? 0 : 3);
+                dx[0] = R(dd[0]); dx[1] = R(dd[1]); dx[2] = R(dd[2]);
+            }
             for (int c = 0; c < 3; ++c) {
-                const R flux = swept * F.v[c][donor];
+                const R va = F.v[c][ia], vb = F.v[c][ib];
+                R fv = lower ? va : vb;
+                if (ds >= 0) {
+                    const R* gc = G + 9 * static_cast<int64_t>(ds) + 3 * c;
+                    const R rec = fv + (R(1) - C) * (gc[0] * dx[0] + gc[1] * dx[1] + gc[2] * dx[2]);
+                    fv = fmax(fmin(va, vb), fmin(fmax(va, vb), rec));
+                }
+                const R flux = swept * fv;
                 d[c] += raw >= 0 ? -flux : flux;
             }
         }
@@ -183,7 +255,7 @@ __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict
             const int raw = M.special_inc[j];
             const int e = raw >= 0 ? raw : ~raw;
             const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
-            const R ke = R(dts * M.nu * M.ecoef[e]);
+            const R ke = R(dts * M.nu * M.edcoef[e]);
             for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
         }
     }
@@ -199,13 +271,6 @@ __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict
 // pseudo-inverted normal matrix and weighted displacements are precomputed per special cell
 // (stencil CSR sg_off / sg_nbr / sg_wd, inverses sg_inv).
 // ---------------------------------------------------------------------------------------
-struct SmagView {
-    const int* off;       // nspecial + 1
-    const int* nbr;       // neighbour dof per stencil entry
-    const double* wd;     // 3 per entry: weight * displacement
-    const double* inv;    // 9 per special cell
-};
-
 __device__ __forceinline__ double strain_rate(const double* g) {
     const double sxx = g[0], syy = g[4], szz = g[8];
     const double sxy = 0.5 * (g[1] + g[3]), sxz = 0.5 * (g[2] + g[6]), syz = 0.5 * (g[5] + g[7]);
@@ -279,7 +344,7 @@ __global__ __launch_bounds__(512) void k_diff_rate(MeshView M, const R* __restri
                 const int raw = M.special_inc[j];
                 const int e = raw >= 0 ? raw : ~raw;
                 const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
-                s += M.ecoef[e] * 0.5 * (na + double(nu[o]));
+                s += M.edcoef[e] * 0.5 * (na + double(nu[o]));
             }
         }
         rate = s / V;
@@ -324,7 +389,7 @@ __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __rest
             const int raw = M.special_inc[j];
             const int e = raw >= 0 ? raw : ~raw;
             const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
-            const R ke = R(dts) * (half * (na + nu[o])) * R(M.ecoef[e]);
+            const R ke = R(dts) * (half * (na + nu[o])) * R(M.edcoef[e]);
             for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
         }
     }
@@ -351,13 +416,26 @@ __global__ __launch_bounds__(512) void k_reconstruct(MeshView M, FieldsView<R> F
     if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) F.f[0][a] = F.v[0][a];
 }
 
+// compact faces: the weighted mean of the two cells' normal components plus the transverse
+// correction both cells' gradients support, minmod(g_a . s, g_b . s), s = x_f - ((1-w) c_a + w c_b)
+// (reconstruct_compact_cell_flux_kernel, amr_advection.cu:268-293)
 template <class R>
-__global__ void k_reconstruct_edges(MeshView M, FieldsView<R> F) {
+__global__ void k_reconstruct_edges(MeshView M, FieldsView<R> F, const R* __restrict__ G) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= M.nedges) return;
     const int ax = M.eaxis[e];
     const R w = R(M.ew[e]);
-    F.fe[e] = (R(1) - w) * F.v[ax][M.ea[e]] + w * F.v[ax][M.eb[e]];
+    const double* da = M.edisp + 6 * static_cast<int64_t>(e);
+    R skew[3];
+    for (int r = 0; r < 3; ++r) skew[r] = (R(1) - w) * R(da[r]) + w * R(da[3 + r]);
+    R corr[2] = {R(0), R(0)};
+    for (int side = 0; side < 2; ++side) {
+        const int sp = M.esp[2 * e + side];
+        if (sp < 0) continue;
+        const R* gc = G + 9 * static_cast<int64_t>(sp) + 3 * ax;
+        corr[side] = gc[0] * skew[0] + gc[1] * skew[1] + gc[2] * skew[2];
+    }
+    F.fe[e] = (R(1) - w) * F.v[ax][M.ea[e]] + w * F.v[ax][M.eb[e]] + minmod(corr[0], corr[1]);
 }
 
 // integrated divergence of the face fluxes around cell a (FP64 accumulation)
@@ -440,12 +518,15 @@ __global__ void k_correct_edges(MeshView M, FieldsView<R> F, const double* __res
     F.fe[e] = R(double(F.fe[e]) - s * (p[M.eb[e]] - p[M.ea[e]]) * (M.ecoef[e] / M.earea[e]));
 }
 
-// wall pressure at a patch centroid from the least-squares stencil
-__device__ __forceinline__ double wall_pressure(const MeshView& M, const double* p, int pch) {
-    const int64_t a = M.patch_dof[pch];
-    double pw = p[a];
-    for (int k = M.lsq_off[pch]; k < M.lsq_off[pch + 1]; ++k) pw += M.lsq_beta[k] * (p[M.lsq_dof[k]] - p[a]);
-    return pw;
+// wall force of patch pch for the pressure p_a + g.(x - c_a) on its wall: (p_a - pref) S + D^T g
+// with D = int_W (x - c_a) (x) n dA; for a planar patch with centroid x_p this is the
+// reference's (p_a + g.(x_p - c_a)) S (amr_advection.cu:391-399, aero_loads.cpp:38-40)
+__device__ __forceinline__ void wall_force_p(const MeshView& M, const double* p, const double* gp, int pch, int sp,
+                                             double pref, double* F) {
+    const double pa = p[M.patch_dof[pch]] - pref;
+    const double* D = M.patch_D + 9 * static_cast<int64_t>(pch);
+    const double* g = gp + 3 * static_cast<int64_t>(sp);
+    for (int c = 0; c < 3; ++c) F[c] = pa * M.patch_S[3 * pch + c] + D[c] * g[0] + D[3 + c] * g[1] + D[6 + c] * g[2];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -457,7 +538,8 @@ __device__ __forceinline__ double wall_pressure(const MeshView& M, const double*
 // instances whose projection did not converge are left unchanged.
 template <class R>
 __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
-                                                 const DevScalars* S, double dt_given, int only_converged) {
+                                                 const double* __restrict__ gp, const DevScalars* S, double dt_given,
+                                                 int only_converged) {
     const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
@@ -485,17 +567,27 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
     double V = h * h * h;
     if (sp >= 0) {
         V = M.special_volume[sp];
+        // compact faces: weighted face pressure plus skew . ((1-w) g_a + w g_b)
+        // (pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390)
         for (int t = M.special_inc_off[sp]; t < M.special_inc_off[sp + 1]; ++t) {
             const int raw = M.special_inc[t];
             const int e = raw >= 0 ? raw : ~raw;
             const double w = M.ew[e];
-            const double imp = s * M.earea[e] * ((1.0 - w) * p[M.ea[e]] + w * p[M.eb[e]]);
+            double pf = (1.0 - w) * p[M.ea[e]] + w * p[M.eb[e]];
+            const int sa = M.esp[2 * e], sb = M.esp[2 * e + 1];
+            if (sa >= 0 && sb >= 0) {
+                const double* da = M.edisp + 6 * static_cast<int64_t>(e);
+                for (int c = 0; c < 3; ++c)
+                    pf += ((1.0 - w) * da[c] + w * da[3 + c]) * ((1.0 - w) * gp[3 * sa + c] + w * gp[3 * sb + c]);
+            }
+            const double imp = s * M.earea[e] * pf;
             d[M.eaxis[e]] += raw >= 0 ? -imp : imp;
         }
         const int pch = M.special_patch[sp];
         if (pch >= 0) {
-            const double pw = wall_pressure(M, p, pch);
-            for (int c = 0; c < 3; ++c) d[c] -= s * pw * M.patch_S[3 * pch + c];
+            double Fw[3];
+            wall_force_p(M, p, gp, pch, sp, 0.0, Fw);
+            for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
         }
     }
     const double iv = 1.0 / V;
@@ -508,17 +600,18 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
 // ---------------------------------------------------------------------------------------
 // pressure force and moment of each patch, then the wall-shear force of the smooth-wall model
 // (wall_force, zero while the model is off) and its moment: 12 partial arrays
-__global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p,
+__global__ __launch_bounds__(256) void k_loads(MeshView M, const double* __restrict__ p, const double* __restrict__ gp,
                                                const double* __restrict__ wall_force, double pref, double ox,
                                                double oy, double oz, int pbegin, int pend, double* partials) {
     const int pch = pbegin + blockIdx.x * blockDim.x + threadIdx.x;
     double v[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (pch < pend) {
-        const double pw = wall_pressure(M, p, pch) - pref;
+        double Fp[3];
+        wall_force_p(M, p, gp, pch, M.patch_sp[pch], pref, Fp);
         const double rx = M.patch_x[3 * pch] - ox, ry = M.patch_x[3 * pch + 1] - oy, rz = M.patch_x[3 * pch + 2] - oz;
         for (int part = 0; part < 2; ++part) {
             double F[3];
-            for (int c = 0; c < 3; ++c) F[c] = part == 0 ? pw * M.patch_S[3 * pch + c] : wall_force[3 * pch + c];
+            for (int c = 0; c < 3; ++c) F[c] = part == 0 ? Fp[c] : wall_force[3 * pch + c];
             double* o = v + 6 * part;
             o[0] = F[0]; o[1] = F[1]; o[2] = F[2];
             o[3] = ry * F[2] - rz * F[1]; o[4] = rz * F[0] - rx * F[2]; o[5] = rx * F[1] - ry * F[0];

@@ -339,7 +339,7 @@ private:
         LAUNCH(k_nn_train_bwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_gy_, nn_sb_, nn_scratch_, nn_g_);
     }
     void setup_learned(const Topology& T);
-    void setup_smagorinsky(const Topology& T);
+    void setup_gradients(const Topology& T);
     void setup_smooth_wall(const Topology& T);
     // non-orthogonal operator and its flexible GMRES outer solve
     void setup_nonorth(const Topology& T);
@@ -469,6 +469,14 @@ private:
     // Smagorinsky model (smagorinsky_cs > 0)
     R* nu_ = nullptr;
     SmagView smag_{};
+    R* grad_v_ = nullptr;        // velocity gradient at the special cells (9 per cell)
+    double* grad_p_ = nullptr;   // pressure gradient at the special cells (3 per cell)
+    void grad_v() {
+        if (M_.nspecial) LAUNCH(k_grad_v<R>, blocks_for(M_.nspecial, 256), 256, M_, fields(), smag_, grad_v_);
+    }
+    void grad_p() {
+        if (M_.nspecial) LAUNCH(k_grad_p, blocks_for(M_.nspecial, 256), 256, M_, smag_, p_, grad_p_);
+    }
     // learned preconditioner state (PCFD_PC_LEARNED)
     bool nn_ = false;             // created with the learned preconditioner
     bool nn_on_ = false;          // network enabled (trained or weights set)
@@ -572,13 +580,39 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
     M_.earea = upload(T.edge_area);
     M_.ecoef = upload(T.edge_coef);
     M_.ew = upload(T.edge_w);
+    M_.edcoef = upload(T.edge_dcoef);
+    M_.nspecial = static_cast<int>(T.special_dof.size());
+    M_.special_dof = upload(T.special_dof);
     M_.npatches = static_cast<int>(T.patch_dof.size());
     M_.patch_dof = upload(T.patch_dof);
     M_.patch_S = upload(T.patch_S);
     M_.patch_x = upload(T.patch_x);
-    M_.lsq_off = upload(T.patch_lsq_off);
-    M_.lsq_dof = upload(T.patch_lsq_dof);
-    M_.lsq_beta = upload(T.patch_lsq_beta);
+    M_.patch_D = upload(T.patch_D);
+    {
+        // per compact edge: special ranks of its cells and the face centroid's offsets from
+        // their centroids; per patch: the special rank of its cell
+        std::vector<int> special_of(T.ndof, -1);
+        for (size_t sp = 0; sp < T.special_dof.size(); ++sp) special_of[T.special_dof[sp]] = static_cast<int>(sp);
+        std::vector<int> esp(2 * static_cast<size_t>(nedges_));
+        std::vector<double> edisp(6 * static_cast<size_t>(nedges_));
+        for (int e = 0; e < nedges_; ++e) {
+            const int64_t a = T.edge_a[e], b = T.edge_b[e];
+            esp[2 * e] = special_of[a];
+            esp[2 * e + 1] = special_of[b];
+            for (int c = 0; c < 3; ++c) {
+                edisp[6 * e + c] = T.edge_x[3 * e + c] - T.centroid[3 * a + c];
+                edisp[6 * e + 3 + c] = T.edge_x[3 * e + c] - T.centroid[3 * b + c];
+            }
+        }
+        M_.esp = upload(esp);
+        M_.edisp = upload(edisp);
+        std::vector<int> psp(T.patch_dof.size());
+        for (size_t q = 0; q < psp.size(); ++q) {
+            psp[q] = special_of[T.patch_dof[q]];
+            if (psp[q] < 0) throw std::logic_error("a wall patch on a cell that is not special");
+        }
+        M_.patch_sp = upload(psp);
+    }
     M_.speed = opt_.speed;
     M_.rho = opt_.rho;
     M_.cfl = opt_.cfl;
@@ -792,7 +826,8 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
         LAUNCH(k_diag0<float>, nb_, 512, M_, static_cast<float*>(diag_));
     }
     if (opt_.preconditioner == PCFD_PC_LEARNED) setup_learned(T);
-    if (opt_.smagorinsky_cs > 0.0) setup_smagorinsky(T);
+    setup_gradients(T);
+    if (opt_.smagorinsky_cs > 0.0) nu_ = alloc<R>(n_);
     setup_smooth_wall(T);
     if (opt_.nonorthogonal) setup_nonorth(T);
     brick_level_h_ = T.brick_level;
@@ -807,8 +842,11 @@ SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
 // Least-squares gradient stencils of the special cells (amr_advection.cu:834): neighbours
 // across the open structured faces (area h^2) and the compact edges (open area), weight
 // area / |x_b - x_a|^2 on the fluid centroids; the 3x3 normal matrix is pseudo-inverted.
+// They give the velocity gradients of the compact-face transport and reconstruction, the
+// pressure gradients of the compact-face and wall traction and of the loads, and the
+// Smagorinsky strain of the special cells.
 template <class R>
-void SolverImpl<R>::setup_smagorinsky(const Topology& T) {
+void SolverImpl<R>::setup_gradients(const Topology& T) {
     const int ns = static_cast<int>(T.special_dof.size());
     std::vector<int> off(ns + 1, 0), nbr;
     std::vector<double> wd, inv(9 * static_cast<size_t>(ns), 0.0);
@@ -853,7 +891,10 @@ void SolverImpl<R>::setup_smagorinsky(const Topology& T) {
     smag_.nbr = upload(nbr);
     smag_.wd = upload(wd);
     smag_.inv = upload(inv);
-    nu_ = alloc<R>(n_);
+    grad_v_ = alloc<R>(9 * static_cast<int64_t>(ns));
+    grad_p_ = alloc<double>(3 * static_cast<int64_t>(ns));
+    HIP_CHECK(hipMemset(grad_v_, 0, std::max<size_t>(1, 9 * static_cast<size_t>(ns)) * sizeof(R)));
+    HIP_CHECK(hipMemset(grad_p_, 0, std::max<size_t>(1, 3 * static_cast<size_t>(ns)) * sizeof(double)));
 }
 
 // Network buffers, brick selection and the initial weights: He-normal (seed 1234) for
@@ -991,7 +1032,10 @@ SolverImpl<R>::~SolverImpl() {
 template <class R>
 void SolverImpl<R>::reconstruct() {
     LAUNCH(k_reconstruct<R>, nb_, 512, M_, fields());
-    if (nedges_) LAUNCH(k_reconstruct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields());
+    if (nedges_) {
+        grad_v();
+        LAUNCH(k_reconstruct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), static_cast<const R*>(grad_v_));
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1464,7 +1508,8 @@ void SolverImpl<R>::project(double dt, bool warm, pcfd_step_stats* st) {
 
 template <class R>
 void SolverImpl<R>::impulse(double dt) {
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, S_, dt, 0);
+    grad_p();
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, dt, 0);
 }
 
 template <class R>
@@ -1483,7 +1528,8 @@ void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
 // face reconstruction
 template <class R>
 void SolverImpl<R>::advect_core() {
-    probed(2, [&] { LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); });
+    grad_v();
+    probed(2, [&] { LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); });
     for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
     const bool smag = opt_.smagorinsky_cs > 0;
     if (smag) {
@@ -1522,7 +1568,8 @@ void SolverImpl<R>::initialize(pcfd_step_stats* st) {
     const double dt = opt_.cfl * M_.finest_h / std::max(1e-9, std::abs(opt_.speed));
     HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
     project(dt, false, nullptr);
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, S_, dt, 1);  // converged instances only
+    grad_p();
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, dt, 1);  // converged instances only
     LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_VMAX, nb_);
     sync_scalars();
@@ -1557,7 +1604,8 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
     run_projection(true, nullptr);
     // impulse and time advance for the instances whose projection converged
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, S_, 0.0, 1);
+    grad_p();
+    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, 0.0, 1);
     for (int i = 0; i < ninst_; ++i)
         if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
     time = inst_time_[0];
@@ -1588,7 +1636,7 @@ void SolverImpl<R>::enqueue_loads(int i, double pref) {
     if (p1 <= p0) return;
     const unsigned gl = blocks_for(p1 - p0, 256);
     const double* o = &inst_origin_[3 * static_cast<size_t>(i)];
-    LAUNCH(k_loads, gl, 256, M_, p_, wall_force_, pref, o[0], o[1], o[2], p0, p1, partials_);
+    LAUNCH(k_loads, gl, 256, M_, p_, grad_p_, wall_force_, pref, o[0], o[1], o[2], p0, p1, partials_);
     LAUNCH(k_finalize, 1, 1024, FIN_LOADS, partials_, static_cast<int>(gl), static_cast<const int*>(nullptr), S_ + i,
            static_cast<DevScalars*>(nullptr), opt_.cfl, 1.0, opt_.rtol, opt_.max_iterations, 0);
 }
@@ -1625,6 +1673,7 @@ void SolverImpl<R>::read_loads(int i, pcfd_loads* out) const {
 
 template <class R>
 void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
+    grad_p();
     enqueue_loads(i, pref);
     sync_scalars();
     read_loads(i, out);
@@ -1632,6 +1681,7 @@ void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
 
 template <class R>
 void SolverImpl<R>::batch_loads(double pref, pcfd_loads* out) {
+    grad_p();
     for (int i = 0; i < ninst_; ++i) enqueue_loads(i, pref);
     sync_scalars();
     for (int i = 0; i < ninst_; ++i) read_loads(i, out + i);
@@ -1784,7 +1834,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
                 else
                     LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 break;
-            case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_); break;
+            case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); break;
             case 3: precondition_dispatch(true); break;
             case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
             case 5: LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;

@@ -3,6 +3,7 @@
 #include "topology.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -444,6 +445,31 @@ private:
     std::vector<double> pts_;
 };
 
+// Weight of the upper cell when a cell value is interpolated to a face at x_f between the
+// centroids c_l and c_u (composite_face_interpolation_weight, amr_advection.cu:732-753): the
+// face's position between the centroids along the axis when it lies between them (within
+// 1e-8), else the distance weight |x_f - c_l| / (|x_f - c_l| + |x_f - c_u|); clamped to [0, 1].
+double face_weight(const double* cl, const double* cu, const double* xf, int axis) {
+    double d2 = 0;
+    for (int r = 0; r < 3; ++r) d2 += (cu[r] - cl[r]) * (cu[r] - cl[r]);
+    const double dist = std::sqrt(d2), normal = cu[axis] - cl[axis];
+    if (!(dist > 0) || !std::isfinite(dist)) throw BuildError("compact face between coincident centroids");
+    if (std::abs(normal) > 1e-12 * dist) {
+        const double w = (xf[axis] - cl[axis]) / normal;
+        if (std::isfinite(w) && w >= -1e-8 && w <= 1 + 1e-8) return std::clamp(w, 0.0, 1.0);
+    }
+    double dl = 0, du = 0;
+    for (int r = 0; r < 3; ++r) {
+        dl += (xf[r] - cl[r]) * (xf[r] - cl[r]);
+        du += (xf[r] - cu[r]) * (xf[r] - cu[r]);
+    }
+    dl = std::sqrt(dl);
+    du = std::sqrt(du);
+    const double total = dl + du;
+    if (!(total > 1e-14 * std::max(1.0, dist)) || !std::isfinite(total)) return 0.5;
+    return std::clamp(dl / total, 0.0, 1.0);
+}
+
 // ------------------------------------------------------------------------------------------
 // builder
 // ------------------------------------------------------------------------------------------
@@ -461,6 +487,7 @@ struct CutData {
     double f = 1;            // fluid volume fraction
     double c[3] = {0, 0, 0}; // fluid centroid
     double ap[6] = {1, 1, 1, 1, 1, 1}; // face apertures -x,+x,-y,+y,-z,+z
+    double fc[6][2] = {};    // mean of the open face samples' two tangential coordinates (axis+1, axis+2)
     double wall[3] = {0, 0, 0};        // mean of fluid samples next to solid samples
     int wall_n = 0;
 };
@@ -473,7 +500,10 @@ struct Leaf {
     std::vector<int> cut;         // 512, index into cut list or -1
 };
 
-struct Neighbor { int leaf; int local; double ap; int level; };
+// A face neighbour: its leaf and local cell, the open fraction of the shared face, its level
+// and the centroid of the face's open part (the open-sample mean of the cut cell that sampled
+// the face, else the face centre).
+struct Neighbor { int leaf; int local; double ap; int level; double xf[3]; };
 
 class Builder {
 public:
@@ -695,15 +725,17 @@ void Builder::classify() {
                     const int axis = dir >> 1, side = dir & 1;
                     const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
                     int open = 0;
+                    double s1 = 0, s2 = 0;
                     for (int b = 0; b < ns; ++b)
                         for (int a = 0; a < ns; ++a) {
                             double p[3];
                             p[axis] = T_.lo[axis] + (I[axis] + side) * h;
                             p[t1] = T_.lo[t1] + (I[t1] + (a + 0.5) / ns) * h;
                             p[t2] = T_.lo[t2] + (I[t2] + (b + 0.5) / ns) * h;
-                            if (!geo_->solid(p[0], p[1], p[2])) ++open;
+                            if (!geo_->solid(p[0], p[1], p[2])) { ++open; s1 += p[t1]; s2 += p[t2]; }
                         }
                     cd.ap[dir] = static_cast<double>(open) / (ns * ns);
+                    if (open > 0) { cd.fc[dir][0] = s1 / open; cd.fc[dir][1] = s2 / open; }
                 }
                 leaf.state[q] = 2;
                 leaf.cut[q] = static_cast<int>(local_cuts[lb].size());
@@ -733,6 +765,7 @@ int Builder::neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boun
     const Leaf& lf = leaves_[leaf];
     const int l = lf.level;
     const int axis = dir >> 1, sign = (dir & 1) ? 1 : -1;
+    const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
     int I[3] = {lf.coord[0] * kBS + (local & 7), lf.coord[1] * kBS + ((local >> 3) & 7), lf.coord[2] * kBS + (local >> 6)};
     int J[3] = {I[0], I[1], I[2]};
     J[axis] += sign;
@@ -740,19 +773,34 @@ int Builder::neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boun
     if (J[axis] < 0 || J[axis] >= G.nb[axis] * kBS) { boundary = true; return 0; }
     const int st = G.status[G.idx(J[0] >> 3, J[1] >> 3, J[2] >> 3)];
     auto loc = [](const int c[3]) { return (c[0] & 7) + 8 * (c[1] & 7) + 64 * (c[2] & 7); };
+    // centroid of the open part of a face on this cell's face plane: the mean of the open
+    // samples of cut `cut` on its face `cdir` when the face is partly open, else the centre of
+    // the face of the cell at integer position C with size hc
+    const double plane = T_.lo[axis] + (I[axis] + (sign > 0 ? 1 : 0)) * G.h;
+    auto face_centroid = [&](double* xf, int cut, int cdir, const int* C, double hc) {
+        xf[axis] = plane;
+        if (cut >= 0 && cuts_[cut].ap[cdir] > 0.0 && cuts_[cut].ap[cdir] < 1.0) {
+            xf[t1] = cuts_[cut].fc[cdir][0];
+            xf[t2] = cuts_[cut].fc[cdir][1];
+        } else {
+            xf[t1] = T_.lo[t1] + (C[t1] + 0.5) * hc;
+            xf[t2] = T_.lo[t2] + (C[t2] + 0.5) * hc;
+        }
+    };
     if (st >= 0) {
-        out[0] = {st, loc(J), 0.0, l};
+        out[0] = {st, loc(J), 0.0, l, {0, 0, 0}};
         // same level: both sides sampled the same face points
         const int ci = lf.cut[local], cj = leaves_[st].cut[loc(J)];
         out[0].ap = ci >= 0 ? cuts_[ci].ap[dir] : (cj >= 0 ? cuts_[cj].ap[dir ^ 1] : 1.0);
+        if (ci >= 0) face_centroid(out[0].xf, ci, dir, I, G.h);
+        else face_centroid(out[0].xf, cj, dir ^ 1, I, G.h);
         return 1;
     }
     if (st == ST_EMPTY) return 0;
     if (st == ST_REFINED) {
-        // four finer cells across the face
+        // four finer cells across the face; each sampled its own (fine) face
         const LevelGrid& F = grid_[l + 1];
         int n = 0;
-        const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
         for (int b = 0; b < 2; ++b)
             for (int a = 0; a < 2; ++a) {
                 int K[3];
@@ -763,11 +811,13 @@ int Builder::neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boun
                 if (fs < 0) continue;  // empty brick
                 const int fq = loc(K);
                 const int cf = leaves_[fs].cut[fq];
-                out[n++] = {fs, fq, cf >= 0 ? cuts_[cf].ap[dir ^ 1] : 1.0, l + 1};
+                out[n] = {fs, fq, cf >= 0 ? cuts_[cf].ap[dir ^ 1] : 1.0, l + 1, {0, 0, 0}};
+                face_centroid(out[n].xf, cf, dir ^ 1, K, F.h);
+                ++n;
             }
         return n;
     }
-    // absent at level l: the coarser leaf at level l-1 covers it
+    // absent at level l: the coarser leaf at level l-1 covers it; this (fine) cell sampled the face
     if (l == 0) throw BuildError("internal: absent level-0 brick");
     const LevelGrid& C = grid_[l - 1];
     int K[3] = {J[0] >> 1, J[1] >> 1, J[2] >> 1};
@@ -775,7 +825,8 @@ int Builder::neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boun
     if (cs == ST_EMPTY) return 0;
     if (cs < 0) throw BuildError("internal: AMR nesting violated");
     const int ci = lf.cut[local];
-    out[0] = {cs, loc(K), ci >= 0 ? cuts_[ci].ap[dir] : 1.0, l - 1};
+    out[0] = {cs, loc(K), ci >= 0 ? cuts_[ci].ap[dir] : 1.0, l - 1, {0, 0, 0}};
+    face_centroid(out[0].xf, ci, dir, I, G.h);
     return 1;
 }
 
@@ -882,28 +933,55 @@ void Builder::finalize() {
         active_ = std::move(na);
     }
 
-    // masks, compact edges, volumes, centroids, closure area vectors
+    // volumes and centroids of every cell, and the embedded nodes: cells with a partial volume
+    // or a partially open face to an active neighbour (the endpoints of the reference's
+    // embedded connections, amr_advection.cu:799).  Every same-level face of an embedded node
+    // is a compact edge, as the reference's EB momentum regular connections are
+    // (amr_advection.cu:800-812), so that its transport, reconstruction and traction carry
+    // the centroid corrections.
     T_.mask.assign(T_.ndof, 0);
     T_.centroid.assign(3 * T_.ndof, 0.0);
     std::vector<double> volume(T_.ndof, 0.0);
-    std::vector<double> S(3 * T_.ndof, 0.0);
-    std::vector<uint8_t> has_compact(T_.ndof, 0);
-    std::vector<double> diffusion_sum(T_.ndof, 0.0);
+    std::vector<uint8_t> embedded(T_.ndof, 0);
     for (int b = 0; b < nb; ++b) {
         const Leaf& lf = leaves_[b];
         const double h = grid_[lf.level].h;
         for (int q = 0; q < kBC; ++q) {
             const int64_t a = static_cast<int64_t>(b) * kBC + q;
             const int I[3] = {lf.coord[0] * kBS + (q & 7), lf.coord[1] * kBS + ((q >> 3) & 7), lf.coord[2] * kBS + (q >> 6)};
-            if (!active_[b][q]) {
-                for (int ax = 0; ax < 3; ++ax) T_.centroid[3 * a + ax] = T_.lo[ax] + (I[ax] + 0.5) * h;
-                continue;
+            const int ci = lf.cut[q];
+            const bool act = active_[b][q] != 0;
+            if (act) volume[a] = (ci >= 0 ? cuts_[ci].f : 1.0) * h * h * h;
+            for (int ax = 0; ax < 3; ++ax)
+                T_.centroid[3 * a + ax] = act && ci >= 0 ? cuts_[ci].c[ax] : T_.lo[ax] + (I[ax] + 0.5) * h;
+            if (!act) continue;
+            if (volume[a] != h * h * h) embedded[a] = 1;
+            for (int dir = 0; dir < 6; ++dir) {
+                Neighbor nbs[4]; bool bnd;
+                const int n = neighbors(b, q, dir, nbs, bnd);
+                for (int k = 0; k < n; ++k)
+                    if (active_[nbs[k].leaf][nbs[k].local] && nbs[k].ap > 0.0 && nbs[k].ap < 1.0) embedded[a] = 1;
             }
+        }
+    }
+    // masks, compact edges, closure area vectors, and the first moment of each cell's open
+    // faces about its centroid (the wall's moment follows from it, below)
+    std::vector<double> S(3 * T_.ndof, 0.0);
+    std::vector<uint8_t> has_compact(T_.ndof, 0);
+    std::vector<double> diffusion_sum(T_.ndof, 0.0);
+    std::vector<int64_t> wall_dof;
+    std::vector<std::array<double, 9>> wall_moment;  // per wall cell in dof order
+    for (int b = 0; b < nb; ++b) {
+        const Leaf& lf = leaves_[b];
+        const double h = grid_[lf.level].h;
+        for (int q = 0; q < kBC; ++q) {
+            const int64_t a = static_cast<int64_t>(b) * kBC + q;
+            const int I[3] = {lf.coord[0] * kBS + (q & 7), lf.coord[1] * kBS + ((q >> 3) & 7), lf.coord[2] * kBS + (q >> 6)};
+            if (!active_[b][q]) continue;
             T_.mask[a] |= CM_ACTIVE;
             const int ci = lf.cut[q];
-            volume[a] = (ci >= 0 ? cuts_[ci].f : 1.0) * h * h * h;
-            for (int ax = 0; ax < 3; ++ax)
-                T_.centroid[3 * a + ax] = ci >= 0 ? cuts_[ci].c[ax] : T_.lo[ax] + (I[ax] + 0.5) * h;
+            const double* ca = &T_.centroid[3 * a];
+            std::array<double, 9> fm{};  // sum over open faces of (x_f - c_a) (x) (A n)_f
             for (int dir = 0; dir < 6; ++dir) {
                 const int axis = dir >> 1, up = dir & 1;
                 const double nsign = up ? 1.0 : -1.0;
@@ -914,6 +992,10 @@ void Builder::finalize() {
                     if (ci >= 0 && cuts_[ci].ap[dir] < 1.0)
                         throw BuildError("a cut cell touches the domain boundary; enlarge the domain");
                     open = h * h;
+                    for (int r = 0; r < 3; ++r) {
+                        const double xf = r == axis ? T_.lo[r] + (I[r] + up) * h : T_.lo[r] + (I[r] + 0.5) * h;
+                        fm[3 * r + axis] += (xf - ca[r]) * nsign * open;
+                    }
                 }
                 for (int k = 0; k < n; ++k) {
                     const Neighbor& nb2 = nbs[k];
@@ -921,28 +1003,51 @@ void Builder::finalize() {
                     const double hf = std::min(h, grid_[nb2.level].h);
                     const double area = nb2.ap * hf * hf;
                     open += area;
+                    for (int r = 0; r < 3; ++r) fm[3 * r + axis] += (nb2.xf[r] - ca[r]) * nsign * area;
                     const int64_t other = static_cast<int64_t>(nb2.leaf) * kBC + nb2.local;
                     const double hn = grid_[nb2.level].h;
                     const double coef = area / (0.5 * (h + hn));
-                    diffusion_sum[a] += coef;
-                    if (nb2.level == lf.level && nb2.ap >= 1.0) {
+                    const bool same = nb2.level == lf.level;
+                    if (same && nb2.ap >= 1.0 && !embedded[a] && !embedded[other]) {
                         T_.mask[a] |= up ? (CM_PX << axis) : (CM_MX << axis);
+                        diffusion_sum[a] += coef;
                         continue;
                     }
+                    const int64_t lower = up ? a : other, upper = up ? other : a;
+                    const double* cl = &T_.centroid[3 * lower];
+                    const double* cu = &T_.centroid[3 * upper];
+                    // diffusion conductance: open area |dx_axis| / |dx|^2 on the centroids for a
+                    // fully open same-level face (the reference's perimeter conductance,
+                    // amr_advection.cu:1180), the operator's coefficient on coarse/fine and
+                    // partially open faces (connection_conductance, :1183)
+                    double dcoef = coef;
+                    if (same && nb2.ap >= 1.0) {
+                        double d2 = 0;
+                        for (int r = 0; r < 3; ++r) d2 += (cu[r] - cl[r]) * (cu[r] - cl[r]);
+                        const double dn = std::abs(cu[axis] - cl[axis]);
+                        if (!(d2 > 0) || !(dn > 0)) throw BuildError("compact face between coincident centroids");
+                        dcoef = area * dn / d2;
+                    }
+                    diffusion_sum[a] += dcoef;
                     // compact edge, created once: same level from the lower cell, c/f from the fine cell
-                    const bool create = nb2.level == lf.level ? up : nb2.level < lf.level;
+                    const bool create = same ? up : nb2.level < lf.level;
                     has_compact[a] = 1;
                     if (!create) continue;
-                    const int64_t lower = up ? a : other, upper = up ? other : a;
-                    const double hl = up ? h : hn, hu = up ? hn : h;
                     T_.edge_a.push_back(static_cast<int>(lower));
                     T_.edge_b.push_back(static_cast<int>(upper));
                     T_.edge_axis.push_back(static_cast<int8_t>(axis));
                     T_.edge_area.push_back(area);
                     T_.edge_coef.push_back(coef);
-                    T_.edge_w.push_back(hl / (hl + hu));
+                    T_.edge_dcoef.push_back(dcoef);
+                    T_.edge_w.push_back(face_weight(cl, cu, nb2.xf, axis));
+                    for (int r = 0; r < 3; ++r) T_.edge_x.push_back(nb2.xf[r]);
                 }
                 S[3 * a + axis] -= nsign * open;
+            }
+            const double smag = std::sqrt(S[3 * a] * S[3 * a] + S[3 * a + 1] * S[3 * a + 1] + S[3 * a + 2] * S[3 * a + 2]);
+            if (smag > 1e-12 * h * h) {
+                wall_dof.push_back(a);
+                wall_moment.push_back(fm);
             }
         }
     }
@@ -950,6 +1055,7 @@ void Builder::finalize() {
     // special cells and patches
     int64_t n_active = 0;
     double max_rate = 0;
+    size_t wall_next = 0;
     T_.brick_special_off.assign(nb + 1, 0);
     std::vector<int> special_of(T_.ndof, -1);
     for (int b = 0; b < nb; ++b) {
@@ -972,6 +1078,14 @@ void Builder::finalize() {
                 T_.special_patch.push_back(static_cast<int>(T_.patch_dof.size()));
                 T_.patch_dof.push_back(static_cast<int>(a));
                 for (int ax = 0; ax < 3; ++ax) T_.patch_S.push_back(S[3 * a + ax]);
+                // first moment of the wall about the centroid, D = int_W (x - c) (x) n dA, from the
+                // divergence theorem over the control volume: int_dV (x - c) (x) n dA = V I, so
+                // D = V I - sum over the open faces of (x_f - c) (x) (A n)_f, as the area vector
+                // S = -sum (A n)_f closes the faces' zeroth moment
+                if (wall_next >= wall_dof.size() || wall_dof[wall_next] != a) throw BuildError("internal: wall moment order");
+                const std::array<double, 9>& fm = wall_moment[wall_next++];
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) T_.patch_D.push_back((r == c ? volume[a] : 0.0) - fm[3 * r + c]);
                 const int ci = leaves_[b].cut[q];
                 if (ci >= 0 && cuts_[ci].wall_n > 0) {
                     for (int ax = 0; ax < 3; ++ax) T_.patch_x.push_back(cuts_[ci].wall[ax]);
@@ -1003,45 +1117,6 @@ void Builder::finalize() {
             T_.special_inc[fill[special_of[T_.edge_b[e]]]++] = ~e;
         }
     }
-    // least-squares wall-pressure reconstruction: p_wall = p_a + sum beta_q (p_q - p_a)
-    const int np = static_cast<int>(T_.patch_dof.size());
-    T_.patch_lsq_off.assign(np + 1, 0);
-    for (int pch = 0; pch < np; ++pch) {
-        T_.patch_lsq_off[pch] = static_cast<int>(T_.patch_lsq_dof.size());
-        const int64_t a = T_.patch_dof[pch];
-        const int b = static_cast<int>(a / kBC), q = static_cast<int>(a % kBC);
-        std::vector<int64_t> nbr;
-        for (int dir = 0; dir < 6; ++dir) {
-            Neighbor nbs[4]; bool bnd;
-            const int n = neighbors(b, q, dir, nbs, bnd);
-            for (int k = 0; k < n; ++k)
-                if (active_[nbs[k].leaf][nbs[k].local] && nbs[k].ap > 0)
-                    nbr.push_back(static_cast<int64_t>(nbs[k].leaf) * kBC + nbs[k].local);
-        }
-        double M[9] = {0};
-        for (int64_t o : nbr)
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c)
-                    M[3 * r + c] += (T_.centroid[3 * o + r] - T_.centroid[3 * a + r]) * (T_.centroid[3 * o + c] - T_.centroid[3 * a + c]);
-        const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
-        const double tr = (M[0] + M[4] + M[8]) / 3.0;
-        if (nbr.size() < 3 || !(det > 1e-6 * tr * tr * tr)) continue;
-        double inv[9];
-        inv[0] = (M[4] * M[8] - M[5] * M[7]) / det; inv[1] = (M[2] * M[7] - M[1] * M[8]) / det; inv[2] = (M[1] * M[5] - M[2] * M[4]) / det;
-        inv[3] = (M[5] * M[6] - M[3] * M[8]) / det; inv[4] = (M[0] * M[8] - M[2] * M[6]) / det; inv[5] = (M[2] * M[3] - M[0] * M[5]) / det;
-        inv[6] = (M[3] * M[7] - M[4] * M[6]) / det; inv[7] = (M[1] * M[6] - M[0] * M[7]) / det; inv[8] = (M[0] * M[4] - M[1] * M[3]) / det;
-        double r[3];
-        for (int ax = 0; ax < 3; ++ax) r[ax] = T_.patch_x[3 * pch + ax] - T_.centroid[3 * a + ax];
-        double g[3];
-        for (int i = 0; i < 3; ++i) g[i] = inv[3 * i] * r[0] + inv[3 * i + 1] * r[1] + inv[3 * i + 2] * r[2];
-        for (int64_t o : nbr) {
-            double beta = 0;
-            for (int ax = 0; ax < 3; ++ax) beta += g[ax] * (T_.centroid[3 * o + ax] - T_.centroid[3 * a + ax]);
-            T_.patch_lsq_dof.push_back(static_cast<int>(o));
-            T_.patch_lsq_beta.push_back(beta);
-        }
-    }
-    T_.patch_lsq_off[np] = static_cast<int>(T_.patch_lsq_dof.size());
     // solid volume of the body: every leaf cell's non-fluid part that lies inside the geometry
     double body = 0;
     for (int l = 0; l < L_; ++l) {
@@ -1406,7 +1481,7 @@ Topology merge_topologies(const std::vector<const Topology*>& parts) {
     T.inst_patch_off.push_back(0);
     T.level_brick_begin.assign(T.levels + 1, 0);
     int64_t dof0 = 0;
-    int brick0 = 0, special0 = 0, edge0 = 0, patch0 = 0, inc0 = 0, lsq0 = 0;
+    int brick0 = 0, special0 = 0, edge0 = 0, patch0 = 0, inc0 = 0;
     std::vector<int> cell0(depth, 0), nnz0(depth, 0), child0(depth, 0);
     auto shift = [](int v, int64_t by) { return v >= 0 ? static_cast<int>(v + by) : v; };
     for (size_t i = 0; i < parts.size(); ++i) {
@@ -1434,14 +1509,13 @@ Topology merge_topologies(const std::vector<const Topology*>& parts) {
         T.edge_area.insert(T.edge_area.end(), P.edge_area.begin(), P.edge_area.end());
         T.edge_coef.insert(T.edge_coef.end(), P.edge_coef.begin(), P.edge_coef.end());
         T.edge_w.insert(T.edge_w.end(), P.edge_w.begin(), P.edge_w.end());
+        T.edge_dcoef.insert(T.edge_dcoef.end(), P.edge_dcoef.begin(), P.edge_dcoef.end());
+        T.edge_x.insert(T.edge_x.end(), P.edge_x.begin(), P.edge_x.end());
         // surface patches
         for (int v : P.patch_dof) T.patch_dof.push_back(static_cast<int>(v + dof0));
         T.patch_S.insert(T.patch_S.end(), P.patch_S.begin(), P.patch_S.end());
         T.patch_x.insert(T.patch_x.end(), P.patch_x.begin(), P.patch_x.end());
-        if (T.patch_lsq_off.empty()) T.patch_lsq_off.push_back(0);
-        for (size_t k = 1; k < P.patch_lsq_off.size(); ++k) T.patch_lsq_off.push_back(P.patch_lsq_off[k] + lsq0);
-        for (int v : P.patch_lsq_dof) T.patch_lsq_dof.push_back(static_cast<int>(v + dof0));
-        T.patch_lsq_beta.insert(T.patch_lsq_beta.end(), P.patch_lsq_beta.begin(), P.patch_lsq_beta.end());
+        T.patch_D.insert(T.patch_D.end(), P.patch_D.begin(), P.patch_D.end());
         // multigrid: level 0 carries the parent map and the cell exponents
         const int pdepth = static_cast<int>(P.mg.size());
         for (int v : P.mg[0].parent) T.mg[0].parent.push_back(pdepth > 1 ? shift(v, cell0[1]) : -1);
@@ -1489,7 +1563,6 @@ Topology merge_topologies(const std::vector<const Topology*>& parts) {
         edge0 += static_cast<int>(P.edge_a.size());
         patch0 += static_cast<int>(P.patch_dof.size());
         inc0 += static_cast<int>(P.special_inc.size());
-        lsq0 += static_cast<int>(P.patch_lsq_dof.size());
         T.inst_brick_off.push_back(brick0);
         T.inst_patch_off.push_back(patch0);
     }
@@ -1498,7 +1571,6 @@ Topology merge_topologies(const std::vector<const Topology*>& parts) {
     T.mg[0].n = static_cast<int>(dof0);
     T.reference_area = P0.reference_area;
     if (T.special_inc_off.empty()) T.special_inc_off.push_back(0);
-    if (T.patch_lsq_off.empty()) T.patch_lsq_off.push_back(0);
     for (size_t m = 1; m < depth; ++m) {
         if (T.mg[m].row_off.empty()) T.mg[m].row_off.push_back(0);
         if (T.mg[m].child_off.empty()) T.mg[m].child_off.push_back(0);

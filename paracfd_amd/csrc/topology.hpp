@@ -73,16 +73,18 @@ struct Topology {
     std::vector<int> special_patch;
     std::vector<int> brick_special_off; // nbricks+1
 
-    // compact edges
+    // compact edges: coarse/fine faces, partially open faces and the fully open same-level
+    // faces of embedded nodes (cells with a partial volume or a partially open face)
     std::vector<int> edge_a, edge_b;
     std::vector<int8_t> edge_axis;
     std::vector<double> edge_area, edge_coef, edge_w;
+    std::vector<double> edge_dcoef;   // diffusion conductance
+    std::vector<double> edge_x;       // 3 per edge: centroid of the face's open part
 
-    // embedded-boundary surface patches (one per cut cell with a wall)
+    // embedded-boundary surface patches (one per cell with a wall)
     std::vector<int> patch_dof;
     std::vector<double> patch_S, patch_x;  // 3 per patch
-    std::vector<int> patch_lsq_off, patch_lsq_dof;
-    std::vector<double> patch_lsq_beta;
+    std::vector<double> patch_D;           // 9 per patch: int_W (x - c) (x) n dA, row-major [r][c]
 
     // multigrid: mg[0] is the composite level (only parent/exponent are filled there)
     std::vector<MGLevel> mg;
