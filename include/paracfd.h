@@ -217,6 +217,10 @@ typedef struct pcfd_options {
                                  PCG */
     int gmres_restart;        /* Krylov vectors per GMRES cycle (default 24, as the reference;
                                  clamped to 2..30) */
+    double memory_budget_bytes; /* device memory the solver may allocate; pcfd_create and any later
+                                 allocation beyond it fail with PCFD_ENOMEM before calling the
+                                 allocator (reference: ExternalAeroExecutionOptions::
+                                 gpu_memory_budget_bytes, external_aero_core.cpp:91-95); 0: no limit */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {

@@ -889,10 +889,15 @@ struct Oracle {
             out->total_cd = out->total_drag / qs; out->total_cs = out->total_side / qs; out->total_cl = out->total_lift / qs;
         }
     }
+    // max |v| over active cells; a non-finite component makes it infinite (std::max alone would
+    // drop a NaN)
     double max_abs_velocity() const {
         double m = 0;
-        for (int64_t a = 0; a < n; ++a)
-            if (active(a)) m = std::max({m, std::abs(vx[a]), std::abs(vy[a]), std::abs(vz[a])});
+        for (int64_t a = 0; a < n; ++a) {
+            if (!active(a)) continue;
+            if (!std::isfinite(vx[a]) || !std::isfinite(vy[a]) || !std::isfinite(vz[a])) return INFINITY;
+            m = std::max({m, std::abs(vx[a]), std::abs(vy[a]), std::abs(vz[a])});
+        }
         return m;
     }
 };

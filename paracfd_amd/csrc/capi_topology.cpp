@@ -31,6 +31,9 @@ pcfd_status pcfd_topology_build_nonorthogonal(pcfd_topology* topology) {
     } catch (const std::bad_alloc&) {
         pcfd::set_error("pcfd_topology_build_nonorthogonal: out of host memory");
         return PCFD_ENOMEM;
+    } catch (const std::exception& e) {
+        pcfd::set_error(std::string("pcfd_topology_build_nonorthogonal: ") + e.what());
+        return PCFD_EINVAL;
     }
 }
 
@@ -96,6 +99,9 @@ pcfd_status pcfd_topology_merge(const pcfd_topology* const* parts, int count, pc
     } catch (const std::bad_alloc&) {
         pcfd::set_error("pcfd_topology_merge: out of host memory");
         return PCFD_ENOMEM;
+    } catch (const std::exception& e) {
+        pcfd::set_error(std::string("pcfd_topology_merge: ") + e.what());
+        return PCFD_EINVAL;
     }
 }
 

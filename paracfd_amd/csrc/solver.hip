@@ -35,6 +35,10 @@ void set_error(const std::string& m);
 struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// device allocation refused: out of device memory, or beyond the caller's budget
+struct OutOfMemory : DeviceError {
+    using DeviceError::DeviceError;
+};
 
 #define HIP_CHECK(expr)                                                                                   \
     do {                                                                                                  \
@@ -84,6 +88,8 @@ public:
     virtual int instances() const = 0;
     virtual void sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
                               int nu, int nv, float* out) = 0;
+    // checkpoint resume: the state was uploaded; every instance continues from physical time t
+    virtual void mark_initialized(double t) = 0;
     virtual void probe_start(unsigned mask) = 0;
     virtual void probe_stop(int64_t* launches, double* total_ms) = 0;
     bool initialized = false;
@@ -96,6 +102,8 @@ class SolverImpl final : public Solver {
 public:
     SolverImpl(const Topology& T, const pcfd_options& o);
     ~SolverImpl() override;
+    void construct(const Topology& T);
+    void release();
     void initialize(pcfd_step_stats* st) override;
     void step(pcfd_step_stats* st) override;
     void advect(double dt, double* dt_used, pcfd_step_stats* st) override;
@@ -124,6 +132,11 @@ public:
         }
     }
     void instance_stats(int instance, pcfd_step_stats* st) const override { *st = inst_stats_.at(instance); }
+    void mark_initialized(double t) override {
+        initialized = true;
+        time = t;
+        std::fill(inst_time_.begin(), inst_time_.end(), t);
+    }
     void instance_loads(int instance, double pref, pcfd_loads* out) override { loads_instance(instance, pref, out); }
     void batch_loads(double pref, pcfd_loads* out) override;
     void autotune(int repetitions, pcfd_tuning* out) override;
@@ -137,8 +150,18 @@ private:
     template <class T>
     T* alloc(int64_t count) {
         if (count <= 0) count = 1;
+        const size_t bytes = static_cast<size_t>(count) * sizeof(T);
+        if (opt_.memory_budget_bytes > 0.0 && static_cast<double>(device_bytes) + static_cast<double>(bytes) > opt_.memory_budget_bytes)
+            throw OutOfMemory("device memory of " + std::to_string(device_bytes + static_cast<int64_t>(bytes)) +
+                              " bytes exceeds the budget of " + std::to_string(static_cast<int64_t>(opt_.memory_budget_bytes)) +
+                              " bytes");
         void* p = nullptr;
-        HIP_CHECK(hipMalloc(&p, static_cast<size_t>(count) * sizeof(T)));
+        const hipError_t err = hipMalloc(&p, bytes);
+        if (err == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            throw OutOfMemory("hipMalloc of " + std::to_string(bytes) + " bytes: out of device memory");
+        }
+        HIP_CHECK(err);
         allocations_.push_back(p);
         device_bytes += count * static_cast<int64_t>(sizeof(T));
         return static_cast<T*>(p);
@@ -531,8 +554,19 @@ private:
     int64_t slice_cap_ = 0;
 };
 
+// A constructor that fails part way releases what it has created before the error leaves it.
 template <class R>
 SolverImpl<R>::SolverImpl(const Topology& T, const pcfd_options& o) : opt_(o) {
+    try {
+        construct(T);
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+template <class R>
+void SolverImpl<R>::construct(const Topology& T) {
     if (opt_.pre_smooth < 1 || opt_.post_smooth < 1) throw std::invalid_argument("V-cycle needs at least one pre- and post-smoothing sweep");
     if (opt_.check_interval < 1) opt_.check_interval = 1;
     if (!(opt_.rho > 0) || !(opt_.cfl > 0) || !(opt_.rtol > 0) || opt_.max_iterations < 1 || opt_.nu < 0)
@@ -1019,14 +1053,27 @@ void SolverImpl<R>::setup_smooth_wall(const Topology& T) {
 
 template <class R>
 SolverImpl<R>::~SolverImpl() {
-    // teardown errors cannot be reported from a destructor; they are ignored deliberately
+    release();
+}
+
+// frees every device resource (idempotent); teardown errors cannot be reported from a
+// destructor and are ignored deliberately
+template <class R>
+void SolverImpl<R>::release() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     drop_graph();
     for (void* p : allocations_) (void)hipFree(p);
-    for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    allocations_.clear();
+    for (auto& e : ev_) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     for (auto& e : probe_ev_) if (e) (void)hipEventDestroy(e);
+    probe_ev_.clear();
     if (host_s_) (void)hipHostFree(host_s_);
+    host_s_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
 }
 
 template <class R>
@@ -1341,8 +1388,9 @@ void SolverImpl<R>::mdot(const double* Vb, int nv, const double* w, double* host
 }
 
 // Flexible GMRES(m), right-preconditioned by the V-cycle of the orthogonal operator (which
-// may differ between applications: FP32/BF16 storage), two-pass classical Gram-Schmidt,
-// Givens rotations on the host.  Converged when the true residual ||rhs - A p|| <= rtol ||rhs||,
+// may differ between applications: FP32/BF16 storage), one-pass classical Gram-Schmidt with a
+// second pass only near breakdown (||w|| after the projection below 1e-6 of before), Givens
+// rotations on the host.  Converged when the true residual ||rhs - A p|| <= rtol ||rhs||,
 // checked at the end of every cycle (the reference's criterion, amr_pressure.cu:1430-1440).
 template <class R>
 void SolverImpl<R>::fgmres_solve() {
@@ -2266,6 +2314,9 @@ pcfd_status guarded(const char* where, F&& f) {
     try {
         f();
         return PCFD_OK;
+    } catch (const pcfd::OutOfMemory& e) {
+        pcfd::set_error(std::string(where) + ": " + e.what());
+        return PCFD_ENOMEM;
     } catch (const pcfd::DeviceError& e) {
         pcfd::set_error(std::string(where) + ": " + e.what());
         return PCFD_EDEVICE;
@@ -2436,9 +2487,8 @@ pcfd_status pcfd_set_faces(pcfd_solver* s, const double* fx, const double* fy, c
 }
 pcfd_status pcfd_mark_initialized(pcfd_solver* s, double physical_time) {
     NEED(s, "pcfd_mark_initialized: null solver");
-    s->impl->initialized = true;
-    s->impl->time = physical_time;
-    return PCFD_OK;
+    NEED(std::isfinite(physical_time), "pcfd_mark_initialized: physical time must be finite");
+    return guarded("pcfd_mark_initialized", [&] { s->impl->mark_initialized(physical_time); });
 }
 int64_t pcfd_device_bytes(const pcfd_solver* s) { return s ? s->impl->device_bytes : -1; }
 pcfd_status pcfd_mg_primitive(pcfd_solver* s, int op, int level, const double* x, const double* b, double* out,

@@ -11,6 +11,7 @@
 
 #include <array>
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -133,10 +134,10 @@ struct Topology {
 void build_nonorthogonal(Topology& T);
 Topology::NonOrth compute_nonorthogonal(const Topology& T);
 
-struct BuildError : std::exception {
-    std::string message;
-    explicit BuildError(std::string m) : message(std::move(m)) {}
-    const char* what() const noexcept override { return message.c_str(); }
+// A geometry, grid or topology the builders cannot represent: a bad argument of the caller
+// (reported as PCFD_EINVAL wherever it crosses the C ABI, also from pcfd_create).
+struct BuildError : std::invalid_argument {
+    explicit BuildError(const std::string& m) : std::invalid_argument(m) {}
 };
 
 Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid);

@@ -170,3 +170,100 @@ def test_records_are_structurally_sound(topo):
     k = topo.array("nf_k").reshape(-1, 3)
     axis = np.where(edge >= 0, topo.array("edge_axis")[np.maximum(edge, 0)], -1 - edge)
     assert np.abs(k[np.arange(k.shape[0]), axis]).max() <= 4e-16
+
+
+def _structured_neighbours(t):
+    """(n, 6) structured neighbour dof (-1 where the face is not a structured face)."""
+    n = t.ndof
+    mask = t.array("mask").astype(np.int64)
+    nbr = t.array("brick_nbr").reshape(-1, 6)
+    idx = np.arange(n)
+    b, q = idx // 512, idx % 512
+    ijk = np.stack([q & 7, (q >> 3) & 7, q >> 6], axis=1)
+    out = -np.ones((n, 6), dtype=np.int64)
+    for axis in range(3):
+        for k, sgn in enumerate((-1, 1)):
+            bit = (1 << axis) if sgn > 0 else (8 << axis)
+            has = (mask & bit) != 0
+            c = ijk.copy()
+            c[:, axis] += sgn
+            bb = b.copy()
+            cross = (c[:, axis] < 0) | (c[:, axis] > 7)
+            bb[cross] = nbr[b[cross], 2 * axis + (1 if sgn > 0 else 0)]
+            c[cross, axis] = 0 if sgn > 0 else 7
+            out[has, 2 * axis + k] = (bb * 512 + c[:, 0] + 8 * c[:, 1] + 64 * c[:, 2])[has]
+    return out
+
+
+def _independent_stencils(t, power=2):
+    """node dof -> {neighbour dof: 3-vector weight} of the least-squares gradient, assembled in
+    numpy from the exported topology: every open connection (structured faces with area h^2,
+    compact edges with their open area), weight area / |d|^power, pseudo-inverse of the normal
+    matrix keeping eigenvalues above 1e-4 of the largest (amr_pressure.cpp:108-124)."""
+    cen = t.array("centroid").reshape(-1, 3)
+    h = np.repeat(t.array("level_h")[t.array("brick_level")], 512)
+    act = (t.array("mask") & 64) != 0
+    sn = _structured_neighbours(t)
+    ea, eb, area = t.array("edge_a"), t.array("edge_b"), t.array("edge_area")
+    conn = {}
+    for e in range(len(ea)):
+        conn.setdefault(int(ea[e]), []).append((int(eb[e]), area[e]))
+        conn.setdefault(int(eb[e]), []).append((int(ea[e]), area[e]))
+    out = {}
+    for a in t.array("ng_dof"):
+        a = int(a)
+        nb = [(int(o), h[a] ** 2) for o in sn[a] if o >= 0 and act[o]]
+        nb += [(o, ar) for o, ar in conn.get(a, []) if act[o]]
+        D = np.array([cen[o] - cen[a] for o, _ in nb])
+        w = np.array([ar for _, ar in nb]) / np.sum(D * D, axis=1) ** (power / 2.0)
+        N = (D * w[:, None]).T @ D
+        lam, Q = np.linalg.eigh(N)
+        keep = lam > 1e-4 * lam.max()
+        Ninv = (Q[:, keep] / lam[keep]) @ Q[:, keep].T
+        out[a] = {o: Ninv @ (wk * dk) for (o, _), wk, dk in zip(nb, w, D)}
+    return out
+
+
+@pytest.mark.parametrize("power,ok", [(2, True), (1, False)])
+def test_gradient_stencils_equal_an_independent_assembly(topo, power, ok):
+    """Every ring-1 node's stencil (neighbours and weights) equals the numpy assembly; with the
+    weight area/|d| instead of area/|d|^2 the comparison fails."""
+    ref = _independent_stencils(topo, power)
+    dof, off, nbr = topo.array("ng_dof"), topo.array("ng_off"), topo.array("ng_nbr")
+    gw = topo.array("ng_w").reshape(-1, 3)
+    ring = topo.array("ng_ring")
+    worst, scale = 0.0, 0.0
+    for n in range(len(dof)):
+        if ring[n] != 1:
+            continue
+        want = ref[int(dof[n])]
+        got = {int(nbr[q]): gw[q] for q in range(off[n], off[n + 1])}
+        assert set(got) == set(want)
+        for o, w in want.items():
+            worst = max(worst, float(np.max(np.abs(got[o] - w))))
+            scale = max(scale, float(np.max(np.abs(w))))
+    assert (worst <= 1e-12 * scale) == ok, (worst, scale)
+
+
+def test_face_derivative_of_quadratic_pressure_uses_the_independent_stencils(topo):
+    """For a quadratic p (not reproduced by any stencil) the records' full derivative equals
+    (orthogonal factor + delta)(p_u - p_l) + k . ((1 - w) g_l + w g_u) with the gradients of the
+    independent assembly."""
+    cen = topo.array("centroid").reshape(-1, 3)
+    p = 0.7 * cen[:, 0] ** 2 - 0.4 * cen[:, 1] * cen[:, 2] + 0.9 * cen[:, 2] ** 2 + 0.3 * cen[:, 0]
+    ref = _independent_stencils(topo)
+    gnode = {a: sum(w * (p[o] - p[a]) for o, w in st.items()) for a, st in ref.items()}
+    lo, up = topo.array("nf_lower"), topo.array("nf_upper")
+    ln, un = topo.array("nf_lnode"), topo.array("nf_unode")
+    k, w, delta = topo.array("nf_k").reshape(-1, 3), topo.array("nf_w"), topo.array("nf_delta")
+    ndof = topo.array("ng_dof")
+    want = np.zeros(len(lo))
+    for f in range(len(lo)):
+        g = np.zeros(3)
+        if ln[f] >= 0:
+            g += (1 - w[f]) * gnode[int(ndof[ln[f]])]
+        if un[f] >= 0:
+            g += w[f] * gnode[int(ndof[un[f]])]
+        want[f] = delta[f] * (p[up[f]] - p[lo[f]]) + k[f] @ g
+    got = NO.face_extra_matrix(topo) @ p
+    assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want))

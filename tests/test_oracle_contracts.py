@@ -401,3 +401,18 @@ def test_smagorinsky_step_adds_dissipation(amr_topo, oracle_built):
         x, y, z, _ = orc.state()
         ke.append(float((x * x + y * y + z * z).sum()))
     assert ke[1] < ke[0]
+
+
+def test_oracle_refuses_a_nonfinite_state(sphere_topo, oracle_built):
+    """A NaN velocity before a step is PCFD_ENONFINITE (external_aero_core.cpp:308), and stepping
+    before a converged initialization is PCFD_ESTATE (:306)."""
+    orc = Oracle(sphere_topo, options())
+    with pytest.raises(RuntimeError, match="status 5"):
+        orc.step()
+    assert orc.initialize().converged
+    x, y, z, _ = orc.state()
+    act = np.flatnonzero((sphere_topo.array("mask") & 64) != 0)
+    x[act[len(act) // 3]] = np.nan
+    orc.set_state(x, y, z, None)
+    with pytest.raises(RuntimeError, match="status 4"):
+        orc.step()
