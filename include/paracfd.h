@@ -385,6 +385,14 @@ int64_t pcfd_device_bytes(const pcfd_solver* solver);
  * (nu * nv floats); only the image crosses PCIe. */
 pcfd_status pcfd_sample_slice(pcfd_solver* solver, int instance, int field, int axis, double coord, const double lo[2],
                               const double hi[2], int nu, int nv, float* out);
+/* pcfd_sample_slice without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (a viewer's render loop, code under HIP graph capture, harnesses
+ * that forbid hipMalloc).  The image is staged in a fixed device array of 2^20 floats per GPU,
+ * held until the process ends; nu * nv above 2^20 is PCFD_EINVAL.  Calls on one GPU share that
+ * array, so calls that overlap on one GPU (other threads, other solvers) race.  Same arguments,
+ * results and errors as pcfd_sample_slice otherwise. */
+pcfd_status pcfd_sample_slice_static(pcfd_solver* solver, int instance, int field, int axis, double coord,
+                                     const double lo[2], const double hi[2], int nu, int nv, float* out);
 
 /* Multigrid primitives on level `level` (0 = composite level), host arrays in, host array out.
  * op: 0 apply A x, 1 residual b - A x, 2 damped-Jacobi sweeps on x (omega, sweeps),

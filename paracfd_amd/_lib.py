@@ -136,7 +136,7 @@ EXPORTED = [
     "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
     "pcfd_integrate_loads_instance", "pcfd_set_reference_instance", "pcfd_integrate_loads_batch",
     "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning", "pcfd_sample_slice", "pcfd_probe_start",
-    "pcfd_probe_stop",
+    "pcfd_probe_stop", "pcfd_sample_slice_static",
 ]
 
 _lib = None
@@ -208,6 +208,7 @@ def load():
         "pcfd_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pcfd_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pcfd_sample_slice": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_int, C.c_int, fp]),
+        "pcfd_sample_slice_static": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_int, C.c_int, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

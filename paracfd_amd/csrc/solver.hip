@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -54,6 +55,31 @@ struct OutOfMemory : DeviceError {
 
 static inline unsigned blocks_for(int64_t n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
 
+// Fixed device staging buffer of pcfd_sample_slice_static, one copy per GPU for the life of the
+// process (4 MiB: images of at most 2^20 pixels).  In host code `scratch` names a host-side
+// stand-in, not a device address: kernels and copies take the address that
+// hipGetSymbolAddress returns on each GPU (scratch_address).  Every call on one GPU shares its
+// copy, so calls that overlap on one GPU race.
+constexpr int64_t kScratchFloats = int64_t(1) << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+// device address of `scratch` on each GPU, looked up once per device with that device current
+std::mutex g_scratch_mutex;
+std::vector<void*> g_scratch_address;
+
+void* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0) throw DeviceError("scratch: invalid device");
+    if (static_cast<size_t>(device) >= g_scratch_address.size()) g_scratch_address.resize(device + 1, nullptr);
+    if (!g_scratch_address[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_address[device] = p;
+    }
+    return g_scratch_address[device];
+}
+
 class Solver {
 public:
     virtual ~Solver() = default;
@@ -88,6 +114,8 @@ public:
     virtual int instances() const = 0;
     virtual void sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
                               int nu, int nv, float* out) = 0;
+    virtual void sample_slice_static(int instance, int field, int axis, double coord, const double* lo,
+                                     const double* hi, int nu, int nv, float* out) = 0;
     // checkpoint resume: the state was uploaded; every instance continues from physical time t
     virtual void mark_initialized(double t) = 0;
     virtual void probe_start(unsigned mask) = 0;
@@ -145,8 +173,16 @@ public:
     int instances() const override { return ninst_; }
     void sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi, int nu,
                       int nv, float* out) override;
+    void sample_slice_static(int instance, int field, int axis, double coord, const double* lo, const double* hi,
+                             int nu, int nv, float* out) override;
 
 private:
+    void build_slice_map(int instance);
+    void check_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi, int nu,
+                     int nv, int64_t max_pixels) const;
+    void slice_into(int instance, int field, int axis, double coord, const double* lo, const double* hi, int nu,
+                    int nv, float* image, float* out);
+    void* scratch_ = nullptr;   // this GPU's address of `scratch`
     template <class T>
     T* alloc(int64_t count) {
         if (count <= 0) count = 1;
@@ -870,6 +906,12 @@ void SolverImpl<R>::construct(const Topology& T) {
     level_h_h_ = T.level_h;
     slice_map_.assign(ninst_, nullptr);
     slice_geo_.assign(ninst_, SliceMap{});
+    for (int i = 0; i < ninst_; ++i) build_slice_map(i);
+    {
+        int device = 0;
+        HIP_CHECK(hipGetDevice(&device));
+        scratch_ = scratch_address(device);
+    }
     HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -1970,56 +2012,88 @@ void SolverImpl<R>::restore_state() {
     wall_applied_ = snap_wall_applied_;
 }
 
-// Field of one instance on a plane (an image of nu x nv values, NaN where solid or outside).
-// The brick maps of the instance are built on the first call; the image is copied to the host.
+// Per-level maps of brick coordinates to the instance's stored bricks, for the slice sampler;
+// built for every instance when the solver is created.  An instance of a batch whose bricks do
+// not fit the batch's first grid (merged cases of different domains) gets no map; sampling it
+// is refused (check_slice).
 template <class R>
-void SolverImpl<R>::sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
-                                 int nu, int nv, float* out) {
+void SolverImpl<R>::build_slice_map(int instance) {
+    const int levels = static_cast<int>(level_h_h_.size());
+    if (levels > 8) return;
+    SliceMap G{};
+    G.levels = levels;
+    int64_t total = 0;
+    for (int l = 0; l < levels; ++l) {
+        G.off[l] = total;
+        for (int d = 0; d < 3; ++d) G.dims[l][d] = base_bricks_[d] << l;
+        total += static_cast<int64_t>(G.dims[l][0]) * G.dims[l][1] * G.dims[l][2];
+        G.h[l] = level_h_h_[l];
+    }
+    for (int d = 0; d < 3; ++d) G.lo[d] = grid_lo_[d];
+    std::vector<int> map(total, -1);
+    for (int b = inst_brick_off_[instance]; b < inst_brick_off_[instance + 1]; ++b) {
+        const int l = brick_level_h_[b];
+        const int* c = &brick_coord_h_[3 * static_cast<size_t>(b)];
+        for (int d = 0; d < 3; ++d)
+            if (c[d] < 0 || c[d] >= G.dims[l][d]) return;
+        map[G.off[l] + (static_cast<int64_t>(c[2]) * G.dims[l][1] + c[1]) * G.dims[l][0] + c[0]] = b;
+    }
+    slice_map_[instance] = upload(map);
+    slice_geo_[instance] = G;
+}
+
+// Field of one instance on a plane (an image of nu x nv values, NaN where solid or outside),
+// sampled into the device buffer `image` (at least nu * nv floats) and copied to the host.
+template <class R>
+void SolverImpl<R>::slice_into(int instance, int field, int axis, double coord, const double* lo, const double* hi,
+                               int nu, int nv, float* image, float* out) {
+    SliceMap G = slice_geo_[instance];
+    G.map = slice_map_[instance];
+    const int64_t npix = static_cast<int64_t>(nu) * nv;
+    const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed;
+    LAUNCH(k_slice<R>, blocks_for(npix, 256), 256, M_, fields(), p_, G, field, axis, coord, lo[0], hi[0], lo[1], hi[1],
+           nu, nv, field == 5 ? q : 1.0, image);
+    HIP_CHECK(hipMemcpyAsync(out, image, npix * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <class R>
+void SolverImpl<R>::check_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
+                                int nu, int nv, int64_t max_pixels) const {
     if (instance < 0 || instance >= ninst_) throw std::invalid_argument("sample_slice: no such instance");
+    if (level_h_h_.size() > 8) throw std::invalid_argument("sample_slice: at most 8 AMR levels");
+    if (!slice_map_[instance])
+        throw std::invalid_argument("sample_slice: the instance's bricks lie outside the batch's first grid");
     if (field < 0 || field > 5) throw std::invalid_argument("sample_slice: field must be 0..5");
     if (axis < 0 || axis > 2) throw std::invalid_argument("sample_slice: axis must be 0, 1 or 2");
-    if (nu < 1 || nv < 1 || static_cast<int64_t>(nu) * nv > (int64_t(1) << 26))
-        throw std::invalid_argument("sample_slice: image size must be 1..2^26 pixels");
+    if (nu < 1 || nv < 1 || static_cast<int64_t>(nu) * nv > max_pixels)
+        throw std::invalid_argument("sample_slice: image size must be 1.." + std::to_string(max_pixels) + " pixels");
     if (!std::isfinite(coord) || !(lo[0] < hi[0]) || !(lo[1] < hi[1]) || !std::isfinite(hi[0] - lo[0]) ||
         !std::isfinite(hi[1] - lo[1]))
         throw std::invalid_argument("sample_slice: invalid plane or window");
-    const int levels = static_cast<int>(level_h_h_.size());
-    if (levels > 8) throw std::invalid_argument("sample_slice: at most 8 AMR levels");
     const double q = 0.5 * opt_.rho * opt_.speed * opt_.speed;
     if (field == 5 && !(q > 0.0)) throw std::invalid_argument("sample_slice: Cp needs a positive dynamic pressure");
-    if (!slice_map_[instance]) {
-        SliceMap G{};
-        G.levels = levels;
-        int64_t total = 0;
-        for (int l = 0; l < levels; ++l) {
-            G.off[l] = total;
-            for (int d = 0; d < 3; ++d) G.dims[l][d] = base_bricks_[d] << l;
-            total += static_cast<int64_t>(G.dims[l][0]) * G.dims[l][1] * G.dims[l][2];
-            G.h[l] = level_h_h_[l];
-        }
-        for (int d = 0; d < 3; ++d) G.lo[d] = grid_lo_[d];
-        std::vector<int> map(total, -1);
-        for (int b = inst_brick_off_[instance]; b < inst_brick_off_[instance + 1]; ++b) {
-            const int l = brick_level_h_[b];
-            const int* c = &brick_coord_h_[3 * static_cast<size_t>(b)];
-            for (int d = 0; d < 3; ++d)
-                if (c[d] < 0 || c[d] >= G.dims[l][d]) throw std::logic_error("sample_slice: brick outside its level grid");
-            map[G.off[l] + (static_cast<int64_t>(c[2]) * G.dims[l][1] + c[1]) * G.dims[l][0] + c[0]] = b;
-        }
-        slice_map_[instance] = upload(map);
-        slice_geo_[instance] = G;
-    }
-    SliceMap G = slice_geo_[instance];
-    G.map = slice_map_[instance];
+}
+
+// The image is staged in a device buffer of the solver, grown when a larger image is asked for.
+template <class R>
+void SolverImpl<R>::sample_slice(int instance, int field, int axis, double coord, const double* lo, const double* hi,
+                                 int nu, int nv, float* out) {
+    check_slice(instance, field, axis, coord, lo, hi, nu, nv, int64_t(1) << 26);
     const int64_t npix = static_cast<int64_t>(nu) * nv;
     if (npix > slice_cap_) {
         slice_buf_ = alloc<float>(npix);
         slice_cap_ = npix;
     }
-    LAUNCH(k_slice<R>, blocks_for(npix, 256), 256, M_, fields(), p_, G, field, axis, coord, lo[0], hi[0], lo[1], hi[1],
-           nu, nv, field == 5 ? q : 1.0, slice_buf_);
-    HIP_CHECK(hipMemcpyAsync(out, slice_buf_, npix * sizeof(float), hipMemcpyDeviceToHost, stream_));
-    HIP_CHECK(hipStreamSynchronize(stream_));
+    slice_into(instance, field, axis, coord, lo, hi, nu, nv, slice_buf_, out);
+}
+
+// The same image staged in this GPU's copy of the static array `scratch` (no allocation).
+template <class R>
+void SolverImpl<R>::sample_slice_static(int instance, int field, int axis, double coord, const double* lo,
+                                        const double* hi, int nu, int nv, float* out) {
+    check_slice(instance, field, axis, coord, lo, hi, nu, nv, kScratchFloats);
+    slice_into(instance, field, axis, coord, lo, hi, nu, nv, static_cast<float*>(scratch_), out);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2454,6 +2528,13 @@ pcfd_status pcfd_sample_slice(pcfd_solver* s, int instance, int field, int axis,
                               const double hi[2], int nu, int nv, float* out) {
     NEED(s && lo && hi && out, "pcfd_sample_slice: null argument");
     return guarded("pcfd_sample_slice", [&] { s->impl->sample_slice(instance, field, axis, coord, lo, hi, nu, nv, out); });
+}
+
+pcfd_status pcfd_sample_slice_static(pcfd_solver* s, int instance, int field, int axis, double coord, const double lo[2],
+                                     const double hi[2], int nu, int nv, float* out) {
+    NEED(s && lo && hi && out, "pcfd_sample_slice_static: null argument");
+    return guarded("pcfd_sample_slice_static",
+                   [&] { s->impl->sample_slice_static(instance, field, axis, coord, lo, hi, nu, nv, out); });
 }
 
 pcfd_status pcfd_integrate_loads_instance(pcfd_solver* s, int instance, double pref, pcfd_loads* loads) {
