@@ -343,6 +343,43 @@ def kernel_probe(args):
                       "copy_bytes": nbytes, "copies": 3, "xcd_chunk_log2": args.probe_chunk}), flush=True)
 
 
+def topology_build(args):
+    """Host build and device build of the case's topology: stage times and bit-identity of every
+    exported array (topology and multigrid levels)."""
+    from paracfd_amd import _lib as L, cases
+    g, d, spec = cases.build_case(args.case)
+    res = {}
+    topos = {}
+    for where, dev in (("device", 0), ("host", None)):
+        log(f"{where} build of {args.case}")
+        t0 = time.perf_counter()
+        topos[where] = L.Topology(g, d, device=dev)
+        wall = time.perf_counter() - t0
+        stages, _ = topos[where].build_times()
+        res[where] = {"wall_s": wall, "stages_ms": {k: round(v, 1) for k, v in stages.items()}}
+    h, dv = topos["host"], topos["device"]
+    identical, values = True, 0
+    for key in L.ARR:
+        if key.startswith(("nf_", "ng_")):
+            continue
+        a, b = h.array(key), dv.array(key)
+        identical &= a.dtype == b.dtype and a.tobytes() == b.tobytes()
+        values += a.size
+    for level in range(h.mg_levels):
+        for key in L.MG_ARR:
+            try:
+                a = h.mg_array(level, key)
+            except KeyError:
+                continue
+            identical &= a.tobytes() == dv.mg_array(level, key).tobytes()
+            values += a.size
+    print(json.dumps({"workload": args.case, "active_cells": h.n_active, "special_cells": int(h.info.special_cells),
+                      "compact_edges": int(h.info.compact_edges), "bit_identical": bool(identical),
+                      "values_compared": int(values), "host": res["host"], "device": res["device"],
+                      "speedup_total": res["host"]["stages_ms"]["total"] / res["device"]["stages_ms"]["total"]}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +408,11 @@ def main():
                     help="keep the default launch shapes instead of autotuning them before the timed steps")
     ap.add_argument("--kernel-probe", type=int, default=None,
                     help="only launch hot kernel K (0 apply, 1 V-cycle sweep, 2 transport) and calibration copies")
+    ap.add_argument("--host-topology", action="store_true",
+                    help="build the topology on the host threads (default: device-side preprocessing)")
+    ap.add_argument("--topology-build", action="store_true",
+                    help="only build the case's topology on the host and on the device, check that every exported "
+                         "array is identical and print the build stage times as one JSON line")
     ap.add_argument("--probe-reps", type=int, default=20)
     ap.add_argument("--probe-chunk", type=int, default=4)
     args = ap.parse_args()
@@ -390,14 +432,19 @@ def main():
     if args.kernel_probe is not None:
         kernel_probe(args)
         return
+    if args.topology_build:
+        topology_build(args)
+        return
     if args.case in cases.SWEEPS:
         run_sweep(args)
         return
     g, d, spec = cases.build_case(args.case)
     t0 = time.perf_counter()
-    topo = L.Topology(g, d)
+    topo = L.Topology(g, d) if args.host_topology else L.Topology(g, d, device=0)
+    build_wall = time.perf_counter() - t0
+    build_stages, build_device = topo.build_times()
     log(f"topology {args.case}: {topo.n_active} active cells, {topo.info.bricks} bricks, "
-        f"{topo.mg_levels} MG levels, built in {time.perf_counter() - t0:.1f}s")
+        f"{topo.mg_levels} MG levels, built in {build_wall:.1f}s (device {build_device})")
 
     vc = L.BF16 if args.vcycle == "bf16" else L.FP32
     learned = run(topo, spec, args, "learned", L.FP32, vc, L.PC_LEARNED)
@@ -516,6 +563,8 @@ def main():
                    "speedup_vs_plain_fp32_vcycle": learned["value"] / plain32["value"],
                    "speedup_vs_plain_fp64_vcycle": learned["value"] / plain64["value"],
                    "errors_vs_fp64": errors},
+        "topology_build": {"device": build_device, "wall_s": build_wall,
+                           "stages_ms": {k: round(v, 1) for k, v in build_stages.items()}},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "roofline_network": nn_roof,

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PCFD_ABI_VERSION 4
+#define PCFD_ABI_VERSION 5
 
 typedef enum pcfd_status {
     PCFD_OK = 0,
@@ -269,7 +269,19 @@ int pcfd_abi_version(void);
  * build_composite_amr_pressure_system). Host only; needs no GPU. */
 pcfd_status pcfd_topology_build(const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid,
                                 pcfd_topology** out);
+/* The same build with the cut-cell sub-sampling (the reference's cut-cell classification,
+ * amr_eb.cpp:82-375, here by sub-sample counting on the analytic body) run on HIP device
+ * `device`: every exported array is bit-identical to pcfd_topology_build's.  Needs
+ * grid->subsamples <= 16.  Device failures are PCFD_EDEVICE. */
+pcfd_status pcfd_topology_build_on_device(const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid,
+                                          int device, pcfd_topology** out);
 void pcfd_topology_free(pcfd_topology* topology);
+/* Wall time of each build stage in milliseconds (no counterpart in the reference): 0 body and
+ * domain, 1 refinement and nesting, 2 cut list, 3 cut-cell sub-sampling (device: including
+ * the transfers), 4 small-cell removal and connected fluid, 5 bricks / masks / compact edges /
+ * wall patches, 6 multigrid, 7 total, 8 the device sampling kernel alone (0 on the host);
+ * *device = the device that sub-sampled, or -1.  Zero for a merged batch. */
+pcfd_status pcfd_topology_build_times(const pcfd_topology* topology, double ms[9], int* device);
 pcfd_status pcfd_topology_get_info(const pcfd_topology* topology, pcfd_topology_info* info);
 /* Build the non-orthogonal correction of the pressure operator into the topology
  * (finalize_nonorthogonal_pressure_topology, amr_pressure.cpp:63-190): per-face centroid

@@ -136,7 +136,7 @@ EXPORTED = [
     "pcfd_learned_gradient", "pcfd_topology_merge", "pcfd_topology_instance", "pcfd_instance_stats",
     "pcfd_integrate_loads_instance", "pcfd_set_reference_instance", "pcfd_integrate_loads_batch",
     "pcfd_autotune", "pcfd_get_tuning", "pcfd_set_tuning", "pcfd_sample_slice", "pcfd_probe_start",
-    "pcfd_probe_stop", "pcfd_sample_slice_static",
+    "pcfd_probe_stop", "pcfd_sample_slice_static", "pcfd_topology_build_on_device", "pcfd_topology_build_times",
 ]
 
 _lib = None
@@ -164,6 +164,9 @@ def load():
         "pcfd_last_error": (C.c_char_p, []),
         "pcfd_abi_version": (C.c_int, []),
         "pcfd_topology_build": (C.c_int, [C.POINTER(GeometryDesc), C.POINTER(GridDesc), C.POINTER(P)]),
+        "pcfd_topology_build_on_device": (C.c_int, [C.POINTER(GeometryDesc), C.POINTER(GridDesc), C.c_int,
+                                                    C.POINTER(P)]),
+        "pcfd_topology_build_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "pcfd_topology_free": (None, [P]),
         "pcfd_topology_get_info": (C.c_int, [P, C.POINTER(TopologyInfo)]),
         "pcfd_topology_array_size": (C.c_int64, [P, C.c_int]),
@@ -247,12 +250,20 @@ def fptr(a):
 class Topology:
     """Host topology (bricks, cut cells, compact edges, patches, multigrid hierarchy)."""
 
-    def __init__(self, geometry: GeometryDesc = None, grid: GridDesc = None, _handle=None):
+    BUILD_STAGES = ("geometry", "refine", "cut_list", "sampling", "components", "finalize", "multigrid", "total",
+                    "device_kernel")
+
+    def __init__(self, geometry: GeometryDesc = None, grid: GridDesc = None, _handle=None, device=None):
+        """Build on the host (pcfd_topology_build), or with the cut cells sub-sampled on HIP
+        device `device` (pcfd_topology_build_on_device: bit-identical arrays)."""
         lib = load()
         self._lib = lib
         if _handle is None:
             h = C.c_void_p()
-            check(lib.pcfd_topology_build(C.byref(geometry), C.byref(grid), C.byref(h)))
+            if device is None:
+                check(lib.pcfd_topology_build(C.byref(geometry), C.byref(grid), C.byref(h)))
+            else:
+                check(lib.pcfd_topology_build_on_device(C.byref(geometry), C.byref(grid), int(device), C.byref(h)))
         else:
             h = _handle
         self.handle = h
@@ -272,6 +283,14 @@ class Topology:
     @property
     def instances(self):
         return int(self.info.instances)
+
+    def build_times(self):
+        """Wall milliseconds per build stage (pcfd_topology_build_times) and the sampling device
+        (-1: host)."""
+        ms = (C.c_double * 9)()
+        dev = C.c_int()
+        check(self._lib.pcfd_topology_build_times(self.handle, ms, C.byref(dev)))
+        return dict(zip(self.BUILD_STAGES, list(ms))), int(dev.value)
 
     def instance(self, i):
         info = InstanceInfo()

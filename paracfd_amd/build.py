@@ -17,7 +17,10 @@ ARCH = os.environ.get("PCFD_OFFLOAD_ARCH", "gfx950")
 # wrong results on this part now and then, with no error
 HIP_FLAGS = ["-std=c++20", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-munsafe-fp-atomics", "-Wno-unused-result"]
-HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC"]
+# no fused multiply-add in the host preprocessing or the device build: the two evaluate the
+# shared per-cell code (solid.hpp, build_graph.hpp) with the same IEEE operations and agree bit
+# for bit
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-ffp-contract=off"]
 
 
 def _run(cmd):
@@ -34,6 +37,9 @@ def build(verbose=True):
         objs.append(obj)
     obj = os.path.join(BUILD, "solver.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "solver.hip"), "-o", obj])
+    objs.append(obj)
+    obj = os.path.join(BUILD, "build_device.o")
+    _run([HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-c", os.path.join(CSRC, "build_device.hip"), "-o", obj])
     objs.append(obj)
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"])

@@ -15,6 +15,30 @@ struct pcfd_topology {
     pcfd::Topology t;
 };
 
+namespace pcfd {
+pcfd_status build_topology_handle(const char* where, const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid,
+                                  BuildAccelerator* accel, pcfd_topology** out) {
+    if (!geometry || !grid || !out) { set_error(std::string(where) + ": null argument"); return PCFD_EINVAL; }
+    *out = nullptr;
+    try {
+        *out = new pcfd_topology{build_topology(*geometry, *grid, accel)};
+        return PCFD_OK;
+    } catch (const BuildError& e) {
+        set_error(e.what());
+        return PCFD_EINVAL;
+    } catch (const DeviceError& e) {
+        set_error(std::string(where) + ": " + e.what());
+        return PCFD_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error(std::string(where) + ": out of host memory");
+        return PCFD_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(std::string(where) + ": " + e.what());
+        return PCFD_EINVAL;
+    }
+}
+}  // namespace pcfd
+
 extern "C" {
 
 const char* pcfd_last_error(void) { return pcfd::g_last_error.c_str(); }
@@ -38,22 +62,14 @@ pcfd_status pcfd_topology_build_nonorthogonal(pcfd_topology* topology) {
 }
 
 pcfd_status pcfd_topology_build(const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid, pcfd_topology** out) {
-    if (!geometry || !grid || !out) { pcfd::set_error("pcfd_topology_build: null argument"); return PCFD_EINVAL; }
-    *out = nullptr;
-    try {
-        auto* t = new pcfd_topology{pcfd::build_topology(*geometry, *grid)};
-        *out = t;
-        return PCFD_OK;
-    } catch (const pcfd::BuildError& e) {
-        pcfd::set_error(e.what());
-        return PCFD_EINVAL;
-    } catch (const std::bad_alloc&) {
-        pcfd::set_error("pcfd_topology_build: out of host memory");
-        return PCFD_ENOMEM;
-    } catch (const std::exception& e) {
-        pcfd::set_error(e.what());
-        return PCFD_EINVAL;
-    }
+    return pcfd::build_topology_handle("pcfd_topology_build", geometry, grid, nullptr, out);
+}
+
+pcfd_status pcfd_topology_build_times(const pcfd_topology* topology, double ms[9], int* device) {
+    if (!topology || !ms || !device) { pcfd::set_error("pcfd_topology_build_times: null argument"); return PCFD_EINVAL; }
+    for (int k = 0; k < 9; ++k) ms[k] = topology->t.build_ms[k];
+    *device = topology->t.sampled_on_device;
+    return PCFD_OK;
 }
 
 void pcfd_topology_free(pcfd_topology* topology) { delete topology; }

@@ -33,9 +33,6 @@ namespace pcfd {
 
 void set_error(const std::string& m);
 
-struct DeviceError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
 // device allocation refused: out of device memory, or beyond the caller's budget
 struct OutOfMemory : DeviceError {
     using DeviceError::DeviceError;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -58,9 +59,14 @@ inline uint64_t morton(int x, int y, int z) {
 // ------------------------------------------------------------------------------------------
 class Geometry {
 public:
+    Geometry() = default;
+    Geometry(const Geometry&) = delete;
+    Geometry& operator=(const Geometry&) = delete;
     virtual ~Geometry() = default;
-    virtual bool solid(double x, double y, double z) const = 0;
+    bool solid(double x, double y, double z) const { return solid_at(desc, x, y, z); }
     virtual void surface(double spacing, std::vector<double>& pts) const = 0;
+    // the body as the shared point-in-solid test sees it (polygon pointers into this object)
+    SolidDesc desc;
     double aabb_lo[3] = {0, 0, 0}, aabb_hi[3] = {-1, -1, -1};
     double ref_area = 0;
     bool empty() const { return aabb_hi[0] < aabb_lo[0]; }
@@ -68,7 +74,6 @@ public:
 
 class NoGeometry final : public Geometry {
 public:
-    bool solid(double, double, double) const override { return false; }
     void surface(double, std::vector<double>&) const override {}
 };
 
@@ -77,10 +82,9 @@ public:
     Sphere(const double c[3], double r) : r_(r) {
         for (int a = 0; a < 3; ++a) { c_[a] = c[a]; aabb_lo[a] = c[a] - r; aabb_hi[a] = c[a] + r; }
         ref_area = kPi * r * r;
-    }
-    bool solid(double x, double y, double z) const override {
-        const double dx = x - c_[0], dy = y - c_[1], dz = z - c_[2];
-        return dx * dx + dy * dy + dz * dz < r_ * r_;
+        desc.kind = SOLID_SPHERE;
+        for (int a = 0; a < 3; ++a) desc.c[a] = c[a];
+        desc.r = r;
     }
     void surface(double spacing, std::vector<double>& pts) const override {
         const int n = std::max(64, static_cast<int>(4 * kPi * r_ * r_ / (spacing * spacing)) + 1);
@@ -98,7 +102,8 @@ private:
     double c_[3], r_;
 };
 
-// 2-D closed polygon with a bucketed vertical-ray crossing test.
+// 2-D closed polygon with a bucketed vertical-ray crossing test (the test itself is
+// poly_inside, shared with the device sampler).
 class Polygon2 {
 public:
     void set(std::vector<double> x, std::vector<double> z) {
@@ -106,44 +111,46 @@ public:
         xmin_ = *std::min_element(x_.begin(), x_.end());
         xmax_ = *std::max_element(x_.begin(), x_.end());
         nb_ = 512;
-        buckets_.assign(nb_, {});
+        SolidDesc G;
+        describe(G);
+        std::vector<std::vector<int>> buckets(nb_);
         const int n = static_cast<int>(x_.size());
         for (int s = 0; s < n; ++s) {
             const int t = (s + 1) % n;
             const double lo = std::min(x_[s], x_[t]), hi = std::max(x_[s], x_[t]);
-            const int b0 = bucket(lo), b1 = bucket(hi);
-            for (int b = b0; b <= b1; ++b) buckets_[b].push_back(s);
+            const int b0 = poly_bucket(G, lo), b1 = poly_bucket(G, hi);
+            for (int b = b0; b <= b1; ++b) buckets[b].push_back(s);
+        }
+        off_.assign(nb_ + 1, 0);
+        seg_.clear();
+        for (int b = 0; b < nb_; ++b) {
+            seg_.insert(seg_.end(), buckets[b].begin(), buckets[b].end());
+            off_[b + 1] = static_cast<int>(seg_.size());
         }
     }
+    // the polygon fields of a solid description (pointers into this polygon)
+    void describe(SolidDesc& G) const {
+        G.px = x_.data(); G.pz = z_.data(); G.np = static_cast<int>(x_.size());
+        G.xmin = xmin_; G.xmax = xmax_; G.nb = nb_;
+        G.boff = off_.data(); G.bseg = seg_.data();
+    }
     bool inside(double x, double z) const {
-        if (x < xmin_ || x > xmax_) return false;
-        const auto& list = buckets_[bucket(x)];
-        const int n = static_cast<int>(x_.size());
-        int crossings = 0;
-        for (int s : list) {
-            const int t = (s + 1) % n;
-            const double xs = x_[s], xt = x_[t];
-            if ((xs <= x && x < xt) || (xt <= x && x < xs)) {
-                const double zc = z_[s] + (z_[t] - z_[s]) * (x - xs) / (xt - xs);
-                if (zc > z) ++crossings;
-            }
-        }
-        return crossings & 1;
+        SolidDesc G;
+        describe(G);
+        return poly_inside(G, x, z);
     }
     const std::vector<double>& xs() const { return x_; }
     const std::vector<double>& zs() const { return z_; }
+    const std::vector<int>& bucket_off() const { return off_; }
+    const std::vector<int>& bucket_seg() const { return seg_; }
     double xmin() const { return xmin_; }
     double xmax() const { return xmax_; }
 
 private:
-    int bucket(double x) const {
-        int b = static_cast<int>((x - xmin_) / (xmax_ - xmin_) * nb_);
-        return std::clamp(b, 0, nb_ - 1);
-    }
     std::vector<double> x_, z_;
     double xmin_ = 0, xmax_ = 0;
     int nb_ = 1;
-    std::vector<std::vector<int>> buckets_;
+    std::vector<int> off_ = {0, 0}, seg_;
 };
 
 // NACA 4-digit section in chord units: camber line and closed-trailing-edge thickness with
@@ -188,17 +195,15 @@ public:
         }
         aabb_lo[1] = le_[1] - half_span_; aabb_hi[1] = le_[1] + half_span_;
         ref_area = g.chord * g.span;
+        desc.kind = SOLID_NACA_WING;
+        for (int a = 0; a < 3; ++a) desc.le[a] = le_[a];
+        desc.ca = ca_; desc.sa = sa_; desc.chord = chord_; desc.half_span = half_span_;
+        poly_.describe(desc);
     }
     void to_world(double xl, double zl, double y, double w[3]) const {
         w[0] = le_[0] + chord_ * (xl * ca_ + zl * sa_);
         w[1] = le_[1] + y;
         w[2] = le_[2] + chord_ * (-xl * sa_ + zl * ca_);
-    }
-    bool solid(double x, double y, double z) const override {
-        if (std::abs(y - le_[1]) > half_span_) return false;
-        const double dx = x - le_[0], dz = z - le_[2];
-        const double xl = (dx * ca_ - dz * sa_) / chord_, zl = (dx * sa_ + dz * ca_) / chord_;
-        return poly_.inside(xl, zl);
     }
     void surface(double spacing, std::vector<double>& pts) const override {
         // resample the section outline by arc length
@@ -295,6 +300,11 @@ public:
         zmin_ = *std::min_element(pz.begin(), pz.end());
         zmax_ = *std::max_element(pz.begin(), pz.end());
         ref_area = area;
+        desc.kind = SOLID_PARAGLIDER;
+        for (int a = 0; a < 3; ++a) desc.le[a] = le_[a];
+        desc.ca = ca_; desc.sa = sa_;
+        desc.b = b_; desc.c0 = c0_; desc.R = R_; desc.tsw = tsw_; desc.flat = flat_ ? 1 : 0;
+        poly_.describe(desc);
         // bounding box from a dense sample of the surface
         std::vector<double> pts;
         surface(0.02 * c0_, pts);
@@ -306,11 +316,8 @@ public:
                 aabb_hi[a] = std::max(aabb_hi[a], pts[i + a] + 0.02 * c0_);
             }
     }
-    double chord(double s) const {
-        const double e = 2.0 * s / b_;
-        return c0_ * std::sqrt(std::max(0.0, 1.0 - e * e));
-    }
-    double x_le(double s) const { return 0.25 * (c0_ - chord(s)) + std::abs(s) * tsw_; }
+    double chord(double s) const { return glider_chord(desc, s); }
+    double x_le(double s) const { return glider_x_le(desc, s); }
     // wing frame (before pitch) -> world
     void to_world(double xw, double yw, double zw, double w[3]) const {
         w[0] = le_[0] + xw * ca_ + zw * sa_;
@@ -328,21 +335,6 @@ public:
             z = rad * std::cos(th) - R_;
         }
         to_world(x_le(s) + c * xi, y, z, w);
-    }
-    bool solid(double x, double y, double z) const override {
-        const double dx = x - le_[0], dz = z - le_[2], yw = y - le_[1];
-        const double xw = dx * ca_ - dz * sa_, zw = dx * sa_ + dz * ca_;
-        double s, off;
-        if (flat_) { s = yw; off = zw; }
-        else {
-            const double zc = zw + R_;
-            s = R_ * std::atan2(yw, zc);
-            off = std::hypot(yw, zc) - R_;
-        }
-        if (std::abs(s) > 0.49 * b_) return false;
-        const double c = chord(s);
-        if (!(c > 0)) return false;
-        return poly_.inside((xw - x_le(s)) / c, off / c);
     }
     void surface(double spacing, std::vector<double>& pts) const override {
         const auto& px = poly_.xs(); const auto& pz = poly_.zs();
@@ -445,36 +437,9 @@ private:
     std::vector<double> pts_;
 };
 
-// Weight of the upper cell when a cell value is interpolated to a face at x_f between the
-// centroids c_l and c_u (composite_face_interpolation_weight, amr_advection.cu:732-753): the
-// face's position between the centroids along the axis when it lies between them (within
-// 1e-8), else the distance weight |x_f - c_l| / (|x_f - c_l| + |x_f - c_u|); clamped to [0, 1].
-double face_weight(const double* cl, const double* cu, const double* xf, int axis) {
-    double d2 = 0;
-    for (int r = 0; r < 3; ++r) d2 += (cu[r] - cl[r]) * (cu[r] - cl[r]);
-    const double dist = std::sqrt(d2), normal = cu[axis] - cl[axis];
-    if (!(dist > 0) || !std::isfinite(dist)) throw BuildError("compact face between coincident centroids");
-    if (std::abs(normal) > 1e-12 * dist) {
-        const double w = (xf[axis] - cl[axis]) / normal;
-        if (std::isfinite(w) && w >= -1e-8 && w <= 1 + 1e-8) return std::clamp(w, 0.0, 1.0);
-    }
-    double dl = 0, du = 0;
-    for (int r = 0; r < 3; ++r) {
-        dl += (xf[r] - cl[r]) * (xf[r] - cl[r]);
-        du += (xf[r] - cu[r]) * (xf[r] - cu[r]);
-    }
-    dl = std::sqrt(dl);
-    du = std::sqrt(du);
-    const double total = dl + du;
-    if (!(total > 1e-14 * std::max(1.0, dist)) || !std::isfinite(total)) return 0.5;
-    return std::clamp(dl / total, 0.0, 1.0);
-}
-
 // ------------------------------------------------------------------------------------------
 // builder
 // ------------------------------------------------------------------------------------------
-constexpr int ST_ABSENT = -2, ST_REFINED = -1, ST_EMPTY = -3;
-
 struct LevelGrid {
     int nb[3] = {0, 0, 0};
     double h = 0;
@@ -483,60 +448,84 @@ struct LevelGrid {
     bool inside(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < nb[0] && y < nb[1] && z < nb[2]; }
 };
 
-struct CutData {
-    double f = 1;            // fluid volume fraction
-    double c[3] = {0, 0, 0}; // fluid centroid
-    double ap[6] = {1, 1, 1, 1, 1, 1}; // face apertures -x,+x,-y,+y,-z,+z
-    double fc[6][2] = {};    // mean of the open face samples' two tangential coordinates (axis+1, axis+2)
-    double wall[3] = {0, 0, 0};        // mean of fluid samples next to solid samples
-    int wall_n = 0;
-};
-
-// Cell state during the build: 0 solid, 1 full fluid (far from the surface), 2 sampled.
+// A leaf brick of the build (its cells are leaf * 512 + q in the flat per-cell arrays)
 struct Leaf {
     int level;
     int coord[3];
-    std::vector<uint8_t> state;   // 512
-    std::vector<int> cut;         // 512, index into cut list or -1
 };
-
-// A face neighbour: its leaf and local cell, the open fraction of the shared face, its level
-// and the centroid of the face's open part (the open-sample mean of the cut cell that sampled
-// the face, else the face centre).
-struct Neighbor { int leaf; int local; double ap; int level; double xf[3]; };
 
 class Builder {
 public:
-    Builder(const pcfd_geometry_desc& g, const pcfd_grid_desc& d) : gd_(g), dd_(d) {}
+    Builder(const pcfd_geometry_desc& g, const pcfd_grid_desc& d, BuildAccelerator* accel)
+        : gd_(g), dd_(d), accel_(accel) {}
     Topology run();
 
 private:
     const pcfd_geometry_desc gd_;
     const pcfd_grid_desc dd_;
+    BuildAccelerator* accel_;
+    std::chrono::steady_clock::time_point t_start_, t_last_;
+    // wall time of the stage that ends now (milliseconds since the previous stage ended)
+    void stage_done(int stage) {
+        const auto now = std::chrono::steady_clock::now();
+        T_.build_ms[stage] += std::chrono::duration<double, std::milli>(now - t_last_).count();
+        t_last_ = now;
+    }
     std::unique_ptr<Geometry> geo_;
     Topology T_;
     int L_ = 1;
     std::vector<LevelGrid> grid_;
     std::vector<Leaf> leaves_;
     std::vector<CutData> cuts_;
-    std::vector<std::vector<uint8_t>> active_;  // per leaf, 512
+    // per cell (leaf * 512 + q): 0 solid, 1 full fluid (far from the surface), 2 sampled; the
+    // index into cuts_ (or -1); active fluid
+    std::vector<uint8_t> state_;
+    std::vector<int> cut_;
+    std::vector<uint8_t> active_;
+    std::vector<int> leaf_level_, leaf_coord_;  // leaves_ as flat tables (graph view)
     int64_t total_cells_[3];
 
     void setup_geometry();
     void setup_domain();
     void refine();
     void classify();
+    // the cell graph over the current leaf order (host pointers; refreshed by sync_leaves)
+    BuildView view_;
+    void sync_leaves();
     // neighbours of (leaf, local) in direction dir (0..5): same level (1 entry), coarser (1), finer (4)
-    int neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boundary) const;
-    double face_ap(int leaf, int local, int dir) const {
-        const int ci = leaves_[leaf].cut[local];
-        return ci >= 0 ? cuts_[ci].ap[dir] : 1.0;
+    int neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boundary) const {
+        const int n = graph_neighbors(view_, leaf, local, dir, out, boundary);
+        if (n < 0) throw BuildError(graph_error_text(-n));
+        return n;
     }
-    bool is_active(int leaf, int local) const { return active_[leaf][local] != 0; }
+    bool is_active(int leaf, int local) const { return active_[static_cast<int64_t>(leaf) * kBC + local] != 0; }
     void components();
+    void host_cell_passes(int nb, CellPasses& P) const;
     void finalize();
     void build_multigrid();
 };
+
+void Builder::sync_leaves() {
+    const size_t nl = leaves_.size();
+    leaf_level_.resize(nl);
+    leaf_coord_.resize(3 * nl);
+    for (size_t lb = 0; lb < nl; ++lb) {
+        leaf_level_[lb] = leaves_[lb].level;
+        for (int a = 0; a < 3; ++a) leaf_coord_[3 * lb + a] = leaves_[lb].coord[a];
+    }
+    view_.levels = L_;
+    for (int a = 0; a < 3; ++a) view_.lo[a] = T_.lo[a];
+    for (int l = 0; l < L_; ++l) {
+        view_.h[l] = grid_[l].h;
+        for (int a = 0; a < 3; ++a) view_.nb[l][a] = grid_[l].nb[a];
+        view_.status[l] = grid_[l].status.data();
+    }
+    view_.leaf_level = leaf_level_.data();
+    view_.leaf_coord = leaf_coord_.data();
+    view_.cut = cut_.data();
+    view_.cuts = cuts_.data();
+    view_.active = active_.data();
+}
 
 void Builder::setup_geometry() {
     switch (gd_.kind) {
@@ -659,218 +648,151 @@ void Builder::classify() {
         geo_->surface(spacing, pts);
         pg.build(pts, 2 * T_.finest_h);
     }
-    std::vector<std::vector<CutData>> local_cuts(leaves_.size());
+    // cells within `spacing` of a surface point are sub-sampled; the others are fluid or solid
+    // by their centre (whole bricks at once when the brick is not near the surface)
+    state_.assign(leaves_.size() * kBC, 1);
+    cut_.assign(leaves_.size() * kBC, -1);
     parallel_for(static_cast<int64_t>(leaves_.size()), [&](int64_t b0, int64_t b1) {
         for (int64_t lb = b0; lb < b1; ++lb) {
-            Leaf& leaf = leaves_[lb];
-            leaf.state.assign(kBC, 1);
-            leaf.cut.assign(kBC, -1);
+            const Leaf& leaf = leaves_[lb];
+            uint8_t* state = &state_[lb * kBC];
             const double h = grid_[leaf.level].h, bh = kBS * h;
             double blo[3], bhi[3];
             for (int a = 0; a < 3; ++a) { blo[a] = T_.lo[a] + leaf.coord[a] * bh; bhi[a] = blo[a] + bh; }
-            const bool near = pg.near_box(blo, bhi, spacing);
-            if (!near) {
+            if (!pg.near_box(blo, bhi, spacing)) {
                 const bool s = geo_->solid(0.5 * (blo[0] + bhi[0]), 0.5 * (blo[1] + bhi[1]), 0.5 * (blo[2] + bhi[2]));
-                if (s) std::fill(leaf.state.begin(), leaf.state.end(), 0);
+                if (s) std::fill(state, state + kBC, 0);
                 continue;
             }
             for (int q = 0; q < kBC; ++q) {
-                const int i = q & 7, j = (q >> 3) & 7, k = q >> 6;
-                const int I[3] = {leaf.coord[0] * kBS + i, leaf.coord[1] * kBS + j, leaf.coord[2] * kBS + k};
+                const int I[3] = {leaf.coord[0] * kBS + (q & 7), leaf.coord[1] * kBS + ((q >> 3) & 7),
+                                  leaf.coord[2] * kBS + (q >> 6)};
                 double clo[3], chi[3];
                 for (int a = 0; a < 3; ++a) { clo[a] = T_.lo[a] + I[a] * h; chi[a] = clo[a] + h; }
-                if (!pg.near_box(clo, chi, spacing)) {
-                    if (geo_->solid(0.5 * (clo[0] + chi[0]), 0.5 * (clo[1] + chi[1]), 0.5 * (clo[2] + chi[2]))) leaf.state[q] = 0;
-                    continue;
-                }
-                CutData cd;
-                // volume samples
-                std::vector<uint8_t> sol(ns * ns * ns);
-                int fluid = 0;
-                double cx = 0, cy = 0, cz = 0;
-                for (int c = 0; c < ns; ++c)
-                    for (int b = 0; b < ns; ++b)
-                        for (int a = 0; a < ns; ++a) {
-                            const double x = T_.lo[0] + (I[0] + (a + 0.5) / ns) * h;
-                            const double y = T_.lo[1] + (I[1] + (b + 0.5) / ns) * h;
-                            const double z = T_.lo[2] + (I[2] + (c + 0.5) / ns) * h;
-                            const bool s = geo_->solid(x, y, z);
-                            sol[(c * ns + b) * ns + a] = s;
-                            if (!s) { ++fluid; cx += x; cy += y; cz += z; }
-                        }
-                cd.f = static_cast<double>(fluid) / (ns * ns * ns);
-                if (fluid > 0) { cd.c[0] = cx / fluid; cd.c[1] = cy / fluid; cd.c[2] = cz / fluid; }
-                // fluid samples adjacent to a solid sample approximate the wall
-                for (int c = 0; c < ns; ++c)
-                    for (int b = 0; b < ns; ++b)
-                        for (int a = 0; a < ns; ++a) {
-                            if (sol[(c * ns + b) * ns + a]) continue;
-                            bool touches = false;
-                            const int nb6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-                            for (auto& o : nb6) {
-                                const int aa = a + o[0], bb = b + o[1], cc = c + o[2];
-                                if (aa < 0 || bb < 0 || cc < 0 || aa >= ns || bb >= ns || cc >= ns) continue;
-                                if (sol[(cc * ns + bb) * ns + aa]) touches = true;
-                            }
-                            if (touches) {
-                                cd.wall[0] += T_.lo[0] + (I[0] + (a + 0.5) / ns) * h;
-                                cd.wall[1] += T_.lo[1] + (I[1] + (b + 0.5) / ns) * h;
-                                cd.wall[2] += T_.lo[2] + (I[2] + (c + 0.5) / ns) * h;
-                                ++cd.wall_n;
-                            }
-                        }
-                if (cd.wall_n) for (int a = 0; a < 3; ++a) cd.wall[a] /= cd.wall_n;
-                // face apertures from the face's own integer position (identical on both sides)
-                for (int dir = 0; dir < 6; ++dir) {
-                    const int axis = dir >> 1, side = dir & 1;
-                    const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
-                    int open = 0;
-                    double s1 = 0, s2 = 0;
-                    for (int b = 0; b < ns; ++b)
-                        for (int a = 0; a < ns; ++a) {
-                            double p[3];
-                            p[axis] = T_.lo[axis] + (I[axis] + side) * h;
-                            p[t1] = T_.lo[t1] + (I[t1] + (a + 0.5) / ns) * h;
-                            p[t2] = T_.lo[t2] + (I[t2] + (b + 0.5) / ns) * h;
-                            if (!geo_->solid(p[0], p[1], p[2])) { ++open; s1 += p[t1]; s2 += p[t2]; }
-                        }
-                    cd.ap[dir] = static_cast<double>(open) / (ns * ns);
-                    if (open > 0) { cd.fc[dir][0] = s1 / open; cd.fc[dir][1] = s2 / open; }
-                }
-                leaf.state[q] = 2;
-                leaf.cut[q] = static_cast<int>(local_cuts[lb].size());
-                local_cuts[lb].push_back(cd);
+                if (pg.near_box(clo, chi, spacing)) state[q] = 2;
+                else if (geo_->solid(0.5 * (clo[0] + chi[0]), 0.5 * (clo[1] + chi[1]), 0.5 * (clo[2] + chi[2])))
+                    state[q] = 0;
             }
         }
     });
-    // gather cut data into one list
-    for (size_t lb = 0; lb < leaves_.size(); ++lb) {
-        const int base = static_cast<int>(cuts_.size());
-        for (auto& c : leaves_[lb].cut) if (c >= 0) c += base;
-        cuts_.insert(cuts_.end(), local_cuts[lb].begin(), local_cuts[lb].end());
+    // the cut list in leaf, then cell order: integer position in the level's cell grid and level
+    std::vector<int> cells;
+    for (size_t lb = 0; lb < leaves_.size(); ++lb)
+        for (int q = 0; q < kBC; ++q)
+            if (state_[lb * kBC + q] == 2) {
+                const Leaf& leaf = leaves_[lb];
+                cut_[lb * kBC + q] = static_cast<int>(cells.size() / 4);
+                cells.insert(cells.end(), {leaf.coord[0] * kBS + (q & 7), leaf.coord[1] * kBS + ((q >> 3) & 7),
+                                           leaf.coord[2] * kBS + (q >> 6), leaf.level});
+            }
+    stage_done(BUILD_CANDIDATES);
+    const int64_t ncut = static_cast<int64_t>(cells.size() / 4);
+    cuts_.assign(ncut, CutData{});
+    if (accel_) {
+        accel_->sample(geo_->desc, T_.lo, T_.level_h, ns, cells, cuts_);
+    } else {
+        parallel_for(ncut, [&](int64_t c0, int64_t c1) {
+            std::vector<uint8_t> sol(cell_samples(ns));
+            for (int64_t i = c0; i < c1; ++i) {
+                const int* I = &cells[4 * i];
+                const double h = T_.level_h[I[3]];
+                for (int s = 0; s < cell_samples(ns); ++s) sol[s] = sample_solid(geo_->desc, T_.lo, I, h, ns, s);
+                for (int part = 0; part < 8; ++part) cut_part(sol.data(), T_.lo, I, h, ns, part, cuts_[i]);
+            }
+        });
     }
+    stage_done(BUILD_SAMPLING);
     // small cut cells become solid (min_volume_fraction)
     const double fmin = dd_.min_volume_fraction;
-    active_.assign(leaves_.size(), std::vector<uint8_t>(kBC, 0));
-    for (size_t lb = 0; lb < leaves_.size(); ++lb)
-        for (int q = 0; q < kBC; ++q) {
-            const Leaf& leaf = leaves_[lb];
-            bool act = leaf.state[q] == 1 || (leaf.state[q] == 2 && cuts_[leaf.cut[q]].f >= fmin && cuts_[leaf.cut[q]].f > 0);
-            active_[lb][q] = act;
-        }
-}
-
-int Builder::neighbors(int leaf, int local, int dir, Neighbor out[4], bool& boundary) const {
-    boundary = false;
-    const Leaf& lf = leaves_[leaf];
-    const int l = lf.level;
-    const int axis = dir >> 1, sign = (dir & 1) ? 1 : -1;
-    const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
-    int I[3] = {lf.coord[0] * kBS + (local & 7), lf.coord[1] * kBS + ((local >> 3) & 7), lf.coord[2] * kBS + (local >> 6)};
-    int J[3] = {I[0], I[1], I[2]};
-    J[axis] += sign;
-    const LevelGrid& G = grid_[l];
-    if (J[axis] < 0 || J[axis] >= G.nb[axis] * kBS) { boundary = true; return 0; }
-    const int st = G.status[G.idx(J[0] >> 3, J[1] >> 3, J[2] >> 3)];
-    auto loc = [](const int c[3]) { return (c[0] & 7) + 8 * (c[1] & 7) + 64 * (c[2] & 7); };
-    // centroid of the open part of a face on this cell's face plane: the mean of the open
-    // samples of cut `cut` on its face `cdir` when the face is partly open, else the centre of
-    // the face of the cell at integer position C with size hc
-    const double plane = T_.lo[axis] + (I[axis] + (sign > 0 ? 1 : 0)) * G.h;
-    auto face_centroid = [&](double* xf, int cut, int cdir, const int* C, double hc) {
-        xf[axis] = plane;
-        if (cut >= 0 && cuts_[cut].ap[cdir] > 0.0 && cuts_[cut].ap[cdir] < 1.0) {
-            xf[t1] = cuts_[cut].fc[cdir][0];
-            xf[t2] = cuts_[cut].fc[cdir][1];
-        } else {
-            xf[t1] = T_.lo[t1] + (C[t1] + 0.5) * hc;
-            xf[t2] = T_.lo[t2] + (C[t2] + 0.5) * hc;
-        }
-    };
-    if (st >= 0) {
-        out[0] = {st, loc(J), 0.0, l, {0, 0, 0}};
-        // same level: both sides sampled the same face points
-        const int ci = lf.cut[local], cj = leaves_[st].cut[loc(J)];
-        out[0].ap = ci >= 0 ? cuts_[ci].ap[dir] : (cj >= 0 ? cuts_[cj].ap[dir ^ 1] : 1.0);
-        if (ci >= 0) face_centroid(out[0].xf, ci, dir, I, G.h);
-        else face_centroid(out[0].xf, cj, dir ^ 1, I, G.h);
-        return 1;
-    }
-    if (st == ST_EMPTY) return 0;
-    if (st == ST_REFINED) {
-        // four finer cells across the face; each sampled its own (fine) face
-        const LevelGrid& F = grid_[l + 1];
-        int n = 0;
-        for (int b = 0; b < 2; ++b)
-            for (int a = 0; a < 2; ++a) {
-                int K[3];
-                K[axis] = sign > 0 ? 2 * J[axis] : 2 * J[axis] + 1;
-                K[t1] = 2 * J[t1] + a;
-                K[t2] = 2 * J[t2] + b;
-                const int fs = F.status[F.idx(K[0] >> 3, K[1] >> 3, K[2] >> 3)];
-                if (fs < 0) continue;  // empty brick
-                const int fq = loc(K);
-                const int cf = leaves_[fs].cut[fq];
-                out[n] = {fs, fq, cf >= 0 ? cuts_[cf].ap[dir ^ 1] : 1.0, l + 1, {0, 0, 0}};
-                face_centroid(out[n].xf, cf, dir ^ 1, K, F.h);
-                ++n;
-            }
-        return n;
-    }
-    // absent at level l: the coarser leaf at level l-1 covers it; this (fine) cell sampled the face
-    if (l == 0) throw BuildError("internal: absent level-0 brick");
-    const LevelGrid& C = grid_[l - 1];
-    int K[3] = {J[0] >> 1, J[1] >> 1, J[2] >> 1};
-    const int cs = C.status[C.idx(K[0] >> 3, K[1] >> 3, K[2] >> 3)];
-    if (cs == ST_EMPTY) return 0;
-    if (cs < 0) throw BuildError("internal: AMR nesting violated");
-    const int ci = lf.cut[local];
-    out[0] = {cs, loc(K), ci >= 0 ? cuts_[ci].ap[dir] : 1.0, l - 1, {0, 0, 0}};
-    face_centroid(out[0].xf, ci, dir, I, G.h);
-    return 1;
+    active_.assign(leaves_.size() * kBC, 0);
+    for (size_t c = 0; c < active_.size(); ++c)
+        active_[c] = state_[c] == 1 || (state_[c] == 2 && cuts_[cut_[c]].f >= fmin && cuts_[cut_[c]].f > 0);
+    sync_leaves();
 }
 
 void Builder::components() {
     // keep only fluid connected to the outlet (or the largest component without an outlet)
-    const int nl = static_cast<int>(leaves_.size());
-    std::vector<std::vector<int>> comp(nl, std::vector<int>(kBC, -1));
+    if (accel_) {
+        accel_->connected_fluid(view_, static_cast<int64_t>(leaves_.size()), T_.outlet, active_);
+        return;
+    }
+    const int64_t ncell = static_cast<int64_t>(leaves_.size()) * kBC;
+    std::vector<int> comp(ncell, -1);
     std::vector<int64_t> sizes;
     std::vector<uint8_t> touches_outlet;
-    std::vector<std::pair<int, int>> stack;
-    for (int lb = 0; lb < nl; ++lb)
-        for (int q = 0; q < kBC; ++q) {
-            if (!active_[lb][q] || comp[lb][q] >= 0) continue;
-            const int id = static_cast<int>(sizes.size());
-            sizes.push_back(0);
-            touches_outlet.push_back(0);
-            stack.clear();
-            stack.push_back({lb, q});
-            comp[lb][q] = id;
-            while (!stack.empty()) {
-                auto [cl, cq] = stack.back();
-                stack.pop_back();
-                ++sizes[id];
-                for (int dir = 0; dir < 6; ++dir) {
-                    Neighbor nb[4]; bool bnd;
-                    const int n = neighbors(cl, cq, dir, nb, bnd);
-                    if (bnd && dir == 1) touches_outlet[id] = 1;
-                    for (int k = 0; k < n; ++k) {
-                        if (nb[k].ap <= 0 || !active_[nb[k].leaf][nb[k].local] || comp[nb[k].leaf][nb[k].local] >= 0) continue;
-                        comp[nb[k].leaf][nb[k].local] = id;
-                        stack.push_back({nb[k].leaf, nb[k].local});
-                    }
+    std::vector<int64_t> stack;
+    for (int64_t c = 0; c < ncell; ++c) {
+        if (!active_[c] || comp[c] >= 0) continue;
+        const int id = static_cast<int>(sizes.size());
+        sizes.push_back(0);
+        touches_outlet.push_back(0);
+        stack.clear();
+        stack.push_back(c);
+        comp[c] = id;
+        while (!stack.empty()) {
+            const int64_t cc = stack.back();
+            stack.pop_back();
+            ++sizes[id];
+            for (int dir = 0; dir < 6; ++dir) {
+                Neighbor nb[4]; bool bnd;
+                const int n = neighbors(static_cast<int>(cc / kBC), static_cast<int>(cc % kBC), dir, nb, bnd);
+                if (bnd && dir == 1) touches_outlet[id] = 1;
+                for (int k = 0; k < n; ++k) {
+                    const int64_t o = static_cast<int64_t>(nb[k].leaf) * kBC + nb[k].local;
+                    if (nb[k].ap <= 0 || !active_[o] || comp[o] >= 0) continue;
+                    comp[o] = id;
+                    stack.push_back(o);
                 }
             }
         }
+    }
     std::vector<uint8_t> keep(sizes.size(), 0);
     if (T_.outlet) {
         for (size_t c = 0; c < sizes.size(); ++c) keep[c] = touches_outlet[c];
     } else if (!sizes.empty()) {
         keep[std::max_element(sizes.begin(), sizes.end()) - sizes.begin()] = 1;
     }
-    for (int lb = 0; lb < nl; ++lb)
-        for (int q = 0; q < kBC; ++q)
-            if (active_[lb][q] && !keep[comp[lb][q]]) active_[lb][q] = 0;
+    for (int64_t c = 0; c < ncell; ++c)
+        if (active_[c] && !keep[comp[c]]) active_[c] = 0;
+}
+
+// finalize's per-cell passes on the host, in dof order
+void Builder::host_cell_passes(int nb, CellPasses& P) const {
+    const int64_t ndof = static_cast<int64_t>(nb) * kBC;
+    P.volume.assign(ndof, 0.0);
+    P.centroid.assign(3 * ndof, 0.0);
+    P.embedded.assign(ndof, 0);
+    for (int64_t a = 0; a < ndof; ++a) {
+        CellGeometry g;
+        const int err = cell_geometry(view_, static_cast<int>(a / kBC), static_cast<int>(a % kBC), g);
+        if (err != GE_NONE) throw BuildError(graph_error_text(err));
+        P.volume[a] = g.volume;
+        for (int r = 0; r < 3; ++r) P.centroid[3 * a + r] = g.centroid[r];
+        P.embedded[a] = g.embedded;
+    }
+    P.mask.assign(ndof, 0);
+    P.has_compact.assign(ndof, 0);
+    P.diffusion_sum.assign(ndof, 0.0);
+    P.S.assign(3 * ndof, 0.0);
+    P.wall_dof.clear();
+    P.wall_moment.clear();
+    P.edges.clear();
+    for (int64_t a = 0; a < ndof; ++a) {
+        if (!active_[a]) continue;
+        CellFaces f;
+        const int err = cell_faces(view_, P.centroid.data(), P.embedded.data(), static_cast<int>(a / kBC),
+                                   static_cast<int>(a % kBC), f, [&](const EdgeRecord& e) { P.edges.push_back(e); });
+        if (err != GE_NONE) throw BuildError(graph_error_text(err));
+        P.mask[a] = f.mask;
+        P.has_compact[a] = f.has_compact;
+        P.diffusion_sum[a] = f.diffusion_sum;
+        for (int r = 0; r < 3; ++r) P.S[3 * a + r] = f.S[r];
+        const double h = grid_[leaves_[a / kBC].level].h;
+        if (closure_norm(f.S) > 1e-12 * h * h) {
+            P.wall_dof.push_back(a);
+            P.wall_moment.insert(P.wall_moment.end(), f.fm, f.fm + 9);
+        }
+    }
 }
 
 void Builder::finalize() {
@@ -879,7 +801,7 @@ void Builder::finalize() {
     std::vector<int> order;
     for (int lb = 0; lb < nl; ++lb) {
         bool any = false;
-        for (int q = 0; q < kBC && !any; ++q) any = active_[lb][q];
+        for (int q = 0; q < kBC && !any; ++q) any = active_[static_cast<int64_t>(lb) * kBC + q];
         const Leaf& lf = leaves_[lb];
         if (!any) grid_[lf.level].status[grid_[lf.level].idx(lf.coord[0], lf.coord[1], lf.coord[2])] = ST_EMPTY;
         else order.push_back(lb);
@@ -927,129 +849,46 @@ void Builder::finalize() {
         for (auto& s : grid_[l].status) if (s >= 0) s = newid[s];
     {
         std::vector<Leaf> nl2(nb);
-        std::vector<std::vector<uint8_t>> na(nb);
-        for (int b = 0; b < nb; ++b) { nl2[b] = std::move(leaves_[order[b]]); na[b] = std::move(active_[order[b]]); }
+        std::vector<uint8_t> na(static_cast<size_t>(nb) * kBC);
+        std::vector<int> nc(static_cast<size_t>(nb) * kBC);
+        for (int b = 0; b < nb; ++b) {
+            nl2[b] = leaves_[order[b]];
+            const int64_t src = static_cast<int64_t>(order[b]) * kBC, dst = static_cast<int64_t>(b) * kBC;
+            std::copy(active_.begin() + src, active_.begin() + src + kBC, na.begin() + dst);
+            std::copy(cut_.begin() + src, cut_.begin() + src + kBC, nc.begin() + dst);
+        }
         leaves_ = std::move(nl2);
         active_ = std::move(na);
+        cut_ = std::move(nc);
+        state_.clear();
+        state_.shrink_to_fit();
     }
+    sync_leaves();
 
-    // volumes and centroids of every cell, and the embedded nodes: cells with a partial volume
-    // or a partially open face to an active neighbour (the endpoints of the reference's
-    // embedded connections, amr_advection.cu:799).  Every same-level face of an embedded node
-    // is a compact edge, as the reference's EB momentum regular connections are
-    // (amr_advection.cu:800-812), so that its transport, reconstruction and traction carry
-    // the centroid corrections.
-    T_.mask.assign(T_.ndof, 0);
-    T_.centroid.assign(3 * T_.ndof, 0.0);
-    std::vector<double> volume(T_.ndof, 0.0);
-    std::vector<uint8_t> embedded(T_.ndof, 0);
-    for (int b = 0; b < nb; ++b) {
-        const Leaf& lf = leaves_[b];
-        const double h = grid_[lf.level].h;
-        for (int q = 0; q < kBC; ++q) {
-            const int64_t a = static_cast<int64_t>(b) * kBC + q;
-            const int I[3] = {lf.coord[0] * kBS + (q & 7), lf.coord[1] * kBS + ((q >> 3) & 7), lf.coord[2] * kBS + (q >> 6)};
-            const int ci = lf.cut[q];
-            const bool act = active_[b][q] != 0;
-            if (act) volume[a] = (ci >= 0 ? cuts_[ci].f : 1.0) * h * h * h;
-            for (int ax = 0; ax < 3; ++ax)
-                T_.centroid[3 * a + ax] = act && ci >= 0 ? cuts_[ci].c[ax] : T_.lo[ax] + (I[ax] + 0.5) * h;
-            if (!act) continue;
-            if (volume[a] != h * h * h) embedded[a] = 1;
-            for (int dir = 0; dir < 6; ++dir) {
-                Neighbor nbs[4]; bool bnd;
-                const int n = neighbors(b, q, dir, nbs, bnd);
-                for (int k = 0; k < n; ++k)
-                    if (active_[nbs[k].leaf][nbs[k].local] && nbs[k].ap > 0.0 && nbs[k].ap < 1.0) embedded[a] = 1;
-            }
-        }
-    }
-    // masks, compact edges, closure area vectors, and the first moment of each cell's open
-    // faces about its centroid (the wall's moment follows from it, below)
-    std::vector<double> S(3 * T_.ndof, 0.0);
-    std::vector<uint8_t> has_compact(T_.ndof, 0);
-    std::vector<double> diffusion_sum(T_.ndof, 0.0);
-    std::vector<int64_t> wall_dof;
-    std::vector<std::array<double, 9>> wall_moment;  // per wall cell in dof order
-    for (int b = 0; b < nb; ++b) {
-        const Leaf& lf = leaves_[b];
-        const double h = grid_[lf.level].h;
-        for (int q = 0; q < kBC; ++q) {
-            const int64_t a = static_cast<int64_t>(b) * kBC + q;
-            const int I[3] = {lf.coord[0] * kBS + (q & 7), lf.coord[1] * kBS + ((q >> 3) & 7), lf.coord[2] * kBS + (q >> 6)};
-            if (!active_[b][q]) continue;
-            T_.mask[a] |= CM_ACTIVE;
-            const int ci = lf.cut[q];
-            const double* ca = &T_.centroid[3 * a];
-            std::array<double, 9> fm{};  // sum over open faces of (x_f - c_a) (x) (A n)_f
-            for (int dir = 0; dir < 6; ++dir) {
-                const int axis = dir >> 1, up = dir & 1;
-                const double nsign = up ? 1.0 : -1.0;
-                Neighbor nbs[4]; bool bnd;
-                const int n = neighbors(b, q, dir, nbs, bnd);
-                double open = 0;
-                if (bnd) {
-                    if (ci >= 0 && cuts_[ci].ap[dir] < 1.0)
-                        throw BuildError("a cut cell touches the domain boundary; enlarge the domain");
-                    open = h * h;
-                    for (int r = 0; r < 3; ++r) {
-                        const double xf = r == axis ? T_.lo[r] + (I[r] + up) * h : T_.lo[r] + (I[r] + 0.5) * h;
-                        fm[3 * r + axis] += (xf - ca[r]) * nsign * open;
-                    }
-                }
-                for (int k = 0; k < n; ++k) {
-                    const Neighbor& nb2 = nbs[k];
-                    if (!active_[nb2.leaf][nb2.local] || nb2.ap <= 0) continue;
-                    const double hf = std::min(h, grid_[nb2.level].h);
-                    const double area = nb2.ap * hf * hf;
-                    open += area;
-                    for (int r = 0; r < 3; ++r) fm[3 * r + axis] += (nb2.xf[r] - ca[r]) * nsign * area;
-                    const int64_t other = static_cast<int64_t>(nb2.leaf) * kBC + nb2.local;
-                    const double hn = grid_[nb2.level].h;
-                    const double coef = area / (0.5 * (h + hn));
-                    const bool same = nb2.level == lf.level;
-                    if (same && nb2.ap >= 1.0 && !embedded[a] && !embedded[other]) {
-                        T_.mask[a] |= up ? (CM_PX << axis) : (CM_MX << axis);
-                        diffusion_sum[a] += coef;
-                        continue;
-                    }
-                    const int64_t lower = up ? a : other, upper = up ? other : a;
-                    const double* cl = &T_.centroid[3 * lower];
-                    const double* cu = &T_.centroid[3 * upper];
-                    // diffusion conductance: open area |dx_axis| / |dx|^2 on the centroids for a
-                    // fully open same-level face (the reference's perimeter conductance,
-                    // amr_advection.cu:1180), the operator's coefficient on coarse/fine and
-                    // partially open faces (connection_conductance, :1183)
-                    double dcoef = coef;
-                    if (same && nb2.ap >= 1.0) {
-                        double d2 = 0;
-                        for (int r = 0; r < 3; ++r) d2 += (cu[r] - cl[r]) * (cu[r] - cl[r]);
-                        const double dn = std::abs(cu[axis] - cl[axis]);
-                        if (!(d2 > 0) || !(dn > 0)) throw BuildError("compact face between coincident centroids");
-                        dcoef = area * dn / d2;
-                    }
-                    diffusion_sum[a] += dcoef;
-                    // compact edge, created once: same level from the lower cell, c/f from the fine cell
-                    const bool create = same ? up : nb2.level < lf.level;
-                    has_compact[a] = 1;
-                    if (!create) continue;
-                    T_.edge_a.push_back(static_cast<int>(lower));
-                    T_.edge_b.push_back(static_cast<int>(upper));
-                    T_.edge_axis.push_back(static_cast<int8_t>(axis));
-                    T_.edge_area.push_back(area);
-                    T_.edge_coef.push_back(coef);
-                    T_.edge_dcoef.push_back(dcoef);
-                    T_.edge_w.push_back(face_weight(cl, cu, nb2.xf, axis));
-                    for (int r = 0; r < 3; ++r) T_.edge_x.push_back(nb2.xf[r]);
-                }
-                S[3 * a + axis] -= nsign * open;
-            }
-            const double smag = std::sqrt(S[3 * a] * S[3 * a] + S[3 * a + 1] * S[3 * a + 1] + S[3 * a + 2] * S[3 * a + 2]);
-            if (smag > 1e-12 * h * h) {
-                wall_dof.push_back(a);
-                wall_moment.push_back(fm);
-            }
-        }
+    // volumes and centroids of every cell, the embedded nodes (every same-level face of an
+    // embedded node is a compact edge, as the reference's EB momentum regular connections are,
+    // amr_advection.cu:800-812, so that its transport, reconstruction and traction carry the
+    // centroid corrections); then the masks, compact edges, closure area vectors and the first
+    // moment of each cell's open faces about its centroid (the wall's moment follows from it)
+    CellPasses P;
+    if (accel_) accel_->cell_passes(view_, nb, P);
+    else host_cell_passes(nb, P);
+    T_.mask = std::move(P.mask);
+    T_.centroid = std::move(P.centroid);
+    const std::vector<double>& volume = P.volume;
+    const std::vector<double>& S = P.S;
+    const std::vector<uint8_t>& has_compact = P.has_compact;
+    const std::vector<double>& diffusion_sum = P.diffusion_sum;
+    const std::vector<int64_t>& wall_dof = P.wall_dof;
+    const size_t ne_created = P.edges.size();
+    T_.edge_a.resize(ne_created); T_.edge_b.resize(ne_created); T_.edge_axis.resize(ne_created);
+    T_.edge_area.resize(ne_created); T_.edge_coef.resize(ne_created); T_.edge_dcoef.resize(ne_created);
+    T_.edge_w.resize(ne_created); T_.edge_x.resize(3 * ne_created);
+    for (size_t e = 0; e < ne_created; ++e) {
+        const EdgeRecord& r = P.edges[e];
+        T_.edge_a[e] = r.a; T_.edge_b[e] = r.b; T_.edge_axis[e] = static_cast<int8_t>(r.axis);
+        T_.edge_area[e] = r.area; T_.edge_coef[e] = r.coef; T_.edge_dcoef[e] = r.dcoef; T_.edge_w[e] = r.w;
+        for (int k = 0; k < 3; ++k) T_.edge_x[3 * e + k] = r.x[k];
     }
     // boundary-face diffusion does not exist; the outlet adds nothing either (zero flux)
     // special cells and patches
@@ -1083,10 +922,10 @@ void Builder::finalize() {
                 // D = V I - sum over the open faces of (x_f - c) (x) (A n)_f, as the area vector
                 // S = -sum (A n)_f closes the faces' zeroth moment
                 if (wall_next >= wall_dof.size() || wall_dof[wall_next] != a) throw BuildError("internal: wall moment order");
-                const std::array<double, 9>& fm = wall_moment[wall_next++];
+                const double* fm = &P.wall_moment[9 * wall_next++];
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) T_.patch_D.push_back((r == c ? volume[a] : 0.0) - fm[3 * r + c]);
-                const int ci = leaves_[b].cut[q];
+                const int ci = cut_[a];
                 if (ci >= 0 && cuts_[ci].wall_n > 0) {
                     for (int ax = 0; ax < 3; ++ax) T_.patch_x.push_back(cuts_[ci].wall[ax]);
                 } else {
@@ -1412,13 +1251,24 @@ void Builder::build_multigrid() {
 }
 
 Topology Builder::run() {
+    t_start_ = t_last_ = std::chrono::steady_clock::now();
     setup_geometry();
     setup_domain();
+    stage_done(BUILD_GEOMETRY);
     refine();
+    stage_done(BUILD_REFINE);
     classify();
     components();
+    stage_done(BUILD_COMPONENTS);
     finalize();
+    stage_done(BUILD_FINALIZE);
     build_multigrid();
+    stage_done(BUILD_MULTIGRID);
+    T_.build_ms[BUILD_TOTAL] = std::chrono::duration<double, std::milli>(t_last_ - t_start_).count();
+    if (accel_) {
+        T_.sampled_on_device = accel_->device();
+        T_.build_ms[BUILD_DEVICE_KERNEL] = accel_->kernel_ms();
+    }
     return std::move(T_);
 }
 
@@ -1449,8 +1299,8 @@ static void set_single_instance(Topology& T) {
     }
 }
 
-Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid) {
-    Builder b(geometry, grid);
+Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid, BuildAccelerator* accel) {
+    Builder b(geometry, grid, accel);
     Topology T = b.run();
     set_single_instance(T);
     return T;

@@ -1,6 +1,7 @@
 // Static preprocessing for one wing/sphere case: 2:1 nested AMR bricks of 8^3 cubic cells,
 // sub-sampled cut cells, compact (coarse/fine and partially open) face records, surface
-// patches and the geometric multigrid hierarchy.  Host only.
+// patches and the geometric multigrid hierarchy.  Host code; the cut-cell sub-sampling can be
+// handed to a CellSampler (the device sampler runs it on a GPU, bit-identical to the host's).
 //
 // Plays the role of AmrHierarchy::build_static (src/core/fluid/amr_grid.cpp:103-142),
 // build_amr_embedded_boundary_atlas (amr_eb.cpp:82-375) and
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "../../include/paracfd.h"
+#include "build_graph.hpp"
 
 namespace pcfd {
 
@@ -43,7 +45,24 @@ struct MGLevel {
     std::vector<double> coarse_inv;     // dense inverse on the coarsest level (row-major)
 };
 
+// Stages of a topology build, as timed into Topology::build_ms (wall milliseconds)
+enum BuildStage : int {
+    BUILD_GEOMETRY = 0,    // body and domain set-up
+    BUILD_REFINE = 1,      // refinement bands and 2:1 nesting
+    BUILD_CANDIDATES = 2,  // cells near the surface (the cut list) and full / solid cells
+    BUILD_SAMPLING = 3,    // cut-cell sub-sampling
+    BUILD_COMPONENTS = 4,  // small-cell removal and fluid connected to the outlet
+    BUILD_FINALIZE = 5,    // brick order, masks, compact edges, wall patches
+    BUILD_MULTIGRID = 6,   // coarse levels and the dense coarse inverse
+    BUILD_TOTAL = 7,
+    BUILD_DEVICE_KERNEL = 8  // the device kernels alone (parts of the stages above)
+};
+
 struct Topology {
+    // wall time of each build stage (BuildStage) in milliseconds; zero for a merged batch
+    double build_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int sampled_on_device = -1;  // device that ran the accelerated build stages, or -1 (host)
+
     // grid
     double lo[3] = {0, 0, 0};
     double h0 = 0;
@@ -139,8 +158,52 @@ Topology::NonOrth compute_nonorthogonal(const Topology& T);
 struct BuildError : std::invalid_argument {
     explicit BuildError(const std::string& m) : std::invalid_argument(m) {}
 };
+// A HIP runtime failure inside a build step that runs on a device (PCFD_EDEVICE).
+struct DeviceError : std::runtime_error {
+    explicit DeviceError(const std::string& m) : std::runtime_error(m) {}
+};
 
-Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid);
+// Outputs of the builder's per-cell passes over the brick-ordered cell graph (dof = brick * 512 + q)
+struct CellPasses {
+    std::vector<double> volume;         // per dof (0 for inactive cells)
+    std::vector<double> centroid;       // 3 per dof
+    std::vector<uint8_t> embedded;      // per dof
+    std::vector<uint8_t> mask;          // per dof: active and structured-face bits
+    std::vector<uint8_t> has_compact;   // per dof
+    std::vector<double> diffusion_sum;  // per dof
+    std::vector<double> S;              // 3 per dof: closure area vector
+    std::vector<int64_t> wall_dof;      // active cells with |S| > 1e-12 h^2, ascending
+    std::vector<double> wall_moment;    // 9 per wall cell: cell_faces' fm
+    std::vector<EdgeRecord> edges;      // compact edges in creation order (dof, then face order)
+};
+
+// Stages of the build that can run elsewhere than on the host threads (the device build does
+// them on a GPU).  Each must give exactly what the host builder gives:
+//   sample: the cut list's cells (4 ints each: integer position in its level's cell grid, then
+//     the level) -> one CutData per cell, as sample_solid / cut_part give it;
+//   connected_fluid: clear every active cell that is not in a component of the graph (faces with
+//     an open fraction > 0 between active cells) touching the x-max boundary (outlet), or,
+//     without an outlet, not in the largest component (ties: the one holding the lowest cell);
+//   cell_passes: cell_geometry on every cell, then cell_faces on every active cell, in dof order.
+class BuildAccelerator {
+public:
+    virtual ~BuildAccelerator() = default;
+    virtual void sample(const SolidDesc& solid, const double lo[3], const std::vector<double>& level_h, int ns,
+                        const std::vector<int>& cells, std::vector<CutData>& out) = 0;
+    virtual void connected_fluid(const BuildView& view, int64_t nleaves, bool outlet, std::vector<uint8_t>& active) = 0;
+    virtual void cell_passes(const BuildView& view, int nbricks, CellPasses& out) = 0;
+    virtual int device() const { return -1; }       // the device it runs on, or -1
+    virtual double kernel_ms() const { return 0.0; } // device kernel time so far
+};
+
+// Build a topology; the stages of `accel` run there when given, else on the host.
+Topology build_topology(const pcfd_geometry_desc& geometry, const pcfd_grid_desc& grid,
+                        BuildAccelerator* accel = nullptr);
+
+// Build a C ABI topology handle (the sampler as for build_topology), mapping a BuildError to
+// PCFD_EINVAL, a DeviceError to PCFD_EDEVICE and host allocation failure to PCFD_ENOMEM.
+pcfd_status build_topology_handle(const char* where, const pcfd_geometry_desc* geometry, const pcfd_grid_desc* grid,
+                                  BuildAccelerator* accel, pcfd_topology** out);
 
 // Concatenate independent topologies into one batch (same AMR levels and cell sizes; each
 // part needs its dense coarse inverse).  Every index is offset into the merged numbering.

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     lib = L.load()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.pcfd_abi_version() == 4
+    assert lib.pcfd_abi_version() == 5
 
 
 def test_create_without_gpu_fails_loudly():
