@@ -38,9 +38,10 @@ def build(verbose=True):
     obj = os.path.join(BUILD, "solver.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "solver.hip"), "-o", obj])
     objs.append(obj)
-    obj = os.path.join(BUILD, "build_device.o")
-    _run([HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-c", os.path.join(CSRC, "build_device.hip"), "-o", obj])
-    objs.append(obj)
+    for src in ("build_device.hip", "build_multigrid.hip"):
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, OUT)

@@ -17,8 +17,9 @@
 //                    edges and wall moment (a block scan gives the cell's offset in its brick, a
 //                    one-workgroup scan the brick's), then writing them at those offsets: the
 //                    host's dof-then-face order.
-// The surrounding stages (refinement bands, the cut list, brick ordering, special cells,
-// multigrid) stay on the host.
+//   multigrid        the coarse levels (build_multigrid.hip).
+// The surrounding stages (refinement bands, the cut list, brick ordering, special cells, the
+// dense coarse inverse) stay on the host.
 //
 // Built with -ffp-contract=off: no multiply-add is fused, as on the host.
 #include <hip/hip_runtime.h>
@@ -30,6 +31,10 @@
 
 namespace pcfd {
 void set_error(const std::string& m);
+namespace devbuild {
+double multigrid(hipStream_t stream, hipEvent_t e0, hipEvent_t e1, const Topology& T, int levels, int coarse_max,
+                 std::vector<MGLevel>& mg);
+}
 namespace {
 
 void check(hipError_t e, const char* what) {
@@ -449,6 +454,11 @@ public:
         download(P.wall_moment.data(), dwm, 9 * totals[1] * sizeof(double));
         raise_device_error();
         accumulate();
+    }
+
+    bool multigrid(const Topology& T, int levels, int coarse_max, std::vector<MGLevel>& mg) override {
+        kernel_ms_ += devbuild::multigrid(stream_, ev_[0], ev_[1], T, levels, coarse_max, mg);
+        return true;
     }
 
 private:

@@ -143,6 +143,28 @@ PCFD_HD inline int graph_neighbors(const BuildView& V, int leaf, int local, int 
     return 1;
 }
 
+// Morton order of a cell or brick position (21 bits per axis)
+PCFD_HD inline uint64_t spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+PCFD_HD inline uint64_t morton(int x, int y, int z) {
+    return spread3(static_cast<uint64_t>(x)) | (spread3(static_cast<uint64_t>(y)) << 1) |
+           (spread3(static_cast<uint64_t>(z)) << 2);
+}
+
+// Multigrid cell key: size exponent (cell size = finest h * 2^e) above the Morton order of the
+// cell's position at that size.  Sorting the keys orders a coarse level.
+PCFD_HD inline uint64_t mg_key(int e, int x, int y, int z) { return (static_cast<uint64_t>(e) << 60) | morton(x, y, z); }
+
+// the coarsest multigrid level is solved exactly (dense inverse) below this many cells
+constexpr int kCoarseMax = 2048;
+
 PCFD_HD inline double clamp01(double w) { return w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w); }
 PCFD_HD inline bool finite_value(double v) { return v == v && v - v == 0.0; }
 

@@ -40,19 +40,6 @@ void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body) 
     for (auto& th : pool) th.join();
 }
 
-inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffULL;
-    v = (v | v << 16) & 0x1f0000ff0000ffULL;
-    v = (v | v << 8) & 0x100f00f00f00f00fULL;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-    v = (v | v << 2) & 0x1249249249249249ULL;
-    return v;
-}
-inline uint64_t morton(int x, int y, int z) {
-    return spread3(static_cast<uint64_t>(x)) | (spread3(static_cast<uint64_t>(y)) << 1) |
-           (spread3(static_cast<uint64_t>(z)) << 2);
-}
 
 // ------------------------------------------------------------------------------------------
 // analytic solids
@@ -503,6 +490,8 @@ private:
     void host_cell_passes(int nb, CellPasses& P) const;
     void finalize();
     void build_multigrid();
+    void host_multigrid();
+    void coarse_inverse();
 };
 
 void Builder::sync_leaves() {
@@ -983,6 +972,11 @@ void Builder::finalize() {
 // (R = P^T).
 void Builder::build_multigrid() {
     T_.mg.clear();
+    if (!accel_ || !accel_->multigrid(T_, L_, kCoarseMax, T_.mg)) host_multigrid();
+    coarse_inverse();
+}
+
+void Builder::host_multigrid() {
     MGLevel L0;
     L0.n = static_cast<int>(T_.ndof);
     L0.exponent.assign(T_.ndof, 0);
@@ -1025,7 +1019,7 @@ void Builder::build_multigrid() {
 
     T_.mg.push_back(std::move(L0));
     const double hf = T_.finest_h;
-    const int coarse_max = 2048;  // the coarsest level is solved exactly (dense inverse)
+    const int coarse_max = kCoarseMax;
     int m = 0;
     std::vector<int> cur_exp = T_.mg[0].exponent;
     std::vector<int> cur_coord = coord;
@@ -1042,9 +1036,7 @@ void Builder::build_multigrid() {
         // smaller cell lies inside the parent's footprint (a finer AMR region that has not
         // caught up yet), and the cell is then carried over unchanged.  Blocked parents never
         // overlap an existing cell, so the coarse cells tile the fluid without overlap.
-        auto key_of = [](int e, int x, int y, int z) {
-            return (static_cast<uint64_t>(e) << 60) | morton(x, y, z);
-        };
+        auto key_of = mg_key;
         int max_e = 0;
         for (int64_t a = 0; a < cur_n; ++a) if (cur_act[a]) max_e = std::max(max_e, cur_exp[a]);
         std::vector<uint64_t> blocked;
@@ -1176,7 +1168,10 @@ void Builder::build_multigrid() {
         m = mm;
     }
     T_.mg.back().parent.assign(T_.mg.back().n, -1);
-    // dense inverse of the coarsest operator (Cholesky); skipped when singular
+}
+
+// dense inverse of the coarsest operator (Cholesky); skipped when singular
+void Builder::coarse_inverse() {
     MGLevel& K = T_.mg.back();
     if (T_.mg.size() > 1 && K.n <= 4096) {
         const int n = K.n;

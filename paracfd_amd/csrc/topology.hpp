@@ -184,7 +184,8 @@ struct CellPasses {
 //   connected_fluid: clear every active cell that is not in a component of the graph (faces with
 //     an open fraction > 0 between active cells) touching the x-max boundary (outlet), or,
 //     without an outlet, not in the largest component (ties: the one holding the lowest cell);
-//   cell_passes: cell_geometry on every cell, then cell_faces on every active cell, in dof order.
+//   cell_passes: cell_geometry on every cell, then cell_faces on every active cell, in dof order;
+//   multigrid: the coarse levels (see Builder::host_multigrid).
 class BuildAccelerator {
 public:
     virtual ~BuildAccelerator() = default;
@@ -192,6 +193,10 @@ public:
                         const std::vector<int>& cells, std::vector<CutData>& out) = 0;
     virtual void connected_fluid(const BuildView& view, int64_t nleaves, bool outlet, std::vector<uint8_t>& active) = 0;
     virtual void cell_passes(const BuildView& view, int nbricks, CellPasses& out) = 0;
+    // the multigrid levels (mg[0] the composite level's exponent and parent, then every coarse
+    // level) from the finished composite topology, as the host's coarsening builds them (the
+    // dense coarse inverse excluded); false to leave them to the host
+    virtual bool multigrid(const Topology& T, int levels, int coarse_max, std::vector<MGLevel>& mg) = 0;
     virtual int device() const { return -1; }       // the device it runs on, or -1
     virtual double kernel_ms() const { return 0.0; } // device kernel time so far
 };
