@@ -73,6 +73,11 @@ typedef struct pcfd_grid_desc {
     double min_volume_fraction; /* cut cells below this fluid fraction become solid */
     int subsamples;           /* sub-samples per axis for cut-cell volume/aperture */
     int outlet_xmax;          /* 1: Dirichlet p = 0 at x-max (pressure_outlet_xmax); 0: all Neumann */
+    double merge_volume_fraction; /* active cut cells below this fluid fraction, or with an axis whose
+                                 open area is below it times h^2, share their momentum updates with the
+                                 neighbour across their largest open face (redistribution groups, the
+                                 reference's same-side merge threshold, paraglider_config.h:37, and
+                                 momentum redistribution, amr_advection.cu:1108-1178); 0: none */
 } pcfd_grid_desc;
 
 typedef struct pcfd_topology pcfd_topology;
@@ -138,6 +143,10 @@ typedef enum pcfd_array_id {
                                       a pressure p + g.(x - c) is p S + D^T g */
     PCFD_ARR_EDGE_X = 21,          /* double[edges*3] centroid of the face's open part */
     PCFD_ARR_EDGE_DCOEF = 22,      /* double[edges] diffusion conductance */
+    PCFD_ARR_REDIST_GROUP = 40,    /* int32[special] redistribution group of each special cell or -1 */
+    PCFD_ARR_REDIST_OFF = 41,      /* int32[groups+1] */
+    PCFD_ARR_REDIST_MEMBER = 42,   /* int32[members] special ranks of each group's cells, ascending */
+    PCFD_ARR_REDIST_VOLUME = 43,   /* double[groups] fluid volume of each group */
     PCFD_ARR_CELL_CENTROID = 23,   /* double[dofs*3] fluid centroid (cell centre for full cells) */
     PCFD_ARR_LEVEL_H = 24,         /* double[levels] */
     /* non-orthogonal correction (after pcfd_topology_build_nonorthogonal; size -1 before) */

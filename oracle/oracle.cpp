@@ -80,6 +80,11 @@ typedef struct oracle_topology {
     const double* const* mg_diag;
     const double* coarse_inv;        /* may be NULL */
     const double* centroid;          /* 3 per dof: fluid centroid (Smagorinsky gradients) */
+    int64_t ngroups;                 /* momentum redistribution groups */
+    const int* redist_group;         /* per special cell: group or -1 (may be NULL when ngroups = 0) */
+    const int* redist_off;           /* groups + 1 */
+    const int* redist_member;        /* special ranks */
+    const double* redist_volume;     /* per group */
 } oracle_topology;
 
 }
@@ -106,6 +111,9 @@ struct Oracle {
     std::vector<double> edcoef;      // diffusion conductance per edge
     std::vector<double> edisp;       // 6 per edge: x_f - c_a, x_f - c_b
     std::vector<double> centroid;
+    // momentum redistribution groups: special rank -> group (or -1), members, volumes
+    std::vector<int> rgroup, roff, rmember;
+    std::vector<double> rvolume;
     double finest_h = 0, max_diff_coef = 0;
     // Smagorinsky least-squares gradient stencils of the special cells (built on first use)
     std::vector<int> sg_off, sg_nbr;
@@ -128,10 +136,36 @@ struct Oracle {
     // moment); 7 compact face pressure without the skew term; 8 compact reconstruction without
     // the transverse correction; 9 compact transport donor-only; 10 compact transport slope
     // without the (1 - C) factor; 11 gradient stencil weight area/|d| instead of area/|d|^2;
-    // 12 wall traction in the impulse without D^T g (the loads keep it).
+    // 12 wall traction in the impulse without D^T g (the loads keep it); 13 a redistribution
+    // group's increment divided by each member's own volume instead of the group's.
     int defect = 0;
 
     double h_of(int64_t a) const { return level_h[brick_level[a / BC]]; }
+    bool in_group(int64_t a) const {
+        const int s = special_of[a];
+        return s >= 0 && !rgroup.empty() && rgroup[s] >= 0;
+    }
+    // Momentum increments d (per cell, 3 components) applied as v += d / V, except on the members
+    // of a redistribution group: they all take the group's summed increment over the group's
+    // volume (apply_or_accumulate_redistributed_momentum + finalize_redistributed_state,
+    // amr_advection.cu:118-149, 1251-1268).
+    void apply_increments(const std::vector<double> (&d)[3]) {
+        for (int64_t a = 0; a < n; ++a) {
+            if (!active(a) || in_group(a)) continue;
+            const double iv = 1.0 / vol(a);
+            vx[a] += d[0][a] * iv; vy[a] += d[1][a] * iv; vz[a] += d[2][a] * iv;
+        }
+        for (size_t g = 0; g + 1 < roff.size(); ++g) {
+            double S[3] = {0, 0, 0};
+            for (int k = roff[g]; k < roff[g + 1]; ++k)
+                for (int c = 0; c < 3; ++c) S[c] += d[c][sdof[rmember[k]]];
+            for (int k = roff[g]; k < roff[g + 1]; ++k) {
+                const int64_t a = sdof[rmember[k]];
+                const double iv = 1.0 / (defect == 13 ? vol(a) : rvolume[g]);
+                vx[a] += S[0] * iv; vy[a] += S[1] * iv; vz[a] += S[2] * iv;
+            }
+        }
+    }
     double vol(int64_t a) const {
         const int s = special_of[a];
         if (s >= 0) return special_volume[s];
@@ -247,11 +281,7 @@ struct Oracle {
                 d[c][eb[e]] += f;
             }
         }
-        for (int64_t a = 0; a < n; ++a) {
-            if (!active(a)) continue;
-            const double iv = 1.0 / vol(a);
-            vx[a] += d[0][a] * iv; vy[a] += d[1][a] * iv; vz[a] += d[2][a] * iv;
-        }
+        apply_increments(d);
     }
     // ---- diffusion and the Smagorinsky model ------------------------------------------------
     // Symmetric 3x3 pseudo-inverse: cyclic Jacobi eigen-decomposition (Golub & Van Loan,
@@ -473,11 +503,7 @@ struct Oracle {
                     d[c][ea[e]] += m; d[c][eb[e]] -= m;
                 }
             }
-            for (int64_t a = 0; a < n; ++a) {
-                if (!active(a)) continue;
-                const double iv = 1.0 / vol(a);
-                vx[a] += d[0][a] * iv; vy[a] += d[1][a] * iv; vz[a] += d[2][a] * iv;
-            }
+            apply_increments(d);
         }
     }
     void reconstruct() {
@@ -748,6 +774,7 @@ struct Oracle {
         for (size_t q = 0; q < np; ++q) order[q] = static_cast<int>(q);
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return patch_dof[x] < patch_dof[y]; });
         std::vector<double> kp(np, 0.0);
+        std::vector<double> dw[3] = {std::vector<double>(n, 0.0), std::vector<double>(n, 0.0), std::vector<double>(n, 0.0)};
         for (size_t j0 = 0; j0 < np;) {
             const int64_t a = patch_dof[order[j0]];
             size_t j1 = j0;
@@ -784,7 +811,11 @@ struct Oracle {
             const double l22 = std::sqrt(a22 - l20 * l20 - l21 * l21);
             const double y0 = v[0] / l00, y1 = (v[1] - l10 * y0) / l11, y2 = (v[2] - l20 * y0 - l21 * y1) / l22;
             const double x2 = y2 / l22, x1 = (y1 - l21 * x2) / l11, x0 = (y0 - l10 * x1 - l20 * x2) / l00;
-            vx[a] = x0; vy[a] = x1; vz[a] = x2;
+            if (in_group(a)) {  // the group shares the increment (amr_advection.cu:1279-1293)
+                dw[0][a] = vol(a) * (x0 - v[0]); dw[1][a] = vol(a) * (x1 - v[1]); dw[2][a] = vol(a) * (x2 - v[2]);
+            } else {
+                vx[a] = x0; vy[a] = x1; vz[a] = x2;
+            }
             for (size_t j = j0; j < j1; ++j) {
                 const int q = order[j];
                 const double* S = &patch_S[3 * q];
@@ -795,6 +826,19 @@ struct Oracle {
                 for (int c = 0; c < 3; ++c) wall_force[3 * q + c] = opt.rho * kp[q] * (xv[c] - un * nn[c]);
             }
             j0 = j1;
+        }
+        if (!roff.empty() && roff.size() > 1) {
+            // members only: the others were set above and carry no increment here
+            for (size_t g = 0; g + 1 < roff.size(); ++g) {
+                double S[3] = {0, 0, 0};
+                for (int k = roff[g]; k < roff[g + 1]; ++k)
+                    for (int c = 0; c < 3; ++c) S[c] += dw[c][sdof[rmember[k]]];
+                for (int k = roff[g]; k < roff[g + 1]; ++k) {
+                    const int64_t a = sdof[rmember[k]];
+                    const double iv = 1.0 / (defect == 13 ? vol(a) : rvolume[g]);
+                    vx[a] += S[0] * iv; vy[a] += S[1] * iv; vz[a] += S[2] * iv;
+                }
+            }
         }
         wall_applied = true;
     }
@@ -845,11 +889,7 @@ struct Oracle {
                 for (int c = 0; c < 3; ++c) F[c] = p[patch_dof[pch]] * patch_S[3 * pch + c];
             for (int c = 0; c < 3; ++c) d[c][patch_dof[pch]] -= s * F[c];
         }
-        for (int64_t a = 0; a < n; ++a) {
-            if (!active(a)) continue;
-            const double iv = 1.0 / vol(a);
-            vx[a] += d[0][a] * iv; vy[a] += d[1][a] * iv; vz[a] += d[2][a] * iv;
-        }
+        apply_increments(d);
     }
     void loads(double pref, pcfd_loads* out) const {
         std::memset(out, 0, sizeof(*out));
@@ -933,6 +973,12 @@ void* oracle_create(const oracle_topology* t, const pcfd_options* o) {
         s->patch_x.assign(t->patch_x, t->patch_x + 3 * t->npatches);
         s->patch_D.assign(t->patch_D, t->patch_D + 9 * t->npatches);
         s->edcoef.assign(t->edge_dcoef, t->edge_dcoef + t->nedges);
+        if (t->ngroups > 0) {
+            s->rgroup.assign(t->redist_group, t->redist_group + t->nspecial);
+            s->roff.assign(t->redist_off, t->redist_off + t->ngroups + 1);
+            s->rmember.assign(t->redist_member, t->redist_member + t->redist_off[t->ngroups]);
+            s->rvolume.assign(t->redist_volume, t->redist_volume + t->ngroups);
+        }
         s->finest_h = t->finest_h;
         s->max_diff_coef = t->max_diffusion_coefficient;
         if (!t->centroid) throw std::invalid_argument("the oracle needs the cell centroids");

@@ -54,7 +54,8 @@ class OracleTopology(C.Structure):
                 ("max_diffusion_coefficient", C.c_double), ("mg_levels", C.c_int), ("mg_n", C.c_void_p),
                 ("mg_parent", C.c_void_p), ("mg_row_off", C.c_void_p), ("mg_col", C.c_void_p),
                 ("mg_coef", C.c_void_p), ("mg_diag", C.c_void_p), ("coarse_inv", C.c_void_p),
-                ("centroid", C.c_void_p)]
+                ("centroid", C.c_void_p), ("ngroups", C.c_int64), ("redist_group", C.c_void_p),
+                ("redist_off", C.c_void_p), ("redist_member", C.c_void_p), ("redist_volume", C.c_void_p)]
 
 
 def _ptr(a):
@@ -104,7 +105,8 @@ class Oracle:
         keep = {}
         for name in ["level_h", "brick_level", "brick_flags", "brick_nbr", "mask", "special_dof", "special_volume",
                      "special_patch", "edge_a", "edge_b", "edge_axis", "edge_area", "edge_coef", "edge_w",
-                     "patch_dof", "patch_S", "patch_x", "patch_D", "edge_x", "edge_dcoef", "centroid"]:
+                     "patch_dof", "patch_S", "patch_x", "patch_D", "edge_x", "edge_dcoef", "centroid",
+                     "redist_group", "redist_off", "redist_member", "redist_volume"]:
             keep[name] = np.ascontiguousarray(t.array(name))
         nl = t.mg_levels
         mg_n = np.array([t.mg_n(m) for m in range(nl)], dtype=np.int64)
@@ -130,6 +132,7 @@ class Oracle:
         ot.outlet = 1 if t.outlet else 0
         for name, a in keep.items():
             setattr(ot, name, _ptr(a))
+        ot.ngroups = int(keep["redist_volume"].size)
         ot.finest_h = float(keep["level_h"][-1])
         ot.max_diffusion_coefficient = t.max_diffusion_coefficient
         ot.mg_levels = nl

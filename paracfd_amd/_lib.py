@@ -30,6 +30,8 @@ ARR = dict(
     nf_delta=(29, np.float64), nf_k=(30, np.float64), nf_w=(31, np.float64), nf_lnode=(32, np.int32),
     nf_unode=(33, np.int32), ng_dof=(34, np.int32), ng_off=(35, np.int32), ng_nbr=(36, np.int32),
     ng_w=(37, np.float64), ng_rank=(38, np.uint8), ng_ring=(39, np.uint8),
+    redist_group=(40, np.int32), redist_off=(41, np.int32), redist_member=(42, np.int32),
+    redist_volume=(43, np.float64),
 )
 MG_ARR = dict(row_off=(0, np.int32), col=(1, np.int32), coef=(2, np.float64), diag=(3, np.float64),
               parent=(4, np.int32), coarse_inv=(5, np.float64))
@@ -44,7 +46,7 @@ class GeometryDesc(C.Structure):
 class GridDesc(C.Structure):
     _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("base_cells_x", C.c_int), ("levels", C.c_int),
                 ("refine_band", C.c_double * 4), ("min_volume_fraction", C.c_double), ("subsamples", C.c_int),
-                ("outlet_xmax", C.c_int)]
+                ("outlet_xmax", C.c_int), ("merge_volume_fraction", C.c_double)]
 
 
 class TopologyInfo(C.Structure):

@@ -11,7 +11,8 @@ import math
 from . import _lib as L
 
 
-def sphere_box(cells_x=128, levels=1, subsamples=4, min_volume_fraction=0.1, outlet=True, band=0.25):
+def sphere_box(cells_x=128, levels=1, subsamples=4, min_volume_fraction=0.1, outlet=True, band=0.25,
+               merge_volume_fraction=0.25):
     """Sphere r=0.25 at (1,1,1) in [0,4]x[0,2]x[0,2]; h = 4/cells_x (128 -> h=1/32, 128x64x64)."""
     g = L.GeometryDesc()
     g.kind = L.GEOM_SPHERE
@@ -25,12 +26,13 @@ def sphere_box(cells_x=128, levels=1, subsamples=4, min_volume_fraction=0.1, out
     d.refine_band[:] = (band, band * 0.5, band * 0.25, 0.0)
     d.min_volume_fraction = min_volume_fraction
     d.subsamples = subsamples
+    d.merge_volume_fraction = merge_volume_fraction
     d.outlet_xmax = 1 if outlet else 0
     return g, d
 
 
 def naca_wing(digits=2412, alpha_deg=4.0, cells_x=128, levels=2, bands=(0.3, 0.0, 0.0), subsamples=4,
-              min_volume_fraction=0.1, chord=1.0, span=2.03):
+              min_volume_fraction=0.1, chord=1.0, span=2.03, merge_volume_fraction=0.25):
     """Straight NACA 4-digit wing, leading edge at the origin, span along y, pitched about +y."""
     g = L.GeometryDesc()
     g.kind = L.GEOM_NACA4_WING
@@ -47,12 +49,14 @@ def naca_wing(digits=2412, alpha_deg=4.0, cells_x=128, levels=2, bands=(0.3, 0.0
     d.refine_band[:] = tuple(bands) + (0.0,) * (4 - len(bands))
     d.min_volume_fraction = min_volume_fraction
     d.subsamples = subsamples
+    d.merge_volume_fraction = merge_volume_fraction
     d.outlet_xmax = 1
     return g, d
 
 
 def paraglider(span=8.0, alpha_deg=6.0, anhedral_deg=25.0, sweep_deg=8.0, digits=2412, reflex=0.02, chord=0.0,
-               cells_x=160, levels=3, bands=(1.0, 0.25), subsamples=4, min_volume_fraction=0.1):
+               cells_x=160, levels=3, bands=(1.0, 0.25), subsamples=4, min_volume_fraction=0.1,
+               merge_volume_fraction=0.25):
     """Synthetic paraglider-like wing: elliptic planform of aspect ratio 5 (chord 0) on a circular
     span arc drooping anhedral_deg at each tip, leading-edge sweep, NACA section with a reflexed
     (upturned) trailing edge, pitched alpha about +y through the root leading edge at the origin.
@@ -79,6 +83,7 @@ def paraglider(span=8.0, alpha_deg=6.0, anhedral_deg=25.0, sweep_deg=8.0, digits
     d.refine_band[:] = tuple(b * c0 for b in bands) + (0.0,) * (4 - len(bands))
     d.min_volume_fraction = min_volume_fraction
     d.subsamples = subsamples
+    d.merge_volume_fraction = merge_volume_fraction
     d.outlet_xmax = 1
     return g, d
 

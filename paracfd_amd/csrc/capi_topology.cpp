@@ -171,6 +171,10 @@ bool lookup(const pcfd::Topology& t, int id, ArrayRef& r) {
         case PCFD_ARR_PATCH_D: return set(t.patch_D);
         case PCFD_ARR_EDGE_X: return set(t.edge_x);
         case PCFD_ARR_EDGE_DCOEF: return set(t.edge_dcoef);
+        case PCFD_ARR_REDIST_GROUP: return set(t.redist_group);
+        case PCFD_ARR_REDIST_OFF: return set(t.redist_off);
+        case PCFD_ARR_REDIST_MEMBER: return set(t.redist_member);
+        case PCFD_ARR_REDIST_VOLUME: return set(t.redist_volume);
         case PCFD_ARR_CELL_CENTROID: return set(t.centroid);
         case PCFD_ARR_LEVEL_H: return set(t.level_h);
         default: break;

@@ -84,6 +84,12 @@ struct MeshView {
     const double* patch_S;
     const double* patch_x;
     const double* patch_D;   // 9 per patch: int_W (x - c) (x) n dA, row-major
+    int ngroups;             // momentum redistribution groups
+    const int* rgroup;       // per special cell: its group or -1
+    const int* roff;         // groups + 1
+    const int* rmember;      // special ranks of each group's members
+    const double* rvolume;   // per group: fluid volume
+    double* rdelta;          // 3 per special cell: a member's momentum increment (scratch)
     double speed, rho, cfl, nu, rtol;
     double finest_h;
     int max_iterations;

@@ -35,6 +35,37 @@ __device__ __forceinline__ R muscl_value(R swept, R pred, R va, R vb, R vl, bool
 
 __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
 
+// A member of a momentum redistribution group hands its increment to the group (k_redistribute)
+// instead of applying it (apply_or_accumulate_redistributed_momentum, amr_advection.cu:118-129).
+__device__ __forceinline__ bool redistributed(const MeshView& M, int sp) {
+    return M.ngroups > 0 && sp >= 0 && M.rgroup[sp] >= 0;
+}
+template <class T>
+__device__ __forceinline__ void hand_over(const MeshView& M, int sp, const T d[3]) {
+    for (int c = 0; c < 3; ++c) M.rdelta[3 * static_cast<int64_t>(sp) + c] = double(d[c]);
+}
+
+// Every member of group g takes the group's summed increment (member order) over the group's
+// volume (finalize_redistributed_state_kernel, amr_advection.cu:142-149); out holds the
+// members' values before the increment.
+template <class R>
+__global__ __launch_bounds__(256) void k_redistribute(MeshView M, R* __restrict__ ox, R* __restrict__ oy,
+                                                      R* __restrict__ oz) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= M.ngroups) return;
+    double S[3] = {0.0, 0.0, 0.0};
+    for (int k = M.roff[g]; k < M.roff[g + 1]; ++k) {
+        const int64_t s = M.rmember[k];
+        for (int c = 0; c < 3; ++c) S[c] += M.rdelta[3 * s + c];
+    }
+    const double iv = 1.0 / M.rvolume[g];
+    R* out[3] = {ox, oy, oz};
+    for (int k = M.roff[g]; k < M.roff[g + 1]; ++k) {
+        const int64_t a = M.special_dof[M.rmember[k]];
+        for (int c = 0; c < 3; ++c) out[c][a] = R(double(out[c][a]) + S[c] * iv);
+    }
+}
+
 // Least-squares gradient stencils of the special cells: neighbours across every open face
 // (structured faces, area h^2, and compact edges, open area), weight area / |d|^2 on the
 // fluid centroids, the pseudo-inverted 3x3 normal matrix (amr_advection.cu:822-865).
@@ -130,6 +161,57 @@ __global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> 
 // the donor's value reconstructed to the face centroid with its least-squares gradient g
 // (velocity before the transport), scaled by 1 - C, C = min(1, |swept| / V_donor), and clamped
 // to the two cells' values (compact_muscl_cell_face_value, amr_advection.cu:94-105).
+// Velocity tile of a brick with a two-cell face halo for the MUSCL stencil: slot
+// (k+2) 144 + (j+2) 12 + (i+2) of a 12^3 box.  The brick's cells and its six face slabs, two
+// cells deep, are filled (zero where there is no same-level neighbour brick); edges and corners
+// are never read.  A slab cell is read only behind a structured face, whose neighbour exists.
+constexpr int kTile2 = 1728;
+__device__ __forceinline__ int tile2_index(int i, int j, int k) { return (k + 2) * 144 + (j + 2) * 12 + (i + 2); }
+
+// every lane: its own cell and, for the 768 halo slots, halo slot q and q + 512; wave w serves
+// one face and layer, so the neighbour brick is a scalar read of the brick record
+template <class R>
+__device__ __forceinline__ void stage_tile2(const MeshView& M, const FieldsView<R>& F, int b, R (*vt)[kTile2],
+                                            uint8_t* mt, uint8_t m_own, R own[3]) {
+    const int q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    for (int c = 0; c < 3; ++c) own[c] = F.v[c][a];
+    R hv[2][3];
+    uint8_t hm[2] = {0, 0};
+    int hidx[2] = {-1, -1};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        for (int c = 0; c < 3; ++c) hv[r][c] = R(0);
+        const int h = q + 512 * r;
+        if (h >= 768) continue;
+        const int face = __builtin_amdgcn_readfirstlane(h >> 7), layer = (h >> 6) & 1, t = h & 63;
+        const int axis = face >> 1, up = face & 1, a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+        int cc[3];
+        cc[a1] = t & 7;
+        cc[a2] = t >> 3;
+        cc[axis] = up ? 8 + layer : -1 - layer;
+        hidx[r] = tile2_index(cc[0], cc[1], cc[2]);
+        const int nb = M.meta[b].nbr[face];
+        if (nb >= 0) {
+            cc[axis] = up ? layer : 7 - layer;
+            const int64_t o = static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2];
+            for (int c = 0; c < 3; ++c) hv[r][c] = F.v[c][o];
+            hm[r] = M.mask[o];
+        }
+    }
+    const int ti = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
+    for (int c = 0; c < 3; ++c) vt[c][ti] = own[c];
+    mt[ti] = m_own;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+        if (hidx[r] >= 0) {
+            for (int c = 0; c < 3; ++c) vt[c][hidx[r]] = hv[r][c];
+            mt[hidx[r]] = hm[r];
+        }
+    __syncthreads();
+}
+
+// Regular faces read the MUSCL stencil (two cells each side) from the LDS tile.
 template <class R>
 __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* __restrict__ ox, R* __restrict__ oy,
                                                 R* __restrict__ oz, const DevScalars* S, const R* __restrict__ G) {
@@ -137,39 +219,44 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    __shared__ R vt[3][kTile2];
+    __shared__ uint8_t mt[kTile2];
+    R va[3];
+    stage_tile2<R>(M, F, b, vt, mt, m, va);
     R* out[3] = {ox, oy, oz};
     if (!(m & kActive)) {
-        for (int c = 0; c < 3; ++c) out[c][a] = F.v[c][a];
+        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
         return;
     }
     const R dt = R(S[M.inst[b]].dt);
     const R h = R(cell_h(M, b)), A = h * h;
-    R va[3], d[3] = {R(0), R(0), R(0)};
-    for (int c = 0; c < 3; ++c) va[c] = F.v[c][a];
+    R d[3] = {R(0), R(0), R(0)};
+    const int ta = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
     for (int axis = 0; axis < 3; ++axis) {
+        const int st = axis == 0 ? 1 : (axis == 1 ? 12 : 144);
         if (m & (kPX << axis)) {  // face (a, b1)
-            const int64_t b1 = neighbor_dof(M, b, q, axis, 1);
-            const uint8_t mb = M.mask[b1];
+            const int b1 = ta + st;
+            const uint8_t mb = mt[b1];
             const bool has_l = (m & (kMX << axis)) != 0, has_u = (mb & (kPX << axis)) != 0;
-            const int64_t l = has_l ? neighbor_dof(M, b, q, axis, -1) : a;
-            const int64_t u = has_u ? neighbor_dof(M, b1, axis, 1) : b1;
+            const int l = has_l ? ta - st : ta;
+            const int u = has_u ? b1 + st : b1;
             const R un = F.f[axis][a];
             const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
             for (int c = 0; c < 3; ++c) {
-                const R fv = muscl_value(swept, pred, va[c], F.v[c][b1], F.v[c][l], has_l, F.v[c][u], has_u);
+                const R fv = muscl_value(swept, pred, va[c], vt[c][b1], vt[c][l], has_l, vt[c][u], has_u);
                 d[c] -= swept * fv;
             }
         }
         if (m & (kMX << axis)) {  // face (b0, a)
-            const int64_t b0 = neighbor_dof(M, b, q, axis, -1);
-            const uint8_t m0 = M.mask[b0];
+            const int b0 = ta - st;
+            const uint8_t m0 = mt[b0];
             const bool has_l = (m0 & (kMX << axis)) != 0, has_u = (m & (kPX << axis)) != 0;
-            const int64_t l = has_l ? neighbor_dof(M, b0, axis, -1) : b0;
-            const int64_t u = has_u ? neighbor_dof(M, b, q, axis, 1) : a;
-            const R un = F.f[axis][b0];
+            const int l = has_l ? b0 - st : b0;
+            const int u = has_u ? ta + st : ta;
+            const R un = F.f[axis][neighbor_dof(M, b, q, axis, -1)];
             const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
             for (int c = 0; c < 3; ++c) {
-                const R fv = muscl_value(swept, pred, F.v[c][b0], va[c], F.v[c][l], has_l, F.v[c][u], has_u);
+                const R fv = muscl_value(swept, pred, vt[c][b0], va[c], vt[c][l], has_l, vt[c][u], has_u);
                 d[c] += swept * fv;
             }
         }
@@ -214,6 +301,11 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             }
         }
     }
+    if (redistributed(M, sp)) {
+        hand_over(M, sp, d);
+        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
+        return;
+    }
     const R iv = R(1) / V;
     for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
 }
@@ -234,19 +326,34 @@ __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict
     R* out[3] = {ox, oy, oz};
     const DevScalars& I = S[M.inst[b]];
     const double dts = I.dts;
-    if (!(m & kActive) || step >= I.subs) {  // this instance needs fewer substeps
+    if (step >= I.subs) {  // this instance needs fewer substeps (the whole brick)
         for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
+        if (redistributed(M, sp)) {
+            const double zero[3] = {0.0, 0.0, 0.0};
+            hand_over(M, sp, zero);
+        }
+        return;
+    }
+    // the three components with a one-cell face halo in LDS
+    __shared__ R tv[3][kTileN];
+    TileFetch<R> tf[3];
+    for (int c = 0; c < 3; ++c) tf[c] = fetch_tile<R>(M, b, [&](int64_t i) -> R { return in[c][i]; });
+    for (int c = 0; c < 3; ++c) stage_tile<R>(tv[c], tf[c]);
+    R va[3];
+    for (int c = 0; c < 3; ++c) va[c] = tf[c].own;
+    if (!(m & kActive)) {
+        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
         return;
     }
     const R h = R(cell_h(M, b));
     const R k = R(dts * M.nu) * h;
-    R va[3], d[3] = {R(0), R(0), R(0)};
-    for (int c = 0; c < 3; ++c) va[c] = in[c][a];
+    R d[3] = {R(0), R(0), R(0)};
+    const int ta = tile_index(q & 7, (q >> 3) & 7, q >> 6);
     for (int axis = 0; axis < 3; ++axis)
         for (int s = -1; s <= 1; s += 2) {
             if (!(m & ((s > 0 ? kPX : kMX) << axis))) continue;
-            const int64_t o = neighbor_dof(M, b, q, axis, s);
-            for (int c = 0; c < 3; ++c) d[c] += k * (in[c][o] - va[c]);
+            const int o = ta + s * (axis == 0 ? 1 : (axis == 1 ? 10 : 100));
+            for (int c = 0; c < 3; ++c) d[c] += k * (tv[c][o] - va[c]);
         }
     R V = h * h * h;
     if (sp >= 0) {
@@ -258,6 +365,11 @@ __global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict
             const R ke = R(dts * M.nu * M.edcoef[e]);
             for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
         }
+    }
+    if (redistributed(M, sp)) {
+        hand_over(M, sp, d);
+        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
+        return;
     }
     const R iv = R(1) / V;
     for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
@@ -369,6 +481,10 @@ __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __rest
     const double dts = I.dts;
     if (!(m & kActive) || step >= I.subs) {
         for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
+        if (redistributed(M, sp)) {
+            const double zero[3] = {0.0, 0.0, 0.0};
+            hand_over(M, sp, zero);
+        }
         return;
     }
     const R h = R(cell_h(M, b));
@@ -392,6 +508,11 @@ __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __rest
             const R ke = R(dts) * (half * (na + nu[o])) * R(M.edcoef[e]);
             for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
         }
+    }
+    if (redistributed(M, sp)) {
+        hand_over(M, sp, d);
+        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
+        return;
     }
     const R iv = R(1) / V;
     for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
@@ -544,8 +665,14 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     const uint8_t m = M.mask[a];
     const DevScalars& I = S[M.inst[b]];
-    if (only_converged && !I.converged) return;  // block-uniform
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    if (only_converged && !I.converged) {  // block-uniform
+        if (redistributed(M, sp)) {
+            const double zero[3] = {0.0, 0.0, 0.0};
+            hand_over(M, sp, zero);
+        }
+        return;
+    }
     if (!(m & kActive)) return;
     const double dt = dt_given > 0.0 ? dt_given : I.dt;
     const double s = dt / M.rho, h = cell_h(M, b), A = h * h;
@@ -589,6 +716,10 @@ __global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, co
             wall_force_p(M, p, gp, pch, sp, 0.0, Fw);
             for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
         }
+    }
+    if (redistributed(M, sp)) {
+        hand_over(M, sp, d);
+        return;
     }
     const double iv = 1.0 / V;
     for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(F.v[c][a]) + d[c] * iv);
@@ -745,7 +876,13 @@ __global__ __launch_bounds__(256) void k_smooth_wall(MeshView M, FieldsView<R> F
     const double l22 = sqrt(a22 - l20 * l20 - l21 * l21);
     const double y0 = v[0] / l00, y1 = (v[1] - l10 * y0) / l11, y2 = (v[2] - l20 * y0 - l21 * y1) / l22;
     const double x2 = y2 / l22, x1 = (y1 - l21 * x2) / l11, x0 = (y0 - l10 * x1 - l20 * x2) / l00;
-    F.v[0][a] = R(x0); F.v[1][a] = R(x1); F.v[2][a] = R(x2);
+    const int sp = M.patch_sp[wall_patch[wall_off[w]]];
+    if (redistributed(M, sp)) {  // the group shares the increment (amr_advection.cu:1279-1293)
+        const double dm[3] = {wall_vol[w] * (x0 - v[0]), wall_vol[w] * (x1 - v[1]), wall_vol[w] * (x2 - v[2])};
+        hand_over(M, sp, dm);
+    } else {
+        F.v[0][a] = R(x0); F.v[1][a] = R(x1); F.v[2][a] = R(x2);
+    }
     // the force from the stored (storage-precision) velocity, as the reference reads it back
     const double vn[3] = {double(R(x0)), double(R(x1)), double(R(x2))};
     for (int j = wall_off[w]; j < wall_off[w + 1]; ++j) {

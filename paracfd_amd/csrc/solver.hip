@@ -370,9 +370,10 @@ private:
     NNView nnv() const { return NNView{nn_w_, nn_wt_, nn_p_ + kNNW}; }
     void set_nn_off(bool off) { if (nn_) HIP_CHECK(hipMemsetAsync(nn_off_, off ? 1 : 0, sizeof(int), stream_)); }
     unsigned nn_grid() const { return static_cast<unsigned>(nn_list_ ? nn_nlist_ : nb_); }
-    // network correction after the V-cycle's last kernel, then the r.z finalisation
+    // network correction after the V-cycle's last kernel, then the r.z finalisation; a network
+    // that is off (untrained, switched off or rejected by validation) is not launched at all
     void pc_tail(int fin) {
-        if (nn_) {
+        if (nn_ && nn_on_) {
             LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
             LAUNCH(k_nn_add, nn_grid(), 512, nn_list_, nn_dz_, z_, S_, nn_off_);
         }
@@ -420,14 +421,27 @@ private:
         (used ? trial_ms_ : trial_plain_ms_) += project_ms;
         if (--trial_left_ > 0) return;
         if (trial_ms_ > trial_plain_ms_) {
-            nn_on_ = false;
+            set_network(false);
             nn_rejected_ = true;
             set_nn_off(true);
         }
     }
+    // the network's state; the captured iteration graphs hold its launches only while it is on
+    void set_network(bool on) {
+        if (on != nn_on_) {
+            HIP_CHECK(hipStreamSynchronize(stream_));
+            drop_graph();
+        }
+        nn_on_ = on;
+    }
     bool pcg_loop(bool watch_stall);
     void aggregate_stats(pcfd_step_stats* st) const;
     void advect_core();
+    // the redistribution groups' pooled increments into the current velocity (after a kernel
+    // that handed them over)
+    void redistribute() {
+        if (M_.ngroups) LAUNCH(k_redistribute<R>, blocks_for(M_.ngroups, 256), 256, M_, v_[0], v_[1], v_[2]);
+    }
     void loads_instance(int i, double pref, pcfd_loads* out);
     void enqueue_loads(int i, double pref);
     void read_loads(int i, pcfd_loads* out) const;
@@ -655,6 +669,14 @@ void SolverImpl<R>::construct(const Topology& T) {
     M_.patch_S = upload(T.patch_S);
     M_.patch_x = upload(T.patch_x);
     M_.patch_D = upload(T.patch_D);
+    M_.ngroups = static_cast<int>(T.redist_volume.size());
+    if (M_.ngroups > 0) {
+        M_.rgroup = upload(T.redist_group);
+        M_.roff = upload(T.redist_off);
+        M_.rmember = upload(T.redist_member);
+        M_.rvolume = upload(T.redist_volume);
+        M_.rdelta = alloc<double>(3 * static_cast<int64_t>(T.special_dof.size()));
+    }
     {
         // per compact edge: special ranks of its cells and the face centroid's offsets from
         // their centroids; per patch: the special rank of its cell
@@ -1597,6 +1619,7 @@ template <class R>
 void SolverImpl<R>::impulse(double dt) {
     grad_p();
     LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, dt, 0);
+    redistribute();
 }
 
 template <class R>
@@ -1618,6 +1641,7 @@ void SolverImpl<R>::advect_core() {
     grad_v();
     probed(2, [&] { LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); });
     for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
+    redistribute();
     const bool smag = opt_.smagorinsky_cs > 0;
     if (smag) {
         // cell viscosity from the transported velocity, substeps from the largest rate
@@ -1638,11 +1662,15 @@ void SolverImpl<R>::advect_core() {
         if (smag) LAUNCH(k_diffuse_var<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], nu_, S_, s);
         else LAUNCH(k_diffuse<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], S_, s);
         for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
+        redistribute();
     }
     // wall shear after the diffusion, before the face reconstruction (external_aero_core.cpp:319)
     if (nwall_) {
+        // group members without a wall contribute no increment to this update
+        if (M_.ngroups) HIP_CHECK(hipMemsetAsync(M_.rdelta, 0, 3 * static_cast<size_t>(M_.nspecial) * sizeof(double), stream_));
         LAUNCH(k_smooth_wall<R>, blocks_for(nwall_, 256), 256, M_, fields(), nwall_, wall_cell_, wall_off_, wall_patch_,
                wall_vol_, wall_dist_, opt_.nu, opt_.rho, wall_k_, wall_force_, S_);
+        redistribute();
         std::fill(wall_applied_.begin(), wall_applied_.end(), 1);
     }
     reconstruct();
@@ -1657,6 +1685,7 @@ void SolverImpl<R>::initialize(pcfd_step_stats* st) {
     project(dt, false, nullptr);
     grad_p();
     LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, dt, 1);  // converged instances only
+    redistribute();
     LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_VMAX, nb_);
     sync_scalars();
@@ -1693,6 +1722,7 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     // impulse and time advance for the instances whose projection converged
     grad_p();
     LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, 0.0, 1);
+    redistribute();
     for (int i = 0; i < ninst_; ++i)
         if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
     time = inst_time_[0];
@@ -2289,7 +2319,7 @@ void SolverImpl<R>::learned_train(int steps, int batch, double lr, pcfd_learned_
     }
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipEventSynchronize(ev_[1]));
-    nn_on_ = true;
+    set_network(true);
     nn_rejected_ = false;
     trial_left_ = 2 * kTrial;
     trial_ms_ = trial_plain_ms_ = 0.0;
@@ -2319,7 +2349,7 @@ void SolverImpl<R>::learned_weights(float* w, float* b, const float* wset, const
         HIP_CHECK(hipMemcpy(nn_p_, wset, kNNW * sizeof(float), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(nn_p_ + kNNW, bset, kNNB * sizeof(float), hipMemcpyHostToDevice));
         LAUNCH(k_nn_pack, blocks_for(kNNW, 256), 256, nn_p_, nn_w_, nn_wt_);
-        nn_on_ = true;
+        set_network(true);
         set_nn_off(false);
         HIP_CHECK(hipStreamSynchronize(stream_));
         return;
@@ -2331,7 +2361,7 @@ void SolverImpl<R>::learned_weights(float* w, float* b, const float* wset, const
 template <class R>
 void SolverImpl<R>::learned_enable(bool on) {
     if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
-    nn_on_ = on;
+    set_network(on);
     trial_left_ = 0;  // an explicit choice ends any validation in progress
     nn_rejected_ = false;
     set_nn_off(!on);

@@ -945,6 +945,70 @@ void Builder::finalize() {
             T_.special_inc[fill[special_of[T_.edge_b[e]]]++] = ~e;
         }
     }
+    // momentum redistribution groups (amr_advection.cu:1108-1178): an active cut cell whose fluid
+    // fraction is below merge_volume_fraction, or with an axis whose open area (both faces) lies
+    // in (0, merge_volume_fraction h^2), is united with the neighbour across its largest compact
+    // edge (the first in edge order, as the reference's largest donor); the groups are the
+    // components of these unions, ordered by their lowest cell
+    T_.redist_group.assign(nsp, -1);
+    T_.redist_off.assign(1, 0);
+    T_.redist_member.clear();
+    T_.redist_volume.clear();
+    if (dd_.merge_volume_fraction > 0) {
+        const double fm = dd_.merge_volume_fraction;
+        std::vector<int> parent(nsp);
+        std::iota(parent.begin(), parent.end(), 0);
+        auto root = [&](int s) {
+            while (parent[s] != s) s = parent[s] = parent[parent[s]];
+            return s;
+        };
+        std::vector<uint8_t> seed(nsp, 0);
+        for (int s = 0; s < nsp; ++s) {
+            const int64_t a = T_.special_dof[s];
+            const double h = grid_[leaves_[a / kBC].level].h;
+            if (!(volume[a] < h * h * h)) continue;
+            double dir[3] = {0, 0, 0}, best = 0;
+            int donor = -1;
+            for (int k = T_.special_inc_off[s]; k < T_.special_inc_off[s + 1]; ++k) {
+                const int raw = T_.special_inc[k], e = raw >= 0 ? raw : ~raw;
+                dir[T_.edge_axis[e]] += T_.edge_area[e];
+                if (T_.edge_area[e] > best) {
+                    best = T_.edge_area[e];
+                    donor = special_of[raw >= 0 ? T_.edge_b[e] : T_.edge_a[e]];
+                }
+            }
+            bool weak = volume[a] < fm * h * h * h;
+            for (int ax = 0; ax < 3; ++ax) weak = weak || (dir[ax] > 0 && dir[ax] < fm * h * h);
+            if (!weak || donor < 0) continue;
+            int ra = root(s), rb = root(donor);
+            if (ra != rb) {
+                if (ra > rb) std::swap(ra, rb);
+                parent[rb] = ra;
+            }
+            seed[s] = 1;
+        }
+        std::vector<uint8_t> grouped(nsp, 0);
+        for (int s = 0; s < nsp; ++s) if (seed[s]) grouped[root(s)] = 1;
+        std::vector<int> group_of_root(nsp, -1);
+        std::vector<std::vector<int>> members;
+        for (int s = 0; s < nsp; ++s) {
+            const int r = root(s);
+            if (!grouped[r]) continue;
+            if (group_of_root[r] < 0) {
+                group_of_root[r] = static_cast<int>(members.size());
+                members.emplace_back();
+            }
+            T_.redist_group[s] = group_of_root[r];
+            members[group_of_root[r]].push_back(s);
+        }
+        for (const auto& g : members) {
+            double v = 0;
+            for (int s : g) v += volume[T_.special_dof[s]];
+            T_.redist_member.insert(T_.redist_member.end(), g.begin(), g.end());
+            T_.redist_off.push_back(static_cast<int>(T_.redist_member.size()));
+            T_.redist_volume.push_back(v);
+        }
+    }
     // solid volume of the body: every leaf cell's non-fluid part that lies inside the geometry
     double body = 0;
     for (int l = 0; l < L_; ++l) {
@@ -1361,6 +1425,14 @@ Topology merge_topologies(const std::vector<const Topology*>& parts) {
         T.patch_S.insert(T.patch_S.end(), P.patch_S.begin(), P.patch_S.end());
         T.patch_x.insert(T.patch_x.end(), P.patch_x.begin(), P.patch_x.end());
         T.patch_D.insert(T.patch_D.end(), P.patch_D.begin(), P.patch_D.end());
+        // redistribution groups
+        const int group0 = static_cast<int>(T.redist_volume.size());
+        for (int v : P.redist_group) T.redist_group.push_back(shift(v, group0));
+        if (T.redist_off.empty()) T.redist_off.push_back(0);
+        const int member0 = static_cast<int>(T.redist_member.size());
+        for (size_t g = 1; g < P.redist_off.size(); ++g) T.redist_off.push_back(P.redist_off[g] + member0);
+        for (int v : P.redist_member) T.redist_member.push_back(v + special0);
+        T.redist_volume.insert(T.redist_volume.end(), P.redist_volume.begin(), P.redist_volume.end());
         // multigrid: level 0 carries the parent map and the cell exponents
         const int pdepth = static_cast<int>(P.mg.size());
         for (int v : P.mg[0].parent) T.mg[0].parent.push_back(pdepth > 1 ? shift(v, cell0[1]) : -1);
@@ -1416,6 +1488,7 @@ Topology merge_topologies(const std::vector<const Topology*>& parts) {
     T.mg[0].n = static_cast<int>(dof0);
     T.reference_area = P0.reference_area;
     if (T.special_inc_off.empty()) T.special_inc_off.push_back(0);
+    if (T.redist_off.empty()) T.redist_off.push_back(0);
     for (size_t m = 1; m < depth; ++m) {
         if (T.mg[m].row_off.empty()) T.mg[m].row_off.push_back(0);
         if (T.mg[m].child_off.empty()) T.mg[m].child_off.push_back(0);

@@ -106,6 +106,12 @@ struct Topology {
     std::vector<double> patch_S, patch_x;  // 3 per patch
     std::vector<double> patch_D;           // 9 per patch: int_W (x - c) (x) n dA, row-major [r][c]
 
+    // momentum redistribution groups of small and directionally weak cut cells: group g's
+    // members are the special ranks redist_member[redist_off[g] .. redist_off[g+1]) (ascending)
+    std::vector<int> redist_group;         // per special cell: its group or -1
+    std::vector<int> redist_off, redist_member;
+    std::vector<double> redist_volume;     // per group: sum of its members' volumes (member order)
+
     // multigrid: mg[0] is the composite level (only parent/exponent are filled there)
     std::vector<MGLevel> mg;
 

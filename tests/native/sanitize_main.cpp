@@ -43,6 +43,7 @@ pcfd_grid_desc grid(double lo0, double lo1, double lo2, double hi0, double hi1, 
     d.refine_band[0] = band0;
     d.refine_band[1] = band1;
     d.min_volume_fraction = 0.1;
+    d.merge_volume_fraction = 0.25;
     d.subsamples = 4;
     d.outlet_xmax = 1;
     return d;
@@ -73,6 +74,10 @@ void run_oracle(const pcfd_topology* t, int steps, double cs, int smooth_wall) {
     auto patch_x = exported<double>(t, PCFD_ARR_PATCH_X);
     auto patch_D = exported<double>(t, PCFD_ARR_PATCH_D);
     auto centroid = exported<double>(t, PCFD_ARR_CELL_CENTROID);
+    auto rgroup = exported<int>(t, PCFD_ARR_REDIST_GROUP);
+    auto roff = exported<int>(t, PCFD_ARR_REDIST_OFF);
+    auto rmember = exported<int>(t, PCFD_ARR_REDIST_MEMBER);
+    auto rvolume = exported<double>(t, PCFD_ARR_REDIST_VOLUME);
     const int nl = static_cast<int>(info.mg_levels);
     std::vector<std::vector<int>> parent(nl), row_off(nl), col(nl);
     std::vector<std::vector<double>> coef(nl), diag(nl);
@@ -105,6 +110,9 @@ void run_oracle(const pcfd_topology* t, int steps, double cs, int smooth_wall) {
     ot.mg_levels = nl; ot.mg_n = mg_n.data(); ot.mg_parent = pp.data(); ot.mg_row_off = pr.data(); ot.mg_col = pc.data();
     ot.mg_coef = pcf.data(); ot.mg_diag = pd.data(); ot.coarse_inv = cinv.empty() ? nullptr : cinv.data();
     ot.centroid = centroid.data();
+    ot.ngroups = static_cast<int64_t>(rvolume.size());
+    ot.redist_group = rgroup.data(); ot.redist_off = roff.data(); ot.redist_member = rmember.data();
+    ot.redist_volume = rvolume.data();
     pcfd_options o{};
     o.rho = 1.0; o.nu = 1e-3; o.speed = 1.0; o.cfl = 0.5; o.rtol = 1e-8; o.max_iterations = 200;
     o.pre_smooth = 2; o.post_smooth = 2; o.check_interval = 2;

@@ -529,3 +529,62 @@ def test_compact_face_geometry(which, request):
     partial_face[eb[part]] = True
     emb = partial_vol | partial_face
     assert np.all(emb[ea[full_same]] | emb[eb[full_same]])
+
+
+# ---------------------------------------------------------------------------------------------
+# momentum redistribution groups (amr_advection.cu:118-149, 1108-1178, 1251-1293)
+# ---------------------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def wing3_pair():
+    """A 3-level wing with its redistribution groups, and the same topology without them."""
+    g, d = cases.naca_wing(digits=2412, alpha_deg=4.0, cells_x=40, levels=3, bands=(1.0, 0.25))
+    tg = L.Topology(g, d)
+    d.merge_volume_fraction = 0.0
+    t0 = L.Topology(g, d)
+    return tg, t0
+
+
+@pytest.mark.parametrize("update", ["impulse", "transport", "diffusion"])
+def test_redistribution_pools_each_groups_increment(wing3_pair, update, oracle_built):
+    """One momentum update with and without the groups: every other cell's increment is the same
+    bit for bit; every member of a group gets the same velocity increment, the sum of the
+    members' own momentum increments over the group's volume (momentum is conserved).  Planted:
+    each member's share divided by its own volume (defect 13)."""
+    tg, t0 = wing3_pair
+    cen, h, vol, act, rank = cells(tg)
+    roff, memb = tg.array("redist_off"), tg.array("redist_member")
+    assert len(roff) > 1, "the case has redistribution groups"
+    for key in ("mask", "special_dof", "edge_a", "edge_area", "special_volume"):
+        assert tg.array(key).tobytes() == t0.array(key).tobytes(), key
+    sd = tg.array("special_dof")
+    member = np.zeros(tg.ndof, bool)
+    member[sd[memb]] = True
+    rng = np.random.default_rng(7)
+    v = [rng.standard_normal(tg.ndof) * act for _ in range(3)]
+    p = rng.standard_normal(tg.ndof) * act
+    faces = [0.2 * rng.standard_normal(tg.ndof) for _ in range(3)] + [0.2 * rng.standard_normal(tg.info.compact_edges)]
+    zero = [np.zeros(tg.ndof)] * 3 + [np.zeros(tg.info.compact_edges)]
+    outcome = {}
+    for defect in (0, 13):
+        dv = []
+        for t in (tg, t0):
+            o = options(speed=0.0, nu=0.0 if update == "transport" else 1e-3)
+            orc = Oracle(t, o)
+            orc.plant(defect)
+            orc.set_state(*v, p)
+            if update == "impulse":
+                orc.impulse(1e-2)
+            else:
+                orc.set_faces(*(faces if update == "transport" else zero))
+                orc.advect(1e-3 if update == "transport" else 1e-2)
+            dv.append(np.stack([orc.state()[c] - v[c] for c in range(3)], axis=1))
+        dg, d0 = dv
+        ok = np.array_equal(dg[act & ~member], d0[act & ~member])
+        scale = np.max(np.abs(d0[act])) + 1e-300
+        for g in range(len(roff) - 1):
+            cells_g = sd[memb[roff[g]:roff[g + 1]]]
+            vg = tg.array("redist_volume")[g]
+            expect = (vol[cells_g, None] * d0[cells_g]).sum(axis=0) / vg
+            ok = ok and np.max(np.abs(dg[cells_g] - expect)) <= 1e-12 * scale
+        outcome[defect] = ok
+    assert outcome == {0: True, 13: False}, outcome

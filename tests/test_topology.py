@@ -219,3 +219,57 @@ def test_paraglider_arc_sweep_and_symmetry(glider):
     left, right = x[:, 1] < 0, x[:, 1] > 0
     assert np.abs(S[left]).sum() == pytest.approx(np.abs(S[right]).sum(), rel=1e-3)
     assert S[left, 1].sum() == pytest.approx(-S[right, 1].sum(), rel=1e-3)
+
+
+def test_redistribution_groups_follow_the_rule():
+    """The groups recomputed from the exported arrays: every active cut cell with a fluid
+    fraction below merge_volume_fraction, or an axis whose open area lies in (0, fraction h^2),
+    joins the neighbour across its largest compact edge (first in edge order); the groups are
+    the connected components of those links, ordered by their lowest cell, members ascending,
+    volumes the members' summed volumes."""
+    from paracfd_amd import _lib as L, cases
+    g, d = cases.naca_wing(digits=2412, alpha_deg=4.0, cells_x=40, levels=3, bands=(1.0, 0.25))
+    t = L.Topology(g, d)
+    fm = d.merge_volume_fraction
+    sd, sv = t.array("special_dof"), t.array("special_volume")
+    h = t.array("level_h")[t.array("brick_level")[sd // 512]]
+    ea, eb, ax, ar = t.array("edge_a"), t.array("edge_b"), t.array("edge_axis"), t.array("edge_area")
+    rank = {int(a): s for s, a in enumerate(sd)}
+    inc = [[] for _ in sd]
+    for e in range(len(ea)):
+        inc[rank[int(ea[e])]].append(e)
+        inc[rank[int(eb[e])]].append(e)
+    parent = list(range(len(sd)))
+
+    def root(s):
+        while parent[s] != s:
+            s = parent[s]
+        return s
+    seeds = []
+    for s in range(len(sd)):
+        if not sv[s] < h[s] ** 3:
+            continue
+        area = np.zeros(3)
+        best, donor = 0.0, -1
+        for e in sorted(inc[s]):
+            area[ax[e]] += ar[e]
+            if ar[e] > best:
+                best, donor = ar[e], rank[int(eb[e] if ea[e] == sd[s] else ea[e])]
+        if sv[s] < fm * h[s] ** 3 or np.any((area > 0) & (area < fm * h[s] ** 2)):
+            ra, rb = root(s), root(donor)
+            parent[max(ra, rb)] = min(ra, rb)
+            seeds.append(s)
+    grouped = {root(s) for s in seeds}
+    comps = {}
+    for s in range(len(sd)):
+        if root(s) in grouped:
+            comps.setdefault(root(s), []).append(s)
+    groups = sorted(comps.values(), key=lambda m: m[0])
+    assert len(groups) > 0
+    off, memb, vol, gid = t.array("redist_off"), t.array("redist_member"), t.array("redist_volume"), t.array("redist_group")
+    assert len(off) == len(groups) + 1
+    for k, m in enumerate(groups):
+        assert list(memb[off[k]:off[k + 1]]) == m
+        assert abs(vol[k] - sv[m].sum()) <= 1e-14 * vol[k]
+        assert all(gid[s] == k for s in m)
+    assert int((gid >= 0).sum()) == sum(len(m) for m in groups)
