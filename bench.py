@@ -10,9 +10,10 @@ gradients.  Three solvers run the same step sequence from the same freestream st
   * plain:   FP32 V-cycle MGPCG (the same arithmetic without the network)
   * plain_fp64_vcycle: FP64 V-cycle MGPCG (plain FP64-outer MGPCG, the reference's method)
 and an FP64-everywhere solver (fields, V-cycle, CG; it matches the CPU oracle to 1e-10 in the
-parity tests) gives the errors of the learned run's velocity, pressure and CL/CD.  A fifth run,
-nonorthogonal_fgmres, solves the full operator with the non-orthogonal correction (flexible
-GMRES, FP32 V-cycle; the reference's default semantics) and reports its throughput beside.
+parity tests) gives the errors of the learned run's velocity, pressure and CL/CD.  Two more
+runs solve the full operator with the non-orthogonal correction (FP32 V-cycle; the reference's
+default semantics) and report their throughput beside: nonorthogonal_fgmres (plain flexible
+GMRES) and nonorthogonal_gcro (GCRO with three recycled corrections, as the reference).
 A "step" is one solver step on the resident state plus the loads integral.  Prints one JSON
 line.  `roofline` is the hot kernel with the largest time per step, timed in-step: HIP events
 around each of its launches on the solver's stream during steps that follow the timed ones
@@ -168,7 +169,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0):
+def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0, gmres_recycle=3):
     """Initialise, pre-steps (with training for the learned solver), warmup, K timed steps."""
     import torch
     from paracfd_amd import ExternalAeroCore, _lib as L, default_options
@@ -176,7 +177,8 @@ def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0):
                           max_iterations=500, field_precision=field, vcycle_precision=vcycle, preconditioner=pc,
                           pre_smooth=2, post_smooth=2, check_interval=args.check_interval,
                           learned_bricks=args.learned_bricks, learned_capacity=args.collects * args.pairs,
-                          smooth_wall=1 if args.smooth_wall else 0, nonorthogonal=nonorthogonal)
+                          smooth_wall=1 if args.smooth_wall else 0, nonorthogonal=nonorthogonal,
+                          gmres_recycle=gmres_recycle)
     core = ExternalAeroCore(topo, opt)
     core.set_reference(topo.info.reference_area)
     st = core.initialize()
@@ -450,7 +452,10 @@ def main():
     learned = run(topo, spec, args, "learned", L.FP32, vc, L.PC_LEARNED)
     plain32 = run(topo, spec, args, "plain", L.FP32, vc, L.PC_MG)
     plain64 = run(topo, spec, args, "plain_fp64_vcycle", L.FP32, L.FP64, L.PC_MG)
-    nonorth = run(topo, spec, args, "nonorthogonal_fgmres", L.FP32, vc, L.PC_MG, nonorthogonal=1) if args.nonorthogonal else None
+    nonorth = run(topo, spec, args, "nonorthogonal_fgmres", L.FP32, vc, L.PC_MG, nonorthogonal=1,
+                  gmres_recycle=0) if args.nonorthogonal else None
+    gcro = run(topo, spec, args, "nonorthogonal_gcro", L.FP32, vc, L.PC_MG, nonorthogonal=1,
+               gmres_recycle=3) if args.nonorthogonal else None
     ref = run(topo, spec, args, "fp64_everywhere", L.FP64, L.FP64, L.PC_MG)
 
     # errors of the learned mixed-precision run against the FP64-everywhere run (same steps)
@@ -560,6 +565,7 @@ def main():
                    "launch_shapes": learned["tuning"],
                    "plain_fp32_vcycle": summary(plain32), "plain_fp64_vcycle": summary(plain64),
                    "nonorthogonal_fgmres": summary(nonorth) if nonorth else None,
+                   "nonorthogonal_gcro_recycle3": summary(gcro) if gcro else None,
                    "speedup_vs_plain_fp32_vcycle": learned["value"] / plain32["value"],
                    "speedup_vs_plain_fp64_vcycle": learned["value"] / plain64["value"],
                    "errors_vs_fp64": errors},

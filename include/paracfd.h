@@ -230,6 +230,12 @@ typedef struct pcfd_options {
                                  allocation beyond it fail with PCFD_ENOMEM before calling the
                                  allocator (reference: ExternalAeroExecutionOptions::
                                  gpu_memory_budget_bytes, external_aero_core.cpp:91-95); 0: no limit */
+    int gmres_recycle;        /* non-orthogonal solve: GCRO recycling (amr_pressure.cu:1390-1698)
+                                 keeps this many realised corrections U and their orthonormal
+                                 images C = A U across cycles and projections (3 is the reference's
+                                 capacity, amr_pressure.h:396; at most 8).  Default 0, plain flexible
+                                 GMRES: with this V-cycle the recycled space leaves the iteration
+                                 counts unchanged and only adds vector traffic */
 } pcfd_options;
 
 typedef struct pcfd_step_stats {

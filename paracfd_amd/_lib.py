@@ -71,7 +71,8 @@ class Options(C.Structure):
                 ("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("check_interval", C.c_int),
                 ("smoother_weight", C.c_double), ("learned_bricks", C.c_int), ("learned_capacity", C.c_int),
                 ("smooth_wall", C.c_int), ("smoother_weight2", C.c_double),
-                ("nonorthogonal", C.c_int), ("gmres_restart", C.c_int), ("memory_budget_bytes", C.c_double)]
+                ("nonorthogonal", C.c_int), ("gmres_restart", C.c_int), ("memory_budget_bytes", C.c_double),
+                ("gmres_recycle", C.c_int)]
 
 
 NN_C = 8

@@ -157,6 +157,50 @@ __global__ __launch_bounds__(256) void k_gs_update(int64_t n, const double* __re
     if (threadIdx.x == 0) partials[blockIdx.x] = ss;
 }
 
+// GCRO recycle update: w -= sum_v a[v] C[v] and, when z is given, z += zsign sum_v a[v] U[v]
+// (the same coefficients keep A z = w when A U = C); partials[block] = that block's sum of the
+// new w^2
+__global__ __launch_bounds__(256) void k_recycle_update(int64_t n, const double* __restrict__ Cb,
+                                                        const double* __restrict__ Ub, int64_t ld, int nv,
+                                                        const double* __restrict__ a, double* __restrict__ w,
+                                                        double* __restrict__ z, double zsign,
+                                                        const uint8_t* __restrict__ mask, double* __restrict__ partials) {
+    double ss = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
+        if (!(mask[i] & kActive)) {
+            w[i] = 0.0;
+            if (z) z[i] = 0.0;
+            continue;
+        }
+        double s = w[i];
+        for (int v = 0; v < nv; ++v) s -= a[v] * Cb[v * ld + i];
+        w[i] = s;
+        ss += s * s;
+        if (z) {
+            double t = 0;
+            for (int v = 0; v < nv; ++v) t += a[v] * Ub[v * ld + i];
+            z[i] += zsign * t;
+        }
+    }
+    ss = block_reduce_sum<256>(ss);
+    if (threadIdx.x == 0) partials[blockIdx.x] = ss;
+}
+
+// y += a x on active cells; partials[block] = that block's sum of the new y^2
+__global__ __launch_bounds__(256) void k_axpy_norm(int64_t n, double* __restrict__ y, double a,
+                                                   const double* __restrict__ x, const uint8_t* __restrict__ mask,
+                                                   double* __restrict__ partials) {
+    double ss = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
+        if (!(mask[i] & kActive)) continue;
+        const double s = y[i] + a * x[i];
+        y[i] = s;
+        ss += s * s;
+    }
+    ss = block_reduce_sum<256>(ss);
+    if (threadIdx.x == 0) partials[blockIdx.x] = ss;
+}
+
 // dst = a * src on active cells, 0 elsewhere
 // (src may be dst)
 __global__ __launch_bounds__(256) void k_scale(int64_t n, const double* src, double a, double* dst,

@@ -400,7 +400,10 @@ private:
     void setup_smooth_wall(const Topology& T);
     // non-orthogonal operator and its flexible GMRES outer solve
     void setup_nonorth(const Topology& T);
-    void full_residual(double* r);
+    void full_residual(double* r, const double* x = nullptr);
+    void apply_full(const double* x, double* y);
+    double partial_norm2();
+    bool recycle_project(double* w, double* z, double zsign, int passes = 2);
     void mdot(const double* Vb, int nv, const double* w, double* host_out);
     void fgmres_solve();
     void fgmres_direction(int j);
@@ -411,6 +414,12 @@ private:
     NonOrthView NV_{};
     double *gm_V_ = nullptr, *gm_Z_ = nullptr, *gm_part_ = nullptr, *gm_dev_ = nullptr;
     int gm_m_ = 0, gm_its_ = 0;
+    // GCRO recycling: up to gm_cap_ realised corrections U and their orthonormal images C = A U
+    // (contiguous bases), kept across cycles and projections
+    double *gm_U_ = nullptr, *gm_C_ = nullptr, *gm_coef_ = nullptr;
+    int gm_cap_ = 0, gm_k_ = 0;
+    int64_t gm_rejections_ = 0;
+    std::vector<double> gm_a_;
     // Validation of a freshly trained network: the next 2 kTrial projections alternate learned
     // and plain MGPCG (learned first), so both see the same stage of the flow; if the learned
     // ones take longer in total, the network is switched off.
@@ -1373,6 +1382,13 @@ void SolverImpl<R>::setup_nonorth(const Topology& T) {
     HIP_CHECK(hipMemset(gm_Z_, 0, static_cast<size_t>(gm_m_) * n_ * sizeof(double)));
     gm_part_ = alloc<double>(static_cast<int64_t>(gm_m_ + 2) * kGmresBlocks);
     gm_dev_ = alloc<double>(gm_m_ + 2);
+    gm_cap_ = std::min(std::max(opt_.gmres_recycle, 0), 8);
+    if (gm_cap_ > 0) {
+        gm_U_ = alloc<double>(static_cast<int64_t>(gm_cap_) * n_);
+        gm_C_ = alloc<double>(static_cast<int64_t>(gm_cap_) * n_);
+        gm_coef_ = alloc<double>(gm_cap_);
+        gm_a_.assign(gm_cap_, 0.0);
+    }
     gm_graph_.assign(gm_m_, nullptr);
     gm_exec_.assign(gm_m_, nullptr);
 }
@@ -1430,12 +1446,45 @@ void SolverImpl<R>::fgmres_direction(int j) {
 
 // r = rhs - A p with the full operator
 template <class R>
-void SolverImpl<R>::full_residual(double* r) {
-    LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r, partials_, S_);
+void SolverImpl<R>::full_residual(double* r, const double* x) {
+    if (!x) x = p_;
+    LAUNCH(k_cg_residual, nb_, 512, M_, x, rhs_, r, partials_, S_);
     if (NV_.nf > 0) {
-        if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, p_);
-        LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, p_, r, -1.0);
+        if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, x);
+        LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, x, r, -1.0);
     }
+}
+// y = A x with the full (non-orthogonal) operator
+template <class R>
+void SolverImpl<R>::apply_full(const double* x, double* y) {
+    LAUNCH(k_cg_dir_apply, nb_, 512, M_, x, x, dbuf_[0], y, partials_, S_, 1);
+    if (NV_.nf > 0) {
+        if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, x);
+        LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, x, y, 1.0);
+    }
+}
+// the sum of the per-block squares a GMRES vector kernel left in gm_part_
+template <class R>
+double SolverImpl<R>::partial_norm2() {
+    LAUNCH(k_mdot_final, 1, 256, kGmresBlocks, gm_part_, gm_dev_);
+    double v = 0;
+    HIP_CHECK(hipMemcpyAsync(&v, gm_dev_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    return v;
+}
+// w -= C a, z += zsign U a with a = C^T w, `passes` times (classical Gram-Schmidt, twice for a
+// re-orthogonalisation; the reference runs modified Gram-Schmidt twice, amr_pressure.cu:1462-1471,
+// 1548-1557); false when a coefficient is not finite
+template <class R>
+bool SolverImpl<R>::recycle_project(double* w, double* z, double zsign, int passes) {
+    for (int pass = 0; pass < passes; ++pass) {
+        mdot(gm_C_, gm_k_, w, gm_a_.data());
+        for (int v = 0; v < gm_k_; ++v)
+            if (!std::isfinite(gm_a_[v])) return false;
+        HIP_CHECK(hipMemcpyAsync(gm_coef_, gm_a_.data(), gm_k_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+        LAUNCH(k_recycle_update, kGmresBlocks, 256, n_, gm_C_, gm_U_, n_, gm_k_, gm_coef_, w, z, zsign, M_.mask, gm_part_);
+    }
+    return true;
 }
 
 // host_out[v] = V[v] . w over active cells, v < nv (consecutive vectors of length n_)
@@ -1448,11 +1497,16 @@ void SolverImpl<R>::mdot(const double* Vb, int nv, const double* w, double* host
     HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
-// Flexible GMRES(m), right-preconditioned by the V-cycle of the orthogonal operator (which
-// may differ between applications: FP32/BF16 storage), one-pass classical Gram-Schmidt with a
-// second pass only near breakdown (||w|| after the projection below 1e-6 of before), Givens
-// rotations on the host.  Converged when the true residual ||rhs - A p|| <= rtol ||rhs||,
-// checked at the end of every cycle (the reference's criterion, amr_pressure.cu:1430-1440).
+// Flexible GCRO / GMRES(m) (amr_pressure.cu:1390-1698), right-preconditioned by the V-cycle of
+// the orthogonal operator (which may differ between applications: FP32/BF16 storage).  Up to
+// gmres_recycle realised corrections U with orthonormal images C = A U persist across cycles
+// and projections: each cycle first minimises over C (validated against the true residual),
+// every Arnoldi image is projected out of C (and its direction out of U with the same
+// coefficients), and each cycle's correction joins U/C when its image has a part outside C.
+// One-pass classical Gram-Schmidt against the Krylov basis with a second pass only near
+// breakdown (||w|| after the projection below 1e-6 of before), Givens rotations on the host.
+// Converged when the true residual ||rhs - A p|| <= rtol ||rhs||, checked at the end of every
+// cycle (the reference's criterion, amr_pressure.cu:1430-1440).
 template <class R>
 void SolverImpl<R>::fgmres_solve() {
     const int m = gm_m_;
@@ -1476,6 +1530,42 @@ void SolverImpl<R>::fgmres_solve() {
     bool converged = res <= opt_.rtol * bnorm;
     std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 1);
     while (!converged && gm_its_ < opt_.max_iterations && std::isfinite(res)) {
+        if (gm_k_ > 0) {
+            // minimise over the recycled images first: x_c = x + U C^T r, r_c = r - C C^T r,
+            // kept when the true residual of x_c is below the one before and r_c is within a
+            // quarter of the tolerance of it; otherwise the recycle space is dropped
+            // (amr_pressure.cu:1452-1497)
+            double* xc = Z;
+            double* rc = Z + n;
+            LAUNCH(k_scale, G, 256, n, p_, 1.0, xc, M_.mask);
+            LAUNCH(k_scale, G, 256, n, r, 1.0, rc, M_.mask);
+            const double before2 = rr;
+            const bool valid = recycle_project(rc, xc, 1.0);
+            double true2 = before2, gap2 = 0;
+            if (valid) {
+                full_residual(r, xc);
+                mdot(r, 1, r, &true2);
+                LAUNCH(k_axpy_norm, G, 256, n, rc, -1.0, r, M_.mask, gm_part_);
+                gap2 = partial_norm2();
+            }
+            const bool accurate = valid && std::isfinite(gap2) && std::sqrt(gap2) <= 0.25 * opt_.rtol * bnorm &&
+                                  true2 < before2;
+            if (accurate) {
+                LAUNCH(k_scale, G, 256, n, xc, 1.0, p_, M_.mask);
+                rr = true2;
+            } else {
+                gm_k_ = 0;
+                ++gm_rejections_;
+                if (valid) full_residual(r);  // the residual of the unchanged iterate
+                rr = before2;
+            }
+            res = std::sqrt(rr);
+            if (!std::isfinite(res)) break;
+            if (res <= opt_.rtol * bnorm) {
+                converged = true;
+                break;
+            }
+        }
         LAUNCH(k_scale, G, 256, n, r, 1.0 / res, V, M_.mask);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = res;
@@ -1483,6 +1573,10 @@ void SolverImpl<R>::fgmres_solve() {
         for (; j < m && gm_its_ < opt_.max_iterations; ++j) {
             double* w = V + (j + 1) * n;
             fgmres_direction(j);
+            // the direction's image leaves the recycled images' span, z the same way in U
+            // (amr_pressure.cu:1544-1557); one pass: the Krylov projection below re-orthogonalises
+            // near breakdown, and the cycle's end checks the true residual
+            if (gm_k_ > 0 && !recycle_project(w, Z + j * n, -1.0, 1)) break;
             // classical Gram-Schmidt; w = V[j+1] follows V[j], so one pass gives V[i].w and w.w.
             // With a V-cycle close to A^-1, A M^-1 v lies nearly in the span and the projection
             // always removes most of w, so Kahan's 0.7 test would re-orthogonalise every time
@@ -1490,8 +1584,8 @@ void SolverImpl<R>::fgmres_solve() {
             // the true residual at the end of every cycle keeps the stopping test honest.
             mdot(V, j + 2, w, h.data());
             double before = std::sqrt(std::max(h[j + 1], 0.0)), hn = 0;
-            auto project_out = [&](const double* coef) {
-                HIP_CHECK(hipMemcpyAsync(gm_dev_, coef, (j + 1) * sizeof(double), hipMemcpyHostToDevice, stream_));
+            // the coefficients are still in gm_dev_, where the mdot just before left them
+            auto project_out = [&]() {
                 LAUNCH(k_gs_update, G, 256, n, V, n, j + 1, gm_dev_, w, M_.mask, gm_part_);
                 LAUNCH(k_mdot_final, 1, 256, kGmresBlocks, gm_part_, gm_dev_);
                 double ww = 0;
@@ -1499,11 +1593,11 @@ void SolverImpl<R>::fgmres_solve() {
                 HIP_CHECK(hipStreamSynchronize(stream_));
                 return std::sqrt(ww);
             };
-            hn = project_out(h.data());
+            hn = project_out();
             std::fill(h2.begin(), h2.end(), 0.0);
             if (hn < 1e-6 * before) {
                 mdot(V, j + 1, w, h2.data());
-                hn = project_out(h2.data());
+                hn = project_out();
             }
             for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
             H[(j + 1) * m + j] = hn;
@@ -1532,11 +1626,46 @@ void SolverImpl<R>::fgmres_solve() {
             y[i] = H[i * m + i] != 0.0 ? s / H[i * m + i] : 0.0;
         }
         HIP_CHECK(hipMemcpyAsync(gm_dev_, y.data(), j * sizeof(double), hipMemcpyHostToDevice, stream_));
-        LAUNCH(k_combine, G, 256, n, Z, n, j, gm_dev_, p_, M_.mask);
+        double* corr = V + n;  // the cycle's correction Z y (the basis is spent)
+        if (gm_cap_ > 0 && j > 0) {
+            LAUNCH(k_scale, G, 256, n, corr, 0.0, corr, M_.mask);
+            LAUNCH(k_combine, G, 256, n, Z, n, j, gm_dev_, corr, M_.mask);
+            LAUNCH(k_axpy_norm, G, 256, n, p_, 1.0, corr, M_.mask, gm_part_);
+        } else if (j > 0) {
+            LAUNCH(k_combine, G, 256, n, Z, n, j, gm_dev_, p_, M_.mask);
+        }
         full_residual(r);
         mdot(r, 1, r, &rr);
         res = std::sqrt(rr);
         converged = res <= opt_.rtol * bnorm;
+        if (gm_cap_ > 0 && j > 0) {
+            // keep the correction as a recycle pair when its image has a part outside the
+            // recycled span (normalised; the oldest pair gives way when full;
+            // amr_pressure.cu:1632-1669)
+            double* wc = V + 2 * n;
+            apply_full(corr, wc);
+            double before2 = 0;
+            mdot(wc, 1, wc, &before2);
+            bool ok = before2 > 0 && std::isfinite(before2);
+            if (ok && gm_k_ > 0) ok = recycle_project(wc, corr, -1.0);
+            double after2 = before2;
+            if (ok && gm_k_ > 0) after2 = partial_norm2();
+            const double after = std::sqrt(std::max(after2, 0.0));
+            if (ok && std::isfinite(after) && after > 1e-12 * std::sqrt(before2)) {
+                if (gm_k_ == gm_cap_) {
+                    for (int v = 1; v < gm_cap_; ++v) {
+                        HIP_CHECK(hipMemcpyAsync(gm_U_ + (v - 1) * n, gm_U_ + v * n, n * sizeof(double),
+                                                 hipMemcpyDeviceToDevice, stream_));
+                        HIP_CHECK(hipMemcpyAsync(gm_C_ + (v - 1) * n, gm_C_ + v * n, n * sizeof(double),
+                                                 hipMemcpyDeviceToDevice, stream_));
+                    }
+                    --gm_k_;
+                }
+                LAUNCH(k_scale, G, 256, n, corr, 1.0 / after, gm_U_ + gm_k_ * n, M_.mask);
+                LAUNCH(k_scale, G, 256, n, wc, 1.0 / after, gm_C_ + gm_k_ * n, M_.mask);
+                ++gm_k_;
+            }
+        }
     }
     LAUNCH(k_no_status, 1, 64, S_, gm_its_, converged ? 1 : 0, res / bnorm);
 }
@@ -1545,7 +1674,10 @@ template <class R>
 void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
     finalize(FIN_BNORM, nb_);
-    if (!warm) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    if (!warm) {
+        HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+        gm_k_ = 0;  // a cold start also forgets the recycled corrections
+    }
     const bool learned = nn_ && nn_on_ && !trial_plain();
     set_nn_off(!learned);
     if (nonorth_) {
@@ -2461,6 +2593,7 @@ void pcfd_default_options(pcfd_options* o) {
     o->smoother_weight2 = 1.0 / (1.2 - 0.8 / std::sqrt(2.0));
     o->nonorthogonal = 0;
     o->gmres_restart = 24;
+    o->gmres_recycle = 0;
     o->learned_bricks = 0;
     o->learned_capacity = 16;
 }
