@@ -247,7 +247,7 @@ def run_sweep(args):
     from paracfd_amd import ExternalAeroCore, _lib as L, cases, default_options
 
     t0 = time.perf_counter()
-    parts, spec = cases.build_sweep(args.case)
+    parts, spec = cases.build_sweep(args.case, device=None if args.host_topology else 0)
     merged = L.Topology.merge(parts)
     n_total = sum(p.n_active for p in parts)
     log(f"sweep {args.case}: {len(parts)} instances, {n_total} active cells "

@@ -125,16 +125,16 @@ SWEEPS = {
 }
 
 
-def build_sweep(name, **overrides):
-    """Topologies of every instance of a sweep (built one after another; each build is
-    multithreaded) and the sweep's parameters."""
+def build_sweep(name, device=None, **overrides):
+    """Topologies of every instance of a sweep (built one after another, on the host threads or
+    with the per-cell stages on HIP device `device`) and the sweep's parameters."""
     spec = dict(SWEEPS[name])
     args = dict(spec["args"])
     args.update(overrides)
     parts = []
     for a in spec["alphas"]:
         g, d = naca_wing(alpha_deg=a, **args)
-        parts.append(L.Topology(g, d))
+        parts.append(L.Topology(g, d, device=device))
     return parts, spec
 
 
