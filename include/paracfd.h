@@ -452,17 +452,19 @@ pcfd_status pcfd_probe_stop(pcfd_solver* solver, int64_t* launches, double* tota
 
 /* ---- Learned preconditioner (PCFD_PC_LEARNED) --------------------------------------------
  * No counterpart in the reference, whose projection runs plain (GCRO / PCG) Krylov with an
- * algebraic V-cycle (amr_pressure.cu:1103-1353).  A 4-layer 3^3 conv net per brick (8
- * channels, BF16 MFMA, FP32 accumulate) corrects the V-cycle inside flexible PCG:
- * z = M_mg(r) + (s_b/h_b) N(r'/s_b) with r' = r - A M_mg(r) and s_b the brick's RMS of r'.
+ * algebraic V-cycle (amr_pressure.cu:1103-1353).  A 4-layer 3^3 conv net per brick (16
+ * channels, one weight set per AMR level, BF16 MFMA, FP32 accumulate; its input reads a
+ * one-cell halo from the 26 neighbouring bricks) corrects the V-cycle inside flexible PCG:
+ * z = M_mg(r) + (s_b/h_b) N_level(r'/s_b) with r' = r - A M_mg(r) and s_b the brick's RMS of r'.
  * It is trained to minimise the energy norm of the error it leaves, dz^T A dz - 2 dz^T r'.
  * A projection whose learned solve stalls or fails restarts from its iterate with plain
  * MGPCG (pcfd_step_stats.fallback_used).  The network starts disabled; it is enabled by
  * pcfd_learned_train or pcfd_learned_set_weights.  After pcfd_learned_train, the next three
  * steps validate it: if their mean projection time exceeds that of the last (up to four)
  * projections without the network, the network is switched off (learned_rejected). */
-#define PCFD_NN_WEIGHTS 6912  /* 4 layers x 27 taps x 8 cout x 8 cin, [layer][tap][cout][cin] */
-#define PCFD_NN_BIASES 32     /* 4 layers x 8 */
+#define PCFD_NN_WEIGHTS 110592  /* 4 sets x 4 layers x 27 taps x 16 cout x 16 cin,
+                                   [set][layer][tap][cout][cin]; set = AMR level (levels >= 3: set 3) */
+#define PCFD_NN_BIASES 256       /* 4 sets x 4 layers x 16 */
 
 typedef struct pcfd_learned_stats {
     int pairs;                /* stored training pairs */
@@ -491,8 +493,9 @@ pcfd_status pcfd_learned_set_weights(pcfd_solver* solver, const float* weights, 
 /* 1 enables, 0 disables the network (the solver then runs plain MGPCG in flexible PCG). */
 pcfd_status pcfd_learned_enable(pcfd_solver* solver, int on);
 /* Verification hooks: raw network output y[dofs] and brick scales s[bricks] for an input
- * residual r (every brick); loss and gradient (weights then biases) of one post-V-cycle
- * residual over the network's bricks with the training kernels, without an update. */
+ * residual r (every brick, the halo from every neighbour); loss and gradient (weights then
+ * biases) of one post-V-cycle residual over the network's bricks with the training kernels
+ * (input read as FP32), without an update. */
 pcfd_status pcfd_learned_forward(pcfd_solver* solver, const double* r, float* y, float* s);
 pcfd_status pcfd_learned_gradient(pcfd_solver* solver, const double* residual, double* loss, float* grad_weights,
                                   float* grad_biases);

@@ -75,9 +75,11 @@ class Options(C.Structure):
                 ("gmres_recycle", C.c_int)]
 
 
-NN_C = 8
-NN_WEIGHTS = 4 * 27 * NN_C * NN_C  # [layer][tap][cout][cin]
-NN_BIASES = 4 * NN_C
+NN_C = 16
+NN_SETS = 4      # one weight set per AMR level (levels >= 3 share the last)
+NN_LAYERS = 4
+NN_WEIGHTS = NN_SETS * NN_LAYERS * 27 * NN_C * NN_C  # [set][layer][tap][cout][cin]
+NN_BIASES = NN_SETS * NN_LAYERS * NN_C
 
 
 class LearnedStats(C.Structure):

@@ -209,17 +209,18 @@ class ExternalAeroCore:
         return st.as_dict()
 
     def learned_weights(self):
-        """FP32 master weights [4, 27, 8, 8] ([layer][tap][cout][cin]) and biases [4, 8]."""
+        """FP32 master weights [sets, layers, 27, 16, 16] ([set][layer][tap][cout][cin]) and biases
+        [sets, layers, 16]; set = AMR level of the brick (levels beyond the last set share it)."""
         w = np.empty(L.NN_WEIGHTS, dtype=np.float32)
         b = np.empty(L.NN_BIASES, dtype=np.float32)
         L.check(self._lib.pcfd_learned_get_weights(self.handle, L.fptr(w), L.fptr(b)))
-        return w.reshape(4, 27, L.NN_C, L.NN_C), b.reshape(4, L.NN_C)
+        return (w.reshape(L.NN_SETS, L.NN_LAYERS, 27, L.NN_C, L.NN_C), b.reshape(L.NN_SETS, L.NN_LAYERS, L.NN_C))
 
     def set_learned_weights(self, w, b):
         w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
         b = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
         if w.size != L.NN_WEIGHTS or b.size != L.NN_BIASES:
-            raise ValueError("weights must hold 4*27*8*8 values and biases 4*8")
+            raise ValueError(f"weights must hold {L.NN_WEIGHTS} values and biases {L.NN_BIASES}")
         L.check(self._lib.pcfd_learned_set_weights(self.handle, L.fptr(w), L.fptr(b)))
 
     def learned_enable(self, on=True):
@@ -234,10 +235,12 @@ class ExternalAeroCore:
         return y, s
 
     def learned_gradient(self, residual):
-        """(loss, dW [4,27,8,8], db [4,8]) of one post-V-cycle residual with the training kernels."""
+        """(loss, dW [sets,4,27,16,16], db [sets,4,16]) of one post-V-cycle residual (read as FP32)
+        on the network's bricks with the training kernels."""
         r = np.ascontiguousarray(residual, dtype=np.float64)
         loss = C.c_double()
         gw = np.empty(L.NN_WEIGHTS, dtype=np.float32)
         gb = np.empty(L.NN_BIASES, dtype=np.float32)
         L.check(self._lib.pcfd_learned_gradient(self.handle, L.dptr(r), C.byref(loss), L.fptr(gw), L.fptr(gb)))
-        return loss.value, gw.reshape(4, 27, L.NN_C, L.NN_C), gb.reshape(4, L.NN_C)
+        return (loss.value, gw.reshape(L.NN_SETS, L.NN_LAYERS, 27, L.NN_C, L.NN_C),
+                gb.reshape(L.NN_SETS, L.NN_LAYERS, L.NN_C))

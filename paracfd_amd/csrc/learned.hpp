@@ -1,9 +1,9 @@
-// Learned preconditioner: a 4-layer 3x3x3 convolutional network (8 channels, BF16 MFMA,
-// FP32 accumulate) applied per 8^3 brick (every AMR level, cell size h_b), correcting the
-// geometric V-cycle inside flexible PCG:
+// Learned preconditioner: a 4-layer 3x3x3 convolutional network (16 channels, BF16 MFMA,
+// FP32 accumulate) applied per 8^3 brick, one weight set per AMR level (cell size h_b),
+// correcting the geometric V-cycle inside flexible PCG:
 //     z  = M_mg(r)                      (V-cycle)
-//     r' = r - A z                      (what the V-cycle left, computed in the kernel)
-//     z += (s_b / h_b) N(r' / s_b, geometry)     on the network's bricks
+//     r' = r - A z                      (what the V-cycle left; k_nn_resid)
+//     z += (s_b / h_b) N_level(r' / s_b, geometry)     on the network's bricks
 // s_b is the brick's RMS of r' over active cells (the correction is positively homogeneous
 // in r'), and the composite operator scales like h.  The network is trained on the device to
 // minimise the energy norm of the error left after the correction,
@@ -12,15 +12,22 @@
 // gradient 2 (A dz - r') needs one operator apply, no exact solution.  Plain MGPCG is the
 // fallback.
 //
-// Input features per active cell (8 channels): r'/s_b, 1 (active), special-cell flag, the
-// parities of i, j, k as +-1, the fraction of the six structured faces that are open, and 0.
-// Inactive cells carry zeros.  Layers: 8 -> 8 channels, zero padding at the brick boundary,
-// ReLU after layers 0..2, outputs of inactive cells forced to zero; the last layer's channel
-// 0 is the network output.  Weights [layer][tap][cout][cin] BF16 (and a tap-flipped,
-// cin/cout-transposed copy for the input gradient), biases FP32, FP32 master weights for
-// Adam.  All network kernels run one 512-lane workgroup (8 waves) per brick, thread q owning
-// cell q.  An activation tile is 10^3 positions x 8 channels x 2 bytes = 16 KB, so four
-// bricks share a CU.
+// Brick set: the network's bricks, position p -> brick list[p] (no list: p = brick), with a
+// neighbour table nbtab[27 p + d] = position of the same-level brick at offset d (reached face
+// by face in x, y, z order) or -1 (no such brick, or not in the set).  r' lives in a compact
+// FP32 store of 512 values per position.
+//
+// Input features per tile position (16 channels; the brick's 8^3 cells and the one-cell halo
+// from its 26 neighbours): r'/s_b, 1 (active), special-cell flag, the parities of the cell's
+// i, j, k in its brick as +-1, the fraction of its six structured faces that are open, the six
+// structured-face bits, 0, 0, 0; inactive positions and positions outside the set carry
+// zeros.  Layers: 16 -> 16 channels; layer 0 reads the halo, the hidden layers pad with zeros;
+// ReLU after layers 0..2, outputs of inactive cells forced to zero; the last layer's channel 0
+// is the network output.  Weights [set][layer][tap][cout][cin] BF16 (and a tap-flipped,
+// cin/cout-transposed copy for the input gradient), biases [set][layer][cout] FP32, FP32
+// master weights for Adam.  All network kernels run one 512-lane workgroup (8 waves) per
+// brick, thread q owning cell q.  An activation tile is 10^3 positions x 16 channels x 2
+// bytes = 32 KB; with one layer's weights (13.5 KB) two bricks share a CU.
 //
 // MFMA v_mfma_f32_16x16x32_bf16 lane layout (checked on gfx950): A[m][k] lane l holds
 // m = l%16, k = 8(l/16)+j; B[k][n]: n = l%16, same k; D[m][n]: m = 4(l/16)+i, n = l%16.
@@ -32,14 +39,19 @@
 namespace pcfd {
 
 constexpr int kNNLayers = 4;
-constexpr int kNNC = 8;
+constexpr int kNNC = 16;
 constexpr int kNNTaps = 27;
-constexpr int kNNWLayer = kNNTaps * kNNC * kNNC;  // weights per layer
-constexpr int kNNW = kNNLayers * kNNWLayer;        // all weights
-constexpr int kNNB = kNNLayers * kNNC;             // all biases
+constexpr int kNNSets = 4;                         // weight sets: AMR levels 0, 1, 2, >= 3
+constexpr int kNNWLayer = kNNTaps * kNNC * kNNC;   // weights per layer
+constexpr int kNNWSet = kNNLayers * kNNWLayer;     // weights per set
+constexpr int kNNW = kNNSets * kNNWSet;            // all weights
+constexpr int kNNBSet = kNNLayers * kNNC;          // biases per set
+constexpr int kNNB = kNNSets * kNNBSet;            // all biases
 constexpr int kNNTile = kTileN * kNNC;             // bf16 values in one activation tile
 constexpr int kNNStore = 3 * 512 * kNNC;           // per-brick training scratch (inputs of layers 1..3)
 constexpr int kNNMT = 4;                           // 16-cell M-tiles per wave (32 tiles, 8 waves)
+constexpr int kNNKSteps = (kNNTaps * kNNC + 31) / 32;  // 32-wide K steps of one convolution (two taps each)
+constexpr int kNNWUnits = kNNWLayer / 8;           // 16-byte units of one layer's weights
 
 typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float nn_floatx4 __attribute__((ext_vector_type(4)));
@@ -55,54 +67,68 @@ __device__ __forceinline__ float nn_bf2f(uint16_t b) { return __uint_as_float(st
 // tap t = (dz+1)*9 + (dy+1)*3 + (dx+1); offset in the 10^3 tile; offset(26 - t) = -offset(t)
 __device__ __forceinline__ int nn_tap_offset(int t) { return (t / 9 - 1) * 100 + ((t / 3) % 3 - 1) * 10 + (t % 3 - 1); }
 __device__ __forceinline__ int nn_pos(int cell) { return tile_index(cell & 7, (cell >> 3) & 7, cell >> 6); }
+__device__ __forceinline__ bool nn_halo_pos(int p) {
+    const int ti = p % 10, tj = (p / 10) % 10, tk = p / 100;
+    return ti == 0 || ti == 9 || tj == 0 || tj == 9 || tk == 0 || tk == 9;
+}
 
 struct NNView {
-    const uint16_t* w;    // [L][27][cout][cin]
-    const uint16_t* wt;   // [L][27][cin][cout], taps flipped: wt[l][t][i][o] = w[l][26-t][o][i]
-    const float* bias;    // [L][8]
+    const uint16_t* w;    // [set][L][27][cout][cin]
+    const uint16_t* wt;   // [set][L][27][cin][cout], taps flipped: wt[s][l][t][i][o] = w[s][l][26-t][o][i]
+    const float* bias;    // [set][L][16]
 };
 
-// LDS of the network kernels (one brick): two activation tiles, one layer's weights, flags
+// the network's brick set
+struct NNSetView {
+    const int* list;      // position -> brick (null: position = brick)
+    const int* nbtab;     // 27 per position: neighbour's position or -1
+};
+
+__device__ __forceinline__ int nn_set(const MeshView& M, int b) {
+    const int l = M.meta[b].level;
+    return l < kNNSets ? l : kNNSets - 1;
+}
+
+// LDS of the network kernels (one brick): two activation tiles, one layer's weights, flags.
+// The last layer's FP32 output (512 values) is written over T0.
 struct NNShared {
     uint16_t T0[kNNTile];
     uint16_t T1[kNNTile];
     uint16_t WL[kNNWLayer];
-    float y[512];
     float red[16];
     uint8_t act[512];
 };
 
-// One 3x3x3 convolution 8 -> 8 channels over the brick: 32 M-tiles of 16 cells, 4 per wave.
-// K = 27 taps x 8 channels, four taps per 32-wide MFMA step (7 steps, the last tap slot
-// zero); N = 16 with output channels 8..15 zero.  Output to a tile (BF16) or, when
-// out_tile == nullptr, channel 0 into yout (FP32, 512).  wl: [27][cout][cin] in LDS.
+// One 3x3x3 convolution 16 -> 16 channels over the brick: 32 M-tiles of 16 cells, 4 per wave.
+// K = 27 taps x 16 channels, two taps per 32-wide MFMA step (14 steps, the last half zero);
+// N = 16 output channels.  Output to a tile (BF16) or, when out_tile == nullptr, channel 0
+// into yout (FP32, 512).  wl: [27][cout][cin] in LDS.
 __device__ __forceinline__ void nn_conv(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wl,
                                         const float* bias, bool relu, const uint8_t* act,
                                         uint16_t* __restrict__ out_tile, float* __restrict__ yout) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int g = lane >> 4, n = lane & 15;
+    const int g = lane >> 4, n = lane & 15, half = (g & 1) * 8;
     nn_floatx4 acc[kNNMT];
     int pos[kNNMT];
     for (int mt = 0; mt < kNNMT; ++mt) {
         acc[mt] = nn_floatx4{0.f, 0.f, 0.f, 0.f};
         pos[mt] = nn_pos((wave * kNNMT + mt) * 16 + n);
     }
-    for (int s = 0; s < 7; ++s) {
-        const int t = 4 * s + g;
+    for (int s = 0; s < kNNKSteps; ++s) {
+        const int t = 2 * s + (g >> 1);
         const bool valid = t < kNNTaps;
         nn_bf16x8 bfrag = {};
         int off = 0;
         if (valid) {
-            if (n < kNNC) bfrag = *reinterpret_cast<const nn_bf16x8*>(wl + (t * kNNC + n) * kNNC);
+            bfrag = *reinterpret_cast<const nn_bf16x8*>(wl + (t * kNNC + n) * kNNC + half);
             off = nn_tap_offset(t);
         }
         for (int mt = 0; mt < kNNMT; ++mt) {
             nn_bf16x8 afrag = {};
-            if (valid) afrag = *reinterpret_cast<const nn_bf16x8*>(in + (pos[mt] + off) * kNNC);
+            if (valid) afrag = *reinterpret_cast<const nn_bf16x8*>(in + (pos[mt] + off) * kNNC + half);
             acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc[mt], 0, 0, 0);
         }
     }
-    if (n >= kNNC) return;
     const float bn = bias ? bias[n] : 0.f;
     for (int mt = 0; mt < kNNMT; ++mt)
         for (int i = 0; i < 4; ++i) {
@@ -115,32 +141,26 @@ __device__ __forceinline__ void nn_conv(const uint16_t* __restrict__ in, const u
         }
 }
 
-// Weight gradient of one layer, dW[t][cout][cin] += sum_cells g[cell][cout] a[cell+off(t)][cin],
-// two taps per MFMA: M = cout (rows 8..15 zero), N = 2 taps x 8 cin, K = 512 cells in 16
-// steps; wave w owns tap pairs w, w+8 (14 pairs).  Bias gradient db[cout] += sum_cells g.
+// Weight gradient of one layer, dW[t][cout][cin] += sum_cells g[cell][cout] a[cell+off(t)][cin]:
+// one tap per MFMA (M = cout, N = cin, K = 512 cells in 16 steps); wave w owns taps w, w+8,
+// w+16, w+24.  Bias gradient db[cout] += sum_cells g.
 __device__ __forceinline__ void nn_weight_grad(const uint16_t* __restrict__ gt, const uint16_t* __restrict__ at,
                                                float* __restrict__ gw, float* __restrict__ gb) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, n = lane & 15;
-    for (int pr = wave; pr < 14; pr += 8) {
-        const int t = 2 * pr + (n >> 3), cin = n & 7;
-        const bool valid = t < kNNTaps;
-        const int off = valid ? nn_tap_offset(t) : 0;
+    for (int t = wave; t < kNNTaps; t += 8) {
+        const int off = nn_tap_offset(t);
         nn_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         for (int s = 0; s < 16; ++s) {
-            nn_bf16x8 afrag = {}, bfrag = {};
+            nn_bf16x8 afrag, bfrag;
             for (int j = 0; j < 8; ++j) {
                 const int p = nn_pos(32 * s + 8 * g + j);
-                if (n < kNNC) afrag[j] = __builtin_bit_cast(__bf16, gt[p * kNNC + n]);
-                if (valid) bfrag[j] = __builtin_bit_cast(__bf16, at[(p + off) * kNNC + cin]);
+                afrag[j] = __builtin_bit_cast(__bf16, gt[p * kNNC + n]);
+                bfrag[j] = __builtin_bit_cast(__bf16, at[(p + off) * kNNC + n]);
             }
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc, 0, 0, 0);
         }
-        if (valid)
-            for (int i = 0; i < 4; ++i) {
-                const int cout = 4 * g + i;
-                if (cout < kNNC) atomicAdd(gw + (t * kNNC + cout) * kNNC + cin, acc[i]);
-            }
+        for (int i = 0; i < 4; ++i) atomicAdd(gw + (t * kNNC + 4 * g + i) * kNNC + n, acc[i]);
     }
     if (threadIdx.x < kNNC) {
         float s = 0.f;
@@ -152,16 +172,27 @@ __device__ __forceinline__ void nn_weight_grad(const uint16_t* __restrict__ gt, 
 __device__ __forceinline__ void nn_zero_tile(uint16_t* tile) {
     for (int i = threadIdx.x; i < kNNTile / 8; i += blockDim.x) reinterpret_cast<uint4*>(tile)[i] = uint4{0u, 0u, 0u, 0u};
 }
+__device__ __forceinline__ void nn_zero_halo(uint16_t* tile) {
+    for (int p = threadIdx.x; p < kTileN; p += blockDim.x)
+        if (nn_halo_pos(p)) {
+            reinterpret_cast<uint4*>(tile)[2 * p] = uint4{0u, 0u, 0u, 0u};
+            reinterpret_cast<uint4*>(tile)[2 * p + 1] = uint4{0u, 0u, 0u, 0u};
+        }
+}
 
-// One layer's weights (kNNWLayer / 8 = 216 units of 16 bytes) through registers into LDS:
-// fetch issues the global load, put stores it, so that the next layer's load can be in
-// flight during the current convolution.
-struct NNWRegs { uint4 v; };
+// One layer's weights (kNNWUnits units of 16 bytes) through registers into LDS: fetch issues
+// the global load, put stores it, so that the next layer's load can be in flight during the
+// current convolution.
+struct NNWRegs { uint4 v[2]; };
 __device__ __forceinline__ void nn_fetch_weights(const uint16_t* __restrict__ src, NNWRegs& R) {
-    if (threadIdx.x < kNNWLayer / 8) R.v = reinterpret_cast<const uint4*>(src)[threadIdx.x];
+    const int t = threadIdx.x;
+    if (t < kNNWUnits) R.v[0] = reinterpret_cast<const uint4*>(src)[t];
+    if (t + 512 < kNNWUnits) R.v[1] = reinterpret_cast<const uint4*>(src)[t + 512];
 }
 __device__ __forceinline__ void nn_put_weights(const NNWRegs& R, uint16_t* dst) {
-    if (threadIdx.x < kNNWLayer / 8) reinterpret_cast<uint4*>(dst)[threadIdx.x] = R.v;
+    const int t = threadIdx.x;
+    if (t < kNNWUnits) reinterpret_cast<uint4*>(dst)[t] = R.v[0];
+    if (t + 512 < kNNWUnits) reinterpret_cast<uint4*>(dst)[t + 512] = R.v[1];
 }
 __device__ __forceinline__ void nn_load_weights(const uint16_t* __restrict__ src, uint16_t* dst) {
     NNWRegs R;
@@ -169,10 +200,9 @@ __device__ __forceinline__ void nn_load_weights(const uint16_t* __restrict__ src
     nn_put_weights(R, dst);
 }
 
-// Brick scale s_b (RMS of v over active cells; 0 when that is zero) and, when s_b > 0, the
-// features of cell q = threadIdx.x into the zeroed tile.  v is cell q's input value, m its
-// mask.  Block-collective (512 lanes).
-__device__ __forceinline__ float nn_features(float v, uint8_t m, uint16_t* tile, NNShared& L) {
+// Brick scale s_b (RMS of v over active cells; 0 when that is zero) and the active flags of
+// the brick's cells into L.act.  v is cell q's input value, m its mask.  Block-collective.
+__device__ __forceinline__ float nn_scale(float v, uint8_t m, NNShared& L) {
     const int q = threadIdx.x;
     const bool act = (m & kActive) != 0;
     L.act[q] = act ? 1 : 0;
@@ -182,42 +212,71 @@ __device__ __forceinline__ float nn_features(float v, uint8_t m, uint16_t* tile,
     __syncthreads();
     float S = 0.f, N = 0.f;
     for (int w = 0; w < 8; ++w) { S += L.red[2 * w]; N += L.red[2 * w + 1]; }
-    const float s = N > 0.f ? sqrtf(S / N) : 0.f;
-    if (act && s > 0.f) {
-        constexpr uint16_t one = 0x3f80, minus_one = 0xbf80;
-        const int open = __popc(static_cast<unsigned>(m) & 63u);
-        uint4 f;
-        f.x = nn_f2bf(v * (1.f / s)) | (unsigned(one) << 16);
-        f.y = ((m & kSpecial) ? one : 0u) | (unsigned((q & 1) ? one : minus_one) << 16);
-        f.z = ((q >> 3) & 1 ? one : minus_one) | (unsigned((q >> 6) & 1 ? one : minus_one) << 16);
-        f.w = nn_f2bf(static_cast<float>(open) / 6.f);
-        reinterpret_cast<uint4*>(tile)[nn_pos(q)] = f;
-    }
-    __syncthreads();
-    return s;
+    return N > 0.f ? sqrtf(S / N) : 0.f;
 }
 
-// Layers 0..3 from the feature tile T0 (T1 zeroed, halos of both zero) into L.y.  When
-// `store` is set, the inputs of layers 1..3 are written there (training).
-__device__ __forceinline__ void nn_layers(const NNView& W, NNShared& L, uint16_t* store,
-                                          const NNWRegs* first = nullptr) {
+// Features of every position of the 10^3 tile of the brick at position `pos` (its cells and
+// the halo from the neighbours in the set) from the residual store, scaled by inv = 1/s_b.
+// Writes every position (zeros where there is no input).
+__device__ __forceinline__ void nn_stage_features(const MeshView& M, const NNSetView& S, int pos,
+                                                  const float* __restrict__ store, float inv, uint16_t* tile) {
+    constexpr uint32_t one = 0x3f80, minus_one = 0xbf80;
+    for (int p = threadIdx.x; p < kTileN; p += blockDim.x) {
+        const int ti = p % 10, tj = (p / 10) % 10, tk = p / 100;
+        const int d = (ti == 0 ? 0 : ti == 9 ? 2 : 1) + 3 * (tj == 0 ? 0 : tj == 9 ? 2 : 1) + 9 * (tk == 0 ? 0 : tk == 9 ? 2 : 1);
+        const int np = S.nbtab[27 * pos + d];
+        uint4 f0 = {0u, 0u, 0u, 0u}, f1 = {0u, 0u, 0u, 0u};
+        if (np >= 0) {
+            const int nb = S.list ? S.list[np] : np;
+            const int ci = (ti + 7) & 7, cj = (tj + 7) & 7, ck = (tk + 7) & 7;
+            const int cell = ci + 8 * cj + 64 * ck;
+            const uint32_t m = M.mask[static_cast<int64_t>(nb) * 512 + cell];
+            if (m & kActive) {
+                const float v = store[static_cast<int64_t>(np) * 512 + cell];
+                auto bit = [&](int k) { return ((m >> k) & 1u) ? one : 0u; };
+                f0.x = nn_f2bf(v * inv) | (one << 16);
+                f0.y = ((m & kSpecial) ? one : 0u) | ((ci & 1 ? one : minus_one) << 16);
+                f0.z = (cj & 1 ? one : minus_one) | ((ck & 1 ? one : minus_one) << 16);
+                f0.w = nn_f2bf(static_cast<float>(__popc(m & 63u)) / 6.f) | (bit(0) << 16);
+                f1.x = bit(1) | (bit(2) << 16);
+                f1.y = bit(3) | (bit(4) << 16);
+                f1.z = bit(5);
+            }
+        }
+        reinterpret_cast<uint4*>(tile)[2 * p] = f0;
+        reinterpret_cast<uint4*>(tile)[2 * p + 1] = f1;
+    }
+}
+
+// Layers 0..3 of weight set `set` from the feature tile T0 (T1 zeroed) into the FP32 output
+// over T0 (returned).  When `store` is set, the inputs of layers 1..3 are written there
+// (training).  T0's halo is cleared once layer 0 has read it (hidden layers pad with zeros).
+__device__ __forceinline__ const float* nn_layers(const NNView& W, int set, NNShared& L, uint16_t* store,
+                                                  const NNWRegs* first = nullptr) {
+    const uint16_t* wset = W.w + static_cast<size_t>(set) * kNNWSet;
     uint16_t* in = L.T0;
     uint16_t* out = L.T1;
     NNWRegs R;
     if (first) R = *first;
-    else nn_fetch_weights(W.w, R);
+    else nn_fetch_weights(wset, R);
     for (int l = 0; l < kNNLayers; ++l) {
         nn_put_weights(R, L.WL);
-        if (l + 1 < kNNLayers) nn_fetch_weights(W.w + static_cast<size_t>(l + 1) * kNNWLayer, R);
+        if (l + 1 < kNNLayers) nn_fetch_weights(wset + static_cast<size_t>(l + 1) * kNNWLayer, R);
         __syncthreads();
-        if (store && l > 0)
-            *reinterpret_cast<uint4*>(store + (static_cast<size_t>(l - 1) * 512 + threadIdx.x) * kNNC) =
-                reinterpret_cast<const uint4*>(in)[nn_pos(threadIdx.x)];
+        if (store && l > 0) {
+            const uint4* src = reinterpret_cast<const uint4*>(in) + 2 * nn_pos(threadIdx.x);
+            uint4* dst = reinterpret_cast<uint4*>(store + (static_cast<size_t>(l - 1) * 512 + threadIdx.x) * kNNC);
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
+        if (l == 1) nn_zero_halo(out);   // T0: layer 0 has read its halo; layer 1 writes its interior
         const bool last = l == kNNLayers - 1;
-        nn_conv(in, L.WL, W.bias + l * kNNC, !last, L.act, last ? nullptr : out, last ? L.y : nullptr);
+        nn_conv(in, L.WL, W.bias + set * kNNBSet + l * kNNC, !last, L.act, last ? nullptr : out,
+                last ? reinterpret_cast<float*>(out) : nullptr);
         __syncthreads();
         uint16_t* t = in; in = out; out = t;
     }
+    return reinterpret_cast<const float*>(in);  // the last layer wrote over T0
 }
 
 // brick and list position of this workgroup: with a list, block p handles list[p];
@@ -229,51 +288,63 @@ __device__ __forceinline__ NNBrick nn_brick(const int* list) {
     return NNBrick{b, b};
 }
 
-// Inference inside PCG, after the V-cycle's last kernel wrote z and the r.z / r_prev.z
-// partials (at brick slots): r' = r - A z on the brick, the network, dz = (s_b / h_b) y into
-// dz[pos] (applied to z by k_nn_add, so that no block reads a z that another block has
-// changed), and the brick's partials increased by r.dz and r_prev.dz.
-__global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, const int* __restrict__ list,
-                                                  const double* __restrict__ r,
-                                                  const double* __restrict__ z, double* __restrict__ dz,
-                                                  const double* __restrict__ rprev, double* partials,
-                                                  int nparts, const DevScalars* S, const int* __restrict__ off) {
-    // every independent load first (brick, mask, z tile and halo, r, compact neighbours, the
-    // first layer's weights), then the flags, then the tile barrier
-    __shared__ __attribute__((aligned(16))) NNShared L;
+// r' = r - A z on the network's bricks into the FP32 store (0 on inactive cells), after the
+// V-cycle's last kernel wrote z.  S / off may be null (no done / switched-off test).
+__global__ __launch_bounds__(512) void k_nn_resid(MeshView M, const int* __restrict__ list, const double* __restrict__ r,
+                                                  const double* __restrict__ z, float* __restrict__ store,
+                                                  const DevScalars* S, const int* __restrict__ off) {
+    __shared__ double ztile[kTileN];
     const NNBrick B = nn_brick(list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     const uint8_t m = M.mask[a];
-    double* ztile = reinterpret_cast<double*>(L.T1);   // the z tile lives in T1 until the layers
     auto zv = [&](int64_t c) -> double { return z[c]; };
     const TileFetch<double> tf = fetch_tile<double>(M, B.b, zv);
     const double ra = r[a];
-    NNWRegs W0;
-    nn_fetch_weights(W.w, W0);
-    const bool skip = S->done || *off;
+    const bool skip = (S && S->done) || (off && *off);
     const int sp = special_rank(M, B.b, q, (m & kSpecial) != 0);
     const CompactFetch<double> cf = fetch_compact<double>(M, sp, zv);
     if (skip) return;
     stage_tile<double>(ztile, tf);
+    __syncthreads();
     double rp = 0.0;
     if (m & kActive) {
         double dg;
         rp = ra - row0f<double>(M, ztile, cf, zv, B.b, q, m, tf.own, dg);
     }
-    __syncthreads();
-    nn_zero_tile(L.T0);
-    nn_zero_tile(L.T1);
-    __syncthreads();
-    const float s = nn_features(static_cast<float>(rp), m, L.T0, L);
+    store[static_cast<int64_t>(B.pos) * 512 + q] = static_cast<float>(rp);
+}
+
+// Inference inside PCG, after k_nn_resid: the network on the brick from the residual store,
+// dz = (s_b / h_b) y into dz[pos] (applied to z by k_nn_add, so that no block reads a z that
+// another block has changed), and the brick's partials increased by r.dz and r_prev.dz.
+__global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+                                                  const double* __restrict__ r, double* __restrict__ dz,
+                                                  const double* __restrict__ rprev, double* partials,
+                                                  int nparts, const DevScalars* S, const int* __restrict__ off) {
+    __shared__ __attribute__((aligned(16))) NNShared L;
+    const NNBrick B = nn_brick(NS.list);
+    const int q = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
+    const uint8_t m = M.mask[a];
+    const float v = store[static_cast<int64_t>(B.pos) * 512 + q];
+    const double ra = r[a];
+    const double rpa = rprev ? rprev[a] : 0.0;
+    const int set = nn_set(M, B.b);
+    NNWRegs W0;
+    nn_fetch_weights(W.w + static_cast<size_t>(set) * kNNWSet, W0);
+    if (S->done || *off) return;
+    const float s = nn_scale(v, m, L);
     double d = 0.0;
     if (s > 0.f) {  // block-uniform
-        nn_layers(W, L, nullptr, &W0);
-        if (m & kActive) d = double(s) / level_h(M, B.b) * double(L.y[q]);
+        nn_zero_tile(L.T1);
+        nn_stage_features(M, NS, B.pos, store, 1.f / s, L.T0);
+        const float* y = nn_layers(W, set, L, nullptr, &W0);
+        if (m & kActive) d = double(s) / level_h(M, B.b) * double(y[q]);
     }
     dz[static_cast<int64_t>(B.pos) * 512 + q] = d;
-    const double t1 = block_sum<512>(r[a] * d);
-    const double t2 = rprev ? block_sum<512>(rprev[a] * d) : 0.0;
+    const double t1 = block_sum<512>(ra * d);
+    const double t2 = rprev ? block_sum<512>(rpa * d) : 0.0;
     if (threadIdx.x == 0) {
         partials[B.b] += t1;
         partials[nparts + B.b] += t2;
@@ -288,47 +359,60 @@ __global__ __launch_bounds__(512) void k_nn_add(const int* __restrict__ list, co
     z[static_cast<int64_t>(B.b) * 512 + threadIdx.x] += dz[static_cast<int64_t>(B.pos) * 512 + threadIdx.x];
 }
 
-// Raw network output for every brick (block = brick): y[dof] (0 on inactive cells and
-// zero-input bricks) and s[brick], for input v.
-__global__ __launch_bounds__(512) void k_nn_forward_raw(MeshView M, NNView W, const double* __restrict__ v,
-                                                        float* __restrict__ yout, float* __restrict__ sout) {
-    __shared__ __attribute__((aligned(16))) NNShared L;
-    const int b = blockIdx.x, q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    nn_zero_tile(L.T0);
-    nn_zero_tile(L.T1);
-    __syncthreads();
-    const uint8_t m = M.mask[a];
-    const float s = nn_features(static_cast<float>(v[a]), m, L.T0, L);
-    if (s > 0.f) nn_layers(W, L, nullptr);
-    yout[a] = s > 0.f ? L.y[q] : 0.f;
-    if (q == 0) sout[b] = s;
+// FP64 vector -> FP32 store of the set's bricks (position-major)
+__global__ __launch_bounds__(512) void k_nn_gather(const int* __restrict__ list, const double* __restrict__ v,
+                                                   float* __restrict__ store) {
+    const int p = blockIdx.x;
+    const int b = list ? list[p] : p;
+    store[static_cast<int64_t>(p) * 512 + threadIdx.x] = static_cast<float>(v[static_cast<int64_t>(b) * 512 + threadIdx.x]);
 }
 
-// ---- training (one recorded post-V-cycle residual rp per pass) ----------------------------
-// forward over the listed bricks: layer inputs to scratch[pos], dz = (s_b/h_b) y into the
-// full-length vector dfull (its other entries stay zero), s_b into sb[pos]
-__global__ __launch_bounds__(512) void k_nn_train_fwd(MeshView M, NNView W, const int* __restrict__ list,
-                                                      const double* __restrict__ rp, uint16_t* __restrict__ scratch,
-                                                      double* __restrict__ dfull, float* __restrict__ sb) {
+// Raw network output of every brick of the set (block = position): y[pos*512+q] (0 on
+// inactive cells and zero-input bricks) and s[pos], input from the residual store.
+__global__ __launch_bounds__(512) void k_nn_forward_raw(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+                                                        float* __restrict__ yout, float* __restrict__ sout) {
     __shared__ __attribute__((aligned(16))) NNShared L;
-    const NNBrick B = nn_brick(list);
+    const int pos = blockIdx.x, q = threadIdx.x;
+    const int b = NS.list ? NS.list[pos] : pos;
+    const uint8_t m = M.mask[static_cast<int64_t>(b) * 512 + q];
+    const float s = nn_scale(store[static_cast<int64_t>(pos) * 512 + q], m, L);
+    float y = 0.f;
+    if (s > 0.f) {
+        nn_zero_tile(L.T1);
+        nn_stage_features(M, NS, pos, store, 1.f / s, L.T0);
+        y = nn_layers(W, nn_set(M, b), L, nullptr)[q];
+    }
+    yout[static_cast<int64_t>(pos) * 512 + q] = y;
+    if (q == 0) sout[pos] = s;
+}
+
+// ---- training (one recorded post-V-cycle residual per pass, in the FP32 store) -------------
+// forward over the set: layer inputs to scratch[pos], dz = (s_b/h_b) y into the full-length
+// vector dfull (its other entries stay zero), s_b into sb[pos]
+__global__ __launch_bounds__(512) void k_nn_train_fwd(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+                                                      uint16_t* __restrict__ scratch, double* __restrict__ dfull,
+                                                      float* __restrict__ sb) {
+    __shared__ __attribute__((aligned(16))) NNShared L;
+    const NNBrick B = nn_brick(NS.list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
-    nn_zero_tile(L.T0);
-    nn_zero_tile(L.T1);
-    __syncthreads();
     const uint8_t m = M.mask[a];
-    const float s = nn_features(static_cast<float>(rp[a]), m, L.T0, L);
-    if (s > 0.f) nn_layers(W, L, scratch + static_cast<size_t>(blockIdx.x) * kNNStore);
-    dfull[a] = (s > 0.f && (m & kActive)) ? double(s) / level_h(M, B.b) * double(L.y[q]) : 0.0;
+    const float s = nn_scale(store[static_cast<int64_t>(B.pos) * 512 + q], m, L);
+    double d = 0.0;
+    if (s > 0.f) {
+        nn_zero_tile(L.T1);
+        nn_stage_features(M, NS, B.pos, store, 1.f / s, L.T0);
+        const float* y = nn_layers(W, nn_set(M, B.b), L, scratch + static_cast<size_t>(blockIdx.x) * kNNStore);
+        if (m & kActive) d = double(s) / level_h(M, B.b) * double(y[q]);
+    }
+    dfull[a] = d;
     if (q == 0) sb[blockIdx.x] = s;
 }
 
-// energy loss and its gradient on the listed bricks:
+// energy loss and its gradient on the set's bricks:
 // gy = 2 (A dz - r') inv_norm,  loss += inv_norm * sum dz (A dz - 2 r')
 __global__ __launch_bounds__(512) void k_nn_energy(MeshView M, const int* __restrict__ list, const double* __restrict__ dfull,
-                                                   const double* __restrict__ rp, double* __restrict__ gy,
+                                                   const float* __restrict__ store, double* __restrict__ gy,
                                                    double* __restrict__ loss, double inv_norm) {
     __shared__ double tile[kTileN];
     const NNBrick B = nn_brick(list);
@@ -341,38 +425,40 @@ __global__ __launch_bounds__(512) void k_nn_energy(MeshView M, const int* __rest
     double g = 0.0, l = 0.0;
     if (m & kActive) {
         double dg;
+        const double rp = store[static_cast<int64_t>(B.pos) * 512 + q];
         const double da = tile[nn_pos(q)];
         const double Ad = row0<double>(M, tile, dv, B.b, q, m, sp, da, dg);
-        g = 2.0 * (Ad - rp[a]) * inv_norm;
-        l = da * (Ad - 2.0 * rp[a]) * inv_norm;
+        g = 2.0 * (Ad - rp) * inv_norm;
+        l = da * (Ad - 2.0 * rp) * inv_norm;
     }
     gy[static_cast<int64_t>(blockIdx.x) * 512 + q] = g;
     const double t = block_sum<512>(l);
     if (q == 0) atomicAdd(loss, t);
 }
 
-// backward over the listed bricks: output gradient (s_b/h_b) gy (BF16), MFMA weight
-// gradients and transposed-convolution input gradients through the ReLU masks, accumulated
-// into grad = [dW (kNNW) | db (kNNB)] with FP32 atomics
-__global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, const int* __restrict__ list,
-                                                      const double* __restrict__ rp, const double* __restrict__ gy,
-                                                      const float* __restrict__ sb, const uint16_t* __restrict__ scratch,
-                                                      float* __restrict__ grad) {
+// backward over the set: output gradient (s_b/h_b) gy (BF16), MFMA weight gradients and
+// transposed-convolution input gradients through the ReLU masks, accumulated into
+// grad = [dW (kNNW) | db (kNNB)] of the brick's weight set with FP32 atomics
+__global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+                                                      const double* __restrict__ gy, const float* __restrict__ sb,
+                                                      const uint16_t* __restrict__ scratch, float* __restrict__ grad) {
     __shared__ __attribute__((aligned(16))) NNShared L;
     __shared__ __attribute__((aligned(16))) uint16_t T2[kNNTile];
-    const NNBrick B = nn_brick(list);
+    const NNBrick B = nn_brick(NS.list);
     const float s = sb[blockIdx.x];
     if (!(s > 0.f)) return;  // block-uniform
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     const uint8_t m = M.mask[a];
-    const uint16_t* store = scratch + static_cast<size_t>(blockIdx.x) * kNNStore;
+    const int set = nn_set(M, B.b);
+    const uint16_t* saved = scratch + static_cast<size_t>(blockIdx.x) * kNNStore;
+    nn_scale(store[static_cast<int64_t>(B.pos) * 512 + q], m, L);   // active flags
     nn_zero_tile(L.T0);
-    nn_zero_tile(L.T1);
     nn_zero_tile(T2);
+    // layer-0 input (features with the neighbour halo) into T1
+    nn_stage_features(M, NS, B.pos, store, 1.f / s, L.T1);
     __syncthreads();
-    // layer-0 input (features) into T1, output gradient into channel 0 of T0
-    nn_features(static_cast<float>(rp[a]), m, L.T1, L);
+    // output gradient into channel 0 of T0
     if (m & kActive) {
         const float gq = static_cast<float>(double(s) / level_h(M, B.b) * gy[static_cast<int64_t>(blockIdx.x) * 512 + q]);
         L.T0[nn_pos(q) * kNNC] = nn_f2bf(gq);
@@ -381,24 +467,28 @@ __global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, cons
     // Three tiles: gcur (gradient w.r.t. layer l's pre-activation output), gnext and T1 (the
     // features, input of layer 0).  The input of layer l > 0 is staged from scratch into
     // gnext, its ReLU pattern is kept in registers, and gnext is then overwritten by the
-    // transposed convolution.
+    // transposed convolution.  The halos of gcur and gnext stay zero.
     uint16_t* gcur = L.T0;
     uint16_t* gnext = T2;
+    float* gw = grad + static_cast<size_t>(set) * kNNWSet;
+    float* gb = grad + kNNW + set * kNNBSet;
     for (int l = kNNLayers - 1; l >= 0; --l) {
         uint16_t* ain = L.T1;
         if (l > 0) {
             ain = gnext;
-            reinterpret_cast<uint4*>(ain)[nn_pos(q)] =
-                *reinterpret_cast<const uint4*>(store + (static_cast<size_t>(l - 1) * 512 + q) * kNNC);
+            const uint4* src = reinterpret_cast<const uint4*>(saved + (static_cast<size_t>(l - 1) * 512 + q) * kNNC);
+            uint4* dst = reinterpret_cast<uint4*>(ain) + 2 * nn_pos(q);
+            dst[0] = src[0];
+            dst[1] = src[1];
         }
         __syncthreads();
-        nn_weight_grad(gcur, ain, grad + static_cast<size_t>(l) * kNNWLayer, grad + kNNW + l * kNNC);
+        nn_weight_grad(gcur, ain, gw + static_cast<size_t>(l) * kNNWLayer, gb + l * kNNC);
         if (l > 0) {
             // ReLU pattern of layer l-1's output (this layer's input) at cell q
             uint32_t pmask = 0;
             for (int c = 0; c < kNNC; ++c)
                 if (nn_bf2f(ain[nn_pos(q) * kNNC + c]) > 0.f) pmask |= 1u << c;
-            nn_load_weights(W.wt + static_cast<size_t>(l) * kNNWLayer, L.WL);
+            nn_load_weights(W.wt + static_cast<size_t>(set) * kNNWSet + static_cast<size_t>(l) * kNNWLayer, L.WL);
             __syncthreads();
             nn_conv(gcur, L.WL, nullptr, false, L.act, gnext, nullptr);
             __syncthreads();
@@ -415,10 +505,11 @@ __global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, cons
 __global__ void k_nn_pack(const float* __restrict__ w32, uint16_t* __restrict__ w, uint16_t* __restrict__ wt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kNNW) return;
-    const int l = i / kNNWLayer, rem = i % kNNWLayer, t = rem / (kNNC * kNNC), cout = (rem / kNNC) % kNNC, cin = rem % kNNC;
+    const int lset = i / kNNWLayer, rem = i % kNNWLayer;   // lset = set * kNNLayers + layer
+    const int t = rem / (kNNC * kNNC), cout = (rem / kNNC) % kNNC, cin = rem % kNNC;
     const uint16_t bits = nn_f2bf(w32[i]);
     w[i] = bits;
-    wt[static_cast<size_t>(l) * kNNWLayer + ((26 - t) * kNNC + cin) * kNNC + cout] = bits;
+    wt[static_cast<size_t>(lset) * kNNWLayer + ((26 - t) * kNNC + cin) * kNNC + cout] = bits;
 }
 
 // Adam on p = [weights | biases] (FP32), gradient cleared after use.
@@ -433,13 +524,15 @@ __global__ void k_nn_adam(float* __restrict__ p, float* __restrict__ m, float* _
     g[i] = 0.f;
 }
 
-// per-brick partials of sum_active v^2 / h_b (normalisation of a recorded residual)
-__global__ __launch_bounds__(512) void k_nn_norm(MeshView M, const double* __restrict__ v, double* partials) {
-    const int b = blockIdx.x, q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const double x = (M.mask[a] & kActive) ? v[a] : 0.0;
+// per-position partials of sum_active v^2 / h_b over the set's bricks (normalisation of a
+// recorded residual in the FP32 store)
+__global__ __launch_bounds__(512) void k_nn_norm(MeshView M, const int* __restrict__ list, const float* __restrict__ store,
+                                                 double* partials) {
+    const int p = blockIdx.x, q = threadIdx.x;
+    const int b = list ? list[p] : p;
+    const double x = (M.mask[static_cast<int64_t>(b) * 512 + q] & kActive) ? double(store[static_cast<int64_t>(p) * 512 + q]) : 0.0;
     const double t = block_sum<512>(x * x / level_h(M, b));
-    if (q == 0) partials[b] = t;
+    if (q == 0) partials[p] = t;
 }
 
 }  // namespace pcfd

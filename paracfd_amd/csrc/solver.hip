@@ -368,32 +368,37 @@ private:
     }
     // learned preconditioner
     NNView nnv() const { return NNView{nn_w_, nn_wt_, nn_p_ + kNNW}; }
+    NNSetView nn_setv() const { return NNSetView{nn_list_, nn_nbtab_}; }
     void set_nn_off(bool off) { if (nn_) HIP_CHECK(hipMemsetAsync(nn_off_, off ? 1 : 0, sizeof(int), stream_)); }
     unsigned nn_grid() const { return static_cast<unsigned>(nn_list_ ? nn_nlist_ : nb_); }
-    // network correction after the V-cycle's last kernel, then the r.z finalisation; a network
-    // that is off (untrained, switched off or rejected by validation) is not launched at all
+    // network correction after the V-cycle's last kernel (post-V-cycle residual, network,
+    // z += dz), then the r.z finalisation; a network that is off (untrained, switched off or
+    // rejected by validation) is not launched at all
     void pc_tail(int fin) {
         if (nn_ && nn_on_) {
-            LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
+            LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, S_, nn_off_);
+            LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_setv(), nn_rstore_, r_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
             LAUNCH(k_nn_add, nn_grid(), 512, nn_list_, nn_dz_, z_, S_, nn_off_);
         }
         finalize(fin, nb_);
     }
-    // sum_active v^2 / h_b of a device vector (normalisation of a recorded residual)
-    double nn_norm(const double* v) {
-        LAUNCH(k_nn_norm, nb_, 512, M_, v, partials_);
-        std::vector<double> h(nb_);
-        HIP_CHECK(hipMemcpyAsync(h.data(), partials_, nb_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    // sum over the set's active cells of v^2 / h_b of a residual store (its normalisation)
+    double nn_norm(const float* store) {
+        const unsigned g = nn_grid();
+        LAUNCH(k_nn_norm, g, 512, M_, nn_list_, store, partials_);
+        std::vector<double> h(g);
+        HIP_CHECK(hipMemcpyAsync(h.data(), partials_, g * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
         double s = 0.0;
         for (double x : h) s += x;
         return s;
     }
-    // one training pass (forward, energy gradient, backward) on residual rp into nn_g_ / nn_loss_
-    void nn_train_pass(const double* rp, double inv_norm) {
-        LAUNCH(k_nn_train_fwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_scratch_, nn_dfull_, nn_sb_);
+    // one training pass (forward, energy gradient, backward) on the residual store `rp` into
+    // nn_g_ / nn_loss_
+    void nn_train_pass(const float* rp, double inv_norm) {
+        LAUNCH(k_nn_train_fwd, nn_grid(), 512, M_, nnv(), nn_setv(), rp, nn_scratch_, nn_dfull_, nn_sb_);
         LAUNCH(k_nn_energy, nn_grid(), 512, M_, nn_list_, nn_dfull_, rp, nn_gy_, nn_loss_, inv_norm);
-        LAUNCH(k_nn_train_bwd, nn_grid(), 512, M_, nnv(), nn_list_, rp, nn_gy_, nn_sb_, nn_scratch_, nn_g_);
+        LAUNCH(k_nn_train_bwd, nn_grid(), 512, M_, nnv(), nn_setv(), rp, nn_gy_, nn_sb_, nn_scratch_, nn_g_);
     }
     void setup_learned(const Topology& T);
     void setup_gradients(const Topology& T);
@@ -569,13 +574,19 @@ private:
     double* nn_loss_ = nullptr;
     uint16_t* nn_scratch_ = nullptr;  // layer inputs per listed brick (training)
     int* nn_list_ = nullptr;      // the network's bricks (null: all bricks, XCD-aware order)
+    int* nn_nbtab_ = nullptr;     // 27 per position: the neighbour's position in the set or -1
+    float* nn_rstore_ = nullptr;  // post-V-cycle residual on the set's bricks (FP32, 512 per position)
+    std::vector<int> nn_host_nbr_;    // brick_nbr (the all-brick set of the forward hook)
+    int* nn_all_nbtab_ = nullptr;     // forward hook: neighbour table of the all-brick set
+    float* nn_all_store_ = nullptr;   // forward hook: its input and output
+    float* nn_all_y_ = nullptr;
     int nn_nlist_ = 0;
     int64_t nn_cells_ = 0;        // active cells of the network's bricks
     double* nn_dz_ = nullptr;     // correction per listed brick (512 per position)
     double* nn_dfull_ = nullptr;  // training: correction as a full-length vector
     double* nn_gy_ = nullptr;     // training: output gradient per listed brick
     float* nn_sb_ = nullptr;      // training: brick scales
-    std::vector<double*> nn_rs_;  // recorded post-V-cycle residuals
+    std::vector<float*> nn_rs_;   // recorded post-V-cycle residuals (stores of the set)
     std::vector<double> nn_inv_norm_;
     int nn_cap_ = 0, nn_count_ = 0, nn_next_ = 0, nn_cursor_ = 0, nn_t_ = 0;
     int trial_left_ = 0;            // projections of the validation still to run
@@ -1003,6 +1014,24 @@ void SolverImpl<R>::setup_gradients(const Topology& T) {
 
 // Network buffers, brick selection and the initial weights: He-normal (seed 1234) for
 // layers 0..2, zero for the output layer, so an untrained network adds nothing.
+// Neighbour table of a brick set (position p -> brick bricks[p]): entry 27 p + d, d = (dx+1)
+// + 3 (dy+1) + 9 (dz+1), is the position of the same-level brick at offset (dx, dy, dz),
+// reached face by face in x, y, z order, or -1 when there is none or it is not in the set.
+static std::vector<int> nn_neighbour_table(const std::vector<int>& nbr, const std::vector<int>& bricks, int nbricks) {
+    std::vector<int> pos_of(nbricks, -1);
+    for (size_t p = 0; p < bricks.size(); ++p) pos_of[bricks[p]] = static_cast<int>(p);
+    std::vector<int> tab(27 * bricks.size(), -1);
+    for (size_t p = 0; p < bricks.size(); ++p)
+        for (int d = 0; d < 27; ++d) {
+            const int off[3] = {d % 3 - 1, (d / 3) % 3 - 1, d / 9 - 1};
+            int c = bricks[p];
+            for (int ax = 0; ax < 3 && c >= 0; ++ax)
+                if (off[ax] != 0) c = nbr[6 * static_cast<size_t>(c) + 2 * ax + (off[ax] > 0 ? 1 : 0)];
+            tab[27 * p + d] = c >= 0 ? pos_of[c] : -1;
+        }
+    return tab;
+}
+
 template <class R>
 void SolverImpl<R>::setup_learned(const Topology& T) {
     if (lv_.empty()) throw std::invalid_argument("the learned preconditioner needs a multigrid hierarchy");
@@ -1019,15 +1048,17 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
         return z ^ (z >> 31);
     };
     auto uniform = [&] { return (static_cast<double>(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
-    for (int l = 0; l + 1 < kNNLayers; ++l) {
-        const double sd = std::sqrt(2.0 / (kNNTaps * kNNC));
-        for (int t = 0; t < kNNTaps; ++t)
-            for (int o = 0; o < kNNC; ++o)
-                for (int i = 0; i < kNNC; ++i) {
-                    const double g = std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform());
-                    p[static_cast<size_t>(l) * kNNWLayer + (t * kNNC + o) * kNNC + i] = static_cast<float>(sd * g);
-                }
-    }
+    for (int set = 0; set < kNNSets; ++set)
+        for (int l = 0; l + 1 < kNNLayers; ++l) {
+            const double sd = std::sqrt(2.0 / (kNNTaps * kNNC));
+            for (int t = 0; t < kNNTaps; ++t)
+                for (int o = 0; o < kNNC; ++o)
+                    for (int i = 0; i < kNNC; ++i) {
+                        const double g = std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform());
+                        p[static_cast<size_t>(set) * kNNWSet + static_cast<size_t>(l) * kNNWLayer + (t * kNNC + o) * kNNC + i] =
+                            static_cast<float>(sd * g);
+                    }
+        }
     nn_p_ = upload(p);
     nn_w_ = alloc<uint16_t>(kNNW);
     nn_wt_ = alloc<uint16_t>(kNNW);
@@ -1062,7 +1093,11 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
     for (int b : list)
         for (int q = 0; q < 512; ++q) nn_cells_ += (T.mask[static_cast<size_t>(b) * 512 + q] & kActive) ? 1 : 0;
     if (nn_cells_ == 0) throw std::invalid_argument("the learned preconditioner's bricks hold no active cell");
+    nn_host_nbr_ = T.brick_nbr;
+    nn_nbtab_ = upload(nn_neighbour_table(nn_host_nbr_, list, nb_));
     const int64_t slots = static_cast<int64_t>(nn_list_ ? nn_nlist_ : nb_) * 512;
+    nn_rstore_ = alloc<float>(slots);
+    HIP_CHECK(hipMemset(nn_rstore_, 0, slots * sizeof(float)));
     nn_scratch_ = alloc<uint16_t>(static_cast<int64_t>(nn_nlist_) * kNNStore);
     nn_dz_ = alloc<double>(slots);
     nn_gy_ = alloc<double>(slots);
@@ -2085,7 +2120,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
                 break;
             case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); break;
             case 3: precondition_dispatch(true); break;
-            case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_list_, r_, z_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
+            case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_setv(), nn_rstore_, r_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
             case 5: LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;
             case 6:  // alpha = 0 and x = z (scratch): the pressure and the residual are untouched
                 if (opt_.vcycle_precision == PCFD_FP32)
@@ -2105,6 +2140,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         if (!nn_) throw std::invalid_argument("kernel 4 needs the learned preconditioner");
         // a residual with every brick non-zero so that each selected brick runs the network
         LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+        LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, nullptr, nullptr);
         set_nn_off(false);
     }
     launch();  // warm
@@ -2121,7 +2157,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         case 1: per = vt + 8 + 1 + vt; break;                  // x in, r in, mask, x out
         case 2: per = 9.0 * sizeof(R) + 1; break;             // v in (3), faces (3), mask, v out (3)
         case 3: per = 0; break;
-        case 4: per = 8 + 1 + 8 + 8 + 8; break;               // r, mask, z, r_prev, dz
+        case 4: per = 4 + 1 + 8 + 8 + 8; break;               // r' (FP32), mask, r, r_prev, dz
         case 5: per = 8 + 8 + 8 + 8 + 1; break;               // z, d_old in; d_new, Ad out; mask
         case 6: per = 16 + 8 + 8 + 8 + 8 + 1 + (vt == 8 ? 8 : 4) + vt; break;  // x in/out, d, r, Ad, r' out, mask, diag, x1 out
     }
@@ -2381,11 +2417,11 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
             if (host_s_[0].done || recorded >= max_pairs) return;
             const int slot = nn_next_;
             if (slot >= static_cast<int>(nn_rs_.size())) {
-                nn_rs_.push_back(alloc<double>(n_));
+                nn_rs_.push_back(alloc<float>(static_cast<int64_t>(nn_grid()) * 512));
                 nn_inv_norm_.push_back(0.0);
             }
-            double* rs = nn_rs_[slot];
-            LAUNCH(k_cg_residual, nb_, 512, M_, z_, r_, rs, partials_, S_);
+            float* rs = nn_rs_[slot];
+            LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, rs, nullptr, nullptr);
             const double norm = nn_norm(rs);
             if (!(norm > 0.0) || !std::isfinite(norm)) return;
             nn_inv_norm_[slot] = 1.0 / norm;
@@ -2504,13 +2540,20 @@ template <class R>
 void SolverImpl<R>::learned_forward(const double* r, float* y, float* s) {
     if (!nn_) throw std::invalid_argument("the solver was not created with the learned preconditioner");
     HIP_CHECK(hipStreamSynchronize(stream_));
+    if (!nn_all_nbtab_) {
+        std::vector<int> all(nb_);
+        for (int b = 0; b < nb_; ++b) all[b] = b;
+        nn_all_nbtab_ = upload(nn_neighbour_table(nn_host_nbr_, all, nb_));
+        nn_all_store_ = alloc<float>(n_);
+        nn_all_y_ = alloc<float>(n_);
+    }
     double* dr = static_cast<double*>(vc0_[0]);
-    float* dy = static_cast<float*>(vc0_[1]);
     float* ds = reinterpret_cast<float*>(partials_);
     HIP_CHECK(hipMemcpy(dr, r, n_ * sizeof(double), hipMemcpyHostToDevice));
-    LAUNCH(k_nn_forward_raw, nb_, 512, M_, nnv(), dr, dy, ds);
+    LAUNCH(k_nn_gather, nb_, 512, nullptr, dr, nn_all_store_);
+    LAUNCH(k_nn_forward_raw, nb_, 512, M_, nnv(), (NNSetView{nullptr, nn_all_nbtab_}), nn_all_store_, nn_all_y_, ds);
     HIP_CHECK(hipStreamSynchronize(stream_));
-    HIP_CHECK(hipMemcpy(y, dy, n_ * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(y, nn_all_y_, n_ * sizeof(float), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(s, ds, nb_ * sizeof(float), hipMemcpyDeviceToHost));
 }
 
@@ -2520,11 +2563,12 @@ void SolverImpl<R>::learned_gradient(const double* r, double* loss, float* gw, f
     HIP_CHECK(hipStreamSynchronize(stream_));
     double* dr = static_cast<double*>(vc0_[0]);
     HIP_CHECK(hipMemcpy(dr, r, n_ * sizeof(double), hipMemcpyHostToDevice));
-    const double norm = nn_norm(dr);
+    LAUNCH(k_nn_gather, nn_grid(), 512, nn_list_, dr, nn_rstore_);
+    const double norm = nn_norm(nn_rstore_);
     if (!(norm > 0.0) || !std::isfinite(norm)) throw std::invalid_argument("residual must be non-zero and finite");
     HIP_CHECK(hipMemset(nn_g_, 0, (kNNW + kNNB) * sizeof(float)));
     HIP_CHECK(hipMemset(nn_loss_, 0, sizeof(double)));
-    nn_train_pass(dr, 1.0 / norm);
+    nn_train_pass(nn_rstore_, 1.0 / norm);
     HIP_CHECK(hipStreamSynchronize(stream_));
     HIP_CHECK(hipMemcpy(loss, nn_loss_, sizeof(double), hipMemcpyDeviceToHost));
     if (gw) HIP_CHECK(hipMemcpy(gw, nn_g_, kNNW * sizeof(float), hipMemcpyDeviceToHost));

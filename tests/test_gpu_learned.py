@@ -43,10 +43,24 @@ def learned_core(topo, **kw):
 
 
 def random_weights(seed, scale=0.2):
+    from oracle import nn_oracle as N
     rng = np.random.default_rng(seed)
-    W = rng.normal(0.0, scale, size=(4, 27, 8, 8)).astype(np.float32)
-    B = rng.normal(0.0, 0.05, size=(4, 8)).astype(np.float32)
+    W = rng.normal(0.0, scale, size=(N.SETS, 4, 27, N.C, N.C)).astype(np.float32)
+    B = rng.normal(0.0, 0.05, size=(N.SETS, 4, N.C)).astype(np.float32)
     return W, B
+
+
+def network_bricks(topo, mode):
+    """The bricks learned_bricks = mode selects: bricks holding special cells (1) or cut cells
+    (2), and their face neighbours."""
+    nbr = topo.array("brick_nbr").reshape(-1, 6)
+    sd, sp = topo.array("special_dof"), topo.array("special_patch")
+    hit = np.unique(sd[sp >= 0] // 512) if mode == 2 else np.unique(sd // 512)
+    sel = np.zeros(topo.ndof // 512, dtype=bool)
+    sel[hit] = True
+    for b in hit:
+        sel[nbr[b][nbr[b] >= 0]] = True
+    return np.nonzero(sel)[0]
 
 
 def test_forward_matches_oracle(lib):
@@ -60,7 +74,8 @@ def test_forward_matches_oracle(lib):
     r = rng.normal(size=topo.ndof)
     y, s = core.learned_forward(r)
     mask = topo.array("mask")
-    y_ref, s_ref = N.network_output(W.astype(np.float64), B.astype(np.float64), mask, r)
+    y_ref, s_ref = N.network_output(W.astype(np.float64), B.astype(np.float64), mask, r, topo.array("brick_nbr"),
+                                    topo.array("brick_level"))
     assert rel_l2(s, s_ref) < 1e-5
     assert np.all(y[(mask & N.ACTIVE) == 0] == 0.0)
     err = rel_l2(y, y_ref)
@@ -68,12 +83,16 @@ def test_forward_matches_oracle(lib):
     assert err < 5e-3
 
 
-def test_gradient_matches_oracle(lib, oracle_built):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_gradient_matches_oracle(lib, oracle_built, mode):
+    """Loss and every weight-set / layer gradient of the training kernels against the oracle,
+    over every brick (mode 0) and over the cut-cell bricks with their neighbours (mode 2: the
+    halo of a brick at the set's edge is zero where the neighbour is outside the set)."""
     from oracle import nn_oracle as N
     from oracle.oracle import Oracle
     topo = sphere()
-    print(f"[learned] gradient parity, dofs={topo.ndof}", flush=True)
-    core = learned_core(topo)
+    print(f"[learned] gradient parity, dofs={topo.ndof}, learned_bricks={mode}", flush=True)
+    core = learned_core(topo, learned_bricks=mode)
     orc = Oracle(topo, core.options)
     W, B = random_weights(3)
     core.set_learned_weights(W, B)
@@ -81,18 +100,26 @@ def test_gradient_matches_oracle(lib, oracle_built):
     rng = np.random.default_rng(4)
     rp = rng.normal(size=topo.ndof) * ((mask & N.ACTIVE) != 0)
     loss, gw, gb = core.learned_gradient(rp)
-    args = (mask, rp, lambda x: orc.mg_primitive(0, 0, x), topo.array("level_h"), topo.array("brick_level"))
+    bricks = None if mode == 0 else network_bricks(topo, mode)
+    args = (mask, rp, lambda x: orc.mg_primitive(0, 0, x), topo.array("brick_nbr"), topo.array("level_h"),
+            topo.array("brick_level"), bricks)
     lref, dW, dB = N.energy_loss_and_grad(W.astype(np.float64), B.astype(np.float64), *args)
     print(f"[learned] loss {loss:.6e} oracle {lref:.6e}", flush=True)
     assert loss == pytest.approx(lref, rel=1e-3)
-    for l in range(4):
-        ew, eb = rel_l2(gw[l], dW[l]), rel_l2(gb[l], dB[l])
-        print(f"[learned] layer {l}: dW rel L2 {ew:.2e}, db rel L2 {eb:.2e}", flush=True)
-        assert ew < 2e-2 and eb < 2e-2
+    sets = np.unique(N.weight_set(topo.array("brick_level")[bricks if bricks is not None else slice(None)]))
+    for st in range(N.SETS):
+        if st not in sets:
+            assert not gw[st].any() and not gb[st].any()
+            continue
+        for l in range(4):
+            ew, eb = rel_l2(gw[st, l], dW[st, l]), rel_l2(gb[st, l], dB[st, l])
+            print(f"[learned] set {st} layer {l}: dW rel L2 {ew:.2e}, db rel L2 {eb:.2e}", flush=True)
+            assert ew < 2e-2 and eb < 2e-2
     # the float64 network without BF16 rounding is close as well
     _, dW64, _ = N.energy_loss_and_grad(W.astype(np.float64), B.astype(np.float64), *args, bf16_round=False)
-    for l in range(4):
-        assert rel_l2(gw[l], dW64[l]) < 1e-1
+    for st in sets:
+        for l in range(4):
+            assert rel_l2(gw[st, l], dW64[st, l]) < 1e-1
     # the gradient pass leaves the weights unchanged
     W2, B2 = core.learned_weights()
     assert np.array_equal(W2, W) and np.array_equal(B2, B)
@@ -116,8 +143,8 @@ def test_preconditioner_matches_oracle(lib, oracle_built):
     z = core.mg_primitive(3, 0, b=r)
     zmg = orc.mg_primitive(3, 0, b=r)
     rp = r - orc.mg_primitive(0, 0, zmg)
-    dz, _ = N.correction(W.astype(np.float64), B.astype(np.float64), mask, rp, topo.array("level_h"),
-                         topo.array("brick_level"))
+    dz, _ = N.correction(W.astype(np.float64), B.astype(np.float64), mask, rp, topo.array("brick_nbr"),
+                         topo.array("level_h"), topo.array("brick_level"))
     err = rel_l2(z - zmg, dz)
     print(f"[learned] correction rel L2 {err:.2e}, |dz|/|z| {np.linalg.norm(dz) / np.linalg.norm(zmg):.2e}", flush=True)
     assert err < 1e-2
@@ -186,8 +213,9 @@ def test_fallback_on_broken_network(lib):
     print(f"[learned] fallback, dofs={topo.ndof}", flush=True)
     core = learned_core(topo)
     core.initialize()
-    W = np.full((4, 27, 8, 8), 1e30, dtype=np.float32)
-    B = np.zeros((4, 8), dtype=np.float32)
+    from oracle import nn_oracle as N
+    W = np.full((N.SETS, 4, 27, N.C, N.C), 1e30, dtype=np.float32)
+    B = np.zeros((N.SETS, 4, N.C), dtype=np.float32)
     core.set_learned_weights(W, B)
     st = core.step()
     assert st["fallback_used"] == 1
