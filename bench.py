@@ -169,7 +169,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0, gmres_recycle=3):
+def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0, gmres_recycle=3, pretrained=None):
     """Initialise, pre-steps (with training for the learned solver), warmup, K timed steps."""
     import torch
     from paracfd_amd import ExternalAeroCore, _lib as L, default_options
@@ -183,6 +183,8 @@ def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0, gmres_recycl
     core.set_reference(topo.info.reference_area)
     st = core.initialize()
     log(f"{name}: initialize {st['iterations']} its, converged {st['converged']}")
+    if pc == L.PC_LEARNED and pretrained is not None:
+        core.set_learned_weights(*pretrained)
     train = None
     for i in range(2 + args.collects):
         if pc == L.PC_LEARNED and i >= 2:
@@ -236,6 +238,34 @@ def run(topo, spec, args, name, field, vcycle, pc, nonorthogonal=0, gmres_recycl
         f"fallbacks {fallbacks}, network used in {used} steps (rejected: {res['learned_rejected']}), "
         f"CL {loads['cl']:.6f} CD {loads['cd']:.6f}")
     return res
+
+
+def pretrain_network(args, spec, vc):
+    """Weights of the network trained across args.pretrain angles of attack in [0, 12] deg on
+    the case's geometry at base resolution args.pretrain_cells_x (None without pretraining)."""
+    if args.pretrain <= 0 or args.case not in ("naca2412_3lvl", "naca2412_2lvl", "paraglider_3lvl"):
+        return None, None
+    from paracfd_amd import _lib as L, cases, default_options, learned as LN
+    opt = default_options(rho=spec["rho"], nu=spec["nu"], speed=spec["speed"], cfl=spec["cfl"], rtol=1e-8,
+                          max_iterations=500, field_precision=L.FP32, vcycle_precision=vc,
+                          preconditioner=L.PC_LEARNED, pre_smooth=2, post_smooth=2,
+                          check_interval=args.check_interval, learned_bricks=args.learned_bricks,
+                          learned_capacity=args.collects * args.pairs)
+
+    def make(alpha, device):
+        g, d, _ = cases.build_case(args.case, cells_x=args.pretrain_cells_x, alpha_deg=alpha)
+        return L.Topology(g, d, device=device)
+
+    angles = LN.default_angles(args.pretrain)
+    W, B, stats = LN.train_across_angles(make, angles, opt, collects=args.collects, pairs=args.pairs,
+                                         train_steps=args.train_steps, lr=args.lr, log=log)
+    log(f"pretraining: {len(angles)} angles in [0, 12] deg, {stats['residuals']} residuals, "
+        f"{stats['brick_samples']} brick samples, {stats['seconds']:.1f}s")
+    summary = {"angles_deg": [round(a, 3) for a in angles], "cells_x": args.pretrain_cells_x,
+               "residuals": stats["residuals"], "brick_samples": stats["brick_samples"],
+               "seconds": round(stats["seconds"], 2),
+               "loss_last": [round(r["loss_last"], 4) for r in stats["instances"]]}
+    return (W, B), summary
 
 
 def run_sweep(args):
@@ -396,6 +426,10 @@ def main():
     ap.add_argument("--collects", type=int, default=4)
     ap.add_argument("--pairs", type=int, default=16)
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--pretrain", type=int, default=6,
+                    help="angles of attack in [0, 12] deg the network is trained on before the case (0: none)")
+    ap.add_argument("--pretrain-cells-x", type=int, default=144,
+                    help="base resolution of the pretraining wings")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--smooth-wall", action="store_true",
                     help="apply the smooth-wall shear model in every step (the reference's default) and "
@@ -449,7 +483,8 @@ def main():
         f"{topo.mg_levels} MG levels, built in {build_wall:.1f}s (device {build_device})")
 
     vc = L.BF16 if args.vcycle == "bf16" else L.FP32
-    learned = run(topo, spec, args, "learned", L.FP32, vc, L.PC_LEARNED)
+    pretrained, pretrain = pretrain_network(args, spec, vc)
+    learned = run(topo, spec, args, "learned", L.FP32, vc, L.PC_LEARNED, pretrained=pretrained)
     plain32 = run(topo, spec, args, "plain", L.FP32, vc, L.PC_MG)
     plain64 = run(topo, spec, args, "plain_fp64_vcycle", L.FP32, L.FP64, L.PC_MG)
     nonorth = run(topo, spec, args, "nonorthogonal_fgmres", L.FP32, vc, L.PC_MG, nonorthogonal=1,
@@ -528,7 +563,8 @@ def main():
     roofline["frac_of_measured_copy"] = achieved / ceilings["copy_GBps"]
     tr = learned["train"]
     ms_nn = timings[4]["ms"]
-    flops_nn = tr["bricks"] * 512 * 4 * 27 * 8 * 8 * 2.0   # useful network flops per launch
+    # useful network flops per launch: layers 0-2 16 -> 16 channels, layer 3 16 -> 1 (channel 0)
+    flops_nn = tr["bricks"] * 512 * 27 * 16 * (3 * 16 + 1) * 2.0
     nn_roof = {"bound": "mfma", "kernel": "k_nn_apply", "achieved": flops_nn / (ms_nn * 1e-3) / 1e12,
                "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": flops_nn / (ms_nn * 1e-3) / 1e12 / PEAK_BF16,
                "us_per_launch": ms_nn * 1e3, "launches_per_step": timings[4]["launches_per_step"]}
@@ -554,7 +590,7 @@ def main():
         "dtype": f"f32 fields + {args.vcycle} V-cycle + bf16 network + f64 PCG",
         "data": "synthetic (analytic geometry, freestream start)",
         "config": {"workload": args.case, "active_cells": n, "amr_levels": topo.info.levels,
-                   "preconditioner": "V-cycle + learned 4-layer 8-channel conv correction, flexible PCG",
+                   "preconditioner": "V-cycle + learned 4-layer 16-channel conv correction (weights per AMR level, neighbour-brick input halo), flexible PCG",
                    "network_bricks": tr["bricks"], "rtol": 1e-8, "mean_pcg_iterations": mean_its,
                    "fallback_steps": learned["fallback_steps"], "network_steps": learned["learned_steps"],
                    "network_rejected_by_validation": learned["learned_rejected"], "cl": learned["loads"]["cl"],
@@ -562,6 +598,7 @@ def main():
                    "cd_viscous": learned["loads"]["cd_viscous"], "total_cd": learned["loads"]["total_cd"],
                    "total_cl": learned["loads"]["total_cl"],
                    "training": {k: tr[k] for k in ("pairs", "train_steps", "loss_first", "loss_last", "train_ms")},
+                   "pretraining": pretrain,
                    "launch_shapes": learned["tuning"],
                    "plain_fp32_vcycle": summary(plain32), "plain_fp64_vcycle": summary(plain64),
                    "nonorthogonal_fgmres": summary(nonorth) if nonorth else None,

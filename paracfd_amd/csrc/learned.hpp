@@ -82,6 +82,7 @@ struct NNView {
 struct NNSetView {
     const int* list;      // position -> brick (null: position = brick)
     const int* nbtab;     // 27 per position: neighbour's position or -1
+    const uint8_t* masks; // cell masks per position (512 each; the mesh's own masks when position = brick)
 };
 
 __device__ __forceinline__ int nn_set(const MeshView& M, int b) {
@@ -114,6 +115,7 @@ __device__ __forceinline__ void nn_conv(const uint16_t* __restrict__ in, const u
         acc[mt] = nn_floatx4{0.f, 0.f, 0.f, 0.f};
         pos[mt] = nn_pos((wave * kNNMT + mt) * 16 + n);
     }
+#pragma unroll 2
     for (int s = 0; s < kNNKSteps; ++s) {
         const int t = 2 * s + (g >> 1);
         const bool valid = t < kNNTaps;
@@ -218,7 +220,7 @@ __device__ __forceinline__ float nn_scale(float v, uint8_t m, NNShared& L) {
 // Features of every position of the 10^3 tile of the brick at position `pos` (its cells and
 // the halo from the neighbours in the set) from the residual store, scaled by inv = 1/s_b.
 // Writes every position (zeros where there is no input).
-__device__ __forceinline__ void nn_stage_features(const MeshView& M, const NNSetView& S, int pos,
+__device__ __forceinline__ void nn_stage_features(const NNSetView& S, int pos,
                                                   const float* __restrict__ store, float inv, uint16_t* tile) {
     constexpr uint32_t one = 0x3f80, minus_one = 0xbf80;
     for (int p = threadIdx.x; p < kTileN; p += blockDim.x) {
@@ -227,12 +229,11 @@ __device__ __forceinline__ void nn_stage_features(const MeshView& M, const NNSet
         const int np = S.nbtab[27 * pos + d];
         uint4 f0 = {0u, 0u, 0u, 0u}, f1 = {0u, 0u, 0u, 0u};
         if (np >= 0) {
-            const int nb = S.list ? S.list[np] : np;
             const int ci = (ti + 7) & 7, cj = (tj + 7) & 7, ck = (tk + 7) & 7;
-            const int cell = ci + 8 * cj + 64 * ck;
-            const uint32_t m = M.mask[static_cast<int64_t>(nb) * 512 + cell];
+            const int64_t c = static_cast<int64_t>(np) * 512 + ci + 8 * cj + 64 * ck;
+            const uint32_t m = S.masks[c];
+            const float v = store[c];
             if (m & kActive) {
-                const float v = store[static_cast<int64_t>(np) * 512 + cell];
                 auto bit = [&](int k) { return ((m >> k) & 1u) ? one : 0u; };
                 f0.x = nn_f2bf(v * inv) | (one << 16);
                 f0.y = ((m & kSpecial) ? one : 0u) | ((ci & 1 ? one : minus_one) << 16);
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(512) void k_nn_resid(MeshView M, const int* __restr
 // Inference inside PCG, after k_nn_resid: the network on the brick from the residual store,
 // dz = (s_b / h_b) y into dz[pos] (applied to z by k_nn_add, so that no block reads a z that
 // another block has changed), and the brick's partials increased by r.dz and r_prev.dz.
-__global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+__global__ __launch_bounds__(512, 4) void k_nn_apply(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
                                                   const double* __restrict__ r, double* __restrict__ dz,
                                                   const double* __restrict__ rprev, double* partials,
                                                   int nparts, const DevScalars* S, const int* __restrict__ off) {
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(512) void k_nn_apply(MeshView M, NNView W, NNSetVie
     double d = 0.0;
     if (s > 0.f) {  // block-uniform
         nn_zero_tile(L.T1);
-        nn_stage_features(M, NS, B.pos, store, 1.f / s, L.T0);
+        nn_stage_features(NS, B.pos, store, 1.f / s, L.T0);
         const float* y = nn_layers(W, set, L, nullptr, &W0);
         if (m & kActive) d = double(s) / level_h(M, B.b) * double(y[q]);
     }
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(512) void k_nn_gather(const int* __restrict__ list,
 
 // Raw network output of every brick of the set (block = position): y[pos*512+q] (0 on
 // inactive cells and zero-input bricks) and s[pos], input from the residual store.
-__global__ __launch_bounds__(512) void k_nn_forward_raw(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+__global__ __launch_bounds__(512, 4) void k_nn_forward_raw(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
                                                         float* __restrict__ yout, float* __restrict__ sout) {
     __shared__ __attribute__((aligned(16))) NNShared L;
     const int pos = blockIdx.x, q = threadIdx.x;
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(512) void k_nn_forward_raw(MeshView M, NNView W, NN
     float y = 0.f;
     if (s > 0.f) {
         nn_zero_tile(L.T1);
-        nn_stage_features(M, NS, pos, store, 1.f / s, L.T0);
+        nn_stage_features(NS, pos, store, 1.f / s, L.T0);
         y = nn_layers(W, nn_set(M, b), L, nullptr)[q];
     }
     yout[static_cast<int64_t>(pos) * 512 + q] = y;
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(512) void k_nn_forward_raw(MeshView M, NNView W, NN
 // ---- training (one recorded post-V-cycle residual per pass, in the FP32 store) -------------
 // forward over the set: layer inputs to scratch[pos], dz = (s_b/h_b) y into the full-length
 // vector dfull (its other entries stay zero), s_b into sb[pos]
-__global__ __launch_bounds__(512) void k_nn_train_fwd(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
+__global__ __launch_bounds__(512, 4) void k_nn_train_fwd(MeshView M, NNView W, NNSetView NS, const float* __restrict__ store,
                                                       uint16_t* __restrict__ scratch, double* __restrict__ dfull,
                                                       float* __restrict__ sb) {
     __shared__ __attribute__((aligned(16))) NNShared L;
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(512) void k_nn_train_fwd(MeshView M, NNView W, NNSe
     double d = 0.0;
     if (s > 0.f) {
         nn_zero_tile(L.T1);
-        nn_stage_features(M, NS, B.pos, store, 1.f / s, L.T0);
+        nn_stage_features(NS, B.pos, store, 1.f / s, L.T0);
         const float* y = nn_layers(W, nn_set(M, B.b), L, scratch + static_cast<size_t>(blockIdx.x) * kNNStore);
         if (m & kActive) d = double(s) / level_h(M, B.b) * double(y[q]);
     }
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(512) void k_nn_train_bwd(MeshView M, NNView W, NNSe
     nn_zero_tile(L.T0);
     nn_zero_tile(T2);
     // layer-0 input (features with the neighbour halo) into T1
-    nn_stage_features(M, NS, B.pos, store, 1.f / s, L.T1);
+    nn_stage_features(NS, B.pos, store, 1.f / s, L.T1);
     __syncthreads();
     // output gradient into channel 0 of T0
     if (m & kActive) {

@@ -368,7 +368,7 @@ private:
     }
     // learned preconditioner
     NNView nnv() const { return NNView{nn_w_, nn_wt_, nn_p_ + kNNW}; }
-    NNSetView nn_setv() const { return NNSetView{nn_list_, nn_nbtab_}; }
+    NNSetView nn_setv() const { return NNSetView{nn_list_, nn_nbtab_, nn_masks_}; }
     void set_nn_off(bool off) { if (nn_) HIP_CHECK(hipMemsetAsync(nn_off_, off ? 1 : 0, sizeof(int), stream_)); }
     unsigned nn_grid() const { return static_cast<unsigned>(nn_list_ ? nn_nlist_ : nb_); }
     // network correction after the V-cycle's last kernel (post-V-cycle residual, network,
@@ -575,6 +575,7 @@ private:
     uint16_t* nn_scratch_ = nullptr;  // layer inputs per listed brick (training)
     int* nn_list_ = nullptr;      // the network's bricks (null: all bricks, XCD-aware order)
     int* nn_nbtab_ = nullptr;     // 27 per position: the neighbour's position in the set or -1
+    const uint8_t* nn_masks_ = nullptr;  // cell masks per position (the mesh's masks without a list)
     float* nn_rstore_ = nullptr;  // post-V-cycle residual on the set's bricks (FP32, 512 per position)
     std::vector<int> nn_host_nbr_;    // brick_nbr (the all-brick set of the forward hook)
     int* nn_all_nbtab_ = nullptr;     // forward hook: neighbour table of the all-brick set
@@ -1095,6 +1096,14 @@ void SolverImpl<R>::setup_learned(const Topology& T) {
     if (nn_cells_ == 0) throw std::invalid_argument("the learned preconditioner's bricks hold no active cell");
     nn_host_nbr_ = T.brick_nbr;
     nn_nbtab_ = upload(nn_neighbour_table(nn_host_nbr_, list, nb_));
+    if (nn_list_) {
+        std::vector<uint8_t> pm(list.size() * 512);
+        for (size_t p = 0; p < list.size(); ++p)
+            std::memcpy(&pm[p * 512], &T.mask[static_cast<size_t>(list[p]) * 512], 512);
+        nn_masks_ = upload(pm);
+    } else {
+        nn_masks_ = M_.mask;
+    }
     const int64_t slots = static_cast<int64_t>(nn_list_ ? nn_nlist_ : nb_) * 512;
     nn_rstore_ = alloc<float>(slots);
     HIP_CHECK(hipMemset(nn_rstore_, 0, slots * sizeof(float)));
@@ -2551,7 +2560,7 @@ void SolverImpl<R>::learned_forward(const double* r, float* y, float* s) {
     float* ds = reinterpret_cast<float*>(partials_);
     HIP_CHECK(hipMemcpy(dr, r, n_ * sizeof(double), hipMemcpyHostToDevice));
     LAUNCH(k_nn_gather, nb_, 512, nullptr, dr, nn_all_store_);
-    LAUNCH(k_nn_forward_raw, nb_, 512, M_, nnv(), (NNSetView{nullptr, nn_all_nbtab_}), nn_all_store_, nn_all_y_, ds);
+    LAUNCH(k_nn_forward_raw, nb_, 512, M_, nnv(), (NNSetView{nullptr, nn_all_nbtab_, M_.mask}), nn_all_store_, nn_all_y_, ds);
     HIP_CHECK(hipStreamSynchronize(stream_));
     HIP_CHECK(hipMemcpy(y, nn_all_y_, n_ * sizeof(float), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(s, ds, nb_ * sizeof(float), hipMemcpyDeviceToHost));
