@@ -82,6 +82,13 @@ __device__ __forceinline__ A row0f(const MeshView& M, const A* tile, const Compa
     const int64_t a = static_cast<int64_t>(b) * 512 + q;           \
     const uint8_t m = M.mask[a]
 
+// two cells per lane (kBL lanes per brick): cell q = t + kBL h, h = 0, 1
+#define PCFD_BRICK2_BEGIN                                                     \
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;               \
+    const int64_t base = static_cast<int64_t>(b) * 512;                       \
+    uint8_t m[2];                                                             \
+    for (int h = 0; h < 2; ++h) m[h] = M.mask[base + t + kBL * h]
+
 #define PCFD_BRICK_PROLOGUE \
     PCFD_BRICK_BEGIN;       \
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0)
@@ -94,93 +101,130 @@ __device__ __forceinline__ bool brick_done(const MeshView& M, const DevScalars* 
 // PCG (amr_pressure.cu:1267-1353 solve_orthogonal)
 // ---------------------------------------------------------------------------------------
 // r = b - A x and the ||r||^2 partial
-__global__ __launch_bounds__(512) void k_cg_residual(MeshView M, const double* __restrict__ x,
+__global__ __launch_bounds__(kBL, 8) void k_cg_residual(MeshView M, const double* __restrict__ x,
                                                      const double* __restrict__ rhs, double* __restrict__ r,
                                                      double* partials, const DevScalars* S) {
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     auto val = [&](int64_t c) -> double { return x[c]; };
-    const TileFetch<double> tf = fetch_tile<double>(M, b, val);
-    const double rb = rhs[a];
+    const TileFetch2<double> tf = fetch_tile_x2<double>(M, b, val);
+    double rb[2];
+    for (int h = 0; h < 2; ++h) rb[h] = rhs[base + t + kBL * h];
     const bool done = brick_done(M, S, b);
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<double> cf = fetch_compact<double>(M, sp, val);
+    int sp[2];
+    CompactFetch<double> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<double>(M, sp[h], val);
+    }
     if (done) return;
     __shared__ double tile[kTileN];
-    stage_tile<double>(tile, tf);
-    double ra = 0.0;
-    if (m & kActive) {
-        double dg;
-        ra = rb - row0f<double>(M, tile, cf, val, b, q, m, tf.own, dg);
+    stage_tile_x2<double>(tile, tf);
+    double sum = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        double ra = 0.0;
+        if (m[h] & kActive) {
+            double dg;
+            ra = rb[h] - row0f<double>(M, tile, cf[h], val, b, q, m[h], tf.own[h], dg);
+        }
+        r[base + q] = ra;
+        sum += ra * ra;
     }
-    r[a] = ra;
-    store_partial(partials, b, block_sum<512>(ra * ra));
+    store_partial(partials, b, block_sum<kBL>(sum));
 }
 
 // A d and the d.Ad partial (used by the measurement hook)
-__global__ __launch_bounds__(512) void k_cg_apply_dot(MeshView M, const double* __restrict__ d, double* __restrict__ Ad,
+__global__ __launch_bounds__(kBL) void k_cg_apply_dot(MeshView M, const double* __restrict__ d, double* __restrict__ Ad,
                                                       double* partials, const DevScalars* S) {
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     auto val = [&](int64_t c) -> double { return d[c]; };
-    const TileFetch<double> tf = fetch_tile<double>(M, b, val);
+    const TileFetch2<double> tf = fetch_tile_x2<double>(M, b, val);
     const bool done = brick_done(M, S, b);
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<double> cf = fetch_compact<double>(M, sp, val);
+    int sp[2];
+    CompactFetch<double> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<double>(M, sp[h], val);
+    }
     if (done) return;
     __shared__ double tile[kTileN];
-    stage_tile<double>(tile, tf);
-    double y = 0.0;
-    if (m & kActive) {
-        double dg;
-        y = row0f<double>(M, tile, cf, val, b, q, m, tf.own, dg);
+    stage_tile_x2<double>(tile, tf);
+    double sum = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        double y = 0.0;
+        if (m[h] & kActive) {
+            double dg;
+            y = row0f<double>(M, tile, cf[h], val, b, q, m[h], tf.own[h], dg);
+        }
+        Ad[base + q] = y;
+        sum += y * tf.own[h];
     }
-    Ad[a] = y;
-    store_partial(partials, b, block_sum<512>(y * tf.own));
+    store_partial(partials, b, block_sum<kBL>(sum));
 }
 
 // CG direction update fused into the operator apply: d_new = z + beta d_old (first
 // iteration: z), Ad = A d_new and the d_new.Ad partial; d is double-buffered
-__global__ __launch_bounds__(512) void k_cg_dir_apply(MeshView M, const double* __restrict__ z,
+__global__ __launch_bounds__(kBL) void k_cg_dir_apply(MeshView M, const double* __restrict__ z,
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
                                                       double* __restrict__ Ad, double* partials, const DevScalars* S,
                                                       int initial) {
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     const DevScalars& I = S[M.meta[b].inst];
     const double beta = initial ? 0.0 : I.beta;
     auto dn = [&](int64_t c) -> double { return initial ? z[c] : z[c] + beta * dold[c]; };
-    const TileFetch<double> tf = fetch_tile<double>(M, b, dn);
+    const TileFetch2<double> tf = fetch_tile_x2<double>(M, b, dn);
     const bool done = I.done != 0;
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<double> cf = fetch_compact<double>(M, sp, dn);
+    int sp[2];
+    CompactFetch<double> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<double>(M, sp[h], dn);
+    }
     if (done) return;
     __shared__ double tile[kTileN];
-    stage_tile<double>(tile, tf);
-    const double da = tf.own;
-    double y = 0.0;
-    if (m & kActive) {
-        double dg;
-        y = row0f<double>(M, tile, cf, dn, b, q, m, da, dg);
+    stage_tile_x2<double>(tile, tf);
+    double sum = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        const double da = tf.own[h];
+        double y = 0.0;
+        if (m[h] & kActive) {
+            double dg;
+            y = row0f<double>(M, tile, cf[h], dn, b, q, m[h], da, dg);
+        }
+        dnew[base + q] = da;
+        Ad[base + q] = y;
+        sum += y * da;
     }
-    dnew[a] = da;
-    Ad[a] = y;
-    store_partial(partials, b, block_sum<512>(y * da));
+    store_partial(partials, b, block_sum<kBL>(sum));
 }
 
 // x += alpha d, r -= alpha Ad (r_prev keeps the old r for flexible CG) and the ||r||^2
 // partial, one brick per workgroup with the brick's instance's alpha
-__global__ __launch_bounds__(512) void k_cg_update(MeshView M, double* __restrict__ x, double* __restrict__ r,
+__global__ __launch_bounds__(kBL) void k_cg_update(MeshView M, double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ rprev, const double* __restrict__ d,
                                                    const double* __restrict__ Ad, double* partials, const DevScalars* S) {
     const int b = brick_of_block(M.xcd_chunk);
-    const int64_t i = static_cast<int64_t>(b) * 512 + threadIdx.x;
-    const double xi = x[i], di = d[i], ri = r[i], adi = Ad[i];
+    const int64_t base = static_cast<int64_t>(b) * 512 + threadIdx.x;
+    double xi[2], di[2], ri[2], adi[2];
+    for (int h = 0; h < 2; ++h) {
+        const int64_t i = base + kBL * h;
+        xi[h] = x[i]; di[h] = d[i]; ri[h] = r[i]; adi[h] = Ad[i];
+    }
     const DevScalars& I = S[M.meta[b].inst];
     if (I.done) return;
     const double alpha = I.alpha;
-    x[i] = xi + alpha * di;
-    if (rprev) rprev[i] = ri;
-    const double rn = ri - alpha * adi;
-    r[i] = rn;
-    store_partial(partials, b, block_sum<512>(rn * rn));
+    double sum = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int64_t i = base + kBL * h;
+        x[i] = xi[h] + alpha * di[h];
+        if (rprev) rprev[i] = ri[h];
+        const double rn = ri[h] - alpha * adi[h];
+        r[i] = rn;
+        sum += rn * rn;
+    }
+    store_partial(partials, b, block_sum<kBL>(sum));
 }
 
 // Scalar finalisation of each reduction (one workgroup, partials summed in block order).
@@ -310,43 +354,57 @@ __global__ __launch_bounds__(512) void k_diag0(MeshView M, DT* __restrict__ diag
 // One sweep.  mode 0: x = w b / D (from zero); 1: x += w (b - A x) / D; 2: last sweep,
 // writes z (FP64) and the r.z (and r_prev.z) partials (two arrays of gridDim.x entries).
 template <class VT>
-__global__ __launch_bounds__(512) void k_smooth0(MeshView M, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
+__global__ __launch_bounds__(kBL) void k_smooth0(MeshView M, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
                                                  const double* __restrict__ rhs, double* __restrict__ zout,
                                                  const double* __restrict__ rprev, double omega, double* partials,
                                                  const DevScalars* S) {
     using A = typename Arith<VT>::type;
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     const bool from_zero = mode == 0 || xin == nullptr;
     auto val = [&](int64_t c) -> A { return from_zero ? A(0) : A(to_f(xin[c])); };
-    const TileFetch<A> tf = fetch_tile<A>(M, b, val);
-    const double rb = rhs[a];
-    const double rp = (mode == 2 && rprev) ? rprev[a] : 0.0;
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, val);
+    double rb[2], rp[2];
+    for (int h = 0; h < 2; ++h) {
+        rb[h] = rhs[base + t + kBL * h];
+        rp[h] = (mode == 2 && rprev) ? rprev[base + t + kBL * h] : 0.0;
+    }
     const bool done = brick_done(M, S, b);
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, val);
+    int sp[2];
+    CompactFetch<A> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<A>(M, sp[h], val);
+    }
     if (done) return;
     __shared__ A tile[kTileN];
-    stage_tile<A>(tile, tf);
-    A v = A(0);
-    if (m & kActive) {
-        const A h = A(level_h(M, b));
-        if (from_zero) {
-            v = A(omega) * A(rb) / diag_row0<A>(M, b, q, m, sp, h);
-        } else {
-            A dg;
-            const A xa = tf.own;
-            const A Ax = row0f<A>(M, tile, cf, val, b, q, m, xa, dg);
-            v = xa + A(omega) * (A(rb) - Ax) / dg;
+    stage_tile_x2<A>(tile, tf);
+    double s1 = 0.0, s2 = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        A v = A(0);
+        if (m[h] & kActive) {
+            const A hb = A(level_h(M, b));
+            if (from_zero) {
+                v = A(omega) * A(rb[h]) / diag_row0<A>(M, b, q, m[h], sp[h], hb);
+            } else {
+                A dg;
+                const A xa = tf.own[h];
+                const A Ax = row0f<A>(M, tile, cf[h], val, b, q, m[h], xa, dg);
+                v = xa + A(omega) * (A(rb[h]) - Ax) / dg;
+            }
         }
+        if (mode != 2) {
+            xout[base + q] = from_f<VT>(v);
+            continue;
+        }
+        const double z = double(v);
+        zout[base + q] = z;
+        s1 += rb[h] * z;
+        s2 += rp[h] * z;
     }
-    if (mode != 2) {
-        xout[a] = from_f<VT>(v);
-        return;
-    }
-    const double z = double(v);
-    zout[a] = z;
-    const double s1 = block_sum<512>(rb * z);
-    const double s2 = rprev ? block_sum<512>(rp * z) : 0.0;
+    if (mode != 2) return;
+    s1 = block_sum<kBL>(s1);
+    s2 = rprev ? block_sum<kBL>(s2) : 0.0;
     if (threadIdx.x == 0) {
         partials[b] = s1;
         partials[M.nbricks + b] = s2;
@@ -378,29 +436,37 @@ __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restr
 
 // two pre-smoothing sweeps from x = 0:  x1 = w1 r / D,  x2 = x1 + w2 (r - A x1) / D
 template <class VT>
-__global__ __launch_bounds__(512, 8) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
+__global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 6 : 8) void k_vc_pre2(MeshView M, const double* __restrict__ rhs,
                                                  const typename Arith<VT>::type* __restrict__ diag, VT* __restrict__ xout,
                                                  double omega, double omega2, const DevScalars* S) {
     using A = typename Arith<VT>::type;
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     const A w = A(omega), w2 = A(omega2);
     auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rhs[c]) / diag[c]); };
-    const TileFetch<A> tf = fetch_tile<A>(M, b, x1);
-    const double rb = rhs[a];
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, x1);
+    double rb[2];
+    for (int h = 0; h < 2; ++h) rb[h] = rhs[base + t + kBL * h];
     const bool done = brick_done(M, S, b);
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, x1);
+    int sp[2];
+    CompactFetch<A> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<A>(M, sp[h], x1);
+    }
     if (done) return;
     __shared__ A tile[kTileN];
-    stage_tile<A>(tile, tf);
-    A v = A(0);
-    if (m & kActive) {
-        A dg;
-        const A xa = tf.own;
-        const A Ax = row0f<A>(M, tile, cf, x1, b, q, m, xa, dg);
-        v = xa + w2 * (A(rb) - Ax) / dg;
+    stage_tile_x2<A>(tile, tf);
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        A v = A(0);
+        if (m[h] & kActive) {
+            A dg;
+            const A xa = tf.own[h];
+            const A Ax = row0f<A>(M, tile, cf[h], x1, b, q, m[h], xa, dg);
+            v = xa + w2 * (A(rb[h]) - Ax) / dg;
+        }
+        xout[base + q] = from_f<VT>(v);
     }
-    xout[a] = from_f<VT>(v);
 }
 
 // The CG update fused into the first V-cycle pass: x += alpha d, r = rin - alpha Ad (written
@@ -409,87 +475,117 @@ __global__ __launch_bounds__(512, 8) void k_vc_pre2(MeshView M, const double* __
 // compact-neighbour values are formed on the fly with the same arithmetic.  The results are
 // those of k_cg_update followed by k_vc_pre2.
 template <class VT>
-__global__ __launch_bounds__(512, 8) void k_vc_update_pre2(MeshView M, double* __restrict__ x, const double* __restrict__ rin,
+__global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 5 : 8) void k_vc_update_pre2(MeshView M, double* __restrict__ x, const double* __restrict__ rin,
                                                         double* __restrict__ rout, const double* __restrict__ d,
                                                         const double* __restrict__ Ad,
                                                         const typename Arith<VT>::type* __restrict__ diag,
                                                         VT* __restrict__ xout, double omega, double omega2,
                                                         double* partials, const DevScalars* S) {
     using A = typename Arith<VT>::type;
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     const DevScalars& I = S[M.meta[b].inst];
     const double alpha = I.alpha;
     const A w = A(omega), w2 = A(omega2);
     auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rin[c] - alpha * Ad[c]) / diag[c]); };
-    const TileFetch<A> tf = fetch_tile<A>(M, b, x1);
-    const double xi = x[a], di = d[a], ri = rin[a], adi = Ad[a];
-    const bool done = I.done != 0;
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, x1);
-    if (done) return;
-    const double rn = ri - alpha * adi;
-    x[a] = xi + alpha * di;
-    rout[a] = rn;
-    __shared__ A tile[kTileN];
-    stage_tile<A>(tile, tf);
-    A v = A(0);
-    if (m & kActive) {
-        A dg;
-        const A xa = tf.own;
-        const A Ax = row0f<A>(M, tile, cf, x1, b, q, m, xa, dg);
-        v = xa + w2 * (A(rn) - Ax) / dg;
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, x1);
+    double xi[2], di[2], ri[2], adi[2];
+    for (int h = 0; h < 2; ++h) {
+        const int64_t a = base + t + kBL * h;
+        xi[h] = x[a]; di[h] = d[a]; ri[h] = rin[a]; adi[h] = Ad[a];
     }
-    xout[a] = from_f<VT>(v);
-    store_partial(partials, b, block_sum<512>(rn * rn));
+    const bool done = I.done != 0;
+    int sp[2];
+    CompactFetch<A> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<A>(M, sp[h], x1);
+    }
+    if (done) return;
+    double rn[2], sum = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int64_t a = base + t + kBL * h;
+        rn[h] = ri[h] - alpha * adi[h];
+        x[a] = xi[h] + alpha * di[h];
+        rout[a] = rn[h];
+        sum += rn[h] * rn[h];
+    }
+    __shared__ A tile[kTileN];
+    stage_tile_x2<A>(tile, tf);
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        A v = A(0);
+        if (m[h] & kActive) {
+            A dg;
+            const A xa = tf.own[h];
+            const A Ax = row0f<A>(M, tile, cf[h], x1, b, q, m[h], xa, dg);
+            v = xa + w2 * (A(rn[h]) - Ax) / dg;
+        }
+        xout[base + q] = from_f<VT>(v);
+    }
+    store_partial(partials, b, block_sum<kBL>(sum));
 }
 
 // residual r - A x summed over each 2x2x2 group into its level-1 parent (children of one
 // parent always lie in one brick); cells that were not merged map one-to-one
 template <class VT>
-__global__ __launch_bounds__(512) void k_vc_restrict0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
+__global__ __launch_bounds__(kBL) void k_vc_restrict0(MeshView M, const VT* __restrict__ x, const double* __restrict__ rhs,
                                                       const int* __restrict__ parent0, VT* __restrict__ coarse,
                                                       const DevScalars* S) {
     using A = typename Arith<VT>::type;
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
-    const TileFetch<A> tf = fetch_tile<A>(M, b, val);
-    const double rb = rhs[a];
-    const int pa = parent0[a];
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, val);
+    double rb[2];
+    int pa[2];
+    for (int h = 0; h < 2; ++h) {
+        rb[h] = rhs[base + t + kBL * h];
+        pa[h] = parent0[base + t + kBL * h];
+    }
     const bool done = brick_done(M, S, b);
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, val);
+    int sp[2];
+    CompactFetch<A> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<A>(M, sp[h], val);
+    }
     if (done) return;
     __shared__ A tile[kTileN];
     __shared__ A sres[512];
     __shared__ int spar[512];
-    stage_tile<A>(tile, tf);
-    A v = A(0);
-    if (m & kActive) {
-        A dg;
-        v = A(rb) - row0f<A>(M, tile, cf, val, b, q, m, tf.own, dg);
+    stage_tile_x2<A>(tile, tf);
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        A v = A(0);
+        if (m[h] & kActive) {
+            A dg;
+            v = A(rb[h]) - row0f<A>(M, tile, cf[h], val, b, q, m[h], tf.own[h], dg);
+        }
+        sres[q] = round_vt<VT>(v);
+        spar[q] = pa[h];
     }
-    sres[q] = round_vt<VT>(v);
-    spar[q] = pa;
     __syncthreads();
-    if ((q & 0x49) != 0) return;  // leaders have even i, j, k
-    const int kids[8] = {q, q + 1, q + 8, q + 9, q + 64, q + 65, q + 72, q + 73};
-    int P = -1;
-    bool shared = true;
-    for (int t = 0; t < 8; ++t) {
-        const int pc = spar[kids[t]];
-        if (pc < 0) continue;
-        if (P < 0) P = pc;
-        else if (pc != P) shared = false;
-    }
-    if (P < 0) return;
-    if (shared) {
-        A s = A(0);
-        for (int t = 0; t < 8; ++t) if (spar[kids[t]] >= 0) s += sres[kids[t]];
-        coarse[P] = from_f<VT>(s);
-    } else {
-        for (int t = 0; t < 8; ++t) {
-            const int pc = spar[kids[t]];
-            if (pc >= 0) coarse[pc] = from_f<VT>(A(0) + sres[kids[t]]);
+    if ((t & 0x49) != 0) return;  // leaders have even i, j, k (lane t leads cells t and t + 256)
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        const int kids[8] = {q, q + 1, q + 8, q + 9, q + 64, q + 65, q + 72, q + 73};
+        int P = -1;
+        bool shared = true;
+        for (int k = 0; k < 8; ++k) {
+            const int pc = spar[kids[k]];
+            if (pc < 0) continue;
+            if (P < 0) P = pc;
+            else if (pc != P) shared = false;
+        }
+        if (P < 0) continue;
+        if (shared) {
+            A sum = A(0);
+            for (int k = 0; k < 8; ++k) if (spar[kids[k]] >= 0) sum += sres[kids[k]];
+            coarse[P] = from_f<VT>(sum);
+        } else {
+            for (int k = 0; k < 8; ++k) {
+                const int pc = spar[kids[k]];
+                if (pc >= 0) coarse[pc] = from_f<VT>(A(0) + sres[kids[k]]);
+            }
         }
     }
 }
@@ -497,42 +593,56 @@ __global__ __launch_bounds__(512) void k_vc_restrict0(MeshView M, const VT* __re
 // prolongation x += e[parent] fused with one post-smoothing sweep; `last` writes z (FP64)
 // and the r.z / r_prev.z partials like k_smooth0 mode 2
 template <class VT>
-__global__ __launch_bounds__(512) void k_vc_post_prolong(MeshView M, int last, const VT* __restrict__ x,
+__global__ __launch_bounds__(kBL) void k_vc_post_prolong(MeshView M, int last, const VT* __restrict__ x,
                                                          const VT* __restrict__ e1, const int* __restrict__ parent0,
                                                          const double* __restrict__ rhs, VT* __restrict__ xout,
                                                          double* __restrict__ zout, const double* __restrict__ rprev,
                                                          double omega, double* partials, const DevScalars* S) {
     using A = typename Arith<VT>::type;
-    PCFD_BRICK_BEGIN;
+    PCFD_BRICK2_BEGIN;
     auto y = [&](int64_t c) -> A {
         const int pc = parent0[c];
         const A xc = A(to_f(x[c]));
         return pc >= 0 ? round_vt<VT>(xc + A(to_f(e1[pc]))) : xc;
     };
-    const TileFetch<A> tf = fetch_tile<A>(M, b, y);
-    const double rb = rhs[a];
-    const double rp = (last && rprev) ? rprev[a] : 0.0;
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, y);
+    double rb[2], rp[2];
+    for (int h = 0; h < 2; ++h) {
+        rb[h] = rhs[base + t + kBL * h];
+        rp[h] = (last && rprev) ? rprev[base + t + kBL * h] : 0.0;
+    }
     const bool done = brick_done(M, S, b);
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    const CompactFetch<A> cf = fetch_compact<A>(M, sp, y);
+    int sp[2];
+    CompactFetch<A> cf[2];
+    for (int h = 0; h < 2; ++h) {
+        sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
+        cf[h] = fetch_compact<A>(M, sp[h], y);
+    }
     if (done) return;
     __shared__ A tile[kTileN];
-    stage_tile<A>(tile, tf);
-    A v = A(0);
-    if (m & kActive) {
-        A dg;
-        const A ya = tf.own;
-        const A Ay = row0f<A>(M, tile, cf, y, b, q, m, ya, dg);
-        v = ya + A(omega) * (A(rb) - Ay) / dg;
+    stage_tile_x2<A>(tile, tf);
+    double s1 = 0.0, s2 = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + kBL * h;
+        A v = A(0);
+        if (m[h] & kActive) {
+            A dg;
+            const A ya = tf.own[h];
+            const A Ay = row0f<A>(M, tile, cf[h], y, b, q, m[h], ya, dg);
+            v = ya + A(omega) * (A(rb[h]) - Ay) / dg;
+        }
+        if (!last) {
+            xout[base + q] = from_f<VT>(v);
+            continue;
+        }
+        const double z = double(v);
+        zout[base + q] = z;
+        s1 += rb[h] * z;
+        s2 += rp[h] * z;
     }
-    if (!last) {
-        xout[a] = from_f<VT>(v);
-        return;
-    }
-    const double z = double(v);
-    zout[a] = z;
-    const double s1 = block_sum<512>(rb * z);
-    const double s2 = rprev ? block_sum<512>(rp * z) : 0.0;
+    if (!last) return;
+    s1 = block_sum<kBL>(s1);
+    s2 = rprev ? block_sum<kBL>(s2) : 0.0;
     if (threadIdx.x == 0) {
         partials[b] = s1;
         partials[M.nbricks + b] = s2;

@@ -217,7 +217,24 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
                                                 R* __restrict__ oz, const DevScalars* S, const R* __restrict__ G) {
     const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
+    // every independent load before the tile barrier: the mask, the cell's +x/+y/+z face
+    // velocities and those of its -x/-y/-z neighbours (their +faces), the instance's timestep
+    // and cell size from the brick record
     const uint8_t m = M.mask[a];
+    R fo[3], fn[3];
+    for (int axis = 0; axis < 3; ++axis) {
+        const int stride = 1 << (3 * axis);
+        fo[axis] = F.f[axis][a];
+        int64_t o = a - stride;
+        if (axis_coord(q, axis) == 0) {
+            const int nb = M.meta[b].nbr[2 * axis];
+            o = nb >= 0 ? static_cast<int64_t>(nb) * 512 + q + 7 * stride : -1;
+        }
+        fn[axis] = o >= 0 ? F.f[axis][o] : R(0);
+    }
+    const double dt_inst = S[M.meta[b].inst].dt;
+    const R h = R(M.meta[b].h), A = h * h;
+    const uint16_t bf = M.meta[b].flags;
     const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
     __shared__ R vt[3][kTile2];
     __shared__ uint8_t mt[kTile2];
@@ -228,8 +245,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
         for (int c = 0; c < 3; ++c) out[c][a] = va[c];
         return;
     }
-    const R dt = R(S[M.inst[b]].dt);
-    const R h = R(cell_h(M, b)), A = h * h;
+    const R dt = R(dt_inst);
     R d[3] = {R(0), R(0), R(0)};
     const int ta = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
     for (int axis = 0; axis < 3; ++axis) {
@@ -240,7 +256,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             const bool has_l = (m & (kMX << axis)) != 0, has_u = (mb & (kPX << axis)) != 0;
             const int l = has_l ? ta - st : ta;
             const int u = has_u ? b1 + st : b1;
-            const R un = F.f[axis][a];
+            const R un = fo[axis];
             const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
             for (int c = 0; c < 3; ++c) {
                 const R fv = muscl_value(swept, pred, va[c], vt[c][b1], vt[c][l], has_l, vt[c][u], has_u);
@@ -253,7 +269,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             const bool has_l = (m0 & (kMX << axis)) != 0, has_u = (m & (kPX << axis)) != 0;
             const int l = has_l ? b0 - st : b0;
             const int u = has_u ? ta + st : ta;
-            const R un = F.f[axis][neighbor_dof(M, b, q, axis, -1)];
+            const R un = fn[axis];
             const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
             for (int c = 0; c < 3; ++c) {
                 const R fv = muscl_value(swept, pred, vt[c][b0], va[c], vt[c][l], has_l, vt[c][u], has_u);
@@ -261,14 +277,13 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             }
         }
     }
-    const uint8_t bf = M.brick_flags[b];
     if ((q & 7) == 0 && (bf & kXMIN)) {
         const R U = R(M.speed), swept = dt * A * U;
         if (swept >= R(0)) d[0] += swept * U;
         else for (int c = 0; c < 3; ++c) d[c] += swept * va[c];
     }
     if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) {
-        const R swept = dt * A * F.f[0][a];
+        const R swept = dt * A * fo[0];
         if (swept >= R(0)) for (int c = 0; c < 3; ++c) d[c] -= swept * va[c];
         else d[0] -= swept * R(M.speed);
     }

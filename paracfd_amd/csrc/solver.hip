@@ -291,7 +291,7 @@ private:
     // One PCG iteration: d_new = z + beta d_old with A d_new fused, alpha, x/r update,
     // preconditioner, beta.  With the fused V-cycle the x/r update runs inside its first pass.
     void cg_iteration(bool initial, double* dold, double* dnew) {
-        probed(5, [&] { LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0); });
+        probed(5, [&] { LAUNCH(k_cg_dir_apply, nb_, kBL, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0); });
         finalize(FIN_APPLY, nb_);
         if (update_fused()) {
             double* rin = r_;
@@ -300,7 +300,7 @@ private:
             precondition_dispatch(false, dnew, rin);
             return;
         }
-        LAUNCH(k_cg_update, nb_, 512, M_, p_, r_, rprev_, dnew, Ad_, partials_, S_);
+        LAUNCH(k_cg_update, nb_, kBL, M_, p_, r_, rprev_, dnew, Ad_, partials_, S_);
         finalize(FIN_UPDATE, nb_);
         precondition_dispatch(false);
     }
@@ -1259,7 +1259,7 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
     const int ini = initial ? 1 : 0;
     const int fin = ini ? FIN_RZ_INITIAL : FIN_RZ;
     if (opt_.preconditioner == PCFD_PC_JACOBI) {
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 2, static_cast<const VT*>(nullptr), static_cast<VT*>(nullptr), r_, z_, rprev_, 1.0,
+        LAUNCH(k_smooth0<VT>, nb_, kBL, M_, 2, static_cast<const VT*>(nullptr), static_cast<VT*>(nullptr), r_, z_, rprev_, 1.0,
                partials_, S_);
         finalize(fin, nb_);
         return;
@@ -1272,29 +1272,29 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
         using A = typename Arith<VT>::type;
         if (upd_d) {
             probed(6, [&] {
-                LAUNCH(k_vc_update_pre2<VT>, nb_, 512, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur,
+                LAUNCH(k_vc_update_pre2<VT>, nb_, kBL, M_, p_, upd_rin, r_, upd_d, Ad_, static_cast<const A*>(diag_), cur,
                        w[0], w[1], partials_, S_);
             });
             finalize(FIN_UPDATE, nb_);
         } else {
-            LAUNCH(k_vc_pre2<VT>, nb_, 512, M_, r_, static_cast<const A*>(diag_), cur, w[0], w[1], S_);
+            LAUNCH(k_vc_pre2<VT>, nb_, kBL, M_, r_, static_cast<const A*>(diag_), cur, w[0], w[1], S_);
         }
-        LAUNCH(k_vc_restrict0<VT>, nb_, 512, M_, cur, r_, parent0_, static_cast<VT*>(lv_[1].B), S_);
+        LAUNCH(k_vc_restrict0<VT>, nb_, kBL, M_, cur, r_, parent0_, static_cast<VT*>(lv_[1].B), S_);
         VT* xc = coarse_cycle<VT>(1);
         const int last = opt_.post_smooth == 1 ? 1 : 0;
-        LAUNCH(k_vc_post_prolong<VT>, nb_, 512, M_, last, cur, xc, parent0_, r_, oth, z_, rprev_, w[0], partials_, S_);
+        LAUNCH(k_vc_post_prolong<VT>, nb_, kBL, M_, last, cur, xc, parent0_, r_, oth, z_, rprev_, w[0], partials_, S_);
         std::swap(cur, oth);
         for (int s = 1; s < opt_.post_smooth; ++s) {
             const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-            probed(1, [&] { LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_); });
+            probed(1, [&] { LAUNCH(k_smooth0<VT>, nb_, kBL, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_); });
             std::swap(cur, oth);
         }
         pc_tail(fin);
         return;
     }
-    LAUNCH(k_smooth0<VT>, nb_, 512, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w[0], partials_, S_);
+    LAUNCH(k_smooth0<VT>, nb_, kBL, M_, 0, static_cast<const VT*>(nullptr), cur, r_, z_, rprev_, w[0], partials_, S_);
     for (int s = 1; s < opt_.pre_smooth; ++s) {
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, 1, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
+        LAUNCH(k_smooth0<VT>, nb_, kBL, M_, 1, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
         std::swap(cur, oth);
     }
     if (lv_.size() > 1) {
@@ -1306,7 +1306,7 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
     }
     for (int s = 0; s < opt_.post_smooth; ++s) {
         const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-        LAUNCH(k_smooth0<VT>, nb_, 512, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
+        LAUNCH(k_smooth0<VT>, nb_, kBL, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
         std::swap(cur, oth);
     }
     pc_tail(fin);
@@ -1330,7 +1330,7 @@ void SolverImpl<R>::precondition_dispatch(bool initial, const double* upd_d, con
 template <class R>
 bool SolverImpl<R>::pcg_loop(bool watch_stall) {
     reset_residual_roles();
-    LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+    LAUNCH(k_cg_residual, nb_, kBL, M_, p_, rhs_, r_, partials_, S_);
     finalize(FIN_RESID, nb_);
     precondition_dispatch(true);
     // first iteration (beta = 0) outside the replayed runs; d is double-buffered
@@ -1457,7 +1457,7 @@ void SolverImpl<R>::enqueue_fgmres_direction(int j) {
     }
     r_ = r_save;
     z_ = z_save;
-    LAUNCH(k_cg_dir_apply, nb_, 512, M_, zj, zj, dbuf_[0], w, partials_, S_, 1);
+    LAUNCH(k_cg_dir_apply, nb_, kBL, M_, zj, zj, dbuf_[0], w, partials_, S_, 1);
     if (NV_.nf > 0) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, zj);
         LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, zj, w, 1.0);
@@ -1492,7 +1492,7 @@ void SolverImpl<R>::fgmres_direction(int j) {
 template <class R>
 void SolverImpl<R>::full_residual(double* r, const double* x) {
     if (!x) x = p_;
-    LAUNCH(k_cg_residual, nb_, 512, M_, x, rhs_, r, partials_, S_);
+    LAUNCH(k_cg_residual, nb_, kBL, M_, x, rhs_, r, partials_, S_);
     if (NV_.nf > 0) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, x);
         LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, x, r, -1.0);
@@ -1501,7 +1501,7 @@ void SolverImpl<R>::full_residual(double* r, const double* x) {
 // y = A x with the full (non-orthogonal) operator
 template <class R>
 void SolverImpl<R>::apply_full(const double* x, double* y) {
-    LAUNCH(k_cg_dir_apply, nb_, 512, M_, x, x, dbuf_[0], y, partials_, S_, 1);
+    LAUNCH(k_cg_dir_apply, nb_, kBL, M_, x, x, dbuf_[0], y, partials_, S_, 1);
     if (NV_.nf > 0) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, x);
         LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, x, y, 1.0);
@@ -2080,7 +2080,7 @@ void SolverImpl<R>::mg_primitive(int op, int level, const double* x, const doubl
         double* cur = dx;
         double* oth = dt;
         for (int s = 0; s < sweeps; ++s) {
-            if (level == 0) LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, cur, oth, db, z_, static_cast<double*>(nullptr), omega, partials_, S_);
+            if (level == 0) LAUNCH(k_smooth0<double>, nb_, kBL, M_, 1, cur, oth, db, z_, static_cast<double*>(nullptr), omega, partials_, S_);
             else LAUNCH((k_smooth_csr<double, double>), blocks_for(n, 256), 256, lv_[level].csr64, 1, cur, oth, db, omega);
             std::swap(cur, oth);
         }
@@ -2118,28 +2118,28 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
     const int vt = opt_.vcycle_precision == PCFD_FP64 ? 8 : (opt_.vcycle_precision == PCFD_FP32 ? 4 : 2);
     auto launch = [&] {
         switch (kernel) {
-            case 0: LAUNCH(k_cg_apply_dot, nb_, 512, M_, d_, Ad_, partials_, S_); break;
+            case 0: LAUNCH(k_cg_apply_dot, nb_, kBL, M_, d_, Ad_, partials_, S_); break;
             case 1:
                 if (opt_.vcycle_precision == PCFD_FP32)
-                    LAUNCH(k_smooth0<float>, nb_, 512, M_, 1, static_cast<float*>(vc0_[0]), static_cast<float*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
+                    LAUNCH(k_smooth0<float>, nb_, kBL, M_, 1, static_cast<float*>(vc0_[0]), static_cast<float*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 else if (opt_.vcycle_precision == PCFD_BF16)
-                    LAUNCH(k_smooth0<bf16>, nb_, 512, M_, 1, static_cast<bf16*>(vc0_[0]), static_cast<bf16*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
+                    LAUNCH(k_smooth0<bf16>, nb_, kBL, M_, 1, static_cast<bf16*>(vc0_[0]), static_cast<bf16*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 else
-                    LAUNCH(k_smooth0<double>, nb_, 512, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
+                    LAUNCH(k_smooth0<double>, nb_, kBL, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 break;
             case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); break;
             case 3: precondition_dispatch(true); break;
             case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_setv(), nn_rstore_, r_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
-            case 5: LAUNCH(k_cg_dir_apply, nb_, 512, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;
+            case 5: LAUNCH(k_cg_dir_apply, nb_, kBL, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;
             case 6:  // alpha = 0 and x = z (scratch): the pressure and the residual are untouched
                 if (opt_.vcycle_precision == PCFD_FP32)
-                    LAUNCH(k_vc_update_pre2<float>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
+                    LAUNCH(k_vc_update_pre2<float>, nb_, kBL, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
                            static_cast<const float*>(diag_), static_cast<float*>(vc0_[0]), 0.6, 0.6, partials_, S_);
                 else if (opt_.vcycle_precision == PCFD_BF16)
-                    LAUNCH(k_vc_update_pre2<bf16>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
+                    LAUNCH(k_vc_update_pre2<bf16>, nb_, kBL, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
                            static_cast<const float*>(diag_), static_cast<bf16*>(vc0_[0]), 0.6, 0.6, partials_, S_);
                 else
-                    LAUNCH(k_vc_update_pre2<double>, nb_, 512, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
+                    LAUNCH(k_vc_update_pre2<double>, nb_, kBL, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
                            static_cast<const double*>(diag_), static_cast<double*>(vc0_[0]), 0.6, 0.6, partials_, S_);
                 break;
             default: throw std::invalid_argument("unknown kernel id");
@@ -2148,7 +2148,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
     if (kernel == 4) {
         if (!nn_) throw std::invalid_argument("kernel 4 needs the learned preconditioner");
         // a residual with every brick non-zero so that each selected brick runs the network
-        LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+        LAUNCH(k_cg_residual, nb_, kBL, M_, p_, rhs_, r_, partials_, S_);
         LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, nullptr, nullptr);
         set_nn_off(false);
     }
@@ -2417,7 +2417,7 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
         // warm start from the current pressure, as the step's projection does
         set_nn_off(true);
         reset_residual_roles();
-        LAUNCH(k_cg_residual, nb_, 512, M_, p_, rhs_, r_, partials_, S_);
+        LAUNCH(k_cg_residual, nb_, kBL, M_, p_, rhs_, r_, partials_, S_);
         finalize(FIN_RESID, nb_);
         precondition_dispatch(true);
         sync_scalars();
