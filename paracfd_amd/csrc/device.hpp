@@ -431,4 +431,29 @@ __device__ __forceinline__ double gather_partials(const double* partials, int n,
     return take_max ? block_max<BLOCK>(acc) : block_sum<BLOCK>(acc);
 }
 
+// The same for NA arrays at once (a, a + stride, ...), U slots of each per round: one memory
+// latency per round for all of them (a brick-level grid of 40 k slots is one round).  Thread t
+// folds slots t, t + BLOCK, ... of each array in that order, as gather_partials does, so the
+// sums are the same bit for bit.
+template <int BLOCK, int NA, int U>
+__device__ __forceinline__ void gather_partials_n(const double* a, int stride, int n, bool take_max, double* out) {
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    for (int base = threadIdx.x; base < n; base += U * BLOCK) {
+        double v[NA][U];
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[k][u] = base + u * BLOCK < n ? a[k * stride + base + u * BLOCK] : 0.0;
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (base + u * BLOCK < n) acc[k] = take_max ? fmax(acc[k], v[k][u]) : acc[k] + v[k][u];
+    }
+#pragma unroll
+    for (int k = 0; k < NA; ++k) out[k] = take_max ? block_max<BLOCK>(acc[k]) : block_sum<BLOCK>(acc[k]);
+}
+
 }  // namespace pcfd

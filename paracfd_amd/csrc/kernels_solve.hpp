@@ -239,7 +239,8 @@ enum FinalizeOp : int {
     FIN_RZ = 7,       // r.z (and r_prev.z) -> beta, rz
     FIN_RZ_INITIAL = 8,
     FIN_LOADS = 9,    // twelve load sums (pressure and wall-shear force and moment)
-    FIN_DIFFRATE = 10 // max explicit diffusion rate (Smagorinsky substeps)
+    FIN_DIFFRATE = 10, // max explicit diffusion rate (Smagorinsky substeps)
+    FIN_RATE_VMAX = 11 // FIN_RATE and FIN_VMAX from two partial arrays
 };
 
 // The instance's scalars that an op reads (done flag, rz, bnorm, iteration count) are loaded
@@ -251,15 +252,23 @@ __device__ __forceinline__ int finalize_one(int op, const double* __restrict__ p
     int done = S->done;
     const double rz = S->rz, bnorm = S->bnorm;
     const int iter = S->iter;
-    const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE;
-    const int arrays = op == FIN_LOADS ? 12 : ((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible ? 2 : 1);
+    const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE || op == FIN_RATE_VMAX;
+    const int arrays = op == FIN_LOADS ? 12 : (((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible) || op == FIN_RATE_VMAX ? 2 : 1);
     double t[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts + beg, end - beg, take_max);
+    if (arrays == 2) gather_partials_n<1024, 2, 20>(partials + beg, nparts, end - beg, take_max, t);
+    else if (arrays == 1) gather_partials_n<1024, 1, 40>(partials + beg, nparts, end - beg, take_max, t);
+    else
+        for (int c = 0; c < arrays; ++c) t[c] = gather_partials<1024>(partials + c * nparts + beg, end - beg, take_max);
     if (threadIdx.x != 0 || (krylov && done)) return done;
     switch (op) {
         case FIN_RATE:
             S->rate = t[0];
             S->dt = cfl / fmax(rate_floor, t[0]);
+            break;
+        case FIN_RATE_VMAX:
+            S->rate = t[0];
+            S->dt = cfl / fmax(rate_floor, t[0]);
+            S->vmax = t[1];
             break;
         case FIN_BNORM:
             S->bnorm = sqrt(t[0]);

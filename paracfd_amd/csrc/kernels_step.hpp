@@ -35,6 +35,61 @@ __device__ __forceinline__ R muscl_value(R swept, R pred, R va, R vb, R vl, bool
 
 __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
 
+// dof across face (axis, sign) of cell q of brick b from the brick record (a scalar load), -1
+// where the brick has no same-level neighbour on that side
+__device__ __forceinline__ int64_t face_dof(const MeshView& M, int b, int q, int axis, int sign) {
+    const int stride = 1 << (3 * axis);
+    const int c = ((q >> (3 * axis)) & 7) + sign;
+    if (c >= 0 && c < 8) return static_cast<int64_t>(b) * 512 + q + sign * stride;
+    const int nb = M.meta[b].nbr[2 * axis + (sign > 0 ? 1 : 0)];
+    return nb >= 0 ? static_cast<int64_t>(nb) * 512 + q - sign * 7 * stride : -1;
+}
+
+// the six face velocities of cell q: its +x/+y/+z faces and the +faces of its -x/-y/-z
+// neighbours (0 where there is no neighbour brick); loaded without waiting for the mask
+template <class R>
+struct FaceVel {
+    R up[3], dn[3];
+};
+template <class R>
+__device__ __forceinline__ FaceVel<R> load_faces(const MeshView& M, const FieldsView<R>& F, int b, int q) {
+    FaceVel<R> f;
+    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        f.up[axis] = F.f[axis][a];
+        const int64_t o = face_dof(M, b, q, axis, -1);
+        f.dn[axis] = o >= 0 ? F.f[axis][o] : R(0);
+    }
+    return f;
+}
+
+// the signed compact-edge fluxes area * fe of a special cell's incidences, summed in
+// incidence order (outgoing positive): groups of four, the edge ids of a group loaded
+// together, then their areas and face velocities; `op(acc, flux)` folds one term
+template <class R, class Op>
+__device__ __forceinline__ void fold_edge_fluxes(const MeshView& M, const FieldsView<R>& F, int sp, double& acc, Op op) {
+    const int k0 = M.special_inc_off[sp], k1 = M.special_inc_off[sp + 1];
+    for (int k = k0; k < k1; k += 4) {
+        int raw[4];
+        double ar[4], fv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) raw[j] = k + j < k1 ? M.special_inc[k + j] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = raw[j] >= 0 ? raw[j] : ~raw[j];
+            ar[j] = k + j < k1 ? M.earea[e] : 0.0;
+            fv[j] = k + j < k1 ? double(F.fe[e]) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1) {
+                const double flux = ar[j] * fv[j];
+                acc = op(acc, raw[j] >= 0 ? flux : -flux);
+            }
+    }
+}
+
 // A member of a momentum redistribution group hands its increment to the group (k_redistribute)
 // instead of applying it (apply_or_accumulate_redistributed_momentum, amr_advection.cu:118-129).
 __device__ __forceinline__ bool redistributed(const MeshView& M, int sp) {
@@ -84,15 +139,30 @@ __global__ __launch_bounds__(256) void k_grad_v(MeshView M, FieldsView<R> F, Sma
     const int sp = blockIdx.x * blockDim.x + threadIdx.x;
     if (sp >= M.nspecial) return;
     const int64_t a = M.special_dof[sp];
+    const int k0 = G.off[sp], k1 = G.off[sp + 1];
     const double va[3] = {double(F.v[0][a]), double(F.v[1][a]), double(F.v[2][a])};
     double rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = G.off[sp]; k < G.off[sp + 1]; ++k) {
-        const int64_t o = G.nbr[k];
-        const double w0 = G.wd[3 * k], w1 = G.wd[3 * k + 1], w2 = G.wd[3 * k + 2];
-        for (int c = 0; c < 3; ++c) {
-            const double dv = double(F.v[c][o]) - va[c];
-            rhs[3 * c] += w0 * dv; rhs[3 * c + 1] += w1 * dv; rhs[3 * c + 2] += w2 * dv;
+    // stencil entries in groups of four: indices and weights of a group loaded together,
+    // then the neighbours' values, summed in stencil order
+    for (int k = k0; k < k1; k += 4) {
+        int64_t o[4];
+        double w[4][3], v[4][3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool ok = k + j < k1;
+            o[j] = ok ? G.nbr[k + j] : a;
+            for (int r = 0; r < 3; ++r) w[j][r] = ok ? G.wd[3 * (k + j) + r] : 0.0;
         }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 3; ++c) v[j][c] = double(F.v[c][o[j]]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1)
+                for (int c = 0; c < 3; ++c) {
+                    const double dv = v[j][c] - va[c];
+                    rhs[3 * c] += w[j][0] * dv; rhs[3 * c + 1] += w[j][1] * dv; rhs[3 * c + 2] += w[j][2] * dv;
+                }
     }
     const double* I = G.inv + 9 * static_cast<int64_t>(sp);
     R* out = g + 9 * static_cast<int64_t>(sp);
@@ -108,11 +178,26 @@ __global__ __launch_bounds__(256) void k_grad_p(MeshView M, SmagView G, const do
     const int sp = blockIdx.x * blockDim.x + threadIdx.x;
     if (sp >= M.nspecial) return;
     const int64_t a = M.special_dof[sp];
+    const int k0 = G.off[sp], k1 = G.off[sp + 1];
     const double pa = p[a];
     double rhs[3] = {0, 0, 0};
-    for (int k = G.off[sp]; k < G.off[sp + 1]; ++k) {
-        const double dp = p[G.nbr[k]] - pa;
-        rhs[0] += G.wd[3 * k] * dp; rhs[1] += G.wd[3 * k + 1] * dp; rhs[2] += G.wd[3 * k + 2] * dp;
+    for (int k = k0; k < k1; k += 4) {  // groups of four, as k_grad_v
+        int64_t o[4];
+        double w[4][3], v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool ok = k + j < k1;
+            o[j] = ok ? G.nbr[k + j] : a;
+            for (int r = 0; r < 3; ++r) w[j][r] = ok ? G.wd[3 * (k + j) + r] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = p[o[j]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k + j < k1) {
+                const double dp = v[j] - pa;
+                rhs[0] += w[j][0] * dp; rhs[1] += w[j][1] * dp; rhs[2] += w[j][2] * dp;
+            }
     }
     const double* I = G.inv + 9 * static_cast<int64_t>(sp);
     for (int r = 0; r < 3; ++r)
@@ -122,36 +207,53 @@ __global__ __launch_bounds__(256) void k_grad_p(MeshView M, SmagView G, const do
 // ---------------------------------------------------------------------------------------
 // CFL: max over cells of sum(max(outflow, 0)) / V  (amr_advection.cu:169-186) -> dt
 // ---------------------------------------------------------------------------------------
+// Two cells per lane (kBL lanes per brick).  With `vmax`, also max |v| over the active cells
+// (a non-finite component makes it infinite) into partials[nbricks + b].
 template <class R>
-__global__ __launch_bounds__(512) void k_outflow_rate(MeshView M, FieldsView<R> F, double* partials) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    double rate = 0.0;
-    if (m & kActive) {
-        const double h = cell_h(M, b), A = h * h;
+__global__ __launch_bounds__(kBL) void k_outflow_rate(MeshView M, FieldsView<R> F, double* partials, int vmax) {
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
+    const double h = M.meta[b].h, A = h * h;
+    const uint16_t bf = M.meta[b].flags;
+    uint8_t m[2];
+    FaceVel<R> cf[2];
+    R v[2][3];
+    int sp[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        m[k] = M.mask[base + q];
+        cf[k] = load_faces<R>(M, F, b, q);
+        for (int c = 0; c < 3; ++c) v[k][c] = vmax ? F.v[c][base + q] : R(0);
+    }
+    for (int k = 0; k < 2; ++k) sp[k] = special_rank(M, b, t + kBL * k, (m[k] & kSpecial) != 0);
+    double rate = 0.0, vm = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        if (!(m[k] & kActive)) continue;
         double out = 0.0;
         for (int axis = 0; axis < 3; ++axis) {
-            if (m & (kPX << axis)) out += fmax(A * double(F.f[axis][a]), 0.0);
-            if (m & (kMX << axis)) out += fmax(-A * double(F.f[axis][neighbor_dof(M, b, q, axis, -1)]), 0.0);
+            if (m[k] & (kPX << axis)) out += fmax(A * double(cf[k].up[axis]), 0.0);
+            if (m[k] & (kMX << axis)) out += fmax(-A * double(cf[k].dn[axis]), 0.0);
         }
-        const uint8_t bf = M.brick_flags[b];
         if ((q & 7) == 0 && (bf & kXMIN)) out += fmax(-A * M.speed, 0.0);
-        if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) out += fmax(A * double(F.f[0][a]), 0.0);
+        if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) out += fmax(A * double(cf[k].up[0]), 0.0);
         double V = h * h * h;
-        if (sp >= 0) {
-            V = M.special_volume[sp];
-            for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
-                const int raw = M.special_inc[k];
-                const int e = raw >= 0 ? raw : ~raw;
-                const double flux = M.earea[e] * double(F.fe[e]);
-                out += fmax(raw >= 0 ? flux : -flux, 0.0);
-            }
+        if (sp[k] >= 0) {
+            V = M.special_volume[sp[k]];
+            fold_edge_fluxes<R>(M, F, sp[k], out, [](double acc, double f) { return acc + fmax(f, 0.0); });
         }
-        rate = out / V;
+        rate = fmax(rate, out / V);
+        const double x = double(v[k][0]), y = double(v[k][1]), z = double(v[k][2]);
+        double w = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) w = INFINITY;
+        vm = fmax(vm, w);
     }
-    store_partial(partials, b, block_max<512>(rate));
+    store_partial(partials, b, block_max<kBL>(rate));
+    if (vmax) {
+        const double mv = block_max<kBL>(vm);
+        if (threadIdx.x == 0) partials[M.nbricks + b] = mv;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -329,65 +431,92 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
 // Explicit diffusion substep (amr_advection.cu:233-241): conductance h on structured faces,
 // open area / distance on compact edges.
 // ---------------------------------------------------------------------------------------
+// Two cells per lane; the three velocity components with a one-cell face halo in LDS; compact
+// edges in groups of four incidences.
 template <class R>
-__global__ __launch_bounds__(512) void k_diffuse(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
+__global__ __launch_bounds__(kBL) void k_diffuse(MeshView M, const R* __restrict__ vx, const R* __restrict__ vy,
                                                  const R* __restrict__ vz, R* __restrict__ ox, R* __restrict__ oy,
                                                  R* __restrict__ oz, const DevScalars* S, int step) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
     const R* in[3] = {vx, vy, vz};
     R* out[3] = {ox, oy, oz};
-    const DevScalars& I = S[M.inst[b]];
+    uint8_t m[2];
+    for (int k = 0; k < 2; ++k) m[k] = M.mask[base + t + kBL * k];
+    TileFetch2<R> tf[3];
+    for (int c = 0; c < 3; ++c) tf[c] = fetch_tile_x2<R>(M, b, [&](int64_t i) -> R { return in[c][i]; });
+    const DevScalars& I = S[M.meta[b].inst];
     const double dts = I.dts;
+    const R h = R(M.meta[b].h);
+    int sp[2];
+    for (int k = 0; k < 2; ++k) sp[k] = special_rank(M, b, t + kBL * k, (m[k] & kSpecial) != 0);
     if (step >= I.subs) {  // this instance needs fewer substeps (the whole brick)
-        for (int c = 0; c < 3; ++c) out[c][a] = in[c][a];
-        if (redistributed(M, sp)) {
-            const double zero[3] = {0.0, 0.0, 0.0};
-            hand_over(M, sp, zero);
+        for (int k = 0; k < 2; ++k) {
+            for (int c = 0; c < 3; ++c) out[c][base + t + kBL * k] = tf[c].own[k];
+            if (redistributed(M, sp[k])) {
+                const double zero[3] = {0.0, 0.0, 0.0};
+                hand_over(M, sp[k], zero);
+            }
         }
         return;
     }
-    // the three components with a one-cell face halo in LDS
     __shared__ R tv[3][kTileN];
-    TileFetch<R> tf[3];
-    for (int c = 0; c < 3; ++c) tf[c] = fetch_tile<R>(M, b, [&](int64_t i) -> R { return in[c][i]; });
-    for (int c = 0; c < 3; ++c) stage_tile<R>(tv[c], tf[c]);
-    R va[3];
-    for (int c = 0; c < 3; ++c) va[c] = tf[c].own;
-    if (!(m & kActive)) {
-        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
-        return;
-    }
-    const R h = R(cell_h(M, b));
-    const R k = R(dts * M.nu) * h;
-    R d[3] = {R(0), R(0), R(0)};
-    const int ta = tile_index(q & 7, (q >> 3) & 7, q >> 6);
-    for (int axis = 0; axis < 3; ++axis)
-        for (int s = -1; s <= 1; s += 2) {
-            if (!(m & ((s > 0 ? kPX : kMX) << axis))) continue;
-            const int o = ta + s * (axis == 0 ? 1 : (axis == 1 ? 10 : 100));
-            for (int c = 0; c < 3; ++c) d[c] += k * (tv[c][o] - va[c]);
+    for (int c = 0; c < 3; ++c) stage_tile_x2<R>(tv[c], tf[c]);
+    const R kf = R(dts * M.nu) * h;
+    for (int kc = 0; kc < 2; ++kc) {
+        const int q = t + kBL * kc;
+        const int64_t a = base + q;
+        R va[3];
+        for (int c = 0; c < 3; ++c) va[c] = tf[c].own[kc];
+        if (!(m[kc] & kActive)) {
+            for (int c = 0; c < 3; ++c) out[c][a] = va[c];
+            continue;
         }
-    R V = h * h * h;
-    if (sp >= 0) {
-        V = R(M.special_volume[sp]);
-        for (int j = M.special_inc_off[sp]; j < M.special_inc_off[sp + 1]; ++j) {
-            const int raw = M.special_inc[j];
-            const int e = raw >= 0 ? raw : ~raw;
-            const int64_t o = raw >= 0 ? M.eb[e] : M.ea[e];
-            const R ke = R(dts * M.nu * M.edcoef[e]);
-            for (int c = 0; c < 3; ++c) d[c] += ke * (in[c][o] - va[c]);
+        R d[3] = {R(0), R(0), R(0)};
+        const int ta = tile_index(q & 7, (q >> 3) & 7, q >> 6);
+        for (int axis = 0; axis < 3; ++axis)
+            for (int sg = -1; sg <= 1; sg += 2) {
+                if (!(m[kc] & ((sg > 0 ? kPX : kMX) << axis))) continue;
+                const int o = ta + sg * (axis == 0 ? 1 : (axis == 1 ? 10 : 100));
+                for (int c = 0; c < 3; ++c) d[c] += kf * (tv[c][o] - va[c]);
+            }
+        R V = h * h * h;
+        const int spc = sp[kc];
+        if (spc >= 0) {
+            V = R(M.special_volume[spc]);
+            const int j0 = M.special_inc_off[spc], j1 = M.special_inc_off[spc + 1];
+            for (int j = j0; j < j1; j += 4) {
+                int raw[4];
+                int64_t o[4];
+                double dc[4];
+                R vo[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) raw[u] = j + u < j1 ? M.special_inc[j + u] : 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = raw[u] >= 0 ? raw[u] : ~raw[u];
+                    const bool ok = j + u < j1;
+                    o[u] = ok ? (raw[u] >= 0 ? M.eb[e] : M.ea[e]) : a;
+                    dc[u] = ok ? M.edcoef[e] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    for (int c = 0; c < 3; ++c) vo[u][c] = in[c][o[u]];
+                for (int u = 0; u < 4; ++u) {
+                    if (j + u >= j1) break;
+                    const R ke = R(dts * M.nu * dc[u]);
+                    for (int c = 0; c < 3; ++c) d[c] += ke * (vo[u][c] - va[c]);
+                }
+            }
         }
+        if (redistributed(M, spc)) {
+            hand_over(M, spc, d);
+            for (int c = 0; c < 3; ++c) out[c][a] = va[c];
+            continue;
+        }
+        const R iv = R(1) / V;
+        for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
     }
-    if (redistributed(M, sp)) {
-        hand_over(M, sp, d);
-        for (int c = 0; c < 3; ++c) out[c][a] = va[c];
-        return;
-    }
-    const R iv = R(1) / V;
-    for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -538,18 +667,34 @@ __global__ __launch_bounds__(512) void k_diffuse_var(MeshView M, const R* __rest
 // normal components; x-max outlet face = interior cell (zero gradient); compact edges use
 // the distance weight.
 // ---------------------------------------------------------------------------------------
+// two cells per lane; the cell's and its +x/+y/+z neighbours' velocities are loaded before
+// the mask is tested
 template <class R>
-__global__ __launch_bounds__(512) void k_reconstruct(MeshView M, FieldsView<R> F) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    if (!(m & kActive)) return;
-    for (int axis = 0; axis < 3; ++axis) {
-        if (!(m & (kPX << axis))) continue;
-        const int64_t o = neighbor_dof(M, b, q, axis, 1);
-        F.f[axis][a] = R(0.5) * (F.v[axis][a] + F.v[axis][o]);
+__global__ __launch_bounds__(kBL) void k_reconstruct(MeshView M, FieldsView<R> F) {
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
+    const uint16_t bf = M.meta[b].flags;
+    uint8_t m[2];
+    R va[2][3], vn[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int64_t a = base + q;
+        m[k] = M.mask[a];
+        for (int axis = 0; axis < 3; ++axis) {
+            va[k][axis] = F.v[axis][a];
+            const int64_t o = face_dof(M, b, q, axis, 1);
+            vn[k][axis] = o >= 0 ? F.v[axis][o] : R(0);
+        }
     }
-    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) F.f[0][a] = F.v[0][a];
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int64_t a = base + q;
+        if (!(m[k] & kActive)) continue;
+        for (int axis = 0; axis < 3; ++axis)
+            if (m[k] & (kPX << axis)) F.f[axis][a] = R(0.5) * (va[k][axis] + vn[k][axis]);
+        if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) F.f[0][a] = va[k][0];
+    }
 }
 
 // compact faces: the weighted mean of the two cells' normal components plus the transverse
@@ -576,74 +721,113 @@ __global__ void k_reconstruct_edges(MeshView M, FieldsView<R> F, const R* __rest
 
 // integrated divergence of the face fluxes around cell a (FP64 accumulation)
 template <class R>
-__device__ __forceinline__ double cell_divergence(const MeshView& M, const FieldsView<R>& F, int b, int q, uint8_t m,
-                                                  int sp, double h) {
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
+__device__ __forceinline__ double cell_divergence(const MeshView& M, const FieldsView<R>& F, const FaceVel<R>& f,
+                                                  uint16_t bf, int q, uint8_t m, int sp, double h) {
     const double A = h * h;
     double div = 0.0;
     for (int axis = 0; axis < 3; ++axis) {
-        if (m & (kPX << axis)) div += A * double(F.f[axis][a]);
-        if (m & (kMX << axis)) div -= A * double(F.f[axis][neighbor_dof(M, b, q, axis, -1)]);
+        if (m & (kPX << axis)) div += A * double(f.up[axis]);
+        if (m & (kMX << axis)) div -= A * double(f.dn[axis]);
     }
-    const uint8_t bf = M.brick_flags[b];
     if ((q & 7) == 0 && (bf & kXMIN)) div -= A * M.speed;
-    if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) div += A * double(F.f[0][a]);
-    if (sp >= 0)
-        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
-            const int raw = M.special_inc[k];
-            const int e = raw >= 0 ? raw : ~raw;
-            const double flux = M.earea[e] * double(F.fe[e]);
-            div += raw >= 0 ? flux : -flux;
-        }
+    if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) div += A * double(f.up[0]);
+    if (sp >= 0) fold_edge_fluxes<R>(M, F, sp, div, [](double acc, double x) { return acc + x; });
     return div;
 }
 
-// RHS = -(rho/dt) * integrated divergence (amr_pressure.cu:255-284); ||b|| reduction
+// RHS = -(rho/dt) * integrated divergence (amr_pressure.cu:255-284); ||b|| reduction; two
+// cells per lane
 template <class R>
-__global__ __launch_bounds__(512) void k_rhs(MeshView M, FieldsView<R> F, double* __restrict__ rhs, double* partials,
+__global__ __launch_bounds__(kBL) void k_rhs(MeshView M, FieldsView<R> F, double* __restrict__ rhs, double* partials,
                                              const DevScalars* S) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    double val = 0.0;
-    if (m & kActive) val = -(M.rho / S[M.inst[b]].dt) * cell_divergence(M, F, b, q, m, sp, cell_h(M, b));
-    rhs[a] = val;
-    store_partial(partials, b, block_sum<512>(val * val));
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
+    const double h = M.meta[b].h;
+    const uint16_t bf = M.meta[b].flags;
+    const double dt = S[M.meta[b].inst].dt;
+    uint8_t m[2];
+    FaceVel<R> cf[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        m[k] = M.mask[base + t + kBL * k];
+        cf[k] = load_faces<R>(M, F, b, t + kBL * k);
+    }
+    double sum = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int sp = special_rank(M, b, q, (m[k] & kSpecial) != 0);
+        double val = 0.0;
+        if (m[k] & kActive) val = -(M.rho / dt) * cell_divergence<R>(M, F, cf[k], bf, q, m[k], sp, h);
+        rhs[base + q] = val;
+        sum += val * val;
+    }
+    store_partial(partials, b, block_sum<kBL>(sum));
 }
 
-// max |integrated divergence| / V after the correction (FP64 check)
+// max |integrated divergence| / V after the correction (FP64 check); two cells per lane
 template <class R>
-__global__ __launch_bounds__(512) void k_divcheck(MeshView M, FieldsView<R> F, double* partials) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
-    double val = 0.0;
-    if (m & kActive) {
-        const double h = cell_h(M, b);
-        const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
-        val = fabs(cell_divergence(M, F, b, q, m, sp, h) / V);
+__global__ __launch_bounds__(kBL) void k_divcheck(MeshView M, FieldsView<R> F, double* partials) {
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
+    const double h = M.meta[b].h;
+    const uint16_t bf = M.meta[b].flags;
+    uint8_t m[2];
+    FaceVel<R> cf[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        m[k] = M.mask[base + t + kBL * k];
+        cf[k] = load_faces<R>(M, F, b, t + kBL * k);
     }
-    store_partial(partials, b, block_max<512>(val));
+    double val = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int sp = special_rank(M, b, q, (m[k] & kSpecial) != 0);
+        if (m[k] & kActive) {
+            const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
+            val = fmax(val, fabs(cell_divergence<R>(M, F, cf[k], bf, q, m[k], sp, h) / V));
+        }
+    }
+    store_partial(partials, b, block_max<kBL>(val));
 }
 
 // face correction u -= (dt/rho) dp/dn (amr_pressure.cu:285-310); outlet p = 0 half a cell away
+// two cells per lane; the pressures of the cell and its +x/+y/+z neighbours and the face
+// velocities are loaded before the mask is tested
 template <class R>
-__global__ __launch_bounds__(512) void k_correct(MeshView M, FieldsView<R> F, const double* __restrict__ p,
+__global__ __launch_bounds__(kBL) void k_correct(MeshView M, FieldsView<R> F, const double* __restrict__ p,
                                                  const DevScalars* S) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    if (!(m & kActive)) return;
-    const double s = S[M.inst[b]].dt / M.rho, h = cell_h(M, b);
-    const double pa = p[a];
-    for (int axis = 0; axis < 3; ++axis) {
-        if (!(m & (kPX << axis))) continue;
-        const int64_t o = neighbor_dof(M, b, q, axis, 1);
-        F.f[axis][a] = R(double(F.f[axis][a]) - s * (p[o] - pa) / h);
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
+    const double s = S[M.meta[b].inst].dt / M.rho, h = M.meta[b].h;
+    const uint16_t bf = M.meta[b].flags;
+    uint8_t m[2];
+    double pa[2], pn[2][3];
+    R fu[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int64_t a = base + q;
+        m[k] = M.mask[a];
+        pa[k] = p[a];
+        for (int axis = 0; axis < 3; ++axis) {
+            fu[k][axis] = F.f[axis][a];
+            const int64_t o = face_dof(M, b, q, axis, 1);
+            pn[k][axis] = o >= 0 ? p[o] : 0.0;
+        }
     }
-    if (M.outlet && (q & 7) == 7 && (M.brick_flags[b] & kXMAX)) F.f[0][a] = R(double(F.f[0][a]) + s * pa / (0.5 * h));
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int64_t a = base + q;
+        if (!(m[k] & kActive)) continue;
+        R f0 = fu[k][0];
+        for (int axis = 0; axis < 3; ++axis) {
+            if (!(m[k] & (kPX << axis))) continue;
+            const R v = R(double(fu[k][axis]) - s * (pn[k][axis] - pa[k]) / h);
+            F.f[axis][a] = v;
+            if (axis == 0) f0 = v;
+        }
+        if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) F.f[0][a] = R(double(f0) + s * pa[k] / (0.5 * h));
+    }
 }
 
 template <class R>
@@ -672,72 +856,120 @@ __device__ __forceinline__ void wall_force_p(const MeshView& M, const double* p,
 // ---------------------------------------------------------------------------------------
 // dt: the given value when positive, else the instance's own step dt; with only_converged,
 // instances whose projection did not converge are left unchanged.
+// Two cells per lane; the pressures of the cell and its six neighbours and the cell velocity
+// are loaded before the mask is tested; compact faces in groups of two incidences.
 template <class R>
-__global__ __launch_bounds__(512) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
+__global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
                                                  const double* __restrict__ gp, const DevScalars* S, double dt_given,
                                                  int only_converged) {
-    const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
-    const int64_t a = static_cast<int64_t>(b) * 512 + q;
-    const uint8_t m = M.mask[a];
-    const DevScalars& I = S[M.inst[b]];
-    const int sp = special_rank(M, b, q, (m & kSpecial) != 0);
+    const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(b) * 512;
+    const DevScalars& I = S[M.meta[b].inst];
+    const double h = M.meta[b].h, A = h * h;
+    const uint16_t bf = M.meta[b].flags;
+    uint8_t m[2];
+    double pa[2], pp[2][3], pm[2][3];
+    R v0[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        const int64_t a = base + q;
+        m[k] = M.mask[a];
+        pa[k] = p[a];
+        for (int axis = 0; axis < 3; ++axis) {
+            const int64_t ou = face_dof(M, b, q, axis, 1), od = face_dof(M, b, q, axis, -1);
+            pp[k][axis] = ou >= 0 ? p[ou] : 0.0;
+            pm[k][axis] = od >= 0 ? p[od] : 0.0;
+            v0[k][axis] = F.v[axis][a];
+        }
+    }
+    int sp[2];
+    for (int k = 0; k < 2; ++k) sp[k] = special_rank(M, b, t + kBL * k, (m[k] & kSpecial) != 0);
     if (only_converged && !I.converged) {  // block-uniform
-        if (redistributed(M, sp)) {
-            const double zero[3] = {0.0, 0.0, 0.0};
-            hand_over(M, sp, zero);
-        }
-        return;
-    }
-    if (!(m & kActive)) return;
-    const double dt = dt_given > 0.0 ? dt_given : I.dt;
-    const double s = dt / M.rho, h = cell_h(M, b), A = h * h;
-    const double pa = p[a];
-    double d[3] = {0.0, 0.0, 0.0};
-    for (int axis = 0; axis < 3; ++axis) {
-        if (m & (kPX << axis)) d[axis] -= s * A * 0.5 * (pa + p[neighbor_dof(M, b, q, axis, 1)]);
-        if (m & (kMX << axis)) d[axis] += s * A * 0.5 * (p[neighbor_dof(M, b, q, axis, -1)] + pa);
-    }
-    const uint8_t bf = M.brick_flags[b];
-    const int i = q & 7, j = (q >> 3) & 7, k = q >> 6;
-    const double bnd = s * A * pa;
-    if (i == 0 && (bf & kXMIN)) d[0] += bnd;
-    if (i == 7 && (bf & kXMAX) && !M.outlet) d[0] -= bnd;
-    if (j == 0 && (bf & kYMIN)) d[1] += bnd;
-    if (j == 7 && (bf & kYMAX)) d[1] -= bnd;
-    if (k == 0 && (bf & kZMIN)) d[2] += bnd;
-    if (k == 7 && (bf & kZMAX)) d[2] -= bnd;
-    double V = h * h * h;
-    if (sp >= 0) {
-        V = M.special_volume[sp];
-        // compact faces: weighted face pressure plus skew . ((1-w) g_a + w g_b)
-        // (pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390)
-        for (int t = M.special_inc_off[sp]; t < M.special_inc_off[sp + 1]; ++t) {
-            const int raw = M.special_inc[t];
-            const int e = raw >= 0 ? raw : ~raw;
-            const double w = M.ew[e];
-            double pf = (1.0 - w) * p[M.ea[e]] + w * p[M.eb[e]];
-            const int sa = M.esp[2 * e], sb = M.esp[2 * e + 1];
-            if (sa >= 0 && sb >= 0) {
-                const double* da = M.edisp + 6 * static_cast<int64_t>(e);
-                for (int c = 0; c < 3; ++c)
-                    pf += ((1.0 - w) * da[c] + w * da[3 + c]) * ((1.0 - w) * gp[3 * sa + c] + w * gp[3 * sb + c]);
+        for (int k = 0; k < 2; ++k)
+            if (redistributed(M, sp[k])) {
+                const double zero[3] = {0.0, 0.0, 0.0};
+                hand_over(M, sp[k], zero);
             }
-            const double imp = s * M.earea[e] * pf;
-            d[M.eaxis[e]] += raw >= 0 ? -imp : imp;
-        }
-        const int pch = M.special_patch[sp];
-        if (pch >= 0) {
-            double Fw[3];
-            wall_force_p(M, p, gp, pch, sp, 0.0, Fw);
-            for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
-        }
-    }
-    if (redistributed(M, sp)) {
-        hand_over(M, sp, d);
         return;
     }
-    const double iv = 1.0 / V;
-    for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(F.v[c][a]) + d[c] * iv);
+    const double dt = dt_given > 0.0 ? dt_given : I.dt;
+    const double s = dt / M.rho;
+    for (int kc = 0; kc < 2; ++kc) {
+        const int q = t + kBL * kc;
+        const int64_t a = base + q;
+        if (!(m[kc] & kActive)) continue;
+        double d[3] = {0.0, 0.0, 0.0};
+        for (int axis = 0; axis < 3; ++axis) {
+            if (m[kc] & (kPX << axis)) d[axis] -= s * A * 0.5 * (pa[kc] + pp[kc][axis]);
+            if (m[kc] & (kMX << axis)) d[axis] += s * A * 0.5 * (pm[kc][axis] + pa[kc]);
+        }
+        const int i = q & 7, j = (q >> 3) & 7, kz = q >> 6;
+        const double bnd = s * A * pa[kc];
+        if (i == 0 && (bf & kXMIN)) d[0] += bnd;
+        if (i == 7 && (bf & kXMAX) && !M.outlet) d[0] -= bnd;
+        if (j == 0 && (bf & kYMIN)) d[1] += bnd;
+        if (j == 7 && (bf & kYMAX)) d[1] -= bnd;
+        if (kz == 0 && (bf & kZMIN)) d[2] += bnd;
+        if (kz == 7 && (bf & kZMAX)) d[2] -= bnd;
+        double V = h * h * h;
+        const int spc = sp[kc];
+        if (spc >= 0) {
+            V = M.special_volume[spc];
+            // compact faces: weighted face pressure plus skew . ((1-w) g_a + w g_b)
+            // (pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390)
+            const int t0 = M.special_inc_off[spc], t1 = M.special_inc_off[spc + 1];
+            for (int tt = t0; tt < t1; tt += 2) {
+                int raw[2], e[2], ia[2], ib[2], sa[2], sb[2], ax[2];
+                double w[2], ar[2], pA[2], pB[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    raw[u] = tt + u < t1 ? M.special_inc[tt + u] : 0;
+                    e[u] = raw[u] >= 0 ? raw[u] : ~raw[u];
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const bool ok = tt + u < t1;
+                    w[u] = ok ? M.ew[e[u]] : 0.0;
+                    ar[u] = ok ? M.earea[e[u]] : 0.0;
+                    ia[u] = ok ? M.ea[e[u]] : 0;
+                    ib[u] = ok ? M.eb[e[u]] : 0;
+                    sa[u] = ok ? M.esp[2 * e[u]] : -1;
+                    sb[u] = ok ? M.esp[2 * e[u] + 1] : -1;
+                    ax[u] = ok ? M.eaxis[e[u]] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    pA[u] = tt + u < t1 ? p[ia[u]] : 0.0;
+                    pB[u] = tt + u < t1 ? p[ib[u]] : 0.0;
+                }
+                for (int u = 0; u < 2; ++u) {
+                    if (tt + u >= t1) break;
+                    double pf = (1.0 - w[u]) * pA[u] + w[u] * pB[u];
+                    if (sa[u] >= 0 && sb[u] >= 0) {
+                        const double* da = M.edisp + 6 * static_cast<int64_t>(e[u]);
+                        for (int c = 0; c < 3; ++c)
+                            pf += ((1.0 - w[u]) * da[c] + w[u] * da[3 + c]) *
+                                  ((1.0 - w[u]) * gp[3 * sa[u] + c] + w[u] * gp[3 * sb[u] + c]);
+                    }
+                    const double imp = s * ar[u] * pf;
+                    d[ax[u]] += raw[u] >= 0 ? -imp : imp;
+                }
+            }
+            const int pch = M.special_patch[spc];
+            if (pch >= 0) {
+                double Fw[3];
+                wall_force_p(M, p, gp, pch, spc, 0.0, Fw);
+                for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
+            }
+        }
+        if (redistributed(M, spc)) {
+            hand_over(M, spc, d);
+            continue;
+        }
+        const double iv = 1.0 / V;
+        for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(v0[kc][c]) + d[c] * iv);
+    }
 }
 
 // ---------------------------------------------------------------------------------------

@@ -560,7 +560,12 @@ private:
     }
     void grad_p() {
         if (M_.nspecial) LAUNCH(k_grad_p, blocks_for(M_.nspecial, 256), 256, M_, smag_, p_, grad_p_);
+        gp_gen_ = p_gen_;
     }
+    // generation of the pressure: raised wherever p may change (solves, uploads, restores), so
+    // that the load integral reuses the pressure gradients of the step's impulse
+    uint64_t p_gen_ = 1, gp_gen_ = 0;
+    void p_changed() { ++p_gen_; }
     // learned preconditioner state (PCFD_PC_LEARNED)
     bool nn_ = false;             // created with the learned preconditioner
     bool nn_on_ = false;          // network enabled (trained or weights set)
@@ -1192,7 +1197,7 @@ void SolverImpl<R>::release() {
 
 template <class R>
 void SolverImpl<R>::reconstruct() {
-    LAUNCH(k_reconstruct<R>, nb_, 512, M_, fields());
+    LAUNCH(k_reconstruct<R>, nb_, kBL, M_, fields());
     if (nedges_) {
         grad_v();
         LAUNCH(k_reconstruct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), static_cast<const R*>(grad_v_));
@@ -1716,7 +1721,8 @@ void SolverImpl<R>::fgmres_solve() {
 
 template <class R>
 void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
-    LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+    p_changed();
+    LAUNCH(k_rhs<R>, nb_, kBL, M_, fields(), rhs_, partials_, S_);
     finalize(FIN_BNORM, nb_);
     if (!warm) {
         HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
@@ -1738,7 +1744,7 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
         fell_back = true;
         set_nn_off(true);
         if (!std::isfinite(host_s_[0].relres)) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
-        LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+        LAUNCH(k_rhs<R>, nb_, kBL, M_, fields(), rhs_, partials_, S_);
         finalize(FIN_BNORM, nb_);
         pcg_loop(false);
         iterations += host_s_[0].iter;
@@ -1748,13 +1754,13 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
             const int64_t d0 = static_cast<int64_t>(inst_brick_off_[i]) * 512, d1 = static_cast<int64_t>(inst_brick_off_[i + 1]) * 512;
             HIP_CHECK(hipMemsetAsync(p_ + d0, 0, (d1 - d0) * sizeof(double), stream_));
         }
-    LAUNCH(k_correct<R>, nb_, 512, M_, fields(), p_, S_);
+    LAUNCH(k_correct<R>, nb_, kBL, M_, fields(), p_, S_);
     if (nedges_) LAUNCH(k_correct_edges<R>, blocks_for(nedges_, 256), 256, M_, fields(), p_, S_);
     if (nonorth_ && NV_.nf > 0) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, p_);
         LAUNCH(k_no_correct<R>, blocks_for(NV_.nf, 256), 256, NV_, fields(), p_, S_, opt_.rho);
     }
-    LAUNCH(k_divcheck<R>, nb_, 512, M_, fields(), partials_);
+    LAUNCH(k_divcheck<R>, nb_, kBL, M_, fields(), partials_);
     finalize(FIN_MAXDIV, nb_);
     sync_scalars();
     for (int i = 0; i < ninst_; ++i) {
@@ -1794,7 +1800,7 @@ void SolverImpl<R>::project(double dt, bool warm, pcfd_step_stats* st) {
 template <class R>
 void SolverImpl<R>::impulse(double dt) {
     grad_p();
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, dt, 0);
+    LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, grad_p_, S_, dt, 0);
     redistribute();
 }
 
@@ -1802,7 +1808,7 @@ template <class R>
 void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
     if (dt > 0) put_dt(dt);
     else {
-        LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_);
+        LAUNCH(k_outflow_rate<R>, nb_, kBL, M_, fields(), partials_, 0);
         finalize(FIN_RATE, nb_);
     }
     advect_core();
@@ -1836,7 +1842,7 @@ void SolverImpl<R>::advect_core() {
     }
     for (int s = 0; s < subs; ++s) {
         if (smag) LAUNCH(k_diffuse_var<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], nu_, S_, s);
-        else LAUNCH(k_diffuse<R>, nb_, 512, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], S_, s);
+        else LAUNCH(k_diffuse<R>, nb_, kBL, M_, v_[0], v_[1], v_[2], vt_[0], vt_[1], vt_[2], S_, s);
         for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
         redistribute();
     }
@@ -1858,9 +1864,10 @@ void SolverImpl<R>::initialize(pcfd_step_stats* st) {
     reconstruct();
     const double dt = opt_.cfl * M_.finest_h / std::max(1e-9, std::abs(opt_.speed));
     HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
+    p_changed();
     project(dt, false, nullptr);
     grad_p();
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, dt, 1);  // converged instances only
+    LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, grad_p_, S_, dt, 1);  // converged instances only
     redistribute();
     LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_VMAX, nb_);
@@ -1881,10 +1888,8 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     if (!initialized) throw std::logic_error("step before a converged initialization");
     for (auto& s : inst_stats_) std::memset(&s, 0, sizeof(s));
     HIP_CHECK(hipEventRecord(ev_[2], stream_));
-    LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
-    finalize(FIN_VMAX, nb_);
-    LAUNCH(k_outflow_rate<R>, nb_, 512, M_, fields(), partials_);
-    finalize(FIN_RATE, nb_);
+    LAUNCH(k_outflow_rate<R>, nb_, kBL, M_, fields(), partials_, 1);
+    finalize(FIN_RATE_VMAX, nb_);
     sync_scalars();
     for (int i = 0; i < ninst_; ++i) {
         if (!std::isfinite(host_s_[i].vmax)) throw std::domain_error("non-finite velocity before timestep");
@@ -1897,7 +1902,7 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     run_projection(true, nullptr);
     // impulse and time advance for the instances whose projection converged
     grad_p();
-    LAUNCH(k_impulse<R>, nb_, 512, M_, fields(), p_, grad_p_, S_, 0.0, 1);
+    LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, grad_p_, S_, 0.0, 1);
     redistribute();
     for (int i = 0; i < ninst_; ++i)
         if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
@@ -1966,7 +1971,7 @@ void SolverImpl<R>::read_loads(int i, pcfd_loads* out) const {
 
 template <class R>
 void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
-    grad_p();
+    if (gp_gen_ != p_gen_) grad_p();
     enqueue_loads(i, pref);
     sync_scalars();
     read_loads(i, out);
@@ -1974,7 +1979,7 @@ void SolverImpl<R>::loads_instance(int i, double pref, pcfd_loads* out) {
 
 template <class R>
 void SolverImpl<R>::batch_loads(double pref, pcfd_loads* out) {
-    grad_p();
+    if (gp_gen_ != p_gen_) grad_p();
     for (int i = 0; i < ninst_; ++i) enqueue_loads(i, pref);
     sync_scalars();
     for (int i = 0; i < ninst_; ++i) read_loads(i, out + i);
@@ -2020,6 +2025,7 @@ template <class R>
 void SolverImpl<R>::set_pressure(const double* p) {
     HIP_CHECK(hipStreamSynchronize(stream_));
     HIP_CHECK(hipMemcpy(p_, p, n_ * sizeof(double), hipMemcpyHostToDevice));
+    p_changed();
 }
 
 template <class R>
@@ -2210,6 +2216,7 @@ void SolverImpl<R>::restore_state() {
     }
     if (nedges_) HIP_CHECK(hipMemcpyAsync(fe_, snap_fe_, nedges_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
     HIP_CHECK(hipMemcpyAsync(p_, snap_p_, n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    p_changed();
     if (M_.npatches)
         HIP_CHECK(hipMemcpyAsync(wall_force_, snap_wall_force_, 3 * static_cast<size_t>(M_.npatches) * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream_));
@@ -2412,7 +2419,7 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
     try {
         double dt = 0.0;
         advect(0.0, &dt, nullptr);
-        LAUNCH(k_rhs<R>, nb_, 512, M_, fields(), rhs_, partials_, S_);
+        LAUNCH(k_rhs<R>, nb_, kBL, M_, fields(), rhs_, partials_, S_);
         finalize(FIN_BNORM, nb_);
         // warm start from the current pressure, as the step's projection does
         set_nn_off(true);
