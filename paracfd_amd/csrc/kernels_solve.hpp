@@ -692,23 +692,23 @@ __device__ __forceinline__ A ell_row(const CsrView& C, int r, Fn y, A yr) {
     const CT* coef = static_cast<const CT*>(C.sl_coef);
     const int s0 = C.sl_off[r >> 6], s1 = C.sl_off[(r >> 6) + 1];
     A Ax = A(static_cast<const CT*>(C.diag)[r]) * yr;
-    int s = s0 + (r & 63);
-    // four slots per round: their columns, coefficients and values are loaded before any is
-    // used, then subtracted in slot order
-    for (; s + 192 < s1; s += 256) {
-        int c[4];
-        A k[4], v[4];
+    // eight slots per round (most coarse rows have at most eight): their columns and
+    // coefficients, then their values, are loaded before any is used; subtracted in slot order
+    for (int s = s0 + (r & 63); s < s1; s += 512) {
+        int c[8];
+        A k[8], v[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            c[j] = C.sl_col[s + 64 * j];
-            k[j] = A(coef[s + 64 * j]);
+        for (int j = 0; j < 8; ++j) {
+            const bool ok = s + 64 * j < s1;
+            c[j] = ok ? C.sl_col[s + 64 * j] : r;
+            k[j] = ok ? A(coef[s + 64 * j]) : A(0);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = y(c[j]);
+        for (int j = 0; j < 8; ++j) v[j] = y(c[j]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) Ax -= k[j] * v[j];
+        for (int j = 0; j < 8; ++j)
+            if (s + 64 * j < s1) Ax -= k[j] * v[j];
     }
-    for (; s < s1; s += 64) Ax -= A(coef[s]) * y(C.sl_col[s]);
     return Ax;
 }
 
