@@ -111,6 +111,9 @@ struct CsrView {
     const int* sl_off;
     const int* sl_col;
     const void* sl_coef;
+    // per slice: the coefficient every slot of the slice holds, when the slice has no padding and
+    // one coefficient throughout (regular coarse cells), else NaN; its rows load no coefficients
+    const void* sl_uni;
 };
 
 // LDS tile of one brick with a one-cell halo: 10 x 10 x 10 values

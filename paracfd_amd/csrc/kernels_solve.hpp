@@ -692,6 +692,22 @@ __device__ __forceinline__ A ell_row(const CsrView& C, int r, Fn y, A yr) {
     const CT* coef = static_cast<const CT*>(C.sl_coef);
     const int s0 = C.sl_off[r >> 6], s1 = C.sl_off[(r >> 6) + 1];
     A Ax = A(static_cast<const CT*>(C.diag)[r]) * yr;
+    const CT u = static_cast<const CT*>(C.sl_uni)[r >> 6];
+    if (u == u) {  // slice-uniform: the same coefficient in every slot, columns only
+        const A ua = A(u);
+        for (int s = s0 + (r & 63); s < s1; s += 512) {
+            int c[8];
+            A v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) c[j] = s + 64 * j < s1 ? C.sl_col[s + 64 * j] : r;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = y(c[j]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (s + 64 * j < s1) Ax -= ua * v[j];
+        }
+        return Ax;
+    }
     // eight slots per round (most coarse rows have at most eight): their columns and
     // coefficients, then their values, are loaded before any is used; subtracted in slot order
     for (int s = s0 + (r & 63); s < s1; s += 512) {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -879,15 +880,28 @@ void SolverImpl<R>::construct(const Topology& T) {
                     for (int k = G.child_off[cc]; k < G.child_off[cc + 1]; ++k)
                         if (G.child[k] != G.child[G.child_off[cc]] + (k - G.child_off[cc]))
                             throw std::logic_error("coarse children are not contiguous");
+            // slices without padding whose slots all hold one coefficient
+            std::vector<double> suni(nsl, std::numeric_limits<double>::quiet_NaN());
+            for (int s = 0; s < nsl; ++s) {
+                const int K = (sl_off[s + 1] - sl_off[s]) / 64, r1 = std::min(G.n, 64 * s + 64);
+                bool uni = K > 0;
+                for (int r = 64 * s; uni && r < r1; ++r) uni = G.row_off[r + 1] - G.row_off[r] == K;
+                const double c0 = K > 0 ? G.coef[G.row_off[64 * s]] : 0.0;
+                for (int r = 64 * s; uni && r < r1; ++r)
+                    for (int e = G.row_off[r]; uni && e < G.row_off[r + 1]; ++e) uni = G.coef[e] == c0;
+                if (uni) suni[s] = c0;
+            }
             c.sl_off = upload(sl_off);
             c.sl_col = upload(scol);
             L.csr64 = c; L.csr32 = c;
             L.csr64.coef = upload(G.coef);
             L.csr64.diag = upload(G.diag);
             L.csr64.sl_coef = upload(scoef);
+            L.csr64.sl_uni = upload(suni);
             L.csr32.coef = upload_as<float>(G.coef);
             L.csr32.diag = upload_as<float>(G.diag);
             L.csr32.sl_coef = upload_as<float>(scoef);
+            L.csr32.sl_uni = upload_as<float>(suni);
             L.parent = c.parent;
             L.B = alloc<double>(G.n); L.X = alloc<double>(G.n); L.T = alloc<double>(G.n);
         }
