@@ -297,7 +297,7 @@ private:
         if (update_fused()) {
             double* rin = r_;
             r_ = r_ == rbuf_[0] ? rbuf_[1] : rbuf_[0];
-            if (flexible_) rprev_ = rin;
+            rprev_ = flexible() ? rin : nullptr;
             precondition_dispatch(false, dnew, rin);
             return;
         }
@@ -311,7 +311,7 @@ private:
     // residual roles at the start of a PCG solve
     void reset_residual_roles() {
         r_ = rbuf_[0];
-        if (flexible_) rprev_ = rbuf_[1];
+        rprev_ = flexible() ? rbuf_[1] : nullptr;
     }
     // Iteration j of the loop after the first one reads d from dbuf[(j+1)&1] and writes
     // dbuf[j&1]; with the fused update it reads r from rbuf[(j+1)&1] and writes rbuf[j&1]
@@ -320,7 +320,7 @@ private:
     void set_roles_for(int j) {
         if (update_fused()) {
             r_ = rbuf_[(j + 1) & 1];
-            if (flexible_) rprev_ = rbuf_[j & 1];
+            rprev_ = flexible() ? rbuf_[j & 1] : nullptr;
         } else {
             reset_residual_roles();
         }
@@ -494,7 +494,11 @@ private:
     // that a neighbour brick's halo read could see), and for flexible CG the old one is r_prev.
     // r_ / rprev_ name the current roles at enqueue time; every PCG solve starts from rbuf_[0].
     double* rbuf_[2] = {};
-    bool flexible_ = false;
+    // Flexible (Polak-Ribiere) CG while the preconditioner varies from call to call: BF16 storage
+    // (rounding of order 1e-3 relative) or the network on.  The FP32 V-cycle is the FP64 one with
+    // rounding of order 1e-7, so Fletcher-Reeves keeps its iteration counts and saves the r_prev
+    // traffic.  The captured graphs are dropped whenever the network is switched.
+    bool flexible() const { return opt_.vcycle_precision == PCFD_BF16 || (nn_ && nn_on_); }
     void* vc0_[2] = {};  // composite-level V-cycle buffers (8 bytes per dof)
     double* partials_ = nullptr;
     DevScalars* S_ = nullptr;     // ninst_ instance records, then the batch aggregate
@@ -755,11 +759,9 @@ void SolverImpl<R>::construct(const Topology& T) {
     tuning_.coarse_block = coarse_block_;
     tuning_.fused_vcycle = fused_ ? 1 : 0;
     tuning_.graph_replay = use_graph_ ? 1 : 0;
-    flexible_ = opt_.vcycle_precision != PCFD_FP64 || opt_.preconditioner == PCFD_PC_LEARNED;
     rbuf_[0] = r_;
     rbuf_[1] = alloc<double>(n_);
     HIP_CHECK(hipMemset(rbuf_[1], 0, n_ * sizeof(double)));
-    if (flexible_) rprev_ = rbuf_[1];
     vc0_[0] = alloc<double>(n_);
     vc0_[1] = alloc<double>(n_);
     const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 12 * (M_.npatches / 256 + 1)});
