@@ -29,8 +29,27 @@ __device__ __forceinline__ R minmod(R a, R b) {
 // 0.5 (1 - min(1, C)) (amr_advection.cu:62-75).
 template <class R>
 __device__ __forceinline__ R muscl_value(R swept, R pred, R va, R vb, R vl, bool has_l, R vu, bool has_u) {
-    if (swept >= R(0)) return has_l ? va + pred * minmod(va - vl, vb - va) : va;
-    return has_u ? vb - pred * minmod(vb - va, vu - vb) : vb;
+    // upwind side selected before the one minmod: up ? va + pred mm(va - vl, vb - va)
+    // : vb - pred mm(vb - va, vu - vb), with the same operations on the selected operands
+    const bool up = swept >= R(0);
+    const R dab = vb - va;
+    const R s1 = up ? va - vl : dab, s2 = up ? dab : vu - vb;
+    const R base = up ? va : vb, p = up ? pred : -pred;
+    return (up ? has_l : has_u) ? base + p * minmod(s1, s2) : base;
+}
+
+// true where 1/h is exact (h a power of two, h and 1/h normal): x * (1/h) == x / h then
+template <class R>
+__device__ __forceinline__ bool exact_reciprocal(R h) {
+    if constexpr (sizeof(R) == 4) {
+        const uint32_t u = __float_as_uint(h);
+        const uint32_t e = (u >> 23) & 0xffu;
+        return (u & 0x807fffffu) == 0u && e > 1u && e < 253u;
+    } else {
+        const uint64_t u = static_cast<uint64_t>(__double_as_longlong(h));
+        const uint64_t e = (u >> 52) & 0x7ffu;
+        return (u & 0x800fffffffffffffull) == 0ull && e > 1u && e < 2045u;
+    }
 }
 
 __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[M.brick_level[b]]; }
@@ -313,15 +332,22 @@ __device__ __forceinline__ void stage_tile2(const MeshView& M, const FieldsView<
     __syncthreads();
 }
 
-// Regular faces read the MUSCL stencil (two cells each side) from the LDS tile.
+// Regular faces read the MUSCL stencil (two cells each side) from the LDS tile.  Each face
+// value is computed once: every cell reconstructs its own +x/+y/+z faces (where it or its
+// neighbour inside the brick has that face), and waves 0-2 reconstruct the 64 -x, -y and -z
+// faces on the brick's low sides (one axis per wave, lane t: cell (t & 7, t >> 3) of the face).
+// The face values then go to LDS over the velocity tile, and every cell sums its faces in the
+// order +x, -x, +y, -y, +z, -z with the same expressions as a cell reconstructing all six itself
+// (bit-identical).  Special cells hand their sum over to k_advect_compact, which adds their
+// compact-face fluxes.
 template <class R>
 __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* __restrict__ ox, R* __restrict__ oy,
-                                                R* __restrict__ oz, const DevScalars* S, const R* __restrict__ G) {
+                                                R* __restrict__ oz, const DevScalars* S, R* __restrict__ dsp) {
     const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(b) * 512 + q;
     // every independent load before the tile barrier: the mask, the cell's +x/+y/+z face
-    // velocities and those of its -x/-y/-z neighbours (their +faces), the instance's timestep
-    // and cell size from the brick record
+    // velocities and those of its -x/-y/-z neighbours (their +faces), the face velocity of the
+    // low-side face this lane reconstructs, the instance's timestep and cell size
     const uint8_t m = M.mask[a];
     R fo[3], fn[3];
     for (int axis = 0; axis < 3; ++axis) {
@@ -334,6 +360,20 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
         }
         fn[axis] = o >= 0 ? F.f[axis][o] : R(0);
     }
+    const bool low = q < 192;  // waves 0-2
+    const int baxis = __builtin_amdgcn_readfirstlane(q >> 6), bt = q & 63;
+    int tb = 0;
+    R ub = R(0);
+    if (low) {
+        const int a1 = baxis == 0 ? 1 : 0, a2 = baxis == 2 ? 1 : 2;
+        int cc[3];
+        cc[a1] = bt & 7;
+        cc[a2] = bt >> 3;
+        cc[baxis] = 0;
+        tb = tile2_index(cc[0], cc[1], cc[2]);
+        const int nb = M.meta[b].nbr[2 * baxis];
+        if (nb >= 0) ub = F.f[baxis][static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2] + 7 * (1 << (3 * baxis))];
+    }
     const double dt_inst = S[M.meta[b].inst].dt;
     const R h = R(M.meta[b].h), A = h * h;
     const uint16_t bf = M.meta[b].flags;
@@ -342,41 +382,78 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
     __shared__ uint8_t mt[kTile2];
     R va[3];
     stage_tile2<R>(M, F, b, vt, mt, m, va);
+    const R dt = R(dt_inst);
+    const int ta = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
+    R fp[3][3];
+    R fb[3] = {R(0), R(0), R(0)};
+    const R ih = R(1) / h;
+    // P2: the Courant fraction |u| dt / h as a product with the exact 1/h (same value)
+    auto reconstruct = [&](auto P2) {
+        auto courant = [&](R un) -> R {
+            if constexpr (decltype(P2)::value) return fabs(un) * dt * ih;
+            else return fabs(un) * dt / h;
+        };
+        // own +faces
+        for (int axis = 0; axis < 3; ++axis) {
+            const int st = axis == 0 ? 1 : (axis == 1 ? 12 : 144);
+            const int b1 = ta + st;
+            const uint8_t mb = mt[b1];
+            for (int c = 0; c < 3; ++c) fp[axis][c] = R(0);
+            const bool inner = axis_coord(q, axis) < 7;
+            if ((m & (kPX << axis)) || (inner && (mb & (kMX << axis)))) {
+                const bool has_l = (m & (kMX << axis)) != 0, has_u = (mb & (kPX << axis)) != 0;
+                const int l = has_l ? ta - st : ta;
+                const int u = has_u ? b1 + st : b1;
+                const R un = fo[axis];
+                const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), courant(un)));
+                for (int c = 0; c < 3; ++c)
+                    fp[axis][c] = muscl_value(swept, pred, va[c], vt[c][b1], vt[c][l], has_l, vt[c][u], has_u);
+            }
+        }
+        // low-side faces (b0 in the neighbour brick, cell tb)
+        if (low) {
+            const int st = baxis == 0 ? 1 : (baxis == 1 ? 12 : 144);
+            const uint8_t mc = mt[tb];
+            if (mc & (kMX << baxis)) {
+                const int b0 = tb - st;
+                const uint8_t m0 = mt[b0];
+                const bool has_l = (m0 & (kMX << baxis)) != 0, has_u = (mc & (kPX << baxis)) != 0;
+                const int l = has_l ? b0 - st : b0;
+                const int u = has_u ? tb + st : tb;
+                const R swept = dt * A * ub, pred = R(0.5) * (R(1) - fmin(R(1), courant(ub)));
+                for (int c = 0; c < 3; ++c)
+                    fb[c] = muscl_value(swept, pred, vt[c][b0], vt[c][tb], vt[c][l], has_l, vt[c][u], has_u);
+            }
+        }
+    };
+    if (exact_reciprocal(h)) reconstruct(std::true_type{});
+    else reconstruct(std::false_type{});
+    __syncthreads();  // every tile read done: the face values go over the tile
+    R* fv = &vt[0][0];  // +faces at (3 axis + c) 512 + cell, low-side faces at 4608 + (3 axis + c) 64 + t
+    for (int axis = 0; axis < 3; ++axis)
+        for (int c = 0; c < 3; ++c) fv[(3 * axis + c) * 512 + q] = fp[axis][c];
+    if (low)
+        for (int c = 0; c < 3; ++c) fv[4608 + (3 * baxis + c) * 64 + bt] = fb[c];
+    __syncthreads();
     R* out[3] = {ox, oy, oz};
     if (!(m & kActive)) {
         for (int c = 0; c < 3; ++c) out[c][a] = va[c];
         return;
     }
-    const R dt = R(dt_inst);
     R d[3] = {R(0), R(0), R(0)};
-    const int ta = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
     for (int axis = 0; axis < 3; ++axis) {
-        const int st = axis == 0 ? 1 : (axis == 1 ? 12 : 144);
         if (m & (kPX << axis)) {  // face (a, b1)
-            const int b1 = ta + st;
-            const uint8_t mb = mt[b1];
-            const bool has_l = (m & (kMX << axis)) != 0, has_u = (mb & (kPX << axis)) != 0;
-            const int l = has_l ? ta - st : ta;
-            const int u = has_u ? b1 + st : b1;
-            const R un = fo[axis];
-            const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
-            for (int c = 0; c < 3; ++c) {
-                const R fv = muscl_value(swept, pred, va[c], vt[c][b1], vt[c][l], has_l, vt[c][u], has_u);
-                d[c] -= swept * fv;
-            }
+            const R swept = dt * A * fo[axis];
+            for (int c = 0; c < 3; ++c) d[c] -= swept * fp[axis][c];
         }
         if (m & (kMX << axis)) {  // face (b0, a)
-            const int b0 = ta - st;
-            const uint8_t m0 = mt[b0];
-            const bool has_l = (m0 & (kMX << axis)) != 0, has_u = (m & (kPX << axis)) != 0;
-            const int l = has_l ? b0 - st : b0;
-            const int u = has_u ? ta + st : ta;
-            const R un = fn[axis];
-            const R swept = dt * A * un, pred = R(0.5) * (R(1) - fmin(R(1), fabs(un) * dt / h));
-            for (int c = 0; c < 3; ++c) {
-                const R fv = muscl_value(swept, pred, vt[c][b0], va[c], vt[c][l], has_l, vt[c][u], has_u);
-                d[c] += swept * fv;
-            }
+            const int ca = axis_coord(q, axis);
+            const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+            const int slot = ca > 0 ? (3 * axis) * 512 + q - (1 << (3 * axis))
+                                    : 4608 + (3 * axis) * 64 + axis_coord(q, a1) + 8 * axis_coord(q, a2);
+            const int cstride = ca > 0 ? 512 : 64;
+            const R swept = dt * A * fn[axis];
+            for (int c = 0; c < 3; ++c) d[c] += swept * fv[slot + c * cstride];
         }
     }
     if ((q & 7) == 0 && (bf & kXMIN)) {
@@ -389,35 +466,88 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
         if (swept >= R(0)) for (int c = 0; c < 3; ++c) d[c] -= swept * va[c];
         else d[0] -= swept * R(M.speed);
     }
-    R V = h * h * h;
-    if (sp >= 0) {
-        V = R(M.special_volume[sp]);
-        for (int k = M.special_inc_off[sp]; k < M.special_inc_off[sp + 1]; ++k) {
-            const int raw = M.special_inc[k];
-            const int e = raw >= 0 ? raw : ~raw;
-            const R swept = dt * R(M.earea[e]) * F.fe[e];
-            const bool lower = swept >= R(0);
-            const int64_t ia = M.ea[e], ib = M.eb[e];
-            const int ds = M.esp[2 * e + (lower ? 0 : 1)];
-            R C = R(1), dx[3] = {R(0), R(0), R(0)};
-            if (ds >= 0) {
-                C = fmin(R(1), fabs(swept) / R(M.special_volume[ds]));
-                const double* dd = M.edisp + 6 * static_cast<int64_t>(e) + (lower ? 0 : 3);
-                dx[0] = R(dd[0]); dx[1] = R(dd[1]); dx[2] = R(dd[2]);
-            }
+    if (sp >= 0) {  // compact faces, volume and redistribution in k_advect_compact
+        for (int c = 0; c < 3; ++c) dsp[3 * static_cast<int64_t>(sp) + c] = d[c];
+        return;
+    }
+    const R V = h * h * h;
+    const R iv = R(1) / V;
+    for (int c = 0; c < 3; ++c) out[c][a] = va[c] + d[c] * iv;
+}
+
+// Compact faces of the active special cells, one lane per cell, continuing the sum that
+// k_advect handed over: the donor's value reconstructed to the face centroid with its
+// least-squares gradient, scaled by 1 - C and clamped to the two cells' values; incidences in
+// pairs (indices, then records, then values), summed in incidence order.
+template <class R>
+__global__ __launch_bounds__(256) void k_advect_compact(MeshView M, FieldsView<R> F, R* __restrict__ ox,
+                                                        R* __restrict__ oy, R* __restrict__ oz, const DevScalars* S,
+                                                        const R* __restrict__ G, const R* __restrict__ dsp) {
+    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sp >= M.nspecial) return;
+    const int64_t a = M.special_dof[sp];
+    const uint8_t m = M.mask[a];
+    const int k0 = M.special_inc_off[sp], k1 = M.special_inc_off[sp + 1];
+    const R dt = R(S[M.meta[a >> 9].inst].dt);
+    const R V = R(M.special_volume[sp]);
+    R va[3], d[3];
+    for (int c = 0; c < 3; ++c) {
+        va[c] = F.v[c][a];
+        d[c] = dsp[3 * static_cast<int64_t>(sp) + c];
+    }
+    if (!(m & kActive)) return;  // k_advect wrote the cell
+    for (int k = k0; k < k1; k += 2) {
+        int raw[2], e[2];
+        for (int j = 0; j < 2; ++j) {
+            raw[j] = k + j < k1 ? M.special_inc[k + j] : 0;
+            e[j] = raw[j] >= 0 ? raw[j] : ~raw[j];
+        }
+        R swept[2];
+        int64_t ia[2], ib[2];
+        int es[2][2];
+        for (int j = 0; j < 2; ++j) {
+            swept[j] = dt * R(M.earea[e[j]]) * F.fe[e[j]];
+            ia[j] = M.ea[e[j]];
+            ib[j] = M.eb[e[j]];
+            es[j][0] = M.esp[2 * e[j]];
+            es[j][1] = M.esp[2 * e[j] + 1];
+        }
+        R vA[2][3], vB[2][3], g[2][9], C[2], dx[2][3];
+        int ds[2];
+        for (int j = 0; j < 2; ++j) {
+            const bool lower = swept[j] >= R(0);
+            ds[j] = es[j][lower ? 0 : 1];
             for (int c = 0; c < 3; ++c) {
-                const R va = F.v[c][ia], vb = F.v[c][ib];
-                R fv = lower ? va : vb;
-                if (ds >= 0) {
-                    const R* gc = G + 9 * static_cast<int64_t>(ds) + 3 * c;
-                    const R rec = fv + (R(1) - C) * (gc[0] * dx[0] + gc[1] * dx[1] + gc[2] * dx[2]);
-                    fv = fmax(fmin(va, vb), fmin(fmax(va, vb), rec));
+                vA[j][c] = F.v[c][ia[j]];
+                vB[j][c] = F.v[c][ib[j]];
+            }
+            C[j] = R(1);
+            for (int r = 0; r < 3; ++r) dx[j][r] = R(0);
+            for (int r = 0; r < 9; ++r) g[j][r] = R(0);
+            if (ds[j] >= 0) {
+                C[j] = fmin(R(1), fabs(swept[j]) / R(M.special_volume[ds[j]]));
+                const double* dd = M.edisp + 6 * static_cast<int64_t>(e[j]) + (lower ? 0 : 3);
+                for (int r = 0; r < 3; ++r) dx[j][r] = R(dd[r]);
+                for (int r = 0; r < 9; ++r) g[j][r] = G[9 * static_cast<int64_t>(ds[j]) + r];
+            }
+        }
+        for (int j = 0; j < 2; ++j) {
+            if (k + j >= k1) break;
+            const bool lower = swept[j] >= R(0);
+            for (int c = 0; c < 3; ++c) {
+                const R fa = vA[j][c], fb = vB[j][c];
+                R fv = lower ? fa : fb;
+                if (ds[j] >= 0) {
+                    const R* gc = g[j] + 3 * c;
+                    const R rec = fv + (R(1) - C[j]) * (gc[0] * dx[j][0] + gc[1] * dx[j][1] + gc[2] * dx[j][2]);
+                    fv = fmax(fmin(fa, fb), fmin(fmax(fa, fb), rec));
                 }
-                const R flux = swept * fv;
-                d[c] += raw >= 0 ? -flux : flux;
+                const R flux = swept[j] * fv;
+                d[c] += raw[j] >= 0 ? -flux : flux;
             }
         }
     }
+    R* out[3] = {ox, oy, oz};
     if (redistributed(M, sp)) {
         hand_over(M, sp, d);
         for (int c = 0; c < 3; ++c) out[c][a] = va[c];

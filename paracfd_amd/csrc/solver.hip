@@ -559,7 +559,16 @@ private:
     R* nu_ = nullptr;
     SmagView smag_{};
     R* grad_v_ = nullptr;        // velocity gradient at the special cells (9 per cell)
+    R* adv_sp_ = nullptr;        // transport sums of the special cells over their structured faces (3 per cell)
     double* grad_p_ = nullptr;   // pressure gradient at the special cells (3 per cell)
+    // MUSCL transport of the velocity into vt_: structured faces per brick, then the compact faces,
+    // volumes and redistribution hand-over of the special cells
+    void launch_advect() {
+        LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, adv_sp_);
+        if (M_.nspecial)
+            LAUNCH(k_advect_compact<R>, blocks_for(M_.nspecial, 256), 256, M_, fields(), vt_[0], vt_[1], vt_[2], S_,
+                   static_cast<const R*>(grad_v_), static_cast<const R*>(adv_sp_));
+    }
     void grad_v() {
         if (M_.nspecial) LAUNCH(k_grad_v<R>, blocks_for(M_.nspecial, 256), 256, M_, fields(), smag_, grad_v_);
     }
@@ -1029,6 +1038,7 @@ void SolverImpl<R>::setup_gradients(const Topology& T) {
     smag_.wd = upload(wd);
     smag_.inv = upload(inv);
     grad_v_ = alloc<R>(9 * static_cast<int64_t>(ns));
+    adv_sp_ = alloc<R>(3 * static_cast<int64_t>(ns));
     grad_p_ = alloc<double>(3 * static_cast<int64_t>(ns));
     HIP_CHECK(hipMemset(grad_v_, 0, std::max<size_t>(1, 9 * static_cast<size_t>(ns)) * sizeof(R)));
     HIP_CHECK(hipMemset(grad_p_, 0, std::max<size_t>(1, 3 * static_cast<size_t>(ns)) * sizeof(double)));
@@ -1837,7 +1847,7 @@ void SolverImpl<R>::advect(double dt, double* dt_used, pcfd_step_stats* st) {
 template <class R>
 void SolverImpl<R>::advect_core() {
     grad_v();
-    probed(2, [&] { LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); });
+    probed(2, [&] { launch_advect(); });
     for (int c = 0; c < 3; ++c) std::swap(v_[c], vt_[c]);
     redistribute();
     const bool smag = opt_.smagorinsky_cs > 0;
@@ -2149,7 +2159,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
                 else
                     LAUNCH(k_smooth0<double>, nb_, kBL, M_, 1, static_cast<double*>(vc0_[0]), static_cast<double*>(vc0_[1]), r_, z_, rprev_, 0.6, partials_, S_);
                 break;
-            case 2: LAUNCH(k_advect<R>, nb_, 512, M_, fields(), vt_[0], vt_[1], vt_[2], S_, static_cast<const R*>(grad_v_)); break;
+            case 2: launch_advect(); break;
             case 3: precondition_dispatch(true); break;
             case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_setv(), nn_rstore_, r_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
             case 5: LAUNCH(k_cg_dir_apply, nb_, kBL, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;
