@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 namespace pcfd {
 
@@ -179,33 +180,66 @@ struct TileFetch2 {
     int hidx[2];
 };
 
-template <class A, class Fn>
-__device__ __forceinline__ TileFetch2<A> fetch_tile_x2(const MeshView& M, int b, Fn val) {
+// Values gathered in three phases so that every load of a phase is in flight before the
+// first is used: `load(c)` reads cell c's raw values (no arithmetic on them), `fill(raw)` the
+// loads that depend on them (an index read in `load`), `comb(raw)` forms the value.  The halo
+// slots are filled without branches: a lane whose slot has no neighbour brick (or that has no
+// slot) reads its own cell and keeps 0, so the loads of one phase form one basic block.
+struct NoFill {
+    template <class T>
+    __device__ __forceinline__ void operator()(T&) const {}
+};
+struct Same {
+    template <class T>
+    __device__ __forceinline__ T operator()(const T& v) const { return v; }
+};
+
+// `keep` (if not null) receives the raw values of the lane's two cells
+template <class A, class Ld, class Fl, class Cb, class Raw = decltype(std::declval<Ld>()(int64_t(0)))>
+__device__ __forceinline__ TileFetch2<A> fetch_tile_x2(const MeshView& M, int b, Ld load, Fl fill, Cb comb,
+                                                       Raw* keep = nullptr) {
     const int t = threadIdx.x;
     TileFetch2<A> f;
     const int64_t base = static_cast<int64_t>(b) * 512;
+    Raw ro[2], rh[2];
+    bool hv[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) f.own[h] = val(base + t + kBL * h);
+    for (int h = 0; h < 2; ++h) ro[h] = load(base + t + kBL * h);
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        f.halo[r] = A(0);
-        f.hidx[r] = -1;
         const int s = t + kBL * r;
-        if (s >= 384) continue;
-        const int face = __builtin_amdgcn_readfirstlane(s >> 6), tt = s & 63, axis = face >> 1, up = face & 1;
+        const bool slot = s < 384;  // uniform over a wave
+        const int face = __builtin_amdgcn_readfirstlane(slot ? s >> 6 : 0), tt = s & 63, axis = face >> 1, up = face & 1;
         const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
         int c[3];
         c[a1] = tt & 7;
         c[a2] = tt >> 3;
         c[axis] = up ? 8 : -1;
-        f.hidx[r] = tile_index(c[0], c[1], c[2]);
+        f.hidx[r] = slot ? tile_index(c[0], c[1], c[2]) : -1;
         const int nb = M.meta[b].nbr[face];
-        if (nb >= 0) {
-            c[axis] = up ? 0 : 7;
-            f.halo[r] = val(static_cast<int64_t>(nb) * 512 + c[0] + 8 * c[1] + 64 * c[2]);
-        }
+        hv[r] = slot && nb >= 0;
+        c[axis] = up ? 0 : 7;
+        rh[r] = load(hv[r] ? static_cast<int64_t>(nb) * 512 + c[0] + 8 * c[1] + 64 * c[2] : base + t);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) fill(ro[h]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) fill(rh[r]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) f.own[h] = comb(ro[h]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) f.halo[r] = hv[r] ? comb(rh[r]) : A(0);
+    if (keep) {
+        keep[0] = ro[0];
+        keep[1] = ro[1];
     }
     return f;
+}
+
+// one-function form: `val(c)` loads and forms the value of cell c
+template <class A, class Fn>
+__device__ __forceinline__ TileFetch2<A> fetch_tile_x2(const MeshView& M, int b, Fn val) {
+    return fetch_tile_x2<A>(M, b, val, NoFill{}, Same{});
 }
 
 // Write the fetched values into the tile and synchronise.  Every lane must call it.
@@ -231,8 +265,8 @@ struct CompactFetch {
     int k0, k1;
 };
 
-template <class A, class Fn>
-__device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int sp, Fn val) {
+template <class A, class Ld, class Fl, class Cb>
+__device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int sp, Ld load, Fl fill, Cb comb) {
     CompactFetch<A> f;
     f.k0 = f.k1 = 0;
 #pragma unroll
@@ -246,10 +280,21 @@ __device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int 
             o[j] = M.sp4_other[4 * static_cast<int64_t>(sp) + j];
             f.c[j] = A(M.sp4_coef[4 * static_cast<int64_t>(sp) + j]);
         }
+        using Raw = decltype(load(int64_t(0)));
+        Raw rv[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) f.v[j] = val(static_cast<int64_t>(o[j]));
+        for (int j = 0; j < 4; ++j) rv[j] = load(static_cast<int64_t>(o[j]));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fill(rv[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f.v[j] = comb(rv[j]);
     }
     return f;
+}
+
+template <class A, class Fn>
+__device__ __forceinline__ CompactFetch<A> fetch_compact(const MeshView& M, int sp, Fn val) {
+    return fetch_compact<A>(M, sp, val, NoFill{}, Same{});
 }
 
 // compact-edge part of (A y)_a from a fetch (in incidence order; incidences past the fourth
@@ -325,6 +370,15 @@ __device__ __forceinline__ void compact_apply(const MeshView& M, int sp, Fn val,
 }
 
 __device__ __forceinline__ int axis_coord(int q, int axis) { return (q >> (3 * axis)) & 7; }
+
+// p[i] where ok, else dflt, read without a branch (from p[safe], a valid index, and
+// discarded), so that the loads of a phase stay in one basic block and are all in flight
+// before the first use
+template <class T>
+__device__ __forceinline__ T load_if(const T* __restrict__ p, bool ok, int64_t i, int64_t safe, T dflt) {
+    const T v = p[ok ? i : safe];
+    return ok ? v : dflt;
+}
 
 // XCD-aware brick order: workgroups are dispatched round-robin over the 8 XCDs of the
 // MI355X, so block g runs on XCD g % 8.  The bricks (Morton order) are dealt to the XCDs in

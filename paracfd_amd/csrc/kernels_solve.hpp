@@ -165,21 +165,34 @@ __global__ __launch_bounds__(kBL) void k_cg_apply_dot(MeshView M, const double* 
 
 // CG direction update fused into the operator apply: d_new = z + beta d_old (first
 // iteration: z), Ad = A d_new and the d_new.Ad partial; d is double-buffered
-__global__ __launch_bounds__(kBL) void k_cg_dir_apply(MeshView M, const double* __restrict__ z,
+template <class ZT>
+struct ZD {
+    ZT z;
+    double d;
+};
+template <class ZT, bool INITIAL>
+__global__ __launch_bounds__(kBL, 6) void k_cg_dir_apply(MeshView M, const ZT* __restrict__ z,
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
-                                                      double* __restrict__ Ad, double* partials, const DevScalars* S,
-                                                      int initial) {
+                                                      double* __restrict__ Ad, double* partials, const DevScalars* S) {
     PCFD_BRICK2_BEGIN;
     const DevScalars& I = S[M.meta[b].inst];
-    const double beta = initial ? 0.0 : I.beta;
-    auto dn = [&](int64_t c) -> double { return initial ? z[c] : z[c] + beta * dold[c]; };
-    const TileFetch2<double> tf = fetch_tile_x2<double>(M, b, dn);
+    const double beta = INITIAL ? 0.0 : I.beta;
+    auto ld = [&](int64_t c) {
+        if constexpr (INITIAL) return z[c];
+        else return ZD<ZT>{z[c], dold[c]};
+    };
+    auto cb = [&](const auto& r) -> double {
+        if constexpr (INITIAL) return double(r);
+        else return double(r.z) + beta * r.d;
+    };
+    auto dn = [&](int64_t c) -> double { return cb(ld(c)); };
+    const TileFetch2<double> tf = fetch_tile_x2<double>(M, b, ld, NoFill{}, cb);
     const bool done = I.done != 0;
     int sp[2];
     CompactFetch<double> cf[2];
     for (int h = 0; h < 2; ++h) {
         sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
-        cf[h] = fetch_compact<double>(M, sp[h], dn);
+        cf[h] = fetch_compact<double>(M, sp[h], ld, NoFill{}, cb);
     }
     if (done) return;
     __shared__ double tile[kTileN];
@@ -361,17 +374,23 @@ __global__ __launch_bounds__(512) void k_diag0(MeshView M, DT* __restrict__ diag
 }
 
 // One sweep.  mode 0: x = w b / D (from zero); 1: x += w (b - A x) / D; 2: last sweep,
-// writes z (FP64) and the r.z (and r_prev.z) partials (two arrays of gridDim.x entries).
-template <class VT>
+// writes z (ZT: FP64, or FP32 when the sweep's arithmetic is FP32 and z is exact in it) and
+// the r.z (and r_prev.z) partials (two arrays of gridDim.x entries).
+template <class VT, class ZT = double>
 __global__ __launch_bounds__(kBL) void k_smooth0(MeshView M, int mode, const VT* __restrict__ xin, VT* __restrict__ xout,
-                                                 const double* __restrict__ rhs, double* __restrict__ zout,
+                                                 const double* __restrict__ rhs, ZT* __restrict__ zout,
                                                  const double* __restrict__ rprev, double omega, double* partials,
                                                  const DevScalars* S) {
     using A = typename Arith<VT>::type;
     PCFD_BRICK2_BEGIN;
     const bool from_zero = mode == 0 || xin == nullptr;
-    auto val = [&](int64_t c) -> A { return from_zero ? A(0) : A(to_f(xin[c])); };
-    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, val);
+    // from zero every neighbour value is 0: the loads read a valid array of at least n values
+    // and are discarded, so that no load sits behind a branch
+    const VT* xsrc = from_zero ? reinterpret_cast<const VT*>(rhs) : xin;
+    auto ld = [&](int64_t c) -> VT { return xsrc[c]; };
+    auto cb = [&](VT v) -> A { return from_zero ? A(0) : A(to_f(v)); };
+    auto val = [&](int64_t c) -> A { return cb(ld(c)); };
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, ld, NoFill{}, cb);
     double rb[2], rp[2];
     for (int h = 0; h < 2; ++h) {
         rb[h] = rhs[base + t + kBL * h];
@@ -382,7 +401,7 @@ __global__ __launch_bounds__(kBL) void k_smooth0(MeshView M, int mode, const VT*
     CompactFetch<A> cf[2];
     for (int h = 0; h < 2; ++h) {
         sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
-        cf[h] = fetch_compact<A>(M, sp[h], val);
+        cf[h] = fetch_compact<A>(M, sp[h], ld, NoFill{}, cb);
     }
     if (done) return;
     __shared__ A tile[kTileN];
@@ -407,7 +426,7 @@ __global__ __launch_bounds__(kBL) void k_smooth0(MeshView M, int mode, const VT*
             continue;
         }
         const double z = double(v);
-        zout[base + q] = z;
+        zout[base + q] = static_cast<ZT>(v);
         s1 += rb[h] * z;
         s2 += rp[h] * z;
     }
@@ -451,8 +470,11 @@ __global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 6 : 8) void k_vc_pre2(MeshVi
     using A = typename Arith<VT>::type;
     PCFD_BRICK2_BEGIN;
     const A w = A(omega), w2 = A(omega2);
-    auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rhs[c]) / diag[c]); };
-    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, x1);
+    struct Raw { double r; A dg; };
+    auto ld = [&](int64_t c) { return Raw{rhs[c], diag[c]}; };
+    auto cb = [&](const Raw& v) -> A { return round_vt<VT>(w * A(v.r) / v.dg); };
+    auto x1 = [&](int64_t c) -> A { return cb(ld(c)); };
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, ld, NoFill{}, cb);
     double rb[2];
     for (int h = 0; h < 2; ++h) rb[h] = rhs[base + t + kBL * h];
     const bool done = brick_done(M, S, b);
@@ -460,7 +482,7 @@ __global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 6 : 8) void k_vc_pre2(MeshVi
     CompactFetch<A> cf[2];
     for (int h = 0; h < 2; ++h) {
         sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
-        cf[h] = fetch_compact<A>(M, sp[h], x1);
+        cf[h] = fetch_compact<A>(M, sp[h], ld, NoFill{}, cb);
     }
     if (done) return;
     __shared__ A tile[kTileN];
@@ -484,7 +506,7 @@ __global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 6 : 8) void k_vc_pre2(MeshVi
 // compact-neighbour values are formed on the fly with the same arithmetic.  The results are
 // those of k_cg_update followed by k_vc_pre2.
 template <class VT>
-__global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 5 : 8) void k_vc_update_pre2(MeshView M, double* __restrict__ x, const double* __restrict__ rin,
+__global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 5 : 7) void k_vc_update_pre2(MeshView M, double* __restrict__ x, const double* __restrict__ rin,
                                                         double* __restrict__ rout, const double* __restrict__ d,
                                                         const double* __restrict__ Ad,
                                                         const typename Arith<VT>::type* __restrict__ diag,
@@ -495,19 +517,23 @@ __global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 5 : 8) void k_vc_update_pre2
     const DevScalars& I = S[M.meta[b].inst];
     const double alpha = I.alpha;
     const A w = A(omega), w2 = A(omega2);
-    auto x1 = [&](int64_t c) -> A { return round_vt<VT>(w * A(rin[c] - alpha * Ad[c]) / diag[c]); };
-    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, x1);
+    struct Raw { double r, ad; A dg; };
+    auto ld = [&](int64_t c) { return Raw{rin[c], Ad[c], diag[c]}; };
+    auto cb = [&](const Raw& v) -> A { return round_vt<VT>(w * A(v.r - alpha * v.ad) / v.dg); };
+    auto x1 = [&](int64_t c) -> A { return cb(ld(c)); };
+    Raw own[2];
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, ld, NoFill{}, cb, own);
     double xi[2], di[2], ri[2], adi[2];
     for (int h = 0; h < 2; ++h) {
         const int64_t a = base + t + kBL * h;
-        xi[h] = x[a]; di[h] = d[a]; ri[h] = rin[a]; adi[h] = Ad[a];
+        xi[h] = x[a]; di[h] = d[a]; ri[h] = own[h].r; adi[h] = own[h].ad;
     }
     const bool done = I.done != 0;
     int sp[2];
     CompactFetch<A> cf[2];
     for (int h = 0; h < 2; ++h) {
         sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
-        cf[h] = fetch_compact<A>(M, sp[h], x1);
+        cf[h] = fetch_compact<A>(M, sp[h], ld, NoFill{}, cb);
     }
     if (done) return;
     double rn[2], sum = 0.0;
@@ -542,8 +568,10 @@ __global__ __launch_bounds__(kBL) void k_vc_restrict0(MeshView M, const VT* __re
                                                       const DevScalars* S) {
     using A = typename Arith<VT>::type;
     PCFD_BRICK2_BEGIN;
-    auto val = [&](int64_t c) -> A { return A(to_f(x[c])); };
-    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, val);
+    auto ld = [&](int64_t c) -> VT { return x[c]; };
+    auto cb = [&](VT v) -> A { return A(to_f(v)); };
+    auto val = [&](int64_t c) -> A { return cb(ld(c)); };
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, ld, NoFill{}, cb);
     double rb[2];
     int pa[2];
     for (int h = 0; h < 2; ++h) {
@@ -555,7 +583,7 @@ __global__ __launch_bounds__(kBL) void k_vc_restrict0(MeshView M, const VT* __re
     CompactFetch<A> cf[2];
     for (int h = 0; h < 2; ++h) {
         sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
-        cf[h] = fetch_compact<A>(M, sp[h], val);
+        cf[h] = fetch_compact<A>(M, sp[h], ld, NoFill{}, cb);
     }
     if (done) return;
     __shared__ A tile[kTileN];
@@ -599,22 +627,31 @@ __global__ __launch_bounds__(kBL) void k_vc_restrict0(MeshView M, const VT* __re
     }
 }
 
-// prolongation x += e[parent] fused with one post-smoothing sweep; `last` writes z (FP64)
-// and the r.z / r_prev.z partials like k_smooth0 mode 2
-template <class VT>
+// prolongation x += e[parent] fused with one post-smoothing sweep; `last` writes z (ZT, as
+// k_smooth0) and the r.z / r_prev.z partials like k_smooth0 mode 2
+template <class VT, class ZT = double>
 __global__ __launch_bounds__(kBL) void k_vc_post_prolong(MeshView M, int last, const VT* __restrict__ x,
                                                          const VT* __restrict__ e1, const int* __restrict__ parent0,
                                                          const double* __restrict__ rhs, VT* __restrict__ xout,
-                                                         double* __restrict__ zout, const double* __restrict__ rprev,
+                                                         ZT* __restrict__ zout, const double* __restrict__ rprev,
                                                          double omega, double* partials, const DevScalars* S) {
     using A = typename Arith<VT>::type;
     PCFD_BRICK2_BEGIN;
-    auto y = [&](int64_t c) -> A {
-        const int pc = parent0[c];
-        const A xc = A(to_f(x[c]));
-        return pc >= 0 ? round_vt<VT>(xc + A(to_f(e1[pc]))) : xc;
+    // x and the parent index, then the parent's correction (index 0 for unmerged cells,
+    // discarded), then the sum
+    struct Raw { VT x; int pc; VT e; };
+    auto ld = [&](int64_t c) { return Raw{x[c], parent0[c], VT{}}; };
+    auto fl = [&](Raw& r) { r.e = e1[r.pc >= 0 ? r.pc : 0]; };
+    auto cb = [&](const Raw& r) -> A {
+        const A xc = A(to_f(r.x));
+        return r.pc >= 0 ? round_vt<VT>(xc + A(to_f(r.e))) : xc;
     };
-    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, y);
+    auto y = [&](int64_t c) -> A {
+        Raw r = ld(c);
+        fl(r);
+        return cb(r);
+    };
+    const TileFetch2<A> tf = fetch_tile_x2<A>(M, b, ld, fl, cb);
     double rb[2], rp[2];
     for (int h = 0; h < 2; ++h) {
         rb[h] = rhs[base + t + kBL * h];
@@ -625,7 +662,7 @@ __global__ __launch_bounds__(kBL) void k_vc_post_prolong(MeshView M, int last, c
     CompactFetch<A> cf[2];
     for (int h = 0; h < 2; ++h) {
         sp[h] = special_rank(M, b, t + kBL * h, (m[h] & kSpecial) != 0);
-        cf[h] = fetch_compact<A>(M, sp[h], y);
+        cf[h] = fetch_compact<A>(M, sp[h], ld, fl, cb);
     }
     if (done) return;
     __shared__ A tile[kTileN];
@@ -645,7 +682,7 @@ __global__ __launch_bounds__(kBL) void k_vc_post_prolong(MeshView M, int last, c
             continue;
         }
         const double z = double(v);
-        zout[base + q] = z;
+        zout[base + q] = static_cast<ZT>(v);
         s1 += rb[h] * z;
         s2 += rp[h] * z;
     }

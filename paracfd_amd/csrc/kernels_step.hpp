@@ -59,9 +59,9 @@ __device__ __forceinline__ double cell_h(const MeshView& M, int b) { return M.h[
 __device__ __forceinline__ int64_t face_dof(const MeshView& M, int b, int q, int axis, int sign) {
     const int stride = 1 << (3 * axis);
     const int c = ((q >> (3 * axis)) & 7) + sign;
-    if (c >= 0 && c < 8) return static_cast<int64_t>(b) * 512 + q + sign * stride;
     const int nb = M.meta[b].nbr[2 * axis + (sign > 0 ? 1 : 0)];
-    return nb >= 0 ? static_cast<int64_t>(nb) * 512 + q - sign * 7 * stride : -1;
+    const int64_t across = nb >= 0 ? static_cast<int64_t>(nb) * 512 + q - sign * 7 * stride : -1;
+    return c >= 0 && c < 8 ? static_cast<int64_t>(b) * 512 + q + sign * stride : across;
 }
 
 // the six face velocities of cell q: its +x/+y/+z faces and the +faces of its -x/-y/-z
@@ -78,7 +78,7 @@ __device__ __forceinline__ FaceVel<R> load_faces(const MeshView& M, const Fields
     for (int axis = 0; axis < 3; ++axis) {
         f.up[axis] = F.f[axis][a];
         const int64_t o = face_dof(M, b, q, axis, -1);
-        f.dn[axis] = o >= 0 ? F.f[axis][o] : R(0);
+        f.dn[axis] = load_if(F.f[axis], o >= 0, o, a, R(0));
     }
     return f;
 }
@@ -93,12 +93,12 @@ __device__ __forceinline__ void fold_edge_fluxes(const MeshView& M, const Fields
         int raw[4];
         double ar[4], fv[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) raw[j] = k + j < k1 ? M.special_inc[k + j] : 0;
+        for (int j = 0; j < 4; ++j) raw[j] = load_if(M.special_inc, k + j < k1, k + j, k, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int e = raw[j] >= 0 ? raw[j] : ~raw[j];
-            ar[j] = k + j < k1 ? M.earea[e] : 0.0;
-            fv[j] = k + j < k1 ? double(F.fe[e]) : 0.0;
+            ar[j] = load_if(M.earea, k + j < k1, e, e, 0.0);
+            fv[j] = double(load_if(F.fe, k + j < k1, e, e, R(0)));
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -304,7 +304,8 @@ __device__ __forceinline__ void stage_tile2(const MeshView& M, const FieldsView<
     for (int r = 0; r < 2; ++r) {
         for (int c = 0; c < 3; ++c) hv[r][c] = R(0);
         const int h = q + 512 * r;
-        if (h >= 768) continue;
+        const bool slot = h < 768;  // uniform over a wave: the other waves skip these loads
+        if (!slot) continue;
         const int face = __builtin_amdgcn_readfirstlane(h >> 7), layer = (h >> 6) & 1, t = h & 63;
         const int axis = face >> 1, up = face & 1, a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
         int cc[3];
@@ -313,12 +314,11 @@ __device__ __forceinline__ void stage_tile2(const MeshView& M, const FieldsView<
         cc[axis] = up ? 8 + layer : -1 - layer;
         hidx[r] = tile2_index(cc[0], cc[1], cc[2]);
         const int nb = M.meta[b].nbr[face];
-        if (nb >= 0) {
-            cc[axis] = up ? layer : 7 - layer;
-            const int64_t o = static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2];
-            for (int c = 0; c < 3; ++c) hv[r][c] = F.v[c][o];
-            hm[r] = M.mask[o];
-        }
+        const bool ok = nb >= 0;
+        cc[axis] = up ? layer : 7 - layer;
+        const int64_t o = static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2];
+        for (int c = 0; c < 3; ++c) hv[r][c] = load_if(F.v[c], ok, o, a, R(0));
+        hm[r] = load_if(M.mask, ok, o, a, uint8_t(0));
     }
     const int ti = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
     for (int c = 0; c < 3; ++c) vt[c][ti] = own[c];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             const int nb = M.meta[b].nbr[2 * axis];
             o = nb >= 0 ? static_cast<int64_t>(nb) * 512 + q + 7 * stride : -1;
         }
-        fn[axis] = o >= 0 ? F.f[axis][o] : R(0);
+        fn[axis] = load_if(F.f[axis], o >= 0, o, a, R(0));
     }
     const bool low = q < 192;  // waves 0-2
     const int baxis = __builtin_amdgcn_readfirstlane(q >> 6), bt = q & 63;
@@ -372,7 +372,8 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
         cc[baxis] = 0;
         tb = tile2_index(cc[0], cc[1], cc[2]);
         const int nb = M.meta[b].nbr[2 * baxis];
-        if (nb >= 0) ub = F.f[baxis][static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2] + 7 * (1 << (3 * baxis))];
+        ub = load_if(F.f[baxis], nb >= 0, static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2] + 7 * (1 << (3 * baxis)),
+                     a, R(0));
     }
     const double dt_inst = S[M.meta[b].inst].dt;
     const R h = R(M.meta[b].h), A = h * h;
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(256) void k_advect_compact(MeshView M, FieldsView<R
     for (int k = k0; k < k1; k += 2) {
         int raw[2], e[2];
         for (int j = 0; j < 2; ++j) {
-            raw[j] = k + j < k1 ? M.special_inc[k + j] : 0;
+            raw[j] = load_if(M.special_inc, k + j < k1, k + j, k, 0);
             e[j] = raw[j] >= 0 ? raw[j] : ~raw[j];
         }
         R swept[2];
@@ -621,13 +622,14 @@ __global__ __launch_bounds__(kBL) void k_diffuse(MeshView M, const R* __restrict
                 double dc[4];
                 R vo[4][3];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) raw[u] = j + u < j1 ? M.special_inc[j + u] : 0;
+                for (int u = 0; u < 4; ++u) raw[u] = load_if(M.special_inc, j + u < j1, j + u, j, 0);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int e = raw[u] >= 0 ? raw[u] : ~raw[u];
                     const bool ok = j + u < j1;
-                    o[u] = ok ? (raw[u] >= 0 ? M.eb[e] : M.ea[e]) : a;
-                    dc[u] = ok ? M.edcoef[e] : 0.0;
+                    const int64_t eb = M.eb[e], ea = M.ea[e];
+                    o[u] = ok ? (raw[u] >= 0 ? eb : ea) : a;
+                    dc[u] = load_if(M.edcoef, ok, e, e, 0.0);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
@@ -814,7 +816,7 @@ __global__ __launch_bounds__(kBL) void k_reconstruct(MeshView M, FieldsView<R> F
         for (int axis = 0; axis < 3; ++axis) {
             va[k][axis] = F.v[axis][a];
             const int64_t o = face_dof(M, b, q, axis, 1);
-            vn[k][axis] = o >= 0 ? F.v[axis][o] : R(0);
+            vn[k][axis] = load_if(F.v[axis], o >= 0, o, a, R(0));
         }
     }
     for (int k = 0; k < 2; ++k) {
@@ -942,7 +944,7 @@ __global__ __launch_bounds__(kBL) void k_correct(MeshView M, FieldsView<R> F, co
         for (int axis = 0; axis < 3; ++axis) {
             fu[k][axis] = F.f[axis][a];
             const int64_t o = face_dof(M, b, q, axis, 1);
-            pn[k][axis] = o >= 0 ? p[o] : 0.0;
+            pn[k][axis] = load_if(p, o >= 0, o, a, 0.0);
         }
     }
     for (int k = 0; k < 2; ++k) {
@@ -1008,8 +1010,8 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
         pa[k] = p[a];
         for (int axis = 0; axis < 3; ++axis) {
             const int64_t ou = face_dof(M, b, q, axis, 1), od = face_dof(M, b, q, axis, -1);
-            pp[k][axis] = ou >= 0 ? p[ou] : 0.0;
-            pm[k][axis] = od >= 0 ? p[od] : 0.0;
+            pp[k][axis] = load_if(p, ou >= 0, ou, a, 0.0);
+            pm[k][axis] = load_if(p, od >= 0, od, a, 0.0);
             v0[k][axis] = F.v[axis][a];
         }
     }
@@ -1054,24 +1056,24 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
                 double w[2], ar[2], pA[2], pB[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    raw[u] = tt + u < t1 ? M.special_inc[tt + u] : 0;
+                    raw[u] = load_if(M.special_inc, tt + u < t1, tt + u, tt, 0);
                     e[u] = raw[u] >= 0 ? raw[u] : ~raw[u];
                 }
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const bool ok = tt + u < t1;
-                    w[u] = ok ? M.ew[e[u]] : 0.0;
-                    ar[u] = ok ? M.earea[e[u]] : 0.0;
-                    ia[u] = ok ? M.ea[e[u]] : 0;
-                    ib[u] = ok ? M.eb[e[u]] : 0;
-                    sa[u] = ok ? M.esp[2 * e[u]] : -1;
-                    sb[u] = ok ? M.esp[2 * e[u] + 1] : -1;
-                    ax[u] = ok ? M.eaxis[e[u]] : 0;
+                    w[u] = load_if(M.ew, ok, e[u], e[u], 0.0);
+                    ar[u] = load_if(M.earea, ok, e[u], e[u], 0.0);
+                    ia[u] = load_if(M.ea, ok, e[u], e[u], 0);
+                    ib[u] = load_if(M.eb, ok, e[u], e[u], 0);
+                    sa[u] = load_if(M.esp, ok, 2 * e[u], 2 * e[u], -1);
+                    sb[u] = load_if(M.esp, ok, 2 * e[u] + 1, 2 * e[u] + 1, -1);
+                    ax[u] = load_if(M.eaxis, ok, e[u], e[u], int8_t(0));
                 }
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    pA[u] = tt + u < t1 ? p[ia[u]] : 0.0;
-                    pB[u] = tt + u < t1 ? p[ib[u]] : 0.0;
+                    pA[u] = load_if(p, tt + u < t1, ia[u], ia[u], 0.0);
+                    pB[u] = load_if(p, tt + u < t1, ib[u], ib[u], 0.0);
                 }
                 for (int u = 0; u < 2; ++u) {
                     if (tt + u >= t1) break;

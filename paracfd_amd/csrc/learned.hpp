@@ -291,15 +291,16 @@ __device__ __forceinline__ NNBrick nn_brick(const int* list) {
 
 // r' = r - A z on the network's bricks into the FP32 store (0 on inactive cells), after the
 // V-cycle's last kernel wrote z.  S / off may be null (no done / switched-off test).
+template <class ZT>
 __global__ __launch_bounds__(512) void k_nn_resid(MeshView M, const int* __restrict__ list, const double* __restrict__ r,
-                                                  const double* __restrict__ z, float* __restrict__ store,
+                                                  const ZT* __restrict__ z, float* __restrict__ store,
                                                   const DevScalars* S, const int* __restrict__ off) {
     __shared__ double ztile[kTileN];
     const NNBrick B = nn_brick(list);
     const int q = threadIdx.x;
     const int64_t a = static_cast<int64_t>(B.b) * 512 + q;
     const uint8_t m = M.mask[a];
-    auto zv = [&](int64_t c) -> double { return z[c]; };
+    auto zv = [&](int64_t c) -> double { return double(z[c]); };
     const TileFetch<double> tf = fetch_tile<double>(M, B.b, zv);
     const double ra = r[a];
     const bool skip = (S && S->done) || (off && *off);

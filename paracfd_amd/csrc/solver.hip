@@ -292,7 +292,15 @@ private:
     // One PCG iteration: d_new = z + beta d_old with A d_new fused, alpha, x/r update,
     // preconditioner, beta.  With the fused V-cycle the x/r update runs inside its first pass.
     void cg_iteration(bool initial, double* dold, double* dnew) {
-        probed(5, [&] { LAUNCH(k_cg_dir_apply, nb_, kBL, M_, z_, dold, dnew, Ad_, partials_, S_, initial ? 1 : 0); });
+        probed(5, [&] {
+            if (zf()) {
+                if (initial) LAUNCH((k_cg_dir_apply<float, true>), nb_, kBL, M_, zf32(), dold, dnew, Ad_, partials_, S_);
+                else LAUNCH((k_cg_dir_apply<float, false>), nb_, kBL, M_, zf32(), dold, dnew, Ad_, partials_, S_);
+            } else {
+                if (initial) LAUNCH((k_cg_dir_apply<double, true>), nb_, kBL, M_, z_, dold, dnew, Ad_, partials_, S_);
+                else LAUNCH((k_cg_dir_apply<double, false>), nb_, kBL, M_, z_, dold, dnew, Ad_, partials_, S_);
+            }
+        });
         finalize(FIN_APPLY, nb_);
         if (update_fused()) {
             double* rin = r_;
@@ -377,7 +385,7 @@ private:
     // rejected by validation) is not launched at all
     void pc_tail(int fin) {
         if (nn_ && nn_on_) {
-            LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, S_, nn_off_);
+            LAUNCH(k_nn_resid<double>, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, S_, nn_off_);
             LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_setv(), nn_rstore_, r_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_);
             LAUNCH(k_nn_add, nn_grid(), 512, nn_list_, nn_dz_, z_, S_, nn_off_);
         }
@@ -499,6 +507,22 @@ private:
     // rounding of order 1e-7, so Fletcher-Reeves keeps its iteration counts and saves the r_prev
     // traffic.  The captured graphs are dropped whenever the network is switched.
     bool flexible() const { return opt_.vcycle_precision == PCFD_BF16 || (nn_ && nn_on_); }
+    // z = M^-1 r is stored in FP32 where the V-cycle's last sweep computes it in FP32 (FP32 or
+    // BF16 storage): the values are the same, 4 bytes fewer per cell to write and to read.  FP64
+    // with the network on (z + dz), with the FP64 V-cycle and for the FGMRES basis.
+    bool zf() const { return opt_.vcycle_precision != PCFD_FP64 && !(nn_ && nn_on_) && !nonorth_; }
+    float* zf32() const { return reinterpret_cast<float*>(z_); }
+    // f(z) with z_ as the z type of a V-cycle in VT
+    template <class VT, class F>
+    void with_z(F&& f) {
+        if constexpr (!std::is_same<VT, double>::value) {
+            if (zf()) {
+                f(zf32());
+                return;
+            }
+        }
+        f(z_);
+    }
     void* vc0_[2] = {};  // composite-level V-cycle buffers (8 bytes per dof)
     double* partials_ = nullptr;
     DevScalars* S_ = nullptr;     // ninst_ instance records, then the batch aggregate
@@ -1290,8 +1314,10 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
     const int ini = initial ? 1 : 0;
     const int fin = ini ? FIN_RZ_INITIAL : FIN_RZ;
     if (opt_.preconditioner == PCFD_PC_JACOBI) {
-        LAUNCH(k_smooth0<VT>, nb_, kBL, M_, 2, static_cast<const VT*>(nullptr), static_cast<VT*>(nullptr), r_, z_, rprev_, 1.0,
-               partials_, S_);
+        with_z<VT>([&](auto* z) {
+            LAUNCH((k_smooth0<VT, std::remove_pointer_t<decltype(z)>>), nb_, kBL, M_, 2, static_cast<const VT*>(nullptr),
+                   static_cast<VT*>(nullptr), r_, z, rprev_, 1.0, partials_, S_);
+        });
         finalize(fin, nb_);
         return;
     }
@@ -1313,11 +1339,19 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
         LAUNCH(k_vc_restrict0<VT>, nb_, kBL, M_, cur, r_, parent0_, static_cast<VT*>(lv_[1].B), S_);
         VT* xc = coarse_cycle<VT>(1);
         const int last = opt_.post_smooth == 1 ? 1 : 0;
-        LAUNCH(k_vc_post_prolong<VT>, nb_, kBL, M_, last, cur, xc, parent0_, r_, oth, z_, rprev_, w[0], partials_, S_);
+        with_z<VT>([&](auto* z) {
+            LAUNCH((k_vc_post_prolong<VT, std::remove_pointer_t<decltype(z)>>), nb_, kBL, M_, last, cur, xc, parent0_, r_, oth,
+                   z, rprev_, w[0], partials_, S_);
+        });
         std::swap(cur, oth);
         for (int s = 1; s < opt_.post_smooth; ++s) {
             const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-            probed(1, [&] { LAUNCH(k_smooth0<VT>, nb_, kBL, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_); });
+            probed(1, [&] {
+                with_z<VT>([&](auto* z) {
+                    LAUNCH((k_smooth0<VT, std::remove_pointer_t<decltype(z)>>), nb_, kBL, M_, mode, cur, oth, r_, z, rprev_,
+                           w[s & 1], partials_, S_);
+                });
+            });
             std::swap(cur, oth);
         }
         pc_tail(fin);
@@ -1337,7 +1371,10 @@ void SolverImpl<R>::precondition(bool initial, const double* upd_d, const double
     }
     for (int s = 0; s < opt_.post_smooth; ++s) {
         const int mode = s == opt_.post_smooth - 1 ? 2 : 1;
-        LAUNCH(k_smooth0<VT>, nb_, kBL, M_, mode, cur, oth, r_, z_, rprev_, w[s & 1], partials_, S_);
+        with_z<VT>([&](auto* z) {
+            LAUNCH((k_smooth0<VT, std::remove_pointer_t<decltype(z)>>), nb_, kBL, M_, mode, cur, oth, r_, z, rprev_, w[s & 1],
+                   partials_, S_);
+        });
         std::swap(cur, oth);
     }
     pc_tail(fin);
@@ -1488,7 +1525,7 @@ void SolverImpl<R>::enqueue_fgmres_direction(int j) {
     }
     r_ = r_save;
     z_ = z_save;
-    LAUNCH(k_cg_dir_apply, nb_, kBL, M_, zj, zj, dbuf_[0], w, partials_, S_, 1);
+    LAUNCH((k_cg_dir_apply<double, true>), nb_, kBL, M_, zj, zj, dbuf_[0], w, partials_, S_);
     if (NV_.nf > 0) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, zj);
         LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, zj, w, 1.0);
@@ -1532,7 +1569,7 @@ void SolverImpl<R>::full_residual(double* r, const double* x) {
 // y = A x with the full (non-orthogonal) operator
 template <class R>
 void SolverImpl<R>::apply_full(const double* x, double* y) {
-    LAUNCH(k_cg_dir_apply, nb_, kBL, M_, x, x, dbuf_[0], y, partials_, S_, 1);
+    LAUNCH((k_cg_dir_apply<double, true>), nb_, kBL, M_, x, x, dbuf_[0], y, partials_, S_);
     if (NV_.nf > 0) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, x);
         LAUNCH(k_no_apply, blocks_for(NV_.ninc, 256), 256, NV_, x, y, 1.0);
@@ -2162,7 +2199,10 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
             case 2: launch_advect(); break;
             case 3: precondition_dispatch(true); break;
             case 4: LAUNCH(k_nn_apply, nn_grid(), 512, M_, nnv(), nn_setv(), nn_rstore_, r_, nn_dz_, rprev_, partials_, nb_, S_, nn_off_); break;
-            case 5: LAUNCH(k_cg_dir_apply, nb_, kBL, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_, 0); break;
+            case 5:
+                if (zf()) LAUNCH((k_cg_dir_apply<float, false>), nb_, kBL, M_, zf32(), dbuf_[0], dbuf_[1], Ad_, partials_, S_);
+                else LAUNCH((k_cg_dir_apply<double, false>), nb_, kBL, M_, z_, dbuf_[0], dbuf_[1], Ad_, partials_, S_);
+                break;
             case 6:  // alpha = 0 and x = z (scratch): the pressure and the residual are untouched
                 if (opt_.vcycle_precision == PCFD_FP32)
                     LAUNCH(k_vc_update_pre2<float>, nb_, kBL, M_, z_, rbuf_[0], rbuf_[1], dbuf_[1], Ad_,
@@ -2181,7 +2221,8 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         if (!nn_) throw std::invalid_argument("kernel 4 needs the learned preconditioner");
         // a residual with every brick non-zero so that each selected brick runs the network
         LAUNCH(k_cg_residual, nb_, kBL, M_, p_, rhs_, r_, partials_, S_);
-        LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, nullptr, nullptr);
+        if (zf()) LAUNCH(k_nn_resid<float>, nn_grid(), 512, M_, nn_list_, r_, zf32(), nn_rstore_, nullptr, nullptr);
+        else LAUNCH(k_nn_resid<double>, nn_grid(), 512, M_, nn_list_, r_, z_, nn_rstore_, nullptr, nullptr);
         set_nn_off(false);
     }
     launch();  // warm
@@ -2463,7 +2504,8 @@ void SolverImpl<R>::learned_collect(int max_pairs, pcfd_learned_stats* st) {
                 nn_inv_norm_.push_back(0.0);
             }
             float* rs = nn_rs_[slot];
-            LAUNCH(k_nn_resid, nn_grid(), 512, M_, nn_list_, r_, z_, rs, nullptr, nullptr);
+            if (zf()) LAUNCH(k_nn_resid<float>, nn_grid(), 512, M_, nn_list_, r_, zf32(), rs, nullptr, nullptr);
+            else LAUNCH(k_nn_resid<double>, nn_grid(), 512, M_, nn_list_, r_, z_, rs, nullptr, nullptr);
             const double norm = nn_norm(rs);
             if (!(norm > 0.0) || !std::isfinite(norm)) return;
             nn_inv_norm_[slot] = 1.0 / norm;
