@@ -182,9 +182,10 @@ struct TileFetch2 {
 
 // Values gathered in three phases so that every load of a phase is in flight before the
 // first is used: `load(c)` reads cell c's raw values (no arithmetic on them), `fill(raw)` the
-// loads that depend on them (an index read in `load`), `comb(raw)` forms the value.  The halo
-// slots are filled without branches: a lane whose slot has no neighbour brick (or that has no
-// slot) reads its own cell and keeps 0, so the loads of one phase form one basic block.
+// loads that depend on them (an index read in `load`), `comb(raw)` forms the value.  With a
+// `comb`, the halo slots are filled without branches: a lane whose slot has no neighbour brick
+// (or that has no slot) reads its own cell and keeps 0, so the loads of one phase form one
+// basic block.  Plain loads (no fill, no comb) skip the missing slots instead.
 struct NoFill {
     template <class T>
     __device__ __forceinline__ void operator()(T&) const {}
@@ -201,6 +202,7 @@ __device__ __forceinline__ TileFetch2<A> fetch_tile_x2(const MeshView& M, int b,
     const int t = threadIdx.x;
     TileFetch2<A> f;
     const int64_t base = static_cast<int64_t>(b) * 512;
+    constexpr bool kPure = std::is_same<Fl, NoFill>::value && std::is_same<Cb, Same>::value;
     Raw ro[2], rh[2];
     bool hv[2];
 #pragma unroll
@@ -219,7 +221,15 @@ __device__ __forceinline__ TileFetch2<A> fetch_tile_x2(const MeshView& M, int b,
         const int nb = M.meta[b].nbr[face];
         hv[r] = slot && nb >= 0;
         c[axis] = up ? 0 : 7;
-        rh[r] = load(hv[r] ? static_cast<int64_t>(nb) * 512 + c[0] + 8 * c[1] + 64 * c[2] : base + t);
+        const int64_t src = static_cast<int64_t>(nb) * 512 + c[0] + 8 * c[1] + 64 * c[2];
+        if constexpr (kPure) {
+            // plain loads: nothing uses them before the other loads are issued, so the
+            // lanes without a neighbour value skip the read
+            rh[r] = Raw{};
+            if (hv[r]) rh[r] = load(src);
+        } else {
+            rh[r] = load(hv[r] ? src : base + t);
+        }
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) fill(ro[h]);

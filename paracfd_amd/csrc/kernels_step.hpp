@@ -314,11 +314,12 @@ __device__ __forceinline__ void stage_tile2(const MeshView& M, const FieldsView<
         cc[axis] = up ? 8 + layer : -1 - layer;
         hidx[r] = tile2_index(cc[0], cc[1], cc[2]);
         const int nb = M.meta[b].nbr[face];
-        const bool ok = nb >= 0;
-        cc[axis] = up ? layer : 7 - layer;
-        const int64_t o = static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2];
-        for (int c = 0; c < 3; ++c) hv[r][c] = load_if(F.v[c], ok, o, a, R(0));
-        hm[r] = load_if(M.mask, ok, o, a, uint8_t(0));
+        if (nb >= 0) {
+            cc[axis] = up ? layer : 7 - layer;
+            const int64_t o = static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2];
+            for (int c = 0; c < 3; ++c) hv[r][c] = F.v[c][o];
+            hm[r] = M.mask[o];
+        }
     }
     const int ti = tile2_index(q & 7, (q >> 3) & 7, q >> 6);
     for (int c = 0; c < 3; ++c) vt[c][ti] = own[c];
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
             const int nb = M.meta[b].nbr[2 * axis];
             o = nb >= 0 ? static_cast<int64_t>(nb) * 512 + q + 7 * stride : -1;
         }
-        fn[axis] = load_if(F.f[axis], o >= 0, o, a, R(0));
+        fn[axis] = o >= 0 ? F.f[axis][o] : R(0);
     }
     const bool low = q < 192;  // waves 0-2
     const int baxis = __builtin_amdgcn_readfirstlane(q >> 6), bt = q & 63;
@@ -372,8 +373,7 @@ __global__ __launch_bounds__(512) void k_advect(MeshView M, FieldsView<R> F, R* 
         cc[baxis] = 0;
         tb = tile2_index(cc[0], cc[1], cc[2]);
         const int nb = M.meta[b].nbr[2 * baxis];
-        ub = load_if(F.f[baxis], nb >= 0, static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2] + 7 * (1 << (3 * baxis)),
-                     a, R(0));
+        if (nb >= 0) ub = F.f[baxis][static_cast<int64_t>(nb) * 512 + cc[0] + 8 * cc[1] + 64 * cc[2] + 7 * (1 << (3 * baxis))];
     }
     const double dt_inst = S[M.meta[b].inst].dt;
     const R h = R(M.meta[b].h), A = h * h;
@@ -944,7 +944,7 @@ __global__ __launch_bounds__(kBL) void k_correct(MeshView M, FieldsView<R> F, co
         for (int axis = 0; axis < 3; ++axis) {
             fu[k][axis] = F.f[axis][a];
             const int64_t o = face_dof(M, b, q, axis, 1);
-            pn[k][axis] = load_if(p, o >= 0, o, a, 0.0);
+            pn[k][axis] = o >= 0 ? p[o] : 0.0;
         }
     }
     for (int k = 0; k < 2; ++k) {
