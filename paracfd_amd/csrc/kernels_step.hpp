@@ -992,8 +992,8 @@ __device__ __forceinline__ void wall_force_p(const MeshView& M, const double* p,
 // are loaded before the mask is tested; compact faces in groups of two incidences.
 template <class R>
 __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
-                                                 const double* __restrict__ gp, const DevScalars* S, double dt_given,
-                                                 int only_converged) {
+                                                 const DevScalars* S, double dt_given, int only_converged,
+                                                 double* __restrict__ dsp) {
     const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
     const int64_t base = static_cast<int64_t>(b) * 512;
     const DevScalars& I = S[M.meta[b].inst];
@@ -1017,14 +1017,7 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
     }
     int sp[2];
     for (int k = 0; k < 2; ++k) sp[k] = special_rank(M, b, t + kBL * k, (m[k] & kSpecial) != 0);
-    if (only_converged && !I.converged) {  // block-uniform
-        for (int k = 0; k < 2; ++k)
-            if (redistributed(M, sp[k])) {
-                const double zero[3] = {0.0, 0.0, 0.0};
-                hand_over(M, sp[k], zero);
-            }
-        return;
-    }
+    if (only_converged && !I.converged) return;  // block-uniform; k_impulse_compact hands over zeros
     const double dt = dt_given > 0.0 ? dt_given : I.dt;
     const double s = dt / M.rho;
     for (int kc = 0; kc < 2; ++kc) {
@@ -1044,64 +1037,98 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
         if (j == 7 && (bf & kYMAX)) d[1] -= bnd;
         if (kz == 0 && (bf & kZMIN)) d[2] += bnd;
         if (kz == 7 && (bf & kZMAX)) d[2] -= bnd;
-        double V = h * h * h;
-        const int spc = sp[kc];
-        if (spc >= 0) {
-            V = M.special_volume[spc];
-            // compact faces: weighted face pressure plus skew . ((1-w) g_a + w g_b)
-            // (pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390)
-            const int t0 = M.special_inc_off[spc], t1 = M.special_inc_off[spc + 1];
-            for (int tt = t0; tt < t1; tt += 2) {
-                int raw[2], e[2], ia[2], ib[2], sa[2], sb[2], ax[2];
-                double w[2], ar[2], pA[2], pB[2];
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    raw[u] = load_if(M.special_inc, tt + u < t1, tt + u, tt, 0);
-                    e[u] = raw[u] >= 0 ? raw[u] : ~raw[u];
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const bool ok = tt + u < t1;
-                    w[u] = load_if(M.ew, ok, e[u], e[u], 0.0);
-                    ar[u] = load_if(M.earea, ok, e[u], e[u], 0.0);
-                    ia[u] = load_if(M.ea, ok, e[u], e[u], 0);
-                    ib[u] = load_if(M.eb, ok, e[u], e[u], 0);
-                    sa[u] = load_if(M.esp, ok, 2 * e[u], 2 * e[u], -1);
-                    sb[u] = load_if(M.esp, ok, 2 * e[u] + 1, 2 * e[u] + 1, -1);
-                    ax[u] = load_if(M.eaxis, ok, e[u], e[u], int8_t(0));
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    pA[u] = load_if(p, tt + u < t1, ia[u], ia[u], 0.0);
-                    pB[u] = load_if(p, tt + u < t1, ib[u], ib[u], 0.0);
-                }
-                for (int u = 0; u < 2; ++u) {
-                    if (tt + u >= t1) break;
-                    double pf = (1.0 - w[u]) * pA[u] + w[u] * pB[u];
-                    if (sa[u] >= 0 && sb[u] >= 0) {
-                        const double* da = M.edisp + 6 * static_cast<int64_t>(e[u]);
-                        for (int c = 0; c < 3; ++c)
-                            pf += ((1.0 - w[u]) * da[c] + w[u] * da[3 + c]) *
-                                  ((1.0 - w[u]) * gp[3 * sa[u] + c] + w[u] * gp[3 * sb[u] + c]);
-                    }
-                    const double imp = s * ar[u] * pf;
-                    d[ax[u]] += raw[u] >= 0 ? -imp : imp;
-                }
-            }
-            const int pch = M.special_patch[spc];
-            if (pch >= 0) {
-                double Fw[3];
-                wall_force_p(M, p, gp, pch, spc, 0.0, Fw);
-                for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
-            }
-        }
-        if (redistributed(M, spc)) {
-            hand_over(M, spc, d);
+        if (sp[kc] >= 0) {  // compact faces, wall, volume and redistribution in k_impulse_compact
+            for (int c = 0; c < 3; ++c) dsp[3 * static_cast<int64_t>(sp[kc]) + c] = d[c];
             continue;
         }
-        const double iv = 1.0 / V;
+        const double iv = 1.0 / (h * h * h);
         for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(v0[kc][c]) + d[c] * iv);
     }
+}
+
+// The special cells' part of the impulse, one lane per cell, continuing the structured-face
+// sum that k_impulse handed over: compact faces (weighted face pressure plus
+// skew . ((1-w) g_a + w g_b), pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390),
+// the wall traction of the cell's patch, the fluid volume and the redistribution hand-over.
+template <class R>
+__global__ __launch_bounds__(256) void k_impulse_compact(MeshView M, FieldsView<R> F, const double* __restrict__ p,
+                                                         const double* __restrict__ gp, const DevScalars* S,
+                                                         double dt_given, int only_converged,
+                                                         const double* __restrict__ dsp) {
+    const int spc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (spc >= M.nspecial) return;
+    const int64_t a = M.special_dof[spc];
+    const uint8_t m = M.mask[a];
+    const DevScalars& I = S[M.meta[a >> 9].inst];
+    R v0[3];
+    double d[3];
+    for (int c = 0; c < 3; ++c) {
+        v0[c] = F.v[c][a];
+        d[c] = dsp[3 * static_cast<int64_t>(spc) + c];
+    }
+    const double V = M.special_volume[spc];
+    if (only_converged && !I.converged) {
+        if (redistributed(M, spc)) {
+            const double zero[3] = {0.0, 0.0, 0.0};
+            hand_over(M, spc, zero);
+        }
+        return;
+    }
+    if (!(m & kActive)) return;
+    const double dt = dt_given > 0.0 ? dt_given : I.dt;
+    const double s = dt / M.rho;
+    // compact faces: weighted face pressure plus skew . ((1-w) g_a + w g_b)
+    // (pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390)
+    const int t0 = M.special_inc_off[spc], t1 = M.special_inc_off[spc + 1];
+    for (int tt = t0; tt < t1; tt += 2) {
+        int raw[2], e[2], ia[2], ib[2], sa[2], sb[2], ax[2];
+        double w[2], ar[2], pA[2], pB[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            raw[u] = load_if(M.special_inc, tt + u < t1, tt + u, tt, 0);
+            e[u] = raw[u] >= 0 ? raw[u] : ~raw[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const bool ok = tt + u < t1;
+            w[u] = load_if(M.ew, ok, e[u], e[u], 0.0);
+            ar[u] = load_if(M.earea, ok, e[u], e[u], 0.0);
+            ia[u] = load_if(M.ea, ok, e[u], e[u], 0);
+            ib[u] = load_if(M.eb, ok, e[u], e[u], 0);
+            sa[u] = load_if(M.esp, ok, 2 * e[u], 2 * e[u], -1);
+            sb[u] = load_if(M.esp, ok, 2 * e[u] + 1, 2 * e[u] + 1, -1);
+            ax[u] = load_if(M.eaxis, ok, e[u], e[u], int8_t(0));
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            pA[u] = load_if(p, tt + u < t1, ia[u], ia[u], 0.0);
+            pB[u] = load_if(p, tt + u < t1, ib[u], ib[u], 0.0);
+        }
+        for (int u = 0; u < 2; ++u) {
+            if (tt + u >= t1) break;
+            double pf = (1.0 - w[u]) * pA[u] + w[u] * pB[u];
+            if (sa[u] >= 0 && sb[u] >= 0) {
+                const double* da = M.edisp + 6 * static_cast<int64_t>(e[u]);
+                for (int c = 0; c < 3; ++c)
+                    pf += ((1.0 - w[u]) * da[c] + w[u] * da[3 + c]) *
+                          ((1.0 - w[u]) * gp[3 * sa[u] + c] + w[u] * gp[3 * sb[u] + c]);
+            }
+            const double imp = s * ar[u] * pf;
+            d[ax[u]] += raw[u] >= 0 ? -imp : imp;
+        }
+    }
+    const int pch = M.special_patch[spc];
+    if (pch >= 0) {
+        double Fw[3];
+        wall_force_p(M, p, gp, pch, spc, 0.0, Fw);
+        for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
+    }
+    if (redistributed(M, spc)) {
+        hand_over(M, spc, d);
+        return;
+    }
+    const double iv = 1.0 / V;
+    for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(v0[c]) + d[c] * iv);
 }
 
 // ---------------------------------------------------------------------------------------

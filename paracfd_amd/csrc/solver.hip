@@ -584,7 +584,16 @@ private:
     SmagView smag_{};
     R* grad_v_ = nullptr;        // velocity gradient at the special cells (9 per cell)
     R* adv_sp_ = nullptr;        // transport sums of the special cells over their structured faces (3 per cell)
+    double* imp_sp_ = nullptr;   // impulse sums of the special cells over their structured faces (3 per cell)
     double* grad_p_ = nullptr;   // pressure gradient at the special cells (3 per cell)
+    // pressure impulse (dt > 0: that dt, else each instance's): structured faces per brick, then
+    // the compact faces, wall traction, volumes and redistribution hand-over of the special cells
+    void launch_impulse(double dt, int only_converged) {
+        LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, S_, dt, only_converged, imp_sp_);
+        if (M_.nspecial)
+            LAUNCH(k_impulse_compact<R>, blocks_for(M_.nspecial, 256), 256, M_, fields(), p_,
+                   static_cast<const double*>(grad_p_), S_, dt, only_converged, static_cast<const double*>(imp_sp_));
+    }
     // MUSCL transport of the velocity into vt_: structured faces per brick, then the compact faces,
     // volumes and redistribution hand-over of the special cells
     void launch_advect() {
@@ -1063,6 +1072,7 @@ void SolverImpl<R>::setup_gradients(const Topology& T) {
     smag_.inv = upload(inv);
     grad_v_ = alloc<R>(9 * static_cast<int64_t>(ns));
     adv_sp_ = alloc<R>(3 * static_cast<int64_t>(ns));
+    imp_sp_ = alloc<double>(3 * static_cast<int64_t>(ns));
     grad_p_ = alloc<double>(3 * static_cast<int64_t>(ns));
     HIP_CHECK(hipMemset(grad_v_, 0, std::max<size_t>(1, 9 * static_cast<size_t>(ns)) * sizeof(R)));
     HIP_CHECK(hipMemset(grad_p_, 0, std::max<size_t>(1, 3 * static_cast<size_t>(ns)) * sizeof(double)));
@@ -1863,7 +1873,7 @@ void SolverImpl<R>::project(double dt, bool warm, pcfd_step_stats* st) {
 template <class R>
 void SolverImpl<R>::impulse(double dt) {
     grad_p();
-    LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, grad_p_, S_, dt, 0);
+    launch_impulse(dt, 0);
     redistribute();
 }
 
@@ -1930,7 +1940,7 @@ void SolverImpl<R>::initialize(pcfd_step_stats* st) {
     p_changed();
     project(dt, false, nullptr);
     grad_p();
-    LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, grad_p_, S_, dt, 1);  // converged instances only
+    launch_impulse(dt, 1);  // converged instances only
     redistribute();
     LAUNCH(k_vmax<R>, nb_, 512, M_, fields(), partials_);
     finalize(FIN_VMAX, nb_);
@@ -1965,7 +1975,7 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     run_projection(true, nullptr);
     // impulse and time advance for the instances whose projection converged
     grad_p();
-    LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, grad_p_, S_, 0.0, 1);
+    launch_impulse(0.0, 1);
     redistribute();
     for (int i = 0; i < ninst_; ++i)
         if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
