@@ -119,12 +119,25 @@ __device__ __forceinline__ void hand_over(const MeshView& M, int sp, const T d[3
     for (int c = 0; c < 3; ++c) M.rdelta[3 * static_cast<int64_t>(sp) + c] = double(d[c]);
 }
 
+// max |component| of a velocity, infinite where a component is not finite
+__device__ __forceinline__ double abs_max3(double x, double y, double z) {
+    const double w = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+    return (!isfinite(x) || !isfinite(y) || !isfinite(z)) ? INFINITY : w;
+}
+
+// max of non-negative doubles through their bit patterns (ordered like the values); the result
+// does not depend on the order of the updates
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
 // Every member of group g takes the group's summed increment (member order) over the group's
 // volume (finalize_redistributed_state_kernel, amr_advection.cu:142-149); out holds the
 // members' values before the increment.
+// With `vmax_part`, each member's |v| also raises its brick's slot (after k_impulse stored it).
 template <class R>
 __global__ __launch_bounds__(256) void k_redistribute(MeshView M, R* __restrict__ ox, R* __restrict__ oy,
-                                                      R* __restrict__ oz) {
+                                                      R* __restrict__ oz, double* vmax_part) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= M.ngroups) return;
     double S[3] = {0.0, 0.0, 0.0};
@@ -136,7 +149,9 @@ __global__ __launch_bounds__(256) void k_redistribute(MeshView M, R* __restrict_
     R* out[3] = {ox, oy, oz};
     for (int k = M.roff[g]; k < M.roff[g + 1]; ++k) {
         const int64_t a = M.special_dof[M.rmember[k]];
-        for (int c = 0; c < 3; ++c) out[c][a] = R(double(out[c][a]) + S[c] * iv);
+        R vn[3];
+        for (int c = 0; c < 3; ++c) out[c][a] = vn[c] = R(double(out[c][a]) + S[c] * iv);
+        if (vmax_part) atomic_max_nonneg(vmax_part + (a >> 9), abs_max3(double(vn[0]), double(vn[1]), double(vn[2])));
     }
 }
 
@@ -226,6 +241,20 @@ __global__ __launch_bounds__(256) void k_grad_p(MeshView M, SmagView G, const do
 // ---------------------------------------------------------------------------------------
 // CFL: max over cells of sum(max(outflow, 0)) / V  (amr_advection.cu:169-186) -> dt
 // ---------------------------------------------------------------------------------------
+// outflow of a cell through its structured faces and the domain's inflow / outlet faces
+template <class R>
+__device__ __forceinline__ double cell_outflow(const MeshView& M, const FaceVel<R>& f, uint16_t bf, int q, uint8_t m,
+                                               double A) {
+    double out = 0.0;
+    for (int axis = 0; axis < 3; ++axis) {
+        if (m & (kPX << axis)) out += fmax(A * double(f.up[axis]), 0.0);
+        if (m & (kMX << axis)) out += fmax(-A * double(f.dn[axis]), 0.0);
+    }
+    if ((q & 7) == 0 && (bf & kXMIN)) out += fmax(-A * M.speed, 0.0);
+    if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) out += fmax(A * double(f.up[0]), 0.0);
+    return out;
+}
+
 // Two cells per lane (kBL lanes per brick).  With `vmax`, also max |v| over the active cells
 // (a non-finite component makes it infinite) into partials[nbricks + b].
 template <class R>
@@ -250,23 +279,14 @@ __global__ __launch_bounds__(kBL) void k_outflow_rate(MeshView M, FieldsView<R> 
     for (int k = 0; k < 2; ++k) {
         const int q = t + kBL * k;
         if (!(m[k] & kActive)) continue;
-        double out = 0.0;
-        for (int axis = 0; axis < 3; ++axis) {
-            if (m[k] & (kPX << axis)) out += fmax(A * double(cf[k].up[axis]), 0.0);
-            if (m[k] & (kMX << axis)) out += fmax(-A * double(cf[k].dn[axis]), 0.0);
-        }
-        if ((q & 7) == 0 && (bf & kXMIN)) out += fmax(-A * M.speed, 0.0);
-        if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) out += fmax(A * double(cf[k].up[0]), 0.0);
+        double out = cell_outflow<R>(M, cf[k], bf, q, m[k], A);
         double V = h * h * h;
         if (sp[k] >= 0) {
             V = M.special_volume[sp[k]];
             fold_edge_fluxes<R>(M, F, sp[k], out, [](double acc, double f) { return acc + fmax(f, 0.0); });
         }
         rate = fmax(rate, out / V);
-        const double x = double(v[k][0]), y = double(v[k][1]), z = double(v[k][2]);
-        double w = fmax(fabs(x), fmax(fabs(y), fabs(z)));
-        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) w = INFINITY;
-        vm = fmax(vm, w);
+        vm = fmax(vm, abs_max3(double(v[k][0]), double(v[k][1]), double(v[k][2])));
     }
     store_partial(partials, b, block_max<kBL>(rate));
     if (vmax) {
@@ -897,11 +917,13 @@ __global__ __launch_bounds__(kBL) void k_rhs(MeshView M, FieldsView<R> F, double
 }
 
 // max |integrated divergence| / V after the correction (FP64 check); two cells per lane
+// With `rate_part`, also the outflow rate of k_outflow_rate from the same (final) faces into
+// rate_part[b]: the next step's timestep without another pass over the faces.
 template <class R>
-__global__ __launch_bounds__(kBL) void k_divcheck(MeshView M, FieldsView<R> F, double* partials) {
+__global__ __launch_bounds__(kBL) void k_divcheck(MeshView M, FieldsView<R> F, double* partials, double* rate_part) {
     const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
     const int64_t base = static_cast<int64_t>(b) * 512;
-    const double h = M.meta[b].h;
+    const double h = M.meta[b].h, A = h * h;
     const uint16_t bf = M.meta[b].flags;
     uint8_t m[2];
     FaceVel<R> cf[2];
@@ -910,16 +932,22 @@ __global__ __launch_bounds__(kBL) void k_divcheck(MeshView M, FieldsView<R> F, d
         m[k] = M.mask[base + t + kBL * k];
         cf[k] = load_faces<R>(M, F, b, t + kBL * k);
     }
-    double val = 0.0;
+    double val = 0.0, rate = 0.0;
     for (int k = 0; k < 2; ++k) {
         const int q = t + kBL * k;
         const int sp = special_rank(M, b, q, (m[k] & kSpecial) != 0);
         if (m[k] & kActive) {
             const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
             val = fmax(val, fabs(cell_divergence<R>(M, F, cf[k], bf, q, m[k], sp, h) / V));
+            if (rate_part) {
+                double out = cell_outflow<R>(M, cf[k], bf, q, m[k], A);
+                if (sp >= 0) fold_edge_fluxes<R>(M, F, sp, out, [](double acc, double f) { return acc + fmax(f, 0.0); });
+                rate = fmax(rate, out / V);
+            }
         }
     }
     store_partial(partials, b, block_max<kBL>(val));
+    if (rate_part) store_partial(rate_part, b, block_max<kBL>(rate));
 }
 
 // face correction u -= (dt/rho) dp/dn (amr_pressure.cu:285-310); outlet p = 0 half a cell away
@@ -993,7 +1021,7 @@ __device__ __forceinline__ void wall_force_p(const MeshView& M, const double* p,
 template <class R>
 __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, const double* __restrict__ p,
                                                  const DevScalars* S, double dt_given, int only_converged,
-                                                 double* __restrict__ dsp) {
+                                                 double* __restrict__ dsp, double* vmax_part) {
     const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;
     const int64_t base = static_cast<int64_t>(b) * 512;
     const DevScalars& I = S[M.meta[b].inst];
@@ -1017,8 +1045,17 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
     }
     int sp[2];
     for (int k = 0; k < 2; ++k) sp[k] = special_rank(M, b, t + kBL * k, (m[k] & kSpecial) != 0);
-    if (only_converged && !I.converged) return;  // block-uniform; k_impulse_compact hands over zeros
+    if (only_converged && !I.converged) {  // block-uniform; k_impulse_compact hands over zeros
+        if (vmax_part) {  // the velocities stay: |v|max of every active cell
+            double vm = 0.0;
+            for (int k = 0; k < 2; ++k)
+                if (m[k] & kActive) vm = fmax(vm, abs_max3(double(v0[k][0]), double(v0[k][1]), double(v0[k][2])));
+            store_partial(vmax_part, b, block_max<kBL>(vm));
+        }
+        return;
+    }
     const double dt = dt_given > 0.0 ? dt_given : I.dt;
+    double vm = 0.0;  // non-special cells; the special ones update the slot atomically later
     const double s = dt / M.rho;
     for (int kc = 0; kc < 2; ++kc) {
         const int q = t + kBL * kc;
@@ -1042,8 +1079,11 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
             continue;
         }
         const double iv = 1.0 / (h * h * h);
-        for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(v0[kc][c]) + d[c] * iv);
+        R vn[3];
+        for (int c = 0; c < 3; ++c) F.v[c][a] = vn[c] = R(double(v0[kc][c]) + d[c] * iv);
+        vm = fmax(vm, abs_max3(double(vn[0]), double(vn[1]), double(vn[2])));
     }
+    if (vmax_part) store_partial(vmax_part, b, block_max<kBL>(vm));
 }
 
 // The special cells' part of the impulse, one lane per cell, continuing the structured-face
@@ -1054,7 +1094,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_impulse_compact(MeshView M, FieldsView<R> F, const double* __restrict__ p,
                                                          const double* __restrict__ gp, const DevScalars* S,
                                                          double dt_given, int only_converged,
-                                                         const double* __restrict__ dsp) {
+                                                         const double* __restrict__ dsp, double* vmax_part) {
     const int spc = blockIdx.x * blockDim.x + threadIdx.x;
     if (spc >= M.nspecial) return;
     const int64_t a = M.special_dof[spc];
@@ -1123,12 +1163,14 @@ __global__ __launch_bounds__(256) void k_impulse_compact(MeshView M, FieldsView<
         wall_force_p(M, p, gp, pch, spc, 0.0, Fw);
         for (int c = 0; c < 3; ++c) d[c] -= s * Fw[c];
     }
-    if (redistributed(M, spc)) {
+    if (redistributed(M, spc)) {  // k_redistribute writes the velocity (and its |v|)
         hand_over(M, spc, d);
         return;
     }
     const double iv = 1.0 / V;
-    for (int c = 0; c < 3; ++c) F.v[c][a] = R(double(v0[c]) + d[c] * iv);
+    R vn[3];
+    for (int c = 0; c < 3; ++c) F.v[c][a] = vn[c] = R(double(v0[c]) + d[c] * iv);
+    if (vmax_part) atomic_max_nonneg(vmax_part + (a >> 9), abs_max3(double(vn[0]), double(vn[1]), double(vn[2])));
 }
 
 // ---------------------------------------------------------------------------------------

@@ -121,6 +121,9 @@ public:
     bool initialized = false;
     double time = 0.0;
     int64_t device_bytes = 0;
+    // The last step left the next step's outflow rate, timestep and |v|max in the device
+    // scalars; any call that may change the velocities or faces clears it.
+    bool next_rate_ready = false;
 };
 
 template <class R>
@@ -359,9 +362,10 @@ private:
         return gexec_[kind][parity];
     }
     // reduction over brick-indexed partials, one finalize workgroup per instance
-    void finalize(int op, int nparts) {
-        LAUNCH(k_finalize, ninst_, 1024, op, partials_, nparts, ninst_ > 1 ? seg_ : nullptr, S_, agg_, opt_.cfl,
-               std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations, rprev_ ? 1 : 0);
+    void finalize(int op, int nparts, const double* parts = nullptr) {
+        LAUNCH(k_finalize, ninst_, 1024, op, parts ? parts : partials_, nparts, ninst_ > 1 ? seg_ : nullptr, S_, agg_,
+               opt_.cfl, std::max(1e-9, std::abs(opt_.speed) / M_.finest_h), opt_.rtol, opt_.max_iterations,
+               rprev_ ? 1 : 0);
     }
     // dense coarse solves of the instances whose coarsest multigrid level is m
     template <class VT, class CT>
@@ -462,8 +466,8 @@ private:
     void advect_core();
     // the redistribution groups' pooled increments into the current velocity (after a kernel
     // that handed them over)
-    void redistribute() {
-        if (M_.ngroups) LAUNCH(k_redistribute<R>, blocks_for(M_.ngroups, 256), 256, M_, v_[0], v_[1], v_[2]);
+    void redistribute(double* vmax_part = nullptr) {
+        if (M_.ngroups) LAUNCH(k_redistribute<R>, blocks_for(M_.ngroups, 256), 256, M_, v_[0], v_[1], v_[2], vmax_part);
     }
     void loads_instance(int i, double pref, pcfd_loads* out);
     void enqueue_loads(int i, double pref);
@@ -588,12 +592,17 @@ private:
     double* grad_p_ = nullptr;   // pressure gradient at the special cells (3 per cell)
     // pressure impulse (dt > 0: that dt, else each instance's): structured faces per brick, then
     // the compact faces, wall traction, volumes and redistribution hand-over of the special cells
-    void launch_impulse(double dt, int only_converged) {
-        LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, S_, dt, only_converged, imp_sp_);
+    // the cells' |v| after the impulse go to vmax_part (per brick) if it is given
+    void launch_impulse(double dt, int only_converged, double* vmax_part = nullptr) {
+        LAUNCH(k_impulse<R>, nb_, kBL, M_, fields(), p_, S_, dt, only_converged, imp_sp_, vmax_part);
         if (M_.nspecial)
             LAUNCH(k_impulse_compact<R>, blocks_for(M_.nspecial, 256), 256, M_, fields(), p_,
-                   static_cast<const double*>(grad_p_), S_, dt, only_converged, static_cast<const double*>(imp_sp_));
+                   static_cast<const double*>(grad_p_), S_, dt, only_converged, static_cast<const double*>(imp_sp_),
+                   vmax_part);
     }
+    // per brick: the outflow rate of the final faces (k_divcheck), then the |v|max of the
+    // velocities after the impulse
+    double* rv_part_ = nullptr;
     // MUSCL transport of the velocity into vt_: structured faces per brick, then the compact faces,
     // volumes and redistribution hand-over of the special cells
     void launch_advect() {
@@ -808,6 +817,8 @@ void SolverImpl<R>::construct(const Topology& T) {
     vc0_[1] = alloc<double>(n_);
     const int64_t npart = std::max<int64_t>({2 * static_cast<int64_t>(nb_), 4096, 12 * (M_.npatches / 256 + 1)});
     partials_ = alloc<double>(npart);
+    rv_part_ = alloc<double>(2 * static_cast<int64_t>(nb_));
+    HIP_CHECK(hipMemset(rv_part_, 0, 2 * static_cast<size_t>(nb_) * sizeof(double)));
     // instances of a batch
     ninst_ = T.instances();
     if (ninst_ < 1) throw std::invalid_argument("topology has no instance");
@@ -1833,7 +1844,7 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
         if (NV_.nn > 0) LAUNCH(k_no_grad, blocks_for(NV_.nn, 256), 256, NV_, p_);
         LAUNCH(k_no_correct<R>, blocks_for(NV_.nf, 256), 256, NV_, fields(), p_, S_, opt_.rho);
     }
-    LAUNCH(k_divcheck<R>, nb_, kBL, M_, fields(), partials_);
+    LAUNCH(k_divcheck<R>, nb_, kBL, M_, fields(), partials_, rv_part_);
     finalize(FIN_MAXDIV, nb_);
     sync_scalars();
     for (int i = 0; i < ninst_; ++i) {
@@ -1961,8 +1972,11 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     if (!initialized) throw std::logic_error("step before a converged initialization");
     for (auto& s : inst_stats_) std::memset(&s, 0, sizeof(s));
     HIP_CHECK(hipEventRecord(ev_[2], stream_));
-    LAUNCH(k_outflow_rate<R>, nb_, kBL, M_, fields(), partials_, 1);
-    finalize(FIN_RATE_VMAX, nb_);
+    if (!next_rate_ready) {
+        LAUNCH(k_outflow_rate<R>, nb_, kBL, M_, fields(), partials_, 1);
+        finalize(FIN_RATE_VMAX, nb_);
+    }
+    next_rate_ready = false;
     sync_scalars();
     for (int i = 0; i < ninst_; ++i) {
         if (!std::isfinite(host_s_[i].vmax)) throw std::domain_error("non-finite velocity before timestep");
@@ -1975,8 +1989,11 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     run_projection(true, nullptr);
     // impulse and time advance for the instances whose projection converged
     grad_p();
-    launch_impulse(0.0, 1);
-    redistribute();
+    launch_impulse(0.0, 1, rv_part_ + nb_);
+    redistribute(rv_part_ + nb_);
+    // the next step's rate, timestep and |v|max from the final faces and velocities (the
+    // values k_outflow_rate would find)
+    finalize(FIN_RATE_VMAX, nb_, rv_part_);
     for (int i = 0; i < ninst_; ++i)
         if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
     time = inst_time_[0];
@@ -1996,6 +2013,7 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
         s.step_ms = s_ms;
         s.physical_time = inst_time_[i];
     }
+    next_rate_ready = true;
     if (st) aggregate_stats(st);
 }
 
@@ -2287,6 +2305,7 @@ void SolverImpl<R>::snapshot_state() {
 
 template <class R>
 void SolverImpl<R>::restore_state() {
+    next_rate_ready = false;
     for (int c = 0; c < 3; ++c) {
         HIP_CHECK(hipMemcpyAsync(v_[c], snap_v_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
         HIP_CHECK(hipMemcpyAsync(f_[c], snap_f_[c], n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
@@ -2757,6 +2776,7 @@ void pcfd_destroy(pcfd_solver* s) { delete s; }
 
 pcfd_status pcfd_initialize(pcfd_solver* s, pcfd_step_stats* st) {
     NEED(s, "pcfd_initialize: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_initialize", [&] { s->impl->initialize(st); });
 }
 pcfd_status pcfd_step(pcfd_solver* s, pcfd_step_stats* st) {
@@ -2766,16 +2786,19 @@ pcfd_status pcfd_step(pcfd_solver* s, pcfd_step_stats* st) {
 pcfd_status pcfd_advect(pcfd_solver* s, double dt, double* dt_used, pcfd_step_stats* st) {
     NEED(s, "pcfd_advect: null solver");
     NEED(std::isfinite(dt), "pcfd_advect: non-finite dt");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_advect", [&] { s->impl->advect(dt, dt_used, st); });
 }
 pcfd_status pcfd_project(pcfd_solver* s, double dt, int warm, pcfd_step_stats* st) {
     NEED(s, "pcfd_project: null solver");
     NEED(dt > 0 && std::isfinite(dt), "pcfd_project: dt must be positive");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_project", [&] { s->impl->project(dt, warm != 0, st); });
 }
 pcfd_status pcfd_apply_pressure_impulse(pcfd_solver* s, double dt) {
     NEED(s, "pcfd_apply_pressure_impulse: null solver");
     NEED(dt > 0 && std::isfinite(dt), "pcfd_apply_pressure_impulse: dt must be positive");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_apply_pressure_impulse", [&] { s->impl->impulse(dt); });
 }
 pcfd_status pcfd_integrate_loads(pcfd_solver* s, double pref, pcfd_loads* loads) {
@@ -2803,6 +2826,7 @@ pcfd_status pcfd_instance_stats(const pcfd_solver* s, int instance, pcfd_step_st
 }
 pcfd_status pcfd_autotune(pcfd_solver* s, int repetitions, pcfd_tuning* out) {
     NEED(s, "pcfd_autotune: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_autotune", [&] { s->impl->autotune(repetitions, out); });
 }
 
@@ -2813,6 +2837,7 @@ pcfd_status pcfd_get_tuning(const pcfd_solver* s, pcfd_tuning* out) {
 
 pcfd_status pcfd_set_tuning(pcfd_solver* s, const pcfd_tuning* t) {
     NEED(s && t, "pcfd_set_tuning: null argument");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_set_tuning", [&] { s->impl->set_tuning(*t); });
 }
 
@@ -2846,6 +2871,7 @@ pcfd_status pcfd_get_velocity(pcfd_solver* s, double* x, double* y, double* z) {
 }
 pcfd_status pcfd_set_velocity(pcfd_solver* s, const double* x, const double* y, const double* z) {
     NEED(s, "pcfd_set_velocity: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_set_velocity", [&] { s->impl->set_velocity(x, y, z); });
 }
 pcfd_status pcfd_get_pressure(pcfd_solver* s, double* p) {
@@ -2854,6 +2880,7 @@ pcfd_status pcfd_get_pressure(pcfd_solver* s, double* p) {
 }
 pcfd_status pcfd_set_pressure(pcfd_solver* s, const double* p) {
     NEED(s && p, "pcfd_set_pressure: null argument");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_set_pressure", [&] { s->impl->set_pressure(p); });
 }
 pcfd_status pcfd_get_faces(pcfd_solver* s, double* fx, double* fy, double* fz, double* fe) {
@@ -2862,11 +2889,13 @@ pcfd_status pcfd_get_faces(pcfd_solver* s, double* fx, double* fy, double* fz, d
 }
 pcfd_status pcfd_set_faces(pcfd_solver* s, const double* fx, const double* fy, const double* fz, const double* fe) {
     NEED(s, "pcfd_set_faces: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_set_faces", [&] { s->impl->set_faces(fx, fy, fz, fe); });
 }
 pcfd_status pcfd_mark_initialized(pcfd_solver* s, double physical_time) {
     NEED(s, "pcfd_mark_initialized: null solver");
     NEED(std::isfinite(physical_time), "pcfd_mark_initialized: physical time must be finite");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_mark_initialized", [&] { s->impl->mark_initialized(physical_time); });
 }
 int64_t pcfd_device_bytes(const pcfd_solver* s) { return s ? s->impl->device_bytes : -1; }
@@ -2874,10 +2903,12 @@ pcfd_status pcfd_mg_primitive(pcfd_solver* s, int op, int level, const double* x
                               double omega, int sweeps) {
     NEED(s && out, "pcfd_mg_primitive: null argument");
     NEED(sweeps >= 0, "pcfd_mg_primitive: negative sweeps");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_mg_primitive", [&] { s->impl->mg_primitive(op, level, x, b, out, omega, sweeps); });
 }
 pcfd_status pcfd_time_kernel(pcfd_solver* s, int kernel, int reps, double* avg_ms, double* bytes_per_launch) {
     NEED(s && avg_ms && bytes_per_launch, "pcfd_time_kernel: null argument");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_time_kernel", [&] { s->impl->time_kernel(kernel, reps, avg_ms, bytes_per_launch); });
 }
 pcfd_status pcfd_probe_start(pcfd_solver* s, unsigned kernel_mask) {
@@ -2890,10 +2921,12 @@ pcfd_status pcfd_probe_stop(pcfd_solver* s, int64_t* launches, double* total_ms)
 }
 pcfd_status pcfd_learned_collect(pcfd_solver* s, int max_pairs, pcfd_learned_stats* st) {
     NEED(s, "pcfd_learned_collect: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_learned_collect", [&] { s->impl->learned_collect(max_pairs, st); });
 }
 pcfd_status pcfd_learned_train(pcfd_solver* s, int steps, int batch, double learning_rate, pcfd_learned_stats* st) {
     NEED(s, "pcfd_learned_train: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_learned_train", [&] { s->impl->learned_train(steps, batch, learning_rate, st); });
 }
 pcfd_status pcfd_learned_get_weights(pcfd_solver* s, float* weights, float* biases) {
@@ -2902,19 +2935,23 @@ pcfd_status pcfd_learned_get_weights(pcfd_solver* s, float* weights, float* bias
 }
 pcfd_status pcfd_learned_set_weights(pcfd_solver* s, const float* weights, const float* biases) {
     NEED(s && weights && biases, "pcfd_learned_set_weights: null argument");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_learned_set_weights", [&] { s->impl->learned_weights(nullptr, nullptr, weights, biases); });
 }
 pcfd_status pcfd_learned_enable(pcfd_solver* s, int on) {
     NEED(s, "pcfd_learned_enable: null solver");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_learned_enable", [&] { s->impl->learned_enable(on != 0); });
 }
 pcfd_status pcfd_learned_forward(pcfd_solver* s, const double* r, float* y, float* scales) {
     NEED(s && r && y && scales, "pcfd_learned_forward: null argument");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_learned_forward", [&] { s->impl->learned_forward(r, y, scales); });
 }
 pcfd_status pcfd_learned_gradient(pcfd_solver* s, const double* residual, double* loss, float* grad_weights,
                                   float* grad_biases) {
     NEED(s && residual && loss, "pcfd_learned_gradient: null argument");
+    s->impl->next_rate_ready = false;
     return guarded("pcfd_learned_gradient", [&] { s->impl->learned_gradient(residual, loss, grad_weights, grad_biases); });
 }
 
