@@ -178,14 +178,16 @@ __global__ __launch_bounds__(256) void k_grad_v(MeshView M, FieldsView<R> F, Sma
     double rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // stencil entries in groups of four: indices and weights of a group loaded together,
     // then the neighbours' values, summed in stencil order
+#pragma unroll 2
     for (int k = k0; k < k1; k += 4) {
         int64_t o[4];
         double w[4][3], v[4][3];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const bool ok = k + j < k1;
-            o[j] = ok ? G.nbr[k + j] : a;
-            for (int r = 0; r < 3; ++r) w[j][r] = ok ? G.wd[3 * (k + j) + r] : 0.0;
+            o[j] = load_if(G.nbr, ok, k + j, k, 0);
+            o[j] = ok ? o[j] : a;
+            for (int r = 0; r < 3; ++r) w[j][r] = load_if(G.wd, ok, 3 * (k + j) + r, 3 * k + r, 0.0);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -215,14 +217,16 @@ __global__ __launch_bounds__(256) void k_grad_p(MeshView M, SmagView G, const do
     const int k0 = G.off[sp], k1 = G.off[sp + 1];
     const double pa = p[a];
     double rhs[3] = {0, 0, 0};
+#pragma unroll 2
     for (int k = k0; k < k1; k += 4) {  // groups of four, as k_grad_v
         int64_t o[4];
         double w[4][3], v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const bool ok = k + j < k1;
-            o[j] = ok ? G.nbr[k + j] : a;
-            for (int r = 0; r < 3; ++r) w[j][r] = ok ? G.wd[3 * (k + j) + r] : 0.0;
+            o[j] = load_if(G.nbr, ok, k + j, k, 0);
+            o[j] = ok ? o[j] : a;
+            for (int r = 0; r < 3; ++r) w[j][r] = load_if(G.wd, ok, 3 * (k + j) + r, 3 * k + r, 0.0);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[j] = p[o[j]];
