@@ -550,14 +550,14 @@ def main():
     # once; FP32 fields, FP32 V-cycle) over the measured step time
     subs = learned["substeps"]
     n_patch = topo.info.patches
-    # PCG: 90 B per cell and iteration, 98 in flexible PCG (the r_prev read) while the network is on
-    pcg = 90 + 8.0 * learned["learned_steps"] / max(1, args.steps)
-    b_cell = (18 + 43 + 24 * subs + 26 + 30 + mean_its * (pcg + 114) + 34 + 38 + 16 + 80.0 * n_patch / topo.n_active)
+    # the reference algorithm's minimum bytes per pass (this build's passes move fewer in places:
+    # Fletcher-Reeves needs no r_prev, z is stored in FP32 where the V-cycle computes it so)
+    b_cell = (18 + 43 + 24 * subs + 26 + 30 + mean_its * (98 + 114) + 34 + 38 + 16 + 80.0 * n_patch / topo.n_active)
     step_gbs = b_cell * topo.n_active / (learned["ms_per_step"] * 1e-3) / 1e9
     step_roofline = {"bytes_per_cell": b_cell, "achieved": step_gbs, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": step_gbs / PEAK_HBM,
                      "model": "18 CFL + 43 transport + 24 per diffusion substep + 26 reconstruction + 30 RHS + "
-                              "iterations x (90 PCG, 98 flexible + 114 FP32 V-cycle) + 34 correction + 38 impulse + 16 divergence "
+                              "iterations x (98 PCG + 114 FP32 V-cycle) + 34 correction + 38 impulse + 16 divergence "
                               "check + 80 per surface patch, bytes per active cell"}
     log(f"step bytes model: {b_cell:.0f} B/cell -> {step_gbs:.0f} GB/s over the measured step ({step_gbs / PEAK_HBM:.3f} of peak)")
     ceilings = device_ceilings()
