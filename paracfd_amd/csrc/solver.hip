@@ -2268,7 +2268,7 @@ void SolverImpl<R>::time_kernel(int kernel, int reps, double* avg_ms, double* by
         case 2: per = 9.0 * sizeof(R) + 1; break;             // v in (3), faces (3), mask, v out (3)
         case 3: per = 0; break;
         case 4: per = 4 + 1 + 8 + 8 + 8; break;               // r' (FP32), mask, r, r_prev, dz
-        case 5: per = 8 + 8 + 8 + 8 + 1; break;               // z, d_old in; d_new, Ad out; mask
+        case 5: per = (zf() ? 4 : 8) + 8 + 8 + 8 + 1; break;  // z (FP32 or FP64), d_old in; d_new, Ad out; mask
         case 6: per = 16 + 8 + 8 + 8 + 8 + 1 + (vt == 8 ? 8 : 4) + vt; break;  // x in/out, d, r, Ad, r' out, mask, diag, x1 out
     }
     if (kernel == 4) {
