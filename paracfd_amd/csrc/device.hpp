@@ -337,10 +337,21 @@ __device__ __forceinline__ void load_tile(const MeshView& M, int b, A* tile, Fn 
     __syncthreads();
 }
 
+// The brick's cell size and outlet flag (x-max brick of a case with an outlet), read with the
+// brick's first loads: after the tile barrier (a memory fence) a read of the record would be
+// issued again and waited for.
+struct BrickScalars {
+    double h;
+    bool outlet;
+};
+__device__ __forceinline__ BrickScalars brick_scalars(const MeshView& M, int b) {
+    return BrickScalars{M.meta[b].h, M.outlet && (M.meta[b].flags & kXMAX) != 0};
+}
+
 // structured part of (A y)_a from the tile: faces in the order +x, -x, +y, -y, +z, -z,
 // then the x-max outlet (2h y_a)
 template <class A>
-__device__ __forceinline__ A tile_apply(const MeshView& M, const A* tile, int b, int q, uint8_t m, A h, A ya, A& dg) {
+__device__ __forceinline__ A tile_apply(const A* tile, int q, uint8_t m, A h, bool outlet, A ya, A& dg) {
     const int c = tile_index(q & 7, (q >> 3) & 7, q >> 6);
     const int step[3] = {1, 10, 100};
     A acc = A(0), d = A(0);
@@ -348,7 +359,7 @@ __device__ __forceinline__ A tile_apply(const MeshView& M, const A* tile, int b,
         if (m & (kPX << axis)) { acc += h * (ya - tile[c + step[axis]]); d += h; }
         if (m & (kMX << axis)) { acc += h * (ya - tile[c - step[axis]]); d += h; }
     }
-    if (M.outlet && (q & 7) == 7 && (M.meta[b].flags & kXMAX)) { acc += A(2) * h * ya; d += A(2) * h; }
+    if (outlet && (q & 7) == 7) { acc += A(2) * h * ya; d += A(2) * h; }
     dg = d;
     return acc;
 }

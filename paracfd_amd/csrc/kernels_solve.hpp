@@ -58,8 +58,9 @@ __device__ __forceinline__ A diag_row0(const MeshView& M, int b, int q, uint8_t 
 // gather, outlet 2h p at x-max) with y already in the brick's LDS tile; val gives y at any
 // dof for the compact edges.  Returns the row and its diagonal.
 template <class A, class Fn>
-__device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, int b, int q, uint8_t m, int sp, A ya, A& dg) {
-    A acc = tile_apply<A>(M, tile, b, q, m, A(level_h(M, b)), ya, dg);
+__device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, const BrickScalars& bs, int q, uint8_t m, int sp,
+                                  A ya, A& dg) {
+    A acc = tile_apply<A>(tile, q, m, A(bs.h), bs.outlet, ya, dg);
     if (sp >= 0) compact_apply<A>(M, sp, val, ya, acc, dg);
     return acc;
 }
@@ -70,9 +71,9 @@ __device__ __forceinline__ A row0(const MeshView& M, const A* tile, Fn val, int 
 // one per dependent step.
 // the same with the compact-edge terms fetched before the tile barrier
 template <class A, class Fn>
-__device__ __forceinline__ A row0f(const MeshView& M, const A* tile, const CompactFetch<A>& cf, Fn val, int b, int q,
-                                   uint8_t m, A ya, A& dg) {
-    A acc = tile_apply<A>(M, tile, b, q, m, A(level_h(M, b)), ya, dg);
+__device__ __forceinline__ A row0f(const MeshView& M, const A* tile, const CompactFetch<A>& cf, Fn val,
+                                   const BrickScalars& bs, int q, uint8_t m, A ya, A& dg) {
+    A acc = tile_apply<A>(tile, q, m, A(bs.h), bs.outlet, ya, dg);
     if (m & kSpecial) compact_apply_fetched<A>(M, cf, val, ya, acc, dg);
     return acc;
 }
@@ -80,12 +81,14 @@ __device__ __forceinline__ A row0f(const MeshView& M, const A* tile, const Compa
 #define PCFD_BRICK_BEGIN                                           \
     const int b = brick_of_block(M.xcd_chunk), q = threadIdx.x;               \
     const int64_t a = static_cast<int64_t>(b) * 512 + q;           \
+    const BrickScalars bs = brick_scalars(M, b);                   \
     const uint8_t m = M.mask[a]
 
 // two cells per lane (kBL lanes per brick): cell q = t + kBL h, h = 0, 1
 #define PCFD_BRICK2_BEGIN                                                     \
     const int b = brick_of_block(M.xcd_chunk), t = threadIdx.x;               \
     const int64_t base = static_cast<int64_t>(b) * 512;                       \
+    const BrickScalars bs = brick_scalars(M, b);                              \
     uint8_t m[2];                                                             \
     for (int h = 0; h < 2; ++h) m[h] = M.mask[base + t + kBL * h]
 
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(kBL, 8) void k_cg_residual(MeshView M, const double
         double ra = 0.0;
         if (m[h] & kActive) {
             double dg;
-            ra = rb[h] - row0f<double>(M, tile, cf[h], val, b, q, m[h], tf.own[h], dg);
+            ra = rb[h] - row0f<double>(M, tile, cf[h], val, bs, q, m[h], tf.own[h], dg);
         }
         r[base + q] = ra;
         sum += ra * ra;
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(kBL) void k_cg_apply_dot(MeshView M, const double* 
         double y = 0.0;
         if (m[h] & kActive) {
             double dg;
-            y = row0f<double>(M, tile, cf[h], val, b, q, m[h], tf.own[h], dg);
+            y = row0f<double>(M, tile, cf[h], val, bs, q, m[h], tf.own[h], dg);
         }
         Ad[base + q] = y;
         sum += y * tf.own[h];
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kBL, 6) void k_cg_dir_apply(MeshView M, const ZT* _
         double y = 0.0;
         if (m[h] & kActive) {
             double dg;
-            y = row0f<double>(M, tile, cf[h], dn, b, q, m[h], da, dg);
+            y = row0f<double>(M, tile, cf[h], dn, bs, q, m[h], da, dg);
         }
         dnew[base + q] = da;
         Ad[base + q] = y;
@@ -411,13 +414,13 @@ __global__ __launch_bounds__(kBL) void k_smooth0(MeshView M, int mode, const VT*
         const int q = t + kBL * h;
         A v = A(0);
         if (m[h] & kActive) {
-            const A hb = A(level_h(M, b));
+            const A hb = A(bs.h);
             if (from_zero) {
                 v = A(omega) * A(rb[h]) / diag_row0<A>(M, b, q, m[h], sp[h], hb);
             } else {
                 A dg;
                 const A xa = tf.own[h];
-                const A Ax = row0f<A>(M, tile, cf[h], val, b, q, m[h], xa, dg);
+                const A Ax = row0f<A>(M, tile, cf[h], val, bs, q, m[h], xa, dg);
                 v = xa + A(omega) * (A(rb[h]) - Ax) / dg;
             }
         }
@@ -457,7 +460,7 @@ __global__ __launch_bounds__(512) void k_residual0(MeshView M, const VT* __restr
     A v = A(0);
     if (m & kActive) {
         A dg;
-        v = A(rb) - row0f<A>(M, tile, cf, val, b, q, m, tf.own, dg);
+        v = A(rb) - row0f<A>(M, tile, cf, val, bs, q, m, tf.own, dg);
     }
     res[a] = from_f<VT>(v);
 }
@@ -493,7 +496,7 @@ __global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 6 : 8) void k_vc_pre2(MeshVi
         if (m[h] & kActive) {
             A dg;
             const A xa = tf.own[h];
-            const A Ax = row0f<A>(M, tile, cf[h], x1, b, q, m[h], xa, dg);
+            const A Ax = row0f<A>(M, tile, cf[h], x1, bs, q, m[h], xa, dg);
             v = xa + w2 * (A(rb[h]) - Ax) / dg;
         }
         xout[base + q] = from_f<VT>(v);
@@ -552,7 +555,7 @@ __global__ __launch_bounds__(kBL, sizeof(VT) == 8 ? 5 : 7) void k_vc_update_pre2
         if (m[h] & kActive) {
             A dg;
             const A xa = tf.own[h];
-            const A Ax = row0f<A>(M, tile, cf[h], x1, b, q, m[h], xa, dg);
+            const A Ax = row0f<A>(M, tile, cf[h], x1, bs, q, m[h], xa, dg);
             v = xa + w2 * (A(rn[h]) - Ax) / dg;
         }
         xout[base + q] = from_f<VT>(v);
@@ -595,7 +598,7 @@ __global__ __launch_bounds__(kBL) void k_vc_restrict0(MeshView M, const VT* __re
         A v = A(0);
         if (m[h] & kActive) {
             A dg;
-            v = A(rb[h]) - row0f<A>(M, tile, cf[h], val, b, q, m[h], tf.own[h], dg);
+            v = A(rb[h]) - row0f<A>(M, tile, cf[h], val, bs, q, m[h], tf.own[h], dg);
         }
         sres[q] = round_vt<VT>(v);
         spar[q] = pa[h];
@@ -674,7 +677,7 @@ __global__ __launch_bounds__(kBL) void k_vc_post_prolong(MeshView M, int last, c
         if (m[h] & kActive) {
             A dg;
             const A ya = tf.own[h];
-            const A Ay = row0f<A>(M, tile, cf[h], y, b, q, m[h], ya, dg);
+            const A Ay = row0f<A>(M, tile, cf[h], y, bs, q, m[h], ya, dg);
             v = ya + A(omega) * (A(rb[h]) - Ay) / dg;
         }
         if (!last) {

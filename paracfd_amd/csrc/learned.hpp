@@ -312,7 +312,7 @@ __global__ __launch_bounds__(512) void k_nn_resid(MeshView M, const int* __restr
     double rp = 0.0;
     if (m & kActive) {
         double dg;
-        rp = ra - row0f<double>(M, ztile, cf, zv, B.b, q, m, tf.own, dg);
+        rp = ra - row0f<double>(M, ztile, cf, zv, brick_scalars(M, B.b), q, m, tf.own, dg);
     }
     store[static_cast<int64_t>(B.pos) * 512 + q] = static_cast<float>(rp);
 }
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(512) void k_nn_energy(MeshView M, const int* __rest
         double dg;
         const double rp = store[static_cast<int64_t>(B.pos) * 512 + q];
         const double da = tile[nn_pos(q)];
-        const double Ad = row0<double>(M, tile, dv, B.b, q, m, sp, da, dg);
+        const double Ad = row0<double>(M, tile, dv, brick_scalars(M, B.b), q, m, sp, da, dg);
         g = 2.0 * (Ad - rp) * inv_norm;
         l = da * (Ad - 2.0 * rp) * inv_norm;
     }
