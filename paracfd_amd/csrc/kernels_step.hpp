@@ -87,7 +87,7 @@ __device__ __forceinline__ FaceVel<R> load_faces(const MeshView& M, const Fields
 // incidence order (outgoing positive): groups of four, the edge ids of a group loaded
 // together, then their areas and face velocities; `op(acc, flux)` folds one term
 template <class R, class Op>
-__device__ __forceinline__ void fold_edge_fluxes(const MeshView& M, const FieldsView<R>& F, int sp, double& acc, Op op) {
+__device__ __forceinline__ void for_each_edge_flux(const MeshView& M, const FieldsView<R>& F, int sp, Op visit) {
     const int k0 = M.special_inc_off[sp], k1 = M.special_inc_off[sp + 1];
     for (int k = k0; k < k1; k += 4) {
         int raw[4];
@@ -104,9 +104,14 @@ __device__ __forceinline__ void fold_edge_fluxes(const MeshView& M, const Fields
         for (int j = 0; j < 4; ++j)
             if (k + j < k1) {
                 const double flux = ar[j] * fv[j];
-                acc = op(acc, raw[j] >= 0 ? flux : -flux);
+                visit(raw[j] >= 0 ? flux : -flux);
             }
     }
+}
+
+template <class R, class Op>
+__device__ __forceinline__ void fold_edge_fluxes(const MeshView& M, const FieldsView<R>& F, int sp, double& acc, Op op) {
+    for_each_edge_flux<R>(M, F, sp, [&](double x) { acc = op(acc, x); });
 }
 
 // A member of a momentum redistribution group hands its increment to the group (k_redistribute)
@@ -877,8 +882,8 @@ __global__ void k_reconstruct_edges(MeshView M, FieldsView<R> F, const R* __rest
 
 // integrated divergence of the face fluxes around cell a (FP64 accumulation)
 template <class R>
-__device__ __forceinline__ double cell_divergence(const MeshView& M, const FieldsView<R>& F, const FaceVel<R>& f,
-                                                  uint16_t bf, int q, uint8_t m, int sp, double h) {
+__device__ __forceinline__ double cell_divergence_faces(const MeshView& M, const FaceVel<R>& f, uint16_t bf, int q,
+                                                        uint8_t m, double h) {
     const double A = h * h;
     double div = 0.0;
     for (int axis = 0; axis < 3; ++axis) {
@@ -887,6 +892,13 @@ __device__ __forceinline__ double cell_divergence(const MeshView& M, const Field
     }
     if ((q & 7) == 0 && (bf & kXMIN)) div -= A * M.speed;
     if (M.outlet && (q & 7) == 7 && (bf & kXMAX)) div += A * double(f.up[0]);
+    return div;
+}
+
+template <class R>
+__device__ __forceinline__ double cell_divergence(const MeshView& M, const FieldsView<R>& F, const FaceVel<R>& f,
+                                                  uint16_t bf, int q, uint8_t m, int sp, double h) {
+    double div = cell_divergence_faces<R>(M, f, bf, q, m, h);
     if (sp >= 0) fold_edge_fluxes<R>(M, F, sp, div, [](double acc, double x) { return acc + x; });
     return div;
 }
@@ -942,12 +954,16 @@ __global__ __launch_bounds__(kBL) void k_divcheck(MeshView M, FieldsView<R> F, d
         const int sp = special_rank(M, b, q, (m[k] & kSpecial) != 0);
         if (m[k] & kActive) {
             const double V = sp >= 0 ? M.special_volume[sp] : h * h * h;
-            val = fmax(val, fabs(cell_divergence<R>(M, F, cf[k], bf, q, m[k], sp, h) / V));
-            if (rate_part) {
-                double out = cell_outflow<R>(M, cf[k], bf, q, m[k], A);
-                if (sp >= 0) fold_edge_fluxes<R>(M, F, sp, out, [](double acc, double f) { return acc + fmax(f, 0.0); });
-                rate = fmax(rate, out / V);
-            }
+            // one walk over the compact edges for both sums (each in incidence order)
+            double div = cell_divergence_faces<R>(M, cf[k], bf, q, m[k], h);
+            double out = cell_outflow<R>(M, cf[k], bf, q, m[k], A);
+            if (sp >= 0)
+                for_each_edge_flux<R>(M, F, sp, [&](double x) {
+                    div += x;
+                    out += fmax(x, 0.0);
+                });
+            val = fmax(val, fabs(div / V));
+            if (rate_part) rate = fmax(rate, out / V);
         }
     }
     store_partial(partials, b, block_max<kBL>(val));
