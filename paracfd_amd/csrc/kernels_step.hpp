@@ -136,6 +136,22 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
     atomicMax(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// slot[key] = max(slot[key], w) for every lane with key >= 0 (w >= 0): the lanes of a wave
+// that share a key combine first, one atomic per distinct key (special cells are ordered by
+// brick, so a wave holds one to a few).  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_max_to_slots(double* slot, int key, double w) {
+    for (;;) {
+        const unsigned long long pending = __ballot(key >= 0);
+        if (pending == 0) break;
+        const int lead = __ffsll(static_cast<long long>(pending)) - 1;
+        const int k = __shfl(key, lead);
+        double v = key == k ? w : 0.0;
+        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+        if (static_cast<int>(__lane_id()) == lead) atomic_max_nonneg(slot + k, v);
+        if (key == k) key = -1;
+    }
+}
+
 // Every member of group g takes the group's summed increment (member order) over the group's
 // volume (finalize_redistributed_state_kernel, amr_advection.cu:142-149); out holds the
 // members' values before the increment.
@@ -1111,12 +1127,10 @@ __global__ __launch_bounds__(kBL) void k_impulse(MeshView M, FieldsView<R> F, co
 // skew . ((1-w) g_a + w g_b), pressure_compact_cell_impulse_kernel, amr_advection.cu:376-390),
 // the wall traction of the cell's patch, the fluid volume and the redistribution hand-over.
 template <class R>
-__global__ __launch_bounds__(256) void k_impulse_compact(MeshView M, FieldsView<R> F, const double* __restrict__ p,
-                                                         const double* __restrict__ gp, const DevScalars* S,
-                                                         double dt_given, int only_converged,
-                                                         const double* __restrict__ dsp, double* vmax_part) {
-    const int spc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (spc >= M.nspecial) return;
+__device__ __forceinline__ void impulse_special_cell(const MeshView& M, const FieldsView<R>& F, const double* __restrict__ p,
+                                                     const double* __restrict__ gp, const DevScalars* S, double dt_given,
+                                                     int only_converged, const double* __restrict__ dsp, int spc,
+                                                     int& key, double& w) {
     const int64_t a = M.special_dof[spc];
     const uint8_t m = M.mask[a];
     const DevScalars& I = S[M.meta[a >> 9].inst];
@@ -1190,7 +1204,20 @@ __global__ __launch_bounds__(256) void k_impulse_compact(MeshView M, FieldsView<
     const double iv = 1.0 / V;
     R vn[3];
     for (int c = 0; c < 3; ++c) F.v[c][a] = vn[c] = R(double(v0[c]) + d[c] * iv);
-    if (vmax_part) atomic_max_nonneg(vmax_part + (a >> 9), abs_max3(double(vn[0]), double(vn[1]), double(vn[2])));
+    key = static_cast<int>(a >> 9);
+    w = abs_max3(double(vn[0]), double(vn[1]), double(vn[2]));
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_impulse_compact(MeshView M, FieldsView<R> F, const double* __restrict__ p,
+                                                         const double* __restrict__ gp, const DevScalars* S,
+                                                         double dt_given, int only_converged,
+                                                         const double* __restrict__ dsp, double* vmax_part) {
+    const int spc = blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    double w = 0.0;
+    if (spc < M.nspecial) impulse_special_cell<R>(M, F, p, gp, S, dt_given, only_converged, dsp, spc, key, w);
+    if (vmax_part) wave_max_to_slots(vmax_part, key, w);  // every lane of the wave reaches it
 }
 
 // ---------------------------------------------------------------------------------------
