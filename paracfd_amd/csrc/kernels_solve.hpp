@@ -892,6 +892,37 @@ __global__ void k_restrict_range(CsrView Cc, const VT* __restrict__ fine, VT* __
     if (c < Cc.n) restrict_range_row<VT>(Cc, c, fine, coarse);
 }
 
+// residual of level m fused with its restriction to level m+1 where the children of
+// consecutive coarse rows lie far apart: eight lanes per coarse row (at most eight children, a
+// contiguous range), lane j computes child j's residual (rounded to VT, as stored by the
+// unfused pair), the group's first lane sums them in order.  Every lane reaches the shuffles.
+template <class VT, class CT>
+__global__ __launch_bounds__(256) void k_ell_residual_restrict_rows(CsrView C, CsrView NC, const VT* __restrict__ x,
+                                                                    const VT* __restrict__ rhs, VT* __restrict__ coarse) {
+    using A = typename Arith<VT>::type;
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 3, j = threadIdx.x & 7;
+    const int lane = static_cast<int>(__lane_id()), g0 = lane & ~7;
+    int k0 = 0, n = 0, f0 = 0;
+    if (c < NC.n) {
+        k0 = NC.child_off[c];
+        n = NC.child_off[c + 1] - k0;
+        f0 = NC.child_first[c];
+    }
+    A v = A(0);
+    if (j < n) {
+        const int r = f0 + j;
+        const A xr = A(to_f(x[r]));
+        const A Ax = ell_row<A, CT>(C, r, [&](int q) -> A { return A(to_f(x[q])); }, xr);
+        v = A(to_f(from_f<VT>(A(to_f(rhs[r])) - Ax)));
+    }
+    A s = A(0);
+    for (int k = 0; k < 8; ++k) {
+        const A u = __shfl(v, g0 + k);
+        if (k < n) s += u;
+    }
+    if (j == 0 && c < NC.n) coarse[c] = from_f<VT>(s);
+}
+
 // residual of level m fused with its restriction to level m+1: a workgroup takes
 // kRestrictGroup consecutive rows of level m+1, whose children form one window of at most 256
 // consecutive rows of level m (checked on the host); each lane computes one row's residual

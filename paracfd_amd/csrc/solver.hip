@@ -552,6 +552,7 @@ private:
         // (each group of kRestrictGroup next-level rows has its children in one window of at
         // most 256 rows of this level)
         bool fused_restrict = false;
+        bool restrict_rows8 = false;  // every coarse row of the next level has at most 8 children
     };
     std::vector<Level> lv_;
     const int* parent0_ = nullptr;
@@ -977,6 +978,9 @@ void SolverImpl<R>::construct(const Topology& T) {
                 if (ok && prev_end - first > 256) ok = false;
             }
             lv_[m].fused_restrict = ok;
+            bool r8 = N.n > 0;
+            for (int c = 0; r8 && c < N.n; ++c) r8 = N.child_off[c + 1] - N.child_off[c] <= 8;
+            lv_[m].restrict_rows8 = r8;
         }
         // dense coarse blocks per level (the coarsest level of each instance)
         dense_.assign(T.mg.size(), DenseLevel{});
@@ -1307,6 +1311,9 @@ VT* SolverImpl<R>::coarse_cycle_t(int m) {
     const CsrView& NC = std::is_same<CT, double>::value ? N.csr64 : N.csr32;
     if (L.fused_restrict && fused_) {
         LAUNCH((k_ell_residual_restrict<VT, CT>), blocks_for(N.n, kRestrictGroup), 256, C, NC, cur, B, static_cast<VT*>(N.B));
+    } else if (L.restrict_rows8 && fused_) {
+        LAUNCH((k_ell_residual_restrict_rows<VT, CT>), blocks_for(8 * static_cast<int64_t>(N.n), 256), 256, C, NC, cur, B,
+               static_cast<VT*>(N.B));
     } else {
         LAUNCH((k_ell_residual<VT, CT>), g, cb, C, cur, B, oth);
         LAUNCH((k_restrict_range<VT>), blocks_for(N.n, cb), cb, NC, oth, static_cast<VT*>(N.B));
