@@ -1982,9 +1982,10 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     if (!next_rate_ready) {
         LAUNCH(k_outflow_rate<R>, nb_, kBL, M_, fields(), partials_, 1);
         finalize(FIN_RATE_VMAX, nb_);
+        sync_scalars();
     }
+    // else the last step read them back with its end
     next_rate_ready = false;
-    sync_scalars();
     for (int i = 0; i < ninst_; ++i) {
         if (!std::isfinite(host_s_[i].vmax)) throw std::domain_error("non-finite velocity before timestep");
         inst_stats_[i].cfl_rate = host_s_[i].rate;
@@ -1999,11 +2000,12 @@ void SolverImpl<R>::step(pcfd_step_stats* st) {
     launch_impulse(0.0, 1, rv_part_ + nb_);
     redistribute(rv_part_ + nb_);
     // the next step's rate, timestep and |v|max from the final faces and velocities (the
-    // values k_outflow_rate would find)
+    // values k_outflow_rate would find), read back with the end of this step
     finalize(FIN_RATE_VMAX, nb_, rv_part_);
     for (int i = 0; i < ninst_; ++i)
         if (inst_stats_[i].converged) inst_time_[i] += host_s_[i].dt;
     time = inst_time_[0];
+    HIP_CHECK(hipMemcpyAsync(host_s_, S_, (ninst_ + 1) * sizeof(DevScalars), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipEventSynchronize(ev_[1]));
     float a_ms = 0, p_ms = 0, s_ms = 0;
