@@ -252,9 +252,10 @@ __device__ __forceinline__ TileFetch2<A> fetch_tile_x2(const MeshView& M, int b,
     return fetch_tile_x2<A>(M, b, val, NoFill{}, Same{});
 }
 
-// Write the fetched values into the tile and synchronise.  Every lane must call it.
+// Write the fetched values into the tile and synchronise (unless `sync` is false: the caller
+// stages more tiles and synchronises once).  Every lane must call it.
 template <class A>
-__device__ __forceinline__ void stage_tile_x2(A* tile, const TileFetch2<A>& f) {
+__device__ __forceinline__ void stage_tile_x2(A* tile, const TileFetch2<A>& f, bool sync = true) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -264,7 +265,7 @@ __device__ __forceinline__ void stage_tile_x2(A* tile, const TileFetch2<A>& f) {
 #pragma unroll
     for (int r = 0; r < 2; ++r)
         if (f.hidx[r] >= 0) tile[f.hidx[r]] = f.halo[r];
-    __syncthreads();
+    if (sync) __syncthreads();
 }
 
 // The first four compact-edge terms of a special cell, fetched before the tile barrier (the

@@ -637,7 +637,8 @@ __global__ __launch_bounds__(kBL) void k_diffuse(MeshView M, const R* __restrict
         return;
     }
     __shared__ R tv[3][kTileN];
-    for (int c = 0; c < 3; ++c) stage_tile_x2<R>(tv[c], tf[c]);
+    for (int c = 0; c < 3; ++c) stage_tile_x2<R>(tv[c], tf[c], false);
+    __syncthreads();
     const R kf = R(dts * M.nu) * h;
     for (int kc = 0; kc < 2; ++kc) {
         const int q = t + kBL * kc;
