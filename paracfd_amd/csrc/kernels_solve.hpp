@@ -256,7 +256,8 @@ enum FinalizeOp : int {
     FIN_RZ_INITIAL = 8,
     FIN_LOADS = 9,    // twelve load sums (pressure and wall-shear force and moment)
     FIN_DIFFRATE = 10, // max explicit diffusion rate (Smagorinsky substeps)
-    FIN_RATE_VMAX = 11 // FIN_RATE and FIN_VMAX from two partial arrays
+    FIN_RATE_VMAX = 11, // FIN_RATE and FIN_VMAX from two partial arrays
+    FIN_BNORM_RESID = 12 // FIN_BNORM, then FIN_RESID, from two partial arrays (||b||^2, ||r||^2)
 };
 
 // The instance's scalars that an op reads (done flag, rz, bnorm, iteration count) are loaded
@@ -269,7 +270,9 @@ __device__ __forceinline__ int finalize_one(int op, const double* __restrict__ p
     const double rz = S->rz, bnorm = S->bnorm;
     const int iter = S->iter;
     const bool take_max = op == FIN_RATE || op == FIN_MAXDIV || op == FIN_VMAX || op == FIN_DIFFRATE || op == FIN_RATE_VMAX;
-    const int arrays = op == FIN_LOADS ? 12 : (((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible) || op == FIN_RATE_VMAX ? 2 : 1);
+    const int arrays = op == FIN_LOADS ? 12
+                       : (((op == FIN_RZ || op == FIN_RZ_INITIAL) && flexible) || op == FIN_RATE_VMAX || op == FIN_BNORM_RESID ? 2
+                                                                                                                           : 1);
     double t[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (arrays == 2) gather_partials_n<1024, 2, 20>(partials + beg, nparts, end - beg, take_max, t);
     else if (arrays == 1) gather_partials_n<1024, 1, 40>(partials + beg, nparts, end - beg, take_max, t);
@@ -287,12 +290,21 @@ __device__ __forceinline__ int finalize_one(int op, const double* __restrict__ p
             S->vmax = t[1];
             break;
         case FIN_BNORM:
-            S->bnorm = sqrt(t[0]);
+        case FIN_BNORM_RESID: {
+            const double bn = sqrt(t[0]);
+            S->bnorm = bn;
             S->iter = 0; S->converged = 0; S->breakdown = 0; S->zero_rhs = 0;
             done = 0;
             if (!(t[0] > 0.0)) { done = 1; S->converged = 1; S->zero_rhs = 1; S->relres = 0.0; }
             S->done = done;
+            if (op == FIN_BNORM_RESID && !done) {  // FIN_RESID with the new ||b||
+                S->rr = t[1];
+                const double rel = sqrt(t[1]) / bn;
+                S->relres = rel;
+                if (rel <= rtol) { done = 1; S->done = 1; S->converged = 1; }
+            }
             break;
+        }
         case FIN_MAXDIV: S->maxdiv = t[0]; break;
         case FIN_VMAX: S->vmax = t[0]; break;
         case FIN_RESID: {

@@ -987,6 +987,46 @@ __global__ __launch_bounds__(kBL) void k_divcheck(MeshView M, FieldsView<R> F, d
     if (rate_part) store_partial(rate_part, b, block_max<kBL>(rate));
 }
 
+// The projection's right-hand side b = -(rho/dt) div (as k_rhs, amr_pressure.cu:255-284) and
+// the initial residual r = b - A x (as k_cg_residual) in one pass; b is not stored.  Partials: ||b||^2 at b's slot,
+// ||r||^2 at nbricks + b (FIN_BNORM_RESID).  Runs before the solve's flags are reset, so it
+// does not test them.
+template <class R>
+__global__ __launch_bounds__(kBL) void k_rhs_residual(MeshView M, FieldsView<R> F, const double* __restrict__ x,
+                                                      double* __restrict__ r, double* partials, const DevScalars* S) {
+    PCFD_BRICK2_BEGIN;
+    const uint16_t bf = M.meta[b].flags;
+    const double dt = S[M.meta[b].inst].dt;
+    FaceVel<R> fv[2];
+    for (int k = 0; k < 2; ++k) fv[k] = load_faces<R>(M, F, b, t + kBL * k);
+    auto val = [&](int64_t c) -> double { return x[c]; };
+    const TileFetch2<double> tf = fetch_tile_x2<double>(M, b, val);
+    int sp[2];
+    CompactFetch<double> cf[2];
+    for (int k = 0; k < 2; ++k) {
+        sp[k] = special_rank(M, b, t + kBL * k, (m[k] & kSpecial) != 0);
+        cf[k] = fetch_compact<double>(M, sp[k], val);
+    }
+    __shared__ double tile[kTileN];
+    stage_tile_x2<double>(tile, tf);
+    double sb = 0.0, sr = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const int q = t + kBL * k;
+        double rb = 0.0, ra = 0.0;
+        if (m[k] & kActive) {
+            rb = -(M.rho / dt) * cell_divergence<R>(M, F, fv[k], bf, q, m[k], sp[k], bs.h);
+            asm volatile("" : "+v"(rb));  // b rounded as stored, not contracted into r = b - A x
+            double dg;
+            ra = rb - row0f<double>(M, tile, cf[k], val, bs, q, m[k], tf.own[k], dg);
+        }
+        r[base + q] = ra;
+        sb += rb * rb;
+        sr += ra * ra;
+    }
+    store_partial(partials, b, block_sum<kBL>(sb));
+    store_partial(partials, M.nbricks + b, block_sum<kBL>(sr));
+}
+
 // face correction u -= (dt/rho) dp/dn (amr_pressure.cu:285-310); outlet p = 0 half a cell away
 // two cells per lane; the pressures of the cell and its +x/+y/+z neighbours and the face
 // velocities are loaded before the mask is tested

@@ -461,7 +461,9 @@ private:
         }
         nn_on_ = on;
     }
-    bool pcg_loop(bool watch_stall);
+    // PCG from the current p_; with `residual_ready` the initial residual is already in r_ (and
+    // FIN_RESID done), else it is formed from rhs_
+    bool pcg_loop(bool watch_stall, bool residual_ready = false);
     void aggregate_stats(pcfd_step_stats* st) const;
     void advect_core();
     // the redistribution groups' pooled increments into the current velocity (after a kernel
@@ -1424,10 +1426,12 @@ void SolverImpl<R>::precondition_dispatch(bool initial, const double* upd_d, con
 // `watch_stall`, returns false as soon as the relative residual read at a check has not
 // halved over the last three checks (or is not finite); true otherwise.
 template <class R>
-bool SolverImpl<R>::pcg_loop(bool watch_stall) {
-    reset_residual_roles();
-    LAUNCH(k_cg_residual, nb_, kBL, M_, p_, rhs_, r_, partials_, S_);
-    finalize(FIN_RESID, nb_);
+bool SolverImpl<R>::pcg_loop(bool watch_stall, bool residual_ready) {
+    if (!residual_ready) {
+        reset_residual_roles();
+        LAUNCH(k_cg_residual, nb_, kBL, M_, p_, rhs_, r_, partials_, S_);
+        finalize(FIN_RESID, nb_);
+    }
     precondition_dispatch(true);
     // first iteration (beta = 0) outside the replayed runs; d is double-buffered
     cg_iteration(true, dbuf_[0], dbuf_[1]);
@@ -1813,11 +1817,20 @@ void SolverImpl<R>::fgmres_solve() {
 template <class R>
 void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
     p_changed();
-    LAUNCH(k_rhs<R>, nb_, kBL, M_, fields(), rhs_, partials_, S_);
-    finalize(FIN_BNORM, nb_);
     if (!warm) {
         HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(double), stream_));
         gm_k_ = 0;  // a cold start also forgets the recycled corrections
+    }
+    // PCG: the right-hand side and the initial residual in one pass (b is not stored); the
+    // non-orthogonal solve keeps b for its full residuals
+    const bool fused_rhs = !nonorth_;
+    if (fused_rhs) {
+        reset_residual_roles();
+        LAUNCH(k_rhs_residual<R>, nb_, kBL, M_, fields(), p_, r_, partials_, S_);
+        finalize(FIN_BNORM_RESID, nb_);
+    } else {
+        LAUNCH(k_rhs<R>, nb_, kBL, M_, fields(), rhs_, partials_, S_);
+        finalize(FIN_BNORM, nb_);
     }
     const bool learned = nn_ && nn_on_ && !trial_plain();
     set_nn_off(!learned);
@@ -1826,7 +1839,7 @@ void SolverImpl<R>::run_projection(bool warm, pcfd_step_stats* st) {
         gm_its_ = 0;
         if (!host_s_[0].zero_rhs) fgmres_solve();
     } else {
-        pcg_loop(learned);
+        pcg_loop(learned, fused_rhs);
     }
     int iterations = nonorth_ ? gm_its_ : host_s_[0].iter;
     bool fell_back = false;
